@@ -1,0 +1,717 @@
+/*
+ * case.c -- oracle restatement of the segregated pressure-velocity outer
+ * iteration on hex8 meshes.  TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Restated (file:line in the reference):
+ *   momentum interior     assemble/flow/segregatedFlow/navierStokesAssembler/
+ *                         navierStokesAssemblerElemTerms.cpp:15-441
+ *   momentum node terms   navierStokesAssemblerNodeTerms.cpp:16-149
+ *   no-slip wall          navierStokesAssemblerElemBoundaryConditions.cpp:2659-2884
+ *                         (macro IP_ZERO_NORMAL_STRESS_FIXED_VEL__ :592-654)
+ *   relaxation            assemble/phiAssembler/phiAssembler.h:743-753, 966-1152
+ *                         navierStokesAssembler.cpp:234-239
+ *   du coefficients       navierStokesAssembler.cpp:84-232
+ *   p' interior           pressureCorrectionAssemblerElemTerms.cpp:17-616
+ *   pressure reference    pressureCorrectionAssembler.cpp:19-63
+ *   mass flow update      model/flow/flowModel.cpp:5272-5688
+ *   mass flow init        flowModel.cpp:3303-3470
+ *   mass divergence       flowModel.cpp:13541-13775
+ *   Green-Gauss gradient  field/nodeField/nodeField.hpp:3253-4950
+ *   field correction      equation/equation.h:204-286
+ *   velocity correction   equation/flow/segregatedFlowEquations.cpp:133-279
+ *   scales                nodeField.hpp:5993-6015, flowModel.cpp:117-128
+ *   dual nodal volume     mesh/meshGeometry.cpp:1277
+ *   exposed area vectors  mesh/meshGeometry.cpp:1441-1528
+ */
+#include "oracle.h"
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SMALL DBL_EPSILON /* src/basic/types.h:179 */
+
+struct or_case {
+    int nn, ne, nbf, nbc;
+    double *xyz;
+    int *conn;
+    int *bf_elem, *bf_ord, *bf_bc;
+    double *bc_U;
+    or_params prm;
+    /* graph */
+    int nnz;
+    int *row_ptr, *cols, *diag_off;
+    /* systems */
+    double *A3, *b3, *x3, *A1, *b1, *x1;
+    /* fields */
+    double *U, *p, *gradU, *gradP, *du, *mdot, *div, *vol, *rho, *mu, *beta;
+    double *bareav; /* exposed area vectors [nbf][4][3] */
+    int *bnode;     /* node is on a wall boundary */
+    double U_scale, p_scale;
+};
+
+static void *xcalloc(size_t n, size_t s)
+{
+    void *p = calloc(n ? n : 1, s);
+    return p;
+}
+
+or_case *or_case_create(int nn, int ne, const double *xyz, const int *conn, int nbf,
+                        const int *bf_elem, const int *bf_ord, const int *bf_bc,
+                        int nbc, const double *bc_U, const or_params *prm)
+{
+    or_case *c = (or_case *)xcalloc(1, sizeof(or_case));
+    c->nn = nn;
+    c->ne = ne;
+    c->nbf = nbf;
+    c->nbc = nbc;
+    c->prm = *prm;
+    c->xyz = (double *)xcalloc((size_t)nn * 3, sizeof(double));
+    memcpy(c->xyz, xyz, sizeof(double) * 3 * (size_t)nn);
+    c->conn = (int *)xcalloc((size_t)ne * 8, sizeof(int));
+    memcpy(c->conn, conn, sizeof(int) * 8 * (size_t)ne);
+    c->bf_elem = (int *)xcalloc((size_t)nbf, sizeof(int));
+    c->bf_ord = (int *)xcalloc((size_t)nbf, sizeof(int));
+    c->bf_bc = (int *)xcalloc((size_t)nbf, sizeof(int));
+    if (nbf) {
+        memcpy(c->bf_elem, bf_elem, sizeof(int) * (size_t)nbf);
+        memcpy(c->bf_ord, bf_ord, sizeof(int) * (size_t)nbf);
+        memcpy(c->bf_bc, bf_bc, sizeof(int) * (size_t)nbf);
+    }
+    c->bc_U = (double *)xcalloc((size_t)nbc * 3, sizeof(double));
+    if (nbc) memcpy(c->bc_U, bc_U, sizeof(double) * 3 * (size_t)nbc);
+
+    c->row_ptr = (int *)xcalloc((size_t)nn + 1, sizeof(int));
+    if (or_graph_build(nn, ne, conn, prm->reduced, c->row_ptr, &c->cols) != 0) {
+        or_case_destroy(c);
+        return NULL;
+    }
+    c->nnz = c->row_ptr[nn];
+    c->diag_off = (int *)xcalloc((size_t)nn, sizeof(int));
+    for (int i = 0; i < nn; ++i) c->diag_off[i] = or_graph_find(c->row_ptr, c->cols, i, i);
+
+    c->A3 = (double *)xcalloc((size_t)c->nnz * 9, sizeof(double));
+    c->b3 = (double *)xcalloc((size_t)nn * 3, sizeof(double));
+    c->x3 = (double *)xcalloc((size_t)nn * 3, sizeof(double));
+    c->A1 = (double *)xcalloc((size_t)c->nnz, sizeof(double));
+    c->b1 = (double *)xcalloc((size_t)nn, sizeof(double));
+    c->x1 = (double *)xcalloc((size_t)nn, sizeof(double));
+
+    c->U = (double *)xcalloc((size_t)nn * 3, sizeof(double));
+    c->p = (double *)xcalloc((size_t)nn, sizeof(double));
+    c->gradU = (double *)xcalloc((size_t)nn * 9, sizeof(double));
+    c->gradP = (double *)xcalloc((size_t)nn * 3, sizeof(double));
+    c->du = (double *)xcalloc((size_t)nn * 3, sizeof(double));
+    c->mdot = (double *)xcalloc((size_t)ne * 12, sizeof(double));
+    c->div = (double *)xcalloc((size_t)nn, sizeof(double));
+    c->vol = (double *)xcalloc((size_t)nn, sizeof(double));
+    c->rho = (double *)xcalloc((size_t)nn, sizeof(double));
+    c->mu = (double *)xcalloc((size_t)nn, sizeof(double));
+    c->beta = (double *)xcalloc((size_t)nn * 3, sizeof(double));
+    c->bareav = (double *)xcalloc((size_t)nbf * 12, sizeof(double));
+    c->bnode = (int *)xcalloc((size_t)nn, sizeof(int));
+    for (int i = 0; i < nn; ++i) {
+        c->rho[i] = prm->rho;
+        c->mu[i] = prm->mu;
+    }
+    /* exposed area vectors from the face nodes in side-ordinal order */
+    const int *sn = or_hex_side_nodes();
+    for (int f = 0; f < nbf; ++f) {
+        const int *en = &c->conn[8 * c->bf_elem[f]];
+        double fc[12];
+        for (int k = 0; k < 4; ++k) {
+            const int nd = en[sn[4 * c->bf_ord[f] + k]];
+            for (int d = 0; d < 3; ++d) fc[3 * k + d] = c->xyz[3 * nd + d];
+            c->bnode[nd] = 1;
+        }
+        or_quad_face_areav(fc, &c->bareav[12 * f]);
+    }
+    or_dual_volume(c);
+    c->U_scale = 1.0;
+    c->p_scale = 1.0;
+    return c;
+}
+
+void or_case_destroy(or_case *c)
+{
+    if (!c) return;
+    void *ptrs[] = {c->xyz, c->conn, c->bf_elem, c->bf_ord, c->bf_bc, c->bc_U,
+                    c->row_ptr, c->cols, c->diag_off, c->A3, c->b3, c->x3,
+                    c->A1, c->b1, c->x1, c->U, c->p, c->gradU, c->gradP, c->du,
+                    c->mdot, c->div, c->vol, c->rho, c->mu, c->beta, c->bareav,
+                    c->bnode};
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
+    free(c);
+}
+
+int or_case_nn(const or_case *c) { return c->nn; }
+int or_case_nnz(const or_case *c) { return c->nnz; }
+const int *or_case_row_ptr(const or_case *c) { return c->row_ptr; }
+const int *or_case_cols(const or_case *c) { return c->cols; }
+const int *or_case_diag_off(const or_case *c) { return c->diag_off; }
+
+double *or_case_field(or_case *c, const char *name)
+{
+    if (!strcmp(name, "U")) return c->U;
+    if (!strcmp(name, "p")) return c->p;
+    if (!strcmp(name, "gradU")) return c->gradU;
+    if (!strcmp(name, "gradP")) return c->gradP;
+    if (!strcmp(name, "du")) return c->du;
+    if (!strcmp(name, "mdot")) return c->mdot;
+    if (!strcmp(name, "div")) return c->div;
+    if (!strcmp(name, "vol")) return c->vol;
+    if (!strcmp(name, "rho")) return c->rho;
+    if (!strcmp(name, "mu")) return c->mu;
+    if (!strcmp(name, "beta")) return c->beta;
+    if (!strcmp(name, "bareav")) return c->bareav;
+    if (!strcmp(name, "xyz")) return c->xyz;
+    return NULL;
+}
+
+double *or_case_vals(or_case *c, int N) { return N == 3 ? c->A3 : c->A1; }
+double *or_case_rhs(or_case *c, int N) { return N == 3 ? c->b3 : c->b1; }
+double *or_case_x(or_case *c, int N) { return N == 3 ? c->x3 : c->x1; }
+
+static void gather_coords(const or_case *c, int e, double *co)
+{
+    const int *en = &c->conn[8 * e];
+    for (int a = 0; a < 8; ++a)
+        for (int d = 0; d < 3; ++d) co[3 * a + d] = c->xyz[3 * en[a] + d];
+}
+
+void or_dual_volume(or_case *c)
+{
+    memset(c->vol, 0, sizeof(double) * (size_t)c->nn);
+    double co[24], v[8];
+    for (int e = 0; e < c->ne; ++e) {
+        gather_coords(c, e, co);
+        or_hex_scv_vol(co, v);
+        for (int a = 0; a < 8; ++a) c->vol[c->conn[8 * e + a]] += v[a];
+    }
+}
+
+/* SCV weights of the element-centre average (pressureCorrectionAssemblerElemTerms.cpp:346-368) */
+static void scv_weights(const double *co, double *w)
+{
+    double v[8], V = 0.0;
+    or_hex_scv_vol(co, v);
+    for (int k = 0; k < 8; ++k) V += v[k];
+    const double inv = 1.0 / V;
+    for (int k = 0; k < 8; ++k) w[k] = 0.0;
+    for (int k = 0; k < 8; ++k) w[k] += v[k] * inv; /* hex SCV ipNodeMap is identity */
+}
+
+/* ------------------------------------------------------------ mass flow */
+static void mdot_init(or_case *c)
+{
+    /* initializeMassFlowRateInterior_: shape functions shifted iff the
+     * velocity interpolation is linear_linear (flowModel.cpp:3310-3316) */
+    double sf[96], co[24], av[36];
+    or_hex_scs_shape(c->prm.u_shifted, sf);
+    for (int e = 0; e < c->ne; ++e) {
+        const int *en = &c->conn[8 * e];
+        gather_coords(c, e, co);
+        or_hex_scs_areav(co, av);
+        for (int ip = 0; ip < 12; ++ip) {
+            double u[3] = {0, 0, 0}, rip = 0;
+            for (int ic = 0; ic < 8; ++ic) {
+                const double r = sf[8 * ip + ic];
+                rip += r * c->rho[en[ic]];
+                for (int j = 0; j < 3; ++j) u[j] += r * c->U[3 * en[ic] + j];
+            }
+            double t = 0;
+            for (int j = 0; j < 3; ++j) t += rip * u[j] * av[3 * ip + j];
+            c->mdot[12 * e + ip] = t;
+        }
+    }
+}
+
+void or_update_mdot(or_case *c)
+{
+    const int *lr = or_hex_lrscv();
+    double sfu[96], sfc[96], co[24], av[36], dndx[288], detj[12], w[8];
+    or_hex_scs_shape(c->prm.u_shifted, sfu);
+    or_hex_scs_shape(0, sfc);
+    const double urf = c->prm.urf_mdot;
+    for (int e = 0; e < c->ne; ++e) {
+        const int *en = &c->conn[8 * e];
+        gather_coords(c, e, co);
+        or_hex_scs_areav(co, av);
+        scv_weights(co, w);
+        or_hex_scs_gradop(c->prm.pgrad_shifted, co, dndx, detj);
+        double *md = &c->mdot[12 * e];
+        for (int ip = 0; ip < 12; ++ip) {
+            const int il = lr[2 * ip], ir = lr[2 * ip + 1];
+            double xip[3] = {0, 0, 0}, uip[3] = {0, 0, 0}, Gp[3] = {0, 0, 0};
+            double dp[3] = {0, 0, 0}, duip[3] = {0, 0, 0}, rip = 0;
+            for (int ic = 0; ic < 8; ++ic) {
+                const int n = en[ic];
+                const double rv = sfu[8 * ip + ic], rc = sfc[8 * ip + ic], ws = w[ic];
+                rip += rv * c->rho[n];
+                for (int j = 0; j < 3; ++j) {
+                    duip[j] += rv * c->du[3 * n + j];
+                    uip[j] += rv * c->U[3 * n + j];
+                    dp[j] += dndx[24 * ip + 3 * ic + j] * c->p[n];
+                    xip[j] += rc * co[3 * ic + j];
+                    Gp[j] += ws * c->gradP[3 * n + j];
+                }
+            }
+            const double ur = md[ip] / rip;
+            /* deferred correction of rho vanishes (blending factor 0 for
+             * incompressible upwind); rhoHR = upwind rho */
+            const double rhoHR = ur > 0 ? c->rho[en[il]] : c->rho[en[ir]];
+            double t = 0;
+            for (int j = 0; j < 3; ++j) {
+                t += rhoHR * uip[j] * av[3 * ip + j];
+                t -= rhoHR * duip[j] * (dp[j] - Gp[j]) * av[3 * ip + j];
+            }
+            md[ip] = urf * t + (1.0 - urf) * md[ip];
+        }
+    }
+}
+
+void or_mass_divergence(or_case *c)
+{
+    const int *lr = or_hex_lrscv();
+    memset(c->div, 0, sizeof(double) * (size_t)c->nn);
+    for (int e = 0; e < c->ne; ++e) {
+        const int *en = &c->conn[8 * e];
+        for (int ip = 0; ip < 12; ++ip) {
+            const double m = c->mdot[12 * e + ip];
+            c->div[en[lr[2 * ip]]] += m;
+            c->div[en[lr[2 * ip + 1]]] -= m;
+        }
+    }
+    /* wall boundaries carry zero side mass flow: no boundary contribution */
+}
+
+/* ------------------------------------------------------------- gradients */
+void or_gradient(or_case *c, int which)
+{
+    const int N = which == 0 ? 3 : 1;
+    double *phi = which == 0 ? c->U : c->p;
+    double *grad = which == 0 ? c->gradU : c->gradP;
+    const int shifted = which == 0 ? c->prm.u_shifted : c->prm.p_shifted;
+    const double gurf = c->prm.grad_urf;
+    const int *lr = or_hex_lrscv();
+    const int *sn = or_hex_side_nodes();
+    /* relax the existing gradient (nodeField.hpp:3268, relax :6362) */
+    for (size_t k = 0; k < (size_t)c->nn * 3 * N; ++k) grad[k] *= (1.0 - gurf);
+    double sf[96], co[24], av[36];
+    or_hex_scs_shape(shifted, sf);
+    for (int e = 0; e < c->ne; ++e) {
+        const int *en = &c->conn[8 * e];
+        gather_coords(c, e, co);
+        or_hex_scs_areav(co, av);
+        for (int ip = 0; ip < 12; ++ip) {
+            const int L = en[lr[2 * ip]], R = en[lr[2 * ip + 1]];
+            double pip[3] = {0, 0, 0};
+            for (int ic = 0; ic < 8; ++ic)
+                for (int j = 0; j < N; ++j) pip[j] += sf[8 * ip + ic] * phi[N * en[ic] + j];
+            const double ivL = 1.0 / c->vol[L], ivR = 1.0 / c->vol[R];
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    grad[3 * N * L + i * N + j] +=
+                        gurf * (pip[i] - phi[N * L + i]) * av[3 * ip + j] * ivL;
+                    grad[3 * N * R + i * N + j] -=
+                        gurf * (pip[i] - phi[N * R + i]) * av[3 * ip + j] * ivR;
+                }
+        }
+    }
+    /* boundary ip contribution (walls; symmetry planes are skipped) */
+    double fsf[16];
+    or_quad_face_shape(shifted, fsf);
+    for (int f = 0; f < c->nbf; ++f) {
+        const int *en = &c->conn[8 * c->bf_elem[f]];
+        int fn[4];
+        for (int k = 0; k < 4; ++k) fn[k] = en[sn[4 * c->bf_ord[f] + k]];
+        const double *aw = &c->bareav[12 * f];
+        for (int ip = 0; ip < 4; ++ip) {
+            const int nn_ = fn[ip];
+            double pip[3] = {0, 0, 0};
+            for (int ic = 0; ic < 4; ++ic)
+                for (int j = 0; j < N; ++j) pip[j] += fsf[4 * ip + ic] * phi[N * fn[ic] + j];
+            const double iv = 1.0 / c->vol[nn_];
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < 3; ++j)
+                    grad[3 * N * nn_ + i * N + j] +=
+                        gurf * (pip[i] - phi[N * nn_ + i]) * aw[3 * ip + j] * iv;
+        }
+    }
+}
+
+/* ------------------------------------------------------------- momentum */
+void or_assemble_momentum(or_case *c)
+{
+    const int nn = c->nn;
+    const or_params *P = &c->prm;
+    memset(c->A3, 0, sizeof(double) * 9 * (size_t)c->nnz);
+    memset(c->b3, 0, sizeof(double) * 3 * (size_t)nn);
+    double lhs[576], rhs[24];
+    int nodes[8];
+
+    /* node terms, steady (navierStokesAssemblerNodeTerms.cpp:16-149) */
+    for (int i = 0; i < nn; ++i) {
+        double l9[9] = {0}, r3[3] = {0};
+        const double rho = c->rho[i], vol = c->vol[i], div = c->div[i];
+        const double lhsfac = rho * vol / P->dt;
+        for (int k = 0; k < 3; ++k) {
+            if (div < 0) l9[4 * k] += -div;
+            const double Ui = c->U[3 * i + k];
+            r3[k] -= -div * Ui;
+            r3[k] -= c->gradP[3 * i + k] * vol;
+            l9[4 * k] += lhsfac;
+        }
+        or_apply_coeff(3, 1, &i, l9, r3, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
+    }
+
+    /* interior element terms */
+    const int *lr = or_hex_lrscv();
+    double sfu[96], sfc[96], co[24], av[36], dndx[288], detj[12];
+    or_hex_scs_shape(P->u_shifted, sfu);
+    or_hex_scs_shape(0, sfc);
+    for (int e = 0; e < c->ne; ++e) {
+        const int *en = &c->conn[8 * e];
+        memset(lhs, 0, sizeof(lhs));
+        memset(rhs, 0, sizeof(rhs));
+        double Uw[24], mu[8];
+        for (int a = 0; a < 8; ++a) {
+            nodes[a] = en[a];
+            mu[a] = c->mu[en[a]];
+            for (int d = 0; d < 3; ++d) Uw[3 * a + d] = c->U[3 * en[a] + d];
+        }
+        gather_coords(c, e, co);
+        or_hex_scs_areav(co, av);
+        or_hex_scs_gradop(P->ugrad_shifted, co, dndx, detj);
+        const double *md = &c->mdot[12 * e];
+        for (int ip = 0; ip < 12; ++ip) {
+            const int il = lr[2 * ip], ir = lr[2 * ip + 1];
+            const double tm = md[ip];
+            double muIp = 0, xip[3] = {0, 0, 0};
+            for (int ic = 0; ic < 8; ++ic) {
+                muIp += sfu[8 * ip + ic] * mu[ic];
+                for (int i = 0; i < 3; ++i) xip[i] += sfc[8 * ip + ic] * co[3 * ic + i];
+            }
+            for (int i = 0; i < 3; ++i) {
+                double uup, dcorr = 0;
+                const int iu = tm > 0 ? il : ir;
+                uup = Uw[3 * iu + i];
+                for (int j = 0; j < 3; ++j)
+                    dcorr += c->beta[3 * en[iu] + i] * (xip[j] - co[3 * iu + j]) *
+                             c->gradU[9 * en[iu] + 3 * i + j];
+                const double aflux = tm * (uup + dcorr);
+                const int iL = 3 * il + i, iR = 3 * ir + i;
+                const int rowL = iL * 24, rowR = iR * 24;
+                rhs[iL] -= aflux;
+                rhs[iR] += aflux;
+                const double aL = 0.5 * (tm + fabs(tm)), aR = 0.5 * (tm - fabs(tm));
+                lhs[rowL + 3 * il + i] += aL;
+                lhs[rowR + 3 * il + i] -= aL;
+                lhs[rowR + 3 * ir + i] -= aR;
+                lhs[rowL + 3 * ir + i] += aR;
+            }
+            for (int ic = 0; ic < 8; ++ic) {
+                const int icN = 3 * ic;
+                for (int i = 0; i < 3; ++i) {
+                    const int iL = 3 * il + i, iR = 3 * ir + i;
+                    const int rowL = iL * 24, rowR = iR * 24;
+                    const double *dn = &dndx[24 * ip + icN];
+                    double lhs_riC_i = 0;
+                    for (int j = 0; j < 3; ++j) {
+                        const double axj = av[3 * ip + j];
+                        const double uj = Uw[icN + j];
+                        lhs_riC_i += -muIp * dn[j] * axj;
+                        const double lfj = -muIp * dn[i] * axj;
+                        lhs[rowL + icN + j] += lfj;
+                        lhs[rowR + icN + j] -= lfj;
+                        rhs[iL] -= lfj * uj;
+                        rhs[iR] += lfj * uj;
+                    }
+                    lhs[rowL + icN + i] += lhs_riC_i;
+                    lhs[rowR + icN + i] -= lhs_riC_i;
+                    const double ui = Uw[icN + i];
+                    rhs[iL] -= lhs_riC_i * ui;
+                    rhs[iR] += lhs_riC_i * ui;
+                }
+            }
+        }
+        or_apply_coeff(3, 8, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
+    }
+
+    /* no-slip walls, laminar (navierStokesAssemblerElemBoundaryConditions.cpp:2659-2884) */
+    const int *sn = or_hex_side_nodes();
+    double fsf[16];
+    or_quad_face_shape(P->u_shifted, fsf);
+    for (int f = 0; f < c->nbf; ++f) {
+        const int e = c->bf_elem[f], ord = c->bf_ord[f];
+        const int *en = &c->conn[8 * e];
+        memset(lhs, 0, sizeof(lhs));
+        memset(rhs, 0, sizeof(rhs));
+        double Uw[24], bcm[8], muf[4];
+        for (int a = 0; a < 8; ++a) {
+            nodes[a] = en[a];
+            bcm[a] = 1.0;
+            for (int d = 0; d < 3; ++d) Uw[3 * a + d] = c->U[3 * en[a] + d];
+        }
+        const double *Ubc = &c->bc_U[3 * c->bf_bc[f]];
+        for (int k = 0; k < 4; ++k) {
+            const int ic = sn[4 * ord + k];
+            muf[k] = c->mu[en[ic]];
+            bcm[ic] = 0.0;
+            for (int d = 0; d < 3; ++d) Uw[3 * ic + d] = Ubc[d];
+        }
+        gather_coords(c, e, co);
+        double fdn[96], fdet[4];
+        or_hex_face_gradop(P->ugrad_shifted, ord, co, fdn, fdet);
+        const double *aw = &c->bareav[12 * f];
+        for (int ip = 0; ip < 4; ++ip) {
+            const int nearest = sn[4 * ord + ip];
+            const double *ax = &aw[3 * ip];
+            const double amag = sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+            double mub = 0;
+            for (int ic = 0; ic < 4; ++ic) mub += fsf[4 * ip + ic] * muf[ic];
+            double nx[3];
+            for (int i = 0; i < 3; ++i) nx[i] = ax[i] / amag;
+            for (int ic = 0; ic < 8; ++ic) {
+                const double *dn = &fdn[24 * ip + 3 * ic];
+                for (int j = 0; j < 3; ++j) {
+                    const double axj = ax[j], dnj = dn[j], uxj = Uw[3 * ic + j];
+                    for (int i = 0; i < 3; ++i) {
+                        const int iR = 3 * nearest + i, rowR = iR * 24;
+                        const double dni = dn[i], uxi = Uw[3 * ic + i], nxi = nx[i];
+                        const double om = 1.0 - nxi * nxi;
+                        double lf = -mub * dnj * axj * om;
+                        lhs[rowR + 3 * ic + i] += lf * bcm[ic];
+                        rhs[iR] -= lf * uxi;
+                        lf = -mub * dni * axj * om;
+                        lhs[rowR + 3 * ic + j] += lf * bcm[ic];
+                        rhs[iR] -= lf * uxj;
+                        for (int l = 0; l < 3; ++l) {
+                            if (l == i) continue;
+                            const double nn_ = nxi * nx[l];
+                            const double uxl = Uw[3 * ic + l], dnl = dn[l];
+                            lf = mub * dnj * axj * nn_;
+                            lhs[rowR + 3 * ic + l] += lf * bcm[ic];
+                            rhs[iR] -= lf * uxl;
+                            lf = mub * dnl * axj * nn_;
+                            lhs[rowR + 3 * ic + j] += lf * bcm[ic];
+                            rhs[iR] -= lf * uxj;
+                        }
+                    }
+                }
+            }
+        }
+        or_apply_coeff(3, 8, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
+    }
+
+    /* postAssemble_: relaxation diag /= urf (phiAssembler.h:966-1006), then
+     * boundary rhs relaxation on wall nodes (navierStokesAssembler.cpp:238,
+     * phiAssembler.h:1008-1152) */
+    const double uinv = 1.0 / P->urf_U;
+    for (int i = 0; i < nn; ++i) {
+        double *d = &c->A3[9 * ((size_t)c->row_ptr[i] + c->diag_off[i])];
+        for (int k = 0; k < 3; ++k) d[4 * k] *= uinv;
+    }
+    for (int i = 0; i < nn; ++i)
+        if (c->bnode[i])
+            for (int k = 0; k < 3; ++k) c->b3[3 * i + k] *= P->bnd_relax;
+}
+
+void or_compute_du(or_case *c)
+{
+    for (int i = 0; i < c->nn; ++i) {
+        const double *d = &c->A3[9 * ((size_t)c->row_ptr[i] + c->diag_off[i])];
+        for (int k = 0; k < 3; ++k) c->du[3 * i + k] = c->vol[i] / (d[4 * k] + SMALL);
+    }
+}
+
+/* -------------------------------------------------------- p correction */
+void or_assemble_pcorr(or_case *c)
+{
+    const int nn = c->nn;
+    const or_params *P = &c->prm;
+    memset(c->A1, 0, sizeof(double) * (size_t)c->nnz);
+    memset(c->b1, 0, sizeof(double) * (size_t)nn);
+    const int *lr = or_hex_lrscv();
+    double sfu[96], sfc[96], co[24], av[36], dndx[288], detj[12], w[8];
+    double lhs[64], rhs[8];
+    int nodes[8];
+    or_hex_scs_shape(P->u_shifted, sfu);
+    or_hex_scs_shape(0, sfc);
+    for (int e = 0; e < c->ne; ++e) {
+        const int *en = &c->conn[8 * e];
+        memset(lhs, 0, sizeof(lhs));
+        memset(rhs, 0, sizeof(rhs));
+        for (int a = 0; a < 8; ++a) nodes[a] = en[a];
+        gather_coords(c, e, co);
+        or_hex_scs_areav(co, av);
+        scv_weights(co, w);
+        or_hex_scs_gradop(P->pgrad_shifted, co, dndx, detj);
+        const double *md = &c->mdot[12 * e];
+        for (int ip = 0; ip < 12; ++ip) {
+            const int il = lr[2 * ip], ir = lr[2 * ip + 1];
+            const double tm = md[ip];
+            const int rowL = il * 8, rowR = ir * 8;
+            double uip[3] = {0, 0, 0}, duip[3] = {0, 0, 0}, dp[3] = {0, 0, 0}, Gp[3] = {0, 0, 0};
+            for (int ic = 0; ic < 8; ++ic) {
+                const int n = en[ic];
+                const double rv = sfu[8 * ip + ic], ws = w[ic];
+                for (int j = 0; j < 3; ++j) {
+                    uip[j] += rv * c->U[3 * n + j];
+                    duip[j] += rv * c->du[3 * n + j];
+                    dp[j] += dndx[24 * ip + 3 * ic + j] * c->p[n];
+                    Gp[j] += ws * c->gradP[3 * n + j];
+                }
+            }
+            const double rhoHR = tm > 0 ? c->rho[en[il]] : c->rho[en[ir]];
+            for (int ic = 0; ic < 8; ++ic) {
+                double lf = 0;
+                for (int j = 0; j < 3; ++j)
+                    lf += -rhoHR * duip[j] * dndx[24 * ip + 3 * ic + j] * av[3 * ip + j];
+                lhs[rowL + ic] += lf;
+                lhs[rowR + ic] -= lf;
+            }
+            double m = 0;
+            for (int j = 0; j < 3; ++j) {
+                m += rhoHR * uip[j] * av[3 * ip + j];
+                m -= rhoHR * duip[j] * (dp[j] - Gp[j]) * av[3 * ip + j];
+            }
+            rhs[il] -= m;
+            rhs[ir] += m;
+        }
+        or_apply_coeff(1, 8, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A1, c->b1);
+    }
+    /* no-slip static walls add nothing to p' (no frame/mesh motion) */
+    /* pressure reference: zero the row, keep the diagonal, b = 0 */
+    const int r = P->p_ref_node;
+    if (r >= 0 && r < nn) {
+        const double d = c->A1[c->row_ptr[r] + c->diag_off[r]];
+        for (int k = c->row_ptr[r]; k < c->row_ptr[r + 1]; ++k) c->A1[k] = 0.0;
+        c->A1[c->row_ptr[r] + c->diag_off[r]] = d;
+        c->b1[r] = 0.0;
+    }
+}
+
+/* ----------------------------------------------------------- correction */
+void or_correct_U(or_case *c, double urf)
+{
+    for (size_t k = 0; k < (size_t)c->nn * 3; ++k) c->U[k] += urf * c->x3[k];
+}
+
+void or_correct_p(or_case *c, double urf)
+{
+    for (int k = 0; k < c->nn; ++k) c->p[k] += urf * c->x1[k];
+}
+
+void or_velocity_correction(or_case *c, int step)
+{
+    const double f = (step == 1 ? 1.0 : -1.0) / c->prm.urf_p;
+    for (size_t k = 0; k < (size_t)c->nn * 3; ++k) c->U[k] += f * c->du[k] * c->gradP[k];
+}
+
+static void field_scale(const double *v, int n, int N, double *scale, double *mx, double *mn)
+{
+    double maxv = -DBL_MAX, minv = DBL_MAX;
+    for (int i = 0; i < n; ++i) {
+        double val;
+        if (N == 1)
+            val = v[i];
+        else {
+            double s = 0;
+            for (int k = 0; k < N; ++k) s += v[N * i + k] * v[N * i + k];
+            val = sqrt(s);
+        }
+        if (val > maxv) maxv = val;
+        if (val < minv) minv = val;
+    }
+    double sc = (maxv > 0 ? maxv : 0.0) - (minv < 0 ? minv : 0.0);
+    if (sc < 1.0e-9) sc = 1.0;
+    if (maxv > sc) sc = maxv;
+    *scale = sc;
+    *mx = maxv;
+    *mn = minv;
+}
+
+void or_update_scales(or_case *c, double *U_scale, double *p_scale)
+{
+    double s, mx, mn, ps, pmx, pmn, rs, rmx, rmn;
+    field_scale(c->U, c->nn, 3, &s, &mx, &mn);
+    field_scale(c->p, c->nn, 1, &ps, &pmx, &pmn);
+    field_scale(c->rho, c->nn, 1, &rs, &rmx, &rmn);
+    c->U_scale = s;
+    const double a = pmx - pmn, b = 0.5 * rs * s * s;
+    c->p_scale = a > b ? a : b;
+    if (U_scale) *U_scale = c->U_scale;
+    if (p_scale) *p_scale = c->p_scale;
+}
+
+void or_scaled_residual(or_case *c, int N, double *out)
+{
+    const double *A = N == 3 ? c->A3 : c->A1;
+    const double *b = N == 3 ? c->b3 : c->b1;
+    for (int k = 0; k < N; ++k) {
+        double s = 0;
+        for (int i = 0; i < c->nn; ++i) {
+            const double d = A[(size_t)N * N * (c->row_ptr[i] + c->diag_off[i]) + (N + 1) * k];
+            const double r = b[N * i + k] / d;
+            s += r * r;
+        }
+        out[k] = sqrt(s / c->nn);
+    }
+}
+
+/* ------------------------------------------------------------ initialize */
+void or_case_initialize(or_case *c)
+{
+    /* navierStokesEquation::initialize (:86-99): gradient, scale */
+    memset(c->gradU, 0, sizeof(double) * 9 * (size_t)c->nn);
+    memset(c->gradP, 0, sizeof(double) * 3 * (size_t)c->nn);
+    or_gradient(c, 0);
+    /* pressureCorrectionEquation::initialize (:59-70) */
+    or_gradient(c, 1);
+    or_update_scales(c, NULL, NULL);
+    /* postInitialize: mass flow rate and divergence (:101-133) */
+    mdot_init(c);
+    or_mass_divergence(c);
+}
+
+/* -------------------------------------------------------------- outer */
+void or_outer_iteration(or_case *c, const double *ctl, double *hist)
+{
+    const or_params *P = &c->prm;
+    double res[2], r3[3], r1;
+    /* residual scales in force when each system is reported: those of the
+     * previous iteration's end (linearSystem.h:475, navierStokesEquation.cpp:230,
+     * pressureCorrectionEquation.cpp:165) */
+    const double us0 = c->U_scale, ps0 = c->p_scale;
+    /* momentum predictor (navierStokesEquation::solve :157-222) */
+    or_assemble_momentum(c);
+    or_compute_du(c);
+    or_scaled_residual(c, 3, r3);
+    memset(c->x3, 0, sizeof(double) * 3 * (size_t)c->nn);
+    const int itU = or_solve(c, 3, 0, (int)ctl[0], ctl[1], 1e-30, 30, res);
+    or_correct_U(c, 1.0);
+    or_update_mdot(c);
+    double us, ps;
+    or_update_scales(c, &us, &ps);
+    /* pressure correction (pressureCorrectionEquation::solve :89-164) */
+    or_assemble_pcorr(c);
+    or_scaled_residual(c, 1, &r1);
+    memset(c->x1, 0, sizeof(double) * (size_t)c->nn);
+    const int itP = or_solve(c, 1, 1, (int)ctl[2], ctl[3], 1e-30, 30, res);
+    or_correct_p(c, P->urf_p);
+    /* segregatedFlowEquations.cpp:106-285 */
+    or_update_mdot(c);
+    or_mass_divergence(c);
+    or_velocity_correction(c, 1);
+    or_gradient(c, 1);
+    or_velocity_correction(c, 2);
+    or_update_scales(c, &us, &ps);
+    or_gradient(c, 0);
+    if (hist) {
+        for (int k = 0; k < 3; ++k) hist[k] = r3[k] / (us0 + SMALL);
+        hist[3] = r1 / (ps0 + SMALL);
+        hist[4] = itU;
+        hist[5] = itP;
+    }
+}

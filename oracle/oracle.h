@@ -1,0 +1,164 @@
+/*
+ * oracle.h -- CPU restatement of OpenAccel's CVFEM assemble-and-solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * product in openaccel_amd/ and the CPU baseline timed by bench.py.  It is
+ * never linked into, loaded by, or called from the product path.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ *
+ * Every function cites the reference file:line it restates.  Arithmetic is
+ * FP64 (reference `scalar = double`, src/basic/types.h:103-105) with int32
+ * local ids (`label = int`).
+ *
+ * Layouts follow the reference's STK field data:
+ *   coordinates / velocity / du / grad p : [node][3]
+ *   grad U                               : [node][9], (i*3+j) = dU_i/dx_j
+ *   mass flow rate                       : [elem][n_scs_ip]
+ *   block-CRS values                     : [nnz][N*N] row-major N x N blocks
+ *   rhs / solution                       : [node][N]
+ */
+#ifndef OA_ORACLE_H
+#define OA_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* topology codes shared with the product's C ABI */
+enum { OR_HEX8 = 0, OR_TET4 = 1, OR_WED6 = 2, OR_PYR5 = 3 };
+
+/* ---------------------------------------------------------------- geometry */
+/* hex8 SCS area vectors [12][3] (MasterElementWork.F:1-231 hex_scs_det,
+ * CalculateQuadArea.F quadAreaByTriangleFacets) */
+void or_hex_scs_areav(const double *coords /*[8][3]*/, double *areav /*[12][3]*/);
+/* hex8 SCV volumes [8] (MasterElementWork.F:233-457 hex_scv_det,
+ * CalculateHexVolume.F, CalculatePolyhedralVolume.F) */
+void or_hex_scv_vol(const double *coords, double *vol /*[8]*/);
+/* hex8 shape functions at npts isoparametric points (MasterElementWork.F:2344) */
+void or_hex_shape(int npts, const double *par /*[npts][3]*/, double *shape /*[npts][8]*/);
+/* hex8 shape derivatives (MasterElementWork.F:2404) */
+void or_hex_deriv(int npts, const double *par, double *deriv /*[npts][8][3]*/);
+/* gradient operator (MasterElementWork.F:2623 hex_gradient_operator);
+ * returns the number of ips with det_j <= 1e6*realmin (err flag) */
+int or_hex_gradop(int npts, const double *deriv, const double *coords,
+                  double *gradop /*[npts][8][3]*/, double *detj /*[npts]*/);
+/* SCS gradient operator at the standard (shifted=0) or edge-midpoint
+ * (shifted=1) ips (Hex8CVFEM.C:612-662) */
+int or_hex_scs_gradop(int shifted, const double *coords, double *gradop, double *detj);
+/* face gradient operator of face `ord` at its 4 ips (Hex8CVFEM.C:686-783) */
+int or_hex_face_gradop(int shifted, int ord, const double *coords,
+                       double *gradop /*[4][8][3]*/, double *detj /*[4]*/);
+/* SCS shape functions [12][8] at standard / shifted ips (Hex8CVFEM.C:833-850) */
+void or_hex_scs_shape(int shifted, double *shape);
+/* quad4 exposed-face area vectors [4][3] (MasterElementWork.F:1467 quad3d_scs_det) */
+void or_quad_face_areav(const double *coords /*[4][3]*/, double *areav /*[4][3]*/);
+/* quad4 face shape functions [4][4] (Quad43DCVFEM.C:38-48, MasterElementWork.F:3651) */
+void or_quad_face_shape(int shifted, double *shape);
+/* master-element tables (Hex8CVFEM.C:264-276, 381-393, 419-426) */
+const int *or_hex_lrscv(void);          /* [12][2] */
+const int *or_hex_side_nodes(void);     /* [6][4]  */
+
+/* ------------------------------------------------------------- CRS graph */
+/* nodeGraph::buildGraph_ (mesh/graph/nodeGraph.cpp:29-392): reduced stencil
+ * (SCS-ip pairs, :68-96) when reduced != 0, else full element stencil
+ * (:98-118).  Owned rows 0..nn-1, sorted local columns.  row_ptr[nn+1] must
+ * be allocated by the caller; cols is allocated here (free with or_free). */
+int or_graph_build(int nn, int ne, const int *conn, int reduced,
+                   int *row_ptr, int **cols_out);
+void or_free(void *p);
+/* position of `col` in row `row`, or -1 (the sorted search of
+ * assembler.h:380-398) */
+int or_graph_find(const int *row_ptr, const int *cols, int row, int col);
+
+/* --------------------------------------------------------------- scatter */
+/* assembler<N>::applyCoeff_ (assemble/assembler.h:311-453): add the dense
+ * local system lhs[(r*N+m)*(n*N)+c*N+k], rhs[r*N+m] over `n` nodes into the
+ * block-CRS (row_ptr, cols, vals[nnz][N*N]) and b, owned rows only
+ * (row < n_owned), columns missing from the row lumped into the diagonal
+ * block (deductUnfound). diag_off[row] is the position of the diagonal. */
+void or_apply_coeff(int N, int n, const int *nodes, const double *lhs,
+                    const double *rhs, int n_owned, const int *row_ptr,
+                    const int *cols, const int *diag_off, double *vals,
+                    double *b);
+
+/* -------------------------------------------------------------- the case */
+typedef struct or_params {
+    double rho, mu;          /* constant density / dynamic viscosity     */
+    double dt;               /* physical (pseudo) timescale              */
+    double urf_U, urf_p;     /* velocity (implicit) / pressure relaxation */
+    double urf_mdot;         /* mass flow relaxation (relaxMass_)        */
+    double grad_urf;         /* gradient relaxation, nodeField.h:152      */
+    double bnd_relax;        /* boundary rhs relaxation, navierStokesAssembler.cpp:238 */
+    int u_shifted;           /* velocity interpolation linear_linear      */
+    int ugrad_shifted;       /* velocity gradient interpolation           */
+    int p_shifted;           /* pressure interpolation linear_linear      */
+    int pgrad_shifted;       /* pressure gradient interpolation           */
+    int reduced;             /* bandwidth_reduction (reduced stencil)     */
+    int p_ref_node;          /* pressure-level node (-1: none)            */
+} or_params;
+
+typedef struct or_case or_case;
+
+/* Build the oracle state for a hex8 mesh.  Boundary faces: parent element,
+ * face ordinal and boundary id; bc_U[nbc][3] wall velocities. */
+or_case *or_case_create(int nn, int ne, const double *xyz, const int *conn,
+                        int nbf, const int *bf_elem, const int *bf_ord,
+                        const int *bf_bc, int nbc, const double *bc_U,
+                        const or_params *prm);
+void or_case_destroy(or_case *c);
+
+/* graph access */
+int or_case_nn(const or_case *c);
+int or_case_nnz(const or_case *c);
+const int *or_case_row_ptr(const or_case *c);
+const int *or_case_cols(const or_case *c);
+const int *or_case_diag_off(const or_case *c);
+
+/* field access (pointers into the case; caller may write them) */
+double *or_case_field(or_case *c, const char *name);
+
+/* flowModel::initializeMassFlowRateInterior_ (flowModel.cpp:3303-3470) and
+ * the initialize()/postInitialize() sequence of navierStokesEquation.cpp:86-133
+ * and pressureCorrectionEquation.cpp:59-74 */
+void or_case_initialize(or_case *c);
+
+/* mesh::updateDualNodalVolumeField_ (meshGeometry.cpp:1277) -> "vol" */
+void or_dual_volume(or_case *c);
+
+/* systems.  *_assemble fill the case's own matrix (vals/b) for N=3 or N=1. */
+void or_assemble_momentum(or_case *c);   /* navierStokesEquation::solve :159-173 */
+void or_compute_du(or_case *c);          /* navierStokesAssembler.cpp:84-232     */
+void or_assemble_pcorr(or_case *c);      /* pressureCorrectionEquation::solve :89-107 */
+double *or_case_vals(or_case *c, int N); /* [nnz][N*N] */
+double *or_case_rhs(or_case *c, int N);  /* [nn][N]     */
+double *or_case_x(or_case *c, int N);    /* [nn][N]     */
+
+/* field updates */
+void or_update_mdot(or_case *c);         /* flowModel.cpp:5272-5688 */
+void or_mass_divergence(or_case *c);     /* flowModel.cpp:13541-13775 */
+void or_gradient(or_case *c, int which); /* nodeField.hpp:3253-4950; 0=U, 1=p */
+void or_correct_U(or_case *c, double urf);   /* equation.h:204-286 (N=3) */
+void or_correct_p(or_case *c, double urf);   /* equation.h:204-286 (N=1) */
+void or_velocity_correction(or_case *c, int step); /* segregatedFlowEquations.cpp:133-279 */
+void or_update_scales(or_case *c, double *U_scale, double *p_scale); /* nodeField.hpp:5993, flowModel.cpp:117 */
+
+/* Krylov solves on the case's system.  kind: 0 = Jacobi-GMRES(m),
+ * 1 = Jacobi-CG, 2 = Jacobi-BiCGStab.  Returns iterations; res[0]/res[1] =
+ * initial / final ||b - A x||_2. */
+int or_solve(or_case *c, int N, int kind, int max_it, double rtol,
+             double atol, int restart, double *res);
+
+/* control residual of a system: RMS over rows of (b/diag), per component
+ * (definition used in place of liblinsolve's scaled_initial_res) */
+void or_scaled_residual(or_case *c, int N, double *out /*[N]*/);
+
+/* one steady outer iteration (simulation.cpp:314-377 ->
+ * segregatedFlowEquations::solve :76-297); ctl[0..3] = momentum solver max_it,
+ * rtol, p' solver max_it, rtol; hist[0..4] = U,V,W,P control residuals,
+ * hist[5..6] = momentum / p' solver iterations. */
+void or_outer_iteration(or_case *c, const double *ctl, double *hist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

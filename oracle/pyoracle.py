@@ -1,0 +1,286 @@
+"""ctypes binding of the oracle (liboracle.so) and of the reference's own
+Fortran master elements built into oracle/_ref/libnalu_me.so.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libnalu_me.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Params(C.Structure):
+    _fields_ = [("rho", C.c_double), ("mu", C.c_double), ("dt", C.c_double),
+                ("urf_U", C.c_double), ("urf_p", C.c_double), ("urf_mdot", C.c_double),
+                ("grad_urf", C.c_double), ("bnd_relax", C.c_double),
+                ("u_shifted", C.c_int), ("ugrad_shifted", C.c_int),
+                ("p_shifted", C.c_int), ("pgrad_shifted", C.c_int),
+                ("reduced", C.c_int), ("p_ref_node", C.c_int)]
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        L.or_case_create.restype = C.c_void_p
+        L.or_case_create.argtypes = [C.c_int, C.c_int, _dp, _ip, C.c_int, _ip, _ip, _ip,
+                                     C.c_int, _dp, C.POINTER(Params)]
+        L.or_case_destroy.argtypes = [C.c_void_p]
+        for f in ("or_case_nn", "or_case_nnz"):
+            getattr(L, f).argtypes = [C.c_void_p]
+            getattr(L, f).restype = C.c_int
+        for f in ("or_case_row_ptr", "or_case_cols", "or_case_diag_off"):
+            getattr(L, f).argtypes = [C.c_void_p]
+            getattr(L, f).restype = _ip
+        L.or_case_field.argtypes = [C.c_void_p, C.c_char_p]
+        L.or_case_field.restype = _dp
+        for f in ("or_case_vals", "or_case_rhs", "or_case_x"):
+            getattr(L, f).argtypes = [C.c_void_p, C.c_int]
+            getattr(L, f).restype = _dp
+        for f in ("or_case_initialize", "or_dual_volume", "or_assemble_momentum",
+                  "or_compute_du", "or_assemble_pcorr", "or_update_mdot",
+                  "or_mass_divergence"):
+            getattr(L, f).argtypes = [C.c_void_p]
+        L.or_gradient.argtypes = [C.c_void_p, C.c_int]
+        L.or_correct_U.argtypes = [C.c_void_p, C.c_double]
+        L.or_correct_p.argtypes = [C.c_void_p, C.c_double]
+        L.or_velocity_correction.argtypes = [C.c_void_p, C.c_int]
+        L.or_update_scales.argtypes = [C.c_void_p, _dp, _dp]
+        L.or_solve.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double,
+                               C.c_double, C.c_int, _dp]
+        L.or_solve.restype = C.c_int
+        L.or_scaled_residual.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.or_outer_iteration.argtypes = [C.c_void_p, _dp, _dp]
+        L.or_graph_build.argtypes = [C.c_int, C.c_int, _ip, C.c_int, _ip, C.POINTER(_ip)]
+        L.or_graph_build.restype = C.c_int
+        L.or_free.argtypes = [C.c_void_p]
+        L.or_graph_find.argtypes = [_ip, _ip, C.c_int, C.c_int]
+        L.or_graph_find.restype = C.c_int
+        L.or_apply_coeff.argtypes = [C.c_int, C.c_int, _ip, _dp, _dp, C.c_int, _ip, _ip,
+                                     _ip, _dp, _dp]
+        L.or_hex_scs_areav.argtypes = [_dp, _dp]
+        L.or_hex_scv_vol.argtypes = [_dp, _dp]
+        L.or_hex_shape.argtypes = [C.c_int, _dp, _dp]
+        L.or_hex_deriv.argtypes = [C.c_int, _dp, _dp]
+        L.or_hex_gradop.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+        L.or_hex_gradop.restype = C.c_int
+        L.or_hex_scs_gradop.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.or_hex_scs_gradop.restype = C.c_int
+        L.or_hex_face_gradop.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+        L.or_hex_face_gradop.restype = C.c_int
+        L.or_hex_scs_shape.argtypes = [C.c_int, _dp]
+        L.or_quad_face_areav.argtypes = [_dp, _dp]
+        L.or_quad_face_shape.argtypes = [C.c_int, _dp]
+        L.or_hex_lrscv.restype = _ip
+        L.or_hex_side_nodes.restype = _ip
+        _lib = L
+    return _lib
+
+
+def default_params(**kw) -> Params:
+    p = Params(rho=1.0, mu=0.01, dt=1.0, urf_U=0.9, urf_p=0.1, urf_mdot=1.0,
+               grad_urf=0.5625, bnd_relax=0.75, u_shifted=0, ugrad_shifted=1,
+               p_shifted=1, pgrad_shifted=1, reduced=1, p_ref_node=0)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Case:
+    """Oracle state of one segregated-flow case on a hex8 mesh."""
+
+    def __init__(self, mesh: dict, params: Params, bc_U: np.ndarray):
+        L = lib()
+        self.mesh = mesh
+        self.params = params
+        self.nn = mesh["xyz"].shape[0]
+        self.ne = mesh["conn"].shape[0]
+        bc = np.ascontiguousarray(bc_U, dtype=np.float64)
+        self._keep = [mesh["xyz"], mesh["conn"], mesh["bf_elem"], mesh["bf_ord"],
+                      mesh["bf_bc"], bc]
+        self.h = L.or_case_create(self.nn, self.ne, dptr(mesh["xyz"]), iptr(mesh["conn"]),
+                                  mesh["bf_elem"].size, iptr(mesh["bf_elem"]),
+                                  iptr(mesh["bf_ord"]), iptr(mesh["bf_bc"]),
+                                  bc.shape[0], dptr(bc), C.byref(params))
+        if not self.h:
+            raise RuntimeError("or_case_create failed")
+        self.nnz = L.or_case_nnz(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_case_destroy(self.h)
+            self.h = None
+
+    def _view(self, ptr, n):
+        return np.ctypeslib.as_array(ptr, shape=(n,))
+
+    @property
+    def row_ptr(self):
+        return self._view(lib().or_case_row_ptr(self.h), self.nn + 1)
+
+    @property
+    def cols(self):
+        return self._view(lib().or_case_cols(self.h), self.nnz)
+
+    @property
+    def diag_off(self):
+        return self._view(lib().or_case_diag_off(self.h), self.nn)
+
+    _sizes = {"U": 3, "p": 1, "gradU": 9, "gradP": 3, "du": 3, "div": 1, "vol": 1,
+              "rho": 1, "mu": 1, "beta": 3, "xyz": 3}
+
+    def field(self, name):
+        ptr = lib().or_case_field(self.h, name.encode())
+        if name == "mdot":
+            return self._view(ptr, self.ne * 12).reshape(self.ne, 12)
+        if name == "bareav":
+            return self._view(ptr, self.mesh["bf_elem"].size * 12).reshape(-1, 4, 3)
+        k = self._sizes[name]
+        v = self._view(ptr, self.nn * k)
+        return v if k == 1 else v.reshape(self.nn, k)
+
+    def vals(self, N):
+        return self._view(lib().or_case_vals(self.h, N), self.nnz * N * N).reshape(self.nnz, N, N)
+
+    def rhs(self, N):
+        return self._view(lib().or_case_rhs(self.h, N), self.nn * N)
+
+    def x(self, N):
+        return self._view(lib().or_case_x(self.h, N), self.nn * N)
+
+    def initialize(self):
+        lib().or_case_initialize(self.h)
+
+    def assemble_momentum(self):
+        lib().or_assemble_momentum(self.h)
+
+    def compute_du(self):
+        lib().or_compute_du(self.h)
+
+    def assemble_pcorr(self):
+        lib().or_assemble_pcorr(self.h)
+
+    def update_mdot(self):
+        lib().or_update_mdot(self.h)
+
+    def mass_divergence(self):
+        lib().or_mass_divergence(self.h)
+
+    def gradient(self, which):
+        lib().or_gradient(self.h, {"U": 0, "p": 1}[which])
+
+    def solve(self, N, kind, max_it, rtol, atol=1e-30, restart=30):
+        res = np.zeros(2)
+        it = lib().or_solve(self.h, N, kind, max_it, rtol, atol, restart, dptr(res))
+        return it, res
+
+    def scaled_residual(self, N):
+        out = np.zeros(N)
+        lib().or_scaled_residual(self.h, N, dptr(out))
+        return out
+
+    def scales(self):
+        u = np.zeros(1)
+        p = np.zeros(1)
+        lib().or_update_scales(self.h, dptr(u), dptr(p))
+        return float(u[0]), float(p[0])
+
+    def outer_iteration(self, ctl):
+        c = np.ascontiguousarray(ctl, dtype=np.float64)
+        h = np.zeros(6)
+        lib().or_outer_iteration(self.h, dptr(c), dptr(h))
+        return h
+
+
+# ---------------------------------------------------------------- reference
+_ref = None
+
+
+def ref_lib():
+    """The reference's Fortran master elements (oracle/_ref), or None."""
+    global _ref
+    if _ref is None and os.path.exists(REF_PATH):
+        _ref = C.CDLL(REF_PATH)
+    return _ref
+
+
+def _i(v):
+    return C.byref(C.c_int(v))
+
+
+def ref_hex_scs_areav(coords: np.ndarray) -> np.ndarray:
+    """hex_scs_det (MasterElementWork.F:1) on one element, [12][3]."""
+    R = ref_lib()
+    c = np.ascontiguousarray(coords, dtype=np.float64)
+    out = np.zeros((12, 3))
+    R.hex_scs_det_(_i(1), _i(8), _i(12), dptr(c), dptr(out))
+    return out
+
+
+def ref_hex_scv_vol(coords):
+    R = ref_lib()
+    c = np.ascontiguousarray(coords, dtype=np.float64)
+    vol = np.zeros(8)
+    err = np.zeros(1)
+    nerr = C.c_int(0)
+    R.hex_scv_det_(_i(1), _i(8), _i(8), dptr(c), dptr(vol), dptr(err), C.byref(nerr))
+    return vol
+
+
+def ref_hex_shape(par):
+    R = ref_lib()
+    p = np.ascontiguousarray(par, dtype=np.float64)
+    n = p.shape[0]
+    out = np.zeros((n, 8))
+    R.hex_shape_fcn_(_i(n), dptr(p), dptr(out))
+    return out
+
+
+def ref_hex_gradop(par, coords):
+    """hex_derivative + hex_gradient_operator at the given points."""
+    R = ref_lib()
+    p = np.ascontiguousarray(par, dtype=np.float64)
+    c = np.ascontiguousarray(coords, dtype=np.float64)
+    n = p.shape[0]
+    deriv = np.zeros((n, 8, 3))
+    R.hex_derivative_(_i(n), dptr(p), dptr(deriv))
+    g = np.zeros((n, 8, 3))
+    det = np.zeros(n)
+    err = np.zeros(1)
+    nerr = C.c_int(0)
+    R.hex_gradient_operator_(_i(1), _i(8), _i(n), dptr(deriv), dptr(c), dptr(g), dptr(det),
+                             dptr(err), C.byref(nerr))
+    return g, det
+
+
+def ref_quad_areav(coords4):
+    R = ref_lib()
+    c = np.ascontiguousarray(coords4, dtype=np.float64)
+    out = np.zeros((4, 3))
+    R.quad3d_scs_det_(_i(1), dptr(c), dptr(out))
+    return out
