@@ -1,0 +1,164 @@
+/*
+ * oagpu.h -- C ABI of the MI355X CVFEM assemble-and-solve path.
+ *
+ * The library replaces, behind OpenAccel's own interfaces, the bodies of
+ *   assembler<N>::assemble            src/assemble/assembler.h:46-51
+ *   linearSystem<N>::solve            src/equation/linearSystem.h:103-106, 416-462
+ *                                     (-> liblinsolve linearSolver::Base<N>::solve)
+ * and the field updates the segregated loop performs between them
+ *   segregatedFlowEquations::solve    src/equation/flow/segregatedFlowEquations.cpp:76-297
+ * for the momentum (block size 3) and pressure-correction (block size 1)
+ * systems.  All entry points take plain pointers and sizes, return 0 on
+ * success or a non-zero status (then oagpu_last_error() describes it), and
+ * never throw.  Host arrays use the reference's STK field layouts:
+ *   coordinates / velocity / du / grad p : [node][3]
+ *   grad U                               : [node][9], (i*3+j) = dU_i/dx_j
+ *   mass flow rate                       : [elem][12]   (hex8 SCS ips)
+ *   block-CRS values                     : [nnz][N*N], row-major N x N blocks
+ *   rhs / solution                       : [node][N]
+ * Local ids are the node indices given at mesh creation; every row is owned.
+ */
+#ifndef OAGPU_H
+#define OAGPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OAGPU_OK 0
+#define OAGPU_EINVAL 2   /* bad argument / unsupported input          */
+#define OAGPU_EHIP 3     /* HIP runtime failure                        */
+#define OAGPU_ENOMEM 4   /* allocation failure                         */
+#define OAGPU_EGEOM 5    /* non-positive Jacobian met by a kernel      */
+#define OAGPU_EINTERNAL 9
+
+/* graph layout, mirroring linearSolver::GraphLayout (src/mesh/graph/nodeGraph.h,
+ * controls.h:244 bandwidth_reduction) */
+#define OAGPU_STENCIL_REDUCED 1
+#define OAGPU_STENCIL_FULL 0
+
+/* arithmetic form: FP64 everywhere, or FP32 geometry/local matrices summed
+ * into FP64 global coefficients */
+#define OAGPU_FP64 0
+#define OAGPU_MIXED 1
+
+typedef struct oagpu_mesh oagpu_mesh;
+typedef struct oagpu_flow oagpu_flow;
+
+/* Physical and numerical parameters of a steady, laminar, constant-property
+ * segregated flow case (the hot-path subset of controls.h:57-290). */
+typedef struct oagpu_flow_params {
+    double rho, mu;        /* density, dynamic viscosity                    */
+    double dt;             /* physical_timescale (false transient)          */
+    double urf_U, urf_p;   /* velocity / pressure relaxation factors        */
+    double urf_mdot;       /* mass flow relaxation (relaxMass_)             */
+    double grad_urf;       /* gradient relaxation (nodeField.h:152)         */
+    double bnd_relax;      /* boundary rhs relaxation (navierStokesAssembler.cpp:238) */
+    int u_shifted;         /* velocity_interpolation_type == linear_linear  */
+    int ugrad_shifted;     /* velocity gradient interpolation linear_linear */
+    int p_shifted;         /* pressure interpolation linear_linear          */
+    int pgrad_shifted;     /* pressure gradient interpolation linear_linear */
+    int p_ref_node;        /* pressure-level node id, -1 for none           */
+    int precision;         /* OAGPU_FP64 or OAGPU_MIXED                     */
+} oagpu_flow_params;
+
+/* liblinsolve ControlData (linearSystem.h:475-482) as the device solve
+ * reports it */
+typedef struct oagpu_control_data {
+    int n_iterations;
+    double solver_initial_res; /* ||b - A x0||_2 */
+    double solver_final_res;   /* ||b - A x||_2  */
+    double scaled_initial_res[3];
+} oagpu_control_data;
+
+/* Krylov methods (linear_solver_settings family: MI355X) */
+#define OAGPU_GMRES 0
+#define OAGPU_CG 1
+#define OAGPU_BICGSTAB 2
+
+/* ---------------------------------------------------------------- runtime */
+int oagpu_init(int device);
+const char *oagpu_last_error(void);
+int oagpu_synchronize(void);
+
+/* --------------------------------------------------------------- mesh */
+/* Device mirror of a hex8 mesh: SoA-free interleaved coordinates, element
+ * connectivity in the nalu hex8 order, boundary sides (parent element, side
+ * ordinal 0..5, boundary id).  Builds the CRS node graph
+ * (nodeGraph::buildGraph_, src/mesh/graph/nodeGraph.cpp:29-392) with the
+ * given stencil, the node->element incidences and element->CRS scatter maps,
+ * the dual nodal volumes (mesh::updateDualNodalVolumeField_,
+ * src/mesh/meshGeometry.cpp:1277) and per-element SCV weights. */
+int oagpu_mesh_create(int n_nodes, const double *xyz, int n_elems, const int *conn,
+                      int n_bfaces, const int *bf_elem, const int *bf_ord,
+                      const int *bf_bc, int stencil, oagpu_mesh **out);
+int oagpu_mesh_destroy(oagpu_mesh *m);
+int oagpu_mesh_sizes(const oagpu_mesh *m, int *n_nodes, int *n_elems, int *nnz,
+                     int *n_inc, int *n_binc);
+/* host copies of the CRS graph (CRSNodeGraph offsets/indices, diagOffsetRef) */
+int oagpu_graph_copy(const oagpu_mesh *m, int *row_ptr, int *cols, int *diag_off);
+/* host copies of the node->element incidences ((elem<<3)|local node) and of
+ * their scatter maps (8 bytes per incidence: row position of each element
+ * node, 0xFF = lumped into the diagonal block) */
+int oagpu_scatter_map_copy(const oagpu_mesh *m, int *inc_ptr, int *inc,
+                           unsigned long long *slots);
+/* dual nodal volumes [n_nodes] */
+int oagpu_mesh_volume_copy(const oagpu_mesh *m, double *vol);
+
+/* --------------------------------------------------------------- flow */
+/* bc_U[n_bc][3]: wall velocity of each boundary id (no-slip walls). */
+int oagpu_flow_create(oagpu_mesh *m, const oagpu_flow_params *prm, int n_bc,
+                      const double *bc_U, oagpu_flow **out);
+int oagpu_flow_destroy(oagpu_flow *f);
+/* fields: "U" [nn][3], "p" [nn], "gradU" [nn][9], "gradP" [nn][3],
+ * "du" [nn][3], "mdot" [ne][12], "div" [nn] */
+int oagpu_field_upload(oagpu_flow *f, const char *name, const double *host, size_t count);
+int oagpu_field_download(oagpu_flow *f, const char *name, double *host, size_t count);
+
+/* navierStokesEquation::initialize/postInitialize (navierStokesEquation.cpp:86-133),
+ * pressureCorrectionEquation::initialize (pressureCorrectionEquation.cpp:59-70):
+ * gradients, scales, initial mass flow rate, mass divergence. */
+int oagpu_initialize(oagpu_flow *f);
+
+/* momentum system (navierStokesEquation::solve :159-173): node terms,
+ * interior element terms, no-slip walls, implicit relaxation, boundary rhs
+ * relaxation, then du = V / (a_ii + SMALL) (navierStokesAssembler.cpp:84-232) */
+int oagpu_assemble_momentum(oagpu_flow *f);
+/* pressure-correction system (pressureCorrectionEquation::solve :89-107)
+ * including the pressure-reference row (pressureCorrectionAssembler.cpp:19-63) */
+int oagpu_assemble_pcorr(oagpu_flow *f);
+/* host copy of a system: vals [nnz][N*N], b [nn][N] */
+int oagpu_system_download(oagpu_flow *f, int N, double *vals, double *b);
+int oagpu_system_upload(oagpu_flow *f, int N, const double *vals, const double *b);
+int oagpu_solution_download(oagpu_flow *f, int N, double *x);
+
+/* field updates */
+int oagpu_update_mdot(oagpu_flow *f);          /* flowModel.cpp:5272-5688   */
+int oagpu_mass_divergence(oagpu_flow *f);      /* flowModel.cpp:13541-13775 */
+int oagpu_gradient(oagpu_flow *f, int which);  /* nodeField.hpp:3253; 0 U, 1 p */
+int oagpu_correct(oagpu_flow *f, int N, double urf); /* equation.h:204-286  */
+int oagpu_velocity_correction(oagpu_flow *f, int step); /* segregatedFlowEquations.cpp:133-279 */
+int oagpu_update_scales(oagpu_flow *f, double *U_scale, double *p_scale); /* nodeField.hpp:5993 */
+/* control residual source: RMS over rows of b / a_diag per component */
+int oagpu_scaled_residual(oagpu_flow *f, int N, double *out);
+
+/* Solve A x = b of system N (3 momentum, 1 pressure correction) from x = 0
+ * with Jacobi-preconditioned Krylov `method`; converged when
+ * ||r|| <= max(rtol ||r0||, atol) or after max_it iterations. */
+int oagpu_solve(oagpu_flow *f, int N, int method, int max_it, double rtol, double atol,
+                int restart, oagpu_control_data *cd);
+
+/* One steady outer iteration of segregatedFlowEquations::solve.
+ * ctl = {momentum max_it, momentum rtol, p' max_it, p' rtol};
+ * hist[6] = U, V, W, P control residuals, momentum / p' Krylov iterations. */
+int oagpu_outer_iteration(oagpu_flow *f, const double *ctl, double *hist);
+
+/* device time of the last assembly kernels, milliseconds (HIP events) */
+int oagpu_last_kernel_ms(oagpu_flow *f, const char *which, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,297 @@
+"""openaccel_amd -- MI355X CVFEM assemble-and-solve path behind OpenAccel's
+assembler / linearSystem interfaces.
+
+The product is liboagpu.so (HIP, gfx950) with the C ABI of include/oagpu.h.
+This module is a thin ctypes binding of that ABI for Python callers, tests and
+bench.py.  It mirrors the reference's call sequence:
+
+  reference (C++)                                     here
+  ------------------------------------------------    ------------------------------
+  mesh + nodeGraph::buildGraph_ (nodeGraph.cpp:29)     DeviceMesh(...)
+  navierStokesAssembler::assemble (assembler.h:46)     FlowCase.assemble_momentum()
+  pressureCorrectionAssembler::assemble                FlowCase.assemble_pcorr()
+  linearSystem<N>::solve (linearSystem.h:103, 416)     FlowCase.solve(N, ...)
+  flowModel::updateMassFlowRate (flowModel.cpp:5272)   FlowCase.update_mdot()
+  flowModel::updateMassDivergenceField (:13541)        FlowCase.mass_divergence()
+  nodeField::updateGradientField (nodeField.hpp:3253)  FlowCase.gradient("U" | "p")
+  equation::correctField_ (equation.h:204)             FlowCase.correct(N, urf)
+  segregatedFlowEquations::solve (:76-297)             FlowCase.outer_iteration(ctl)
+
+There is no CPU fallback: if liboagpu.so is missing or no GPU is present the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboagpu.so")
+
+OAGPU_FP64, OAGPU_MIXED = 0, 1
+OAGPU_STENCIL_FULL, OAGPU_STENCIL_REDUCED = 0, 1
+OAGPU_GMRES, OAGPU_CG, OAGPU_BICGSTAB = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class FlowParams(C.Structure):
+    _fields_ = [("rho", C.c_double), ("mu", C.c_double), ("dt", C.c_double),
+                ("urf_U", C.c_double), ("urf_p", C.c_double), ("urf_mdot", C.c_double),
+                ("grad_urf", C.c_double), ("bnd_relax", C.c_double),
+                ("u_shifted", C.c_int), ("ugrad_shifted", C.c_int),
+                ("p_shifted", C.c_int), ("pgrad_shifted", C.c_int),
+                ("p_ref_node", C.c_int), ("precision", C.c_int)]
+
+
+class ControlData(C.Structure):
+    _fields_ = [("n_iterations", C.c_int), ("solver_initial_res", C.c_double),
+                ("solver_final_res", C.c_double), ("scaled_initial_res", C.c_double * 3)]
+
+
+class OagpuError(RuntimeError):
+    pass
+
+
+# exported entry points and their signatures (mirrors include/oagpu.h)
+_SIGS = {
+    "oagpu_init": [C.c_int],
+    "oagpu_last_error": [],
+    "oagpu_synchronize": [],
+    "oagpu_mesh_create": [C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, _ip, _ip, C.c_int,
+                          C.POINTER(C.c_void_p)],
+    "oagpu_mesh_destroy": [C.c_void_p],
+    "oagpu_mesh_sizes": [C.c_void_p, _ip, _ip, _ip, _ip, _ip],
+    "oagpu_graph_copy": [C.c_void_p, _ip, _ip, _ip],
+    "oagpu_scatter_map_copy": [C.c_void_p, _ip, _ip, C.POINTER(C.c_ulonglong)],
+    "oagpu_mesh_volume_copy": [C.c_void_p, _dp],
+    "oagpu_flow_create": [C.c_void_p, C.POINTER(FlowParams), C.c_int, _dp,
+                          C.POINTER(C.c_void_p)],
+    "oagpu_flow_destroy": [C.c_void_p],
+    "oagpu_field_upload": [C.c_void_p, C.c_char_p, _dp, C.c_size_t],
+    "oagpu_field_download": [C.c_void_p, C.c_char_p, _dp, C.c_size_t],
+    "oagpu_initialize": [C.c_void_p],
+    "oagpu_assemble_momentum": [C.c_void_p],
+    "oagpu_assemble_pcorr": [C.c_void_p],
+    "oagpu_system_download": [C.c_void_p, C.c_int, _dp, _dp],
+    "oagpu_system_upload": [C.c_void_p, C.c_int, _dp, _dp],
+    "oagpu_solution_download": [C.c_void_p, C.c_int, _dp],
+    "oagpu_update_mdot": [C.c_void_p],
+    "oagpu_mass_divergence": [C.c_void_p],
+    "oagpu_gradient": [C.c_void_p, C.c_int],
+    "oagpu_correct": [C.c_void_p, C.c_int, C.c_double],
+    "oagpu_velocity_correction": [C.c_void_p, C.c_int],
+    "oagpu_update_scales": [C.c_void_p, _dp, _dp],
+    "oagpu_scaled_residual": [C.c_void_p, C.c_int, _dp],
+    "oagpu_solve": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                    C.POINTER(ControlData)],
+    "oagpu_outer_iteration": [C.c_void_p, _dp, _dp],
+    "oagpu_last_kernel_ms": [C.c_void_p, C.c_char_p, _dp],
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load liboagpu.so and declare its signatures (no GPU call is made)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise OagpuError(f"liboagpu.so not built: {path} (run python -m openaccel_amd.build)")
+        L = C.CDLL(path)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_char_p if name == "oagpu_last_error" else C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = load_library().oagpu_last_error()
+        raise OagpuError(f"oagpu status {rc}: {msg.decode() if msg else ''}")
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+def init(device: int = 0):
+    _check(load_library().oagpu_init(device))
+
+
+def default_params(precision: int = OAGPU_FP64, **kw) -> FlowParams:
+    """Defaults of controls.h:57-290 for the lid-driven cavity
+    (velocity interpolation trilinear, gradients linear_linear)."""
+    p = FlowParams(rho=1.0, mu=0.01, dt=1.0, urf_U=0.9, urf_p=0.1, urf_mdot=1.0,
+                   grad_urf=0.5625, bnd_relax=0.75, u_shifted=0, ugrad_shifted=1,
+                   p_shifted=1, pgrad_shifted=1, p_ref_node=0, precision=precision)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class DeviceMesh:
+    """Device mirror of a hex8 mesh (oagpu_mesh_create)."""
+
+    def __init__(self, mesh: dict, stencil: int = OAGPU_STENCIL_REDUCED):
+        L = load_library()
+        xyz = np.ascontiguousarray(mesh["xyz"], dtype=np.float64)
+        conn = np.ascontiguousarray(mesh["conn"], dtype=np.int32)
+        bfe = np.ascontiguousarray(mesh["bf_elem"], dtype=np.int32)
+        bfo = np.ascontiguousarray(mesh["bf_ord"], dtype=np.int32)
+        bfb = np.ascontiguousarray(mesh["bf_bc"], dtype=np.int32)
+        h = C.c_void_p()
+        _check(L.oagpu_mesh_create(xyz.shape[0], _d(xyz), conn.shape[0], _i(conn), bfe.size,
+                                   _i(bfe), _i(bfo), _i(bfb), stencil, C.byref(h)))
+        self.h = h
+        nn, ne, nnz, ninc, nbinc = (C.c_int() for _ in range(5))
+        _check(L.oagpu_mesh_sizes(h, C.byref(nn), C.byref(ne), C.byref(nnz), C.byref(ninc),
+                                  C.byref(nbinc)))
+        self.nn, self.ne, self.nnz, self.ninc, self.nbinc = (v.value for v in (nn, ne, nnz, ninc, nbinc))
+
+    def graph(self):
+        rp = np.zeros(self.nn + 1, np.int32)
+        cols = np.zeros(self.nnz, np.int32)
+        doff = np.zeros(self.nn, np.int32)
+        _check(load_library().oagpu_graph_copy(self.h, _i(rp), _i(cols), _i(doff)))
+        return rp, cols, doff
+
+    def scatter_map(self):
+        ptr = np.zeros(self.nn + 1, np.int32)
+        inc = np.zeros(self.ninc, np.int32)
+        slots = np.zeros(self.ninc, np.uint64)
+        _check(load_library().oagpu_scatter_map_copy(
+            self.h, _i(ptr), _i(inc), slots.ctypes.data_as(C.POINTER(C.c_ulonglong))))
+        return ptr, inc, slots
+
+    def dual_volume(self):
+        v = np.zeros(self.nn)
+        _check(load_library().oagpu_mesh_volume_copy(self.h, _d(v)))
+        return v
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().oagpu_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class FlowCase:
+    """Device state of one segregated-flow case (oagpu_flow)."""
+
+    def __init__(self, mesh: DeviceMesh, params: FlowParams, bc_U):
+        self.mesh = mesh
+        self.params = params
+        bc = np.ascontiguousarray(bc_U, dtype=np.float64).reshape(-1, 3)
+        h = C.c_void_p()
+        _check(load_library().oagpu_flow_create(mesh.h, C.byref(params), bc.shape[0], _d(bc),
+                                                C.byref(h)))
+        self.h = h
+
+    _comp = {"U": 3, "p": 1, "gradU": 9, "gradP": 3, "du": 3, "div": 1}
+
+    def _size(self, name):
+        if name == "mdot":
+            return self.mesh.ne * 12
+        return self.mesh.nn * self._comp[name]
+
+    def upload(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        _check(load_library().oagpu_field_upload(self.h, name.encode(), _d(a), a.size))
+
+    def download(self, name):
+        n = self._size(name)
+        a = np.zeros(n)
+        _check(load_library().oagpu_field_download(self.h, name.encode(), _d(a), n))
+        if name == "mdot":
+            return a.reshape(self.mesh.ne, 12)
+        k = self._comp[name]
+        return a if k == 1 else a.reshape(self.mesh.nn, k)
+
+    def initialize(self):
+        _check(load_library().oagpu_initialize(self.h))
+
+    def assemble_momentum(self):
+        _check(load_library().oagpu_assemble_momentum(self.h))
+
+    def assemble_pcorr(self):
+        _check(load_library().oagpu_assemble_pcorr(self.h))
+
+    def system(self, N):
+        vals = np.zeros(self.mesh.nnz * N * N)
+        b = np.zeros(self.mesh.nn * N)
+        _check(load_library().oagpu_system_download(self.h, N, _d(vals), _d(b)))
+        return vals.reshape(self.mesh.nnz, N, N), b
+
+    def set_system(self, N, vals, b):
+        v = np.ascontiguousarray(vals, dtype=np.float64).ravel()
+        bb = np.ascontiguousarray(b, dtype=np.float64).ravel()
+        _check(load_library().oagpu_system_upload(self.h, N, _d(v), _d(bb)))
+
+    def solution(self, N):
+        x = np.zeros(self.mesh.nn * N)
+        _check(load_library().oagpu_solution_download(self.h, N, _d(x)))
+        return x
+
+    def update_mdot(self):
+        _check(load_library().oagpu_update_mdot(self.h))
+
+    def mass_divergence(self):
+        _check(load_library().oagpu_mass_divergence(self.h))
+
+    def gradient(self, which):
+        _check(load_library().oagpu_gradient(self.h, {"U": 0, "p": 1}[which]))
+
+    def correct(self, N, urf):
+        _check(load_library().oagpu_correct(self.h, N, urf))
+
+    def velocity_correction(self, step):
+        _check(load_library().oagpu_velocity_correction(self.h, step))
+
+    def scales(self):
+        u, p = C.c_double(), C.c_double()
+        _check(load_library().oagpu_update_scales(self.h, C.byref(u), C.byref(p)))
+        return u.value, p.value
+
+    def scaled_residual(self, N):
+        out = np.zeros(N)
+        _check(load_library().oagpu_scaled_residual(self.h, N, _d(out)))
+        return out
+
+    def solve(self, N, method, max_it, rtol, atol=1e-30, restart=30):
+        cd = ControlData()
+        _check(load_library().oagpu_solve(self.h, N, method, max_it, rtol, atol, restart,
+                                          C.byref(cd)))
+        return cd
+
+    def outer_iteration(self, ctl):
+        c = np.ascontiguousarray(ctl, dtype=np.float64)
+        h = np.zeros(6)
+        _check(load_library().oagpu_outer_iteration(self.h, _d(c), _d(h)))
+        return h
+
+    def kernel_ms(self, which):
+        v = C.c_double()
+        _check(load_library().oagpu_last_kernel_ms(self.h, which.encode(), C.byref(v)))
+        return v.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().oagpu_flow_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

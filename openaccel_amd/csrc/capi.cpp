@@ -1,0 +1,446 @@
+// capi.cpp -- extern "C" entry points of liboagpu (declared in include/oagpu.h)
+// and the host driver of one steady outer iteration.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "flow.hpp"
+
+struct oagpu_mesh {
+    oagpu::DevMesh m;
+};
+struct oagpu_flow {
+    oagpu::Flow f;
+};
+
+namespace {
+thread_local std::string g_last_error;
+bool g_init = false;
+
+template <typename F>
+int guarded(F&& fn) {
+    try {
+        fn();
+        return OAGPU_OK;
+    } catch (const oagpu::Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return OAGPU_ENOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return OAGPU_EINTERNAL;
+    }
+}
+
+void ensure_init() {
+    if (!g_init) {
+        int n = 0;
+        OAGPU_HIP(hipGetDeviceCount(&n));
+        OAGPU_REQUIRE(n > 0, "no HIP device");
+        g_init = true;
+    }
+}
+
+double* field_ptr(oagpu::Flow& f, const char* name, size_t* count) {
+    const size_t nn = size_t(f.m->nn), ne = size_t(f.m->ne);
+    struct {
+        const char* n;
+        oagpu::DBuf<double>* b;
+        size_t c;
+    } tab[] = {{"U", &f.U, nn * 3},          {"p", &f.p, nn},       {"gradU", &f.gradU, nn * 9},
+               {"gradP", &f.gradP, nn * 3},  {"du", &f.du, nn * 3}, {"mdot", &f.mdot, ne * 12},
+               {"div", &f.div, nn}};
+    for (auto& t : tab)
+        if (!std::strcmp(name, t.n)) {
+            *count = t.c;
+            return t.b->p;
+        }
+    throw oagpu::Error(OAGPU_EINVAL, std::string("unknown field ") + name);
+}
+
+void check_err(oagpu::Flow& f) {
+    int e = 0;
+    f.m->err.download(&e, 1, f.stream);
+    OAGPU_HIP(hipStreamSynchronize(f.stream));
+    if (e) {
+        f.m->err.zero(f.stream);
+        throw oagpu::Error(OAGPU_EGEOM, "non-positive Jacobian determinant in a kernel");
+    }
+}
+
+template <typename F>
+double timed(oagpu::Flow& f, F&& fn) {
+    OAGPU_HIP(hipEventRecord(f.ev0, f.stream));
+    fn();
+    OAGPU_HIP(hipEventRecord(f.ev1, f.stream));
+    OAGPU_HIP(hipEventSynchronize(f.ev1));
+    float ms = 0.f;
+    OAGPU_HIP(hipEventElapsedTime(&ms, f.ev0, f.ev1));
+    return double(ms);
+}
+}  // namespace
+
+extern "C" {
+
+int oagpu_init(int device) {
+    return guarded([&] {
+        int n = 0;
+        OAGPU_HIP(hipGetDeviceCount(&n));
+        OAGPU_REQUIRE(device >= 0 && device < n, "device index out of range");
+        OAGPU_HIP(hipSetDevice(device));
+        g_init = true;
+    });
+}
+
+const char* oagpu_last_error(void) { return g_last_error.c_str(); }
+
+int oagpu_synchronize(void) {
+    return guarded([&] { OAGPU_HIP(hipDeviceSynchronize()); });
+}
+
+int oagpu_mesh_create(int n_nodes, const double* xyz, int n_elems, const int* conn, int n_bfaces,
+                      const int* bf_elem, const int* bf_ord, const int* bf_bc, int stencil,
+                      oagpu_mesh** out) {
+    return guarded([&] {
+        OAGPU_REQUIRE(out, "null output handle");
+        OAGPU_REQUIRE(xyz && conn, "null mesh arrays");
+        OAGPU_REQUIRE(n_bfaces == 0 || (bf_elem && bf_ord && bf_bc), "null boundary arrays");
+        ensure_init();
+        auto h = std::make_unique<oagpu_mesh>();
+        oagpu::HostMesh hm;
+        hm.nn = n_nodes;
+        hm.ne = n_elems;
+        hm.nbf = n_bfaces;
+        hm.xyz = xyz;
+        hm.conn = conn;
+        hm.bf_elem = bf_elem;
+        hm.bf_ord = bf_ord;
+        hm.bf_bc = bf_bc;
+        oagpu::build_mesh(hm, stencil == OAGPU_STENCIL_FULL ? 0 : 1, h->m);
+        oagpu::launch_mesh_geometry(h->m, nullptr);
+        OAGPU_HIP(hipDeviceSynchronize());
+        *out = h.release();
+    });
+}
+
+int oagpu_mesh_destroy(oagpu_mesh* m) {
+    return guarded([&] { delete m; });
+}
+
+int oagpu_mesh_sizes(const oagpu_mesh* m, int* nn, int* ne, int* nnz, int* ninc, int* nbinc) {
+    return guarded([&] {
+        OAGPU_REQUIRE(m, "null mesh");
+        if (nn) *nn = m->m.nn;
+        if (ne) *ne = m->m.ne;
+        if (nnz) *nnz = m->m.nnz;
+        if (ninc) *ninc = m->m.ninc;
+        if (nbinc) *nbinc = m->m.nbinc;
+    });
+}
+
+int oagpu_graph_copy(const oagpu_mesh* m, int* row_ptr, int* cols, int* diag_off) {
+    return guarded([&] {
+        OAGPU_REQUIRE(m, "null mesh");
+        if (row_ptr) std::memcpy(row_ptr, m->m.h_row_ptr.data(), sizeof(int) * m->m.h_row_ptr.size());
+        if (cols) std::memcpy(cols, m->m.h_cols.data(), sizeof(int) * m->m.h_cols.size());
+        if (diag_off) std::memcpy(diag_off, m->m.h_diag_off.data(), sizeof(int) * m->m.h_diag_off.size());
+    });
+}
+
+int oagpu_scatter_map_copy(const oagpu_mesh* m, int* inc_ptr, int* inc, unsigned long long* slots) {
+    return guarded([&] {
+        OAGPU_REQUIRE(m, "null mesh");
+        const auto& d = m->m;
+        if (inc_ptr) std::memcpy(inc_ptr, d.h_inc_ptr.data(), sizeof(int) * d.h_inc_ptr.size());
+        if (inc) std::memcpy(inc, d.h_inc.data(), sizeof(int) * d.h_inc.size());
+        if (slots) {
+            // copy from the device so the check covers what the kernels read
+            std::vector<uint64_t> s(d.ninc);
+            d.inc_slots.download(s.data(), s.size());
+            OAGPU_HIP(hipDeviceSynchronize());
+            std::memcpy(slots, s.data(), sizeof(uint64_t) * s.size());
+        }
+    });
+}
+
+int oagpu_mesh_volume_copy(const oagpu_mesh* m, double* vol) {
+    return guarded([&] {
+        OAGPU_REQUIRE(m && vol, "null argument");
+        m->m.vol.download(vol, m->m.nn);
+        OAGPU_HIP(hipDeviceSynchronize());
+    });
+}
+
+int oagpu_flow_create(oagpu_mesh* m, const oagpu_flow_params* prm, int n_bc, const double* bc_U,
+                      oagpu_flow** out) {
+    return guarded([&] {
+        OAGPU_REQUIRE(m && prm && out, "null argument");
+        OAGPU_REQUIRE(n_bc >= 0 && (n_bc == 0 || bc_U), "bad boundary velocity table");
+        OAGPU_REQUIRE(prm->precision == OAGPU_FP64 || prm->precision == OAGPU_MIXED, "bad precision");
+        OAGPU_REQUIRE(prm->dt > 0 && prm->urf_U > 0 && prm->urf_p > 0 && prm->rho > 0, "bad parameters");
+        OAGPU_REQUIRE(prm->p_ref_node < m->m.nn, "pressure reference node out of range");
+        auto h = std::make_unique<oagpu_flow>();
+        oagpu::Flow& f = h->f;
+        f.m = &m->m;
+        f.prm = *prm;
+        f.nbc = n_bc;
+        std::vector<double> bc(bc_U, bc_U + 3 * size_t(n_bc));
+        if (bc.empty()) bc.assign(3, 0.0);
+        f.bc_U.upload(bc);
+        for (int fc = 0; fc < m->m.nbf; ++fc) {
+            // validated lazily: boundary ids must index the table
+        }
+        const size_t nn = size_t(m->m.nn), ne = size_t(m->m.ne), nnz = size_t(m->m.nnz);
+        f.U.alloc(nn * 3);
+        f.p.alloc(nn);
+        f.gradU.alloc(nn * 9);
+        f.gradP.alloc(nn * 3);
+        f.du.alloc(nn * 3);
+        f.mdot.alloc(ne * 12);
+        f.div.alloc(nn);
+        f.A3.alloc(nnz * 9);
+        f.b3.alloc(nn * 3);
+        f.x3.alloc(nn * 3);
+        f.A1.alloc(nnz);
+        f.b1.alloc(nn);
+        f.x1.alloc(nn);
+        for (auto* b : {&f.dinv, &f.r, &f.z, &f.pk, &f.q, &f.w, &f.t, &f.s, &f.r0, &f.ph, &f.sh})
+            b->alloc(nn * 3);
+        f.part.alloc(size_t(64) * oagpu::kRedGrid);
+        f.red.alloc(64);
+        f.scal.alloc(16);
+        f.flag.alloc(4);
+        for (auto* b : {&f.U, &f.p, &f.gradU, &f.gradP, &f.du, &f.mdot, &f.div, &f.A3, &f.b3, &f.x3,
+                        &f.A1, &f.b1, &f.x1})
+            b->zero();
+        OAGPU_HIP(hipStreamCreate(&f.stream));
+        OAGPU_HIP(hipEventCreate(&f.ev0));
+        OAGPU_HIP(hipEventCreate(&f.ev1));
+        // boundary ids must index the velocity table
+        {
+            std::vector<int> ids(m->m.nbf ? m->m.nbf : 1);
+            if (m->m.nbf) {
+                m->m.bf_bc.download(ids.data(), m->m.nbf);
+                OAGPU_HIP(hipDeviceSynchronize());
+                for (int i = 0; i < m->m.nbf; ++i)
+                    OAGPU_REQUIRE(ids[i] < n_bc, "boundary id without a wall velocity");
+            }
+        }
+        OAGPU_HIP(hipDeviceSynchronize());
+        *out = h.release();
+    });
+}
+
+int oagpu_flow_destroy(oagpu_flow* f) {
+    return guarded([&] {
+        if (!f) return;
+        if (f->f.stream) (void)hipStreamSynchronize(f->f.stream);
+        if (f->f.ev0) (void)hipEventDestroy(f->f.ev0);
+        if (f->f.ev1) (void)hipEventDestroy(f->f.ev1);
+        if (f->f.stream) (void)hipStreamDestroy(f->f.stream);
+        delete f;
+    });
+}
+
+int oagpu_field_upload(oagpu_flow* h, const char* name, const double* host, size_t count) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && name && host, "null argument");
+        size_t n = 0;
+        double* d = field_ptr(h->f, name, &n);
+        OAGPU_REQUIRE(count == n, std::string("size mismatch for field ") + name);
+        OAGPU_HIP(hipMemcpyAsync(d, host, n * sizeof(double), hipMemcpyHostToDevice, h->f.stream));
+        OAGPU_HIP(hipStreamSynchronize(h->f.stream));
+    });
+}
+
+int oagpu_field_download(oagpu_flow* h, const char* name, double* host, size_t count) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && name && host, "null argument");
+        size_t n = 0;
+        double* d = field_ptr(h->f, name, &n);
+        OAGPU_REQUIRE(count == n, std::string("size mismatch for field ") + name);
+        OAGPU_HIP(hipMemcpyAsync(host, d, n * sizeof(double), hipMemcpyDeviceToHost, h->f.stream));
+        OAGPU_HIP(hipStreamSynchronize(h->f.stream));
+    });
+}
+
+int oagpu_initialize(oagpu_flow* h) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h, "null flow");
+        oagpu::Flow& f = h->f;
+        f.gradU.zero(f.stream);
+        f.gradP.zero(f.stream);
+        oagpu::launch_gradient(f, 0);
+        oagpu::launch_gradient(f, 1);
+        oagpu::field_scales(f, nullptr, nullptr);
+        oagpu::launch_mdot_init(f);
+        oagpu::launch_mass_divergence(f);
+        check_err(f);
+    });
+}
+
+int oagpu_assemble_momentum(oagpu_flow* h) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h, "null flow");
+        h->f.ms_momentum = timed(h->f, [&] { oagpu::launch_assemble_momentum(h->f); });
+        check_err(h->f);
+    });
+}
+
+int oagpu_assemble_pcorr(oagpu_flow* h) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h, "null flow");
+        h->f.ms_pcorr = timed(h->f, [&] { oagpu::launch_assemble_pcorr(h->f); });
+        check_err(h->f);
+    });
+}
+
+int oagpu_system_download(oagpu_flow* h, int N, double* vals, double* b) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && (N == 1 || N == 3), "bad argument");
+        oagpu::Flow& f = h->f;
+        if (vals) (N == 3 ? f.A3 : f.A1).download(vals, size_t(f.m->nnz) * N * N, f.stream);
+        if (b) (N == 3 ? f.b3 : f.b1).download(b, size_t(f.m->nn) * N, f.stream);
+        OAGPU_HIP(hipStreamSynchronize(f.stream));
+    });
+}
+
+int oagpu_system_upload(oagpu_flow* h, int N, const double* vals, const double* b) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && (N == 1 || N == 3), "bad argument");
+        oagpu::Flow& f = h->f;
+        if (vals) (N == 3 ? f.A3 : f.A1).upload(vals, size_t(f.m->nnz) * N * N, f.stream);
+        if (b) (N == 3 ? f.b3 : f.b1).upload(b, size_t(f.m->nn) * N, f.stream);
+        OAGPU_HIP(hipStreamSynchronize(f.stream));
+    });
+}
+
+int oagpu_solution_download(oagpu_flow* h, int N, double* x) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && x && (N == 1 || N == 3), "bad argument");
+        oagpu::Flow& f = h->f;
+        (N == 3 ? f.x3 : f.x1).download(x, size_t(f.m->nn) * N, f.stream);
+        OAGPU_HIP(hipStreamSynchronize(f.stream));
+    });
+}
+
+int oagpu_update_mdot(oagpu_flow* h) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h, "null flow");
+        h->f.ms_mdot = timed(h->f, [&] { oagpu::launch_update_mdot(h->f); });
+        check_err(h->f);
+    });
+}
+
+int oagpu_mass_divergence(oagpu_flow* h) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h, "null flow");
+        oagpu::launch_mass_divergence(h->f);
+        OAGPU_HIP(hipStreamSynchronize(h->f.stream));
+    });
+}
+
+int oagpu_gradient(oagpu_flow* h, int which) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && (which == 0 || which == 1), "bad argument");
+        h->f.ms_grad = timed(h->f, [&] { oagpu::launch_gradient(h->f, which); });
+    });
+}
+
+int oagpu_correct(oagpu_flow* h, int N, double urf) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && (N == 1 || N == 3), "bad argument");
+        oagpu::launch_correct(h->f, N, urf);
+        OAGPU_HIP(hipStreamSynchronize(h->f.stream));
+    });
+}
+
+int oagpu_velocity_correction(oagpu_flow* h, int step) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && (step == 1 || step == 2), "bad argument");
+        oagpu::launch_velocity_correction(h->f, step);
+        OAGPU_HIP(hipStreamSynchronize(h->f.stream));
+    });
+}
+
+int oagpu_update_scales(oagpu_flow* h, double* U_scale, double* p_scale) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h, "null flow");
+        oagpu::field_scales(h->f, U_scale, p_scale);
+    });
+}
+
+int oagpu_scaled_residual(oagpu_flow* h, int N, double* out) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && out && (N == 1 || N == 3), "bad argument");
+        oagpu::scaled_residual(h->f, N, out);
+    });
+}
+
+int oagpu_solve(oagpu_flow* h, int N, int method, int max_it, double rtol, double atol, int restart,
+                oagpu_control_data* cd) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && cd, "null argument");
+        OAGPU_REQUIRE(max_it >= 0, "negative iteration limit");
+        std::memset(cd, 0, sizeof(*cd));
+        oagpu::krylov_solve(h->f, N, method, max_it, rtol, atol, restart, cd);
+    });
+}
+
+// simulation::assembleAndSolveSystems -> segregatedFlowEquations::solve
+// (segregatedFlowEquations.cpp:76-297) for one p' sub-iteration
+int oagpu_outer_iteration(oagpu_flow* h, const double* ctl, double* hist) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && ctl, "null argument");
+        oagpu::Flow& f = h->f;
+        const double us0 = f.U_scale, ps0 = f.p_scale;
+        const double kSmall = 2.220446049250313e-16;
+        oagpu_control_data cdU{}, cdP{};
+        double rU[3], rP[1];
+        // momentum predictor (navierStokesEquation::solve :157-222)
+        f.ms_momentum = timed(f, [&] { oagpu::launch_assemble_momentum(f); });
+        oagpu::scaled_residual(f, 3, rU);
+        oagpu::krylov_solve(f, 3, OAGPU_GMRES, int(ctl[0]), ctl[1], 1e-30, 30, &cdU);
+        oagpu::launch_correct(f, 3, 1.0);
+        f.ms_mdot = timed(f, [&] { oagpu::launch_update_mdot(f); });
+        oagpu::field_scales(f, nullptr, nullptr);
+        // pressure correction (pressureCorrectionEquation::solve :89-164)
+        f.ms_pcorr = timed(f, [&] { oagpu::launch_assemble_pcorr(f); });
+        oagpu::scaled_residual(f, 1, rP);
+        oagpu::krylov_solve(f, 1, OAGPU_CG, int(ctl[2]), ctl[3], 1e-30, 30, &cdP);
+        oagpu::launch_correct(f, 1, f.prm.urf_p);
+        // mass flow, divergence, velocity correction, gradients (:106-285)
+        oagpu::launch_update_mdot(f);
+        oagpu::launch_mass_divergence(f);
+        oagpu::launch_velocity_correction(f, 1);
+        oagpu::launch_gradient(f, 1);
+        oagpu::launch_velocity_correction(f, 2);
+        oagpu::field_scales(f, nullptr, nullptr);
+        f.ms_grad = timed(f, [&] { oagpu::launch_gradient(f, 0); });
+        check_err(f);
+        if (hist) {
+            for (int k = 0; k < 3; ++k) hist[k] = rU[k] / (us0 + kSmall);
+            hist[3] = rP[0] / (ps0 + kSmall);
+            hist[4] = cdU.n_iterations;
+            hist[5] = cdP.n_iterations;
+        }
+    });
+}
+
+int oagpu_last_kernel_ms(oagpu_flow* h, const char* which, double* ms) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && which && ms, "null argument");
+        const oagpu::Flow& f = h->f;
+        if (!std::strcmp(which, "momentum")) *ms = f.ms_momentum;
+        else if (!std::strcmp(which, "pcorr")) *ms = f.ms_pcorr;
+        else if (!std::strcmp(which, "mdot")) *ms = f.ms_mdot;
+        else if (!std::strcmp(which, "gradU")) *ms = f.ms_grad;
+        else throw oagpu::Error(OAGPU_EINVAL, "unknown kernel name");
+    });
+}
+
+}  // extern "C"

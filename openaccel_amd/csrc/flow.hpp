@@ -1,0 +1,50 @@
+// flow.hpp -- device state of a segregated pressure-velocity case.
+#pragma once
+
+#include "../../include/oagpu.h"
+#include "mesh.hpp"
+
+namespace oagpu {
+
+constexpr int kRedGrid = 512;   // fixed partition of every reduction: deterministic sums
+constexpr int kRedBlock = 256;
+
+struct Flow {
+    DevMesh* m = nullptr;
+    oagpu_flow_params prm{};
+    int nbc = 0;
+    DBuf<double> bc_U;                   // [nbc][3]
+    // nodal / element fields, reference layouts
+    DBuf<double> U, p, gradU, gradP, du, mdot, div;
+    // systems
+    DBuf<double> A3, b3, x3, A1, b1, x1;
+    // Krylov workspace
+    DBuf<double> dinv, r, z, pk, q, w, t, s, r0, ph, sh;
+    DBuf<double> V;                      // GMRES basis [(m+1)][n]
+    DBuf<double> part;                   // reduction partials [64][kRedGrid]
+    DBuf<double> red;                    // reduced values [64]
+    DBuf<double> scal;                   // device-resident Krylov scalars
+    DBuf<int> flag;                      // device-resident convergence flag / counters
+    int V_cols = 0;
+    double U_scale = 1.0, p_scale = 1.0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double ms_momentum = 0, ms_pcorr = 0, ms_mdot = 0, ms_grad = 0;
+};
+
+// kernels.hip
+void launch_mesh_geometry(DevMesh& m, hipStream_t s);
+void launch_mdot_init(Flow& f);
+void launch_update_mdot(Flow& f);
+void launch_mass_divergence(Flow& f);
+void launch_gradient(Flow& f, int which);
+void launch_assemble_momentum(Flow& f);
+void launch_assemble_pcorr(Flow& f);
+void launch_correct(Flow& f, int N, double urf);
+void launch_velocity_correction(Flow& f, int step);
+void field_scales(Flow& f, double* U_scale, double* p_scale);
+void scaled_residual(Flow& f, int N, double* out);
+int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double atol, int restart,
+                 oagpu_control_data* cd);
+
+}  // namespace oagpu

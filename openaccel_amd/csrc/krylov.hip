@@ -1,0 +1,497 @@
+// krylov.hip -- device Krylov solvers for the block-CRS systems (N = 1, 3).
+//
+// Replaces liblinsolve's linearSolver::Base<N>::solve (reached from
+// linearSystem<N>::solveSystem_, src/equation/linearSystem.h:416-462).  The
+// reference delegates to PETSc / Trilinos / HYPRE; these are Jacobi
+// preconditioned textbook methods (Saad, Iterative Methods for Sparse Linear
+// Systems, 2nd ed.: GMRES Alg. 9.5 with right preconditioning and classical
+// Gram-Schmidt applied twice, PCG Alg. 9.1, BiCGStab Alg. 7.7 right
+// preconditioned), all vectors resident in HBM.
+//
+// Every reduction uses one fixed partition (kRedGrid blocks of kRedBlock
+// lanes, grid-stride) and a fixed-order tree, so results are bitwise
+// reproducible run to run.  CG keeps its scalars on the device: each
+// consumer block re-reduces the kRedGrid partials itself, so an iteration is
+// three launches with no host synchronisation; the host only polls the
+// convergence flag every few iterations.
+#include "flow.hpp"
+
+#include <cmath>
+
+namespace oagpu {
+
+template <int N>
+__global__ void __launch_bounds__(256) k_jacobi_inv(int nn, const int* rp, const int* doff,
+                                                    const double* A, double* dinv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const double* d = A + size_t(N * N) * (rp[i] + doff[i]);
+    for (int k = 0; k < N; ++k) dinv[N * size_t(i) + k] = 1.0 / d[(N + 1) * k];
+}
+
+// y = A (s .* x) (s = nullptr: y = A x)
+template <int N>
+__global__ void __launch_bounds__(kRedBlock) k_spmv(int nn, const int* rp, const int* cols,
+                                                    const double* A, const double* x,
+                                                    const double* s, double* y) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
+        double acc[N];
+        for (int m = 0; m < N; ++m) acc[m] = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            const double* blk = A + size_t(N * N) * k;
+            const size_t c = size_t(N) * cols[k];
+            double xv[N];
+            for (int q = 0; q < N; ++q) xv[q] = s ? s[c + q] * x[c + q] : x[c + q];
+            for (int m = 0; m < N; ++m)
+                for (int q = 0; q < N; ++q) acc[m] += blk[N * m + q] * xv[q];
+        }
+        for (int m = 0; m < N; ++m) y[N * size_t(i) + m] = acc[m];
+    }
+}
+
+__device__ inline double block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// every block reduces the kRedGrid partials of slot `k` in the same order
+__device__ inline double reduce_partials(const double* part, int k, double* sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < kRedGrid; i += blockDim.x) v += part[size_t(k) * kRedGrid + i];
+    return block_sum(v, sh);
+}
+
+struct DotPairs {
+    const double* x[64];
+    const double* y[64];
+};
+
+__global__ void __launch_bounds__(kRedBlock) k_dots(size_t n, int np, DotPairs P, double* part) {
+    __shared__ double sh[kRedBlock];
+    for (int k = 0; k < np; ++k) {
+        double v = 0.0;
+        const double* x = P.x[k];
+        const double* y = P.y[k];
+        for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+             i += size_t(gridDim.x) * blockDim.x)
+            v += x[i] * y[i];
+        const double s = block_sum(v, sh);
+        if (threadIdx.x == 0) part[size_t(k) * kRedGrid + blockIdx.x] = s;
+    }
+}
+
+__global__ void __launch_bounds__(kRedBlock) k_reduce(int np, const double* part, double* out) {
+    __shared__ double sh[kRedBlock];
+    const int k = blockIdx.x;
+    if (k >= np) return;
+    const double s = reduce_partials(part, k, sh);
+    if (threadIdx.x == 0) out[k] = s;
+}
+
+// host-side: np dot products in one pass, results copied to the host
+static void dots(Flow& f, size_t n, int np, const double* const* xs, const double* const* ys,
+                 double* out) {
+    DotPairs P;
+    for (int k = 0; k < np; ++k) {
+        P.x[k] = xs[k];
+        P.y[k] = ys[k];
+    }
+    k_dots<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, np, P, f.part.p);
+    k_reduce<<<np, kRedBlock, 0, f.stream>>>(np, f.part.p, f.red.p);
+    OAGPU_HIP(hipGetLastError());
+    f.red.download(out, np, f.stream);
+    OAGPU_HIP(hipStreamSynchronize(f.stream));
+}
+
+// ------------------------------------------------------------------ GMRES
+// w -= sum_k h[k] V_k
+__global__ void k_gs_update(size_t n, int nk, const double* V, const double* h, double* w) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < nk; ++k) s += h[k] * V[size_t(k) * n + i];
+    w[i] -= s;
+}
+
+__global__ void k_scale_to(size_t n, double a, const double* x, double* y) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * x[i];
+}
+
+// x += dinv .* (V y)
+__global__ void k_gmres_x(size_t n, int nk, const double* V, const double* y, const double* dinv,
+                          double* x) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < nk; ++k) s += y[k] * V[size_t(k) * n + i];
+    x[i] += dinv[i] * s;
+}
+
+__global__ void k_residual_copy(size_t n, const double* b, const double* Ax, double* r) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = b[i] - Ax[i];
+}
+
+template <int N>
+static void spmv(Flow& f, const double* A, const double* x, const double* s, double* y) {
+    DevMesh& m = *f.m;
+    k_spmv<N><<<kRedGrid, kRedBlock, 0, f.stream>>>(m.nn, m.row_ptr.p, m.cols.p, A, x, s, y);
+    OAGPU_HIP(hipGetLastError());
+}
+
+template <int N>
+static int gmres(Flow& f, int max_it, double rtol, double atol, int mres, oagpu_control_data* cd) {
+    DevMesh& m = *f.m;
+    const size_t n = size_t(m.nn) * N;
+    const double* A = N == 3 ? f.A3.p : f.A1.p;
+    const double* b = N == 3 ? f.b3.p : f.b1.p;
+    double* x = N == 3 ? f.x3.p : f.x1.p;
+    const unsigned B = 256, G = unsigned((n + B - 1) / B);
+    if (mres < 1) mres = 1;
+    if (mres > 60) mres = 60;
+    if (f.V_cols < mres + 1 || f.V.n < n * size_t(mres + 1)) {
+        f.V.alloc(n * size_t(mres + 1));
+        f.V_cols = mres + 1;
+    }
+    OAGPU_HIP(hipMemsetAsync(x, 0, n * sizeof(double), f.stream));
+    // x0 = 0: r0 = b
+    double beta;
+    {
+        const double* xs[1] = {b};
+        dots(f, n, 1, xs, xs, &beta);
+        beta = std::sqrt(beta);
+    }
+    const double r0 = beta;
+    const double tol = std::max(rtol * r0, atol);
+    cd->solver_initial_res = r0;
+    std::vector<double> H(size_t(mres + 1) * mres), cs(mres), sn(mres), g(mres + 1), y(mres),
+        h(mres + 1), h2(mres + 1);
+    const double* rvec = b;
+    int it = 0;
+    while (it < max_it && beta > tol) {
+        k_scale_to<<<G, B, 0, f.stream>>>(n, 1.0 / beta, rvec, f.V.p);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int jend = -1;
+        for (int j = 0; j < mres && it < max_it; ++j) {
+            spmv<N>(f, A, f.V.p + size_t(j) * n, f.dinv.p, f.w.p);
+            std::vector<const double*> xs(j + 1), ys(j + 1, f.w.p);
+            for (int k = 0; k <= j; ++k) xs[k] = f.V.p + size_t(k) * n;
+            // classical Gram-Schmidt, twice
+            dots(f, n, j + 1, xs.data(), ys.data(), h.data());
+            f.red.upload(h.data(), j + 1, f.stream);
+            k_gs_update<<<G, B, 0, f.stream>>>(n, j + 1, f.V.p, f.red.p, f.w.p);
+            dots(f, n, j + 1, xs.data(), ys.data(), h2.data());
+            f.red.upload(h2.data(), j + 1, f.stream);
+            k_gs_update<<<G, B, 0, f.stream>>>(n, j + 1, f.V.p, f.red.p, f.w.p);
+            for (int k = 0; k <= j; ++k) h[k] += h2[k];
+            double hn;
+            {
+                const double* ws[1] = {f.w.p};
+                dots(f, n, 1, ws, ws, &hn);
+                hn = std::sqrt(hn);
+            }
+            for (int k = 0; k <= j; ++k) H[size_t(mres) * k + j] = h[k];
+            H[size_t(mres) * (j + 1) + j] = hn;
+            if (hn > 0) k_scale_to<<<G, B, 0, f.stream>>>(n, 1.0 / hn, f.w.p, f.V.p + size_t(j + 1) * n);
+            for (int k = 0; k < j; ++k) {
+                const double a0 = H[size_t(mres) * k + j], b0 = H[size_t(mres) * (k + 1) + j];
+                H[size_t(mres) * k + j] = cs[k] * a0 + sn[k] * b0;
+                H[size_t(mres) * (k + 1) + j] = -sn[k] * a0 + cs[k] * b0;
+            }
+            const double a0 = H[size_t(mres) * j + j], b0 = H[size_t(mres) * (j + 1) + j];
+            const double rr = std::sqrt(a0 * a0 + b0 * b0);
+            cs[j] = rr > 0 ? a0 / rr : 1.0;
+            sn[j] = rr > 0 ? b0 / rr : 0.0;
+            H[size_t(mres) * j + j] = rr;
+            H[size_t(mres) * (j + 1) + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            ++it;
+            jend = j;
+            if (std::fabs(g[j + 1]) <= tol || hn == 0.0) break;
+        }
+        for (int k = jend; k >= 0; --k) {
+            double t = g[k];
+            for (int q = k + 1; q <= jend; ++q) t -= H[size_t(mres) * k + q] * y[q];
+            y[k] = t / H[size_t(mres) * k + k];
+        }
+        f.red.upload(y.data(), jend + 1, f.stream);
+        k_gmres_x<<<G, B, 0, f.stream>>>(n, jend + 1, f.V.p, f.red.p, f.dinv.p, x);
+        spmv<N>(f, A, x, nullptr, f.w.p);
+        k_residual_copy<<<G, B, 0, f.stream>>>(n, b, f.w.p, f.r.p);
+        rvec = f.r.p;
+        const double* rs[1] = {f.r.p};
+        dots(f, n, 1, rs, rs, &beta);
+        beta = std::sqrt(beta);
+    }
+    cd->solver_final_res = beta;
+    return it;
+}
+
+// -------------------------------------------------------------------- CG
+// scal: [0..1] rz ping-pong, [2] tol^2, [3] last rr, [4] rr0
+__global__ void __launch_bounds__(kRedBlock) k_cg_init(size_t n, const double* b,
+                                                       const double* dinv, double* x, double* r,
+                                                       double* z, double* p, double* part) {
+    __shared__ double sh[kRedBlock];
+    double rz = 0.0, rr = 0.0;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x) {
+        const double ri = b[i], zi = dinv[i] * ri;
+        x[i] = 0.0;
+        r[i] = ri;
+        z[i] = zi;
+        p[i] = zi;
+        rz += ri * zi;
+        rr += ri * ri;
+    }
+    const double s0 = block_sum(rz, sh);
+    const double s1 = block_sum(rr, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = s0;
+        part[kRedGrid + blockIdx.x] = s1;
+    }
+}
+
+__global__ void __launch_bounds__(kRedBlock) k_cg_start(const double* part, double rtol, double atol,
+                                                        double* scal, int* flag) {
+    __shared__ double sh[kRedBlock];
+    const double rz = reduce_partials(part, 0, sh);
+    const double rr = reduce_partials(part, 1, sh);
+    if (threadIdx.x == 0) {
+        const double tol2 = fmax(rtol * rtol * rr, atol * atol);
+        scal[0] = rz;
+        scal[2] = tol2;
+        scal[3] = rr;
+        scal[4] = rr;
+        flag[0] = rr <= tol2 ? 1 : 0;
+        flag[1] = 0;
+    }
+}
+
+template <int N>
+__global__ void __launch_bounds__(kRedBlock) k_cg_spmv(int nn, const int* rp, const int* cols,
+                                                       const double* A, const double* p, double* q,
+                                                       double* part, const int* flag) {
+    __shared__ double sh[kRedBlock];
+    if (flag[0]) return;
+    double pq = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
+        double acc[N];
+        for (int m = 0; m < N; ++m) acc[m] = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            const double* blk = A + size_t(N * N) * k;
+            const size_t c = size_t(N) * cols[k];
+            for (int m = 0; m < N; ++m)
+                for (int qq = 0; qq < N; ++qq) acc[m] += blk[N * m + qq] * p[c + qq];
+        }
+        for (int m = 0; m < N; ++m) {
+            q[N * size_t(i) + m] = acc[m];
+            pq += p[N * size_t(i) + m] * acc[m];
+        }
+    }
+    const double s = block_sum(pq, sh);
+    if (threadIdx.x == 0) part[2 * kRedGrid + blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kRedBlock) k_cg_update1(size_t n, int it, const double* p,
+                                                          const double* q, const double* dinv,
+                                                          double* x, double* r, double* z,
+                                                          double* part, const double* scal,
+                                                          const int* flag) {
+    __shared__ double sh[kRedBlock];
+    if (flag[0]) return;
+    const double pq = reduce_partials(part, 2, sh);
+    const double alpha = scal[it & 1] / pq;
+    double rz = 0.0, rr = 0.0;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        const double zi = dinv[i] * ri;
+        z[i] = zi;
+        rz += ri * zi;
+        rr += ri * ri;
+    }
+    const double s0 = block_sum(rz, sh);
+    const double s1 = block_sum(rr, sh);
+    if (threadIdx.x == 0) {
+        part[3 * kRedGrid + blockIdx.x] = s0;
+        part[4 * kRedGrid + blockIdx.x] = s1;
+    }
+}
+
+__global__ void __launch_bounds__(kRedBlock) k_cg_update2(size_t n, int it, const double* z,
+                                                          double* p, const double* part,
+                                                          double* scal, int* flag) {
+    __shared__ double sh[kRedBlock];
+    if (flag[0]) return;
+    const double rz = reduce_partials(part, 3, sh);
+    const double rr = reduce_partials(part, 4, sh);
+    const bool conv = rr <= scal[2];
+    if (!conv) {
+        const double beta = rz / scal[it & 1];
+        for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+             i += size_t(gridDim.x) * blockDim.x)
+            p[i] = z[i] + beta * p[i];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[(it + 1) & 1] = rz;
+        scal[3] = rr;
+        flag[1] = it + 1;
+        if (conv) flag[0] = 1;
+    }
+}
+
+template <int N>
+static int cg(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
+    DevMesh& m = *f.m;
+    const size_t n = size_t(m.nn) * N;
+    const double* A = N == 3 ? f.A3.p : f.A1.p;
+    const double* b = N == 3 ? f.b3.p : f.b1.p;
+    double* x = N == 3 ? f.x3.p : f.x1.p;
+    k_cg_init<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, b, f.dinv.p, x, f.r.p, f.z.p, f.pk.p, f.part.p);
+    k_cg_start<<<1, kRedBlock, 0, f.stream>>>(f.part.p, rtol, atol, f.scal.p, f.flag.p);
+    OAGPU_HIP(hipGetLastError());
+    const int poll = 8;
+    int hflag[2] = {0, 0};
+    for (int it = 0; it < max_it; ++it) {
+        k_cg_spmv<N><<<kRedGrid, kRedBlock, 0, f.stream>>>(m.nn, m.row_ptr.p, m.cols.p, A, f.pk.p,
+                                                            f.q.p, f.part.p, f.flag.p);
+        k_cg_update1<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, it, f.pk.p, f.q.p, f.dinv.p, x, f.r.p,
+                                                           f.z.p, f.part.p, f.scal.p, f.flag.p);
+        k_cg_update2<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, it, f.z.p, f.pk.p, f.part.p, f.scal.p,
+                                                           f.flag.p);
+        if ((it + 1) % poll == 0 || it + 1 == max_it) {
+            OAGPU_HIP(hipGetLastError());
+            f.flag.download(hflag, 2, f.stream);
+            OAGPU_HIP(hipStreamSynchronize(f.stream));
+            if (hflag[0]) break;
+        }
+    }
+    f.flag.download(hflag, 2, f.stream);
+    double sc[5];
+    f.scal.download(sc, 5, f.stream);
+    OAGPU_HIP(hipStreamSynchronize(f.stream));
+    cd->solver_initial_res = std::sqrt(sc[4]);
+    cd->solver_final_res = std::sqrt(sc[3]);
+    return hflag[1];
+}
+
+// --------------------------------------------------------------- BiCGStab
+__global__ void k_bicg_p(size_t n, double beta, double omega, const double* r, const double* v,
+                         double* p) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+
+__global__ void k_lin2(size_t n, double a, const double* x, double b, const double* y, double* out) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a * x[i] + b * y[i];
+}
+
+__global__ void k_bicg_x(size_t n, double alpha, const double* p, double omega, const double* s,
+                         const double* dinv, double* x) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += dinv[i] * (alpha * p[i] + omega * s[i]);
+}
+
+template <int N>
+static int bicgstab(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
+    DevMesh& m = *f.m;
+    const size_t n = size_t(m.nn) * N;
+    const double* A = N == 3 ? f.A3.p : f.A1.p;
+    const double* b = N == 3 ? f.b3.p : f.b1.p;
+    double* x = N == 3 ? f.x3.p : f.x1.p;
+    const unsigned B = 256, G = unsigned((n + B - 1) / B);
+    OAGPU_HIP(hipMemsetAsync(x, 0, n * sizeof(double), f.stream));
+    OAGPU_HIP(hipMemcpyAsync(f.r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, f.stream));
+    OAGPU_HIP(hipMemcpyAsync(f.r0.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, f.stream));
+    OAGPU_HIP(hipMemsetAsync(f.pk.p, 0, n * sizeof(double), f.stream));
+    OAGPU_HIP(hipMemsetAsync(f.q.p, 0, n * sizeof(double), f.stream));  // v
+    double rn;
+    {
+        const double* xs[1] = {f.r.p};
+        dots(f, n, 1, xs, xs, &rn);
+        rn = std::sqrt(rn);
+    }
+    cd->solver_initial_res = rn;
+    const double tol = std::max(rtol * rn, atol);
+    double rho = 1, alpha = 1, omega = 1;
+    int it = 0;
+    while (it < max_it && rn > tol) {
+        double rho1;
+        {
+            const double* xs[1] = {f.r0.p};
+            const double* ys[1] = {f.r.p};
+            dots(f, n, 1, xs, ys, &rho1);
+        }
+        const double beta = (rho1 / rho) * (alpha / omega);
+        rho = rho1;
+        k_bicg_p<<<G, B, 0, f.stream>>>(n, beta, omega, f.r.p, f.q.p, f.pk.p);
+        spmv<N>(f, A, f.pk.p, f.dinv.p, f.q.p);  // v = A M^-1 p
+        double r0v;
+        {
+            const double* xs[1] = {f.r0.p};
+            const double* ys[1] = {f.q.p};
+            dots(f, n, 1, xs, ys, &r0v);
+        }
+        alpha = rho / r0v;
+        k_lin2<<<G, B, 0, f.stream>>>(n, 1.0, f.r.p, -alpha, f.q.p, f.s.p);
+        spmv<N>(f, A, f.s.p, f.dinv.p, f.t.p);  // t = A M^-1 s
+        double tt_ts[2];
+        {
+            const double* xs[2] = {f.t.p, f.t.p};
+            const double* ys[2] = {f.t.p, f.s.p};
+            dots(f, n, 2, xs, ys, tt_ts);
+        }
+        omega = tt_ts[0] > 0 ? tt_ts[1] / tt_ts[0] : 0.0;
+        k_bicg_x<<<G, B, 0, f.stream>>>(n, alpha, f.pk.p, omega, f.s.p, f.dinv.p, x);
+        k_lin2<<<G, B, 0, f.stream>>>(n, 1.0, f.s.p, -omega, f.t.p, f.r.p);
+        ++it;
+        const double* xs[1] = {f.r.p};
+        dots(f, n, 1, xs, xs, &rn);
+        rn = std::sqrt(rn);
+        if (omega == 0.0) break;
+    }
+    cd->solver_final_res = rn;
+    return it;
+}
+
+int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double atol, int restart,
+                 oagpu_control_data* cd) {
+    DevMesh& m = *f.m;
+    OAGPU_REQUIRE(N == 1 || N == 3, "block size must be 1 or 3");
+    const unsigned B = 256;
+    if (N == 3)
+        k_jacobi_inv<3><<<(m.nn + B - 1) / B, B, 0, f.stream>>>(m.nn, m.row_ptr.p, m.diag_off.p,
+                                                                 f.A3.p, f.dinv.p);
+    else
+        k_jacobi_inv<1><<<(m.nn + B - 1) / B, B, 0, f.stream>>>(m.nn, m.row_ptr.p, m.diag_off.p,
+                                                                 f.A1.p, f.dinv.p);
+    OAGPU_HIP(hipGetLastError());
+    int it = 0;
+    if (method == OAGPU_GMRES)
+        it = N == 3 ? gmres<3>(f, max_it, rtol, atol, restart, cd)
+                    : gmres<1>(f, max_it, rtol, atol, restart, cd);
+    else if (method == OAGPU_CG)
+        it = N == 3 ? cg<3>(f, max_it, rtol, atol, cd) : cg<1>(f, max_it, rtol, atol, cd);
+    else if (method == OAGPU_BICGSTAB)
+        it = N == 3 ? bicgstab<3>(f, max_it, rtol, atol, cd) : bicgstab<1>(f, max_it, rtol, atol, cd);
+    else
+        throw Error(OAGPU_EINVAL, "unknown Krylov method");
+    cd->n_iterations = it;
+    return it;
+}
+
+}  // namespace oagpu

@@ -1,0 +1,202 @@
+// mesh.cpp -- host construction of the device mesh mirror: CRS node graph,
+// node -> element incidences and element -> CRS scatter maps.
+//
+// The graph follows nodeGraph::buildGraph_ (src/mesh/graph/nodeGraph.cpp:29-392):
+// reduced stencil = for every SCS ip (il, ir) both nodes enter rows il and ir
+// (:68-96); full stencil = all element node pairs (:98-118); columns sorted
+// ascending (:392, sortPrimaryIndices_), all rows owned (one GPU holds the
+// whole mesh).  The scatter maps encode applyCoeff_'s sorted search of each
+// element column in a row and its lumping of absent columns into the diagonal
+// block (src/assemble/assembler.h:380-445).
+#include "mesh.hpp"
+
+#include <algorithm>
+#include <numeric>
+
+namespace oagpu {
+
+static const int kLrscv[12][2] = {{0, 1}, {1, 2}, {2, 3}, {0, 3}, {4, 5}, {5, 6},
+                                  {6, 7}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+static const int kSideNodes[6][4] = {{0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6},
+                                     {0, 4, 7, 3}, {0, 3, 2, 1}, {4, 5, 6, 7}};
+
+static uint64_t row_slots(const std::vector<int>& rp, const std::vector<int>& cols, int row,
+                          const int* en, int* max_pos) {
+    uint64_t s = 0;
+    const int* b = cols.data() + rp[row];
+    const int len = rp[row + 1] - rp[row];
+    for (int c = 0; c < 8; ++c) {
+        const int* it = std::lower_bound(b, b + len, en[c]);
+        uint64_t pos = 0xFF;
+        if (it != b + len && *it == en[c]) {
+            pos = static_cast<uint64_t>(it - b);
+            if (static_cast<int>(pos) > *max_pos) *max_pos = static_cast<int>(pos);
+        }
+        s |= pos << (8 * c);
+    }
+    return s;
+}
+
+void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
+    OAGPU_REQUIRE(hm.nn > 0 && hm.ne > 0, "empty mesh");
+    OAGPU_REQUIRE(hm.ne < (1 << 28), "element count exceeds the incidence encoding");
+    const int nn = hm.nn, ne = hm.ne;
+    for (size_t k = 0; k < size_t(ne) * 8; ++k)
+        OAGPU_REQUIRE(hm.conn[k] >= 0 && hm.conn[k] < nn, "connectivity out of range");
+    for (int f = 0; f < hm.nbf; ++f) {
+        OAGPU_REQUIRE(hm.bf_elem[f] >= 0 && hm.bf_elem[f] < ne, "boundary face element out of range");
+        OAGPU_REQUIRE(hm.bf_ord[f] >= 0 && hm.bf_ord[f] < 6, "boundary face ordinal out of range");
+        OAGPU_REQUIRE(hm.bf_bc[f] >= 0, "negative boundary id");
+    }
+    dm.nn = nn;
+    dm.ne = ne;
+    dm.nbf = hm.nbf;
+    dm.reduced = reduced;
+
+    // ---- CRS graph: per-row candidate lists, sorted and uniqued (std::set)
+    std::vector<int> cnt(size_t(nn) + 1, 0);
+    for (int e = 0; e < ne; ++e) {
+        const int* en = hm.conn + 8 * size_t(e);
+        if (reduced) {
+            for (int ip = 0; ip < 12; ++ip) {
+                cnt[en[kLrscv[ip][0]]] += 2;
+                cnt[en[kLrscv[ip][1]]] += 2;
+            }
+        } else {
+            for (int a = 0; a < 8; ++a) cnt[en[a]] += 8;
+        }
+    }
+    std::vector<size_t> start(size_t(nn) + 1, 0);
+    for (int i = 0; i < nn; ++i) start[i + 1] = start[i] + size_t(cnt[i]);
+    std::vector<int> cand(start[nn]);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int e = 0; e < ne; ++e) {
+        const int* en = hm.conn + 8 * size_t(e);
+        if (reduced) {
+            for (int ip = 0; ip < 12; ++ip) {
+                const int L = en[kLrscv[ip][0]], R = en[kLrscv[ip][1]];
+                cand[start[L] + cnt[L]++] = L;
+                cand[start[L] + cnt[L]++] = R;
+                cand[start[R] + cnt[R]++] = L;
+                cand[start[R] + cnt[R]++] = R;
+            }
+        } else {
+            for (int a = 0; a < 8; ++a) {
+                const int r = en[a];
+                for (int c = 0; c < 8; ++c) cand[start[r] + cnt[r]++] = en[c];
+            }
+        }
+    }
+    dm.h_row_ptr.assign(size_t(nn) + 1, 0);
+    size_t nnz = 0;
+    for (int i = 0; i < nn; ++i) {
+        int* b = cand.data() + start[i];
+        std::sort(b, b + cnt[i]);
+        const int u = int(std::unique(b, b + cnt[i]) - b);
+        cnt[i] = u;
+        nnz += size_t(u);
+        OAGPU_REQUIRE(nnz < (size_t(1) << 31), "graph too large for int32 offsets");
+        dm.h_row_ptr[i + 1] = int(nnz);
+        dm.max_row = std::max(dm.max_row, u);
+    }
+    OAGPU_REQUIRE(dm.max_row <= 255, "row longer than the 8-bit scatter map");
+    dm.nnz = int(nnz);
+    dm.h_cols.resize(nnz);
+    dm.h_diag_off.resize(nn);
+    for (int i = 0; i < nn; ++i) {
+        std::copy_n(cand.data() + start[i], cnt[i], dm.h_cols.data() + dm.h_row_ptr[i]);
+        const int* b = dm.h_cols.data() + dm.h_row_ptr[i];
+        const int* it = std::lower_bound(b, b + cnt[i], i);
+        OAGPU_REQUIRE(it != b + cnt[i] && *it == i, "row without a diagonal");
+        dm.h_diag_off[i] = int(it - b);
+    }
+    std::vector<int>().swap(cand);
+
+    // ---- node -> element incidences in ascending element order
+    dm.h_inc_ptr.assign(size_t(nn) + 1, 0);
+    for (size_t k = 0; k < size_t(ne) * 8; ++k) dm.h_inc_ptr[hm.conn[k] + 1]++;
+    for (int i = 0; i < nn; ++i) dm.h_inc_ptr[i + 1] += dm.h_inc_ptr[i];
+    dm.ninc = dm.h_inc_ptr[nn];
+    dm.h_inc.resize(dm.ninc);
+    dm.h_inc_slots.resize(dm.ninc);
+    {
+        std::vector<int> fill(dm.h_inc_ptr.begin(), dm.h_inc_ptr.end() - 1);
+        int max_pos = 0;
+        for (int e = 0; e < ne; ++e) {
+            const int* en = hm.conn + 8 * size_t(e);
+            for (int a = 0; a < 8; ++a) {
+                const int r = en[a];
+                const int k = fill[r]++;
+                dm.h_inc[k] = (e << 3) | a;
+                dm.h_inc_slots[k] = row_slots(dm.h_row_ptr, dm.h_cols, r, en, &max_pos);
+            }
+        }
+    }
+
+    // ---- node -> wall-face incidences: the face ip nearest to the node
+    std::vector<int> isb(nn, 0);
+    dm.h_binc_ptr.assign(size_t(nn) + 1, 0);
+    for (int f = 0; f < hm.nbf; ++f) {
+        const int* en = hm.conn + 8 * size_t(hm.bf_elem[f]);
+        for (int k = 0; k < 4; ++k) {
+            const int nd = en[kSideNodes[hm.bf_ord[f]][k]];
+            dm.h_binc_ptr[nd + 1]++;
+            isb[nd] = 1;
+        }
+    }
+    for (int i = 0; i < nn; ++i) dm.h_binc_ptr[i + 1] += dm.h_binc_ptr[i];
+    dm.nbinc = dm.h_binc_ptr[nn];
+    dm.h_binc.resize(dm.nbinc);
+    dm.h_binc_slots.resize(dm.nbinc);
+    {
+        std::vector<int> fill(dm.h_binc_ptr.begin(), dm.h_binc_ptr.end() - 1);
+        int max_pos = 0;
+        for (int f = 0; f < hm.nbf; ++f) {
+            const int* en = hm.conn + 8 * size_t(hm.bf_elem[f]);
+            for (int k = 0; k < 4; ++k) {
+                const int nd = en[kSideNodes[hm.bf_ord[f]][k]];
+                const int q = fill[nd]++;
+                dm.h_binc[q] = (f << 2) | k;
+                dm.h_binc_slots[q] = row_slots(dm.h_row_ptr, dm.h_cols, nd, en, &max_pos);
+            }
+        }
+    }
+
+    // ---- upload
+    dm.xyz.alloc(size_t(nn) * 3);
+    dm.xyz.upload(hm.xyz, size_t(nn) * 3);
+    dm.conn.alloc(size_t(ne) * 8);
+    dm.conn.upload(hm.conn, size_t(ne) * 8);
+    dm.row_ptr.upload(dm.h_row_ptr);
+    dm.cols.upload(dm.h_cols);
+    dm.diag_off.upload(dm.h_diag_off);
+    dm.inc_ptr.upload(dm.h_inc_ptr);
+    dm.inc.upload(dm.h_inc);
+    dm.inc_slots.upload(dm.h_inc_slots);
+    dm.binc_ptr.upload(dm.h_binc_ptr);
+    if (dm.nbinc) {
+        dm.binc.upload(dm.h_binc);
+        dm.binc_slots.upload(dm.h_binc_slots);
+    } else {
+        dm.binc.alloc(1);
+        dm.binc_slots.alloc(1);
+    }
+    std::vector<int> tmp;
+    tmp.assign(hm.bf_elem, hm.bf_elem + hm.nbf);
+    if (tmp.empty()) tmp.push_back(0);
+    dm.bf_elem.upload(tmp);
+    tmp.assign(hm.bf_ord, hm.bf_ord + hm.nbf);
+    if (tmp.empty()) tmp.push_back(0);
+    dm.bf_ord.upload(tmp);
+    tmp.assign(hm.bf_bc, hm.bf_bc + hm.nbf);
+    if (tmp.empty()) tmp.push_back(0);
+    dm.bf_bc.upload(tmp);
+    dm.bnode.upload(isb);
+    dm.vol.alloc(nn);
+    dm.scvw.alloc(size_t(ne) * 8);
+    dm.err.alloc(1);
+    dm.err.zero();
+    OAGPU_HIP(hipDeviceSynchronize());
+}
+
+}  // namespace oagpu

@@ -1,0 +1,57 @@
+// mesh.hpp -- device mirror of an OpenAccel hex8 mesh.
+#pragma once
+
+#include "common.hpp"
+
+namespace oagpu {
+
+// Host-side description handed over at the C ABI (reference layouts: STK
+// node coordinates [node][3], element connectivity [elem][8] in the nalu hex8
+// node order, boundary sides as parent element + side ordinal + boundary id).
+struct HostMesh {
+    int nn = 0, ne = 0, nbf = 0;
+    const double* xyz = nullptr;
+    const int* conn = nullptr;
+    const int* bf_elem = nullptr;
+    const int* bf_ord = nullptr;
+    const int* bf_bc = nullptr;
+};
+
+// Everything the row-owned kernels need, resident in HBM.
+//
+// * CRS node graph (reduced or full stencil) with sorted local columns and
+//   the diagonal position of every row (nodeGraph.cpp:29-392).
+// * node -> element incidences: (elem << 3 | local node), elements in
+//   ascending id, so each row sums element contributions in the reference's
+//   element order.
+// * per incidence, the element -> CRS scatter map of that row: 8 bytes, the
+//   position in the row of each of the element's 8 nodes, 0xFF when the
+//   column is not in the (reduced) row and is lumped into the diagonal block
+//   (assembler.h:423-445).
+// * node -> wall-face incidences (face << 2 | face ip whose nearest node is
+//   the row), with their own scatter maps.
+struct DevMesh {
+    int nn = 0, ne = 0, nbf = 0, nnz = 0, ninc = 0, nbinc = 0, max_row = 0;
+    int reduced = 1;
+    DBuf<double> xyz;          // [nn][3]
+    DBuf<int> conn;            // [ne][8]
+    DBuf<int> row_ptr, cols, diag_off;
+    DBuf<int> inc_ptr, inc;    // [nn+1], [ninc]
+    DBuf<uint64_t> inc_slots;  // [ninc]
+    DBuf<int> binc_ptr, binc;  // [nn+1], [nbinc]
+    DBuf<uint64_t> binc_slots; // [nbinc]
+    DBuf<int> bf_elem, bf_ord, bf_bc;
+    DBuf<int> bnode;           // node lies on a wall
+    DBuf<double> vol;          // dual nodal volume [nn]
+    DBuf<double> scvw;         // SCV weights of the element-centre average [ne][8]
+    DBuf<int> err;             // device error word (non-positive Jacobians)
+
+    // host copies kept for the bit-exact graph/scatter-map checks
+    std::vector<int> h_row_ptr, h_cols, h_diag_off, h_inc_ptr, h_inc, h_binc_ptr, h_binc;
+    std::vector<uint64_t> h_inc_slots, h_binc_slots;
+};
+
+void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm);
+void compute_mesh_geometry(DevMesh& dm, hipStream_t s);
+
+}  // namespace oagpu
