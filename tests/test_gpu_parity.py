@@ -133,7 +133,7 @@ def test_field_updates(oracle, oa, meshgen, case):
     assert rel_err(f.download("div"), c.field("div")) < 1e-12
     for w in ("U", "p"):
         c.gradient(w); f.gradient(w)
-        name = "grad" + w
+        name = "gradU" if w == "U" else "gradP"
         assert rel_err(f.download(name), c.field(name)) < 1e-12, name
     for step in (1, 2):
         oracle.lib().or_velocity_correction(c.h, step)
