@@ -94,3 +94,20 @@ __device__ inline unsigned xcd_block() {
 }
 
 }  // namespace oagpu
+
+#include <utility>
+
+namespace oagpu {
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I = 0..N-1, so
+// that indices into per-lane arrays are constants (registers, not scratch).
+template <typename F, int... Is>
+__host__ __device__ __forceinline__ void sfor_impl(F&& f, std::integer_sequence<int, Is...>) {
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+__host__ __device__ __forceinline__ void sfor(F&& f) {
+    sfor_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+}  // namespace oagpu

@@ -257,14 +257,17 @@ __device__ inline T face_gradop(int ord, int k, int shifted, const T c[8][3], T 
 // side-ordinal node order)
 template <typename T>
 __device__ inline void quad_face_areav(const T p[4][3], int k, T A[3]) {
-    T e_a[3], e_b[3], cc[3];
-    const int ia = (k + 3) & 3, ib = k;  // e(k-1) and e(k), e(i) = mid(p_i, p_{i+1})
+    // sub-face k spans corner p_k, edge midpoints e(k-1), e(k) and the centre,
+    // e(i) = mid(p_i, p_{i+1}); selects keep p in registers for a runtime k
+    T e[4][3], cc[3], pk[3], e_a[3], e_b[3];
     for (int d = 0; d < 3; ++d) {
-        e_a[d] = (p[ia][d] + p[(ia + 1) & 3][d]) * T(0.5);
-        e_b[d] = (p[ib][d] + p[(ib + 1) & 3][d]) * T(0.5);
+        for (int i = 0; i < 4; ++i) e[i][d] = (p[i][d] + p[(i + 1) & 3][d]) * T(0.5);
         cc[d] = (p[0][d] + p[1][d] + p[2][d] + p[3][d]) * T(0.25);
+        pk[d] = k == 0 ? p[0][d] : k == 1 ? p[1][d] : k == 2 ? p[2][d] : p[3][d];
+        e_a[d] = k == 0 ? e[3][d] : k == 1 ? e[0][d] : k == 2 ? e[1][d] : e[2][d];
+        e_b[d] = k == 0 ? e[0][d] : k == 1 ? e[1][d] : k == 2 ? e[2][d] : e[3][d];
     }
-    const T dx13 = cc[0] - p[k][0], dy13 = cc[1] - p[k][1], dz13 = cc[2] - p[k][2];
+    const T dx13 = cc[0] - pk[0], dy13 = cc[1] - pk[1], dz13 = cc[2] - pk[2];
     const T dx24 = e_a[0] - e_b[0], dy24 = e_a[1] - e_b[1], dz24 = e_a[2] - e_b[2];
     A[0] = T(0.5) * (dz24 * dy13 - dz13 * dy24);
     A[1] = T(0.5) * (dx24 * dz13 - dx13 * dz24);
