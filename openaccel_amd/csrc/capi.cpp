@@ -2,6 +2,7 @@
 // and the host driver of one steady outer iteration.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -158,11 +159,35 @@ int oagpu_scatter_map_copy(const oagpu_mesh* m, int* inc_ptr, int* inc, unsigned
         if (inc_ptr) std::memcpy(inc_ptr, d.h_inc_ptr.data(), sizeof(int) * d.h_inc_ptr.size());
         if (inc) std::memcpy(inc, d.h_inc.data(), sizeof(int) * d.h_inc.size());
         if (slots) {
-            // copy from the device so the check covers what the kernels read
-            std::vector<uint64_t> s(d.ninc);
-            d.inc_slots.download(s.data(), s.size());
+            // rebuilt from the device's local-node-major copy, so the check
+            // covers what the kernels read
+            const size_t nn = size_t(d.nn);
+            std::vector<int> inc_la(8 * nn), ext_ptr(nn + 1), ext(std::max(d.next, 1));
+            std::vector<uint64_t> slots_la(8 * nn), ext_slots(std::max(d.next, 1));
+            d.inc_la.download(inc_la.data(), inc_la.size());
+            d.slots_la.download(slots_la.data(), slots_la.size());
+            d.ext_ptr.download(ext_ptr.data(), ext_ptr.size());
+            d.ext.download(ext.data(), ext.size());
+            d.ext_slots.download(ext_slots.data(), ext_slots.size());
             OAGPU_HIP(hipDeviceSynchronize());
-            std::memcpy(slots, s.data(), sizeof(uint64_t) * s.size());
+            for (size_t r = 0; r < nn; ++r) {
+                for (int k = d.h_inc_ptr[r]; k < d.h_inc_ptr[r + 1]; ++k) {
+                    const int e = d.h_inc[k] >> 3, la = d.h_inc[k] & 7;
+                    uint64_t v = ~uint64_t(0);
+                    bool found = false;
+                    if (inc_la[size_t(la) * nn + r] == e) {
+                        v = slots_la[size_t(la) * nn + r];
+                        found = true;
+                    }
+                    for (int q = ext_ptr[r]; q < ext_ptr[r + 1] && !found; ++q)
+                        if (ext[q] == d.h_inc[k]) {
+                            v = ext_slots[q];
+                            found = true;
+                        }
+                    OAGPU_REQUIRE(found, "incidence missing from the device copy");
+                    slots[k] = v;
+                }
+            }
         }
     });
 }

@@ -21,6 +21,7 @@ struct Flow {
     // Krylov workspace
     DBuf<double> dinv, r, z, pk, q, w, t, s, r0, ph, sh;
     DBuf<double> V;                      // GMRES basis [(m+1)][n]
+    DBuf<float> Af, fb, fx, fr, fz, fp, fq, fdinv;  // FP32 inner solve of the mixed form
     DBuf<double> part;                   // reduction partials [64][kRedGrid]
     DBuf<double> red;                    // reduced values [64]
     DBuf<double> scal;                   // device-resident Krylov scalars

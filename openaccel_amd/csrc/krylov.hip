@@ -238,21 +238,23 @@ static int gmres(Flow& f, int max_it, double rtol, double atol, int mres, oagpu_
 }
 
 // -------------------------------------------------------------------- CG
+// Vectors and matrix values of type V (double, or float for the inner solve
+// of the mixed form); dot products are always accumulated in double.
 // scal: [0..1] rz ping-pong, [2] tol^2, [3] last rr, [4] rr0
-__global__ void __launch_bounds__(kRedBlock) k_cg_init(size_t n, const double* b,
-                                                       const double* dinv, double* x, double* r,
-                                                       double* z, double* p, double* part) {
+template <typename V>
+__global__ void __launch_bounds__(kRedBlock) k_cg_init(size_t n, const V* b, const V* dinv, V* x,
+                                                       V* r, V* z, V* p, double* part) {
     __shared__ double sh[kRedBlock];
     double rz = 0.0, rr = 0.0;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
          i += size_t(gridDim.x) * blockDim.x) {
-        const double ri = b[i], zi = dinv[i] * ri;
-        x[i] = 0.0;
+        const V ri = b[i], zi = dinv[i] * ri;
+        x[i] = V(0);
         r[i] = ri;
         z[i] = zi;
         p[i] = zi;
-        rz += ri * zi;
-        rr += ri * ri;
+        rz += double(ri) * double(zi);
+        rr += double(ri) * double(ri);
     }
     const double s0 = block_sum(rz, sh);
     const double s1 = block_sum(rr, sh);
@@ -278,50 +280,50 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_start(const double* part, doub
     }
 }
 
-template <int N>
+template <int N, typename V>
 __global__ void __launch_bounds__(kRedBlock) k_cg_spmv(int nn, const int* rp, const int* cols,
-                                                       const double* A, const double* p, double* q,
-                                                       double* part, const int* flag) {
+                                                       const V* A, const V* p, V* q, double* part,
+                                                       const int* flag) {
     __shared__ double sh[kRedBlock];
     if (flag[0]) return;
     double pq = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
-        double acc[N];
-        for (int m = 0; m < N; ++m) acc[m] = 0.0;
+        V acc[N];
+        for (int m = 0; m < N; ++m) acc[m] = V(0);
         for (int k = rp[i]; k < rp[i + 1]; ++k) {
-            const double* blk = A + size_t(N * N) * k;
+            const V* blk = A + size_t(N * N) * k;
             const size_t c = size_t(N) * cols[k];
             for (int m = 0; m < N; ++m)
                 for (int qq = 0; qq < N; ++qq) acc[m] += blk[N * m + qq] * p[c + qq];
         }
         for (int m = 0; m < N; ++m) {
             q[N * size_t(i) + m] = acc[m];
-            pq += p[N * size_t(i) + m] * acc[m];
+            pq += double(p[N * size_t(i) + m]) * double(acc[m]);
         }
     }
     const double s = block_sum(pq, sh);
     if (threadIdx.x == 0) part[2 * kRedGrid + blockIdx.x] = s;
 }
 
-__global__ void __launch_bounds__(kRedBlock) k_cg_update1(size_t n, int it, const double* p,
-                                                          const double* q, const double* dinv,
-                                                          double* x, double* r, double* z,
+template <typename V>
+__global__ void __launch_bounds__(kRedBlock) k_cg_update1(size_t n, int it, const V* p, const V* q,
+                                                          const V* dinv, V* x, V* r, V* z,
                                                           double* part, const double* scal,
                                                           const int* flag) {
     __shared__ double sh[kRedBlock];
     if (flag[0]) return;
     const double pq = reduce_partials(part, 2, sh);
-    const double alpha = scal[it & 1] / pq;
+    const V alpha = V(scal[it & 1] / pq);
     double rz = 0.0, rr = 0.0;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
          i += size_t(gridDim.x) * blockDim.x) {
         x[i] += alpha * p[i];
-        const double ri = r[i] - alpha * q[i];
+        const V ri = r[i] - alpha * q[i];
         r[i] = ri;
-        const double zi = dinv[i] * ri;
+        const V zi = dinv[i] * ri;
         z[i] = zi;
-        rz += ri * zi;
-        rr += ri * ri;
+        rz += double(ri) * double(zi);
+        rr += double(ri) * double(ri);
     }
     const double s0 = block_sum(rz, sh);
     const double s1 = block_sum(rr, sh);
@@ -331,16 +333,17 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_update1(size_t n, int it, cons
     }
 }
 
-__global__ void __launch_bounds__(kRedBlock) k_cg_update2(size_t n, int it, const double* z,
-                                                          double* p, const double* part,
-                                                          double* scal, int* flag) {
+template <typename V>
+__global__ void __launch_bounds__(kRedBlock) k_cg_update2(size_t n, int it, const V* z, V* p,
+                                                          const double* part, double* scal,
+                                                          int* flag) {
     __shared__ double sh[kRedBlock];
     if (flag[0]) return;
     const double rz = reduce_partials(part, 3, sh);
     const double rr = reduce_partials(part, 4, sh);
     const bool conv = rr <= scal[2];
     if (!conv) {
-        const double beta = rz / scal[it & 1];
+        const V beta = V(rz / scal[it & 1]);
         for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
              i += size_t(gridDim.x) * blockDim.x)
             p[i] = z[i] + beta * p[i];
@@ -353,25 +356,25 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_update2(size_t n, int it, cons
     }
 }
 
-template <int N>
-static int cg(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
+// Device-resident PCG on A x = b (values V); returns iterations, fills
+// initial / final ||r||_2 in V arithmetic.
+template <int N, typename V>
+static int cg_core(Flow& f, const V* A, const V* b, const V* dinv, V* x, V* r, V* z, V* p, V* q,
+                   int max_it, double rtol, double atol, double* r0, double* r1) {
     DevMesh& m = *f.m;
     const size_t n = size_t(m.nn) * N;
-    const double* A = N == 3 ? f.A3.p : f.A1.p;
-    const double* b = N == 3 ? f.b3.p : f.b1.p;
-    double* x = N == 3 ? f.x3.p : f.x1.p;
-    k_cg_init<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, b, f.dinv.p, x, f.r.p, f.z.p, f.pk.p, f.part.p);
+    k_cg_init<V><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, b, dinv, x, r, z, p, f.part.p);
     k_cg_start<<<1, kRedBlock, 0, f.stream>>>(f.part.p, rtol, atol, f.scal.p, f.flag.p);
     OAGPU_HIP(hipGetLastError());
     const int poll = 8;
     int hflag[2] = {0, 0};
     for (int it = 0; it < max_it; ++it) {
-        k_cg_spmv<N><<<kRedGrid, kRedBlock, 0, f.stream>>>(m.nn, m.row_ptr.p, m.cols.p, A, f.pk.p,
-                                                            f.q.p, f.part.p, f.flag.p);
-        k_cg_update1<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, it, f.pk.p, f.q.p, f.dinv.p, x, f.r.p,
-                                                           f.z.p, f.part.p, f.scal.p, f.flag.p);
-        k_cg_update2<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, it, f.z.p, f.pk.p, f.part.p, f.scal.p,
-                                                           f.flag.p);
+        k_cg_spmv<N, V><<<kRedGrid, kRedBlock, 0, f.stream>>>(m.nn, m.row_ptr.p, m.cols.p, A, p, q,
+                                                               f.part.p, f.flag.p);
+        k_cg_update1<V><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, it, p, q, dinv, x, r, z, f.part.p,
+                                                              f.scal.p, f.flag.p);
+        k_cg_update2<V><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, it, z, p, f.part.p, f.scal.p,
+                                                              f.flag.p);
         if ((it + 1) % poll == 0 || it + 1 == max_it) {
             OAGPU_HIP(hipGetLastError());
             f.flag.download(hflag, 2, f.stream);
@@ -383,9 +386,77 @@ static int cg(Flow& f, int max_it, double rtol, double atol, oagpu_control_data*
     double sc[5];
     f.scal.download(sc, 5, f.stream);
     OAGPU_HIP(hipStreamSynchronize(f.stream));
-    cd->solver_initial_res = std::sqrt(sc[4]);
-    cd->solver_final_res = std::sqrt(sc[3]);
+    *r0 = std::sqrt(sc[4]);
+    *r1 = std::sqrt(sc[3]);
     return hflag[1];
+}
+
+template <int N>
+static int cg(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
+    const double* A = N == 3 ? f.A3.p : f.A1.p;
+    const double* b = N == 3 ? f.b3.p : f.b1.p;
+    double* x = N == 3 ? f.x3.p : f.x1.p;
+    return cg_core<N, double>(f, A, b, f.dinv.p, x, f.r.p, f.z.p, f.pk.p, f.q.p, max_it, rtol, atol,
+                              &cd->solver_initial_res, &cd->solver_final_res);
+}
+
+// ---------------------------------------------- mixed: FP32 CG + FP64 refinement
+__global__ void k_to_float(size_t n, double s, const double* x, float* y) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = float(s * x[i]);
+}
+
+__global__ void k_refine(size_t n, double s, const float* d, double* x) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += s * double(d[i]);
+}
+
+// Iterative refinement: r = b - A x in FP64, FP32 PCG on A d = r / ||r||,
+// x += ||r|| d in FP64, until ||r|| <= max(rtol ||b||, atol) or the total
+// inner iteration budget max_it is spent.  The FP32 matrix is A rounded.
+template <int N>
+static int cg_mixed(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
+    DevMesh& m = *f.m;
+    const size_t n = size_t(m.nn) * N, nv = size_t(m.nnz) * N * N;
+    const double* A = N == 3 ? f.A3.p : f.A1.p;
+    const double* b = N == 3 ? f.b3.p : f.b1.p;
+    double* x = N == 3 ? f.x3.p : f.x1.p;
+    if (f.Af.n < nv) f.Af.alloc(nv);
+    for (auto* v : {&f.fb, &f.fx, &f.fr, &f.fz, &f.fp, &f.fq, &f.fdinv})
+        if (v->n < n) v->alloc(n);
+    const unsigned B = 256;
+    k_to_float<<<unsigned((nv + B - 1) / B), B, 0, f.stream>>>(nv, 1.0, A, f.Af.p);
+    k_to_float<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, 1.0, f.dinv.p, f.fdinv.p);
+    OAGPU_HIP(hipMemsetAsync(x, 0, n * sizeof(double), f.stream));
+    OAGPU_HIP(hipMemcpyAsync(f.r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, f.stream));
+    double rn;
+    {
+        const double* xs[1] = {f.r.p};
+        dots(f, n, 1, xs, xs, &rn);
+        rn = std::sqrt(rn);
+    }
+    cd->solver_initial_res = rn;
+    const double tol = std::max(rtol * rn, atol);
+    // an FP32 inner solve cannot usefully go below ~1e-5 of its right-hand side
+    const double inner_rtol = std::max(rtol, 1.0e-5);
+    int used = 0;
+    while (used < max_it && rn > tol) {
+        const double inner = std::max(inner_rtol, 0.5 * tol / rn);
+        k_to_float<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, 1.0 / rn, f.r.p, f.fb.p);
+        double i0, i1;
+        const int it = cg_core<N, float>(f, f.Af.p, f.fb.p, f.fdinv.p, f.fx.p, f.fr.p, f.fz.p, f.fp.p,
+                                         f.fq.p, max_it - used, inner, 0.0, &i0, &i1);
+        used += std::max(it, 1);
+        k_refine<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, rn, f.fx.p, x);
+        spmv<N>(f, A, x, nullptr, f.w.p);
+        k_residual_copy<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, b, f.w.p, f.r.p);
+        const double* xs[1] = {f.r.p};
+        dots(f, n, 1, xs, xs, &rn);
+        rn = std::sqrt(rn);
+        if (it == 0) break;
+    }
+    cd->solver_final_res = rn;
+    return used;
 }
 
 // --------------------------------------------------------------- BiCGStab
@@ -484,6 +555,8 @@ int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double ato
     if (method == OAGPU_GMRES)
         it = N == 3 ? gmres<3>(f, max_it, rtol, atol, restart, cd)
                     : gmres<1>(f, max_it, rtol, atol, restart, cd);
+    else if (method == OAGPU_CG && f.prm.precision == OAGPU_MIXED)
+        it = N == 3 ? cg_mixed<3>(f, max_it, rtol, atol, cd) : cg_mixed<1>(f, max_it, rtol, atol, cd);
     else if (method == OAGPU_CG)
         it = N == 3 ? cg<3>(f, max_it, rtol, atol, cd) : cg<1>(f, max_it, rtol, atol, cd);
     else if (method == OAGPU_BICGSTAB)

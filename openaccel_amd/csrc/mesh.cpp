@@ -172,7 +172,37 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
     dm.diag_off.upload(dm.h_diag_off);
     dm.inc_ptr.upload(dm.h_inc_ptr);
     dm.inc.upload(dm.h_inc);
-    dm.inc_slots.upload(dm.h_inc_slots);
+    {
+        // local-node major copy of the incidences for the row kernels
+        std::vector<int> inc_la(size_t(8) * nn, -1);
+        std::vector<uint64_t> slots_la(size_t(8) * nn, ~uint64_t(0));
+        std::vector<int> ext_ptr(size_t(nn) + 1, 0), ext;
+        std::vector<uint64_t> ext_slots;
+        for (int r = 0; r < nn; ++r) {
+            for (int k = dm.h_inc_ptr[r]; k < dm.h_inc_ptr[r + 1]; ++k) {
+                const int la = dm.h_inc[k] & 7;
+                const size_t o = size_t(la) * nn + r;
+                if (inc_la[o] < 0) {
+                    inc_la[o] = dm.h_inc[k] >> 3;
+                    slots_la[o] = dm.h_inc_slots[k];
+                } else {
+                    ext.push_back(dm.h_inc[k]);
+                    ext_slots.push_back(dm.h_inc_slots[k]);
+                }
+            }
+            ext_ptr[r + 1] = int(ext.size());
+        }
+        dm.next = int(ext.size());
+        dm.inc_la.upload(inc_la);
+        dm.slots_la.upload(slots_la);
+        dm.ext_ptr.upload(ext_ptr);
+        if (ext.empty()) {
+            ext.push_back(0);
+            ext_slots.push_back(0);
+        }
+        dm.ext.upload(ext);
+        dm.ext_slots.upload(ext_slots);
+    }
     dm.binc_ptr.upload(dm.h_binc_ptr);
     if (dm.nbinc) {
         dm.binc.upload(dm.h_binc);

@@ -21,9 +21,12 @@ struct HostMesh {
 //
 // * CRS node graph (reduced or full stencil) with sorted local columns and
 //   the diagonal position of every row (nodeGraph.cpp:29-392).
-// * node -> element incidences: (elem << 3 | local node), elements in
-//   ascending id, so each row sums element contributions in the reference's
-//   element order.
+// * node -> element incidences, stored local-node major: inc_la[la][row] is
+//   the element in which `row` is local node la (-1 if none), so that the
+//   row kernels loop over la = 0..7 with compile-time indices and every lane
+//   of a wavefront takes the same path; a node that is local node la of more
+//   than one element (unstructured meshes) keeps the others in a small CSR
+//   list (ext_*).
 // * per incidence, the element -> CRS scatter map of that row: 8 bytes, the
 //   position in the row of each of the element's 8 nodes, 0xFF when the
 //   column is not in the (reduced) row and is lumped into the diagonal block
@@ -31,13 +34,16 @@ struct HostMesh {
 // * node -> wall-face incidences (face << 2 | face ip whose nearest node is
 //   the row), with their own scatter maps.
 struct DevMesh {
-    int nn = 0, ne = 0, nbf = 0, nnz = 0, ninc = 0, nbinc = 0, max_row = 0;
+    int nn = 0, ne = 0, nbf = 0, nnz = 0, ninc = 0, nbinc = 0, max_row = 0, next = 0;
     int reduced = 1;
     DBuf<double> xyz;          // [nn][3]
     DBuf<int> conn;            // [ne][8]
     DBuf<int> row_ptr, cols, diag_off;
-    DBuf<int> inc_ptr, inc;    // [nn+1], [ninc]
-    DBuf<uint64_t> inc_slots;  // [ninc]
+    DBuf<int> inc_ptr, inc;    // [nn+1], [ninc] element-ordered (set-up kernels)
+    DBuf<int> inc_la;          // [8][nn]
+    DBuf<uint64_t> slots_la;   // [8][nn]
+    DBuf<int> ext_ptr, ext;    // [nn+1], [next]: (elem << 3 | la) beyond the first per la
+    DBuf<uint64_t> ext_slots;  // [next]
     DBuf<int> binc_ptr, binc;  // [nn+1], [nbinc]
     DBuf<uint64_t> binc_slots; // [nbinc]
     DBuf<int> bf_elem, bf_ord, bf_bc;

@@ -178,9 +178,11 @@ struct MomArgs {
     const int* conn;
     const int* row_ptr;
     const int* diag_off;
-    const int* inc_ptr;
-    const int* inc;
-    const uint64_t* inc_slots;
+    const int* inc_la;
+    const uint64_t* slots_la;
+    const int* ext_ptr;
+    const int* ext;
+    const uint64_t* ext_slots;
     const int* binc_ptr;
     const int* binc;
     const uint64_t* binc_slots;
@@ -380,10 +382,17 @@ __global__ void __launch_bounds__(256) k_momentum_rows(const MomArgs a) {
     for (int s = rp; s < re; ++s)
         if (s - rp != dof)
             for (int k = 0; k < 9; ++k) a.vals[9 * size_t(s) + k] = 0.0;
-    for (int k = a.inc_ptr[row]; k < a.inc_ptr[row + 1]; ++k) {
-        const int pk = a.inc[k];
+    // incidences local-node major: wave-uniform control flow
+    sfor<8>([&](auto la_) {
+        constexpr int LA = decltype(la_)::value;
+        const size_t o = size_t(LA) * a.nn + row;
+        const int e = a.inc_la[o];
+        if (e >= 0) mom_incidence<T, SH, LA>(a, e, a.slots_la[o], rp, dof, xr, D, R, bad);
+    });
+    for (int k = a.ext_ptr[row]; k < a.ext_ptr[row + 1]; ++k) {
+        const int pk = a.ext[k];
         const int e = pk >> 3;
-        const uint64_t sl = a.inc_slots[k];
+        const uint64_t sl = a.ext_slots[k];
         switch (pk & 7) {
             case 0: mom_incidence<T, SH, 0>(a, e, sl, rp, dof, xr, D, R, bad); break;
             case 1: mom_incidence<T, SH, 1>(a, e, sl, rp, dof, xr, D, R, bad); break;
@@ -418,9 +427,11 @@ void launch_assemble_momentum(Flow& f) {
     a.conn = m.conn.p;
     a.row_ptr = m.row_ptr.p;
     a.diag_off = m.diag_off.p;
-    a.inc_ptr = m.inc_ptr.p;
-    a.inc = m.inc.p;
-    a.inc_slots = m.inc_slots.p;
+    a.inc_la = m.inc_la.p;
+    a.slots_la = m.slots_la.p;
+    a.ext_ptr = m.ext_ptr.p;
+    a.ext = m.ext.p;
+    a.ext_slots = m.ext_slots.p;
     a.binc_ptr = m.binc_ptr.p;
     a.binc = m.binc.p;
     a.binc_slots = m.binc_slots.p;
@@ -463,9 +474,11 @@ struct PcArgs {
     const int* conn;
     const int* row_ptr;
     const int* diag_off;
-    const int* inc_ptr;
-    const int* inc;
-    const uint64_t* inc_slots;
+    const int* inc_la;
+    const uint64_t* slots_la;
+    const int* ext_ptr;
+    const int* ext;
+    const uint64_t* ext_slots;
     const double* scvw;
     const double* U;
     const double* du;
@@ -571,10 +584,16 @@ __global__ void __launch_bounds__(256) k_pcorr_rows(const PcArgs a) {
     double D = 0.0, R = 0.0;
     for (int s = rp; s < re; ++s)
         if (s - rp != dof) a.vals[s] = 0.0;
-    for (int k = a.inc_ptr[row]; k < a.inc_ptr[row + 1]; ++k) {
-        const int pk = a.inc[k];
+    sfor<8>([&](auto la_) {
+        constexpr int LA = decltype(la_)::value;
+        const size_t o = size_t(LA) * a.nn + row;
+        const int e = a.inc_la[o];
+        if (e >= 0) pc_incidence<T, SP, LA>(a, e, a.slots_la[o], rp, dof, xr, pr, D, R, bad);
+    });
+    for (int k = a.ext_ptr[row]; k < a.ext_ptr[row + 1]; ++k) {
+        const int pk = a.ext[k];
         const int e = pk >> 3;
-        const uint64_t sl = a.inc_slots[k];
+        const uint64_t sl = a.ext_slots[k];
         switch (pk & 7) {
             case 0: pc_incidence<T, SP, 0>(a, e, sl, rp, dof, xr, pr, D, R, bad); break;
             case 1: pc_incidence<T, SP, 1>(a, e, sl, rp, dof, xr, pr, D, R, bad); break;
@@ -604,9 +623,11 @@ void launch_assemble_pcorr(Flow& f) {
     a.conn = m.conn.p;
     a.row_ptr = m.row_ptr.p;
     a.diag_off = m.diag_off.p;
-    a.inc_ptr = m.inc_ptr.p;
-    a.inc = m.inc.p;
-    a.inc_slots = m.inc_slots.p;
+    a.inc_la = m.inc_la.p;
+    a.slots_la = m.slots_la.p;
+    a.ext_ptr = m.ext_ptr.p;
+    a.ext = m.ext.p;
+    a.ext_slots = m.ext_slots.p;
     a.scvw = m.scvw.p;
     a.U = f.U.p;
     a.du = f.du.p;
@@ -789,7 +810,8 @@ __device__ __forceinline__ void grad_incidence(const double* xyz, const int* con
 
 template <int N>
 __global__ void __launch_bounds__(256) k_gradient(int nn, const double* xyz, const int* conn,
-                                                  const int* inc_ptr, const int* inc,
+                                                  const int* inc_la, const int* ext_ptr,
+                                                  const int* ext,
                                                   const int* binc_ptr, const int* binc,
                                                   const int* bf_elem, const int* bf_ord,
                                                   const double* vol, const double* phi, int shifted,
@@ -802,9 +824,14 @@ __global__ void __launch_bounds__(256) k_gradient(int nn, const double* xyz, con
     for (int i = 0; i < N; ++i) pa[i] = phi[N * size_t(a) + i];
     const double iv = 1.0 / vol[a];
     const double xr[3] = {xyz[3 * size_t(a)], xyz[3 * size_t(a) + 1], xyz[3 * size_t(a) + 2]};
-    for (int k = inc_ptr[a]; k < inc_ptr[a + 1]; ++k) {
-        const int e = inc[k] >> 3;
-        switch (inc[k] & 7) {
+    sfor<8>([&](auto la_) {
+        constexpr int LA = decltype(la_)::value;
+        const int e = inc_la[size_t(LA) * nn + a];
+        if (e >= 0) grad_incidence<N, LA>(xyz, conn, phi, e, xr, pa, iv, gurf, shifted, G);
+    });
+    for (int k = ext_ptr[a]; k < ext_ptr[a + 1]; ++k) {
+        const int e = ext[k] >> 3;
+        switch (ext[k] & 7) {
             case 0: grad_incidence<N, 0>(xyz, conn, phi, e, xr, pa, iv, gurf, shifted, G); break;
             case 1: grad_incidence<N, 1>(xyz, conn, phi, e, xr, pa, iv, gurf, shifted, G); break;
             case 2: grad_incidence<N, 2>(xyz, conn, phi, e, xr, pa, iv, gurf, shifted, G); break;
@@ -840,12 +867,12 @@ void launch_gradient(Flow& f, int which) {
     const unsigned B = 256;
     if (which == 0)
         k_gradient<3><<<xcd_grid(m.nn, B), B, 0, f.stream>>>(
-            m.nn, m.xyz.p, m.conn.p, m.inc_ptr.p, m.inc.p, m.binc_ptr.p, m.binc.p, m.bf_elem.p,
-            m.bf_ord.p, m.vol.p, f.U.p, f.prm.u_shifted, f.prm.grad_urf, f.gradU.p);
+            m.nn, m.xyz.p, m.conn.p, m.inc_la.p, m.ext_ptr.p, m.ext.p, m.binc_ptr.p, m.binc.p,
+            m.bf_elem.p, m.bf_ord.p, m.vol.p, f.U.p, f.prm.u_shifted, f.prm.grad_urf, f.gradU.p);
     else
         k_gradient<1><<<xcd_grid(m.nn, B), B, 0, f.stream>>>(
-            m.nn, m.xyz.p, m.conn.p, m.inc_ptr.p, m.inc.p, m.binc_ptr.p, m.binc.p, m.bf_elem.p,
-            m.bf_ord.p, m.vol.p, f.p.p, f.prm.p_shifted, f.prm.grad_urf, f.gradP.p);
+            m.nn, m.xyz.p, m.conn.p, m.inc_la.p, m.ext_ptr.p, m.ext.p, m.binc_ptr.p, m.binc.p,
+            m.bf_elem.p, m.bf_ord.p, m.vol.p, f.p.p, f.prm.p_shifted, f.prm.grad_urf, f.gradP.p);
     OAGPU_HIP(hipGetLastError());
 }
 

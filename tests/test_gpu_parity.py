@@ -226,3 +226,33 @@ def test_large_mesh_properties(oa, meshgen):
     f.assemble_pcorr()
     v2, b2 = f.system(1)
     assert np.array_equal(v1, v2) and np.array_equal(b1, b2)
+
+
+def test_mixed_refinement_solve(oracle, oa, meshgen):
+    """FP32 PCG under FP64 iterative refinement reaches the FP64 tolerance:
+    ||b - A x|| <= 1e-8 ||b|| in FP64, x within 1e-6 of the FP64 CG solution."""
+    c, dm, f64, m = pair(oracle, oa, meshgen, n=10, perturb=0.15, seed=5)
+    fmx = oa.FlowCase(dm, oa.default_params(1), BC)
+    sync_state(c, f64, np.random.default_rng(16))
+    c.assemble_pcorr()
+    for f in (f64, fmx):
+        f.set_system(1, c.vals(1), c.rhs(1))
+    cd64 = f64.solve(1, oa.OAGPU_CG, 2000, 1e-10)
+    cdmx = fmx.solve(1, oa.OAGPU_CG, 2000, 1e-8)
+    assert cdmx.solver_final_res <= 1e-8 * cdmx.solver_initial_res * 1.0001
+    assert abs(cdmx.solver_initial_res - cd64.solver_initial_res) <= 1e-12 * cd64.solver_initial_res
+    assert rel_err(fmx.solution(1), f64.solution(1)) < 1e-6
+
+
+def test_mixed_outer_iterations_track_fp64(oa, meshgen):
+    """The mixed form follows the FP64 form's residual history (stated bar:
+    1e-3 relative on the control residuals over 5 outer iterations)."""
+    m = meshgen.cavity_hex(12, perturb=0.1, seed=2)
+    hist = {}
+    for prec in (0, 1):
+        dm = oa.DeviceMesh(m)
+        f = oa.FlowCase(dm, oa.default_params(prec), BC)
+        f.upload("U", meshgen.initial_velocity(dm.nn, 0))
+        f.initialize()
+        hist[prec] = np.array([f.outer_iteration([20, 0.1, 400, 1e-8])[:4] for _ in range(5)])
+    np.testing.assert_allclose(hist[1], hist[0], rtol=1e-3)
