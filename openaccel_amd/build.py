@@ -21,7 +21,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [
     "-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}",
-    # packed FP32 instructions stay off ("Build")
+    # packed FP32 instructions (v_pk_*_f32) stay off: hipcc -O3 would emit them
+    # from plain float code and some of their forms are not trusted on gfx950
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     "-Wall", "-Wno-unused-result", "-Wno-unused-variable",

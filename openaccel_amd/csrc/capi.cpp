@@ -219,7 +219,7 @@ int oagpu_flow_create(oagpu_mesh* m, const oagpu_flow_params* prm, int n_bc, con
         for (int fc = 0; fc < m->m.nbf; ++fc) {
             // validated lazily: boundary ids must index the table
         }
-        const size_t nn = size_t(m->m.nn), ne = size_t(m->m.ne), nnz = size_t(m->m.nnz);
+        const size_t nn = size_t(m->m.nn), ne = size_t(m->m.ne), nsell = m->m.nsell;
         f.U.alloc(nn * 3);
         f.p.alloc(nn);
         f.gradU.alloc(nn * 9);
@@ -227,10 +227,10 @@ int oagpu_flow_create(oagpu_mesh* m, const oagpu_flow_params* prm, int n_bc, con
         f.du.alloc(nn * 3);
         f.mdot.alloc(ne * 12);
         f.div.alloc(nn);
-        f.A3.alloc(nnz * 9);
+        f.A3.alloc(nsell * 9);
         f.b3.alloc(nn * 3);
         f.x3.alloc(nn * 3);
-        f.A1.alloc(nnz);
+        f.A1.alloc(nsell);
         f.b1.alloc(nn);
         f.x1.alloc(nn);
         for (auto* b : {&f.dinv, &f.r, &f.z, &f.pk, &f.q, &f.w, &f.t, &f.s, &f.r0, &f.ph, &f.sh})
@@ -328,9 +328,14 @@ int oagpu_system_download(oagpu_flow* h, int N, double* vals, double* b) {
     return guarded([&] {
         OAGPU_REQUIRE(h && (N == 1 || N == 3), "bad argument");
         oagpu::Flow& f = h->f;
-        if (vals) (N == 3 ? f.A3 : f.A1).download(vals, size_t(f.m->nnz) * N * N, f.stream);
+        std::vector<double> sell;
+        if (vals) {
+            sell.resize(f.m->nsell * N * N);
+            (N == 3 ? f.A3 : f.A1).download(sell.data(), sell.size(), f.stream);
+        }
         if (b) (N == 3 ? f.b3 : f.b1).download(b, size_t(f.m->nn) * N, f.stream);
         OAGPU_HIP(hipStreamSynchronize(f.stream));
+        if (vals) oagpu::sell_to_crs(*f.m, N * N, sell.data(), vals);
     });
 }
 
@@ -338,7 +343,12 @@ int oagpu_system_upload(oagpu_flow* h, int N, const double* vals, const double* 
     return guarded([&] {
         OAGPU_REQUIRE(h && (N == 1 || N == 3), "bad argument");
         oagpu::Flow& f = h->f;
-        if (vals) (N == 3 ? f.A3 : f.A1).upload(vals, size_t(f.m->nnz) * N * N, f.stream);
+        std::vector<double> sell;
+        if (vals) {
+            sell.assign(f.m->nsell * N * N, 0.0);
+            oagpu::crs_to_sell(*f.m, N * N, vals, sell.data());
+            (N == 3 ? f.A3 : f.A1).upload(sell.data(), sell.size(), f.stream);
+        }
         if (b) (N == 3 ? f.b3 : f.b1).upload(b, size_t(f.m->nn) * N, f.stream);
         OAGPU_HIP(hipStreamSynchronize(f.stream));
     });

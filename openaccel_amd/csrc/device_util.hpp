@@ -93,6 +93,25 @@ __device__ inline void load_conn(const int* conn, int e, int nd[8]) {
     nd[4] = b.x; nd[5] = b.y; nd[6] = b.z; nd[7] = b.w;
 }
 
+// Sliced-ELL row handle of the owning lane: the slice's first slot and the
+// lane's position in the slice (rows of one wavefront share a slice).
+struct SellRow {
+    size_t base;  // (slice_ptr[row / 64] * NN) * 64 + row % 64
+    int width;    // slots of the slice
+};
+
+template <int NN>
+__device__ __forceinline__ SellRow sell_row(const int* slice_ptr, const int* slice_w, int row) {
+    const int sl = row >> 6;
+    return {size_t(slice_ptr[sl]) * NN * 64 + (row & 63), slice_w[sl]};
+}
+
+// value k of the NN-block in slot s: base + (s * NN + k) * 64
+template <int NN>
+__device__ __forceinline__ double* sell_val(double* vals, const SellRow& r, int s, int k) {
+    return vals + r.base + size_t(s * NN + k) * 64;
+}
+
 __device__ inline int slot_of(uint64_t sl, int c) { return int((sl >> (8 * c)) & 0xFFu); }
 
 

@@ -170,16 +170,18 @@ void field_scales(Flow& f, double* U_scale, double* p_scale) {
 
 // RMS over rows of b/diag per component
 template <int N>
-__global__ void __launch_bounds__(kRedBlock) k_scaled_res(int nn, const int* rp, const int* doff,
+__global__ void __launch_bounds__(kRedBlock) k_scaled_res(int nn, const int* slice_ptr,
+                                                          const int* slice_w, const int* doff,
                                                           const double* A, const double* b,
                                                           double* part) {
     __shared__ double s[N][kRedBlock];
     double acc[N];
     for (int k = 0; k < N; ++k) acc[k] = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
-        const double* d = A + size_t(N * N) * (rp[i] + doff[i]);
+        const SellRow sr = sell_row<N * N>(slice_ptr, slice_w, i);
+        const double* d = A + sr.base + size_t(doff[i]) * (N * N * 64);
         for (int k = 0; k < N; ++k) {
-            const double r = b[N * size_t(i) + k] / d[(N + 1) * k];
+            const double r = b[N * size_t(i) + k] / d[(N + 1) * k * 64];
             acc[k] += r * r;
         }
     }
@@ -197,10 +199,10 @@ void scaled_residual(Flow& f, int N, double* out) {
     const int G = 128;
     DevMesh& m = *f.m;
     if (N == 3)
-        k_scaled_res<3><<<G, kRedBlock, 0, f.stream>>>(m.nn, m.row_ptr.p, m.diag_off.p, f.A3.p,
+        k_scaled_res<3><<<G, kRedBlock, 0, f.stream>>>(m.nn, m.slice_ptr.p, m.slice_w.p, m.diag_off.p, f.A3.p,
                                                        f.b3.p, f.part.p);
     else
-        k_scaled_res<1><<<G, kRedBlock, 0, f.stream>>>(m.nn, m.row_ptr.p, m.diag_off.p, f.A1.p,
+        k_scaled_res<1><<<G, kRedBlock, 0, f.stream>>>(m.nn, m.slice_ptr.p, m.slice_w.p, m.diag_off.p, f.A1.p,
                                                        f.b1.p, f.part.p);
     OAGPU_HIP(hipGetLastError());
     std::vector<double> h(size_t(N) * G);

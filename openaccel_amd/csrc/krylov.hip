@@ -1,4 +1,5 @@
-// krylov.hip -- device Krylov solvers for the block-CRS systems (N = 1, 3).
+// krylov.hip -- device Krylov solvers for the block systems (N = 1, 3), whose
+// values are stored sliced-ELL (64-row slices, lane-contiguous).
 //
 // Replaces liblinsolve's linearSolver::Base<N>::solve (reached from
 // linearSystem<N>::solveSystem_, src/equation/linearSystem.h:416-462).  The
@@ -14,6 +15,7 @@
 // consumer block re-reduces the kRedGrid partials itself, so an iteration is
 // three launches with no host synchronisation; the host only polls the
 // convergence flag every few iterations.
+#include "device_util.hpp"
 #include "flow.hpp"
 
 #include <cmath>
@@ -21,30 +23,48 @@
 namespace oagpu {
 
 template <int N>
-__global__ void __launch_bounds__(256) k_jacobi_inv(int nn, const int* rp, const int* doff,
-                                                    const double* A, double* dinv) {
+__global__ void __launch_bounds__(256) k_jacobi_inv(int nn, const int* slice_ptr, const int* slice_w,
+                                                    const int* doff, const double* A, double* dinv) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nn) return;
-    const double* d = A + size_t(N * N) * (rp[i] + doff[i]);
-    for (int k = 0; k < N; ++k) dinv[N * size_t(i) + k] = 1.0 / d[(N + 1) * k];
+    const SellRow sr = sell_row<N * N>(slice_ptr, slice_w, i);
+    for (int k = 0; k < N; ++k)
+        dinv[N * size_t(i) + k] = 1.0 / A[sr.base + size_t(doff[i] * N * N + (N + 1) * k) * 64];
+}
+
+// acc = sum over the sliced-ELL slots of row i of A_block * xv, with
+// xv = s .* x (s = nullptr: xv = x).  The lanes of a wavefront own the 64 rows
+// of one slice: every column index and value load is one contiguous 256/512 B
+// wavefront access.
+template <int N, typename V>
+__device__ __forceinline__ void sell_row_product(int i, const int* slice_ptr, const int* slice_w,
+                                                 const int* scols, const V* A, const V* x,
+                                                 const V* s, V acc[N]) {
+    const int sl = i >> 6, ln = i & 63;
+    const size_t s0 = size_t(slice_ptr[sl]);
+    const int w = slice_w[sl];
+    const int* cp = scols + s0 * 64 + ln;
+    const V* ap = A + s0 * (N * N * 64) + ln;
+    for (int m = 0; m < N; ++m) acc[m] = V(0);
+#pragma unroll 2
+    for (int q = 0; q < w; ++q) {
+        const size_t c = size_t(N) * cp[64 * q];
+        V xv[N];
+        for (int r = 0; r < N; ++r) xv[r] = s ? s[c + r] * x[c + r] : x[c + r];
+        const V* blk = ap + size_t(q) * (N * N * 64);
+        for (int m = 0; m < N; ++m)
+            for (int r = 0; r < N; ++r) acc[m] += blk[(N * m + r) * 64] * xv[r];
+    }
 }
 
 // y = A (s .* x) (s = nullptr: y = A x)
 template <int N>
-__global__ void __launch_bounds__(kRedBlock) k_spmv(int nn, const int* rp, const int* cols,
-                                                    const double* A, const double* x,
-                                                    const double* s, double* y) {
+__global__ void __launch_bounds__(kRedBlock) k_spmv(int nn, const int* slice_ptr, const int* slice_w,
+                                                    const int* scols, const double* A,
+                                                    const double* x, const double* s, double* y) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
         double acc[N];
-        for (int m = 0; m < N; ++m) acc[m] = 0.0;
-        for (int k = rp[i]; k < rp[i + 1]; ++k) {
-            const double* blk = A + size_t(N * N) * k;
-            const size_t c = size_t(N) * cols[k];
-            double xv[N];
-            for (int q = 0; q < N; ++q) xv[q] = s ? s[c + q] * x[c + q] : x[c + q];
-            for (int m = 0; m < N; ++m)
-                for (int q = 0; q < N; ++q) acc[m] += blk[N * m + q] * xv[q];
-        }
+        sell_row_product<N, double>(i, slice_ptr, slice_w, scols, A, x, s, acc);
         for (int m = 0; m < N; ++m) y[N * size_t(i) + m] = acc[m];
     }
 }
@@ -143,7 +163,8 @@ __global__ void k_residual_copy(size_t n, const double* b, const double* Ax, dou
 template <int N>
 static void spmv(Flow& f, const double* A, const double* x, const double* s, double* y) {
     DevMesh& m = *f.m;
-    k_spmv<N><<<kRedGrid, kRedBlock, 0, f.stream>>>(m.nn, m.row_ptr.p, m.cols.p, A, x, s, y);
+    k_spmv<N><<<kRedGrid, kRedBlock, 0, f.stream>>>(m.nn, m.slice_ptr.p, m.slice_w.p, m.sell_cols.p,
+                                                     A, x, s, y);
     OAGPU_HIP(hipGetLastError());
 }
 
@@ -281,7 +302,8 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_start(const double* part, doub
 }
 
 template <int N, typename V>
-__global__ void __launch_bounds__(kRedBlock) k_cg_spmv(int nn, const int* rp, const int* cols,
+__global__ void __launch_bounds__(kRedBlock) k_cg_spmv(int nn, const int* slice_ptr,
+                                                       const int* slice_w, const int* scols,
                                                        const V* A, const V* p, V* q, double* part,
                                                        const int* flag) {
     __shared__ double sh[kRedBlock];
@@ -289,13 +311,7 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_spmv(int nn, const int* rp, co
     double pq = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
         V acc[N];
-        for (int m = 0; m < N; ++m) acc[m] = V(0);
-        for (int k = rp[i]; k < rp[i + 1]; ++k) {
-            const V* blk = A + size_t(N * N) * k;
-            const size_t c = size_t(N) * cols[k];
-            for (int m = 0; m < N; ++m)
-                for (int qq = 0; qq < N; ++qq) acc[m] += blk[N * m + qq] * p[c + qq];
-        }
+        sell_row_product<N, V>(i, slice_ptr, slice_w, scols, A, p, nullptr, acc);
         for (int m = 0; m < N; ++m) {
             q[N * size_t(i) + m] = acc[m];
             pq += double(p[N * size_t(i) + m]) * double(acc[m]);
@@ -369,8 +385,8 @@ static int cg_core(Flow& f, const V* A, const V* b, const V* dinv, V* x, V* r, V
     const int poll = 8;
     int hflag[2] = {0, 0};
     for (int it = 0; it < max_it; ++it) {
-        k_cg_spmv<N, V><<<kRedGrid, kRedBlock, 0, f.stream>>>(m.nn, m.row_ptr.p, m.cols.p, A, p, q,
-                                                               f.part.p, f.flag.p);
+        k_cg_spmv<N, V><<<kRedGrid, kRedBlock, 0, f.stream>>>(
+            m.nn, m.slice_ptr.p, m.slice_w.p, m.sell_cols.p, A, p, q, f.part.p, f.flag.p);
         k_cg_update1<V><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, it, p, q, dinv, x, r, z, f.part.p,
                                                               f.scal.p, f.flag.p);
         k_cg_update2<V><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, it, z, p, f.part.p, f.scal.p,
@@ -417,7 +433,7 @@ __global__ void k_refine(size_t n, double s, const float* d, double* x) {
 template <int N>
 static int cg_mixed(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
     DevMesh& m = *f.m;
-    const size_t n = size_t(m.nn) * N, nv = size_t(m.nnz) * N * N;
+    const size_t n = size_t(m.nn) * N, nv = m.nsell * N * N;
     const double* A = N == 3 ? f.A3.p : f.A1.p;
     const double* b = N == 3 ? f.b3.p : f.b1.p;
     double* x = N == 3 ? f.x3.p : f.x1.p;
@@ -545,11 +561,11 @@ int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double ato
     OAGPU_REQUIRE(N == 1 || N == 3, "block size must be 1 or 3");
     const unsigned B = 256;
     if (N == 3)
-        k_jacobi_inv<3><<<(m.nn + B - 1) / B, B, 0, f.stream>>>(m.nn, m.row_ptr.p, m.diag_off.p,
-                                                                 f.A3.p, f.dinv.p);
+        k_jacobi_inv<3><<<(m.nn + B - 1) / B, B, 0, f.stream>>>(m.nn, m.slice_ptr.p, m.slice_w.p,
+                                                                 m.diag_off.p, f.A3.p, f.dinv.p);
     else
-        k_jacobi_inv<1><<<(m.nn + B - 1) / B, B, 0, f.stream>>>(m.nn, m.row_ptr.p, m.diag_off.p,
-                                                                 f.A1.p, f.dinv.p);
+        k_jacobi_inv<1><<<(m.nn + B - 1) / B, B, 0, f.stream>>>(m.nn, m.slice_ptr.p, m.slice_w.p,
+                                                                 m.diag_off.p, f.A1.p, f.dinv.p);
     OAGPU_HIP(hipGetLastError());
     int it = 0;
     if (method == OAGPU_GMRES)

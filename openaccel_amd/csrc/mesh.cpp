@@ -112,6 +112,31 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
     }
     std::vector<int>().swap(cand);
 
+    // ---- sliced-ELL layout of the values (64-row slices)
+    dm.nslice = (nn + 63) / 64;
+    dm.h_slice_ptr.assign(size_t(dm.nslice) + 1, 0);
+    dm.h_slice_w.assign(dm.nslice, 0);
+    for (int sl = 0; sl < dm.nslice; ++sl) {
+        int w = 0;
+        for (int r = 64 * sl; r < std::min(nn, 64 * sl + 64); ++r)
+            w = std::max(w, dm.h_row_ptr[r + 1] - dm.h_row_ptr[r]);
+        dm.h_slice_w[sl] = w;
+        dm.h_slice_ptr[sl + 1] = dm.h_slice_ptr[sl] + w;
+    }
+    dm.nsell = size_t(dm.h_slice_ptr[dm.nslice]) * 64;
+    {
+        std::vector<int> sc(dm.nsell, 0);
+        for (int r = 0; r < nn; ++r) {
+            const int len = dm.h_row_ptr[r + 1] - dm.h_row_ptr[r];
+            const int base = dm.h_slice_ptr[r / 64];
+            for (int q = 0; q < dm.h_slice_w[r / 64]; ++q)
+                sc[size_t(base + q) * 64 + (r % 64)] = q < len ? dm.h_cols[dm.h_row_ptr[r] + q] : r;
+        }
+        dm.sell_cols.upload(sc);
+    }
+    dm.slice_ptr.upload(dm.h_slice_ptr);
+    dm.slice_w.upload(dm.h_slice_w);
+
     // ---- node -> element incidences in ascending element order
     dm.h_inc_ptr.assign(size_t(nn) + 1, 0);
     for (size_t k = 0; k < size_t(ne) * 8; ++k) dm.h_inc_ptr[hm.conn[k] + 1]++;
@@ -227,6 +252,24 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
     dm.err.alloc(1);
     dm.err.zero();
     OAGPU_HIP(hipDeviceSynchronize());
+}
+
+static inline size_t sell_index(const DevMesh& dm, int row, int q, int nb, int k) {
+    return (size_t(dm.h_slice_ptr[row / 64] + q) * nb + k) * 64 + size_t(row % 64);
+}
+
+void sell_to_crs(const DevMesh& dm, int nb, const double* sell, double* crs) {
+    for (int r = 0; r < dm.nn; ++r)
+        for (int j = dm.h_row_ptr[r]; j < dm.h_row_ptr[r + 1]; ++j)
+            for (int k = 0; k < nb; ++k)
+                crs[size_t(j) * nb + k] = sell[sell_index(dm, r, j - dm.h_row_ptr[r], nb, k)];
+}
+
+void crs_to_sell(const DevMesh& dm, int nb, const double* crs, double* sell) {
+    for (int r = 0; r < dm.nn; ++r)
+        for (int j = dm.h_row_ptr[r]; j < dm.h_row_ptr[r + 1]; ++j)
+            for (int k = 0; k < nb; ++k)
+                sell[sell_index(dm, r, j - dm.h_row_ptr[r], nb, k)] = crs[size_t(j) * nb + k];
 }
 
 }  // namespace oagpu

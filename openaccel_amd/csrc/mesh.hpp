@@ -39,6 +39,16 @@ struct DevMesh {
     DBuf<double> xyz;          // [nn][3]
     DBuf<int> conn;            // [ne][8]
     DBuf<int> row_ptr, cols, diag_off;
+    // Values of the CRS matrix are stored sliced-ELL: slices of 64 rows (one
+    // wavefront), slot-major and lane-contiguous, so a wavefront's lanes (one
+    // row each) read and write consecutive addresses.  Component k of the
+    // N x N block in slot s of row r lives at
+    //   ((slice_ptr[r / 64] + s) * N * N + k) * 64 + r % 64.
+    // Slots beyond a row's length (up to its slice's width) hold the row's
+    // own column and value 0.
+    int nslice = 0;
+    size_t nsell = 0;          // slice_ptr[nslice] * 64
+    DBuf<int> slice_ptr, slice_w, sell_cols;
     DBuf<int> inc_ptr, inc;    // [nn+1], [ninc] element-ordered (set-up kernels)
     DBuf<int> inc_la;          // [8][nn]
     DBuf<uint64_t> slots_la;   // [8][nn]
@@ -54,10 +64,14 @@ struct DevMesh {
 
     // host copies kept for the bit-exact graph/scatter-map checks
     std::vector<int> h_row_ptr, h_cols, h_diag_off, h_inc_ptr, h_inc, h_binc_ptr, h_binc;
+    std::vector<int> h_slice_ptr, h_slice_w;
     std::vector<uint64_t> h_inc_slots, h_binc_slots;
 };
 
 void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm);
+// value layout conversions (host): CRS blocks of nb values <-> sliced-ELL
+void sell_to_crs(const DevMesh& dm, int nb, const double* sell, double* crs);
+void crs_to_sell(const DevMesh& dm, int nb, const double* crs, double* sell);
 void compute_mesh_geometry(DevMesh& dm, hipStream_t s);
 
 }  // namespace oagpu
