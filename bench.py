@@ -90,12 +90,25 @@ def main():
     ap.add_argument("--perturb", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=64, help="oracle sample grid (elements scale)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device once (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every GPU box here has a single MI355X and the "
-              "multi-GPU (halo exchange) path is a design, not a build; run with --gpus 1.",
+        print("bench.py measures one MI355X only: the multi-GPU path (element partition + "
+              "halo exchange) is designed but not implemented; run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        import torch
+        nb = args.copy_probe << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nb}))
+        return
 
     import openaccel_amd as oa
     from openaccel_amd import meshgen

@@ -62,6 +62,8 @@ typedef struct oagpu_flow_params {
     int pgrad_shifted;     /* pressure gradient interpolation linear_linear */
     int p_ref_node;        /* pressure-level node id, -1 for none           */
     int precision;         /* OAGPU_FP64 or OAGPU_MIXED                     */
+    int p_method;          /* p' solver of oagpu_outer_iteration: OAGPU_CG or
+                              OAGPU_CG_AMG                                  */
 } oagpu_flow_params;
 
 /* liblinsolve ControlData (linearSystem.h:475-482) as the device solve
@@ -77,6 +79,9 @@ typedef struct oagpu_control_data {
 #define OAGPU_GMRES 0
 #define OAGPU_CG 1
 #define OAGPU_BICGSTAB 2
+#define OAGPU_CG_AMG 3   /* CG preconditioned by one aggregation-AMG V-cycle
+                            (scalar systems; the role of HYPRE BoomerAMG in
+                            liblinsolve's hypre family, linearSystem.h:331) */
 
 /* ---------------------------------------------------------------- runtime */
 int oagpu_init(int device);
@@ -145,13 +150,24 @@ int oagpu_update_scales(oagpu_flow *f, double *U_scale, double *p_scale); /* nod
 int oagpu_scaled_residual(oagpu_flow *f, int N, double *out);
 
 /* Solve A x = b of system N (3 momentum, 1 pressure correction) from x = 0
- * with Jacobi-preconditioned Krylov `method`; converged when
- * ||r|| <= max(rtol ||r0||, atol) or after max_it iterations. */
+ * with Krylov `method` (Jacobi-preconditioned, or AMG-preconditioned CG);
+ * converged when ||r|| <= max(rtol ||r0||, atol) or after max_it iterations.
+ * Replaces linearSolver::Base<N>::solve behind linearSystem<N>::solveSystem_
+ * (linearSystem.h:416-462). */
 int oagpu_solve(oagpu_flow *f, int N, int method, int max_it, double rtol, double atol,
                 int restart, oagpu_control_data *cd);
+/* AMG hierarchy of the scalar system (built by the first OAGPU_CG_AMG solve):
+ * number of levels and rows per level (sizes[16]). */
+int oagpu_amg_info(oagpu_flow *f, int *n_levels, int *sizes);
+/* level >= 1: aggregate of every row of level-1 (agg[rows of level-1]), the
+ * level's CRS pattern and its values from the last set-up, in the precision
+ * of that set-up (NULL pointers are skipped). */
+int oagpu_amg_level_copy(oagpu_flow *f, int level, int *agg, int *row_ptr, int *cols,
+                         double *vals);
 
 /* One steady outer iteration of segregatedFlowEquations::solve.
- * ctl = {momentum max_it, momentum rtol, p' max_it, p' rtol};
+ * ctl = {momentum max_it, momentum rtol, p' max_it, p' rtol}; momentum by
+ * GMRES, p' by the flow's p_method;
  * hist[6] = U, V, W, P control residuals, momentum / p' Krylov iterations. */
 int oagpu_outer_iteration(oagpu_flow *f, const double *ctl, double *hist);
 

@@ -32,7 +32,7 @@ LIB_PATH = os.path.join(HERE, "liboagpu.so")
 
 OAGPU_FP64, OAGPU_MIXED = 0, 1
 OAGPU_STENCIL_FULL, OAGPU_STENCIL_REDUCED = 0, 1
-OAGPU_GMRES, OAGPU_CG, OAGPU_BICGSTAB = 0, 1, 2
+OAGPU_GMRES, OAGPU_CG, OAGPU_BICGSTAB, OAGPU_CG_AMG = 0, 1, 2, 3
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -44,7 +44,7 @@ class FlowParams(C.Structure):
                 ("grad_urf", C.c_double), ("bnd_relax", C.c_double),
                 ("u_shifted", C.c_int), ("ugrad_shifted", C.c_int),
                 ("p_shifted", C.c_int), ("pgrad_shifted", C.c_int),
-                ("p_ref_node", C.c_int), ("precision", C.c_int)]
+                ("p_ref_node", C.c_int), ("precision", C.c_int), ("p_method", C.c_int)]
 
 
 class ControlData(C.Structure):
@@ -88,6 +88,8 @@ _SIGS = {
     "oagpu_scaled_residual": [C.c_void_p, C.c_int, _dp],
     "oagpu_solve": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                     C.POINTER(ControlData)],
+    "oagpu_amg_info": [C.c_void_p, _ip, _ip],
+    "oagpu_amg_level_copy": [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp],
     "oagpu_outer_iteration": [C.c_void_p, _dp, _dp],
     "oagpu_last_kernel_ms": [C.c_void_p, C.c_char_p, _dp],
 }
@@ -135,7 +137,8 @@ def default_params(precision: int = OAGPU_FP64, **kw) -> FlowParams:
     (velocity interpolation trilinear, gradients linear_linear)."""
     p = FlowParams(rho=1.0, mu=0.01, dt=1.0, urf_U=0.9, urf_p=0.1, urf_mdot=1.0,
                    grad_urf=0.5625, bnd_relax=0.75, u_shifted=0, ugrad_shifted=1,
-                   p_shifted=1, pgrad_shifted=1, p_ref_node=0, precision=precision)
+                   p_shifted=1, pgrad_shifted=1, p_ref_node=0, precision=precision,
+                   p_method=OAGPU_CG)
     for k, v in kw.items():
         setattr(p, k, v)
     return p
@@ -276,6 +279,25 @@ class FlowCase:
         _check(load_library().oagpu_solve(self.h, N, method, max_it, rtol, atol, restart,
                                           C.byref(cd)))
         return cd
+
+    def amg_levels(self):
+        """Rows per level of the AMG hierarchy (empty before the first AMG solve)."""
+        n = C.c_int()
+        sizes = np.zeros(16, dtype=np.int32)
+        _check(load_library().oagpu_amg_info(self.h, C.byref(n), _i(sizes)))
+        return [int(v) for v in sizes[:n.value]]
+
+    def amg_level(self, level):
+        """(agg of the finer level's rows, row_ptr, cols, vals) of AMG level >= 1."""
+        sizes = self.amg_levels()
+        nf, nc = sizes[level - 1], sizes[level]
+        agg = np.zeros(nf, dtype=np.int32)
+        rp = np.zeros(nc + 1, dtype=np.int32)
+        _check(load_library().oagpu_amg_level_copy(self.h, level, _i(agg), _i(rp), None, None))
+        cols = np.zeros(int(rp[-1]), dtype=np.int32)
+        vals = np.zeros(int(rp[-1]))
+        _check(load_library().oagpu_amg_level_copy(self.h, level, None, None, _i(cols), _d(vals)))
+        return agg, rp, cols, vals
 
     def outer_iteration(self, ctl):
         c = np.ascontiguousarray(ctl, dtype=np.float64)

@@ -1,0 +1,487 @@
+// amg.hip -- aggregation algebraic multigrid preconditioner for the scalar
+// (pressure-correction) systems.
+//
+// The reference reaches algebraic multigrid through liblinsolve's HYPRE
+// family (BoomerAMG inside GMRES/CG, linearSystem.h:309-358); the library is
+// not part of the reference tree, so this is a design of its own, chosen for
+// the MI355X: plain (unsmoothed) aggregation (Vanek, Mandel, Brezina,
+// Computing 56 (1996), phase 1-3 greedy aggregation on the matrix graph) with
+// piecewise-constant prolongation, so that the Galerkin product P^T A P is a
+// plain sum of fine coefficients into their aggregate's row.
+//
+// * The hierarchy structure (aggregates, coarse patterns, the fine-slot ->
+//   coarse-slot map) depends only on the mesh graph and is built once, on the
+//   host.  Every solve recomputes the coarse values on the device from the
+//   current fine matrix: one lane per coarse row sums the slots of its member
+//   rows in a fixed order (no atomics, bitwise reproducible).
+// * Cycle: V(1,1) with damped Jacobi (omega = 2/3) and the coarse correction
+//   scaled by alpha = 2 (unsmoothed aggregation under-corrects), symmetric, so
+//   it preconditions CG.  The coarsest level (<= 64 rows) is solved with its
+//   dense inverse, computed per solve by Gauss-Jordan in one workgroup's LDS.
+// * Values and vectors of every level are of the Krylov type V: double for
+//   the FP64 form, float for the FP32 inner solve of the mixed form.
+#include "device_util.hpp"
+#include "flow.hpp"
+#include "sell.hpp"
+
+#include <numeric>
+#include <type_traits>
+
+namespace oagpu {
+
+constexpr int kDenseMax = 64;      // largest coarsest level inverted densely
+constexpr int kMaxLevels = 16;
+constexpr double kOmega = 2.0 / 3.0;
+constexpr double kAlpha = 2.0;
+
+struct SellView {
+    int n;
+    const int* slice_ptr;
+    const int* slice_w;
+    const int* cols;
+    const int* diag_off;
+};
+
+__device__ __forceinline__ size_t sv_base(const SellView& s, int i, int& w) {
+    const int sl = i >> 6;
+    w = s.slice_w[sl];
+    return size_t(s.slice_ptr[sl]) * 64 + (i & 63);
+}
+
+// --------------------------------------------------------------- set-up
+template <typename V>
+__global__ void __launch_bounds__(256) k_amg_galerkin(SellView c, const int* mem_ptr, const int* mem,
+                                                      SellView fl, const uint8_t* gmap, const V* Af,
+                                                      V* Ac) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= c.n) return;
+    int cw;
+    const size_t cb = sv_base(c, I, cw);
+    for (int q = 0; q < cw; ++q) Ac[cb + size_t(q) * 64] = V(0);
+    for (int k = mem_ptr[I]; k < mem_ptr[I + 1]; ++k) {
+        const int i = mem[k];
+        int fw;
+        const size_t fb = sv_base(fl, i, fw);
+        for (int q = 0; q < fw; ++q) {
+            const int g = gmap[fb + size_t(q) * 64];
+            if (g != 0xFF) Ac[cb + size_t(g) * 64] += Af[fb + size_t(q) * 64];
+        }
+    }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256) k_amg_dinv(SellView s, const V* A, V* dinv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    int w;
+    const size_t b = sv_base(s, i, w);
+    const V d = A[b + size_t(s.diag_off[i]) * 64];
+    dinv[i] = d != V(0) ? V(1) / d : V(0);
+}
+
+// In-place Gauss-Jordan inverse of the coarsest matrix (n <= kDenseMax) in
+// LDS, double arithmetic whatever V is.  A pivot that vanishes (a singular,
+// pure-Neumann coarsest level) drops its row and column from the inverse.
+template <typename V>
+__global__ void __launch_bounds__(256) k_amg_dense_inverse(SellView s, const V* A, double* inv,
+                                                           V* vinv) {
+    __shared__ double M[kDenseMax][kDenseMax + 1];
+    __shared__ double colk[kDenseMax];
+    __shared__ double piv;
+    __shared__ int skip;
+    const int n = s.n;
+    __shared__ double dmax;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) M[e / n][e % n] = 0.0;
+    __syncthreads();
+    if (threadIdx.x < n) {
+        const int i = threadIdx.x;
+        int w;
+        const size_t b = sv_base(s, i, w);
+        for (int q = 0; q < w; ++q) M[i][s.cols[b + size_t(q) * 64]] += double(A[b + size_t(q) * 64]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int i = 0; i < n; ++i) m = fmax(m, fabs(M[i][i]));
+        dmax = m;
+    }
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        if (threadIdx.x == 0) {
+            piv = M[k][k];
+            skip = !(fabs(piv) > 1.0e-13 * dmax);
+        }
+        __syncthreads();
+        if (skip) {
+            for (int j = threadIdx.x; j < n; j += blockDim.x) {
+                M[k][j] = 0.0;
+                M[j][k] = 0.0;
+            }
+            __syncthreads();
+            continue;
+        }
+        // row k of the inverse-in-progress, column k saved before elimination
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            colk[j] = M[j][k];
+            M[k][j] = (j == k ? 1.0 : M[k][j]) / piv;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+            const int i = e / n, j = e % n;
+            if (i != k) M[i][j] = (j == k ? 0.0 : M[i][j]) - colk[i] * M[k][j];
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        inv[e] = M[e / n][e % n];
+        vinv[e] = V(M[e / n][e % n]);
+    }
+}
+
+// ----------------------------------------------------------------- cycle
+// x = omega dinv .* r
+template <typename V>
+__global__ void __launch_bounds__(256) k_amg_jacobi0(int n, V omega, const V* dinv, const V* r, V* x,
+                                                     const int* flag) {
+    if (flag[0]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = omega * dinv[i] * r[i];
+}
+
+// res = r - A x
+template <typename V>
+__global__ void __launch_bounds__(256) k_amg_residual(SellView s, const V* A, const V* x, const V* r,
+                                                      V* res, const int* flag) {
+    if (flag[0]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    int w;
+    const size_t b = sv_base(s, i, w);
+    V acc = V(0);
+    for (int q = 0; q < w; ++q) acc += A[b + size_t(q) * 64] * x[s.cols[b + size_t(q) * 64]];
+    res[i] = r[i] - acc;
+}
+
+// rc[I] = sum of res over the members of I; xc = omega dinv_c rc (first
+// smoothing step of the coarse level from a zero guess; skipped when
+// dinv_c is null)
+template <typename V>
+__global__ void __launch_bounds__(256) k_amg_restrict(int nc, const int* mem_ptr, const int* mem,
+                                                      const V* res, V omega, const V* dinvc, V* rc,
+                                                      V* xc, const int* flag) {
+    if (flag[0]) return;
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= nc) return;
+    V s = V(0);
+    for (int k = mem_ptr[I]; k < mem_ptr[I + 1]; ++k) s += res[mem[k]];
+    rc[I] = s;
+    if (dinvc) xc[I] = omega * dinvc[I] * s;
+}
+
+// t = x + alpha P ec
+template <typename V>
+__global__ void __launch_bounds__(256) k_amg_prolong(int n, const int* agg, V alpha, const V* ec,
+                                                     const V* x, V* t, const int* flag) {
+    if (flag[0]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) t[i] = x[i] + alpha * ec[agg[i]];
+}
+
+// xo = t + omega dinv (r - A t)
+template <typename V>
+__global__ void __launch_bounds__(256) k_amg_smooth(SellView s, const V* A, const V* dinv, V omega,
+                                                    const V* r, const V* t, V* xo, const int* flag) {
+    if (flag[0]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    int w;
+    const size_t b = sv_base(s, i, w);
+    V acc = V(0);
+    for (int q = 0; q < w; ++q) acc += A[b + size_t(q) * 64] * t[s.cols[b + size_t(q) * 64]];
+    xo[i] = t[i] + omega * dinv[i] * (r[i] - acc);
+}
+
+// x = Minv r on the coarsest level (one workgroup)
+template <typename V>
+__global__ void __launch_bounds__(kDenseMax) k_amg_dense_apply(int n, const V* minv, const V* r, V* x,
+                                                               const int* flag) {
+    if (flag[0]) return;
+    const int i = threadIdx.x;
+    if (i >= n) return;
+    V acc = V(0);
+    for (int j = 0; j < n; ++j) acc += minv[size_t(i) * n + j] * r[j];
+    x[i] = acc;
+}
+
+// ------------------------------------------------------------ host side
+static SellView view_of(const DevMesh& m) {
+    return {m.nn, m.slice_ptr.p, m.slice_w.p, m.sell_cols.p, m.diag_off.p};
+}
+static SellView view_of(const AmgLevel& l) {
+    return {l.n, l.slice_ptr.p, l.slice_w.p, l.cols.p, l.diag_off.p};
+}
+
+// greedy aggregation on the graph of a CRS pattern (rows include their diagonal)
+static int aggregate(int n, const std::vector<int>& rp, const std::vector<int>& ci,
+                     std::vector<int>& agg) {
+    agg.assign(n, -1);
+    int na = 0;
+    // phase 1: a node whose whole neighbourhood is free roots a new aggregate
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        bool free = true;
+        for (int k = rp[i]; k < rp[i + 1] && free; ++k) free = agg[ci[k]] < 0;
+        if (!free) continue;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) agg[ci[k]] = na;
+        agg[i] = na;
+        ++na;
+    }
+    // phase 2: remaining nodes join a neighbouring phase-1 aggregate
+    std::vector<int> a2 = agg;
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        for (int k = rp[i]; k < rp[i + 1]; ++k)
+            if (agg[ci[k]] >= 0) {
+                a2[i] = agg[ci[k]];
+                break;
+            }
+    }
+    // phase 3: what is still free forms aggregates with its free neighbours
+    for (int i = 0; i < n; ++i) {
+        if (a2[i] >= 0) continue;
+        for (int k = rp[i]; k < rp[i + 1]; ++k)
+            if (a2[ci[k]] < 0) a2[ci[k]] = na;
+        a2[i] = na;
+        ++na;
+    }
+    agg.swap(a2);
+    return na;
+}
+
+template <typename T>
+static void alloc_vec(DBuf<T>& b, size_t n) {
+    if (b.n < n) b.alloc(n);
+}
+
+void amg_build(Flow& f) {
+    if (f.amg.built) return;
+    DevMesh& m = *f.m;
+    Amg& amg = f.amg;
+    amg.lv.clear();
+    amg.lv.reserve(kMaxLevels);
+    amg.lv.emplace_back();
+    {
+        AmgLevel& l0 = amg.lv[0];
+        l0.n = m.nn;
+        l0.h_row_ptr = m.h_row_ptr;
+        l0.h_cols = m.h_cols;
+    }
+    std::vector<int> fine_slice_ptr = m.h_slice_ptr, fine_slice_w = m.h_slice_w;
+    size_t fine_nsell = m.nsell;
+    while (int(amg.lv.size()) < kMaxLevels && amg.lv.back().n > kDenseMax) {
+        AmgLevel& fl = amg.lv.back();
+        const int nf = fl.n;
+        std::vector<int> agg;
+        const int nc = aggregate(nf, fl.h_row_ptr, fl.h_cols, agg);
+        if (nc >= nf || nc * 5 > nf * 4) break;  // coarsening has stalled
+        // members of each aggregate, ascending
+        std::vector<int> mem_ptr(size_t(nc) + 1, 0), mem(nf);
+        for (int i = 0; i < nf; ++i) ++mem_ptr[agg[i] + 1];
+        std::partial_sum(mem_ptr.begin(), mem_ptr.end(), mem_ptr.begin());
+        {
+            std::vector<int> pos(mem_ptr.begin(), mem_ptr.end() - 1);
+            for (int i = 0; i < nf; ++i) mem[pos[agg[i]]++] = i;
+        }
+        // coarse pattern: aggregates of the neighbours of the members, sorted
+        std::vector<int> crp(size_t(nc) + 1, 0), ccols;
+        std::vector<int> mark(nc, -1), row;
+        for (int I = 0; I < nc; ++I) {
+            row.clear();
+            for (int k = mem_ptr[I]; k < mem_ptr[I + 1]; ++k) {
+                const int i = mem[k];
+                for (int q = fl.h_row_ptr[i]; q < fl.h_row_ptr[i + 1]; ++q) {
+                    const int J = agg[fl.h_cols[q]];
+                    if (mark[J] != I) {
+                        mark[J] = I;
+                        row.push_back(J);
+                    }
+                }
+            }
+            std::sort(row.begin(), row.end());
+            if (row.size() > 255) break;
+            ccols.insert(ccols.end(), row.begin(), row.end());
+            crp[I + 1] = int(ccols.size());
+        }
+        if (int(ccols.size()) != crp[nc] || crp[nc] == 0) break;  // a row too long for 8-bit slots
+        // fine slot -> coarse slot
+        std::vector<uint8_t> gmap(fine_nsell, 0xFF);
+        for (int i = 0; i < nf; ++i) {
+            const int I = agg[i];
+            const int* cb = ccols.data() + crp[I];
+            const int clen = crp[I + 1] - crp[I];
+            for (int q = fl.h_row_ptr[i]; q < fl.h_row_ptr[i + 1]; ++q) {
+                const int J = agg[fl.h_cols[q]];
+                const int pos = int(std::lower_bound(cb, cb + clen, J) - cb);
+                gmap[sell_index(fine_slice_ptr, i, q - fl.h_row_ptr[i], 1, 0)] = uint8_t(pos);
+            }
+        }
+        AmgLevel cl;
+        cl.n = nc;
+        cl.h_row_ptr = crp;
+        cl.h_cols = ccols;
+        cl.h_agg = agg;
+        SellHost sh = make_sell(nc, crp, ccols);
+        cl.nslice = sh.nslice;
+        cl.nsell = sh.nsell;
+        std::vector<int> doff(nc);
+        for (int I = 0; I < nc; ++I) {
+            const int* b = ccols.data() + crp[I];
+            doff[I] = int(std::lower_bound(b, b + (crp[I + 1] - crp[I]), I) - b);
+        }
+        cl.slice_ptr.upload(sh.slice_ptr);
+        cl.slice_w.upload(sh.slice_w);
+        cl.cols.upload(sh.cols);
+        cl.diag_off.upload(doff);
+        cl.agg.upload(agg);
+        cl.mem_ptr.upload(mem_ptr);
+        cl.mem.upload(mem);
+        cl.gmap.upload(gmap);
+        cl.A.alloc(cl.nsell);
+        cl.fA.alloc(cl.nsell);
+        cl.A.zero(f.stream);
+        cl.fA.zero(f.stream);
+        for (auto* b : {&cl.dinv, &cl.X, &cl.T, &cl.R, &cl.S}) b->alloc(nc);
+        for (auto* b : {&cl.fdinv, &cl.fX, &cl.fT, &cl.fR, &cl.fS}) b->alloc(nc);
+        fine_slice_ptr = sh.slice_ptr;
+        fine_slice_w = sh.slice_w;
+        fine_nsell = sh.nsell;
+        amg.lv.push_back(std::move(cl));
+    }
+    // fine-level temporaries
+    AmgLevel& l0 = amg.lv[0];
+    for (auto* b : {&l0.T, &l0.S}) b->alloc(m.nn);
+    for (auto* b : {&l0.fT, &l0.fS}) b->alloc(m.nn);
+    const int nl = int(amg.lv.size());
+    amg.dense_n = nl > 1 && amg.lv[nl - 1].n <= kDenseMax ? amg.lv[nl - 1].n : 0;
+    if (amg.dense_n) {
+        amg.dense.alloc(size_t(amg.dense_n) * amg.dense_n);
+        amg.fdense.alloc(size_t(amg.dense_n) * amg.dense_n);
+    }
+    OAGPU_HIP(hipStreamSynchronize(f.stream));
+    amg.built = true;
+}
+
+template <typename V>
+static V* lvA(AmgLevel& l) {
+    if constexpr (std::is_same_v<V, float>) return l.fA.p;
+    else return l.A.p;
+}
+template <typename V>
+static V* lvv(AmgLevel& l, int which) {  // 0 dinv, 1 X, 2 T, 3 R, 4 S
+    if constexpr (std::is_same_v<V, float>) {
+        DBuf<float>* b[5] = {&l.fdinv, &l.fX, &l.fT, &l.fR, &l.fS};
+        return b[which]->p;
+    } else {
+        DBuf<double>* b[5] = {&l.dinv, &l.X, &l.T, &l.R, &l.S};
+        return b[which]->p;
+    }
+}
+
+static inline unsigned blocks(int n) { return unsigned((n + 255) / 256); }
+
+template <typename V>
+void amg_setup(Flow& f, const V* A0, const V* dinv0) {
+    (void)dinv0;
+    amg_build(f);
+    Amg& amg = f.amg;
+    amg.last_float = std::is_same_v<V, float>;
+    const int nl = int(amg.lv.size());
+    for (int l = 1; l < nl; ++l) {
+        AmgLevel& c = amg.lv[l];
+        const SellView fv = l == 1 ? view_of(*f.m) : view_of(amg.lv[l - 1]);
+        const V* Af = l == 1 ? A0 : lvA<V>(amg.lv[l - 1]);
+        k_amg_galerkin<V><<<blocks(c.n), 256, 0, f.stream>>>(view_of(c), c.mem_ptr.p, c.mem.p, fv,
+                                                             c.gmap.p, Af, lvA<V>(c));
+        k_amg_dinv<V><<<blocks(c.n), 256, 0, f.stream>>>(view_of(c), lvA<V>(c), lvv<V>(c, 0));
+    }
+    if (amg.dense_n) {
+        AmgLevel& c = amg.lv[nl - 1];
+        V* vinv;
+        if constexpr (std::is_same_v<V, float>) vinv = amg.fdense.p;
+        else vinv = amg.dense.p;
+        k_amg_dense_inverse<V><<<1, 256, 0, f.stream>>>(view_of(c), lvA<V>(c), amg.dense.p, vinv);
+    }
+    OAGPU_HIP(hipGetLastError());
+}
+
+template <typename V>
+void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z) {
+    Amg& amg = f.amg;
+    const int nl = int(amg.lv.size());
+    const int* flag = f.flag.p;
+    const V om = V(kOmega), al = V(kAlpha);
+    AmgLevel& l0 = amg.lv[0];
+    const SellView v0 = view_of(*f.m);
+    if (nl == 1) {  // no coarse level: two Jacobi steps
+        k_amg_jacobi0<V><<<blocks(l0.n), 256, 0, f.stream>>>(l0.n, om, dinv0, r, lvv<V>(l0, 2), flag);
+        k_amg_smooth<V><<<blocks(l0.n), 256, 0, f.stream>>>(v0, A0, dinv0, om, r, lvv<V>(l0, 2), z,
+                                                            flag);
+        return;
+    }
+    // downward: level l has rhs R_l (r at l = 0), pre-smoothed X_l (z at l = 0)
+    auto R = [&](int l) -> const V* { return l == 0 ? r : lvv<V>(amg.lv[l], 3); };
+    auto X = [&](int l) -> V* { return l == 0 ? z : lvv<V>(amg.lv[l], 1); };
+    auto A = [&](int l) -> const V* { return l == 0 ? A0 : lvA<V>(amg.lv[l]); };
+    auto D = [&](int l) -> const V* { return l == 0 ? dinv0 : lvv<V>(amg.lv[l], 0); };
+    auto view = [&](int l) { return l == 0 ? v0 : view_of(amg.lv[l]); };
+    const int last = nl - 1;
+    k_amg_jacobi0<V><<<blocks(l0.n), 256, 0, f.stream>>>(l0.n, om, dinv0, r, z, flag);
+    for (int l = 0; l < last; ++l) {
+        AmgLevel& fl = amg.lv[l];
+        AmgLevel& cl = amg.lv[l + 1];
+        V* S = lvv<V>(fl, 4);
+        k_amg_residual<V><<<blocks(fl.n), 256, 0, f.stream>>>(view(l), A(l), X(l), R(l), S, flag);
+        const bool coarsest = l + 1 == last;
+        k_amg_restrict<V><<<blocks(cl.n), 256, 0, f.stream>>>(
+            cl.n, cl.mem_ptr.p, cl.mem.p, S, om, coarsest ? nullptr : D(l + 1), lvv<V>(cl, 3),
+            X(l + 1), flag);
+    }
+    {
+        AmgLevel& c = amg.lv[last];
+        if (amg.dense_n) {
+            const V* inv;
+            if constexpr (std::is_same_v<V, float>) inv = amg.fdense.p;
+            else inv = amg.dense.p;
+            k_amg_dense_apply<V><<<1, kDenseMax, 0, f.stream>>>(c.n, inv, R(last), X(last), flag);
+        } else {  // coarsening stalled above the dense size: Jacobi steps
+            k_amg_jacobi0<V><<<blocks(c.n), 256, 0, f.stream>>>(c.n, om, D(last), R(last), X(last),
+                                                                flag);
+            for (int it = 0; it < 4; ++it) {
+                k_amg_smooth<V><<<blocks(c.n), 256, 0, f.stream>>>(view(last), A(last), D(last), om,
+                                                                   R(last), X(last), lvv<V>(c, 2),
+                                                                   flag);
+                k_amg_smooth<V><<<blocks(c.n), 256, 0, f.stream>>>(view(last), A(last), D(last), om,
+                                                                   R(last), lvv<V>(c, 2), X(last),
+                                                                   flag);
+            }
+        }
+    }
+    // upward: T_l = X_l + alpha P X_{l+1}; X_l = T_l + omega D^-1 (R_l - A T_l)
+    for (int l = last - 1; l >= 0; --l) {
+        AmgLevel& fl = amg.lv[l];
+        AmgLevel& cl = amg.lv[l + 1];
+        V* T = lvv<V>(fl, 2);
+        k_amg_prolong<V><<<blocks(fl.n), 256, 0, f.stream>>>(fl.n, cl.agg.p, al, X(l + 1), X(l), T,
+                                                             flag);
+        k_amg_smooth<V><<<blocks(fl.n), 256, 0, f.stream>>>(view(l), A(l), D(l), om, R(l), T, X(l),
+                                                            flag);
+    }
+    OAGPU_HIP(hipGetLastError());
+}
+
+template void amg_setup<double>(Flow&, const double*, const double*);
+template void amg_setup<float>(Flow&, const float*, const float*);
+template void amg_vcycle<double>(Flow&, const double*, const double*, const double*, double*);
+template void amg_vcycle<float>(Flow&, const float*, const float*, const float*, float*);
+
+}  // namespace oagpu

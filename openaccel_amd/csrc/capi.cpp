@@ -8,6 +8,7 @@
 #include <string>
 
 #include "flow.hpp"
+#include "sell.hpp"
 
 struct oagpu_mesh {
     oagpu::DevMesh m;
@@ -426,6 +427,56 @@ int oagpu_solve(oagpu_flow* h, int N, int method, int max_it, double rtol, doubl
     });
 }
 
+int oagpu_amg_info(oagpu_flow* h, int* n_levels, int* sizes) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && n_levels, "null argument");
+        const oagpu::Amg& a = h->f.amg;
+        *n_levels = a.built ? int(a.lv.size()) : 0;
+        if (sizes)
+            for (int l = 0; l < *n_levels && l < 16; ++l) sizes[l] = a.lv[l].n;
+    });
+}
+
+int oagpu_amg_level_copy(oagpu_flow* h, int level, int* agg, int* row_ptr, int* cols,
+                         double* vals) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h, "null flow");
+        oagpu::Amg& a = h->f.amg;
+        OAGPU_REQUIRE(a.built && level >= 1 && level < int(a.lv.size()), "no such AMG level");
+        oagpu::AmgLevel& l = a.lv[level];
+        if (agg) std::copy(l.h_agg.begin(), l.h_agg.end(), agg);
+        if (row_ptr) std::copy(l.h_row_ptr.begin(), l.h_row_ptr.end(), row_ptr);
+        if (cols) std::copy(l.h_cols.begin(), l.h_cols.end(), cols);
+        if (vals) {
+            std::vector<double> sv(l.nsell);
+            if (a.last_float) {
+                std::vector<float> fv(l.nsell);
+                l.fA.download(fv.data(), fv.size(), h->f.stream);
+                OAGPU_HIP(hipStreamSynchronize(h->f.stream));
+                std::copy(fv.begin(), fv.end(), sv.begin());
+            } else {
+                l.A.download(sv.data(), sv.size(), h->f.stream);
+                OAGPU_HIP(hipStreamSynchronize(h->f.stream));
+            }
+            std::vector<int> sp;
+            {
+                // slice starts of the level's sliced-ELL layout
+                const int nsl = (l.n + 63) / 64;
+                sp.assign(size_t(nsl) + 1, 0);
+                for (int s2 = 0; s2 < nsl; ++s2) {
+                    int w = 0;
+                    for (int r = 64 * s2; r < std::min(l.n, 64 * s2 + 64); ++r)
+                        w = std::max(w, l.h_row_ptr[r + 1] - l.h_row_ptr[r]);
+                    sp[s2 + 1] = sp[s2] + w;
+                }
+            }
+            for (int r = 0; r < l.n; ++r)
+                for (int j = l.h_row_ptr[r]; j < l.h_row_ptr[r + 1]; ++j)
+                    vals[j] = sv[oagpu::sell_index(sp, r, j - l.h_row_ptr[r], 1, 0)];
+        }
+    });
+}
+
 // simulation::assembleAndSolveSystems -> segregatedFlowEquations::solve
 // (segregatedFlowEquations.cpp:76-297) for one p' sub-iteration
 int oagpu_outer_iteration(oagpu_flow* h, const double* ctl, double* hist) {
@@ -446,7 +497,7 @@ int oagpu_outer_iteration(oagpu_flow* h, const double* ctl, double* hist) {
         // pressure correction (pressureCorrectionEquation::solve :89-164)
         f.ms_pcorr = timed(f, [&] { oagpu::launch_assemble_pcorr(f); });
         oagpu::scaled_residual(f, 1, rP);
-        oagpu::krylov_solve(f, 1, OAGPU_CG, int(ctl[2]), ctl[3], 1e-30, 30, &cdP);
+        oagpu::krylov_solve(f, 1, f.prm.p_method, int(ctl[2]), ctl[3], 1e-30, 30, &cdP);
         oagpu::launch_correct(f, 1, f.prm.urf_p);
         // mass flow, divergence, velocity correction, gradients (:106-285)
         oagpu::launch_update_mdot(f);

@@ -6,8 +6,32 @@
 
 namespace oagpu {
 
-constexpr int kRedGrid = 512;   // fixed partition of every reduction: deterministic sums
+constexpr int kRedGrid = 2048;  // fixed partition of every reduction: deterministic sums
 constexpr int kRedBlock = 256;
+
+// One coarse level of the aggregation AMG hierarchy (values sliced-ELL like
+// the fine matrix).  Level l's rows are aggregates of level l-1's rows.
+struct AmgLevel {
+    int n = 0;
+    int nslice = 0;
+    size_t nsell = 0;
+    DBuf<int> slice_ptr, slice_w, cols, diag_off;
+    std::vector<int> h_row_ptr, h_cols, h_agg;  // host pattern and aggregation
+    DBuf<int> agg;           // [rows of level l-1] -> row of level l
+    DBuf<int> mem_ptr, mem;  // rows of level l-1 aggregated into each row (ascending)
+    DBuf<uint8_t> gmap;      // [level l-1 sliced-ELL slots] -> slot of the receiving row here
+    DBuf<double> A, dinv, X, T, R, S;
+    DBuf<float> fA, fdinv, fX, fT, fR, fS;
+};
+
+struct Amg {
+    bool built = false;
+    bool last_float = false;   // precision of the last numeric set-up
+    std::vector<AmgLevel> lv;  // lv[0]: fine level (pattern of the mesh graph; vectors only)
+    int dense_n = 0;           // coarsest level solved by its dense inverse when > 0
+    DBuf<double> dense;        // [dense_n][dense_n] inverse of the coarsest matrix
+    DBuf<float> fdense;
+};
 
 struct Flow {
     DevMesh* m = nullptr;
@@ -31,6 +55,7 @@ struct Flow {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double ms_momentum = 0, ms_pcorr = 0, ms_mdot = 0, ms_grad = 0;
+    Amg amg;
 };
 
 // kernels.hip
@@ -47,5 +72,11 @@ void field_scales(Flow& f, double* U_scale, double* p_scale);
 void scaled_residual(Flow& f, int N, double* out);
 int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double atol, int restart,
                  oagpu_control_data* cd);
+// amg.hip
+void amg_build(Flow& f);                       // hierarchy structure (once per mesh)
+template <typename V>
+void amg_setup(Flow& f, const V* A0, const V* dinv0);  // numeric Galerkin levels for A0
+template <typename V>
+void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z);  // z = M^-1 r
 
 }  // namespace oagpu

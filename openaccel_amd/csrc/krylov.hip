@@ -262,19 +262,24 @@ static int gmres(Flow& f, int max_it, double rtol, double atol, int mres, oagpu_
 // Vectors and matrix values of type V (double, or float for the inner solve
 // of the mixed form); dot products are always accumulated in double.
 // scal: [0..1] rz ping-pong, [2] tol^2, [3] last rr, [4] rr0
-template <typename V>
+// PRE: the preconditioner is applied by separate launches (AMG); the kernel
+// then leaves z, p and r.z to k_cg_pz.
+template <typename V, bool PRE>
 __global__ void __launch_bounds__(kRedBlock) k_cg_init(size_t n, const V* b, const V* dinv, V* x,
                                                        V* r, V* z, V* p, double* part) {
     __shared__ double sh[kRedBlock];
     double rz = 0.0, rr = 0.0;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
          i += size_t(gridDim.x) * blockDim.x) {
-        const V ri = b[i], zi = dinv[i] * ri;
+        const V ri = b[i];
         x[i] = V(0);
         r[i] = ri;
-        z[i] = zi;
-        p[i] = zi;
-        rz += double(ri) * double(zi);
+        if (!PRE) {
+            const V zi = dinv[i] * ri;
+            z[i] = zi;
+            p[i] = zi;
+            rz += double(ri) * double(zi);
+        }
         rr += double(ri) * double(ri);
     }
     const double s0 = block_sum(rz, sh);
@@ -321,7 +326,7 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_spmv(int nn, const int* slice_
     if (threadIdx.x == 0) part[2 * kRedGrid + blockIdx.x] = s;
 }
 
-template <typename V>
+template <typename V, bool PRE>
 __global__ void __launch_bounds__(kRedBlock) k_cg_update1(size_t n, int it, const V* p, const V* q,
                                                           const V* dinv, V* x, V* r, V* z,
                                                           double* part, const double* scal,
@@ -336,9 +341,11 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_update1(size_t n, int it, cons
         x[i] += alpha * p[i];
         const V ri = r[i] - alpha * q[i];
         r[i] = ri;
-        const V zi = dinv[i] * ri;
-        z[i] = zi;
-        rz += double(ri) * double(zi);
+        if (!PRE) {
+            const V zi = dinv[i] * ri;
+            z[i] = zi;
+            rz += double(ri) * double(zi);
+        }
         rr += double(ri) * double(ri);
     }
     const double s0 = block_sum(rz, sh);
@@ -347,6 +354,27 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_update1(size_t n, int it, cons
         part[3 * kRedGrid + blockIdx.x] = s0;
         part[4 * kRedGrid + blockIdx.x] = s1;
     }
+}
+
+__global__ void k_clear_flag(int* flag) { flag[0] = 0; }
+
+// r.z partials into slot `slot` after an external preconditioner; with
+// init set, also p = z (first iteration)
+template <typename V>
+__global__ void __launch_bounds__(kRedBlock) k_cg_pz(size_t n, const V* r, const V* z, V* p,
+                                                     int init, int slot, double* part,
+                                                     const int* flag) {
+    __shared__ double sh[kRedBlock];
+    if (flag[0]) return;
+    double rz = 0.0;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x) {
+        const V zi = z[i];
+        if (init) p[i] = zi;
+        rz += double(r[i]) * double(zi);
+    }
+    const double s0 = block_sum(rz, sh);
+    if (threadIdx.x == 0) part[size_t(slot) * kRedGrid + blockIdx.x] = s0;
 }
 
 template <typename V>
@@ -373,13 +401,25 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_update2(size_t n, int it, cons
 }
 
 // Device-resident PCG on A x = b (values V); returns iterations, fills
-// initial / final ||r||_2 in V arithmetic.
+// initial / final ||r||_2 in V arithmetic.  amg: preconditioner = one AMG
+// V-cycle (N = 1), else Jacobi.
 template <int N, typename V>
 static int cg_core(Flow& f, const V* A, const V* b, const V* dinv, V* x, V* r, V* z, V* p, V* q,
-                   int max_it, double rtol, double atol, double* r0, double* r1) {
+                   int max_it, double rtol, double atol, double* r0, double* r1, bool amg = false) {
     DevMesh& m = *f.m;
     const size_t n = size_t(m.nn) * N;
-    k_cg_init<V><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, b, dinv, x, r, z, p, f.part.p);
+    k_clear_flag<<<1, 1, 0, f.stream>>>(f.flag.p);  // a kernel, not a memset: capturable
+    if (amg) {
+        if constexpr (N == 1) {
+            k_cg_init<V, true><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, b, dinv, x, r, z, p, f.part.p);
+            amg_vcycle<V>(f, A, dinv, r, z);
+            k_cg_pz<V><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, r, z, p, 1, 0, f.part.p, f.flag.p);
+        } else {
+            throw Error(OAGPU_EINVAL, "AMG preconditioning is for scalar systems");
+        }
+    } else {
+        k_cg_init<V, false><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, b, dinv, x, r, z, p, f.part.p);
+    }
     k_cg_start<<<1, kRedBlock, 0, f.stream>>>(f.part.p, rtol, atol, f.scal.p, f.flag.p);
     OAGPU_HIP(hipGetLastError());
     const int poll = 8;
@@ -387,8 +427,18 @@ static int cg_core(Flow& f, const V* A, const V* b, const V* dinv, V* x, V* r, V
     for (int it = 0; it < max_it; ++it) {
         k_cg_spmv<N, V><<<kRedGrid, kRedBlock, 0, f.stream>>>(
             m.nn, m.slice_ptr.p, m.slice_w.p, m.sell_cols.p, A, p, q, f.part.p, f.flag.p);
-        k_cg_update1<V><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, it, p, q, dinv, x, r, z, f.part.p,
-                                                              f.scal.p, f.flag.p);
+        if (amg) {
+            if constexpr (N == 1) {
+                k_cg_update1<V, true><<<kRedGrid, kRedBlock, 0, f.stream>>>(
+                    n, it, p, q, dinv, x, r, z, f.part.p, f.scal.p, f.flag.p);
+                amg_vcycle<V>(f, A, dinv, r, z);
+                k_cg_pz<V><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, r, z, p, 0, 3, f.part.p,
+                                                                  f.flag.p);
+            }
+        } else {
+            k_cg_update1<V, false><<<kRedGrid, kRedBlock, 0, f.stream>>>(
+                n, it, p, q, dinv, x, r, z, f.part.p, f.scal.p, f.flag.p);
+        }
         k_cg_update2<V><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, it, z, p, f.part.p, f.scal.p,
                                                               f.flag.p);
         if ((it + 1) % poll == 0 || it + 1 == max_it) {
@@ -408,12 +458,13 @@ static int cg_core(Flow& f, const V* A, const V* b, const V* dinv, V* x, V* r, V
 }
 
 template <int N>
-static int cg(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
+static int cg(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd, bool amg) {
     const double* A = N == 3 ? f.A3.p : f.A1.p;
     const double* b = N == 3 ? f.b3.p : f.b1.p;
     double* x = N == 3 ? f.x3.p : f.x1.p;
+    if (amg) amg_setup<double>(f, A, f.dinv.p);
     return cg_core<N, double>(f, A, b, f.dinv.p, x, f.r.p, f.z.p, f.pk.p, f.q.p, max_it, rtol, atol,
-                              &cd->solver_initial_res, &cd->solver_final_res);
+                              &cd->solver_initial_res, &cd->solver_final_res, amg);
 }
 
 // ---------------------------------------------- mixed: FP32 CG + FP64 refinement
@@ -431,7 +482,7 @@ __global__ void k_refine(size_t n, double s, const float* d, double* x) {
 // x += ||r|| d in FP64, until ||r|| <= max(rtol ||b||, atol) or the total
 // inner iteration budget max_it is spent.  The FP32 matrix is A rounded.
 template <int N>
-static int cg_mixed(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
+static int cg_mixed(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd, bool amg) {
     DevMesh& m = *f.m;
     const size_t n = size_t(m.nn) * N, nv = m.nsell * N * N;
     const double* A = N == 3 ? f.A3.p : f.A1.p;
@@ -443,6 +494,7 @@ static int cg_mixed(Flow& f, int max_it, double rtol, double atol, oagpu_control
     const unsigned B = 256;
     k_to_float<<<unsigned((nv + B - 1) / B), B, 0, f.stream>>>(nv, 1.0, A, f.Af.p);
     k_to_float<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, 1.0, f.dinv.p, f.fdinv.p);
+    if (amg) amg_setup<float>(f, f.Af.p, f.fdinv.p);
     OAGPU_HIP(hipMemsetAsync(x, 0, n * sizeof(double), f.stream));
     OAGPU_HIP(hipMemcpyAsync(f.r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, f.stream));
     double rn;
@@ -461,7 +513,7 @@ static int cg_mixed(Flow& f, int max_it, double rtol, double atol, oagpu_control
         k_to_float<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, 1.0 / rn, f.r.p, f.fb.p);
         double i0, i1;
         const int it = cg_core<N, float>(f, f.Af.p, f.fb.p, f.fdinv.p, f.fx.p, f.fr.p, f.fz.p, f.fp.p,
-                                         f.fq.p, max_it - used, inner, 0.0, &i0, &i1);
+                                         f.fq.p, max_it - used, inner, 0.0, &i0, &i1, amg);
         used += std::max(it, 1);
         k_refine<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, rn, f.fx.p, x);
         spmv<N>(f, A, x, nullptr, f.w.p);
@@ -571,10 +623,15 @@ int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double ato
     if (method == OAGPU_GMRES)
         it = N == 3 ? gmres<3>(f, max_it, rtol, atol, restart, cd)
                     : gmres<1>(f, max_it, rtol, atol, restart, cd);
-    else if (method == OAGPU_CG && f.prm.precision == OAGPU_MIXED)
-        it = N == 3 ? cg_mixed<3>(f, max_it, rtol, atol, cd) : cg_mixed<1>(f, max_it, rtol, atol, cd);
-    else if (method == OAGPU_CG)
-        it = N == 3 ? cg<3>(f, max_it, rtol, atol, cd) : cg<1>(f, max_it, rtol, atol, cd);
+    else if (method == OAGPU_CG || method == OAGPU_CG_AMG) {
+        const bool amg = method == OAGPU_CG_AMG;
+        OAGPU_REQUIRE(!amg || N == 1, "AMG preconditioning is for scalar systems (N = 1)");
+        if (f.prm.precision == OAGPU_MIXED)
+            it = N == 3 ? cg_mixed<3>(f, max_it, rtol, atol, cd, false)
+                        : cg_mixed<1>(f, max_it, rtol, atol, cd, amg);
+        else
+            it = N == 3 ? cg<3>(f, max_it, rtol, atol, cd, false) : cg<1>(f, max_it, rtol, atol, cd, amg);
+    }
     else if (method == OAGPU_BICGSTAB)
         it = N == 3 ? bicgstab<3>(f, max_it, rtol, atol, cd) : bicgstab<1>(f, max_it, rtol, atol, cd);
     else

@@ -9,6 +9,7 @@
 // element column in a row and its lumping of absent columns into the diagonal
 // block (src/assemble/assembler.h:380-445).
 #include "mesh.hpp"
+#include "sell.hpp"
 
 #include <algorithm>
 #include <numeric>
@@ -113,29 +114,16 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
     std::vector<int>().swap(cand);
 
     // ---- sliced-ELL layout of the values (64-row slices)
-    dm.nslice = (nn + 63) / 64;
-    dm.h_slice_ptr.assign(size_t(dm.nslice) + 1, 0);
-    dm.h_slice_w.assign(dm.nslice, 0);
-    for (int sl = 0; sl < dm.nslice; ++sl) {
-        int w = 0;
-        for (int r = 64 * sl; r < std::min(nn, 64 * sl + 64); ++r)
-            w = std::max(w, dm.h_row_ptr[r + 1] - dm.h_row_ptr[r]);
-        dm.h_slice_w[sl] = w;
-        dm.h_slice_ptr[sl + 1] = dm.h_slice_ptr[sl] + w;
-    }
-    dm.nsell = size_t(dm.h_slice_ptr[dm.nslice]) * 64;
     {
-        std::vector<int> sc(dm.nsell, 0);
-        for (int r = 0; r < nn; ++r) {
-            const int len = dm.h_row_ptr[r + 1] - dm.h_row_ptr[r];
-            const int base = dm.h_slice_ptr[r / 64];
-            for (int q = 0; q < dm.h_slice_w[r / 64]; ++q)
-                sc[size_t(base + q) * 64 + (r % 64)] = q < len ? dm.h_cols[dm.h_row_ptr[r] + q] : r;
-        }
-        dm.sell_cols.upload(sc);
+        SellHost sh = make_sell(nn, dm.h_row_ptr, dm.h_cols);
+        dm.nslice = sh.nslice;
+        dm.nsell = sh.nsell;
+        dm.h_slice_ptr = std::move(sh.slice_ptr);
+        dm.h_slice_w = std::move(sh.slice_w);
+        dm.sell_cols.upload(sh.cols);
+        dm.slice_ptr.upload(dm.h_slice_ptr);
+        dm.slice_w.upload(dm.h_slice_w);
     }
-    dm.slice_ptr.upload(dm.h_slice_ptr);
-    dm.slice_w.upload(dm.h_slice_w);
 
     // ---- node -> element incidences in ascending element order
     dm.h_inc_ptr.assign(size_t(nn) + 1, 0);
@@ -254,22 +242,18 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
     OAGPU_HIP(hipDeviceSynchronize());
 }
 
-static inline size_t sell_index(const DevMesh& dm, int row, int q, int nb, int k) {
-    return (size_t(dm.h_slice_ptr[row / 64] + q) * nb + k) * 64 + size_t(row % 64);
-}
-
 void sell_to_crs(const DevMesh& dm, int nb, const double* sell, double* crs) {
     for (int r = 0; r < dm.nn; ++r)
         for (int j = dm.h_row_ptr[r]; j < dm.h_row_ptr[r + 1]; ++j)
             for (int k = 0; k < nb; ++k)
-                crs[size_t(j) * nb + k] = sell[sell_index(dm, r, j - dm.h_row_ptr[r], nb, k)];
+                crs[size_t(j) * nb + k] = sell[sell_index(dm.h_slice_ptr, r, j - dm.h_row_ptr[r], nb, k)];
 }
 
 void crs_to_sell(const DevMesh& dm, int nb, const double* crs, double* sell) {
     for (int r = 0; r < dm.nn; ++r)
         for (int j = dm.h_row_ptr[r]; j < dm.h_row_ptr[r + 1]; ++j)
             for (int k = 0; k < nb; ++k)
-                sell[sell_index(dm, r, j - dm.h_row_ptr[r], nb, k)] = crs[size_t(j) * nb + k];
+                sell[sell_index(dm.h_slice_ptr, r, j - dm.h_row_ptr[r], nb, k)] = crs[size_t(j) * nb + k];
 }
 
 }  // namespace oagpu

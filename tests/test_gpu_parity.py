@@ -1,7 +1,7 @@
 """GPU parity of the HIP path (liboagpu.so through its C ABI) against the
 oracle on identical seeded inputs.
 
-Bars (north star):
+Bars:
 * CRS graph, column indices, diagonal offsets and element -> CRS scatter
   maps: bit-exact;
 * FP64 form: matrix coefficients, rhs and field updates within 1e-12

@@ -1,6 +1,6 @@
 """Pins the oracle's graph, scatter, assembly and field-update restatements
 with properties the mathematics fixes (the reference ships no tests or golden
-vectors for these, SURVEY 8c):
+vectors for these):
 
 * CRS graph: closed-form nnz of the reduced (7-point) and full (27-point)
   stencils on n^3 grids, sorted columns, diagonal present;
