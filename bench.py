@@ -3,8 +3,8 @@ one MI355X, mixed form vs FP64 form, with assembly kernel roofline figures
 and the oracle's CPU time on the same workload.
 
 A "step" is one outer iteration (momentum assembly + GMRES solve, mass flow
-update, p' assembly + CG solve, corrections, gradients) over the whole mesh,
-inputs resident in HBM.  One JSON line is printed.
+update, p' assembly + AMG-preconditioned CG solve, corrections, gradients)
+over the whole mesh, inputs resident in HBM.  One JSON line is printed.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--n 128] [--perturb 0]
 """
@@ -34,20 +34,41 @@ def algorithmic_bytes(nn, ne, nnz):
     pc = nn * (24 + 24 + 8 + 24 + 24 + 8) + ne * (conn + mdot + slots) + nnz * 8 + nn * 8
     md = nn * 112 + ne * (conn + 16 * 12)
     gu = nn * (24 + 24 + 8 + 72) + ne * conn
-    return {"momentum": mom, "pcorr": pc, "mdot": md, "gradU": gu}
+    # fine-level AMG kernels of the FP32 V-cycle (mixed form): sliced-ELL
+    # values + column indices of the true nonzeros, row vectors
+    amg_pre = nnz * (4 + 4) + nn * 4 * 4           # r, dinv in; x, residual out
+    amg_post = nnz * (4 + 4) + nn * (4 * 5 + 4)    # x, agg, r, dinv, coarse e in; z out
+    return {"momentum": mom, "pcorr": pc, "mdot": md, "gradU": gu, "amg_pre": amg_pre,
+            "amg_post": amg_post}
 
 
-def run_gpu(oa, mesh, precision, steps, warmup):
+def measured_traffic(kernel, n, precision):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc FETCH_SIZE
+    and WRITE_SIZE on this workload (profiles/traffic_<kernel>_<n>.json, with
+    the copy-kernel calibration factors applied), or None if not measured."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{n}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    rec = t.get("fp64" if precision == 0 else "mixed")
+    if not rec:
+        return None
+    return int(rec["fetch_kib"] * 1024 * t["fetch_scale"] + rec["write_kib"] * 1024 * t["write_scale"])
+
+
+def run_gpu(oa, mesh, precision, steps, warmup, p_method):
     dm = oa.DeviceMesh(mesh)
-    f = oa.FlowCase(dm, oa.default_params(precision), BC)
+    f = oa.FlowCase(dm, oa.default_params(precision, p_method=p_method), BC)
     from openaccel_amd import meshgen
     f.upload("U", meshgen.initial_velocity(dm.nn, 0))
     f.initialize()
     for _ in range(warmup):
         f.outer_iteration(CTL)
     oa.load_library().oagpu_synchronize()
-    ms = {k: 0.0 for k in ("momentum", "pcorr", "mdot", "gradU")}
+    ms = {k: 0.0 for k in ("momentum", "pcorr", "mdot", "gradU", "amg_pre", "amg_post")}
     hist = []
+    v0 = f.kernel_ms("amg_vcycles")
     t0 = time.perf_counter()
     for _ in range(steps):
         hist.append(f.outer_iteration(CTL))
@@ -57,8 +78,12 @@ def run_gpu(oa, mesh, precision, steps, warmup):
     t = time.perf_counter() - t0
     for k in ms:
         ms[k] /= steps
-    return {"seconds": t, "kernel_ms": ms, "hist": np.array(hist), "nn": dm.nn, "ne": dm.ne,
-            "nnz": dm.nnz}
+    # launches per step: mdot is updated twice per outer iteration, the
+    # fine-level AMG kernels once per V-cycle
+    vps = (f.kernel_ms("amg_vcycles") - v0) / steps
+    per_step = {"momentum": 1, "pcorr": 1, "mdot": 2, "gradU": 1, "amg_pre": vps, "amg_post": vps}
+    return {"seconds": t, "kernel_ms": ms, "launches": per_step, "hist": np.array(hist),
+            "nn": dm.nn, "ne": dm.ne, "nnz": dm.nnz}
 
 
 def cpu_baseline(n_sample, perturb):
@@ -116,10 +141,10 @@ def main():
     mesh = meshgen.cavity_hex(args.n, perturb=args.perturb, seed=1)
     print(f"[bench] mesh {args.n}^3 perturb={args.perturb}: {mesh['conn'].shape[0]} elements",
           file=sys.stderr, flush=True)
-    r64 = run_gpu(oa, mesh, oa.OAGPU_FP64, args.steps, args.warmup)
+    r64 = run_gpu(oa, mesh, oa.OAGPU_FP64, args.steps, args.warmup, oa.OAGPU_CG_AMG)
     print(f"[bench] fp64 {args.steps / r64['seconds']:.3f} it/s {r64['kernel_ms']}", file=sys.stderr,
           flush=True)
-    rmx = run_gpu(oa, mesh, oa.OAGPU_MIXED, args.steps, args.warmup)
+    rmx = run_gpu(oa, mesh, oa.OAGPU_MIXED, args.steps, args.warmup, oa.OAGPU_CG_AMG)
     print(f"[bench] mixed {args.steps / rmx['seconds']:.3f} it/s {rmx['kernel_ms']}", file=sys.stderr,
           flush=True)
     nn, ne, nnz = rmx["nn"], rmx["ne"], rmx["nnz"]
@@ -129,13 +154,16 @@ def main():
         out = {}
         for k, ms in r["kernel_ms"].items():
             gbs = ab[k] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-            out[k] = {"ms": round(ms, 4), "Melem_s": round(ne / (ms * 1e-3) / 1e6, 1) if ms > 0 else 0,
-                      "GB_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-                      "bytes": int(ab[k])}
+            out[k] = {"ms": round(ms, 4), "GB_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                      "bytes": int(ab[k]), "launches_per_step": round(r["launches"][k], 2),
+                      "ms_per_step": round(ms * r["launches"][k], 3)}
+            if not k.startswith("amg"):
+                out[k]["Melem_s"] = round(ne / (ms * 1e-3) / 1e6, 1) if ms > 0 else 0
         return out
 
     k64, kmx = kern(r64), kern(rmx)
-    dom = max(kmx, key=lambda k: kmx[k]["ms"])
+    # the dominant kernel: largest device time per outer iteration
+    dom = max(kmx, key=lambda k: kmx[k]["ms_per_step"])
     it_s_mx = args.steps / rmx["seconds"]
     it_s_64 = args.steps / r64["seconds"]
     cpu = None
@@ -165,15 +193,19 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"lid-driven cavity, uniform{' perturbed' if args.perturb else ''} "
                                f"{args.n}^3 hex8 ({ne} elements, {nn} nodes), Re=100, mixed form; "
-                               f"GMRES(20, 0.1) momentum, Jacobi-CG(200, 1e-6) p'",
+                               f"Jacobi-GMRES(20 it, rtol 0.1) momentum, aggregation-AMG-CG "
+                               f"(200 it, rtol 1e-6) p'",
                    "n": args.n, "elements": ne, "nodes": nn, "nnz": nnz},
         "fp64": {"value": round(it_s_64, 4), "ms_per_step": round(1e3 * r64["seconds"] / args.steps, 3),
-                 "assembly": k64},
+                 "krylov_iters_mean": [round(float(v), 1) for v in r64["hist"][:, 4:6].mean(axis=0)],
+                 "kernels": k64},
+        "krylov_iters_mean": [round(float(v), 1) for v in rmx["hist"][:, 4:6].mean(axis=0)],
         "mixed_over_fp64": round(it_s_mx / it_s_64, 3),
-        "assembly": kmx,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": kmx[dom]["GB_s"],
+        "kernels": kmx,
+        "roofline": {"bound": "hbm", "kernel": dom, "launch_ms": kmx[dom]["ms"],
+                     "achieved": kmx[dom]["GB_s"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kmx[dom]["hbm_frac"],
-                     "traffic": None},
+                     "traffic": measured_traffic(dom, args.n, 1) if not args.perturb else None},
         "cpu_baseline": cpu,
         "residuals_last": [float(x) for x in rmx["hist"][-1][:4]],
     }

@@ -64,6 +64,8 @@ typedef struct oagpu_flow_params {
     int precision;         /* OAGPU_FP64 or OAGPU_MIXED                     */
     int p_method;          /* p' solver of oagpu_outer_iteration: OAGPU_CG or
                               OAGPU_CG_AMG                                  */
+    int use_graphs;        /* 1: CG iterations replayed from a captured HIP
+                              graph; 0: launched one by one (same results)  */
 } oagpu_flow_params;
 
 /* liblinsolve ControlData (linearSystem.h:475-482) as the device solve
@@ -171,7 +173,11 @@ int oagpu_amg_level_copy(oagpu_flow *f, int level, int *agg, int *row_ptr, int *
  * hist[6] = U, V, W, P control residuals, momentum / p' Krylov iterations. */
 int oagpu_outer_iteration(oagpu_flow *f, const double *ctl, double *hist);
 
-/* device time of the last assembly kernels, milliseconds (HIP events) */
+/* device time of the last launch of a kernel, milliseconds (HIP events):
+ * "momentum", "pcorr", "mdot", "gradU" (assembly and field kernels of the last
+ * call), "amg_pre" / "amg_post" (fine-level AMG pre-smooth + residual and
+ * prolongation + post-smooth of the V-cycle that started the last AMG-CG
+ * solve); "amg_vcycles" returns the V-cycle count of the flow so far. */
 int oagpu_last_kernel_ms(oagpu_flow *f, const char *which, double *ms);
 
 #ifdef __cplusplus

@@ -44,7 +44,8 @@ class FlowParams(C.Structure):
                 ("grad_urf", C.c_double), ("bnd_relax", C.c_double),
                 ("u_shifted", C.c_int), ("ugrad_shifted", C.c_int),
                 ("p_shifted", C.c_int), ("pgrad_shifted", C.c_int),
-                ("p_ref_node", C.c_int), ("precision", C.c_int), ("p_method", C.c_int)]
+                ("p_ref_node", C.c_int), ("precision", C.c_int), ("p_method", C.c_int),
+                ("use_graphs", C.c_int)]
 
 
 class ControlData(C.Structure):
@@ -138,7 +139,7 @@ def default_params(precision: int = OAGPU_FP64, **kw) -> FlowParams:
     p = FlowParams(rho=1.0, mu=0.01, dt=1.0, urf_U=0.9, urf_p=0.1, urf_mdot=1.0,
                    grad_urf=0.5625, bnd_relax=0.75, u_shifted=0, ugrad_shifted=1,
                    p_shifted=1, pgrad_shifted=1, p_ref_node=0, precision=precision,
-                   p_method=OAGPU_CG)
+                   p_method=OAGPU_CG, use_graphs=1)
     for k, v in kw.items():
         setattr(p, k, v)
     return p

@@ -132,9 +132,11 @@ __global__ void __launch_bounds__(256) k_amg_dense_inverse(SellView s, const V* 
         }
         __syncthreads();
     }
+    // stored transposed: lane i of the apply kernel reads inv[j][i], coalesced
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-        inv[e] = M[e / n][e % n];
-        vinv[e] = V(M[e / n][e % n]);
+        const int i = e / n, j = e % n;
+        inv[size_t(j) * n + i] = M[i][j];
+        vinv[size_t(j) * n + i] = V(M[i][j]);
     }
 }
 
@@ -173,6 +175,7 @@ __global__ void __launch_bounds__(256) k_amg_restrict(int nc, const int* mem_ptr
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= nc) return;
     V s = V(0);
+#pragma unroll 4
     for (int k = mem_ptr[I]; k < mem_ptr[I + 1]; ++k) s += res[mem[k]];
     rc[I] = s;
     if (dinvc) xc[I] = omega * dinvc[I] * s;
@@ -201,7 +204,79 @@ __global__ void __launch_bounds__(256) k_amg_smooth(SellView s, const V* A, cons
     xo[i] = t[i] + omega * dinv[i] * (r[i] - acc);
 }
 
-// x = Minv r on the coarsest level (one workgroup)
+__device__ inline double amg_block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// Pre-smoothing from a zero guess fused with the residual:
+//   x = omega D^-1 r,  res = r - A x  (x of the neighbours formed on the fly)
+// FINE only names the fine-level instance separately (profiles, timing).
+template <typename V, bool FINE>
+__global__ void __launch_bounds__(256) k_amg_pre_residual(SellView s, const V* A, const V* dinv,
+                                                          V omega, const V* r, V* x, V* res,
+                                                          const int* flag) {
+    if (flag[0]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    int w;
+    const size_t b = sv_base(s, i, w);
+    V acc = V(0);
+#pragma unroll 4
+    for (int q = 0; q < w; ++q) {
+        const int j = s.cols[b + size_t(q) * 64];
+        acc += A[b + size_t(q) * 64] * (dinv[j] * r[j]);
+    }
+    const V ri = r[i];
+    x[i] = omega * dinv[i] * ri;
+    res[i] = ri - omega * acc;
+}
+
+// Prolongation fused with post-smoothing (out of place):
+//   t = x + alpha P ec (formed on the fly for the row and its neighbours),
+//   out = t + omega D^-1 (r - A t).
+// Grid-stride; with part != nullptr the block also writes its partial of
+// r . out (a fixed partition of gridDim blocks) and, with p != nullptr,
+// copies out into p (first CG iteration).
+template <typename V, bool FINE>
+__global__ void __launch_bounds__(256) k_amg_prolong_smooth(SellView s, const V* A, const V* dinv,
+                                                            V omega, const V* r, const V* x,
+                                                            const int* agg, V alpha, const V* ec,
+                                                            V* out, double* part, V* p,
+                                                            const int* flag) {
+    __shared__ double sh[256];
+    if (flag[0]) return;
+    double rz = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+        int w;
+        const size_t b = sv_base(s, i, w);
+        V acc = V(0);
+#pragma unroll 4
+        for (int q = 0; q < w; ++q) {
+            const int j = s.cols[b + size_t(q) * 64];
+            acc += A[b + size_t(q) * 64] * (x[j] + alpha * ec[agg[j]]);
+        }
+        const V ri = r[i];
+        const V ti = x[i] + alpha * ec[agg[i]];
+        const V oi = ti + omega * dinv[i] * (ri - acc);
+        out[i] = oi;
+        if (p) p[i] = oi;
+        rz += double(ri) * double(oi);
+    }
+    if (part) {
+        const double t = amg_block_sum(rz, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = t;
+    }
+}
+
+// x = Minv r on the coarsest level (one workgroup; Minv stored transposed)
 template <typename V>
 __global__ void __launch_bounds__(kDenseMax) k_amg_dense_apply(int n, const V* minv, const V* r, V* x,
                                                                const int* flag) {
@@ -209,7 +284,8 @@ __global__ void __launch_bounds__(kDenseMax) k_amg_dense_apply(int n, const V* m
     const int i = threadIdx.x;
     if (i >= n) return;
     V acc = V(0);
-    for (int j = 0; j < n; ++j) acc += minv[size_t(i) * n + j] * r[j];
+#pragma unroll 8
+    for (int j = 0; j < n; ++j) acc += minv[size_t(j) * n + i] * r[j];
     x[i] = acc;
 }
 
@@ -404,7 +480,7 @@ void amg_setup(Flow& f, const V* A0, const V* dinv0) {
                                                              c.gmap.p, Af, lvA<V>(c));
         k_amg_dinv<V><<<blocks(c.n), 256, 0, f.stream>>>(view_of(c), lvA<V>(c), lvv<V>(c, 0));
     }
-    if (amg.dense_n) {
+    if (amg.dense_n && nl > 1) {
         AmgLevel& c = amg.lv[nl - 1];
         V* vinv;
         if constexpr (std::is_same_v<V, float>) vinv = amg.fdense.p;
@@ -415,73 +491,76 @@ void amg_setup(Flow& f, const V* A0, const V* dinv0) {
 }
 
 template <typename V>
-void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z) {
+void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* rz_part, V* p_init,
+                bool timing) {
     Amg& amg = f.amg;
     const int nl = int(amg.lv.size());
     const int* flag = f.flag.p;
     const V om = V(kOmega), al = V(kAlpha);
     AmgLevel& l0 = amg.lv[0];
     const SellView v0 = view_of(*f.m);
-    if (nl == 1) {  // no coarse level: two Jacobi steps
-        k_amg_jacobi0<V><<<blocks(l0.n), 256, 0, f.stream>>>(l0.n, om, dinv0, r, lvv<V>(l0, 2), flag);
-        k_amg_smooth<V><<<blocks(l0.n), 256, 0, f.stream>>>(v0, A0, dinv0, om, r, lvv<V>(l0, 2), z,
-                                                            flag);
-        return;
-    }
-    // downward: level l has rhs R_l (r at l = 0), pre-smoothed X_l (z at l = 0)
+    OAGPU_REQUIRE(nl > 1, "AMG hierarchy without a coarse level");
     auto R = [&](int l) -> const V* { return l == 0 ? r : lvv<V>(amg.lv[l], 3); };
-    auto X = [&](int l) -> V* { return l == 0 ? z : lvv<V>(amg.lv[l], 1); };
+    auto X = [&](int l) -> V* { return lvv<V>(amg.lv[l], l == 0 ? 2 : 1); };  // T0 on level 0
     auto A = [&](int l) -> const V* { return l == 0 ? A0 : lvA<V>(amg.lv[l]); };
     auto D = [&](int l) -> const V* { return l == 0 ? dinv0 : lvv<V>(amg.lv[l], 0); };
     auto view = [&](int l) { return l == 0 ? v0 : view_of(amg.lv[l]); };
     const int last = nl - 1;
-    k_amg_jacobi0<V><<<blocks(l0.n), 256, 0, f.stream>>>(l0.n, om, dinv0, r, z, flag);
+    // downward: pre-smooth + residual, restrict
     for (int l = 0; l < last; ++l) {
         AmgLevel& fl = amg.lv[l];
         AmgLevel& cl = amg.lv[l + 1];
         V* S = lvv<V>(fl, 4);
-        k_amg_residual<V><<<blocks(fl.n), 256, 0, f.stream>>>(view(l), A(l), X(l), R(l), S, flag);
-        const bool coarsest = l + 1 == last;
-        k_amg_restrict<V><<<blocks(cl.n), 256, 0, f.stream>>>(
-            cl.n, cl.mem_ptr.p, cl.mem.p, S, om, coarsest ? nullptr : D(l + 1), lvv<V>(cl, 3),
-            X(l + 1), flag);
+        if (l == 0) {
+            if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[0], f.stream));
+            k_amg_pre_residual<V, true><<<blocks(fl.n), 256, 0, f.stream>>>(view(l), A(l), D(l), om,
+                                                                            R(l), X(l), S, flag);
+            if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[1], f.stream));
+        } else {
+            k_amg_pre_residual<V, false><<<blocks(fl.n), 256, 0, f.stream>>>(view(l), A(l), D(l), om,
+                                                                             R(l), X(l), S, flag);
+        }
+        k_amg_restrict<V><<<blocks(cl.n), 256, 0, f.stream>>>(cl.n, cl.mem_ptr.p, cl.mem.p, S, om,
+                                                              nullptr, lvv<V>(cl, 3), nullptr, flag);
     }
-    {
-        AmgLevel& c = amg.lv[last];
-        if (amg.dense_n) {
-            const V* inv;
-            if constexpr (std::is_same_v<V, float>) inv = amg.fdense.p;
-            else inv = amg.dense.p;
-            k_amg_dense_apply<V><<<1, kDenseMax, 0, f.stream>>>(c.n, inv, R(last), X(last), flag);
-        } else {  // coarsening stalled above the dense size: Jacobi steps
-            k_amg_jacobi0<V><<<blocks(c.n), 256, 0, f.stream>>>(c.n, om, D(last), R(last), X(last),
-                                                                flag);
-            for (int it = 0; it < 4; ++it) {
-                k_amg_smooth<V><<<blocks(c.n), 256, 0, f.stream>>>(view(last), A(last), D(last), om,
-                                                                   R(last), X(last), lvv<V>(c, 2),
-                                                                   flag);
-                k_amg_smooth<V><<<blocks(c.n), 256, 0, f.stream>>>(view(last), A(last), D(last), om,
-                                                                   R(last), lvv<V>(c, 2), X(last),
-                                                                   flag);
-            }
+    AmgLevel& c = amg.lv[last];
+    if (amg.dense_n) {
+        const V* inv;
+        if constexpr (std::is_same_v<V, float>) inv = amg.fdense.p;
+        else inv = amg.dense.p;
+        k_amg_dense_apply<V><<<1, kDenseMax, 0, f.stream>>>(c.n, inv, R(last), X(last), flag);
+    } else {  // coarsening stalled above the dense size: Jacobi steps
+        k_amg_jacobi0<V><<<blocks(c.n), 256, 0, f.stream>>>(c.n, om, D(last), R(last), X(last), flag);
+        for (int it = 0; it < 4; ++it) {
+            k_amg_smooth<V><<<blocks(c.n), 256, 0, f.stream>>>(view(last), A(last), D(last), om,
+                                                               R(last), X(last), lvv<V>(c, 2), flag);
+            k_amg_smooth<V><<<blocks(c.n), 256, 0, f.stream>>>(view(last), A(last), D(last), om,
+                                                               R(last), lvv<V>(c, 2), X(last), flag);
         }
     }
-    // upward: T_l = X_l + alpha P X_{l+1}; X_l = T_l + omega D^-1 (R_l - A T_l)
-    for (int l = last - 1; l >= 0; --l) {
+    // upward: prolong + post-smooth, out of place (coarse levels into T_l)
+    const V* ec = X(last);
+    for (int l = last - 1; l >= 1; --l) {
         AmgLevel& fl = amg.lv[l];
-        AmgLevel& cl = amg.lv[l + 1];
-        V* T = lvv<V>(fl, 2);
-        k_amg_prolong<V><<<blocks(fl.n), 256, 0, f.stream>>>(fl.n, cl.agg.p, al, X(l + 1), X(l), T,
-                                                             flag);
-        k_amg_smooth<V><<<blocks(fl.n), 256, 0, f.stream>>>(view(l), A(l), D(l), om, R(l), T, X(l),
-                                                            flag);
+        V* out = lvv<V>(fl, 2);
+        k_amg_prolong_smooth<V, false><<<blocks(fl.n), 256, 0, f.stream>>>(
+            view(l), A(l), D(l), om, R(l), X(l), amg.lv[l + 1].agg.p, al, ec, out, nullptr, nullptr,
+            flag);
+        ec = out;
     }
+    if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[2], f.stream));
+    k_amg_prolong_smooth<V, true><<<kRedGrid, 256, 0, f.stream>>>(v0, A0, dinv0, om, r, X(0),
+                                                                  amg.lv[1].agg.p, al, ec, z,
+                                                                  rz_part, p_init, flag);
+    if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[3], f.stream));
     OAGPU_HIP(hipGetLastError());
 }
 
 template void amg_setup<double>(Flow&, const double*, const double*);
 template void amg_setup<float>(Flow&, const float*, const float*);
-template void amg_vcycle<double>(Flow&, const double*, const double*, const double*, double*);
-template void amg_vcycle<float>(Flow&, const float*, const float*, const float*, float*);
+template void amg_vcycle<double>(Flow&, const double*, const double*, const double*, double*,
+                                 double*, double*, bool);
+template void amg_vcycle<float>(Flow&, const float*, const float*, const float*, float*, double*,
+                                float*, bool);
 
 }  // namespace oagpu

@@ -246,6 +246,7 @@ int oagpu_flow_create(oagpu_mesh* m, const oagpu_flow_params* prm, int n_bc, con
         OAGPU_HIP(hipStreamCreate(&f.stream));
         OAGPU_HIP(hipEventCreate(&f.ev0));
         OAGPU_HIP(hipEventCreate(&f.ev1));
+        for (auto& e : f.ev_amg) OAGPU_HIP(hipEventCreate(&e));
         // boundary ids must index the velocity table
         {
             std::vector<int> ids(m->m.nbf ? m->m.nbf : 1);
@@ -267,6 +268,8 @@ int oagpu_flow_destroy(oagpu_flow* f) {
         if (f->f.stream) (void)hipStreamSynchronize(f->f.stream);
         if (f->f.ev0) (void)hipEventDestroy(f->f.ev0);
         if (f->f.ev1) (void)hipEventDestroy(f->f.ev1);
+        for (auto& e : f->f.ev_amg)
+            if (e) (void)hipEventDestroy(e);
         if (f->f.stream) (void)hipStreamDestroy(f->f.stream);
         delete f;
     });
@@ -525,6 +528,9 @@ int oagpu_last_kernel_ms(oagpu_flow* h, const char* which, double* ms) {
         else if (!std::strcmp(which, "pcorr")) *ms = f.ms_pcorr;
         else if (!std::strcmp(which, "mdot")) *ms = f.ms_mdot;
         else if (!std::strcmp(which, "gradU")) *ms = f.ms_grad;
+        else if (!std::strcmp(which, "amg_pre")) *ms = f.ms_amg_pre;
+        else if (!std::strcmp(which, "amg_post")) *ms = f.ms_amg_post;
+        else if (!std::strcmp(which, "amg_vcycles")) *ms = double(f.amg_vcycles);
         else throw oagpu::Error(OAGPU_EINVAL, "unknown kernel name");
     });
 }

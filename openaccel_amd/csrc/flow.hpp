@@ -4,6 +4,8 @@
 #include "../../include/oagpu.h"
 #include "mesh.hpp"
 
+#include <array>
+
 namespace oagpu {
 
 constexpr int kRedGrid = 2048;  // fixed partition of every reduction: deterministic sums
@@ -33,6 +35,18 @@ struct Amg {
     DBuf<float> fdense;
 };
 
+// An instantiated HIP graph of a block of CG iterations and the operands it
+// was captured with.
+struct CgGraph {
+    hipGraphExec_t exec = nullptr;
+    std::array<const void*, 10> key{};
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+    }
+    ~CgGraph() { reset(); }
+};
+
 struct Flow {
     DevMesh* m = nullptr;
     oagpu_flow_params prm{};
@@ -55,7 +69,13 @@ struct Flow {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double ms_momentum = 0, ms_pcorr = 0, ms_mdot = 0, ms_grad = 0;
+    // fine-level AMG kernels, timed on the eager V-cycle that starts each CG
+    // solve: [0..1] pre-smooth+residual, [2..3] prolong+post-smooth
+    hipEvent_t ev_amg[4] = {nullptr, nullptr, nullptr, nullptr};
+    double ms_amg_pre = 0, ms_amg_post = 0;
+    long amg_vcycles = 0;   // V-cycles run (counts launches of the fine kernels)
     Amg amg;
+    CgGraph cg_graph;
 };
 
 // kernels.hip
@@ -76,7 +96,11 @@ int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double ato
 void amg_build(Flow& f);                       // hierarchy structure (once per mesh)
 template <typename V>
 void amg_setup(Flow& f, const V* A0, const V* dinv0);  // numeric Galerkin levels for A0
+// z = M^-1 r; the block partials of r . z go to rz_part (kRedGrid blocks) and,
+// with p_init, p = z as well
+// timing: HIP events around the fine-level kernels (eager launches only)
 template <typename V>
-void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z);  // z = M^-1 r
+void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* rz_part, V* p_init,
+                bool timing = false);
 
 }  // namespace oagpu

@@ -129,3 +129,34 @@ def test_amg_outer_iterations_converge(oa, meshgen):
         f.initialize()
         hist[pm] = np.array([f.outer_iteration([50, 1e-8, 3000, 1e-10])[:4] for _ in range(4)])
     np.testing.assert_allclose(hist[oa.OAGPU_CG_AMG], hist[oa.OAGPU_CG], rtol=1e-5)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("method", ["CG", "CG_AMG"])
+def test_graph_replay_matches_eager(oa, meshgen, precision, method):
+    """CG iterations replayed from a captured HIP graph give bitwise the
+    results of the same kernels launched one by one, on every replay, with
+    eager work (assembly, uploads, other solves) between the solves."""
+    m = meshgen.cavity_hex(14, perturb=0.1, seed=9)
+    meth = getattr(oa, "OAGPU_" + method)
+    out = {}
+    for graphs in (0, 1):
+        print(f"[case] graphs={graphs} precision={precision} method={method}", flush=True)
+        dm = oa.DeviceMesh(m)
+        f = oa.FlowCase(dm, oa.default_params(precision, use_graphs=graphs), BC)
+        rng = np.random.default_rng(31)
+        f.upload("U", rng.normal(scale=0.3, size=(dm.nn, 3)))
+        f.initialize()
+        f.upload("du", rng.uniform(1e-3, 2e-3, size=(dm.nn, 3)))
+        res = []
+        for rep in range(3):
+            f.upload("p", rng.normal(scale=0.1, size=dm.nn))
+            f.assemble_pcorr()
+            f.assemble_momentum()
+            f.solve(3, oa.OAGPU_CG, 5, 1e-3)       # another (eager or captured) solve between
+            cd = f.solve(1, meth, 150, 1e-7)
+            res.append((cd.n_iterations, f.solution(1)))
+        out[graphs] = res
+    for (i0, x0), (i1, x1) in zip(out[0], out[1]):
+        assert i0 == i1
+        assert np.array_equal(x0, x1)
