@@ -23,32 +23,10 @@
  *   dual nodal volume     mesh/meshGeometry.cpp:1277
  *   exposed area vectors  mesh/meshGeometry.cpp:1441-1528
  */
-#include "oracle.h"
-#include <float.h>
+#include "case_internal.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
-
-#define SMALL DBL_EPSILON /* src/basic/types.h:179 */
-
-struct or_case {
-    int nn, ne, nbf, nbc;
-    double *xyz;
-    int *conn;
-    int *bf_elem, *bf_ord, *bf_bc;
-    double *bc_U;
-    or_params prm;
-    /* graph */
-    int nnz;
-    int *row_ptr, *cols, *diag_off;
-    /* systems */
-    double *A3, *b3, *x3, *A1, *b1, *x1;
-    /* fields */
-    double *U, *p, *gradU, *gradP, *du, *mdot, *div, *vol, *rho, *mu, *beta;
-    double *bareav; /* exposed area vectors [nbf][4][3] */
-    int *bnode;     /* node is on a wall boundary */
-    double U_scale, p_scale;
-};
 
 static void *xcalloc(size_t n, size_t s)
 {
@@ -139,7 +117,8 @@ void or_case_destroy(or_case *c)
                     c->row_ptr, c->cols, c->diag_off, c->A3, c->b3, c->x3,
                     c->A1, c->b1, c->x1, c->U, c->p, c->gradU, c->gradP, c->du,
                     c->mdot, c->div, c->vol, c->rho, c->mu, c->beta, c->bareav,
-                    c->bnode};
+                    c->bnode, c->k, c->w, c->gradK, c->gradW, c->ymin, c->mut, c->F1,
+                    c->Pk, c->Pk_wall, c->kprev};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
     free(c);
 }
@@ -165,6 +144,16 @@ double *or_case_field(or_case *c, const char *name)
     if (!strcmp(name, "beta")) return c->beta;
     if (!strcmp(name, "bareav")) return c->bareav;
     if (!strcmp(name, "xyz")) return c->xyz;
+    if (!strcmp(name, "k")) return c->k;
+    if (!strcmp(name, "omega")) return c->w;
+    if (!strcmp(name, "gradK")) return c->gradK;
+    if (!strcmp(name, "gradW")) return c->gradW;
+    if (!strcmp(name, "ymin")) return c->ymin;
+    if (!strcmp(name, "mut")) return c->mut;
+    if (!strcmp(name, "F1")) return c->F1;
+    if (!strcmp(name, "Pk")) return c->Pk;
+    if (!strcmp(name, "Pk_wall")) return c->Pk_wall;
+    if (!strcmp(name, "kprev")) return c->kprev;
     return NULL;
 }
 
@@ -172,12 +161,7 @@ double *or_case_vals(or_case *c, int N) { return N == 3 ? c->A3 : c->A1; }
 double *or_case_rhs(or_case *c, int N) { return N == 3 ? c->b3 : c->b1; }
 double *or_case_x(or_case *c, int N) { return N == 3 ? c->x3 : c->x1; }
 
-static void gather_coords(const or_case *c, int e, double *co)
-{
-    const int *en = &c->conn[8 * e];
-    for (int a = 0; a < 8; ++a)
-        for (int d = 0; d < 3; ++d) co[3 * a + d] = c->xyz[3 * en[a] + d];
-}
+#define gather_coords or_gather_coords
 
 void or_dual_volume(or_case *c)
 {
@@ -288,10 +272,14 @@ void or_mass_divergence(or_case *c)
 /* ------------------------------------------------------------- gradients */
 void or_gradient(or_case *c, int which)
 {
-    const int N = which == 0 ? 3 : 1;
-    double *phi = which == 0 ? c->U : c->p;
-    double *grad = which == 0 ? c->gradU : c->gradP;
-    const int shifted = which == 0 ? c->prm.u_shifted : c->prm.p_shifted;
+    if (which == 0) or_gradient_field(c, c->U, c->gradU, 3, c->prm.u_shifted);
+    else if (which == 1) or_gradient_field(c, c->p, c->gradP, 1, c->prm.p_shifted);
+    else if (which == 2 && c->k) or_gradient_field(c, c->k, c->gradK, 1, c->sst.k_shifted);
+    else if (which == 3 && c->w) or_gradient_field(c, c->w, c->gradW, 1, c->sst.w_shifted);
+}
+
+void or_gradient_field(or_case *c, const double *phi, double *grad, int N, int shifted)
+{
     const double gurf = c->prm.grad_urf;
     const int *lr = or_hex_lrscv();
     const int *sn = or_hex_side_nodes();
@@ -630,6 +618,12 @@ static void field_scale(const double *v, int n, int N, double *scale, double *mx
     *scale = sc;
     *mx = maxv;
     *mn = minv;
+}
+
+void or_field_scale(const double *v, int n, int N, double *scale)
+{
+    double mx, mn;
+    field_scale(v, n, N, scale, &mx, &mn);
 }
 
 void or_update_scales(or_case *c, double *U_scale, double *p_scale)

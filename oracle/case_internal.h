@@ -1,0 +1,49 @@
+/*
+ * case_internal.h -- state of an oracle case shared by case.c and sst.c.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ */
+#ifndef OA_CASE_INTERNAL_H
+#define OA_CASE_INTERNAL_H
+#include "oracle.h"
+#include <float.h>
+
+#define SMALL DBL_EPSILON /* src/basic/types.h:179 */
+
+struct or_case {
+    int nn, ne, nbf, nbc;
+    double *xyz;
+    int *conn;
+    int *bf_elem, *bf_ord, *bf_bc;
+    double *bc_U;
+    or_params prm;
+    /* graph */
+    int nnz;
+    int *row_ptr, *cols, *diag_off;
+    /* systems */
+    double *A3, *b3, *x3, *A1, *b1, *x1;
+    /* fields */
+    double *U, *p, *gradU, *gradP, *du, *mdot, *div, *vol, *rho, *mu, *beta;
+    double *bareav; /* exposed area vectors [nbf][4][3] */
+    int *bnode;     /* node is on a wall boundary */
+    double U_scale, p_scale;
+    /* k-omega SST (sst.c); NULL until or_sst_enable */
+    or_sst_params sst;
+    double *k, *w, *gradK, *gradW, *ymin, *mut, *F1, *Pk, *Pk_wall, *kprev;
+    double k_scale, w_scale;
+    int scalar_terms;   /* or_set_scalar_terms mask (test hook), default all */
+};
+
+static inline void or_gather_coords(const or_case *c, int e, double *co)
+{
+    const int *en = &c->conn[8 * e];
+    for (int a = 0; a < 8; ++a)
+        for (int d = 0; d < 3; ++d) co[3 * a + d] = c->xyz[3 * en[a] + d];
+}
+
+/* Green-Gauss gradient of an N-component nodal field (case.c) */
+void or_gradient_field(or_case *c, const double *phi, double *grad, int N, int shifted);
+/* scale of an N-component field, nodeField::updateScale (case.c) */
+void or_field_scale(const double *v, int n, int N, double *scale);
+
+
+#endif

@@ -158,6 +158,50 @@ void or_scaled_residual(or_case *c, int N, double *out /*[N]*/);
  * hist[5..6] = momentum / p' solver iterations. */
 void or_outer_iteration(or_case *c, const double *ctl, double *hist);
 
+/* ------------------------------------------------------- k-omega SST */
+/* Constants: shearStressTransportModel.h:21-37 (sigma_k1 0.85, sigma_k2 1.0,
+ * sigma_w1 0.5, sigma_w2 0.856, beta1 0.075, beta2 0.0828, gamma1 0.5532,
+ * gamma2 0.4403, a1 0.31), RANSModel.h:20-22 (beta* 0.09, production limit
+ * 10), turbulenceModel.h:20 (C_mu 0.09). */
+typedef struct or_sst_params {
+    double urf_k, urf_w;     /* implicit relaxation of the k / omega rows   */
+    double urf_mut;          /* eddy-viscosity relaxation (mutRef().urf())  */
+    int k_shifted, w_shifted;         /* k / omega interpolation linear_linear */
+    int kgrad_shifted, wgrad_shifted; /* gradient operators at shifted ips     */
+} or_sst_params;
+
+/* Allocate the SST fields: "k", "omega", "gradK", "gradW", "ymin" (wall
+ * distance), "mut", "F1", "Pk", "Pk_wall" (production at no-slip wall nodes,
+ * supplied by the wall treatment), "kprev".  "mu" becomes mu_eff. */
+void or_sst_enable(or_case *c, const or_sst_params *s);
+/* shearStressTransportModel::updateTurbulentDynamicViscosity (:597-690) and
+ * turbulenceModel::updateEffectiveDynamicViscosity (turbulenceModel.cpp:121) */
+void or_sst_update_mut(or_case *c);
+/* shearStressTransportModel::updateFOneBlending (:95-174) */
+void or_sst_update_F1(or_case *c);
+/* shearStressTransportModel::updateTurbulentProduction (:176-340),
+ * incompressible, no frame rotation; no-slip wall nodes take Pk_wall */
+void or_sst_update_Pk(or_case *c);
+/* k (which = 0) or omega (1) system into the scalar (N = 1) system:
+ * turbulent*SSTAssemblerNodeTerms.cpp (steady), phiAssemblerElemTerms.hpp
+ * :10-413 (upwind), relaxation phiAssembler.h:966-1006, boundary rhs
+ * relaxation :1008-1152 (x 0.75), omega rows fixed at no-slip walls
+ * (turbulentEddyFrequencySSTAssembler.h postAssemble_) */
+void or_assemble_scalar(or_case *c, int which);
+/* test hook: restrict or_assemble_scalar to some of its parts (bit 1 node
+ * transient + divergence terms, 2 SST sources, 4 interior element terms,
+ * 8 relaxation / boundary relaxation / wall rows); 15 = all (default) */
+void or_set_scalar_terms(or_case *c, int mask);
+/* phi += x with the lower clip at SMALL (equation.h:204-286, CLIP = 1),
+ * then the field scale (nodeField.hpp:5993) */
+void or_correct_scalar(or_case *c, int which);
+/* one SST pass after the flow equations (simulation.cpp:314-377 order;
+ * segregatedShearStressTransportEquations.cpp preSolve/solve): mut, mu_eff,
+ * F1, Pk, k solve + correction, omega solve + correction, grad k, grad omega.
+ * ctl = {max_it, rtol} of the Jacobi-GMRES(30) scalar solves; hist = {k, omega
+ * control residuals, k, omega solver iterations}. */
+void or_sst_iteration(or_case *c, const double *ctl, double *hist);
+
 #ifdef __cplusplus
 }
 #endif

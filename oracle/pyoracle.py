@@ -28,6 +28,20 @@ class Params(C.Structure):
                 ("reduced", C.c_int), ("p_ref_node", C.c_int)]
 
 
+class SstParams(C.Structure):
+    _fields_ = [("urf_k", C.c_double), ("urf_w", C.c_double), ("urf_mut", C.c_double),
+                ("k_shifted", C.c_int), ("w_shifted", C.c_int),
+                ("kgrad_shifted", C.c_int), ("wgrad_shifted", C.c_int)]
+
+
+def default_sst(**kw) -> SstParams:
+    s = SstParams(urf_k=0.8, urf_w=0.8, urf_mut=1.0, k_shifted=1, w_shifted=1,
+                  kgrad_shifted=1, wgrad_shifted=1)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
 def dptr(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(_dp)
@@ -97,6 +111,12 @@ def lib():
         L.or_quad_face_areav.argtypes = [_dp, _dp]
         L.or_quad_face_shape.argtypes = [C.c_int, _dp]
         L.or_hex_lrscv.restype = _ip
+        L.or_sst_enable.argtypes = [C.c_void_p, C.POINTER(SstParams)]
+        for f in ("or_sst_update_mut", "or_sst_update_F1", "or_sst_update_Pk"):
+            getattr(L, f).argtypes = [C.c_void_p]
+        for f in ("or_assemble_scalar", "or_correct_scalar", "or_set_scalar_terms"):
+            getattr(L, f).argtypes = [C.c_void_p, C.c_int]
+        L.or_sst_iteration.argtypes = [C.c_void_p, _dp, _dp]
         L.or_hex_side_nodes.restype = _ip
         _lib = L
     return _lib
@@ -152,7 +172,8 @@ class Case:
         return self._view(lib().or_case_diag_off(self.h), self.nn)
 
     _sizes = {"U": 3, "p": 1, "gradU": 9, "gradP": 3, "du": 3, "div": 1, "vol": 1,
-              "rho": 1, "mu": 1, "beta": 3, "xyz": 3}
+              "rho": 1, "mu": 1, "beta": 3, "xyz": 3, "k": 1, "omega": 1, "gradK": 3,
+              "gradW": 3, "ymin": 1, "mut": 1, "F1": 1, "Pk": 1, "Pk_wall": 1, "kprev": 1}
 
     def field(self, name):
         ptr = lib().or_case_field(self.h, name.encode())
@@ -192,7 +213,35 @@ class Case:
         lib().or_mass_divergence(self.h)
 
     def gradient(self, which):
-        lib().or_gradient(self.h, {"U": 0, "p": 1}[which])
+        lib().or_gradient(self.h, {"U": 0, "p": 1, "k": 2, "omega": 3}[which])
+
+    # k-omega SST
+    def sst_enable(self, sst: SstParams):
+        self._sst = sst
+        lib().or_sst_enable(self.h, C.byref(sst))
+
+    def sst_update_mut(self):
+        lib().or_sst_update_mut(self.h)
+
+    def sst_update_F1(self):
+        lib().or_sst_update_F1(self.h)
+
+    def sst_update_Pk(self):
+        lib().or_sst_update_Pk(self.h)
+
+    def assemble_scalar(self, which, terms=15):
+        lib().or_set_scalar_terms(self.h, terms)
+        lib().or_assemble_scalar(self.h, {"k": 0, "omega": 1}[which])
+        lib().or_set_scalar_terms(self.h, 15)
+
+    def correct_scalar(self, which):
+        lib().or_correct_scalar(self.h, {"k": 0, "omega": 1}[which])
+
+    def sst_iteration(self, ctl):
+        c = np.ascontiguousarray(ctl, dtype=np.float64)
+        h = np.zeros(4)
+        lib().or_sst_iteration(self.h, dptr(c), dptr(h))
+        return h
 
     def solve(self, N, kind, max_it, rtol, atol=1e-30, restart=30):
         res = np.zeros(2)
