@@ -68,6 +68,17 @@ typedef struct oagpu_flow_params {
                               graph; 0: launched one by one (same results)  */
 } oagpu_flow_params;
 
+/* k-omega SST transport (shearStressTransportModel.h:21-37 constants are
+ * built in).  The wall treatment stays with the caller: the wall distance
+ * "ymin", omega at no-slip wall nodes (rows fixed there, so it is kept) and
+ * the production at wall nodes "Pk_wall" are fields it uploads. */
+typedef struct oagpu_sst_params {
+    double urf_k, urf_w;   /* implicit relaxation of the k / omega rows      */
+    double urf_mut;        /* eddy-viscosity relaxation (mutRef().urf())     */
+    int k_shifted, w_shifted;         /* k / omega interpolation linear_linear */
+    int kgrad_shifted, wgrad_shifted; /* their gradient operators shifted      */
+} oagpu_sst_params;
+
 /* liblinsolve ControlData (linearSystem.h:475-482) as the device solve
  * reports it */
 typedef struct oagpu_control_data {
@@ -120,7 +131,8 @@ int oagpu_flow_create(oagpu_mesh *m, const oagpu_flow_params *prm, int n_bc,
                       const double *bc_U, oagpu_flow **out);
 int oagpu_flow_destroy(oagpu_flow *f);
 /* fields: "U" [nn][3], "p" [nn], "gradU" [nn][9], "gradP" [nn][3],
- * "du" [nn][3], "mdot" [ne][12], "div" [nn] */
+ * "du" [nn][3], "mdot" [ne][12], "div" [nn], "mu" [nn] (effective
+ * viscosity), and the SST fields once enabled */
 int oagpu_field_upload(oagpu_flow *f, const char *name, const double *host, size_t count);
 int oagpu_field_download(oagpu_flow *f, const char *name, double *host, size_t count);
 
@@ -144,7 +156,7 @@ int oagpu_solution_download(oagpu_flow *f, int N, double *x);
 /* field updates */
 int oagpu_update_mdot(oagpu_flow *f);          /* flowModel.cpp:5272-5688   */
 int oagpu_mass_divergence(oagpu_flow *f);      /* flowModel.cpp:13541-13775 */
-int oagpu_gradient(oagpu_flow *f, int which);  /* nodeField.hpp:3253; 0 U, 1 p */
+int oagpu_gradient(oagpu_flow *f, int which);  /* nodeField.hpp:3253; 0 U, 1 p, 2 k, 3 omega */
 int oagpu_correct(oagpu_flow *f, int N, double urf); /* equation.h:204-286  */
 int oagpu_velocity_correction(oagpu_flow *f, int step); /* segregatedFlowEquations.cpp:133-279 */
 int oagpu_update_scales(oagpu_flow *f, double *U_scale, double *p_scale); /* nodeField.hpp:5993 */
@@ -172,6 +184,28 @@ int oagpu_amg_level_copy(oagpu_flow *f, int level, int *agg, int *row_ptr, int *
  * GMRES, p' by the flow's p_method;
  * hist[6] = U, V, W, P control residuals, momentum / p' Krylov iterations. */
 int oagpu_outer_iteration(oagpu_flow *f, const double *ctl, double *hist);
+
+/* ---------------------------------------------------------- k-omega SST */
+/* allocate the SST fields: "k", "omega", "gradK", "gradW", "ymin", "mut",
+ * "F1", "Pk", "Pk_wall", "kprev"; momentum then uses "mu" = mu + mu_t */
+int oagpu_sst_enable(oagpu_flow *f, const oagpu_sst_params *s);
+/* shearStressTransportModel::updateTurbulentDynamicViscosity (:597-690),
+ * turbulenceModel::updateEffectiveDynamicViscosity (turbulenceModel.cpp:121),
+ * updateFOneBlending (:95-174), updateTurbulentProduction (:176-340), fused */
+int oagpu_sst_update(oagpu_flow *f);
+/* k (which 0) / omega (1) system into the scalar system (N = 1): node terms
+ * and SST sources (turbulent*SSTAssemblerNodeTerms.cpp), interior
+ * advection-diffusion (phiAssemblerElemTerms.hpp:10-413), relaxation,
+ * boundary rhs relaxation, omega rows fixed at no-slip walls
+ * (turbulentEddyFrequencySSTAssembler.h postAssemble_) */
+int oagpu_assemble_scalar(oagpu_flow *f, int which);
+/* phi += x clipped below at SMALL, then its scale (equation.h:204-286) */
+int oagpu_correct_scalar(oagpu_flow *f, int which);
+/* one SST pass after the flow equations (segregatedShearStressTransportEquations
+ * preSolve + solve): mu_t, mu_eff, F1, P_k, k and omega assembled and solved
+ * by Jacobi-GMRES(30) (ctl = {max_it, rtol}), corrected, their gradients;
+ * hist[4] = k, omega control residuals, k, omega iterations */
+int oagpu_sst_iteration(oagpu_flow *f, const double *ctl, double *hist);
 
 /* device time of the last launch of a kernel, milliseconds (HIP events):
  * "momentum", "pcorr", "mdot", "gradU" (assembly and field kernels of the last

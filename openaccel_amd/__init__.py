@@ -48,6 +48,22 @@ class FlowParams(C.Structure):
                 ("use_graphs", C.c_int)]
 
 
+class SstParams(C.Structure):
+    _fields_ = [("urf_k", C.c_double), ("urf_w", C.c_double), ("urf_mut", C.c_double),
+                ("k_shifted", C.c_int), ("w_shifted", C.c_int),
+                ("kgrad_shifted", C.c_int), ("wgrad_shifted", C.c_int)]
+
+
+def default_sst(**kw) -> SstParams:
+    """k-omega SST numerics: relaxation 0.8 for k and omega, no eddy-viscosity
+    relaxation, linear_linear interpolation and gradients."""
+    s = SstParams(urf_k=0.8, urf_w=0.8, urf_mut=1.0, k_shifted=1, w_shifted=1,
+                  kgrad_shifted=1, wgrad_shifted=1)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
 class ControlData(C.Structure):
     _fields_ = [("n_iterations", C.c_int), ("solver_initial_res", C.c_double),
                 ("solver_final_res", C.c_double), ("scaled_initial_res", C.c_double * 3)]
@@ -89,6 +105,11 @@ _SIGS = {
     "oagpu_scaled_residual": [C.c_void_p, C.c_int, _dp],
     "oagpu_solve": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                     C.POINTER(ControlData)],
+    "oagpu_sst_enable": [C.c_void_p, C.POINTER(SstParams)],
+    "oagpu_sst_update": [C.c_void_p],
+    "oagpu_assemble_scalar": [C.c_void_p, C.c_int],
+    "oagpu_correct_scalar": [C.c_void_p, C.c_int],
+    "oagpu_sst_iteration": [C.c_void_p, _dp, _dp],
     "oagpu_amg_info": [C.c_void_p, _ip, _ip],
     "oagpu_amg_level_copy": [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp],
     "oagpu_outer_iteration": [C.c_void_p, _dp, _dp],
@@ -205,7 +226,9 @@ class FlowCase:
                                                 C.byref(h)))
         self.h = h
 
-    _comp = {"U": 3, "p": 1, "gradU": 9, "gradP": 3, "du": 3, "div": 1}
+    _comp = {"U": 3, "p": 1, "gradU": 9, "gradP": 3, "du": 3, "div": 1, "mu": 1, "k": 1,
+             "omega": 1, "gradK": 3, "gradW": 3, "ymin": 1, "mut": 1, "F1": 1, "Pk": 1,
+             "Pk_wall": 1, "kprev": 1}
 
     def _size(self, name):
         if name == "mdot":
@@ -257,7 +280,27 @@ class FlowCase:
         _check(load_library().oagpu_mass_divergence(self.h))
 
     def gradient(self, which):
-        _check(load_library().oagpu_gradient(self.h, {"U": 0, "p": 1}[which]))
+        _check(load_library().oagpu_gradient(self.h, {"U": 0, "p": 1, "k": 2, "omega": 3}[which]))
+
+    # k-omega SST
+    def sst_enable(self, sst: SstParams):
+        self._sst = sst
+        _check(load_library().oagpu_sst_enable(self.h, C.byref(sst)))
+
+    def sst_update(self):
+        _check(load_library().oagpu_sst_update(self.h))
+
+    def assemble_scalar(self, which):
+        _check(load_library().oagpu_assemble_scalar(self.h, {"k": 0, "omega": 1}[which]))
+
+    def correct_scalar(self, which):
+        _check(load_library().oagpu_correct_scalar(self.h, {"k": 0, "omega": 1}[which]))
+
+    def sst_iteration(self, ctl):
+        c = np.ascontiguousarray(ctl, dtype=np.float64)
+        h = np.zeros(4)
+        _check(load_library().oagpu_sst_iteration(self.h, _d(c), _d(h)))
+        return h
 
     def correct(self, N, urf):
         _check(load_library().oagpu_correct(self.h, N, urf))

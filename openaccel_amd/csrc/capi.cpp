@@ -55,9 +55,14 @@ double* field_ptr(oagpu::Flow& f, const char* name, size_t* count) {
         size_t c;
     } tab[] = {{"U", &f.U, nn * 3},          {"p", &f.p, nn},       {"gradU", &f.gradU, nn * 9},
                {"gradP", &f.gradP, nn * 3},  {"du", &f.du, nn * 3}, {"mdot", &f.mdot, ne * 12},
-               {"div", &f.div, nn}};
+               {"div", &f.div, nn},          {"mu", &f.mueff, nn},  {"k", &f.k, nn},
+               {"omega", &f.om, nn},          {"gradK", &f.gradK, nn * 3},
+               {"gradW", &f.gradW, nn * 3},  {"ymin", &f.ymin, nn}, {"mut", &f.mut, nn},
+               {"F1", &f.F1, nn},            {"Pk", &f.Pk, nn},     {"Pk_wall", &f.Pk_wall, nn},
+               {"kprev", &f.kprev, nn}};
     for (auto& t : tab)
         if (!std::strcmp(name, t.n)) {
+            OAGPU_REQUIRE(t.b->p, std::string("field ") + name + " not allocated (SST not enabled)");
             *count = t.c;
             return t.b->p;
         }
@@ -243,6 +248,7 @@ int oagpu_flow_create(oagpu_mesh* m, const oagpu_flow_params* prm, int n_bc, con
         for (auto* b : {&f.U, &f.p, &f.gradU, &f.gradP, &f.du, &f.mdot, &f.div, &f.A3, &f.b3, &f.x3,
                         &f.A1, &f.b1, &f.x1})
             b->zero();
+        f.mueff.upload(std::vector<double>(nn, prm->mu));
         OAGPU_HIP(hipStreamCreate(&f.stream));
         OAGPU_HIP(hipEventCreate(&f.ev0));
         OAGPU_HIP(hipEventCreate(&f.ev1));
@@ -385,7 +391,7 @@ int oagpu_mass_divergence(oagpu_flow* h) {
 
 int oagpu_gradient(oagpu_flow* h, int which) {
     return guarded([&] {
-        OAGPU_REQUIRE(h && (which == 0 || which == 1), "bad argument");
+        OAGPU_REQUIRE(h && which >= 0 && which <= 3, "bad argument");
         h->f.ms_grad = timed(h->f, [&] { oagpu::launch_gradient(h->f, which); });
     });
 }
@@ -427,6 +433,84 @@ int oagpu_solve(oagpu_flow* h, int N, int method, int max_it, double rtol, doubl
         OAGPU_REQUIRE(max_it >= 0, "negative iteration limit");
         std::memset(cd, 0, sizeof(*cd));
         oagpu::krylov_solve(h->f, N, method, max_it, rtol, atol, restart, cd);
+    });
+}
+
+int oagpu_sst_enable(oagpu_flow* h, const oagpu_sst_params* sp) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && sp, "null argument");
+        OAGPU_REQUIRE(sp->urf_k > 0 && sp->urf_w > 0 && sp->urf_mut > 0, "relaxation factors must be > 0");
+        oagpu::Flow& f = h->f;
+        const size_t nn = size_t(f.m->nn);
+        f.sst = *sp;
+        if (!f.sst_on) {
+            for (auto* b : {&f.k, &f.om, &f.ymin, &f.mut, &f.F1, &f.Pk, &f.Pk_wall, &f.kprev})
+                b->alloc(nn);
+            f.gradK.alloc(3 * nn);
+            f.gradW.alloc(3 * nn);
+            for (auto* b : {&f.k, &f.om, &f.ymin, &f.mut, &f.F1, &f.Pk, &f.Pk_wall, &f.kprev, &f.gradK,
+                            &f.gradW})
+                b->zero(f.stream);
+            OAGPU_HIP(hipStreamSynchronize(f.stream));
+        }
+        f.sst_on = true;
+        f.k_scale = f.w_scale = 1.0;
+    });
+}
+
+int oagpu_sst_update(oagpu_flow* h) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h, "null flow");
+        oagpu::launch_sst_update(h->f);
+        OAGPU_HIP(hipStreamSynchronize(h->f.stream));
+    });
+}
+
+int oagpu_assemble_scalar(oagpu_flow* h, int which) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h, "null flow");
+        oagpu::launch_assemble_scalar(h->f, which);
+        check_err(h->f);
+    });
+}
+
+int oagpu_correct_scalar(oagpu_flow* h, int which) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && (which == 0 || which == 1), "bad argument");
+        oagpu::launch_correct_scalar(h->f, which);
+    });
+}
+
+// segregatedShearStressTransportEquations preSolve + solve (one pass)
+int oagpu_sst_iteration(oagpu_flow* h, const double* ctl, double* hist) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && ctl, "null argument");
+        oagpu::Flow& f = h->f;
+        OAGPU_REQUIRE(f.sst_on, "SST not enabled on this flow");
+        const double ks0 = f.k_scale, ws0 = f.w_scale;
+        const double kSmall = 2.220446049250313e-16;
+        OAGPU_HIP(hipMemcpyAsync(f.kprev.p, f.k.p, f.k.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                 f.stream));
+        oagpu::launch_sst_update(f);
+        oagpu_control_data cdk{}, cdw{};
+        double rk, rw;
+        oagpu::launch_assemble_scalar(f, 0);
+        oagpu::scaled_residual(f, 1, &rk);
+        oagpu::krylov_solve(f, 1, OAGPU_GMRES, int(ctl[0]), ctl[1], 1e-30, 30, &cdk);
+        oagpu::launch_correct_scalar(f, 0);
+        oagpu::launch_assemble_scalar(f, 1);
+        oagpu::scaled_residual(f, 1, &rw);
+        oagpu::krylov_solve(f, 1, OAGPU_GMRES, int(ctl[0]), ctl[1], 1e-30, 30, &cdw);
+        oagpu::launch_correct_scalar(f, 1);
+        oagpu::launch_gradient(f, 2);
+        oagpu::launch_gradient(f, 3);
+        check_err(f);
+        if (hist) {
+            hist[0] = rk / (ks0 + kSmall);
+            hist[1] = rw / (ws0 + kSmall);
+            hist[2] = cdk.n_iterations;
+            hist[3] = cdw.n_iterations;
+        }
     });
 }
 

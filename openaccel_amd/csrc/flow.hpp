@@ -76,6 +76,13 @@ struct Flow {
     long amg_vcycles = 0;   // V-cycles run (counts launches of the fine kernels)
     Amg amg;
     CgGraph cg_graph;
+    // effective viscosity mu + mu_t per node (mu alone while laminar)
+    DBuf<double> mueff;
+    // k-omega SST state (oagpu_sst_enable)
+    bool sst_on = false;
+    oagpu_sst_params sst{};
+    DBuf<double> k, om, gradK, gradW, ymin, mut, F1, Pk, Pk_wall, kprev;  // om: omega
+    double k_scale = 1.0, w_scale = 1.0;
 };
 
 // kernels.hip
@@ -92,6 +99,10 @@ void field_scales(Flow& f, double* U_scale, double* p_scale);
 void scaled_residual(Flow& f, int N, double* out);
 int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double atol, int restart,
                  oagpu_control_data* cd);
+// k-omega SST (sst.hip, rows.hip)
+void launch_sst_update(Flow& f);               // mu_t, mu_eff, F1, P_k
+void launch_assemble_scalar(Flow& f, int which);  // 0 k, 1 omega -> system N = 1
+void launch_correct_scalar(Flow& f, int which);   // phi += x, clipped; field scale
 // amg.hip
 void amg_build(Flow& f);                       // hierarchy structure (once per mesh)
 template <typename V>

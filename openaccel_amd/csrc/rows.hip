@@ -197,8 +197,9 @@ struct MomArgs {
     const double* gradP;
     const double* div;
     const double* mdot;
+    const double* mueff;
     double rho, mu, dt, urf_inv, bnd_relax;
-    int ugrad_shifted;
+    int ugrad_shifted, us;
     double* vals;
     double* b;
     double* du;
@@ -228,7 +229,14 @@ __device__ __forceinline__ void mom_incidence(const MomArgs& a, int la, int e, u
             m[q] = T(a.mdot[12 * size_t(e) + ip]);
         }
     }
-    const T mu = T(a.mu);
+    // effective viscosity at the ips, velocity shape functions
+    T muq[3] = {0, 0, 0};
+#pragma unroll 1
+    for (int n = 0; n < 8; ++n) {
+        const T mn = T(a.mueff[nd[n]]);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) muq[q] += T(c_hex.N[a.us][ipq[q]][n]) * mn;
+    }
     T rl[3] = {0, 0, 0};
     // upwind advection flux (the deferred correction vanishes for upwind)
 #pragma unroll
@@ -255,6 +263,7 @@ __device__ __forceinline__ void mom_incidence(const MomArgs& a, int la, int e, u
             T g[3];
             gradN_scs(SH, ip, cn, Ji[q], g);
             const T gA = g[0] * A[q][0] + g[1] * A[q][1] + g[2] * A[q][2];
+            const T mu = muq[q];
             for (int i = 0; i < 3; ++i) {
                 T ri = T(0);
                 for (int j = 0; j < 3; ++j) {
@@ -300,7 +309,9 @@ __device__ __forceinline__ void mom_wall(const MomArgs& a, int f, int fk, uint64
     }
     const T amag = sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
     const T nx[3] = {A[0] / amag, A[1] / amag, A[2] / amag};
-    const T mu = T(a.mu);
+    T mu = T(0);
+    for (int q = 0; q < 4; ++q)
+        mu += T(hex8::c_quad_shape[a.us][fk][q]) * T(a.mueff[a.conn[8 * size_t(e) + sn[q]]]);
     const T ub[3] = {T(a.bc_U[3 * bc]), T(a.bc_U[3 * bc + 1]), T(a.bc_U[3 * bc + 2])};
     T rl[3] = {0, 0, 0};
 #pragma unroll 1
@@ -434,6 +445,8 @@ void launch_assemble_momentum(Flow& f) {
     a.urf_inv = 1.0 / f.prm.urf_U;
     a.bnd_relax = f.prm.bnd_relax;
     a.ugrad_shifted = f.prm.ugrad_shifted;
+    a.us = f.prm.u_shifted ? 1 : 0;
+    a.mueff = f.mueff.p;
     a.vals = f.A3.p;
     a.b = f.b3.p;
     a.du = f.du.p;
@@ -620,6 +633,253 @@ void launch_assemble_pcorr(Flow& f) {
     } else {
         if (mixed) k_pcorr_rows<float, 0><<<G, B, 0, f.stream>>>(a);
         else k_pcorr_rows<double, 0><<<G, B, 0, f.stream>>>(a);
+    }
+    OAGPU_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------ k / omega (SST)
+namespace sstc {  // shearStressTransportModel.h:21-37, RANSModel.h:20-22, turbulenceModel.h:20
+constexpr double kSigK1 = 0.85, kSigK2 = 1.0, kSigW1 = 0.5, kSigW2 = 0.856;
+constexpr double kBeta1 = 0.075, kBeta2 = 0.0828, kGam1 = 0.5532, kGam2 = 0.4403;
+constexpr double kA1 = 0.31, kBetaStar = 0.09, kProdLim = 10.0, kCmu = 0.09;
+}  // namespace sstc
+
+struct ScArgs {
+    int nn;
+    const double* xyz;
+    const int* conn;
+    const int* slice_ptr;
+    const int* slice_w;
+    const int* diag_off;
+    const int* inc_la;
+    const uint64_t* slots_la;
+    const int* ext_ptr;
+    const int* ext;
+    const uint64_t* ext_slots;
+    const int* bnode;
+    const double* vol;
+    const double* div;
+    const double* mdot;
+    const double* phi;  // k or omega
+    const double* k;
+    const double* w;
+    const double* kprev;
+    const double* gradU;
+    const double* gradK;
+    const double* gradW;
+    const double* ymin;
+    const double* mut;
+    const double* F1;
+    const double* Pk;
+    double rho, mu, dt, urf_inv, sig1, sig2;
+    int which, us;
+    double* vals;
+    double* b;
+    int* err;
+};
+
+// node terms of the k / omega rows (turbulentKineticEnergySSTAssemblerNodeTerms.cpp:13-128,
+// turbulentEddyFrequencySSTAssemblerNodeTerms.cpp:13-219), steady, FP64
+__device__ void sst_node_terms(const ScArgs& a, int i, double& lhs, double& rhs) {
+    using namespace sstc;
+    const double rho = a.rho, vol = a.vol[i], div = a.div[i], w = a.w[i];
+    const double phi = a.phi[i];
+    double l = rho * vol / a.dt, r = 0.0;
+    if (div < 0) l += -div;
+    r -= -div * phi;
+    if (a.which == 0) {
+        const double Dk = kBetaStar * rho * w * a.k[i];
+        double Pk = a.Pk[i];
+        if (Pk > kProdLim * Dk) Pk = kProdLim * Dk;
+        r += (Pk - Dk) * vol;
+        l += kBetaStar * rho * w * vol;
+    } else {
+        const double mu = a.mu, y = a.ymin[i], kPre = a.kprev[i];
+        double g[9];
+        for (int q = 0; q < 9; ++q) g[q] = a.gradU[9 * size_t(i) + q];
+        double P = 0.0, s = 0.0;
+        for (int p = 0; p < 3; ++p)
+            for (int j = 0; j < 3; ++j) {
+                P += g[3 * p + j] * (g[3 * p + j] + g[3 * j + p]);
+                const double rs = 0.5 * (g[3 * p + j] + g[3 * j + p]);
+                s += rs * rs;
+            }
+        P = fmax(P, 0.0);
+        const double sij = sqrt(2.0 * s);
+        const double arg2 = fmin(fmax(2 * sqrt(kPre) / (kCmu * w * y + kSmall),
+                                      500 * mu / (y * y * rho * w + kSmall)),
+                                 100.0);
+        const double F2 = tanh(arg2 * arg2);
+        P = fmin(P, ((kProdLim * kBetaStar) / kA1) * w * fmax(kA1 * w, F2 * sqrt(2.0) * sij));
+        double cross = 0.0;
+        for (int j = 0; j < 3; ++j) cross += a.gradK[3 * size_t(i) + j] * a.gradW[3 * size_t(i) + j];
+        const double F1 = a.F1[i], om = 1.0 - F1;
+        const double beta = F1 * kBeta1 + om * kBeta2;
+        const double gamma = F1 * kGam1 + om * kGam2;
+        const double sigmaD = 2.0 * om * kSigW2;
+        const double Pw = gamma * rho * P;
+        const double Dw = beta * rho * w * w;
+        const double Sw = sigmaD * rho * cross / (w + kSmall);
+        r += (Pw - Dw + Sw) * vol;
+        l += (2.0 * beta * rho * w + fmax(Sw / (w + kSmall), 0.0)) * vol;
+    }
+    lhs = l;
+    rhs = r;
+}
+
+// element e's upwind advection-diffusion contribution to the scalar row of
+// its local node la (phiAssemblerElemTerms.hpp:233-406 restricted to la's
+// ips; the deferred correction vanishes for upwind advection)
+template <typename T, int SG>
+__device__ __forceinline__ void sc_incidence(const ScArgs& a, int la, int e, uint64_t sl,
+                                             const SellRow& sr, int dof, const double xr[3],
+                                             double& D, double& R, int& bad) {
+    int nd[8];
+    load_conn(a.conn, e, nd);
+    T A[3][3], Ji[3][9], sg[3], Gip[3] = {0, 0, 0}, m[3];
+    int ipq[3];
+    {
+        T c[8][3];
+        gather_rel(a.xyz, nd, xr, c);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int ip = c_hex.node_ip[la][q];
+            ipq[q] = ip;
+            sg[q] = T(c_hex.node_sg[la][q]);
+            area_scs(ip, c, A[q]);
+            jinv_scs(SG, ip, c, Ji[q], bad);
+            m[q] = T(a.mdot[12 * size_t(e) + ip]);
+        }
+    }
+    // Gamma = mu + mu_t sigma(F1) at the nodes, interpolated to the ips
+#pragma unroll 1
+    for (int n = 0; n < 8; ++n) {
+        const double F1 = a.F1[nd[n]];
+        const T G = T(a.mu + a.mut[nd[n]] * (F1 * a.sig1 + (1.0 - F1) * a.sig2));
+#pragma unroll
+        for (int q = 0; q < 3; ++q) Gip[q] += T(c_hex.N[a.us][ipq[q]][n]) * G;
+    }
+    T rl = T(0);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int L = c_hex.lr[ipq[q]][0], Rr = c_hex.lr[ipq[q]][1];
+        const T up = T(a.phi[m[q] > T(0) ? nd[L] : nd[Rr]]);
+        rl -= sg[q] * (m[q] * up);
+    }
+#pragma unroll 1
+    for (int cn = 0; cn < 8; ++cn) {
+        const T pc = T(a.phi[nd[cn]]);
+        T acc = T(0);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int ip = ipq[q];
+            if (cn == c_hex.lr[ip][0]) acc += sg[q] * (T(0.5) * (m[q] + fabs(m[q])));
+            if (cn == c_hex.lr[ip][1]) acc += sg[q] * (T(0.5) * (m[q] - fabs(m[q])));
+            T g[3];
+            gradN_scs(SG, ip, cn, Ji[q], g);
+            T lf = T(0);
+            for (int j = 0; j < 3; ++j) lf += -Gip[q] * g[j] * A[q][j];
+            acc += sg[q] * lf;
+            rl -= sg[q] * (lf * pc);
+        }
+        const int sp = slot_of(sl, cn);
+        if (sp == 0xFF || sp == dof) D += double(acc);
+        else *sell_val<1>(a.vals, sr, sp, 0) += double(acc);
+    }
+    R += double(rl);
+}
+
+// one k / omega row: node terms and sources, interior terms, relaxation
+// (phiAssembler.h:966-1006), boundary rhs relaxation x 0.75
+// (turbulentKineticEnergyAssembler.h:44-48, phiAssembler.h:1008-1152), and
+// for omega the no-slip wall rows reduced to their diagonal with b = 0
+template <typename T, int SG>
+__global__ void __launch_bounds__(256) k_scalar_rows(const ScArgs a) {
+    const int row = xcd_block() * blockDim.x + threadIdx.x;
+    if (row >= a.nn) return;
+    const SellRow sr = sell_row<1>(a.slice_ptr, a.slice_w, row);
+    const int dof = a.diag_off[row];
+    const double xr[3] = {a.xyz[3 * size_t(row)], a.xyz[3 * size_t(row) + 1],
+                          a.xyz[3 * size_t(row) + 2]};
+    int bad = 0;
+    double D, R;
+    sst_node_terms(a, row, D, R);
+#pragma unroll 1
+    for (int s = 0; s < sr.width; ++s)
+        if (s != dof) *sell_val<1>(a.vals, sr, s, 0) = 0.0;
+#pragma unroll 1
+    for (int la = 0; la < 8; ++la) {
+        const size_t o = size_t(la) * a.nn + row;
+        const int e = a.inc_la[o];
+        if (e >= 0) sc_incidence<T, SG>(a, la, e, a.slots_la[o], sr, dof, xr, D, R, bad);
+    }
+#pragma unroll 1
+    for (int k = a.ext_ptr[row]; k < a.ext_ptr[row + 1]; ++k)
+        sc_incidence<T, SG>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, D, R, bad);
+    D *= a.urf_inv;
+    const bool wall = a.bnode[row] != 0;
+    if (wall) R *= 0.75;
+    if (a.which == 1 && wall) {
+        for (int s = 0; s < sr.width; ++s)
+            if (s != dof) *sell_val<1>(a.vals, sr, s, 0) = 0.0;
+        R = 0.0;
+    }
+    *sell_val<1>(a.vals, sr, dof, 0) = D;
+    a.b[row] = R;
+    if (bad) *a.err = 1;
+}
+
+void launch_assemble_scalar(Flow& f, int which) {
+    OAGPU_REQUIRE(f.sst_on, "SST not enabled on this flow");
+    OAGPU_REQUIRE(which == 0 || which == 1, "scalar system: 0 k, 1 omega");
+    DevMesh& m = *f.m;
+    ScArgs a;
+    a.nn = m.nn;
+    a.xyz = m.xyz.p;
+    a.conn = m.conn.p;
+    a.slice_ptr = m.slice_ptr.p;
+    a.slice_w = m.slice_w.p;
+    a.diag_off = m.diag_off.p;
+    a.inc_la = m.inc_la.p;
+    a.slots_la = m.slots_la.p;
+    a.ext_ptr = m.ext_ptr.p;
+    a.ext = m.ext.p;
+    a.ext_slots = m.ext_slots.p;
+    a.bnode = m.bnode.p;
+    a.vol = m.vol.p;
+    a.div = f.div.p;
+    a.mdot = f.mdot.p;
+    a.phi = which == 0 ? f.k.p : f.om.p;
+    a.k = f.k.p;
+    a.w = f.om.p;
+    a.kprev = f.kprev.p;
+    a.gradU = f.gradU.p;
+    a.gradK = f.gradK.p;
+    a.gradW = f.gradW.p;
+    a.ymin = f.ymin.p;
+    a.mut = f.mut.p;
+    a.F1 = f.F1.p;
+    a.Pk = f.Pk.p;
+    a.rho = f.prm.rho;
+    a.mu = f.prm.mu;
+    a.dt = f.prm.dt;
+    a.urf_inv = 1.0 / (which == 0 ? f.sst.urf_k : f.sst.urf_w);
+    a.sig1 = which == 0 ? sstc::kSigK1 : sstc::kSigW1;
+    a.sig2 = which == 0 ? sstc::kSigK2 : sstc::kSigW2;
+    a.which = which;
+    a.us = (which == 0 ? f.sst.k_shifted : f.sst.w_shifted) ? 1 : 0;
+    a.vals = f.A1.p;
+    a.b = f.b1.p;
+    a.err = m.err.p;
+    const unsigned B = 256, G = xcd_grid(m.nn, B);
+    const bool mixed = f.prm.precision == OAGPU_MIXED;
+    const bool gs = (which == 0 ? f.sst.kgrad_shifted : f.sst.wgrad_shifted) != 0;
+    if (gs) {
+        if (mixed) k_scalar_rows<float, 1><<<G, B, 0, f.stream>>>(a);
+        else k_scalar_rows<double, 1><<<G, B, 0, f.stream>>>(a);
+    } else {
+        if (mixed) k_scalar_rows<float, 0><<<G, B, 0, f.stream>>>(a);
+        else k_scalar_rows<double, 0><<<G, B, 0, f.stream>>>(a);
     }
     OAGPU_HIP(hipGetLastError());
 }
@@ -831,10 +1091,21 @@ void launch_gradient(Flow& f, int which) {
         k_gradient<3><<<xcd_grid(m.nn, B), B, 0, f.stream>>>(
             m.nn, m.xyz.p, m.conn.p, m.inc_la.p, m.slots_la.p, m.ext_ptr.p, m.ext.p,
             m.ext_slots.p, m.binc_ptr.p, m.binc.p, m.bf_elem.p, m.bf_ord.p, m.vol.p, f.U.p, f.prm.u_shifted, f.prm.grad_urf, f.gradU.p);
-    else
+    else {
+        const double* phi = f.p.p;
+        double* grad = f.gradP.p;
+        int sh = f.prm.p_shifted;
+        if (which == 2 || which == 3) {
+            OAGPU_REQUIRE(f.sst_on, "SST not enabled on this flow");
+            phi = which == 2 ? f.k.p : f.om.p;
+            grad = which == 2 ? f.gradK.p : f.gradW.p;
+            sh = which == 2 ? f.sst.k_shifted : f.sst.w_shifted;
+        }
         k_gradient<1><<<xcd_grid(m.nn, B), B, 0, f.stream>>>(
             m.nn, m.xyz.p, m.conn.p, m.inc_la.p, m.slots_la.p, m.ext_ptr.p, m.ext.p,
-            m.ext_slots.p, m.binc_ptr.p, m.binc.p, m.bf_elem.p, m.bf_ord.p, m.vol.p, f.p.p, f.prm.p_shifted, f.prm.grad_urf, f.gradP.p);
+            m.ext_slots.p, m.binc_ptr.p, m.binc.p, m.bf_elem.p, m.bf_ord.p, m.vol.p, phi, sh,
+            f.prm.grad_urf, grad);
+    }
     OAGPU_HIP(hipGetLastError());
 }
 
