@@ -22,6 +22,8 @@
 #include "device_util.hpp"
 #include "tables.hpp"
 
+#include <type_traits>
+
 namespace oagpu {
 
 // hex8 master-element constants (Hex8CVFEM.C ip tables, MasterElementWork.F
@@ -159,16 +161,58 @@ __device__ __forceinline__ void gather_rel(const double* xyz, const int nd[8], c
     }
 }
 
-template <typename T>
+// Row slots whose off-diagonal 3x3 blocks are summed in the wavefront's LDS
+// tile (a reduced-stencil hex row has at most 7 slots); further slots of
+// longer rows are read-modify-written in HBM.
+constexpr int kLdsSlots = 7;
+
+// LT: type of the LDS partial sums (lt = this lane's tile column, stride 64
+// per value) or void (all off-diagonal blocks read-modify-written in HBM)
+template <typename T, typename LT>
 __device__ __forceinline__ void scatter_block(double* __restrict__ vals, const SellRow& sr, int dof,
-                                              uint64_t sl, int cn, const T acc[9], double D[9]) {
+                                              uint64_t sl, int cn, const T acc[9], double D[9],
+                                              LT* lt) {
     const int sp = slot_of(sl, cn);
     if (sp == 0xFF || sp == dof) {
         for (int k = 0; k < 9; ++k) D[k] += double(acc[k]);
-    } else {
-        double* v = sell_val<9>(vals, sr, sp, 0);
-        for (int k = 0; k < 9; ++k) v[64 * k] += double(acc[k]);
+        return;
     }
+    if constexpr (!std::is_void_v<LT>) {
+        if (sp < kLdsSlots) {
+            LT* t = lt + sp * 9 * 64;
+            for (int k = 0; k < 9; ++k) t[64 * k] += LT(acc[k]);
+            return;
+        }
+    }
+    double* v = sell_val<9>(vals, sr, sp, 0);
+    for (int k = 0; k < 9; ++k) v[64 * k] += double(acc[k]);
+}
+
+// Scalar rows: slots < kLdsSlots accumulate in the wavefront's LDS tile
+// (lt = this lane's column, stride 64), longer rows in HBM.
+__device__ __forceinline__ void scalar_scatter(double* vals, const SellRow& sr, double* lt, int sp,
+                                               double v) {
+    if (sp < kLdsSlots) lt[sp * 64] += v;
+    else *sell_val<1>(vals, sr, sp, 0) += v;
+}
+
+// zero the off-diagonal slots of a scalar row (LDS tile and HBM)
+__device__ __forceinline__ void scalar_row_zero(double* vals, const SellRow& sr, double* lt, int dof) {
+#pragma unroll 1
+    for (int s = 0; s < sr.width; ++s) {
+        if (s == dof) continue;
+        if (s < kLdsSlots) lt[s * 64] = 0.0;
+        else *sell_val<1>(vals, sr, s, 0) = 0.0;
+    }
+}
+
+// store the LDS part of a scalar row's off-diagonal slots
+__device__ __forceinline__ void scalar_row_store(double* vals, const SellRow& sr, const double* lt,
+                                                 int dof) {
+    const int ns = sr.width < kLdsSlots ? sr.width : kLdsSlots;
+#pragma unroll 1
+    for (int s = 0; s < ns; ++s)
+        if (s != dof) *sell_val<1>(vals, sr, s, 0) = lt[s * 64];
 }
 
 // --------------------------------------------------------------- momentum
@@ -208,10 +252,10 @@ struct MomArgs {
 
 // element e's contribution to the momentum row of its local node la
 // (navierStokesAssemblerElemTerms.cpp:256-435 restricted to la's ips)
-template <typename T, int SH>
+template <typename T, int SH, typename LT>
 __device__ __forceinline__ void mom_incidence(const MomArgs& a, int la, int e, uint64_t sl,
-                                              const SellRow& sr, int dof, const double xr[3], double D[9], double R[3],
-                                              int& bad) {
+                                              const SellRow& sr, int dof, const double xr[3],
+                                              double D[9], double R[3], int& bad, LT* lt) {
     int nd[8];
     load_conn(a.conn, e, nd);
     T A[3][3], Ji[3][9], m[3], sg[3];
@@ -229,13 +273,17 @@ __device__ __forceinline__ void mom_incidence(const MomArgs& a, int la, int e, u
             m[q] = T(a.mdot[12 * size_t(e) + ip]);
         }
     }
-    // effective viscosity at the ips, velocity shape functions
-    T muq[3] = {0, 0, 0};
+    // effective viscosity at the ips, velocity shape functions (a constant
+    // viscosity needs no interpolation: the shape functions sum to one)
+    T muq[3] = {T(a.mu), T(a.mu), T(a.mu)};
+    if (a.mueff) {
+        muq[0] = muq[1] = muq[2] = T(0);
 #pragma unroll 1
-    for (int n = 0; n < 8; ++n) {
-        const T mn = T(a.mueff[nd[n]]);
+        for (int n = 0; n < 8; ++n) {
+            const T mn = T(a.mueff[nd[n]]);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) muq[q] += T(c_hex.N[a.us][ipq[q]][n]) * mn;
+            for (int q = 0; q < 3; ++q) muq[q] += T(c_hex.N[a.us][ipq[q]][n]) * mn;
+        }
     }
     T rl[3] = {0, 0, 0};
     // upwind advection flux (the deferred correction vanishes for upwind)
@@ -275,7 +323,7 @@ __device__ __forceinline__ void mom_incidence(const MomArgs& a, int la, int e, u
                 rl[i] -= sg[q] * ri;
             }
         }
-        scatter_block(a.vals, sr, dof, sl, cn, acc, D);
+        scatter_block(a.vals, sr, dof, sl, cn, acc, D, lt);
     }
     for (int i = 0; i < 3; ++i) R[i] += double(rl[i]);
 }
@@ -283,10 +331,10 @@ __device__ __forceinline__ void mom_incidence(const MomArgs& a, int la, int e, u
 // laminar no-slip wall: the face ip of side (f, fk) nearest to this row
 // (navierStokesAssemblerElemBoundaryConditions.cpp:2659-2884,
 // IP_ZERO_NORMAL_STRESS_FIXED_VEL__ :592-654)
-template <typename T>
+template <typename T, typename LT>
 __device__ __forceinline__ void mom_wall(const MomArgs& a, int f, int fk, uint64_t sl,
-                                         const SellRow& sr, int dof, const double xr[3], double D[9], double R[3],
-                                         int& bad) {
+                                         const SellRow& sr, int dof, const double xr[3],
+                                         double D[9], double R[3], int& bad, LT* lt) {
     const int e = a.bf_elem[f], ord = a.bf_ord[f], bc = a.bf_bc[f];
     const int* sn = hex8::c_side_nodes[ord];
     // face ip location: exposed-face ip, or the side node itself when shifted
@@ -309,9 +357,12 @@ __device__ __forceinline__ void mom_wall(const MomArgs& a, int f, int fk, uint64
     }
     const T amag = sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
     const T nx[3] = {A[0] / amag, A[1] / amag, A[2] / amag};
-    T mu = T(0);
-    for (int q = 0; q < 4; ++q)
-        mu += T(hex8::c_quad_shape[a.us][fk][q]) * T(a.mueff[a.conn[8 * size_t(e) + sn[q]]]);
+    T mu = T(a.mu);
+    if (a.mueff) {
+        mu = T(0);
+        for (int q = 0; q < 4; ++q)
+            mu += T(hex8::c_quad_shape[a.us][fk][q]) * T(a.mueff[a.conn[8 * size_t(e) + sn[q]]]);
+    }
     const T ub[3] = {T(a.bc_U[3 * bc]), T(a.bc_U[3 * bc + 1]), T(a.bc_U[3 * bc + 2])};
     T rl[3] = {0, 0, 0};
 #pragma unroll 1
@@ -348,7 +399,7 @@ __device__ __forceinline__ void mom_wall(const MomArgs& a, int f, int fk, uint64
                 }
             }
         }
-        scatter_block(a.vals, sr, dof, sl, ic, acc, D);
+        scatter_block(a.vals, sr, dof, sl, ic, acc, D, lt);
     }
     for (int i = 0; i < 3; ++i) R[i] += double(rl[i]);
 }
@@ -357,8 +408,16 @@ __device__ __forceinline__ void mom_wall(const MomArgs& a, int f, int fk, uint64
 // interior element terms, no-slip walls, relaxation (phiAssembler.h:966-1006),
 // boundary rhs relaxation (phiAssembler.h:1008-1152, navierStokesAssembler.cpp:238)
 // and du = V/(a_ii + SMALL) (navierStokesAssembler.cpp:188-190)
-template <typename T, int SH>
-__global__ void __launch_bounds__(256) k_momentum_rows(const MomArgs a) {
+// LDS: 0 = off-diagonal blocks read-modify-written in HBM (256-lane blocks);
+// 1 / 2 = summed in a per-wavefront LDS tile in float / double and written
+// once (64-lane blocks, one wavefront each).
+template <typename T, int SH, int LDS>
+__global__ void __launch_bounds__(LDS ? 64 : 256) k_momentum_rows(const MomArgs a) {
+    using LT = std::conditional_t<LDS == 0, void, std::conditional_t<LDS == 2, double, float>>;
+    using LS = std::conditional_t<LDS == 2, double, float>;
+    __shared__ LS tile[LDS ? kLdsSlots * 9 * 64 : 1];
+    LT* lt = nullptr;
+    if constexpr (LDS != 0) lt = tile + threadIdx.x;
     const int row = xcd_block() * blockDim.x + threadIdx.x;
     if (row >= a.nn) return;
     const SellRow sr = sell_row<9>(a.slice_ptr, a.slice_w, row);
@@ -383,24 +442,40 @@ __global__ void __launch_bounds__(256) k_momentum_rows(const MomArgs a) {
     }
     // off-diagonal slots (and the slice's padding slots) start from zero
 #pragma unroll 1
-    for (int s = 0; s < sr.width; ++s)
-        if (s != dof)
-            for (int k = 0; k < 9; ++k) *sell_val<9>(a.vals, sr, s, k) = 0.0;
+    for (int s = 0; s < sr.width; ++s) {
+        if (s == dof) continue;
+        if constexpr (LDS != 0) {
+            if (s < kLdsSlots) {
+                for (int k = 0; k < 9; ++k) lt[(s * 9 + k) * 64] = LS(0);
+                continue;
+            }
+        }
+        for (int k = 0; k < 9; ++k) *sell_val<9>(a.vals, sr, s, k) = 0.0;
+    }
     // local node la is the (scalar, wave-uniform) loop counter; the rare extra
     // incidences of unstructured nodes follow in a second loop
 #pragma unroll 1
     for (int la = 0; la < 8; ++la) {
         const size_t o = size_t(la) * a.nn + row;
         const int e = a.inc_la[o];
-        if (e >= 0) mom_incidence<T, SH>(a, la, e, a.slots_la[o], sr, dof, xr, D, R, bad);
+        if (e >= 0) mom_incidence<T, SH>(a, la, e, a.slots_la[o], sr, dof, xr, D, R, bad, lt);
     }
 #pragma unroll 1
     for (int k = a.ext_ptr[row]; k < a.ext_ptr[row + 1]; ++k)
-        mom_incidence<T, SH>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, D, R, bad);
+        mom_incidence<T, SH>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, D, R, bad,
+                             lt);
 #pragma unroll 1
     for (int k = a.binc_ptr[row]; k < a.binc_ptr[row + 1]; ++k) {
         const int pk = a.binc[k];
-        mom_wall<T>(a, pk >> 2, pk & 3, a.binc_slots[k], sr, dof, xr, D, R, bad);
+        mom_wall<T>(a, pk >> 2, pk & 3, a.binc_slots[k], sr, dof, xr, D, R, bad, lt);
+    }
+    if constexpr (LDS != 0) {
+        const int ns = sr.width < kLdsSlots ? sr.width : kLdsSlots;
+#pragma unroll 1
+        for (int s = 0; s < ns; ++s)
+            if (s != dof)
+                for (int k = 0; k < 9; ++k)
+                    *sell_val<9>(a.vals, sr, s, k) = double(lt[(s * 9 + k) * 64]);
     }
     for (int k = 0; k < 3; ++k) D[4 * k] *= a.urf_inv;
     if (a.bnode[row])
@@ -446,19 +521,25 @@ void launch_assemble_momentum(Flow& f) {
     a.bnd_relax = f.prm.bnd_relax;
     a.ugrad_shifted = f.prm.ugrad_shifted;
     a.us = f.prm.u_shifted ? 1 : 0;
-    a.mueff = f.mueff.p;
+    a.mueff = f.sst_on ? f.mueff.p : nullptr;  // laminar: the constant mu
     a.vals = f.A3.p;
     a.b = f.b3.p;
     a.du = f.du.p;
     a.err = m.err.p;
-    const unsigned B = 256, G = xcd_grid(m.nn, B);
+    // off-diagonal blocks summed in a per-wavefront LDS tile: double in the
+    // FP64 form, float in the mixed form (the element contributions are FP32
+    // there; the at most four of them per block are summed in FP32 before the
+    // single FP64 store)
     const bool mixed = f.prm.precision == OAGPU_MIXED;
+    auto go = [&](auto kern, unsigned B) {
+        kern<<<xcd_grid(m.nn, B), B, 0, f.stream>>>(a);
+    };
     if (a.ugrad_shifted) {
-        if (mixed) k_momentum_rows<float, 1><<<G, B, 0, f.stream>>>(a);
-        else k_momentum_rows<double, 1><<<G, B, 0, f.stream>>>(a);
+        if (mixed) go(k_momentum_rows<float, 1, 1>, 64);
+        else go(k_momentum_rows<double, 1, 2>, 64);
     } else {
-        if (mixed) k_momentum_rows<float, 0><<<G, B, 0, f.stream>>>(a);
-        else k_momentum_rows<double, 0><<<G, B, 0, f.stream>>>(a);
+        if (mixed) go(k_momentum_rows<float, 0, 1>, 64);
+        else go(k_momentum_rows<double, 0, 2>, 64);
     }
     OAGPU_HIP(hipGetLastError());
 }
@@ -492,8 +573,9 @@ struct PcArgs {
 // (pressureCorrectionAssemblerElemTerms.cpp:390-610 restricted to la's ips)
 template <typename T, int SP>
 __device__ __forceinline__ void pc_incidence(const PcArgs& a, int la, int e, uint64_t sl,
-                                             const SellRow& sr, int dof, const double xr[3], double pr, double& D,
-                                             double& R, int& bad) {
+                                             const SellRow& sr, int dof, const double xr[3],
+                                             double pr, double& D, double& R, int& bad,
+                                             double* lt) {
     int nd[8];
     load_conn(a.conn, e, nd);
     T A[3][3], Ji[3][9], sg[3];
@@ -559,7 +641,7 @@ __device__ __forceinline__ void pc_incidence(const PcArgs& a, int la, int e, uin
         }
         const int sp = slot_of(sl, n);
         if (sp == 0xFF || sp == dof) D += double(acc);
-        else *sell_val<1>(a.vals, sr, sp, 0) += double(acc);
+        else scalar_scatter(a.vals, sr, lt, sp, double(acc));
     }
     R += double(rl);
 }
@@ -567,7 +649,9 @@ __device__ __forceinline__ void pc_incidence(const PcArgs& a, int la, int e, uin
 // pressureCorrectionAssembler interior terms for one row, with the
 // pressure-reference row (pressureCorrectionAssembler.cpp:19-63)
 template <typename T, int SP>
-__global__ void __launch_bounds__(256) k_pcorr_rows(const PcArgs a) {
+__global__ void __launch_bounds__(64) k_pcorr_rows(const PcArgs a) {
+    __shared__ double tile[kLdsSlots * 64];
+    double* lt = tile + threadIdx.x;
     const int row = xcd_block() * blockDim.x + threadIdx.x;
     if (row >= a.nn) return;
     const SellRow sr = sell_row<1>(a.slice_ptr, a.slice_w, row);
@@ -577,24 +661,22 @@ __global__ void __launch_bounds__(256) k_pcorr_rows(const PcArgs a) {
     const double pr = a.p[row];
     int bad = 0;
     double D = 0.0, R = 0.0;
-#pragma unroll 1
-    for (int s = 0; s < sr.width; ++s)
-        if (s != dof) *sell_val<1>(a.vals, sr, s, 0) = 0.0;
+    scalar_row_zero(a.vals, sr, lt, dof);
 #pragma unroll 1
     for (int la = 0; la < 8; ++la) {
         const size_t o = size_t(la) * a.nn + row;
         const int e = a.inc_la[o];
-        if (e >= 0) pc_incidence<T, SP>(a, la, e, a.slots_la[o], sr, dof, xr, pr, D, R, bad);
+        if (e >= 0) pc_incidence<T, SP>(a, la, e, a.slots_la[o], sr, dof, xr, pr, D, R, bad, lt);
     }
 #pragma unroll 1
     for (int k = a.ext_ptr[row]; k < a.ext_ptr[row + 1]; ++k)
         pc_incidence<T, SP>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, pr, D, R,
-                            bad);
+                            bad, lt);
     if (row == a.p_ref) {
-        for (int s = 0; s < sr.width; ++s)
-            if (s != dof) *sell_val<1>(a.vals, sr, s, 0) = 0.0;
+        scalar_row_zero(a.vals, sr, lt, dof);
         R = 0.0;
     }
+    scalar_row_store(a.vals, sr, lt, dof);
     *sell_val<1>(a.vals, sr, dof, 0) = D;
     a.b[row] = R;
     if (bad) *a.err = 1;
@@ -625,7 +707,7 @@ void launch_assemble_pcorr(Flow& f) {
     a.vals = f.A1.p;
     a.b = f.b1.p;
     a.err = m.err.p;
-    const unsigned B = 256, G = xcd_grid(m.nn, B);
+    const unsigned B = 64, G = xcd_grid(m.nn, B);
     const bool mixed = f.prm.precision == OAGPU_MIXED;
     if (f.prm.pgrad_shifted) {
         if (mixed) k_pcorr_rows<float, 1><<<G, B, 0, f.stream>>>(a);
@@ -733,7 +815,7 @@ __device__ void sst_node_terms(const ScArgs& a, int i, double& lhs, double& rhs)
 template <typename T, int SG>
 __device__ __forceinline__ void sc_incidence(const ScArgs& a, int la, int e, uint64_t sl,
                                              const SellRow& sr, int dof, const double xr[3],
-                                             double& D, double& R, int& bad) {
+                                             double& D, double& R, int& bad, double* lt) {
     int nd[8];
     load_conn(a.conn, e, nd);
     T A[3][3], Ji[3][9], sg[3], Gip[3] = {0, 0, 0}, m[3];
@@ -784,7 +866,7 @@ __device__ __forceinline__ void sc_incidence(const ScArgs& a, int la, int e, uin
         }
         const int sp = slot_of(sl, cn);
         if (sp == 0xFF || sp == dof) D += double(acc);
-        else *sell_val<1>(a.vals, sr, sp, 0) += double(acc);
+        else scalar_scatter(a.vals, sr, lt, sp, double(acc));
     }
     R += double(rl);
 }
@@ -794,7 +876,9 @@ __device__ __forceinline__ void sc_incidence(const ScArgs& a, int la, int e, uin
 // (turbulentKineticEnergyAssembler.h:44-48, phiAssembler.h:1008-1152), and
 // for omega the no-slip wall rows reduced to their diagonal with b = 0
 template <typename T, int SG>
-__global__ void __launch_bounds__(256) k_scalar_rows(const ScArgs a) {
+__global__ void __launch_bounds__(64) k_scalar_rows(const ScArgs a) {
+    __shared__ double tile[kLdsSlots * 64];
+    double* lt = tile + threadIdx.x;
     const int row = xcd_block() * blockDim.x + threadIdx.x;
     if (row >= a.nn) return;
     const SellRow sr = sell_row<1>(a.slice_ptr, a.slice_w, row);
@@ -804,26 +888,25 @@ __global__ void __launch_bounds__(256) k_scalar_rows(const ScArgs a) {
     int bad = 0;
     double D, R;
     sst_node_terms(a, row, D, R);
-#pragma unroll 1
-    for (int s = 0; s < sr.width; ++s)
-        if (s != dof) *sell_val<1>(a.vals, sr, s, 0) = 0.0;
+    scalar_row_zero(a.vals, sr, lt, dof);
 #pragma unroll 1
     for (int la = 0; la < 8; ++la) {
         const size_t o = size_t(la) * a.nn + row;
         const int e = a.inc_la[o];
-        if (e >= 0) sc_incidence<T, SG>(a, la, e, a.slots_la[o], sr, dof, xr, D, R, bad);
+        if (e >= 0) sc_incidence<T, SG>(a, la, e, a.slots_la[o], sr, dof, xr, D, R, bad, lt);
     }
 #pragma unroll 1
     for (int k = a.ext_ptr[row]; k < a.ext_ptr[row + 1]; ++k)
-        sc_incidence<T, SG>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, D, R, bad);
+        sc_incidence<T, SG>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, D, R, bad,
+                            lt);
     D *= a.urf_inv;
     const bool wall = a.bnode[row] != 0;
     if (wall) R *= 0.75;
     if (a.which == 1 && wall) {
-        for (int s = 0; s < sr.width; ++s)
-            if (s != dof) *sell_val<1>(a.vals, sr, s, 0) = 0.0;
+        scalar_row_zero(a.vals, sr, lt, dof);
         R = 0.0;
     }
+    scalar_row_store(a.vals, sr, lt, dof);
     *sell_val<1>(a.vals, sr, dof, 0) = D;
     a.b[row] = R;
     if (bad) *a.err = 1;
@@ -871,7 +954,7 @@ void launch_assemble_scalar(Flow& f, int which) {
     a.vals = f.A1.p;
     a.b = f.b1.p;
     a.err = m.err.p;
-    const unsigned B = 256, G = xcd_grid(m.nn, B);
+    const unsigned B = 64, G = xcd_grid(m.nn, B);
     const bool mixed = f.prm.precision == OAGPU_MIXED;
     const bool gs = (which == 0 ? f.sst.kgrad_shifted : f.sst.wgrad_shifted) != 0;
     if (gs) {
