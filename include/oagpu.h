@@ -39,6 +39,13 @@ extern "C" {
 #define OAGPU_STENCIL_REDUCED 1
 #define OAGPU_STENCIL_FULL 0
 
+/* element topologies (nalu node orders of Hex8CVFEM.C, Tet4CVFEM.C,
+ * Wed6CVFEM.C, Pyr5CVFEM.C) */
+#define OAGPU_HEX8 0
+#define OAGPU_TET4 1
+#define OAGPU_WED6 2
+#define OAGPU_PYR5 3
+
 /* arithmetic form: FP64 everywhere, or FP32 geometry/local matrices summed
  * into FP64 global coefficients */
 #define OAGPU_FP64 0
@@ -124,6 +131,15 @@ int oagpu_scatter_map_copy(const oagpu_mesh *m, int *inc_ptr, int *inc,
                            unsigned long long *slots);
 /* dual nodal volumes [n_nodes] */
 int oagpu_mesh_volume_copy(const oagpu_mesh *m, double *vol);
+
+/* Master-element geometry of n_elems elements of one topology on the device
+ * (MasterElementWork.F *_scs_det, *_scv_det, *_gradient_operator with the
+ * ip tables of the *CVFEM.C master elements): coords [n][npe][3] in,
+ * areav [n][nscs][3], scv [n][npe], dndx [n][nscs][npe][3] at the standard
+ * (shifted = 0) or shifted SCS ips, detj [n][nscs] out (NULL: not wanted).
+ * npe / nscs: hex8 8 / 12, tet4 4 / 6, wedge6 6 / 9, pyramid5 5 / 12. */
+int oagpu_element_geometry(int topology, int shifted, int n_elems, const double *coords,
+                           double *areav, double *scv, double *dndx, double *detj);
 
 /* --------------------------------------------------------------- flow */
 /* bc_U[n_bc][3]: wall velocity of each boundary id (no-slip walls). */

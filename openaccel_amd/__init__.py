@@ -33,6 +33,8 @@ LIB_PATH = os.path.join(HERE, "liboagpu.so")
 OAGPU_FP64, OAGPU_MIXED = 0, 1
 OAGPU_STENCIL_FULL, OAGPU_STENCIL_REDUCED = 0, 1
 OAGPU_GMRES, OAGPU_CG, OAGPU_BICGSTAB, OAGPU_CG_AMG = 0, 1, 2, 3
+OAGPU_HEX8, OAGPU_TET4, OAGPU_WED6, OAGPU_PYR5 = 0, 1, 2, 3
+TOPOLOGY_SIZES = {OAGPU_HEX8: (8, 12), OAGPU_TET4: (4, 6), OAGPU_WED6: (6, 9), OAGPU_PYR5: (5, 12)}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -105,6 +107,7 @@ _SIGS = {
     "oagpu_scaled_residual": [C.c_void_p, C.c_int, _dp],
     "oagpu_solve": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                     C.POINTER(ControlData)],
+    "oagpu_element_geometry": [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp],
     "oagpu_sst_enable": [C.c_void_p, C.POINTER(SstParams)],
     "oagpu_sst_update": [C.c_void_p],
     "oagpu_assemble_scalar": [C.c_void_p, C.c_int],
@@ -152,6 +155,21 @@ def _i(a):
 
 def init(device: int = 0):
     _check(load_library().oagpu_init(device))
+
+
+def element_geometry(topology: int, coords, shifted: int = 0):
+    """Device master-element geometry of a batch of elements of one topology:
+    (areav [n][nscs][3], scv [n][npe], dndx [n][nscs][npe][3], detj [n][nscs])."""
+    npe, ns = TOPOLOGY_SIZES[topology]
+    c = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, npe, 3)
+    n = c.shape[0]
+    A = np.zeros((n, ns, 3))
+    V = np.zeros((n, npe))
+    G = np.zeros((n, ns, npe, 3))
+    D = np.zeros((n, ns))
+    _check(load_library().oagpu_element_geometry(topology, shifted, n, _d(c), _d(A), _d(V), _d(G),
+                                                 _d(D)))
+    return A, V, G, D
 
 
 def default_params(precision: int = OAGPU_FP64, **kw) -> FlowParams:

@@ -436,6 +436,34 @@ int oagpu_solve(oagpu_flow* h, int N, int method, int max_it, double rtol, doubl
     });
 }
 
+int oagpu_element_geometry(int topology, int shifted, int n_elems, const double* coords,
+                           double* areav, double* scv, double* dndx, double* detj) {
+    return guarded([&] {
+        OAGPU_REQUIRE(topology >= OAGPU_HEX8 && topology <= OAGPU_PYR5, "unknown topology");
+        OAGPU_REQUIRE(n_elems >= 0 && (coords || n_elems == 0), "bad element batch");
+        if (n_elems == 0) return;
+        const size_t n = size_t(n_elems);
+        const size_t npe = oagpu::topology_npe(topology), ns = oagpu::topology_nscs(topology);
+        oagpu::DBuf<double> c(n * npe * 3), A, V, G, D;
+        oagpu::DBuf<int> err(1);
+        err.zero();
+        c.upload(coords, n * npe * 3);
+        if (areav) A.alloc(n * ns * 3);
+        if (scv) V.alloc(n * npe);
+        if (dndx) G.alloc(n * ns * npe * 3);
+        if (detj) D.alloc(n * ns);
+        oagpu::element_geometry(topology, shifted ? 1 : 0, n_elems, c.p, A.p, V.p, G.p, D.p, err.p, 0);
+        int e = 0;
+        err.download(&e, 1);
+        if (areav) A.download(areav, A.n);
+        if (scv) V.download(scv, V.n);
+        if (dndx) G.download(dndx, G.n);
+        if (detj) D.download(detj, D.n);
+        OAGPU_HIP(hipDeviceSynchronize());
+        if (e) throw oagpu::Error(OAGPU_EGEOM, "non-positive Jacobian determinant in a kernel");
+    });
+}
+
 int oagpu_sst_enable(oagpu_flow* h, const oagpu_sst_params* sp) {
     return guarded([&] {
         OAGPU_REQUIRE(h && sp, "null argument");

@@ -103,6 +103,12 @@ int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double ato
 void launch_sst_update(Flow& f);               // mu_t, mu_eff, F1, P_k
 void launch_assemble_scalar(Flow& f, int which);  // 0 k, 1 omega -> system N = 1
 void launch_correct_scalar(Flow& f, int which);   // phi += x, clipped; field scale
+// topo.hip: master-element geometry of a batch of elements of one topology
+// (0 hex8, 1 tet4, 2 wedge6, 3 pyramid5), device arrays
+int topology_npe(int topo);
+int topology_nscs(int topo);
+void element_geometry(int topo, int shifted, int n, const double* coords, double* A, double* V,
+                      double* G, double* detj, int* err, hipStream_t s);
 // amg.hip
 void amg_build(Flow& f);                       // hierarchy structure (once per mesh)
 template <typename V>

@@ -58,6 +58,26 @@ void or_quad_face_shape(int shifted, double *shape);
 const int *or_hex_lrscv(void);          /* [12][2] */
 const int *or_hex_side_nodes(void);     /* [6][4]  */
 
+/* tet4 / wedge6 / pyramid5 master elements (topo.c); topo = OR_* above.
+ * Vertex order, SCS ips and L/R tables are those of Tet4CVFEM.C, Wed6CVFEM.C,
+ * Pyr5CVFEM.C; OR_HEX8 routes to the hex8 functions. */
+int or_topo_npe(int topo);
+int or_topo_nscs(int topo);
+const int *or_topo_lrscv(int topo);                       /* [nscs][2] */
+const double *or_topo_scs_ips(int topo, int shifted);     /* [nscs][3] */
+/* tet_scs_det / wed_scs_det / pyr_scs_det: area vectors [nscs][3] */
+void or_topo_scs_areav(int topo, const double *coords, double *areav);
+/* tet_scv_det / wed_scv_det / pyr_scv_det: SCV volumes [npe] */
+void or_topo_scv_vol(int topo, const double *coords, double *vol);
+/* tet_derivative, wedge_derivative, pyr_derivative / shifted_pyr_derivative */
+void or_topo_deriv(int topo, int shifted, int npts, const double *par, double *deriv);
+/* *_gradient_operator for any npe; returns the count of det_j <= 1e6*realmin */
+int or_topo_gradop(int npe, int npts, const double *deriv, const double *coords,
+                   double *gradop, double *detj);
+/* SCS gradient operator [nscs][npe][3] at the standard or shifted ips */
+int or_topo_scs_gradop(int topo, int shifted, const double *coords, double *gradop,
+                       double *detj);
+
 /* ------------------------------------------------------------- CRS graph */
 /* nodeGraph::buildGraph_ (mesh/graph/nodeGraph.cpp:29-392): reduced stencil
  * (SCS-ip pairs, :68-96) when reduced != 0, else full element stencil

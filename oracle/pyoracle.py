@@ -111,6 +111,19 @@ def lib():
         L.or_quad_face_areav.argtypes = [_dp, _dp]
         L.or_quad_face_shape.argtypes = [C.c_int, _dp]
         L.or_hex_lrscv.restype = _ip
+        L.or_topo_npe.argtypes = [C.c_int]
+        L.or_topo_nscs.argtypes = [C.c_int]
+        L.or_topo_lrscv.argtypes = [C.c_int]
+        L.or_topo_lrscv.restype = _ip
+        L.or_topo_scs_ips.argtypes = [C.c_int, C.c_int]
+        L.or_topo_scs_ips.restype = _dp
+        L.or_topo_scs_areav.argtypes = [C.c_int, _dp, _dp]
+        L.or_topo_scv_vol.argtypes = [C.c_int, _dp, _dp]
+        L.or_topo_deriv.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.or_topo_gradop.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.or_topo_gradop.restype = C.c_int
+        L.or_topo_scs_gradop.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+        L.or_topo_scs_gradop.restype = C.c_int
         L.or_sst_enable.argtypes = [C.c_void_p, C.POINTER(SstParams)]
         for f in ("or_sst_update_mut", "or_sst_update_F1", "or_sst_update_Pk"):
             getattr(L, f).argtypes = [C.c_void_p]
@@ -325,6 +338,60 @@ def ref_hex_gradop(par, coords):
     R.hex_gradient_operator_(_i(1), _i(8), _i(n), dptr(deriv), dptr(c), dptr(g), dptr(det),
                              dptr(err), C.byref(nerr))
     return g, det
+
+
+TOPO = {"hex8": 0, "tet4": 1, "wed6": 2, "pyr5": 3}
+_FTN = {1: "tet", 2: "wed", 3: "pyr"}
+
+
+def topo_geometry(topo: int, coords, shifted=0):
+    """Oracle SCS areas [nscs][3], SCV volumes [npe], SCS gradient operator
+    [nscs][npe][3] and det J [nscs] of one element."""
+    L = lib()
+    c = np.ascontiguousarray(coords, dtype=np.float64)
+    npe, ns = L.or_topo_npe(topo), L.or_topo_nscs(topo)
+    a = np.zeros((ns, 3))
+    v = np.zeros(npe)
+    g = np.zeros((ns, npe, 3))
+    d = np.zeros(ns)
+    L.or_topo_scs_areav(topo, dptr(c), dptr(a))
+    L.or_topo_scv_vol(topo, dptr(c), dptr(v))
+    L.or_topo_scs_gradop(topo, shifted, dptr(c), dptr(g), dptr(d))
+    return a, v, g, d
+
+
+def topo_deriv(topo: int, shifted: int, par):
+    L = lib()
+    p = np.ascontiguousarray(par, dtype=np.float64)
+    npe = L.or_topo_npe(topo)
+    d = np.zeros((p.shape[0], npe, 3))
+    L.or_topo_deriv(topo, shifted, p.shape[0], dptr(p), dptr(d))
+    return d
+
+
+def ref_topo_geometry(topo: int, coords, deriv):
+    """The reference Fortran *_scs_det, *_scv_det and *_gradient_operator
+    (MasterElementWork.F) on one element; deriv [nint][npe][3] feeds the
+    gradient operator."""
+    R = ref_lib()
+    c = np.ascontiguousarray(coords, dtype=np.float64)
+    dv = np.ascontiguousarray(deriv, dtype=np.float64)
+    npe = c.shape[0]
+    ns = {1: 6, 2: 9, 3: 12}[topo]
+    name = _FTN[topo]
+    a = np.zeros((ns, 3))
+    getattr(R, f"{name}_scs_det_")(_i(1), _i(npe), _i(ns), dptr(c), dptr(a))
+    v = np.zeros(npe)
+    err = np.zeros(1)
+    nerr = C.c_int(0)
+    getattr(R, f"{name}_scv_det_")(_i(1), _i(npe), _i(npe), dptr(c), dptr(v), dptr(err),
+                                   C.byref(nerr))
+    nint = dv.shape[0]
+    g = np.zeros((nint, npe, 3))
+    det = np.zeros(nint)
+    getattr(R, f"{name}_gradient_operator_")(_i(1), _i(npe), _i(nint), dptr(dv), dptr(c),
+                                             dptr(g), dptr(det), dptr(err), C.byref(nerr))
+    return a, v, g, det
 
 
 def ref_quad_areav(coords4):
