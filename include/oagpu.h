@@ -73,6 +73,9 @@ typedef struct oagpu_flow_params {
                               OAGPU_CG_AMG                                  */
     int use_graphs;        /* 1: CG iterations replayed from a captured HIP
                               graph; 0: launched one by one (same results)  */
+    int high_res;          /* advection_scheme high_resolution: limited
+                              deferred correction (blending factors "beta",
+                              "beta_k", "beta_w"); 0: upwind                */
 } oagpu_flow_params;
 
 /* k-omega SST transport (shearStressTransportModel.h:21-37 constants are
@@ -176,6 +179,8 @@ int oagpu_gradient(oagpu_flow *f, int which);  /* nodeField.hpp:3253; 0 U, 1 p, 
 int oagpu_correct(oagpu_flow *f, int N, double urf); /* equation.h:204-286  */
 int oagpu_velocity_correction(oagpu_flow *f, int step); /* segregatedFlowEquations.cpp:133-279 */
 int oagpu_update_scales(oagpu_flow *f, double *U_scale, double *p_scale); /* nodeField.hpp:5993 */
+/* nodeField::updateBlendingFactorField (nodeField.hpp:2069): 0 U, 2 k, 3 omega */
+int oagpu_update_blending(oagpu_flow *f, int which);
 /* control residual source: RMS over rows of b / a_diag per component */
 int oagpu_scaled_residual(oagpu_flow *f, int N, double *out);
 

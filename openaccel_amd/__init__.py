@@ -47,7 +47,7 @@ class FlowParams(C.Structure):
                 ("u_shifted", C.c_int), ("ugrad_shifted", C.c_int),
                 ("p_shifted", C.c_int), ("pgrad_shifted", C.c_int),
                 ("p_ref_node", C.c_int), ("precision", C.c_int), ("p_method", C.c_int),
-                ("use_graphs", C.c_int)]
+                ("use_graphs", C.c_int), ("high_res", C.c_int)]
 
 
 class SstParams(C.Structure):
@@ -108,6 +108,7 @@ _SIGS = {
     "oagpu_solve": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                     C.POINTER(ControlData)],
     "oagpu_element_geometry": [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp],
+    "oagpu_update_blending": [C.c_void_p, C.c_int],
     "oagpu_sst_enable": [C.c_void_p, C.POINTER(SstParams)],
     "oagpu_sst_update": [C.c_void_p],
     "oagpu_assemble_scalar": [C.c_void_p, C.c_int],
@@ -178,7 +179,7 @@ def default_params(precision: int = OAGPU_FP64, **kw) -> FlowParams:
     p = FlowParams(rho=1.0, mu=0.01, dt=1.0, urf_U=0.9, urf_p=0.1, urf_mdot=1.0,
                    grad_urf=0.5625, bnd_relax=0.75, u_shifted=0, ugrad_shifted=1,
                    p_shifted=1, pgrad_shifted=1, p_ref_node=0, precision=precision,
-                   p_method=OAGPU_CG, use_graphs=1)
+                   p_method=OAGPU_CG, use_graphs=1, high_res=0)
     for k, v in kw.items():
         setattr(p, k, v)
     return p
@@ -246,7 +247,7 @@ class FlowCase:
 
     _comp = {"U": 3, "p": 1, "gradU": 9, "gradP": 3, "du": 3, "div": 1, "mu": 1, "k": 1,
              "omega": 1, "gradK": 3, "gradW": 3, "ymin": 1, "mut": 1, "F1": 1, "Pk": 1,
-             "Pk_wall": 1, "kprev": 1}
+             "Pk_wall": 1, "kprev": 1, "beta": 3, "beta_k": 1, "beta_w": 1}
 
     def _size(self, name):
         if name == "mdot":
@@ -299,6 +300,9 @@ class FlowCase:
 
     def gradient(self, which):
         _check(load_library().oagpu_gradient(self.h, {"U": 0, "p": 1, "k": 2, "omega": 3}[which]))
+
+    def update_blending(self, which):
+        _check(load_library().oagpu_update_blending(self.h, {"U": 0, "k": 2, "omega": 3}[which]))
 
     # k-omega SST
     def sst_enable(self, sst: SstParams):

@@ -59,7 +59,8 @@ double* field_ptr(oagpu::Flow& f, const char* name, size_t* count) {
                {"omega", &f.om, nn},          {"gradK", &f.gradK, nn * 3},
                {"gradW", &f.gradW, nn * 3},  {"ymin", &f.ymin, nn}, {"mut", &f.mut, nn},
                {"F1", &f.F1, nn},            {"Pk", &f.Pk, nn},     {"Pk_wall", &f.Pk_wall, nn},
-               {"kprev", &f.kprev, nn}};
+               {"kprev", &f.kprev, nn},      {"beta", &f.beta, nn * 3},
+               {"beta_k", &f.beta_k, nn},    {"beta_w", &f.beta_w, nn}};
     for (auto& t : tab)
         if (!std::strcmp(name, t.n)) {
             OAGPU_REQUIRE(t.b->p, std::string("field ") + name + " not allocated (SST not enabled)");
@@ -249,6 +250,10 @@ int oagpu_flow_create(oagpu_mesh* m, const oagpu_flow_params* prm, int n_bc, con
                         &f.A1, &f.b1, &f.x1})
             b->zero();
         f.mueff.upload(std::vector<double>(nn, prm->mu));
+        if (prm->high_res) {
+            f.beta.alloc(nn * 3);
+            f.beta.zero();
+        }
         OAGPU_HIP(hipStreamCreate(&f.stream));
         OAGPU_HIP(hipEventCreate(&f.ev0));
         OAGPU_HIP(hipEventCreate(&f.ev1));
@@ -310,6 +315,7 @@ int oagpu_initialize(oagpu_flow* h) {
         f.gradU.zero(f.stream);
         f.gradP.zero(f.stream);
         oagpu::launch_gradient(f, 0);
+        if (f.prm.high_res) oagpu::launch_blending(f, 0);  // navierStokesEquation.cpp:95
         oagpu::launch_gradient(f, 1);
         oagpu::field_scales(f, nullptr, nullptr);
         oagpu::launch_mdot_init(f);
@@ -436,6 +442,15 @@ int oagpu_solve(oagpu_flow* h, int N, int method, int max_it, double rtol, doubl
     });
 }
 
+int oagpu_update_blending(oagpu_flow* h, int which) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && (which == 0 || which == 2 || which == 3), "bad argument");
+        OAGPU_REQUIRE(h->f.prm.high_res, "high resolution advection is off for this flow");
+        oagpu::launch_blending(h->f, which);
+        OAGPU_HIP(hipStreamSynchronize(h->f.stream));
+    });
+}
+
 int oagpu_element_geometry(int topology, int shifted, int n_elems, const double* coords,
                            double* areav, double* scv, double* dndx, double* detj) {
     return guarded([&] {
@@ -476,6 +491,12 @@ int oagpu_sst_enable(oagpu_flow* h, const oagpu_sst_params* sp) {
                 b->alloc(nn);
             f.gradK.alloc(3 * nn);
             f.gradW.alloc(3 * nn);
+            if (f.prm.high_res) {
+                f.beta_k.alloc(nn);
+                f.beta_w.alloc(nn);
+                f.beta_k.zero(f.stream);
+                f.beta_w.zero(f.stream);
+            }
             for (auto* b : {&f.k, &f.om, &f.ymin, &f.mut, &f.F1, &f.Pk, &f.Pk_wall, &f.kprev, &f.gradK,
                             &f.gradW})
                 b->zero(f.stream);
@@ -622,6 +643,7 @@ int oagpu_outer_iteration(oagpu_flow* h, const double* ctl, double* hist) {
         oagpu::launch_velocity_correction(f, 2);
         oagpu::field_scales(f, nullptr, nullptr);
         f.ms_grad = timed(f, [&] { oagpu::launch_gradient(f, 0); });
+        if (f.prm.high_res) oagpu::launch_blending(f, 0);  // segregatedFlowEquations.cpp:284-288
         check_err(f);
         if (hist) {
             for (int k = 0; k < 3; ++k) hist[k] = rU[k] / (us0 + kSmall);

@@ -78,6 +78,8 @@ struct Flow {
     CgGraph cg_graph;
     // effective viscosity mu + mu_t per node (mu alone while laminar)
     DBuf<double> mueff;
+    // blending factors of the limited deferred correction (high resolution)
+    DBuf<double> beta, beta_k, beta_w;
     // k-omega SST state (oagpu_sst_enable)
     bool sst_on = false;
     oagpu_sst_params sst{};
@@ -91,6 +93,7 @@ void launch_mdot_init(Flow& f);
 void launch_update_mdot(Flow& f);
 void launch_mass_divergence(Flow& f);
 void launch_gradient(Flow& f, int which);
+void launch_blending(Flow& f, int which);   // 0 U, 2 k, 3 omega
 void launch_assemble_momentum(Flow& f);
 void launch_assemble_pcorr(Flow& f);
 void launch_correct(Flow& f, int N, double urf);

@@ -242,6 +242,8 @@ struct MomArgs {
     const double* div;
     const double* mdot;
     const double* mueff;
+    const double* beta;    // blending factors [nn][3] (high resolution) or null
+    const double* gradU;
     double rho, mu, dt, urf_inv, bnd_relax;
     int ugrad_shifted, us;
     double* vals;
@@ -260,6 +262,7 @@ __device__ __forceinline__ void mom_incidence(const MomArgs& a, int la, int e, u
     load_conn(a.conn, e, nd);
     T A[3][3], Ji[3][9], m[3], sg[3];
     int ipq[3];
+    T dxu[3][3];  // x_ip - x_upwind (high resolution)
     {
         T c[8][3];
         gather_rel(a.xyz, nd, xr, c);
@@ -271,6 +274,14 @@ __device__ __forceinline__ void mom_incidence(const MomArgs& a, int la, int e, u
             area_scs(ip, c, A[q]);
             jinv_scs(SH, ip, c, Ji[q], bad);
             m[q] = T(a.mdot[12 * size_t(e) + ip]);
+            if (a.beta) {
+                const int up = m[q] > T(0) ? c_hex.lr[ip][0] : c_hex.lr[ip][1];
+                for (int d = 0; d < 3; ++d) {
+                    T xi = T(0);
+                    for (int n = 0; n < 8; ++n) xi += T(c_hex.N[0][ip][n]) * c[n][d];
+                    dxu[q][d] = xi - c[up][d];
+                }
+            }
         }
     }
     // effective viscosity at the ips, velocity shape functions (a constant
@@ -291,7 +302,16 @@ __device__ __forceinline__ void mom_incidence(const MomArgs& a, int la, int e, u
     for (int q = 0; q < 3; ++q) {
         const int nu = m[q] > T(0) ? a.conn[8 * size_t(e) + c_hex.lr[ipq[q]][0]]
                                    : a.conn[8 * size_t(e) + c_hex.lr[ipq[q]][1]];
-        for (int i = 0; i < 3; ++i) rl[i] -= sg[q] * (m[q] * T(a.U[3 * size_t(nu) + i]));
+        for (int i = 0; i < 3; ++i) {
+            T up = T(a.U[3 * size_t(nu) + i]);
+            if (a.beta) {  // limited deferred correction (navierStokesAssemblerElemTerms.cpp:312-340)
+                T dc = T(0);
+                for (int j = 0; j < 3; ++j)
+                    dc += T(a.beta[3 * size_t(nu) + i]) * dxu[q][j] * T(a.gradU[9 * size_t(nu) + 3 * i + j]);
+                up += dc;
+            }
+            rl[i] -= sg[q] * (m[q] * up);
+        }
     }
 #pragma unroll 1
     for (int cn = 0; cn < 8; ++cn) {
@@ -522,6 +542,8 @@ void launch_assemble_momentum(Flow& f) {
     a.ugrad_shifted = f.prm.ugrad_shifted;
     a.us = f.prm.u_shifted ? 1 : 0;
     a.mueff = f.sst_on ? f.mueff.p : nullptr;  // laminar: the constant mu
+    a.beta = f.prm.high_res ? f.beta.p : nullptr;
+    a.gradU = f.gradU.p;
     a.vals = f.A3.p;
     a.b = f.b3.p;
     a.du = f.du.p;
@@ -753,6 +775,8 @@ struct ScArgs {
     const double* mut;
     const double* F1;
     const double* Pk;
+    const double* beta;   // blending factor of phi (high resolution) or null
+    const double* gphi;   // grad phi [nn][3]
     double rho, mu, dt, urf_inv, sig1, sig2;
     int which, us;
     double* vals;
@@ -820,6 +844,7 @@ __device__ __forceinline__ void sc_incidence(const ScArgs& a, int la, int e, uin
     load_conn(a.conn, e, nd);
     T A[3][3], Ji[3][9], sg[3], Gip[3] = {0, 0, 0}, m[3];
     int ipq[3];
+    T dxu[3][3];
     {
         T c[8][3];
         gather_rel(a.xyz, nd, xr, c);
@@ -831,6 +856,14 @@ __device__ __forceinline__ void sc_incidence(const ScArgs& a, int la, int e, uin
             area_scs(ip, c, A[q]);
             jinv_scs(SG, ip, c, Ji[q], bad);
             m[q] = T(a.mdot[12 * size_t(e) + ip]);
+            if (a.beta) {
+                const int up = m[q] > T(0) ? c_hex.lr[ip][0] : c_hex.lr[ip][1];
+                for (int d = 0; d < 3; ++d) {
+                    T xi = T(0);
+                    for (int n = 0; n < 8; ++n) xi += T(c_hex.N[0][ip][n]) * c[n][d];
+                    dxu[q][d] = xi - c[up][d];
+                }
+            }
         }
     }
     // Gamma = mu + mu_t sigma(F1) at the nodes, interpolated to the ips
@@ -845,7 +878,13 @@ __device__ __forceinline__ void sc_incidence(const ScArgs& a, int la, int e, uin
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         const int L = c_hex.lr[ipq[q]][0], Rr = c_hex.lr[ipq[q]][1];
-        const T up = T(a.phi[m[q] > T(0) ? nd[L] : nd[Rr]]);
+        const int nu = m[q] > T(0) ? nd[L] : nd[Rr];
+        T up = T(a.phi[nu]);
+        if (a.beta) {  // limited deferred correction (phiAssemblerElemTerms.hpp:291-340)
+            T dc = T(0);
+            for (int j = 0; j < 3; ++j) dc += T(a.beta[nu]) * dxu[q][j] * T(a.gphi[3 * size_t(nu) + j]);
+            up += dc;
+        }
         rl -= sg[q] * (m[q] * up);
     }
 #pragma unroll 1
@@ -943,6 +982,8 @@ void launch_assemble_scalar(Flow& f, int which) {
     a.mut = f.mut.p;
     a.F1 = f.F1.p;
     a.Pk = f.Pk.p;
+    a.beta = f.prm.high_res ? (which == 0 ? f.beta_k.p : f.beta_w.p) : nullptr;
+    a.gphi = which == 0 ? f.gradK.p : f.gradW.p;
     a.rho = f.prm.rho;
     a.mu = f.prm.mu;
     a.dt = f.prm.dt;
@@ -1165,6 +1206,104 @@ __global__ void __launch_bounds__(256) k_gradient(int nn, const double* xyz, con
         }
     }
     for (int k = 0; k < 3 * N; ++k) grad[3 * N * size_t(a) + k] = G[k];
+}
+
+// ---------------------------------------------------------- blending factor
+// nodeField::updateBlendingFactorField (nodeField.hpp:2069-3250) with
+// updateMinMaxFields (:1843-2068) for one node per lane: min / max over the
+// nodes sharing an SCS with it, then the smooth Barth-Jespersen limiter over
+// those SCS ips and the boundary-face ips nearest to it (blendingFactorMax 1).
+// FP64, absolute coordinates, as the reference.
+__device__ __forceinline__ void bj_limit(double& beta, double mx, double mn, double phi,
+                                         const double g[3], const double dx[3]) {
+    double rgrad = g[0] * dx[0] + g[1] * dx[1] + g[2] * dx[2];
+    rgrad = rgrad > 0 ? rgrad + kSmall : rgrad - kSmall;
+    const double bl = fmax((mx - phi) / rgrad, (mn - phi) / rgrad);
+    const double b12 = fmax(-(mx - phi) / rgrad, -(mn - phi) / rgrad);
+    double y = fmin(bl, b12);
+    y = (y * y + 2.0 * y) / (y * y + y + 2.0);
+    if (rgrad > 100 * kSmall || rgrad < -100 * kSmall) beta = fmin(beta, y);
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) k_blending(int nn, const double* xyz, const int* conn,
+                                                  const int* inc_la, const int* ext_ptr,
+                                                  const int* ext, const int* binc_ptr,
+                                                  const int* binc, const int* bf_elem,
+                                                  const int* bf_ord, const double* phi,
+                                                  const double* grad, double* beta) {
+    const int a = xcd_block() * blockDim.x + threadIdx.x;
+    if (a >= nn) return;
+    double pa[N], mn[N], mx[N], b[N], g[N][3];
+    for (int i = 0; i < N; ++i) {
+        pa[i] = mn[i] = mx[i] = phi[N * size_t(a) + i];
+        b[i] = 1.0;
+        for (int j = 0; j < 3; ++j) g[i][j] = grad[3 * N * size_t(a) + 3 * i + j];
+    }
+    const double xa[3] = {xyz[3 * size_t(a)], xyz[3 * size_t(a) + 1], xyz[3 * size_t(a) + 2]};
+    auto incidence = [&](int la, int e, bool limit) {
+        int nd[8];
+        load_conn(conn, e, nd);
+        for (int q = 0; q < 3; ++q) {
+            const int ip = c_hex.node_ip[la][q];
+            const int other = c_hex.node_sg[la][q] > 0 ? c_hex.lr[ip][1] : c_hex.lr[ip][0];
+            if (!limit) {
+                for (int i = 0; i < N; ++i) {
+                    const double v = phi[N * size_t(nd[other]) + i];
+                    mn[i] = fmin(mn[i], v);
+                    mx[i] = fmax(mx[i], v);
+                }
+                continue;
+            }
+            double dx[3];
+            for (int d = 0; d < 3; ++d) {
+                double xi = 0.0;
+                for (int n = 0; n < 8; ++n) xi += c_hex.N[0][ip][n] * xyz[3 * size_t(nd[n]) + d];
+                dx[d] = xi - xa[d];
+            }
+            for (int i = 0; i < N; ++i) bj_limit(b[i], mx[i], mn[i], pa[i], g[i], dx);
+        }
+    };
+    for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll 1
+        for (int la = 0; la < 8; ++la) {
+            const int e = inc_la[size_t(la) * nn + a];
+            if (e >= 0) incidence(la, e, pass == 1);
+        }
+#pragma unroll 1
+        for (int k = ext_ptr[a]; k < ext_ptr[a + 1]; ++k) incidence(ext[k] & 7, ext[k] >> 3, pass == 1);
+    }
+#pragma unroll 1
+    for (int k = binc_ptr[a]; k < binc_ptr[a + 1]; ++k) {
+        const int f = binc[k] >> 2, fk = binc[k] & 3;
+        const int e = bf_elem[f], ord = bf_ord[f];
+        double dx[3] = {0, 0, 0};
+        for (int q = 0; q < 4; ++q) {
+            const int n = conn[8 * size_t(e) + hex8::c_side_nodes[ord][q]];
+            for (int d = 0; d < 3; ++d) dx[d] += hex8::c_quad_shape[0][fk][q] * xyz[3 * size_t(n) + d];
+        }
+        for (int d = 0; d < 3; ++d) dx[d] -= xa[d];
+        for (int i = 0; i < N; ++i) bj_limit(b[i], mx[i], mn[i], pa[i], g[i], dx);
+    }
+    for (int i = 0; i < N; ++i) beta[N * size_t(a) + i] = b[i];
+}
+
+void launch_blending(Flow& f, int which) {
+    DevMesh& m = *f.m;
+    const unsigned B = 256, G = xcd_grid(m.nn, B);
+    if (which == 0) {
+        OAGPU_REQUIRE(f.beta.p, "blending factors not allocated (high resolution off)");
+        k_blending<3><<<G, B, 0, f.stream>>>(m.nn, m.xyz.p, m.conn.p, m.inc_la.p, m.ext_ptr.p, m.ext.p,
+                                             m.binc_ptr.p, m.binc.p, m.bf_elem.p, m.bf_ord.p, f.U.p,
+                                             f.gradU.p, f.beta.p);
+    } else {
+        OAGPU_REQUIRE((which == 2 || which == 3) && f.sst_on && f.beta_k.p, "bad blending request");
+        k_blending<1><<<G, B, 0, f.stream>>>(
+            m.nn, m.xyz.p, m.conn.p, m.inc_la.p, m.ext_ptr.p, m.ext.p, m.binc_ptr.p, m.binc.p,
+            m.bf_elem.p, m.bf_ord.p, which == 2 ? f.k.p : f.om.p, which == 2 ? f.gradK.p : f.gradW.p,
+            which == 2 ? f.beta_k.p : f.beta_w.p);
+    }
+    OAGPU_HIP(hipGetLastError());
 }
 
 void launch_gradient(Flow& f, int which) {

@@ -118,7 +118,7 @@ void or_case_destroy(or_case *c)
                     c->A1, c->b1, c->x1, c->U, c->p, c->gradU, c->gradP, c->du,
                     c->mdot, c->div, c->vol, c->rho, c->mu, c->beta, c->bareav,
                     c->bnode, c->k, c->w, c->gradK, c->gradW, c->ymin, c->mut, c->F1,
-                    c->Pk, c->Pk_wall, c->kprev};
+                    c->Pk, c->Pk_wall, c->kprev, c->beta_k, c->beta_w};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
     free(c);
 }
@@ -154,6 +154,8 @@ double *or_case_field(or_case *c, const char *name)
     if (!strcmp(name, "Pk")) return c->Pk;
     if (!strcmp(name, "Pk_wall")) return c->Pk_wall;
     if (!strcmp(name, "kprev")) return c->kprev;
+    if (!strcmp(name, "beta_k")) return c->beta_k;
+    if (!strcmp(name, "beta_w")) return c->beta_w;
     return NULL;
 }
 
@@ -654,6 +656,104 @@ void or_scaled_residual(or_case *c, int N, double *out)
     }
 }
 
+/* ------------------------------------------------------ blending factor */
+static double limiter_y(double mx, double mn, double phi, double rgrad)
+{
+    rgrad = rgrad > 0 ? rgrad + SMALL : rgrad - SMALL;
+    const double betaLocal = fmax((mx - phi) / rgrad, (mn - phi) / rgrad);
+    const double beta12Local = fmax(-(mx - phi) / rgrad, -(mn - phi) / rgrad);
+    double y = fmin(betaLocal, beta12Local);
+    y = (y * y + 2.0 * y) / (y * y + y + 2.0); /* blendingFactorMax_ = 1 < 2 */
+    return y;
+}
+
+/* beta = min(beta, y) unless the projected gradient is negligible */
+static void limit(double *beta, double mx, double mn, double phi, const double *g, const double *dx)
+{
+    double rgrad = 0;
+    for (int j = 0; j < 3; ++j) rgrad += g[j] * dx[j];
+    const double rg = rgrad > 0 ? rgrad + SMALL : rgrad - SMALL;
+    const double y = limiter_y(mx, mn, phi, rgrad);
+    if (rg > 100 * SMALL || rg < -100 * SMALL) *beta = fmin(*beta, y);
+}
+
+void or_update_blending(or_case *c, int which)
+{
+    const int N = which == 0 ? 3 : 1;
+    const double *phi = which == 0 ? c->U : which == 2 ? c->k : c->w;
+    const double *grad = which == 0 ? c->gradU : which == 2 ? c->gradK : c->gradW;
+    double *beta;
+    if (which == 0) {
+        beta = c->beta;
+    } else {
+        double **bp = which == 2 ? &c->beta_k : &c->beta_w;
+        if (!*bp) *bp = (double *)xcalloc((size_t)c->nn, sizeof(double));
+        beta = *bp;
+    }
+    const size_t n = (size_t)c->nn * N;
+    double *mn = (double *)xcalloc(n, sizeof(double)), *mx = (double *)xcalloc(n, sizeof(double));
+    for (size_t k = 0; k < n; ++k) mn[k] = mx[k] = phi[k];
+    const int *lr = or_hex_lrscv();
+    for (int e = 0; e < c->ne; ++e) {
+        const int *en = &c->conn[8 * e];
+        for (int ip = 0; ip < 12; ++ip) {
+            const int L = en[lr[2 * ip]], R = en[lr[2 * ip + 1]];
+            for (int i = 0; i < N; ++i) {
+                mn[N * L + i] = fmin(mn[N * L + i], phi[N * R + i]);
+                mx[N * L + i] = fmax(mx[N * L + i], phi[N * R + i]);
+                mn[N * R + i] = fmin(mn[N * R + i], phi[N * L + i]);
+                mx[N * R + i] = fmax(mx[N * R + i], phi[N * L + i]);
+            }
+        }
+    }
+    for (size_t k = 0; k < n; ++k) beta[k] = 1.0; /* blendingFactorMax_ */
+    double sfc[96], co[24];
+    or_hex_scs_shape(0, sfc); /* coordinates: standard shape functions */
+    for (int e = 0; e < c->ne; ++e) {
+        const int *en = &c->conn[8 * e];
+        gather_coords(c, e, co);
+        for (int ip = 0; ip < 12; ++ip) {
+            double xip[3] = {0, 0, 0};
+            for (int ic = 0; ic < 8; ++ic)
+                for (int j = 0; j < 3; ++j) xip[j] += sfc[8 * ip + ic] * co[3 * ic + j];
+            for (int side = 0; side < 2; ++side) {
+                const int a = lr[2 * ip + side], node = en[a];
+                double dx[3];
+                for (int j = 0; j < 3; ++j) dx[j] = xip[j] - co[3 * a + j];
+                for (int i = 0; i < N; ++i)
+                    limit(&beta[N * node + i], mx[N * node + i], mn[N * node + i], phi[N * node + i],
+                          &grad[3 * N * node + 3 * i], dx);
+            }
+        }
+    }
+    /* boundary faces: each face ip limits its nearest node */
+    const int *sn = or_hex_side_nodes();
+    double fsf[16];
+    or_quad_face_shape(0, fsf);
+    for (int f = 0; f < c->nbf; ++f) {
+        const int *en = &c->conn[8 * c->bf_elem[f]];
+        int fn[4];
+        double fc[12];
+        for (int k = 0; k < 4; ++k) {
+            fn[k] = en[sn[4 * c->bf_ord[f] + k]];
+            for (int d = 0; d < 3; ++d) fc[3 * k + d] = c->xyz[3 * fn[k] + d];
+        }
+        for (int ip = 0; ip < 4; ++ip) {
+            double xip[3] = {0, 0, 0};
+            for (int ic = 0; ic < 4; ++ic)
+                for (int j = 0; j < 3; ++j) xip[j] += fsf[4 * ip + ic] * fc[3 * ic + j];
+            const int node = fn[ip];
+            double dx[3];
+            for (int j = 0; j < 3; ++j) dx[j] = xip[j] - fc[3 * ip + j];
+            for (int i = 0; i < N; ++i)
+                limit(&beta[N * node + i], mx[N * node + i], mn[N * node + i], phi[N * node + i],
+                      &grad[3 * N * node + 3 * i], dx);
+        }
+    }
+    free(mn);
+    free(mx);
+}
+
 /* ------------------------------------------------------------ initialize */
 void or_case_initialize(or_case *c)
 {
@@ -661,6 +761,7 @@ void or_case_initialize(or_case *c)
     memset(c->gradU, 0, sizeof(double) * 9 * (size_t)c->nn);
     memset(c->gradP, 0, sizeof(double) * 3 * (size_t)c->nn);
     or_gradient(c, 0);
+    if (c->prm.high_res) or_update_blending(c, 0);
     /* pressureCorrectionEquation::initialize (:59-70) */
     or_gradient(c, 1);
     or_update_scales(c, NULL, NULL);
@@ -702,6 +803,7 @@ void or_outer_iteration(or_case *c, const double *ctl, double *hist)
     or_velocity_correction(c, 2);
     or_update_scales(c, &us, &ps);
     or_gradient(c, 0);
+    if (P->high_res) or_update_blending(c, 0); /* segregatedFlowEquations.cpp:284-288 */
     if (hist) {
         for (int k = 0; k < 3; ++k) hist[k] = r3[k] / (us0 + SMALL);
         hist[3] = r1 / (ps0 + SMALL);

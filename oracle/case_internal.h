@@ -28,7 +28,7 @@ struct or_case {
     double U_scale, p_scale;
     /* k-omega SST (sst.c); NULL until or_sst_enable */
     or_sst_params sst;
-    double *k, *w, *gradK, *gradW, *ymin, *mut, *F1, *Pk, *Pk_wall, *kprev;
+    double *k, *w, *gradK, *gradW, *ymin, *mut, *F1, *Pk, *Pk_wall, *kprev, *beta_k, *beta_w;
     double k_scale, w_scale;
     int scalar_terms;   /* or_set_scalar_terms mask (test hook), default all */
 };

@@ -115,6 +115,8 @@ typedef struct or_params {
     int pgrad_shifted;       /* pressure gradient interpolation           */
     int reduced;             /* bandwidth_reduction (reduced stencil)     */
     int p_ref_node;          /* pressure-level node (-1: none)            */
+    int high_res;            /* advection_scheme high_resolution: limited
+                                deferred correction (else upwind)         */
 } or_params;
 
 typedef struct or_case or_case;
@@ -161,6 +163,13 @@ void or_correct_U(or_case *c, double urf);   /* equation.h:204-286 (N=3) */
 void or_correct_p(or_case *c, double urf);   /* equation.h:204-286 (N=1) */
 void or_velocity_correction(or_case *c, int step); /* segregatedFlowEquations.cpp:133-279 */
 void or_update_scales(or_case *c, double *U_scale, double *p_scale); /* nodeField.hpp:5993, flowModel.cpp:117 */
+
+/* nodeField::updateBlendingFactorField (field/nodeField/nodeField.hpp:2069-3250)
+ * with updateMinMaxFields (:1843-2068): Barth-Jespersen limiter over the SCS
+ * ips (interior and boundary faces) with the smooth form for a maximum
+ * blending factor of 1 (nodeField.h:150).  which: 0 U -> "beta" [nn][3],
+ * 2 k -> "beta_k", 3 omega -> "beta_w". */
+void or_update_blending(or_case *c, int which);
 
 /* Krylov solves on the case's system.  kind: 0 = Jacobi-GMRES(m),
  * 1 = Jacobi-CG, 2 = Jacobi-BiCGStab.  Returns iterations; res[0]/res[1] =

@@ -25,7 +25,7 @@ class Params(C.Structure):
                 ("grad_urf", C.c_double), ("bnd_relax", C.c_double),
                 ("u_shifted", C.c_int), ("ugrad_shifted", C.c_int),
                 ("p_shifted", C.c_int), ("pgrad_shifted", C.c_int),
-                ("reduced", C.c_int), ("p_ref_node", C.c_int)]
+                ("reduced", C.c_int), ("p_ref_node", C.c_int), ("high_res", C.c_int)]
 
 
 class SstParams(C.Structure):
@@ -130,6 +130,7 @@ def lib():
         for f in ("or_assemble_scalar", "or_correct_scalar", "or_set_scalar_terms"):
             getattr(L, f).argtypes = [C.c_void_p, C.c_int]
         L.or_sst_iteration.argtypes = [C.c_void_p, _dp, _dp]
+        L.or_update_blending.argtypes = [C.c_void_p, C.c_int]
         L.or_hex_side_nodes.restype = _ip
         _lib = L
     return _lib
@@ -138,7 +139,7 @@ def lib():
 def default_params(**kw) -> Params:
     p = Params(rho=1.0, mu=0.01, dt=1.0, urf_U=0.9, urf_p=0.1, urf_mdot=1.0,
                grad_urf=0.5625, bnd_relax=0.75, u_shifted=0, ugrad_shifted=1,
-               p_shifted=1, pgrad_shifted=1, reduced=1, p_ref_node=0)
+               p_shifted=1, pgrad_shifted=1, reduced=1, p_ref_node=0, high_res=0)
     for k, v in kw.items():
         setattr(p, k, v)
     return p
@@ -186,7 +187,8 @@ class Case:
 
     _sizes = {"U": 3, "p": 1, "gradU": 9, "gradP": 3, "du": 3, "div": 1, "vol": 1,
               "rho": 1, "mu": 1, "beta": 3, "xyz": 3, "k": 1, "omega": 1, "gradK": 3,
-              "gradW": 3, "ymin": 1, "mut": 1, "F1": 1, "Pk": 1, "Pk_wall": 1, "kprev": 1}
+              "gradW": 3, "ymin": 1, "mut": 1, "F1": 1, "Pk": 1, "Pk_wall": 1, "kprev": 1,
+              "beta_k": 1, "beta_w": 1}
 
     def field(self, name):
         ptr = lib().or_case_field(self.h, name.encode())
@@ -232,6 +234,9 @@ class Case:
     def sst_enable(self, sst: SstParams):
         self._sst = sst
         lib().or_sst_enable(self.h, C.byref(sst))
+
+    def update_blending(self, which):
+        lib().or_update_blending(self.h, {"U": 0, "k": 2, "omega": 3}[which])
 
     def sst_update_mut(self):
         lib().or_sst_update_mut(self.h)

@@ -25,8 +25,8 @@
  *   solve order           segregatedShearStressTransportEquations.cpp preSolve,
  *                         solve
  *
- * Advection is upwind: the blending factor of the deferred correction is 0
- * (advection_scheme upwind), so that term vanishes identically.  Constant
+ * Advection is upwind, with the limited deferred correction (blending
+ * factors beta_k / beta_w) when the case is high-resolution.  Constant
  * density; incompressible (no dilatation terms); no frame rotation.
  */
 #include "case_internal.h"
@@ -176,8 +176,13 @@ void or_assemble_scalar(or_case *c, int which)
     }
     /* diffusivity Gamma = mu + sigma(F1) mut (computeGamma_ / setup lambdas) */
     const int *lr = or_hex_lrscv();
-    double sf[96], co[24], av[36], dndx[288], detj[12], lhs[64], rhs[8];
+    double sf[96], sfc[96], co[24], av[36], dndx[288], detj[12], lhs[64], rhs[8];
     or_hex_scs_shape(shifted, sf);
+    or_hex_scs_shape(0, sfc); /* coordinates: standard shape functions */
+    /* high-resolution advection: limited deferred correction (blending
+     * factor beta_k / beta_w, phiAssemblerElemTerms.hpp:291-340) */
+    const double *beta = c->prm.high_res ? (which == 0 ? c->beta_k : c->beta_w) : NULL;
+    const double *gphi = which == 0 ? c->gradK : c->gradW;
     for (int e = 0; e < ((c->scalar_terms & 4) ? c->ne : 0); ++e) {
         const int *en = &c->conn[8 * e];
         double phi[8], G[8];
@@ -197,8 +202,17 @@ void or_assemble_scalar(or_case *c, int which)
             const double tm = c->mdot[12 * e + ip];
             double Gip = 0.0;
             for (int ic = 0; ic < 8; ++ic) Gip += sf[8 * ip + ic] * G[ic];
-            /* upwind advection */
-            const double aflux = tm * (tm > 0 ? phi[il] : phi[ir]);
+            /* upwind advection (+ deferred correction) */
+            const int iu = tm > 0 ? il : ir;
+            double dcorr = 0.0;
+            if (beta) {
+                double xip[3] = {0, 0, 0};
+                for (int ic = 0; ic < 8; ++ic)
+                    for (int j = 0; j < 3; ++j) xip[j] += sfc[8 * ip + ic] * co[3 * ic + j];
+                for (int j = 0; j < 3; ++j)
+                    dcorr += beta[en[iu]] * (xip[j] - co[3 * iu + j]) * gphi[3 * en[iu] + j];
+            }
+            const double aflux = tm * (phi[iu] + dcorr);
             rhs[il] -= aflux;
             rhs[ir] += aflux;
             const double aL = 0.5 * (tm + fabs(tm)), aR = 0.5 * (tm - fabs(tm));
