@@ -76,6 +76,117 @@ constexpr HexTables make_hex_tables() {
     return t;
 }
 __constant__ HexTables c_hex = make_hex_tables();
+// the same tables as compile-time constants: loads with constant indices
+// fold to immediates and zero weights drop out
+static constexpr HexTables kHex = make_hex_tables();
+
+// Canonical SCS.  The rotations of the reference cube map every directed
+// edge la -> (neighbour across one of la's SCS) onto the edge 0 -> 1.
+// Relabelling an element's nodes by such a rotation moves no physical point
+// (areas, ip locations and physical shape-function gradients are those of
+// the original element) and keeps the Jacobian positive, so each of a row's
+// (incidence, ip) pairs is computed as SCS ip 0 of a relabelled element,
+// with every master-element constant known at compile time.  For one
+// incidence the three relabellings differ by the rotation about the 0-6
+// diagonal that cycles the axes (x -> y -> z), kRho below, so the element
+// is gathered once and its registers are permuted between the ips.  Only the
+// node permutation, the original ip id and its orientation come from a
+// wave-uniform table.
+constexpr int kRho[8] = {0, 3, 7, 4, 1, 2, 6, 5};  // new node k = old node kRho[k]
+constexpr int kCanIp[3] = {0, 3, 8};               // node 0's ips (towards 1, 3, 4), unrotated
+constexpr int kFar[4] = {2, 5, 6, 7};             // nodes sharing no SCS with node 0
+
+struct RotTables {
+    int perm[24][8];  // [3 la + q]: original local node of canonical node k
+    int ip[24];       // original ip of canonical ip 0
+    int sg[24];       // +1 if la is that ip's left node
+};
+
+constexpr RotTables make_rot_tables() {
+    RotTables t{};
+    constexpr int P[6][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {0, 2, 1}, {2, 1, 0}, {1, 0, 2}};
+    for (int la = 0; la < 8; ++la) {
+        int perm0[8] = {};
+        bool found = false;
+        for (int p = 0; p < 6 && !found; ++p)
+            for (int s = 0; s < 8 && !found; ++s) {
+                // (Q x)_i = sgn_i x[P[p][i]]; a rotation iff det Q = +1
+                const int sgn[3] = {(s & 1) ? -1 : 1, (s & 2) ? -1 : 1, (s & 4) ? -1 : 1};
+                if ((p < 3 ? 1 : -1) * sgn[0] * sgn[1] * sgn[2] != 1) continue;
+                bool hit = true;
+                for (int i = 0; i < 3; ++i)
+                    if (sgn[i] * tbl::SIG[la][P[p][i]] != tbl::SIG[0][i]) hit = false;
+                if (!hit) continue;
+                for (int k = 0; k < 8; ++k)
+                    for (int j = 0; j < 8; ++j) {
+                        bool eq = true;
+                        for (int i = 0; i < 3; ++i)
+                            if (sgn[i] * tbl::SIG[j][P[p][i]] != tbl::SIG[k][i]) eq = false;
+                        if (eq) perm0[k] = j;
+                    }
+                found = true;
+            }
+        int cur[8] = {};
+        for (int k = 0; k < 8; ++k) cur[k] = perm0[k];
+        for (int q = 0; q < 3; ++q) {
+            const int r = 3 * la + q;
+            for (int k = 0; k < 8; ++k) t.perm[r][k] = cur[k];
+            for (int ip = 0; ip < 12; ++ip) {
+                if (tbl::LR[ip][0] == la && tbl::LR[ip][1] == cur[1]) t.ip[r] = ip, t.sg[r] = 1;
+                if (tbl::LR[ip][1] == la && tbl::LR[ip][0] == cur[1]) t.ip[r] = ip, t.sg[r] = -1;
+            }
+            int nxt[8] = {};
+            for (int k = 0; k < 8; ++k) nxt[k] = cur[kRho[k]];
+            for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
+        }
+    }
+    return t;
+}
+static constexpr RotTables kRot = make_rot_tables();
+__constant__ RotTables c_rot = make_rot_tables();
+
+constexpr bool scs_partners(int a, int b) {
+    for (int ip = 0; ip < 12; ++ip)
+        if ((tbl::LR[ip][0] == a && tbl::LR[ip][1] == b) || (tbl::LR[ip][1] == a && tbl::LR[ip][0] == b))
+            return true;
+    return false;
+}
+constexpr bool rot_tables_valid() {
+    for (int r = 0; r < 24; ++r) {
+        const int la = r / 3, ip = kRot.ip[r];
+        const int* pm = kRot.perm[r];
+        if (pm[0] != la) return false;
+        int seen = 0;
+        for (int k = 0; k < 8; ++k) seen |= 1 << pm[k];
+        if (seen != 0xFF) return false;
+        if (tbl::LR[ip][kRot.sg[r] > 0 ? 0 : 1] != la || tbl::LR[ip][kRot.sg[r] > 0 ? 1 : 0] != pm[1])
+            return false;
+        if (!scs_partners(la, pm[3]) || !scs_partners(la, pm[4])) return false;
+        for (int p = 0; p < 4; ++p)
+            if (scs_partners(la, pm[kFar[p]])) return false;
+        // a proper relabelling: the edges 0-1, 0-3, 0-4 map onto a
+        // right-handed triple, and every node lands where the map sends it
+        int d[3][3] = {};
+        for (int i = 0; i < 3; ++i) {
+            d[0][i] = tbl::SIG[pm[1]][i] - tbl::SIG[la][i];
+            d[1][i] = tbl::SIG[pm[3]][i] - tbl::SIG[la][i];
+            d[2][i] = tbl::SIG[pm[4]][i] - tbl::SIG[la][i];
+        }
+        const int det = d[0][0] * (d[1][1] * d[2][2] - d[1][2] * d[2][1]) -
+                        d[0][1] * (d[1][0] * d[2][2] - d[1][2] * d[2][0]) +
+                        d[0][2] * (d[1][0] * d[2][1] - d[1][1] * d[2][0]);
+        if (det != 8) return false;
+        for (int k = 0; k < 8; ++k)
+            for (int i = 0; i < 3; ++i) {
+                // SIG[pm[k]] = SIG[la] + sum over set bits of k's position of the edges
+                const int x = tbl::SIG[k][0] > 0, y = tbl::SIG[k][1] > 0, z = tbl::SIG[k][2] > 0;
+                if (tbl::SIG[pm[k]][i] != tbl::SIG[la][i] + x * d[0][i] + y * d[1][i] + z * d[2][i])
+                    return false;
+            }
+    }
+    return kRot.perm[0][6] == 6 && kRot.perm[0][1] == 1;  // edge 0 -> 1: the identity
+}
+static_assert(rot_tables_valid(), "hex8 relabelling tables");
 
 template <typename T>
 __device__ __forceinline__ T recip(T x) {
@@ -85,6 +196,10 @@ template <>
 __device__ __forceinline__ float recip<float>(float x) {
     return __builtin_amdgcn_rcpf(x);  // 1 ulp; the mixed form's local arithmetic
 }
+
+template <typename T>
+__device__ __forceinline__ void jinv_finish(T dx1, T dx2, T dx3, T dy1, T dy2, T dy3, T dz1, T dz2,
+                                            T dz3, T Ji[9], int& bad);
 
 // inverse Jacobian at SCS ip `ip` of location set `sh`: Ji[k*3+d] = ds_k/dx_d
 // (hex_gradient_operator, MasterElementWork.F:2666-2745)
@@ -105,6 +220,14 @@ __device__ __forceinline__ void jinv_scs(int sh, int ip, const T c[8][3], T Ji[9
         dz2 += d1 * c[n][2];
         dz3 += d2 * c[n][2];
     }
+    jinv_finish(dx1, dx2, dx3, dy1, dy2, dy3, dz1, dz2, dz3, Ji, bad);
+}
+
+// inverse of the Jacobian d(x,y,z)/d(s1,s2,s3) given column by column
+// (dx1 = dx/ds1, ...); a non-positive determinant flags the element
+template <typename T>
+__device__ __forceinline__ void jinv_finish(T dx1, T dx2, T dx3, T dy1, T dy2, T dy3, T dz1, T dz2,
+                                            T dz3, T Ji[9], int& bad) {
     const T det = dx1 * (dy2 * dz3 - dz2 * dy3) + dy1 * (dz2 * dx3 - dx2 * dz3) +
                   dz1 * (dx2 * dy3 - dy2 * dx3);
     T test = det;
@@ -147,6 +270,66 @@ __device__ __forceinline__ void area_scs(int ip, const T c[8][3], T A[3]) {
             q[k][d] = v;
         }
     hex8::quad_area(q, A);
+}
+
+// ---- compile-time ip versions (canonical incidences).  Every table index is
+// a constant after unrolling, so the weights become immediates and the
+// zero weights (half of them at the edge-midpoint ips) drop out.
+template <typename T>
+__device__ __forceinline__ void area_c(int ip, const T c[8][3], T A[3]) {
+    T q[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            T v = T(0);
+#pragma unroll
+            for (int n = 0; n < 8; ++n)
+                if (kHex.fw[ip][k][n] != 0.0) v += T(kHex.fw[ip][k][n]) * c[n][d];
+            q[k][d] = v;
+        }
+    hex8::quad_area(q, A);
+}
+
+template <int SH, typename T>
+__device__ __forceinline__ void jinv_c(int ip, const T c[8][3], T Ji[9], int& bad) {
+    T s[3][3];  // s[d][k] = dx_d / ds_k
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            T v = T(0);
+#pragma unroll
+            for (int n = 0; n < 8; ++n)
+                if (kHex.dn[SH][ip][n][k] != 0.0) v += T(kHex.dn[SH][ip][n][k]) * c[n][d];
+            s[d][k] = v;
+        }
+    jinv_finish(s[0][0], s[0][1], s[0][2], s[1][0], s[1][1], s[1][2], s[2][0], s[2][1], s[2][2], Ji,
+                bad);
+}
+
+// dN_n/dx at ip
+template <int SH, typename T>
+__device__ __forceinline__ void gradN_c(int ip, int n, const T Ji[9], T g[3]) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        T v = T(0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (kHex.dn[SH][ip][n][d] != 0.0) v += T(kHex.dn[SH][ip][n][d]) * Ji[3 * d + j];
+        g[j] = v;
+    }
+}
+
+// viscous 3x3 block of one column at one ip, added into acc:
+// -mu (g_i A_j + (g . A) delta_ij)
+template <typename T>
+__device__ __forceinline__ void visc_block(T mu, const T g[3], const T A[3], T acc[9]) {
+    const T gA = g[0] * A[0] + g[1] * A[1] + g[2] * A[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) acc[3 * i + j] -= mu * (g[i] * A[j] + (i == j ? gA : T(0)));
 }
 
 template <typename T>
@@ -252,100 +435,170 @@ struct MomArgs {
     int* err;
 };
 
-// element e's contribution to the momentum row of its local node la
-// (navierStokesAssemblerElemTerms.cpp:256-435 restricted to la's ips)
-template <typename T, int SH, typename LT>
+// SCS ip 0 of the relabelled element r = (la, q) of element e added to the
+// momentum row of la (navierStokesAssemblerElemTerms.cpp:256-435 for one
+// ip; the row node is its left node).  The explicit viscous flux uses the
+// velocity gradient at the ip (the sum over all columns of block times U is
+// -mu (grad U + grad U^T) . A), and the columns lumped into the diagonal need
+// no blocks of their own: the shape-function gradients sum to zero, so the
+// diagonal takes minus the gradients of the kept columns.
+template <typename T, int SH, bool HR, typename LT>
+__device__ __forceinline__ void mom_scs(const MomArgs& a, int r, int e, const int nd[8],
+                                        const T c[8][3], uint64_t sl, const SellRow& sr, int dof,
+                                        double D[9], double R[3], int& bad, LT* lt) {
+    constexpr int IP = 0, NB = 1;
+    const T m = T(c_rot.sg[r]) * T(a.mdot[12 * size_t(e) + c_rot.ip[r]]);
+    T A[3], Ji[9], rl[3];
+    area_c(IP, c, A);
+    jinv_c<SH>(IP, c, Ji, bad);
+    {
+        // upwind advection plus the limited deferred correction
+        // (navierStokesAssemblerElemTerms.cpp:312-340)
+        const bool fw = m > T(0);
+        const size_t nu = size_t(fw ? nd[0] : nd[NB]);
+        T up[3] = {T(a.U[3 * nu]), T(a.U[3 * nu + 1]), T(a.U[3 * nu + 2])};
+        if constexpr (HR) {
+            T dx[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                T xi = T(0);
+#pragma unroll
+                for (int n = 0; n < 8; ++n) xi += T(kHex.N[0][IP][n]) * c[n][d];
+                dx[d] = xi - (fw ? c[0][d] : c[NB][d]);
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                T dc = T(0);
+#pragma unroll
+                for (int j = 0; j < 3; ++j) dc += T(a.beta[3 * nu + i]) * dx[j] * T(a.gradU[9 * nu + 3 * i + j]);
+                up[i] += dc;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) rl[i] = -m * up[i];
+    }
+    // effective viscosity at the ip (velocity shape functions); a constant
+    // viscosity needs no interpolation: the shape functions sum to one
+    T mu = T(a.mu);
+    if (a.mueff) {
+        mu = T(0);
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+            mu += (a.us ? T(kHex.N[1][IP][n]) : T(kHex.N[0][IP][n])) * T(a.mueff[nd[n]]);
+    }
+    // explicit viscous flux + mu (G + G^T) . A, G_ij = du_i/dx_j at the ip
+    // from the parametric derivatives P_id = sum_n u_n,i dN_n/ds_d
+    {
+        T P[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) P[i][0] = P[i][1] = P[i][2] = T(0);
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            if (kHex.dn[SH][IP][n][0] == 0.0 && kHex.dn[SH][IP][n][1] == 0.0 &&
+                kHex.dn[SH][IP][n][2] == 0.0)
+                continue;
+            const size_t o = 3 * size_t(nd[n]);
+            const T u[3] = {T(a.U[o]), T(a.U[o + 1]), T(a.U[o + 2])};
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                if (kHex.dn[SH][IP][n][d] != 0.0)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) P[i][d] += T(kHex.dn[SH][IP][n][d]) * u[i];
+        }
+        T G[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) G[i][j] = P[i][0] * Ji[j] + P[i][1] * Ji[3 + j] + P[i][2] * Ji[6 + j];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            rl[i] += mu * ((G[i][0] + G[0][i]) * A[0] + (G[i][1] + G[1][i]) * A[1] +
+                           (G[i][2] + G[2][i]) * A[2]);
+    }
+    T gD[3] = {T(0), T(0), T(0)};  // gradient sum of the diagonal's columns
+#pragma unroll
+    for (int p = 1; p < 5; ++p) {  // the row node's SCS neighbours 1 (across ip 0), 3, 4
+        if (p == 2) continue;
+        T g[3], acc[9];
+        gradN_c<SH>(IP, p, Ji, g);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) acc[k] = T(0);
+        visc_block(mu, g, A, acc);
+        if (p == NB) {  // the node across this SCS
+            const T adv = T(0.5) * (m - fabs(m));
+            acc[0] += adv;
+            acc[4] += adv;
+            acc[8] += adv;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gD[d] -= g[d];
+        scatter_block(a.vals, sr, dof, sl, c_rot.perm[r][p], acc, D, lt);
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {  // kept only by full-stencil rows
+        const int k = kFar[p];
+        if (kHex.dn[SH][IP][k][0] == 0.0 && kHex.dn[SH][IP][k][1] == 0.0 && kHex.dn[SH][IP][k][2] == 0.0)
+            continue;  // zero block at this ip
+        const int cn = c_rot.perm[r][k];
+        if (slot_of(sl, cn) == 0xFF) continue;
+        T g[3], acc[9];
+        gradN_c<SH>(IP, k, Ji, g);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc[q] = T(0);
+        visc_block(mu, g, A, acc);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gD[d] -= g[d];
+        scatter_block(a.vals, sr, dof, sl, cn, acc, D, lt);
+    }
+    {  // the row node's own column and the lumped ones
+        T acc[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) acc[k] = T(0);
+        visc_block(mu, gD, A, acc);
+        const T adv = T(0.5) * (m + fabs(m));
+        acc[0] += adv;
+        acc[4] += adv;
+        acc[8] += adv;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) D[k] += double(acc[k]);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) R[i] += double(rl[i]);
+}
+
+// element e's contribution to the momentum row of its local node la: the
+// element is gathered once, relabelled so that la is node 0, and its three
+// SCS ips touching la are added as ip 0 of the three relabellings
+template <typename T, int SH, bool HR, typename LT>
 __device__ __forceinline__ void mom_incidence(const MomArgs& a, int la, int e, uint64_t sl,
                                               const SellRow& sr, int dof, const double xr[3],
                                               double D[9], double R[3], int& bad, LT* lt) {
+    const int* ce = a.conn + 8 * size_t(e);
     int nd[8];
-    load_conn(a.conn, e, nd);
-    T A[3][3], Ji[3][9], m[3], sg[3];
-    int ipq[3];
-    T dxu[3][3];  // x_ip - x_upwind (high resolution)
-    {
-        T c[8][3];
-        gather_rel(a.xyz, nd, xr, c);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int ip = c_hex.node_ip[la][q];
-            ipq[q] = ip;
-            sg[q] = T(c_hex.node_sg[la][q]);
-            area_scs(ip, c, A[q]);
-            jinv_scs(SH, ip, c, Ji[q], bad);
-            m[q] = T(a.mdot[12 * size_t(e) + ip]);
-            if (a.beta) {
-                const int up = m[q] > T(0) ? c_hex.lr[ip][0] : c_hex.lr[ip][1];
-                for (int d = 0; d < 3; ++d) {
-                    T xi = T(0);
-                    for (int n = 0; n < 8; ++n) xi += T(c_hex.N[0][ip][n]) * c[n][d];
-                    dxu[q][d] = xi - c[up][d];
-                }
-            }
-        }
-    }
-    // effective viscosity at the ips, velocity shape functions (a constant
-    // viscosity needs no interpolation: the shape functions sum to one)
-    T muq[3] = {T(a.mu), T(a.mu), T(a.mu)};
-    if (a.mueff) {
-        muq[0] = muq[1] = muq[2] = T(0);
+    for (int k = 0; k < 8; ++k) nd[k] = ce[c_rot.perm[3 * la][k]];
+    T c[8][3];
+    gather_rel(a.xyz, nd, xr, c);
 #pragma unroll 1
-        for (int n = 0; n < 8; ++n) {
-            const T mn = T(a.mueff[nd[n]]);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) muq[q] += T(c_hex.N[a.us][ipq[q]][n]) * mn;
-        }
-    }
-    T rl[3] = {0, 0, 0};
-    // upwind advection flux (the deferred correction vanishes for upwind)
-#pragma unroll
     for (int q = 0; q < 3; ++q) {
-        const int nu = m[q] > T(0) ? a.conn[8 * size_t(e) + c_hex.lr[ipq[q]][0]]
-                                   : a.conn[8 * size_t(e) + c_hex.lr[ipq[q]][1]];
-        for (int i = 0; i < 3; ++i) {
-            T up = T(a.U[3 * size_t(nu) + i]);
-            if (a.beta) {  // limited deferred correction (navierStokesAssemblerElemTerms.cpp:312-340)
-                T dc = T(0);
-                for (int j = 0; j < 3; ++j)
-                    dc += T(a.beta[3 * size_t(nu) + i]) * dxu[q][j] * T(a.gradU[9 * size_t(nu) + 3 * i + j]);
-                up += dc;
-            }
-            rl[i] -= sg[q] * (m[q] * up);
-        }
-    }
-#pragma unroll 1
-    for (int cn = 0; cn < 8; ++cn) {
-        const size_t o = 3 * size_t(a.conn[8 * size_t(e) + cn]);
-        const T uc[3] = {T(a.U[o]), T(a.U[o + 1]), T(a.U[o + 2])};
-        T acc[9];
-        for (int k = 0; k < 9; ++k) acc[k] = T(0);
+        mom_scs<T, SH, HR>(a, 3 * la + q, e, nd, c, sl, sr, dof, D, R, bad, lt);
+        int t[8];
+        T u[8][3];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int ip = ipq[q];
-            T adv = T(0);
-            if (cn == c_hex.lr[ip][0]) adv += T(0.5) * (m[q] + fabs(m[q]));
-            if (cn == c_hex.lr[ip][1]) adv += T(0.5) * (m[q] - fabs(m[q]));
-            acc[0] += sg[q] * adv;
-            acc[4] += sg[q] * adv;
-            acc[8] += sg[q] * adv;
-            T g[3];
-            gradN_scs(SH, ip, cn, Ji[q], g);
-            const T gA = g[0] * A[q][0] + g[1] * A[q][1] + g[2] * A[q][2];
-            const T mu = muq[q];
-            for (int i = 0; i < 3; ++i) {
-                T ri = T(0);
-                for (int j = 0; j < 3; ++j) {
-                    T blk = -mu * g[i] * A[q][j];
-                    if (i == j) blk += -mu * gA;
-                    acc[3 * i + j] += sg[q] * blk;
-                    ri += blk * uc[j];
-                }
-                rl[i] -= sg[q] * ri;
-            }
+        for (int k = 0; k < 8; ++k) {
+            t[k] = nd[kRho[k]];
+            u[k][0] = c[kRho[k]][0];
+            u[k][1] = c[kRho[k]][1];
+            u[k][2] = c[kRho[k]][2];
         }
-        scatter_block(a.vals, sr, dof, sl, cn, acc, D, lt);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            nd[k] = t[k];
+            c[k][0] = u[k][0];
+            c[k][1] = u[k][1];
+            c[k][2] = u[k][2];
+        }
     }
-    for (int i = 0; i < 3; ++i) R[i] += double(rl[i]);
 }
 
 // laminar no-slip wall: the face ip of side (f, fk) nearest to this row
@@ -431,7 +684,7 @@ __device__ __forceinline__ void mom_wall(const MomArgs& a, int f, int fk, uint64
 // LDS: 0 = off-diagonal blocks read-modify-written in HBM (256-lane blocks);
 // 1 / 2 = summed in a per-wavefront LDS tile in float / double and written
 // once (64-lane blocks, one wavefront each).
-template <typename T, int SH, int LDS>
+template <typename T, int SH, int LDS, bool HR>
 __global__ void __launch_bounds__(LDS ? 64 : 256) k_momentum_rows(const MomArgs a) {
     using LT = std::conditional_t<LDS == 0, void, std::conditional_t<LDS == 2, double, float>>;
     using LS = std::conditional_t<LDS == 2, double, float>;
@@ -478,12 +731,12 @@ __global__ void __launch_bounds__(LDS ? 64 : 256) k_momentum_rows(const MomArgs 
     for (int la = 0; la < 8; ++la) {
         const size_t o = size_t(la) * a.nn + row;
         const int e = a.inc_la[o];
-        if (e >= 0) mom_incidence<T, SH>(a, la, e, a.slots_la[o], sr, dof, xr, D, R, bad, lt);
+        if (e >= 0) mom_incidence<T, SH, HR>(a, la, e, a.slots_la[o], sr, dof, xr, D, R, bad, lt);
     }
 #pragma unroll 1
     for (int k = a.ext_ptr[row]; k < a.ext_ptr[row + 1]; ++k)
-        mom_incidence<T, SH>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, D, R, bad,
-                             lt);
+        mom_incidence<T, SH, HR>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, D, R,
+                                 bad, lt);
 #pragma unroll 1
     for (int k = a.binc_ptr[row]; k < a.binc_ptr[row + 1]; ++k) {
         const int pk = a.binc[k];
@@ -556,12 +809,13 @@ void launch_assemble_momentum(Flow& f) {
     auto go = [&](auto kern, unsigned B) {
         kern<<<xcd_grid(m.nn, B), B, 0, f.stream>>>(a);
     };
+    const bool hr = a.beta != nullptr;
     if (a.ugrad_shifted) {
-        if (mixed) go(k_momentum_rows<float, 1, 1>, 64);
-        else go(k_momentum_rows<double, 1, 2>, 64);
+        if (mixed) go(hr ? k_momentum_rows<float, 1, 1, true> : k_momentum_rows<float, 1, 1, false>, 64);
+        else go(hr ? k_momentum_rows<double, 1, 2, true> : k_momentum_rows<double, 1, 2, false>, 64);
     } else {
-        if (mixed) go(k_momentum_rows<float, 0, 1>, 64);
-        else go(k_momentum_rows<double, 0, 2>, 64);
+        if (mixed) go(hr ? k_momentum_rows<float, 0, 1, true> : k_momentum_rows<float, 0, 1, false>, 64);
+        else go(hr ? k_momentum_rows<double, 0, 2, true> : k_momentum_rows<double, 0, 2, false>, 64);
     }
     OAGPU_HIP(hipGetLastError());
 }
@@ -591,81 +845,146 @@ struct PcArgs {
     int* err;
 };
 
-// element e's contribution to the p' row of its local node la
-// (pressureCorrectionAssemblerElemTerms.cpp:390-610 restricted to la's ips)
+// rotate the registers of a relabelled element to the next relabelling
+template <typename V, int W>
+__device__ __forceinline__ void rotate_nodes(V v[8][W]) {
+    V t[8][W];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int j = 0; j < W; ++j) t[k][j] = v[kRho[k]][j];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int j = 0; j < W; ++j) v[k][j] = t[k][j];
+}
+
+// SCS ip 0 of the relabelled element r = (la, q) added to the p' row of la
+// (pressureCorrectionAssemblerElemTerms.cpp:390-610 for one ip; the row node
+// is its left node).  The columns lumped into the diagonal take minus the
+// kept columns' coefficients: the shape-function gradients sum to zero.
+// Nodal values: in registers, rotated with the coordinates (REG, the FP32
+// form), or read per ip from L1/L2 (the FP64 form, whose registers are full).
+template <typename T, int SP, bool REG>
+__device__ __forceinline__ void pc_scs(const PcArgs& a, int r, const int nd[8], double pr,
+                                       const T c[8][3], const T un[8][3], const T dn[8][3],
+                                       const T pn[8][1], const T Gp[3], uint64_t sl,
+                                       const SellRow& sr, int dof, double& D, double& R, int& bad,
+                                       double* lt) {
+    constexpr int IP = 0;
+    T A[3], Ji[9];
+    area_c(IP, c, A);
+    jinv_c<SP>(IP, c, Ji, bad);
+    const bool us = a.u_shifted;
+    T uip[3] = {0, 0, 0}, duip[3] = {0, 0, 0}, P[3] = {0, 0, 0};
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const T w = us ? T(kHex.N[1][IP][n]) : T(kHex.N[0][IP][n]);
+        T u[3], du[3], pv;
+        if constexpr (REG) {
+            for (int j = 0; j < 3; ++j) u[j] = un[n][j], du[j] = dn[n][j];
+            pv = pn[n][0];
+        } else {
+            const size_t o = 3 * size_t(nd[n]);
+            for (int j = 0; j < 3; ++j) u[j] = T(a.U[o + j]), du[j] = T(a.du[o + j]);
+            pv = T(a.p[nd[n]] - pr);
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            uip[j] += w * u[j];
+            duip[j] += w * du[j];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (kHex.dn[SP][IP][n][d] != 0.0) P[d] += T(kHex.dn[SP][IP][n][d]) * pv;
+    }
+    const T rho = T(a.rho);
+    T md = T(0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const T dp = P[0] * Ji[j] + P[1] * Ji[3 + j] + P[2] * Ji[6 + j];
+        md += rho * uip[j] * A[j];
+        md -= rho * duip[j] * (dp - Gp[j]) * A[j];
+    }
+    R -= double(md);
+    T lD = T(0);
+    auto column = [&](int k) {
+        T g[3];
+        gradN_c<SP>(IP, k, Ji, g);
+        T lf = T(0);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) lf += -rho * duip[j] * g[j] * A[j];
+        lD -= lf;
+        return lf;
+    };
+#pragma unroll
+    for (int k = 1; k < 5; ++k) {  // the row node's SCS neighbours 1 (across ip 0), 3, 4
+        if (k == 2) continue;
+        const T lf = column(k);
+        const int sp = slot_of(sl, c_rot.perm[r][k]);
+        if (sp == 0xFF || sp == dof) D += double(lf);
+        else scalar_scatter(a.vals, sr, lt, sp, double(lf));
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {  // kept only by full-stencil rows
+        const int k = kFar[p];
+        if (kHex.dn[SP][IP][k][0] == 0.0 && kHex.dn[SP][IP][k][1] == 0.0 && kHex.dn[SP][IP][k][2] == 0.0)
+            continue;  // zero coefficient at this ip
+        const int sp = slot_of(sl, c_rot.perm[r][k]);
+        if (sp == 0xFF) continue;
+        const T lf = column(k);
+        if (sp == dof) D += double(lf);
+        else scalar_scatter(a.vals, sr, lt, sp, double(lf));
+    }
+    D += double(lD);
+}
+
+// element e's contribution to the p' row of its local node la: gathered
+// once, relabelled so that la is node 0, its three SCS ips touching la added
+// as ip 0 of the three relabellings
 template <typename T, int SP>
 __device__ __forceinline__ void pc_incidence(const PcArgs& a, int la, int e, uint64_t sl,
                                              const SellRow& sr, int dof, const double xr[3],
                                              double pr, double& D, double& R, int& bad,
                                              double* lt) {
+    const int* ce = a.conn + 8 * size_t(e);
     int nd[8];
-    load_conn(a.conn, e, nd);
-    T A[3][3], Ji[3][9], sg[3];
-    int ipq[3];
-    {
-        T c[8][3];
-        gather_rel(a.xyz, nd, xr, c);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int ip = c_hex.node_ip[la][q];
-            ipq[q] = ip;
-            sg[q] = T(c_hex.node_sg[la][q]);
-            area_scs(ip, c, A[q]);
-            jinv_scs(SP, ip, c, Ji[q], bad);
-        }
-    }
-    const int us = a.u_shifted ? 1 : 0;
-    T uip[3][3], duip[3][3], dp[3][3], Gp[3] = {0, 0, 0};
-    for (int q = 0; q < 3; ++q)
-        for (int j = 0; j < 3; ++j) uip[q][j] = duip[q][j] = dp[q][j] = T(0);
-#pragma unroll 1
+    for (int k = 0; k < 8; ++k) nd[k] = ce[c_rot.perm[3 * la][k]];
+    constexpr bool REG = sizeof(T) == 4;
+    T c[8][3], un[REG ? 8 : 1][3], dn[REG ? 8 : 1][3], pn[REG ? 8 : 1][1], Gp[3] = {0, 0, 0};
+    gather_rel(a.xyz, nd, xr, c);
+#pragma unroll
     for (int n = 0; n < 8; ++n) {
-        const int node = a.conn[8 * size_t(e) + n];
-        const size_t o = 3 * size_t(node);
-        const T w = T(a.scvw[8 * size_t(e) + n]);
-        const T un[3] = {T(a.U[o]), T(a.U[o + 1]), T(a.U[o + 2])};
-        const T dn[3] = {T(a.du[o]), T(a.du[o + 1]), T(a.du[o + 2])};
-        const T pn = T(a.p[node] - pr);
+        const size_t o = 3 * size_t(nd[n]);
+        const T w = T(a.scvw[8 * size_t(e) + c_rot.perm[3 * la][n]]);
+#pragma unroll
         for (int j = 0; j < 3; ++j) Gp[j] += w * T(a.gradP[o + j]);
+        if constexpr (REG) {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const T r = T(c_hex.N[us][ipq[q]][n]);
-            T g[3];
-            gradN_scs(SP, ipq[q], n, Ji[q], g);
             for (int j = 0; j < 3; ++j) {
-                uip[q][j] += r * un[j];
-                duip[q][j] += r * dn[j];
-                dp[q][j] += g[j] * pn;
+                un[n][j] = T(a.U[o + j]);
+                dn[n][j] = T(a.du[o + j]);
             }
+            pn[n][0] = T(a.p[nd[n]] - pr);
         }
-    }
-    const T rho = T(a.rho);
-    T rl = T(0);
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        T md = T(0);
-        for (int j = 0; j < 3; ++j) {
-            md += rho * uip[q][j] * A[q][j];
-            md -= rho * duip[q][j] * (dp[q][j] - Gp[j]) * A[q][j];
-        }
-        rl -= sg[q] * md;
     }
 #pragma unroll 1
-    for (int n = 0; n < 8; ++n) {
-        T acc = T(0);
+    for (int q = 0; q < 3; ++q) {
+        pc_scs<T, SP, REG>(a, 3 * la + q, nd, pr, c, un, dn, pn, Gp, sl, sr, dof, D, R, bad, lt);
+        rotate_nodes<T, 3>(c);
+        if constexpr (REG) {
+            rotate_nodes<T, 3>(un);
+            rotate_nodes<T, 3>(dn);
+            rotate_nodes<T, 1>(pn);
+        } else {
+            int t[8];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            T g[3];
-            gradN_scs(SP, ipq[q], n, Ji[q], g);
-            T lf = T(0);
-            for (int j = 0; j < 3; ++j) lf += -rho * duip[q][j] * g[j] * A[q][j];
-            acc += sg[q] * lf;
+            for (int k = 0; k < 8; ++k) t[k] = nd[kRho[k]];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) nd[k] = t[k];
         }
-        const int sp = slot_of(sl, n);
-        if (sp == 0xFF || sp == dof) D += double(acc);
-        else scalar_scatter(a.vals, sr, lt, sp, double(acc));
     }
-    R += double(rl);
 }
 
 // pressureCorrectionAssembler interior terms for one row, with the
@@ -833,88 +1152,142 @@ __device__ void sst_node_terms(const ScArgs& a, int i, double& lhs, double& rhs)
     rhs = r;
 }
 
-// element e's upwind advection-diffusion contribution to the scalar row of
-// its local node la (phiAssemblerElemTerms.hpp:233-406 restricted to la's
-// ips; the deferred correction vanishes for upwind advection)
-template <typename T, int SG>
+// SCS ip 0 of the relabelled element r = (la, q) added to the k / omega row
+// of la (phiAssemblerElemTerms.hpp:171-406 for one ip; the row node is its
+// left node): upwind advection with the limited deferred correction and
+// diffusion with Gamma = mu + sigma(F1) mu_t interpolated to the ip.  The
+// explicit diffusion flux uses grad phi at the ip; the lumped columns take
+// minus the kept columns' coefficients (the shape-function gradients sum to
+// zero).  Nodal values: in registers, rotated with the coordinates (REG),
+// or read per ip from L1/L2.
+template <typename T, int SG, bool HR, bool REG>
+__device__ __forceinline__ void sc_scs(const ScArgs& a, int r, int e, const int nd[8],
+                                       const T c[8][3], const T ph[8][1], const T gm[8][1],
+                                       uint64_t sl, const SellRow& sr, int dof, double& D,
+                                       double& R, int& bad, double* lt) {
+    constexpr int IP = 0;
+    const T m = T(c_rot.sg[r]) * T(a.mdot[12 * size_t(e) + c_rot.ip[r]]);
+    T A[3], Ji[9];
+    area_c(IP, c, A);
+    jinv_c<SG>(IP, c, Ji, bad);
+    T Gip = T(0), P[3] = {0, 0, 0};
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        T pv, gv;
+        if constexpr (REG) {
+            pv = ph[n][0];
+            gv = gm[n][0];
+        } else {
+            pv = T(a.phi[nd[n]]);
+            const double F1 = a.F1[nd[n]];
+            gv = T(a.mu + a.mut[nd[n]] * (F1 * a.sig1 + (1.0 - F1) * a.sig2));
+        }
+        Gip += (a.us ? T(kHex.N[1][IP][n]) : T(kHex.N[0][IP][n])) * gv;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (kHex.dn[SG][IP][n][d] != 0.0) P[d] += T(kHex.dn[SG][IP][n][d]) * pv;
+    }
+    // upwind advection plus the limited deferred correction
+    // (phiAssemblerElemTerms.hpp:291-340)
+    const bool fw = m > T(0);
+    const size_t nu = size_t(fw ? nd[0] : nd[1]);
+    T up = T(a.phi[nu]);
+    if constexpr (HR) {
+        T dc = T(0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            T xi = T(0);
+#pragma unroll
+            for (int n = 0; n < 8; ++n) xi += T(kHex.N[0][IP][n]) * c[n][d];
+            dc += T(a.beta[nu]) * (xi - (fw ? c[0][d] : c[1][d])) * T(a.gphi[3 * nu + d]);
+        }
+        up += dc;
+    }
+    T rl = -(m * up);
+    // explicit diffusion flux: + Gamma grad phi . A
+    {
+        T f = T(0);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) f += (P[0] * Ji[j] + P[1] * Ji[3 + j] + P[2] * Ji[6 + j]) * A[j];
+        rl += Gip * f;
+    }
+    T lD = T(0);
+    auto column = [&](int k) {
+        T g[3];
+        gradN_c<SG>(IP, k, Ji, g);
+        T lf = T(0);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) lf += -Gip * g[j] * A[j];
+        lD -= lf;
+        return lf;
+    };
+#pragma unroll
+    for (int k = 1; k < 5; ++k) {  // the row node's SCS neighbours 1 (across ip 0), 3, 4
+        if (k == 2) continue;
+        T lf = column(k);
+        if (k == 1) lf += T(0.5) * (m - fabs(m));
+        const int sp = slot_of(sl, c_rot.perm[r][k]);
+        if (sp == 0xFF || sp == dof) D += double(lf);
+        else scalar_scatter(a.vals, sr, lt, sp, double(lf));
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {  // kept only by full-stencil rows
+        const int k = kFar[p];
+        if (kHex.dn[SG][IP][k][0] == 0.0 && kHex.dn[SG][IP][k][1] == 0.0 && kHex.dn[SG][IP][k][2] == 0.0)
+            continue;  // zero coefficient at this ip
+        const int sp = slot_of(sl, c_rot.perm[r][k]);
+        if (sp == 0xFF) continue;
+        const T lf = column(k);
+        if (sp == dof) D += double(lf);
+        else scalar_scatter(a.vals, sr, lt, sp, double(lf));
+    }
+    D += double(lD + T(0.5) * (m + fabs(m)));
+    R += double(rl);
+}
+
+// element e's contribution to the k / omega row of its local node la:
+// gathered once, relabelled so that la is node 0, its three SCS ips touching
+// la added as ip 0 of the three relabellings
+template <typename T, int SG, bool HR>
 __device__ __forceinline__ void sc_incidence(const ScArgs& a, int la, int e, uint64_t sl,
                                              const SellRow& sr, int dof, const double xr[3],
                                              double& D, double& R, int& bad, double* lt) {
+    constexpr bool REG = sizeof(T) == 4;
+    const int* ce = a.conn + 8 * size_t(e);
     int nd[8];
-    load_conn(a.conn, e, nd);
-    T A[3][3], Ji[3][9], sg[3], Gip[3] = {0, 0, 0}, m[3];
-    int ipq[3];
-    T dxu[3][3];
-    {
-        T c[8][3];
-        gather_rel(a.xyz, nd, xr, c);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int ip = c_hex.node_ip[la][q];
-            ipq[q] = ip;
-            sg[q] = T(c_hex.node_sg[la][q]);
-            area_scs(ip, c, A[q]);
-            jinv_scs(SG, ip, c, Ji[q], bad);
-            m[q] = T(a.mdot[12 * size_t(e) + ip]);
-            if (a.beta) {
-                const int up = m[q] > T(0) ? c_hex.lr[ip][0] : c_hex.lr[ip][1];
-                for (int d = 0; d < 3; ++d) {
-                    T xi = T(0);
-                    for (int n = 0; n < 8; ++n) xi += T(c_hex.N[0][ip][n]) * c[n][d];
-                    dxu[q][d] = xi - c[up][d];
-                }
-            }
+    for (int k = 0; k < 8; ++k) nd[k] = ce[c_rot.perm[3 * la][k]];
+    T c[8][3], ph[REG ? 8 : 1][1], gm[REG ? 8 : 1][1];
+    gather_rel(a.xyz, nd, xr, c);
+    if constexpr (REG) {
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            ph[n][0] = T(a.phi[nd[n]]);
+            const double F1 = a.F1[nd[n]];
+            gm[n][0] = T(a.mu + a.mut[nd[n]] * (F1 * a.sig1 + (1.0 - F1) * a.sig2));
         }
     }
-    // Gamma = mu + mu_t sigma(F1) at the nodes, interpolated to the ips
 #pragma unroll 1
-    for (int n = 0; n < 8; ++n) {
-        const double F1 = a.F1[nd[n]];
-        const T G = T(a.mu + a.mut[nd[n]] * (F1 * a.sig1 + (1.0 - F1) * a.sig2));
-#pragma unroll
-        for (int q = 0; q < 3; ++q) Gip[q] += T(c_hex.N[a.us][ipq[q]][n]) * G;
-    }
-    T rl = T(0);
-#pragma unroll
     for (int q = 0; q < 3; ++q) {
-        const int L = c_hex.lr[ipq[q]][0], Rr = c_hex.lr[ipq[q]][1];
-        const int nu = m[q] > T(0) ? nd[L] : nd[Rr];
-        T up = T(a.phi[nu]);
-        if (a.beta) {  // limited deferred correction (phiAssemblerElemTerms.hpp:291-340)
-            T dc = T(0);
-            for (int j = 0; j < 3; ++j) dc += T(a.beta[nu]) * dxu[q][j] * T(a.gphi[3 * size_t(nu) + j]);
-            up += dc;
+        sc_scs<T, SG, HR, REG>(a, 3 * la + q, e, nd, c, ph, gm, sl, sr, dof, D, R, bad, lt);
+        rotate_nodes<T, 3>(c);
+        if constexpr (REG) {
+            rotate_nodes<T, 1>(ph);
+            rotate_nodes<T, 1>(gm);
         }
-        rl -= sg[q] * (m[q] * up);
-    }
-#pragma unroll 1
-    for (int cn = 0; cn < 8; ++cn) {
-        const T pc = T(a.phi[nd[cn]]);
-        T acc = T(0);
+        int t[8];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int ip = ipq[q];
-            if (cn == c_hex.lr[ip][0]) acc += sg[q] * (T(0.5) * (m[q] + fabs(m[q])));
-            if (cn == c_hex.lr[ip][1]) acc += sg[q] * (T(0.5) * (m[q] - fabs(m[q])));
-            T g[3];
-            gradN_scs(SG, ip, cn, Ji[q], g);
-            T lf = T(0);
-            for (int j = 0; j < 3; ++j) lf += -Gip[q] * g[j] * A[q][j];
-            acc += sg[q] * lf;
-            rl -= sg[q] * (lf * pc);
-        }
-        const int sp = slot_of(sl, cn);
-        if (sp == 0xFF || sp == dof) D += double(acc);
-        else scalar_scatter(a.vals, sr, lt, sp, double(acc));
+        for (int k = 0; k < 8; ++k) t[k] = nd[kRho[k]];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) nd[k] = t[k];
     }
-    R += double(rl);
 }
 
 // one k / omega row: node terms and sources, interior terms, relaxation
 // (phiAssembler.h:966-1006), boundary rhs relaxation x 0.75
 // (turbulentKineticEnergyAssembler.h:44-48, phiAssembler.h:1008-1152), and
 // for omega the no-slip wall rows reduced to their diagonal with b = 0
-template <typename T, int SG>
+template <typename T, int SG, bool HR>
 __global__ void __launch_bounds__(64) k_scalar_rows(const ScArgs a) {
     __shared__ double tile[kLdsSlots * 64];
     double* lt = tile + threadIdx.x;
@@ -932,12 +1305,12 @@ __global__ void __launch_bounds__(64) k_scalar_rows(const ScArgs a) {
     for (int la = 0; la < 8; ++la) {
         const size_t o = size_t(la) * a.nn + row;
         const int e = a.inc_la[o];
-        if (e >= 0) sc_incidence<T, SG>(a, la, e, a.slots_la[o], sr, dof, xr, D, R, bad, lt);
+        if (e >= 0) sc_incidence<T, SG, HR>(a, la, e, a.slots_la[o], sr, dof, xr, D, R, bad, lt);
     }
 #pragma unroll 1
     for (int k = a.ext_ptr[row]; k < a.ext_ptr[row + 1]; ++k)
-        sc_incidence<T, SG>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, D, R, bad,
-                            lt);
+        sc_incidence<T, SG, HR>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, D, R,
+                                bad, lt);
     D *= a.urf_inv;
     const bool wall = a.bnode[row] != 0;
     if (wall) R *= 0.75;
@@ -998,12 +1371,14 @@ void launch_assemble_scalar(Flow& f, int which) {
     const unsigned B = 64, G = xcd_grid(m.nn, B);
     const bool mixed = f.prm.precision == OAGPU_MIXED;
     const bool gs = (which == 0 ? f.sst.kgrad_shifted : f.sst.wgrad_shifted) != 0;
+    const bool hr = a.beta != nullptr;
+    auto go = [&](auto kern) { kern<<<G, B, 0, f.stream>>>(a); };
     if (gs) {
-        if (mixed) k_scalar_rows<float, 1><<<G, B, 0, f.stream>>>(a);
-        else k_scalar_rows<double, 1><<<G, B, 0, f.stream>>>(a);
+        if (mixed) go(hr ? k_scalar_rows<float, 1, true> : k_scalar_rows<float, 1, false>);
+        else go(hr ? k_scalar_rows<double, 1, true> : k_scalar_rows<double, 1, false>);
     } else {
-        if (mixed) k_scalar_rows<float, 0><<<G, B, 0, f.stream>>>(a);
-        else k_scalar_rows<double, 0><<<G, B, 0, f.stream>>>(a);
+        if (mixed) go(hr ? k_scalar_rows<float, 0, true> : k_scalar_rows<float, 0, false>);
+        else go(hr ? k_scalar_rows<double, 0, true> : k_scalar_rows<double, 0, false>);
     }
     OAGPU_HIP(hipGetLastError());
 }
@@ -1123,53 +1498,50 @@ void launch_update_mdot(Flow& f) {
 // -------------------------------------------------------------- gradients
 // nodeField::updateGradientField (nodeField.hpp:3253-4950): incremental
 // Green-Gauss with relaxation, interior ips then wall-side ips
-template <int N>
+template <int N, int SH>
 __device__ __forceinline__ void grad_incidence(const double* xyz, const int* conn, const double* phi,
                                                int la, int e, const double xr[3],
-                                               const double pa[N], double iv, double gurf, int sh,
+                                               const double pa[N], double iv, double gurf,
                                                double G[3 * N]) {
+    // the element relabelled so that la is node 0: its ips are 0, 3, 8, each
+    // with the row node on the left
+    const int* ce = conn + 8 * size_t(e);
     int nd[8];
-    load_conn(conn, e, nd);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) nd[k] = ce[c_rot.perm[3 * la][k]];
     double c[8][3];
     gather_rel(xyz, nd, xr, c);
     double pip[3][N];
     for (int q = 0; q < 3; ++q)
         for (int i = 0; i < N; ++i) pip[q][i] = 0.0;
-    int ipq[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) ipq[q] = c_hex.node_ip[la][q];
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
         double pn[N];
         for (int i = 0; i < N; ++i) pn[i] = phi[N * size_t(nd[n]) + i];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            const double r = c_hex.N[sh][ipq[q]][n];
-            for (int i = 0; i < N; ++i) pip[q][i] += r * pn[i];
+            const double r = kHex.N[SH][kCanIp[q]][n];
+            if (r != 0.0)
+                for (int i = 0; i < N; ++i) pip[q][i] += r * pn[i];
         }
     }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         double A[3];
-        area_scs(ipq[q], c, A);
-        const bool left = c_hex.node_sg[la][q] > 0;
+        area_c(kCanIp[q], c, A);
         for (int i = 0; i < N; ++i)
-            for (int j = 0; j < 3; ++j) {
-                const double t = gurf * (pip[q][i] - pa[i]) * A[j] * iv;
-                if (left) G[3 * i + j] += t;
-                else G[3 * i + j] -= t;
-            }
+            for (int j = 0; j < 3; ++j) G[3 * i + j] += gurf * (pip[q][i] - pa[i]) * A[j] * iv;
     }
 }
 
-template <int N>
+template <int N, int SH>
 __global__ void __launch_bounds__(256) k_gradient(int nn, const double* xyz, const int* conn,
                                                   const int* inc_la, const uint64_t* slots_la,
                                                   const int* ext_ptr, const int* ext,
                                                   const uint64_t* ext_slots, const int* binc_ptr,
                                                   const int* binc, const int* bf_elem,
                                                   const int* bf_ord, const double* vol,
-                                                  const double* phi, int shifted, double gurf,
+                                                  const double* phi, double gurf,
                                                   double* grad) {
     const int a = xcd_block() * blockDim.x + threadIdx.x;
     if (a >= nn) return;
@@ -1182,11 +1554,11 @@ __global__ void __launch_bounds__(256) k_gradient(int nn, const double* xyz, con
 #pragma unroll 1
     for (int la = 0; la < 8; ++la) {
         const int e = inc_la[size_t(la) * nn + a];
-        if (e >= 0) grad_incidence<N>(xyz, conn, phi, la, e, xr, pa, iv, gurf, shifted, G);
+        if (e >= 0) grad_incidence<N, SH>(xyz, conn, phi, la, e, xr, pa, iv, gurf, G);
     }
 #pragma unroll 1
     for (int k = ext_ptr[a]; k < ext_ptr[a + 1]; ++k)
-        grad_incidence<N>(xyz, conn, phi, ext[k] & 7, ext[k] >> 3, xr, pa, iv, gurf, shifted, G);
+        grad_incidence<N, SH>(xyz, conn, phi, ext[k] & 7, ext[k] >> 3, xr, pa, iv, gurf, G);
 #pragma unroll 1
     for (int k = binc_ptr[a]; k < binc_ptr[a + 1]; ++k) {
         const int f = binc[k] >> 2, fk = binc[k] & 3;
@@ -1201,7 +1573,7 @@ __global__ void __launch_bounds__(256) k_gradient(int nn, const double* xyz, con
         hex8::quad_face_areav(fp, fk, A);
         for (int i = 0; i < N; ++i) {
             double pip = 0.0;
-            for (int q = 0; q < 4; ++q) pip += hex8::c_quad_shape[shifted][fk][q] * pf[q][i];
+            for (int q = 0; q < 4; ++q) pip += hex8::c_quad_shape[SH][fk][q] * pf[q][i];
             for (int j = 0; j < 3; ++j) G[3 * i + j] += gurf * (pip - pa[i]) * A[j] * iv;
         }
     }
@@ -1308,12 +1680,16 @@ void launch_blending(Flow& f, int which) {
 
 void launch_gradient(Flow& f, int which) {
     DevMesh& m = *f.m;
-    const unsigned B = 256;
-    if (which == 0)
-        k_gradient<3><<<xcd_grid(m.nn, B), B, 0, f.stream>>>(
-            m.nn, m.xyz.p, m.conn.p, m.inc_la.p, m.slots_la.p, m.ext_ptr.p, m.ext.p,
-            m.ext_slots.p, m.binc_ptr.p, m.binc.p, m.bf_elem.p, m.bf_ord.p, m.vol.p, f.U.p, f.prm.u_shifted, f.prm.grad_urf, f.gradU.p);
-    else {
+    const unsigned B = 256, G = xcd_grid(m.nn, B);
+    auto go = [&](auto kern, const double* phi, double* grad) {
+        kern<<<G, B, 0, f.stream>>>(m.nn, m.xyz.p, m.conn.p, m.inc_la.p, m.slots_la.p, m.ext_ptr.p,
+                                    m.ext.p, m.ext_slots.p, m.binc_ptr.p, m.binc.p, m.bf_elem.p,
+                                    m.bf_ord.p, m.vol.p, phi, f.prm.grad_urf, grad);
+    };
+    if (which == 0) {
+        if (f.prm.u_shifted) go(k_gradient<3, 1>, f.U.p, f.gradU.p);
+        else go(k_gradient<3, 0>, f.U.p, f.gradU.p);
+    } else {
         const double* phi = f.p.p;
         double* grad = f.gradP.p;
         int sh = f.prm.p_shifted;
@@ -1323,10 +1699,8 @@ void launch_gradient(Flow& f, int which) {
             grad = which == 2 ? f.gradK.p : f.gradW.p;
             sh = which == 2 ? f.sst.k_shifted : f.sst.w_shifted;
         }
-        k_gradient<1><<<xcd_grid(m.nn, B), B, 0, f.stream>>>(
-            m.nn, m.xyz.p, m.conn.p, m.inc_la.p, m.slots_la.p, m.ext_ptr.p, m.ext.p,
-            m.ext_slots.p, m.binc_ptr.p, m.binc.p, m.bf_elem.p, m.bf_ord.p, m.vol.p, phi, sh,
-            f.prm.grad_urf, grad);
+        if (sh) go(k_gradient<1, 1>, phi, grad);
+        else go(k_gradient<1, 0>, phi, grad);
     }
     OAGPU_HIP(hipGetLastError());
 }
