@@ -25,19 +25,21 @@ BC = np.array([[1.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
 CTL = [20, 0.1, 200, 1.0e-6]   # momentum GMRES max_it/rtol, p' CG max_it/rtol
 
 
-def algorithmic_bytes(nn, ne, nnz):
+def algorithmic_bytes(nn, ne, nnz, amg_value_bytes=4):
     """Compulsory HBM bytes of one launch of each row-owned kernel (each array
-    read once, each output written once; FP64 values, int32 indices)."""
+    read once, each output written once; FP64 values, int32 indices; the AMG
+    hierarchy in FP32 for the mixed form, FP64 for the FP64 form)."""
     conn, mdot, slots = 4 * 8, 8 * 12, 4 * 32
     mom = (nn * (24 + 24 + 8) + ne * (conn + mdot + slots) + nnz * 72 + nn * 24
            + nn * (8 + 8 + 24) + nn * 24)          # fused node terms + du write
     pc = nn * (24 + 24 + 8 + 24 + 24 + 8) + ne * (conn + mdot + slots) + nnz * 8 + nn * 8
     md = nn * 112 + ne * (conn + 16 * 12)
     gu = nn * (24 + 24 + 8 + 72) + ne * conn
-    # fine-level AMG kernels of the FP32 V-cycle (mixed form): sliced-ELL
-    # values + column indices of the true nonzeros, row vectors
-    amg_pre = nnz * (4 + 4) + nn * 4 * 4           # r, dinv in; x, residual out
-    amg_post = nnz * (4 + 4) + nn * (4 * 5 + 4)    # x, agg, r, dinv, coarse e in; z out
+    # fine-level AMG kernels of the V-cycle: sliced-ELL values + column
+    # indices of the true nonzeros, row vectors
+    v = amg_value_bytes
+    amg_pre = nnz * (v + 4) + nn * v * 4           # r, dinv in; x, residual out
+    amg_post = nnz * (v + 4) + nn * (v * 5 + 4)    # x, agg, r, dinv, coarse e in; z out
     return {"momentum": mom, "pcorr": pc, "mdot": md, "gradU": gu, "amg_pre": amg_pre,
             "amg_post": amg_post}
 
@@ -148,9 +150,9 @@ def main():
     print(f"[bench] mixed {args.steps / rmx['seconds']:.3f} it/s {rmx['kernel_ms']}", file=sys.stderr,
           flush=True)
     nn, ne, nnz = rmx["nn"], rmx["ne"], rmx["nnz"]
-    ab = algorithmic_bytes(nn, ne, nnz)
 
     def kern(r):
+        ab = algorithmic_bytes(nn, ne, nnz, 4 if r is rmx else 8)
         out = {}
         for k, ms in r["kernel_ms"].items():
             gbs = ab[k] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
