@@ -114,5 +114,16 @@ __device__ __forceinline__ double* sell_val(double* vals, const SellRow& r, int 
 
 __device__ inline int slot_of(uint64_t sl, int c) { return int((sl >> (8 * c)) & 0xFFu); }
 
+// Item (row or element) of this lane when 64-item groups are walked in the
+// order `order` (ngroups entries): a block of 64 S lanes takes S consecutive
+// groups, lane l of a group its item 64 g + l.  -1 beyond the last item.
+__device__ inline int ordered_item(const int* order, int ngroups, int nitems) {
+    const int per = int(blockDim.x >> 6);
+    const int g = int(xcd_block()) * per + int(threadIdx.x >> 6);
+    if (g >= ngroups) return -1;
+    const int i = order[g] * 64 + int(threadIdx.x & 63u);
+    return i < nitems ? i : -1;
+}
+
 
 }  // namespace oagpu

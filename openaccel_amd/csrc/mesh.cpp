@@ -12,6 +12,7 @@
 #include "sell.hpp"
 
 #include <algorithm>
+#include <functional>
 #include <numeric>
 
 namespace oagpu {
@@ -123,6 +124,60 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
         dm.sell_cols.upload(sh.cols);
         dm.slice_ptr.upload(dm.h_slice_ptr);
         dm.slice_w.upload(dm.h_slice_w);
+    }
+
+    // ---- Morton traversal orders of the slices and element groups
+    {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int i = 0; i < nn; ++i)
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = std::min(lo[d], hm.xyz[3 * size_t(i) + d]);
+                hi[d] = std::max(hi[d], hm.xyz[3 * size_t(i) + d]);
+            }
+        auto morton = [&](const double x[3]) {
+            uint64_t code = 0;
+            uint32_t q[3];
+            for (int d = 0; d < 3; ++d) {
+                const double span = hi[d] > lo[d] ? hi[d] - lo[d] : 1.0;
+                const double t = std::min(std::max((x[d] - lo[d]) / span, 0.0), 1.0);
+                q[d] = uint32_t(t * 1023.0 + 0.5);
+            }
+            for (int b = 9; b >= 0; --b)
+                for (int d = 0; d < 3; ++d) code = (code << 1) | ((q[d] >> b) & 1u);
+            return code;
+        };
+        auto order_groups = [&](int nitems, int nper, const std::function<void(int, double[3])>& pos) {
+            const int ng = (nitems + 63) / 64;
+            std::vector<std::pair<uint64_t, int>> key(ng);
+            for (int g = 0; g < ng; ++g) {
+                double c[3] = {0, 0, 0};
+                const int b = 64 * g, e = std::min(nitems, b + 64);
+                for (int i = b; i < e; ++i) {
+                    double x[3];
+                    pos(i, x);
+                    for (int d = 0; d < 3; ++d) c[d] += x[d];
+                }
+                for (int d = 0; d < 3; ++d) c[d] /= double(e - b);
+                key[g] = {morton(c), g};
+            }
+            std::stable_sort(key.begin(), key.end());
+            std::vector<int> ord(ng);
+            for (int g = 0; g < ng; ++g) ord[g] = key[g].second;
+            (void)nper;
+            return ord;
+        };
+        const std::vector<int> so = order_groups(nn, 1, [&](int i, double x[3]) {
+            for (int d = 0; d < 3; ++d) x[d] = hm.xyz[3 * size_t(i) + d];
+        });
+        OAGPU_REQUIRE(int(so.size()) == dm.nslice, "slice order size");
+        dm.slice_order.upload(so);
+        const std::vector<int> eo = order_groups(ne, 8, [&](int e, double x[3]) {
+            for (int d = 0; d < 3; ++d) x[d] = 0.0;
+            for (int k = 0; k < 8; ++k)
+                for (int d = 0; d < 3; ++d) x[d] += 0.125 * hm.xyz[3 * size_t(hm.conn[8 * size_t(e) + k]) + d];
+        });
+        dm.negroup = int(eo.size());
+        dm.egroup_order.upload(eo);
     }
 
     // ---- node -> element incidences in ascending element order

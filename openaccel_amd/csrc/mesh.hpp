@@ -49,6 +49,14 @@ struct DevMesh {
     int nslice = 0;
     size_t nsell = 0;          // slice_ptr[nslice] * 64
     DBuf<int> slice_ptr, slice_w, sell_cols;
+    // Traversal orders of the row-owned and element-owned kernels: the 64-row
+    // slices and the 64-element groups sorted along a Morton curve of their
+    // centroids.  Wavefronts resident at one time then cover a compact region
+    // of the mesh, and element / node data shared by neighbouring rows is
+    // still in the XCD's L2 at its second use.
+    int negroup = 0;
+    DBuf<int> slice_order;     // [nslice]
+    DBuf<int> egroup_order;    // [negroup]
     DBuf<int> inc_ptr, inc;    // [nn+1], [ninc] element-ordered (set-up kernels)
     DBuf<int> inc_la;          // [8][nn]
     DBuf<uint64_t> slots_la;   // [8][nn]

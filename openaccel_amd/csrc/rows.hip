@@ -400,7 +400,8 @@ __device__ __forceinline__ void scalar_row_store(double* vals, const SellRow& sr
 
 // --------------------------------------------------------------- momentum
 struct MomArgs {
-    int nn;
+    int nn, nslice;
+    const int* slice_order;
     const double* xyz;
     const int* conn;
     const int* slice_ptr;
@@ -691,8 +692,8 @@ __global__ void __launch_bounds__(LDS ? 64 : 256) k_momentum_rows(const MomArgs 
     __shared__ LS tile[LDS ? kLdsSlots * 9 * 64 : 1];
     LT* lt = nullptr;
     if constexpr (LDS != 0) lt = tile + threadIdx.x;
-    const int row = xcd_block() * blockDim.x + threadIdx.x;
-    if (row >= a.nn) return;
+    const int row = ordered_item(a.slice_order, a.nslice, a.nn);
+    if (row < 0) return;
     const SellRow sr = sell_row<9>(a.slice_ptr, a.slice_w, row);
     const int dof = a.diag_off[row];
     const double xr[3] = {a.xyz[3 * size_t(row)], a.xyz[3 * size_t(row) + 1],
@@ -764,6 +765,8 @@ void launch_assemble_momentum(Flow& f) {
     DevMesh& m = *f.m;
     MomArgs a;
     a.nn = m.nn;
+    a.nslice = m.nslice;
+    a.slice_order = m.slice_order.p;
     a.xyz = m.xyz.p;
     a.conn = m.conn.p;
     a.slice_ptr = m.slice_ptr.p;
@@ -822,7 +825,8 @@ void launch_assemble_momentum(Flow& f) {
 
 // ------------------------------------------------------- pressure correction
 struct PcArgs {
-    int nn;
+    int nn, nslice;
+    const int* slice_order;
     const double* xyz;
     const int* conn;
     const int* slice_ptr;
@@ -993,8 +997,8 @@ template <typename T, int SP>
 __global__ void __launch_bounds__(64) k_pcorr_rows(const PcArgs a) {
     __shared__ double tile[kLdsSlots * 64];
     double* lt = tile + threadIdx.x;
-    const int row = xcd_block() * blockDim.x + threadIdx.x;
-    if (row >= a.nn) return;
+    const int row = ordered_item(a.slice_order, a.nslice, a.nn);
+    if (row < 0) return;
     const SellRow sr = sell_row<1>(a.slice_ptr, a.slice_w, row);
     const int dof = a.diag_off[row];
     const double xr[3] = {a.xyz[3 * size_t(row)], a.xyz[3 * size_t(row) + 1],
@@ -1027,6 +1031,8 @@ void launch_assemble_pcorr(Flow& f) {
     DevMesh& m = *f.m;
     PcArgs a;
     a.nn = m.nn;
+    a.nslice = m.nslice;
+    a.slice_order = m.slice_order.p;
     a.xyz = m.xyz.p;
     a.conn = m.conn.p;
     a.slice_ptr = m.slice_ptr.p;
@@ -1068,7 +1074,8 @@ constexpr double kA1 = 0.31, kBetaStar = 0.09, kProdLim = 10.0, kCmu = 0.09;
 }  // namespace sstc
 
 struct ScArgs {
-    int nn;
+    int nn, nslice;
+    const int* slice_order;
     const double* xyz;
     const int* conn;
     const int* slice_ptr;
@@ -1291,8 +1298,8 @@ template <typename T, int SG, bool HR>
 __global__ void __launch_bounds__(64) k_scalar_rows(const ScArgs a) {
     __shared__ double tile[kLdsSlots * 64];
     double* lt = tile + threadIdx.x;
-    const int row = xcd_block() * blockDim.x + threadIdx.x;
-    if (row >= a.nn) return;
+    const int row = ordered_item(a.slice_order, a.nslice, a.nn);
+    if (row < 0) return;
     const SellRow sr = sell_row<1>(a.slice_ptr, a.slice_w, row);
     const int dof = a.diag_off[row];
     const double xr[3] = {a.xyz[3 * size_t(row)], a.xyz[3 * size_t(row) + 1],
@@ -1330,6 +1337,8 @@ void launch_assemble_scalar(Flow& f, int which) {
     DevMesh& m = *f.m;
     ScArgs a;
     a.nn = m.nn;
+    a.nslice = m.nslice;
+    a.slice_order = m.slice_order.p;
     a.xyz = m.xyz.p;
     a.conn = m.conn.p;
     a.slice_ptr = m.slice_ptr.p;
@@ -1416,13 +1425,14 @@ __global__ void __launch_bounds__(256) k_mdot_init(int ne, const double* xyz, co
 
 // updateMassFlowRateInterior_ (flowModel.cpp:5272-5688): Rhie-Chow with relaxation
 template <int US, int SP>
-__global__ void __launch_bounds__(256) k_mdot_update(int ne, const double* xyz, const int* conn,
+__global__ void __launch_bounds__(256) k_mdot_update(int ne, int negroup, const int* egroup_order,
+                                                     const double* xyz, const int* conn,
                                                      const double* scvw, const double* U,
                                                      const double* du, const double* p,
                                                      const double* gradP, double rho, double urf,
                                                      double* mdot, int* err) {
-    const int e = xcd_block() * blockDim.x + threadIdx.x;
-    if (e >= ne) return;
+    const int e = ordered_item(egroup_order, negroup, ne);
+    if (e < 0) return;
     int nd[8];
     load_conn(conn, e, nd);
     int bad = 0;
@@ -1484,8 +1494,9 @@ void launch_update_mdot(Flow& f) {
     DevMesh& m = *f.m;
     const unsigned B = 256, G = xcd_grid(m.ne, B);
     auto args = [&](auto kern) {
-        kern<<<G, B, 0, f.stream>>>(m.ne, m.xyz.p, m.conn.p, m.scvw.p, f.U.p, f.du.p, f.p.p,
-                                    f.gradP.p, f.prm.rho, f.prm.urf_mdot, f.mdot.p, m.err.p);
+        kern<<<G, B, 0, f.stream>>>(m.ne, m.negroup, m.egroup_order.p, m.xyz.p, m.conn.p, m.scvw.p,
+                                    f.U.p, f.du.p, f.p.p, f.gradP.p, f.prm.rho, f.prm.urf_mdot,
+                                    f.mdot.p, m.err.p);
     };
     const int us = f.prm.u_shifted ? 1 : 0, sp = f.prm.pgrad_shifted ? 1 : 0;
     if (us && sp) args(k_mdot_update<1, 1>);
@@ -1535,7 +1546,8 @@ __device__ __forceinline__ void grad_incidence(const double* xyz, const int* con
 }
 
 template <int N, int SH>
-__global__ void __launch_bounds__(256) k_gradient(int nn, const double* xyz, const int* conn,
+__global__ void __launch_bounds__(256) k_gradient(int nn, int nslice, const int* slice_order,
+                                                  const double* xyz, const int* conn,
                                                   const int* inc_la, const uint64_t* slots_la,
                                                   const int* ext_ptr, const int* ext,
                                                   const uint64_t* ext_slots, const int* binc_ptr,
@@ -1543,8 +1555,8 @@ __global__ void __launch_bounds__(256) k_gradient(int nn, const double* xyz, con
                                                   const int* bf_ord, const double* vol,
                                                   const double* phi, double gurf,
                                                   double* grad) {
-    const int a = xcd_block() * blockDim.x + threadIdx.x;
-    if (a >= nn) return;
+    const int a = ordered_item(slice_order, nslice, nn);
+    if (a < 0) return;
     double G[3 * N];
     for (int k = 0; k < 3 * N; ++k) G[k] = grad[3 * N * size_t(a) + k] * (1.0 - gurf);
     double pa[N];
@@ -1682,7 +1694,8 @@ void launch_gradient(Flow& f, int which) {
     DevMesh& m = *f.m;
     const unsigned B = 256, G = xcd_grid(m.nn, B);
     auto go = [&](auto kern, const double* phi, double* grad) {
-        kern<<<G, B, 0, f.stream>>>(m.nn, m.xyz.p, m.conn.p, m.inc_la.p, m.slots_la.p, m.ext_ptr.p,
+        kern<<<G, B, 0, f.stream>>>(m.nn, m.nslice, m.slice_order.p, m.xyz.p, m.conn.p, m.inc_la.p,
+                                    m.slots_la.p, m.ext_ptr.p,
                                     m.ext.p, m.ext_slots.p, m.binc_ptr.p, m.binc.p, m.bf_elem.p,
                                     m.bf_ord.p, m.vol.p, phi, f.prm.grad_urf, grad);
     };
