@@ -289,6 +289,118 @@ __global__ void __launch_bounds__(kDenseMax) k_amg_dense_apply(int n, const V* m
     x[i] = acc;
 }
 
+// ----------------------------------------------------- coarse-level tail
+// The levels from `t` down, once they are small (at most kTailRows rows),
+// run inside one workgroup: restriction from level t-1, pre-smoothing and
+// residual, restriction, the coarsest solve, then prolongation and
+// post-smoothing back up to level t, phases separated by workgroup
+// barriers.  Each row is computed exactly as by the one-level kernels above
+// (same loops, same order), so the result is bitwise that of the launch
+// sequence it replaces — which on these levels was bound by launch latency.
+constexpr int kTailRows = 2048;
+constexpr int kTailThreads = 1024;
+
+template <typename V>
+struct TailLevel {
+    SellView s;
+    const V* A;
+    const V* dinv;
+    V *R, *X, *S, *T;
+    const int* agg;      // level l-1 row -> level l row
+    const int* mem_ptr;  // members (level l-1 rows) of each level l row
+    const int* mem;
+};
+
+template <typename V>
+struct TailArgs {
+    int t, last, dense_n;
+    V omega, alpha;
+    const V* res_in;  // residual of level t-1
+    const V* inv;     // dense inverse of the coarsest level (transposed) or null
+    TailLevel<V> lv[kMaxLevels];
+};
+
+template <typename V>
+__device__ __forceinline__ void tail_restrict(const TailLevel<V>& c, const V* res) {
+    for (int I = threadIdx.x; I < c.s.n; I += blockDim.x) {
+        V s = V(0);
+#pragma unroll 4
+        for (int k = c.mem_ptr[I]; k < c.mem_ptr[I + 1]; ++k) s += res[c.mem[k]];
+        c.R[I] = s;
+    }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kTailThreads) k_amg_tail(const TailArgs<V> a, const int* flag) {
+    if (flag[0]) return;
+    const V om = a.omega, al = a.alpha;
+    tail_restrict(a.lv[a.t], a.res_in);
+    __syncthreads();
+    for (int l = a.t; l < a.last; ++l) {
+        const TailLevel<V>& L = a.lv[l];
+        for (int i = threadIdx.x; i < L.s.n; i += blockDim.x) {  // as k_amg_pre_residual
+            int w;
+            const size_t b = sv_base(L.s, i, w);
+            V acc = V(0);
+#pragma unroll 4
+            for (int q = 0; q < w; ++q) {
+                const int j = L.s.cols[b + size_t(q) * 64];
+                acc += L.A[b + size_t(q) * 64] * (L.dinv[j] * L.R[j]);
+            }
+            const V ri = L.R[i];
+            L.X[i] = om * L.dinv[i] * ri;
+            L.S[i] = ri - om * acc;
+        }
+        __syncthreads();
+        tail_restrict(a.lv[l + 1], L.S);
+        __syncthreads();
+    }
+    const TailLevel<V>& C = a.lv[a.last];
+    if (a.dense_n) {  // as k_amg_dense_apply
+        for (int i = threadIdx.x; i < C.s.n; i += blockDim.x) {
+            V acc = V(0);
+#pragma unroll 8
+            for (int j = 0; j < C.s.n; ++j) acc += a.inv[size_t(j) * C.s.n + i] * C.R[j];
+            C.X[i] = acc;
+        }
+        __syncthreads();
+    } else {  // as k_amg_jacobi0 and 4 x 2 k_amg_smooth
+        for (int i = threadIdx.x; i < C.s.n; i += blockDim.x) C.X[i] = om * C.dinv[i] * C.R[i];
+        __syncthreads();
+        for (int it = 0; it < 8; ++it) {
+            const V* tin = (it & 1) ? C.T : C.X;
+            V* tout = (it & 1) ? C.X : C.T;
+            for (int i = threadIdx.x; i < C.s.n; i += blockDim.x) {
+                int w;
+                const size_t b = sv_base(C.s, i, w);
+                V acc = V(0);
+                for (int q = 0; q < w; ++q) acc += C.A[b + size_t(q) * 64] * tin[C.s.cols[b + size_t(q) * 64]];
+                tout[i] = tin[i] + om * C.dinv[i] * (C.R[i] - acc);
+            }
+            __syncthreads();
+        }
+    }
+    for (int l = a.last - 1; l >= a.t; --l) {  // as k_amg_prolong_smooth<V, false>
+        const TailLevel<V>& L = a.lv[l];
+        const V* ec = l + 1 == a.last ? C.X : a.lv[l + 1].T;
+        const int* agg = a.lv[l + 1].agg;
+        for (int i = threadIdx.x; i < L.s.n; i += blockDim.x) {
+            int w;
+            const size_t b = sv_base(L.s, i, w);
+            V acc = V(0);
+#pragma unroll 4
+            for (int q = 0; q < w; ++q) {
+                const int j = L.s.cols[b + size_t(q) * 64];
+                acc += L.A[b + size_t(q) * 64] * (L.X[j] + al * ec[agg[j]]);
+            }
+            const V ri = L.R[i];
+            const V ti = L.X[i] + al * ec[agg[i]];
+            L.T[i] = ti + om * L.dinv[i] * (ri - acc);
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------ host side
 static SellView view_of(const DevMesh& m) {
     return {m.nn, m.slice_ptr.p, m.slice_w.p, m.sell_cols.p, m.diag_off.p};
@@ -506,8 +618,15 @@ void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* 
     auto D = [&](int l) -> const V* { return l == 0 ? dinv0 : lvv<V>(amg.lv[l], 0); };
     auto view = [&](int l) { return l == 0 ? v0 : view_of(amg.lv[l]); };
     const int last = nl - 1;
+    // first level handled by the one-workgroup tail (all deeper ones are smaller)
+    int t = last + 1;
+    for (int l = 1; l <= last; ++l)
+        if (amg.lv[l].n <= kTailRows) {
+            t = l;
+            break;
+        }
     // downward: pre-smooth + residual, restrict
-    for (int l = 0; l < last; ++l) {
+    for (int l = 0; l < last && l < t; ++l) {
         AmgLevel& fl = amg.lv[l];
         AmgLevel& cl = amg.lv[l + 1];
         V* S = lvv<V>(fl, 4);
@@ -520,27 +639,47 @@ void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* 
             k_amg_pre_residual<V, false><<<blocks(fl.n), 256, 0, f.stream>>>(view(l), A(l), D(l), om,
                                                                              R(l), X(l), S, flag);
         }
+        if (l + 1 == t) break;  // the tail restricts into level t
         k_amg_restrict<V><<<blocks(cl.n), 256, 0, f.stream>>>(cl.n, cl.mem_ptr.p, cl.mem.p, S, om,
                                                               nullptr, lvv<V>(cl, 3), nullptr, flag);
     }
     AmgLevel& c = amg.lv[last];
-    if (amg.dense_n) {
-        const V* inv;
-        if constexpr (std::is_same_v<V, float>) inv = amg.fdense.p;
-        else inv = amg.dense.p;
-        k_amg_dense_apply<V><<<1, kDenseMax, 0, f.stream>>>(c.n, inv, R(last), X(last), flag);
-    } else {  // coarsening stalled above the dense size: Jacobi steps
-        k_amg_jacobi0<V><<<blocks(c.n), 256, 0, f.stream>>>(c.n, om, D(last), R(last), X(last), flag);
-        for (int it = 0; it < 4; ++it) {
-            k_amg_smooth<V><<<blocks(c.n), 256, 0, f.stream>>>(view(last), A(last), D(last), om,
-                                                               R(last), X(last), lvv<V>(c, 2), flag);
-            k_amg_smooth<V><<<blocks(c.n), 256, 0, f.stream>>>(view(last), A(last), D(last), om,
-                                                               R(last), lvv<V>(c, 2), X(last), flag);
+    const V* dinv_inv;
+    if constexpr (std::is_same_v<V, float>) dinv_inv = amg.fdense.p;
+    else dinv_inv = amg.dense.p;
+    const V* ec = X(last);
+    if (t <= last) {
+        TailArgs<V> ta{};
+        ta.t = t;
+        ta.last = last;
+        ta.dense_n = amg.dense_n;
+        ta.omega = om;
+        ta.alpha = al;
+        ta.res_in = lvv<V>(amg.lv[t - 1], 4);
+        ta.inv = amg.dense_n ? dinv_inv : nullptr;
+        for (int l = t; l <= last; ++l) {
+            AmgLevel& L = amg.lv[l];
+            ta.lv[l] = {view(l), A(l), D(l), lvv<V>(L, 3), lvv<V>(L, 1), lvv<V>(L, 4), lvv<V>(L, 2),
+                        L.agg.p, L.mem_ptr.p, L.mem.p};
         }
+        k_amg_tail<V><<<1, kTailThreads, 0, f.stream>>>(ta, flag);
+        ec = t == last ? X(last) : lvv<V>(amg.lv[t], 2);
+    } else {
+        if (amg.dense_n) {
+            k_amg_dense_apply<V><<<1, kDenseMax, 0, f.stream>>>(c.n, dinv_inv, R(last), X(last), flag);
+        } else {  // coarsening stalled above the dense size: Jacobi steps
+            k_amg_jacobi0<V><<<blocks(c.n), 256, 0, f.stream>>>(c.n, om, D(last), R(last), X(last), flag);
+            for (int it = 0; it < 4; ++it) {
+                k_amg_smooth<V><<<blocks(c.n), 256, 0, f.stream>>>(view(last), A(last), D(last), om,
+                                                                   R(last), X(last), lvv<V>(c, 2), flag);
+                k_amg_smooth<V><<<blocks(c.n), 256, 0, f.stream>>>(view(last), A(last), D(last), om,
+                                                                   R(last), lvv<V>(c, 2), X(last), flag);
+            }
+        }
+        t = last;
     }
     // upward: prolong + post-smooth, out of place (coarse levels into T_l)
-    const V* ec = X(last);
-    for (int l = last - 1; l >= 1; --l) {
+    for (int l = t - 1; l >= 1; --l) {
         AmgLevel& fl = amg.lv[l];
         V* out = lvv<V>(fl, 2);
         k_amg_prolong_smooth<V, false><<<blocks(fl.n), 256, 0, f.stream>>>(
