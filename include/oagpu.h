@@ -145,6 +145,11 @@ int oagpu_element_geometry(int topology, int shifted, int n_elems, const double 
                            double *areav, double *scv, double *dndx, double *detj);
 
 /* --------------------------------------------------------------- flow */
+/* boundary physical types (the zone boundaries' types of the reference,
+ * boundaryPhysicalType): no-slip wall, symmetry plane */
+#define OAGPU_BC_WALL 0
+#define OAGPU_BC_SYMMETRY 3
+
 /* bc_U[n_bc][3]: wall velocity of each boundary id (no-slip walls). */
 int oagpu_flow_create(oagpu_mesh *m, const oagpu_flow_params *prm, int n_bc,
                       const double *bc_U, oagpu_flow **out);
@@ -179,6 +184,13 @@ int oagpu_gradient(oagpu_flow *f, int which);  /* nodeField.hpp:3253; 0 U, 1 p, 
 int oagpu_correct(oagpu_flow *f, int N, double urf); /* equation.h:204-286  */
 int oagpu_velocity_correction(oagpu_flow *f, int step); /* segregatedFlowEquations.cpp:133-279 */
 int oagpu_update_scales(oagpu_flow *f, double *U_scale, double *p_scale); /* nodeField.hpp:5993 */
+/* Physical type of each boundary id (default: every id a no-slip wall).
+ * Symmetry planes (navierStokesAssemblerElemBoundaryConditions.cpp:2082,
+ * symmetry projection and normal relaxation navierStokesAssembler.cpp:387-696,
+ * gradient corrections nodeVectorField.cpp:46 / nodeScalarField.cpp:45, the
+ * assembled symmetry area meshGeometry.cpp:1787) carry no mass flow and no
+ * scalar flux.  Other types are rejected (OAGPU_EINVAL). */
+int oagpu_flow_set_bc_types(oagpu_flow *f, int n_bc, const int *type);
 /* nodeField::updateBlendingFactorField (nodeField.hpp:2069): 0 U, 2 k, 3 omega */
 int oagpu_update_blending(oagpu_flow *f, int which);
 /* control residual source: RMS over rows of b / a_diag per component */

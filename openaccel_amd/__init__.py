@@ -33,6 +33,7 @@ LIB_PATH = os.path.join(HERE, "liboagpu.so")
 OAGPU_FP64, OAGPU_MIXED = 0, 1
 OAGPU_STENCIL_FULL, OAGPU_STENCIL_REDUCED = 0, 1
 OAGPU_GMRES, OAGPU_CG, OAGPU_BICGSTAB, OAGPU_CG_AMG = 0, 1, 2, 3
+OAGPU_BC_WALL, OAGPU_BC_SYMMETRY = 0, 3
 OAGPU_HEX8, OAGPU_TET4, OAGPU_WED6, OAGPU_PYR5 = 0, 1, 2, 3
 TOPOLOGY_SIZES = {OAGPU_HEX8: (8, 12), OAGPU_TET4: (4, 6), OAGPU_WED6: (6, 9), OAGPU_PYR5: (5, 12)}
 
@@ -109,6 +110,7 @@ _SIGS = {
                     C.POINTER(ControlData)],
     "oagpu_element_geometry": [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp],
     "oagpu_update_blending": [C.c_void_p, C.c_int],
+    "oagpu_flow_set_bc_types": [C.c_void_p, C.c_int, _ip],
     "oagpu_sst_enable": [C.c_void_p, C.POINTER(SstParams)],
     "oagpu_sst_update": [C.c_void_p],
     "oagpu_assemble_scalar": [C.c_void_p, C.c_int],
@@ -247,7 +249,7 @@ class FlowCase:
 
     _comp = {"U": 3, "p": 1, "gradU": 9, "gradP": 3, "du": 3, "div": 1, "mu": 1, "k": 1,
              "omega": 1, "gradK": 3, "gradW": 3, "ymin": 1, "mut": 1, "F1": 1, "Pk": 1,
-             "Pk_wall": 1, "kprev": 1, "beta": 3, "beta_k": 1, "beta_w": 1}
+             "Pk_wall": 1, "kprev": 1, "beta": 3, "beta_k": 1, "beta_w": 1, "symA": 3}
 
     def _size(self, name):
         if name == "mdot":
@@ -300,6 +302,11 @@ class FlowCase:
 
     def gradient(self, which):
         _check(load_library().oagpu_gradient(self.h, {"U": 0, "p": 1, "k": 2, "omega": 3}[which]))
+
+    def set_bc_types(self, types):
+        """Physical type per boundary id: OAGPU_BC_WALL or OAGPU_BC_SYMMETRY."""
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        _check(load_library().oagpu_flow_set_bc_types(self.h, len(t), _i(t)))
 
     def update_blending(self, which):
         _check(load_library().oagpu_update_blending(self.h, {"U": 0, "k": 2, "omega": 3}[which]))

@@ -60,10 +60,11 @@ double* field_ptr(oagpu::Flow& f, const char* name, size_t* count) {
                {"gradW", &f.gradW, nn * 3},  {"ymin", &f.ymin, nn}, {"mut", &f.mut, nn},
                {"F1", &f.F1, nn},            {"Pk", &f.Pk, nn},     {"Pk_wall", &f.Pk_wall, nn},
                {"kprev", &f.kprev, nn},      {"beta", &f.beta, nn * 3},
-               {"beta_k", &f.beta_k, nn},    {"beta_w", &f.beta_w, nn}};
+               {"beta_k", &f.beta_k, nn},    {"beta_w", &f.beta_w, nn},
+               {"symA", &f.symA, nn * 3}};
     for (auto& t : tab)
         if (!std::strcmp(name, t.n)) {
-            OAGPU_REQUIRE(t.b->p, std::string("field ") + name + " not allocated (SST not enabled)");
+            OAGPU_REQUIRE(t.b->p, std::string("field ") + name + " not allocated (SST, high resolution or symmetry off)");
             *count = t.c;
             return t.b->p;
         }
@@ -254,6 +255,10 @@ int oagpu_flow_create(oagpu_mesh* m, const oagpu_flow_params* prm, int n_bc, con
             f.beta.alloc(nn * 3);
             f.beta.zero();
         }
+        f.h_bc_type.assign(size_t(n_bc > 0 ? n_bc : 1), OAGPU_BC_WALL);
+        f.bc_type.upload(f.h_bc_type);
+        f.wnode.alloc(nn);
+        OAGPU_HIP(hipMemcpy(f.wnode.p, m->m.bnode.p, nn * sizeof(int), hipMemcpyDeviceToDevice));
         OAGPU_HIP(hipStreamCreate(&f.stream));
         OAGPU_HIP(hipEventCreate(&f.ev0));
         OAGPU_HIP(hipEventCreate(&f.ev1));
@@ -439,6 +444,26 @@ int oagpu_solve(oagpu_flow* h, int N, int method, int max_it, double rtol, doubl
         OAGPU_REQUIRE(max_it >= 0, "negative iteration limit");
         std::memset(cd, 0, sizeof(*cd));
         oagpu::krylov_solve(h->f, N, method, max_it, rtol, atol, restart, cd);
+    });
+}
+
+int oagpu_flow_set_bc_types(oagpu_flow* h, int n_bc, const int* type) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && type, "null argument");
+        oagpu::Flow& f = h->f;
+        OAGPU_REQUIRE(n_bc == f.nbc, "one type per boundary id");
+        bool sym = false;
+        for (int b = 0; b < n_bc; ++b) {
+            OAGPU_REQUIRE(type[b] == OAGPU_BC_WALL || type[b] == OAGPU_BC_SYMMETRY,
+                          "unsupported boundary type (walls and symmetry planes)");
+            sym = sym || type[b] == OAGPU_BC_SYMMETRY;
+        }
+        f.h_bc_type.assign(type, type + n_bc);
+        f.bc_type.upload(f.h_bc_type);
+        f.has_sym = sym;
+        if (sym && !f.symA.p) f.symA.alloc(size_t(f.m->nn) * 3);
+        oagpu::launch_boundary_types(f);
+        OAGPU_HIP(hipStreamSynchronize(f.stream));
     });
 }
 

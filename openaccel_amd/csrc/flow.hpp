@@ -80,6 +80,13 @@ struct Flow {
     DBuf<double> mueff;
     // blending factors of the limited deferred correction (high resolution)
     DBuf<double> beta, beta_k, beta_w;
+    // boundary physical types per boundary id (OAGPU_BC_*), no-slip wall
+    // nodes, and the assembled symmetry area vector per node (zero off the
+    // symmetry planes; allocated only when a symmetry plane exists)
+    std::vector<int> h_bc_type;
+    DBuf<int> bc_type, wnode;
+    DBuf<double> symA;
+    bool has_sym = false;
     // k-omega SST state (oagpu_sst_enable)
     bool sst_on = false;
     oagpu_sst_params sst{};
@@ -94,6 +101,7 @@ void launch_update_mdot(Flow& f);
 void launch_mass_divergence(Flow& f);
 void launch_gradient(Flow& f, int which);
 void launch_blending(Flow& f, int which);   // 0 U, 2 k, 3 omega
+void launch_boundary_types(Flow& f);        // wall-node flags and symmetry areas
 void launch_assemble_momentum(Flow& f);
 void launch_assemble_pcorr(Flow& f);
 void launch_correct(Flow& f, int N, double urf);

@@ -428,6 +428,8 @@ struct MomArgs {
     const double* mueff;
     const double* beta;    // blending factors [nn][3] (high resolution) or null
     const double* gradU;
+    const int* bc_type;    // physical type per boundary id
+    const double* symA;    // assembled symmetry area [nn][3], null without symmetry
     double rho, mu, dt, urf_inv, bnd_relax;
     int ugrad_shifted, us;
     double* vals;
@@ -678,6 +680,61 @@ __device__ __forceinline__ void mom_wall(const MomArgs& a, int f, int fk, uint64
     for (int i = 0; i < 3; ++i) R[i] += double(rl[i]);
 }
 
+// symmetry plane: the face ip of side (f, fk) nearest to this row
+// (navierStokesAssemblerElemBoundaryConditions.cpp:2082-2375,
+// IP_FULL_STRESS__ :296-343; incompressible, so no divU term)
+template <typename T, typename LT>
+__device__ __forceinline__ void mom_sym(const MomArgs& a, int f, int fk, uint64_t sl,
+                                        const SellRow& sr, int dof, const double xr[3],
+                                        double D[9], double R[3], int& bad, LT* lt) {
+    const int e = a.bf_elem[f], ord = a.bf_ord[f];
+    const int* sn = hex8::c_side_nodes[ord];
+    T s[3];
+    for (int d = 0; d < 3; ++d)
+        s[d] = a.ugrad_shifted ? T(hex8::c_node_loc[sn[fk]][d]) : T(hex8::c_face_loc[ord][fk][d]);
+    T Ji[9], A[3];
+    {
+        int nd[8];
+        load_conn(a.conn, e, nd);
+        T c[8][3];
+        gather_rel(a.xyz, nd, xr, c);
+        jac_inv(s[0], s[1], s[2], c, Ji, &bad);
+        T fp[4][3];
+        for (int q = 0; q < 4; ++q) {
+            const size_t o = 3 * size_t(a.conn[8 * size_t(e) + sn[q]]);
+            for (int d = 0; d < 3; ++d) fp[q][d] = T(a.xyz[o + d] - xr[d]);
+        }
+        hex8::quad_face_areav(fp, fk, A);
+    }
+    T mu = T(a.mu);
+    if (a.mueff) {
+        mu = T(0);
+        for (int q = 0; q < 4; ++q)
+            mu += T(hex8::c_quad_shape[a.us][fk][q]) * T(a.mueff[a.conn[8 * size_t(e) + sn[q]]]);
+    }
+    T rl[3] = {0, 0, 0};
+#pragma unroll 1
+    for (int ic = 0; ic < 8; ++ic) {
+        const size_t o = 3 * size_t(a.conn[8 * size_t(e) + ic]);
+        const T uc[3] = {T(a.U[o]), T(a.U[o + 1]), T(a.U[o + 2])};
+        T g[3];
+        grad_N(ic, s[0], s[1], s[2], Ji, g);
+        T acc[9];
+        for (int k = 0; k < 9; ++k) acc[k] = T(0);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                T lf = -mu * g[j] * A[j];
+                acc[3 * i + i] += lf;
+                rl[i] -= lf * uc[i];
+                lf = -mu * g[i] * A[j];
+                acc[3 * i + j] += lf;
+                rl[i] -= lf * uc[j];
+            }
+        scatter_block(a.vals, sr, dof, sl, ic, acc, D, lt);
+    }
+    for (int i = 0; i < 3; ++i) R[i] += double(rl[i]);
+}
+
 // navierStokesAssembler for one row: node terms (NodeTerms.cpp:16-149),
 // interior element terms, no-slip walls, relaxation (phiAssembler.h:966-1006),
 // boundary rhs relaxation (phiAssembler.h:1008-1152, navierStokesAssembler.cpp:238)
@@ -741,7 +798,10 @@ __global__ void __launch_bounds__(LDS ? 64 : 256) k_momentum_rows(const MomArgs 
 #pragma unroll 1
     for (int k = a.binc_ptr[row]; k < a.binc_ptr[row + 1]; ++k) {
         const int pk = a.binc[k];
-        mom_wall<T>(a, pk >> 2, pk & 3, a.binc_slots[k], sr, dof, xr, D, R, bad, lt);
+        if (a.symA && a.bc_type[a.bf_bc[pk >> 2]] == OAGPU_BC_SYMMETRY)
+            mom_sym<T>(a, pk >> 2, pk & 3, a.binc_slots[k], sr, dof, xr, D, R, bad, lt);
+        else
+            mom_wall<T>(a, pk >> 2, pk & 3, a.binc_slots[k], sr, dof, xr, D, R, bad, lt);
     }
     if constexpr (LDS != 0) {
         const int ns = sr.width < kLdsSlots ? sr.width : kLdsSlots;
@@ -754,8 +814,33 @@ __global__ void __launch_bounds__(LDS ? 64 : 256) k_momentum_rows(const MomArgs 
     for (int k = 0; k < 3; ++k) D[4 * k] *= a.urf_inv;
     if (a.bnode[row])
         for (int k = 0; k < 3; ++k) R[k] *= a.bnd_relax;
+    double nrm[3] = {0, 0, 0};
+    bool sym = false;
+    if (a.symA) {  // applySymmetryConditions_ (navierStokesAssembler.cpp:387-471)
+        const double* sa = a.symA + 3 * size_t(row);
+        const double amag = sqrt(sa[0] * sa[0] + sa[1] * sa[1] + sa[2] * sa[2]);
+        sym = amag > 0.0;
+        if (sym) {
+            double dot = 0.0;
+            for (int j = 0; j < 3; ++j) {
+                nrm[j] = sa[j] / amag;
+                dot += R[j] * nrm[j];
+            }
+            for (int j = 0; j < 3; ++j) R[j] -= dot * nrm[j];
+        }
+    }
     const double vol = a.vol[row];
     for (int k = 0; k < 3; ++k) a.du[3 * size_t(row) + k] = vol / (D[4 * k] + kSmall);
+    if (sym) {  // assembleNormalRelaxation (:473-696) with urf 0.75, after du
+        double mod[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double v = 0.0;
+                for (int k = 0; k < 3; ++k) v += D[3 * i + k] * nrm[k] * nrm[j];
+                mod[3 * i + j] = v;
+            }
+        for (int k = 0; k < 9; ++k) D[k] += (1.0 - 0.75) * mod[k];
+    }
     for (int kk = 0; kk < 9; ++kk) *sell_val<9>(a.vals, sr, dof, kk) = D[kk];
     for (int k = 0; k < 3; ++k) a.b[3 * size_t(row) + k] = R[k];
     if (bad) *a.err = 1;
@@ -800,6 +885,8 @@ void launch_assemble_momentum(Flow& f) {
     a.mueff = f.sst_on ? f.mueff.p : nullptr;  // laminar: the constant mu
     a.beta = f.prm.high_res ? f.beta.p : nullptr;
     a.gradU = f.gradU.p;
+    a.bc_type = f.bc_type.p;
+    a.symA = f.has_sym ? f.symA.p : nullptr;
     a.vals = f.A3.p;
     a.b = f.b3.p;
     a.du = f.du.p;
@@ -1076,6 +1163,7 @@ constexpr double kA1 = 0.31, kBetaStar = 0.09, kProdLim = 10.0, kCmu = 0.09;
 struct ScArgs {
     int nn, nslice;
     const int* slice_order;
+    const int* wnode;
     const double* xyz;
     const int* conn;
     const int* slice_ptr;
@@ -1319,9 +1407,8 @@ __global__ void __launch_bounds__(64) k_scalar_rows(const ScArgs a) {
         sc_incidence<T, SG, HR>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, D, R,
                                 bad, lt);
     D *= a.urf_inv;
-    const bool wall = a.bnode[row] != 0;
-    if (wall) R *= 0.75;
-    if (a.which == 1 && wall) {
+    if (a.bnode[row]) R *= 0.75;
+    if (a.which == 1 && a.wnode[row]) {  // omega rows fixed at no-slip wall nodes
         scalar_row_zero(a.vals, sr, lt, dof);
         R = 0.0;
     }
@@ -1339,6 +1426,7 @@ void launch_assemble_scalar(Flow& f, int which) {
     a.nn = m.nn;
     a.nslice = m.nslice;
     a.slice_order = m.slice_order.p;
+    a.wnode = f.wnode.p;
     a.xyz = m.xyz.p;
     a.conn = m.conn.p;
     a.slice_ptr = m.slice_ptr.p;
@@ -1552,9 +1640,10 @@ __global__ void __launch_bounds__(256) k_gradient(int nn, int nslice, const int*
                                                   const int* ext_ptr, const int* ext,
                                                   const uint64_t* ext_slots, const int* binc_ptr,
                                                   const int* binc, const int* bf_elem,
-                                                  const int* bf_ord, const double* vol,
-                                                  const double* phi, double gurf,
-                                                  double* grad) {
+                                                  const int* bf_ord, const int* bf_bc,
+                                                  const int* bc_type, const double* symA,
+                                                  const double* vol, const double* phi,
+                                                  double gurf, double* grad) {
     const int a = ordered_item(slice_order, nslice, nn);
     if (a < 0) return;
     double G[3 * N];
@@ -1574,6 +1663,7 @@ __global__ void __launch_bounds__(256) k_gradient(int nn, int nslice, const int*
 #pragma unroll 1
     for (int k = binc_ptr[a]; k < binc_ptr[a + 1]; ++k) {
         const int f = binc[k] >> 2, fk = binc[k] & 3;
+        if (bc_type && bc_type[bf_bc[f]] == OAGPU_BC_SYMMETRY) continue;  // no face term
         const int e = bf_elem[f], ord = bf_ord[f];
         double fp[4][3], pf[4][N];
         for (int q = 0; q < 4; ++q) {
@@ -1587,6 +1677,31 @@ __global__ void __launch_bounds__(256) k_gradient(int nn, int nslice, const int*
             double pip = 0.0;
             for (int q = 0; q < 4; ++q) pip += hex8::c_quad_shape[SH][fk][q] * pf[q][i];
             for (int j = 0; j < 3; ++j) G[3 * i + j] += gurf * (pip - pa[i]) * A[j] * iv;
+        }
+    }
+    if (symA) {  // symmetry planes: nodeVectorField.cpp:46-140, nodeScalarField.cpp:45-110
+        const double* sa = symA + 3 * size_t(a);
+        const double amag = sqrt(sa[0] * sa[0] + sa[1] * sa[1] + sa[2] * sa[2]);
+        if (amag > 0.0) {
+            const double n[3] = {sa[0] / amag, sa[1] / amag, sa[2] / amag};
+            if constexpr (N == 1) {
+                const double dot = G[0] * n[0] + G[1] * n[1] + G[2] * n[2];
+                for (int j = 0; j < 3; ++j) G[j] -= dot * n[j];
+            } else {
+                double ng[3] = {0, 0, 0}, gn[3] = {0, 0, 0}, ngn = 0.0;
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) {
+                        ng[i] += G[3 * i + j] * n[j];
+                        gn[j] += G[3 * i + j] * n[i];
+                    }
+                for (int j = 0; j < 3; ++j) ngn += gn[j] * n[j];
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) {
+                        G[3 * i + j] -= ng[i] * n[j];
+                        G[3 * i + j] -= n[i] * gn[j];
+                        G[3 * i + j] += 2.0 * n[i] * n[j] * ngn;
+                    }
+            }
         }
     }
     for (int k = 0; k < 3 * N; ++k) grad[3 * N * size_t(a) + k] = G[k];
@@ -1690,14 +1805,59 @@ void launch_blending(Flow& f, int which) {
     OAGPU_HIP(hipGetLastError());
 }
 
+// no-slip wall nodes and the assembled symmetry area of every node
+// (mesh::updateAssembledSymmetryAreaField_, meshGeometry.cpp:1787-1850: each
+// symmetry face ip's exposed area added to its nearest node), summed in the
+// node's face order
+__global__ void __launch_bounds__(256) k_boundary_types(int nn, const double* xyz, const int* conn,
+                                                        const int* binc_ptr, const int* binc,
+                                                        const int* bf_elem, const int* bf_ord,
+                                                        const int* bf_bc, const int* bc_type,
+                                                        int* wnode, double* symA) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nn) return;
+    int w = 0;
+    double s[3] = {0, 0, 0};
+    const double xr[3] = {xyz[3 * size_t(a)], xyz[3 * size_t(a) + 1], xyz[3 * size_t(a) + 2]};
+    for (int k = binc_ptr[a]; k < binc_ptr[a + 1]; ++k) {
+        const int f = binc[k] >> 2, fk = binc[k] & 3;
+        const int t = bc_type[bf_bc[f]];
+        if (t == OAGPU_BC_WALL) w = 1;
+        if (t != OAGPU_BC_SYMMETRY || !symA) continue;
+        double fp[4][3], A[3];
+        for (int q = 0; q < 4; ++q) {
+            const size_t o = 3 * size_t(conn[8 * size_t(bf_elem[f]) + hex8::c_side_nodes[bf_ord[f]][q]]);
+            for (int d = 0; d < 3; ++d) fp[q][d] = xyz[o + d] - xr[d];
+        }
+        hex8::quad_face_areav(fp, fk, A);
+        for (int d = 0; d < 3; ++d) s[d] += A[d];
+    }
+    wnode[a] = w;
+    if (symA)
+        for (int d = 0; d < 3; ++d) symA[3 * size_t(a) + d] = s[d];
+}
+
+void launch_boundary_types(Flow& f) {
+    DevMesh& m = *f.m;
+    const unsigned B = 256;
+    k_boundary_types<<<(m.nn + B - 1) / B, B, 0, f.stream>>>(
+        m.nn, m.xyz.p, m.conn.p, m.binc_ptr.p, m.binc.p, m.bf_elem.p, m.bf_ord.p, m.bf_bc.p,
+        f.bc_type.p, f.wnode.p, f.has_sym ? f.symA.p : nullptr);
+    OAGPU_HIP(hipGetLastError());
+}
+
 void launch_gradient(Flow& f, int which) {
     DevMesh& m = *f.m;
     const unsigned B = 256, G = xcd_grid(m.nn, B);
+    // the symmetry correction is applied to velocity and pressure
+    // (velocity.cpp:52, pressure.cpp:48), not to k / omega (default off)
+    const double* sym = f.has_sym && which <= 1 ? f.symA.p : nullptr;
+    const int* types = f.has_sym ? f.bc_type.p : nullptr;  // symmetry faces add no term
     auto go = [&](auto kern, const double* phi, double* grad) {
         kern<<<G, B, 0, f.stream>>>(m.nn, m.nslice, m.slice_order.p, m.xyz.p, m.conn.p, m.inc_la.p,
-                                    m.slots_la.p, m.ext_ptr.p,
-                                    m.ext.p, m.ext_slots.p, m.binc_ptr.p, m.binc.p, m.bf_elem.p,
-                                    m.bf_ord.p, m.vol.p, phi, f.prm.grad_urf, grad);
+                                    m.slots_la.p, m.ext_ptr.p, m.ext.p, m.ext_slots.p, m.binc_ptr.p,
+                                    m.binc.p, m.bf_elem.p, m.bf_ord.p, m.bf_bc.p, types, sym,
+                                    m.vol.p, phi, f.prm.grad_urf, grad);
     };
     if (which == 0) {
         if (f.prm.u_shifted) go(k_gradient<3, 1>, f.U.p, f.gradU.p);

@@ -48,7 +48,7 @@ struct SstNodeArgs {
     int nn;
     double rho, mu, urf_mut;
     const double *k, *w, *ymin, *gradU, *gradK, *gradW, *Pk_wall;
-    const int* bnode;
+    const int* wnode;  // no-slip wall nodes
     double *mut, *mueff, *F1, *Pk;
 };
 
@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256) k_sst_update(const SstNodeArgs a) {
                             4.0 * rho * kSigW2 * k / (CDkw + kSmall) / (minDSq + kSmall));
     a.F1[i] = tanh(arg * arg * arg * arg);
     // production
-    a.Pk[i] = a.bnode[i] ? a.Pk_wall[i] : fmax(mut * pk_by_mut(g), 0.0);
+    a.Pk[i] = a.wnode[i] ? a.Pk_wall[i] : fmax(mut * pk_by_mut(g), 0.0);
 }
 
 void launch_sst_update(Flow& f) {
@@ -99,7 +99,7 @@ void launch_sst_update(Flow& f) {
     a.gradK = f.gradK.p;
     a.gradW = f.gradW.p;
     a.Pk_wall = f.Pk_wall.p;
-    a.bnode = m.bnode.p;
+    a.wnode = f.wnode.p;
     a.mut = f.mut.p;
     a.mueff = f.mueff.p;
     a.F1 = f.F1.p;

@@ -88,6 +88,9 @@ or_case *or_case_create(int nn, int ne, const double *xyz, const int *conn, int 
     c->beta = (double *)xcalloc((size_t)nn * 3, sizeof(double));
     c->bareav = (double *)xcalloc((size_t)nbf * 12, sizeof(double));
     c->bnode = (int *)xcalloc((size_t)nn, sizeof(int));
+    c->wnode = (int *)xcalloc((size_t)nn, sizeof(int));
+    c->symA = (double *)xcalloc((size_t)nn * 3, sizeof(double));
+    c->bc_type = (int *)xcalloc((size_t)nbc, sizeof(int));
     for (int i = 0; i < nn; ++i) {
         c->rho[i] = prm->rho;
         c->mu[i] = prm->mu;
@@ -104,6 +107,7 @@ or_case *or_case_create(int nn, int ne, const double *xyz, const int *conn, int 
         }
         or_quad_face_areav(fc, &c->bareav[12 * f]);
     }
+    memcpy(c->wnode, c->bnode, sizeof(int) * (size_t)nn);
     or_dual_volume(c);
     c->U_scale = 1.0;
     c->p_scale = 1.0;
@@ -117,10 +121,48 @@ void or_case_destroy(or_case *c)
                     c->row_ptr, c->cols, c->diag_off, c->A3, c->b3, c->x3,
                     c->A1, c->b1, c->x1, c->U, c->p, c->gradU, c->gradP, c->du,
                     c->mdot, c->div, c->vol, c->rho, c->mu, c->beta, c->bareav,
-                    c->bnode, c->k, c->w, c->gradK, c->gradW, c->ymin, c->mut, c->F1,
+                    c->bnode, c->wnode, c->symA, c->bc_type,
+                    c->k, c->w, c->gradK, c->gradW, c->ymin, c->mut, c->F1,
                     c->Pk, c->Pk_wall, c->kprev, c->beta_k, c->beta_w};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
     free(c);
+}
+
+/* boundary physical types (mesh.cpp zone boundaries): no-slip wall nodes
+ * and the assembled symmetry area (mesh::updateAssembledSymmetryAreaField_,
+ * meshGeometry.cpp:1787-1850: each symmetry face ip's exposed area added to
+ * its nearest node) */
+int or_case_set_bc_types(or_case *c, const int *type)
+{
+    for (int b = 0; b < c->nbc; ++b)
+        if (type[b] != OR_BC_WALL && type[b] != OR_BC_SYMMETRY) return 1;
+    memcpy(c->bc_type, type, sizeof(int) * (size_t)c->nbc);
+    memset(c->wnode, 0, sizeof(int) * (size_t)c->nn);
+    memset(c->symA, 0, sizeof(double) * 3 * (size_t)c->nn);
+    c->has_sym = 0;
+    const int *sn = or_hex_side_nodes();
+    for (int f = 0; f < c->nbf; ++f) {
+        const int *en = &c->conn[8 * c->bf_elem[f]];
+        const int t = c->bc_type[c->bf_bc[f]];
+        for (int k = 0; k < 4; ++k) {
+            const int nd = en[sn[4 * c->bf_ord[f] + k]];
+            if (t == OR_BC_WALL) c->wnode[nd] = 1;
+            if (t == OR_BC_SYMMETRY) {
+                for (int d = 0; d < 3; ++d) c->symA[3 * nd + d] += c->bareav[12 * f + 3 * k + d];
+                c->has_sym = 1;
+            }
+        }
+    }
+    return 0;
+}
+
+static int sym_normal(const or_case *c, int i, double n[3])
+{
+    const double *a = &c->symA[3 * i];
+    const double amag = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    if (!(amag > 0)) return 0;
+    for (int d = 0; d < 3; ++d) n[d] = a[d] / amag;
+    return 1;
 }
 
 int or_case_nn(const or_case *c) { return c->nn; }
@@ -154,6 +196,7 @@ double *or_case_field(or_case *c, const char *name)
     if (!strcmp(name, "Pk")) return c->Pk;
     if (!strcmp(name, "Pk_wall")) return c->Pk_wall;
     if (!strcmp(name, "kprev")) return c->kprev;
+    if (!strcmp(name, "symA")) return c->symA;
     if (!strcmp(name, "beta_k")) return c->beta_k;
     if (!strcmp(name, "beta_w")) return c->beta_w;
     return NULL;
@@ -272,10 +315,47 @@ void or_mass_divergence(or_case *c)
 }
 
 /* ------------------------------------------------------------- gradients */
+/* symmetry-plane gradient corrections (velocity and pressure set
+ * correctGradient_, velocity.cpp:52, pressure.cpp:48):
+ * vector, nodeVectorField.cpp:46-140: G -= (G n) n^T + n (n^T G) - 2 n n^T (n^T G n);
+ * scalar, nodeScalarField.cpp:45-110: g -= (g . n) n */
+static void symmetry_correct(or_case *c, double *grad, int N)
+{
+    for (int i = 0; i < c->nn; ++i) {
+        double n[3];
+        if (!sym_normal(c, i, n)) continue;
+        double *g = &grad[3 * N * i];
+        if (N == 1) {
+            double dot = 0;
+            for (int j = 0; j < 3; ++j) dot += g[j] * n[j];
+            for (int j = 0; j < 3; ++j) g[j] -= dot * n[j];
+            continue;
+        }
+        double ng[3] = {0, 0, 0}, gn[3] = {0, 0, 0}, ngn = 0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                ng[a] += g[3 * a + b] * n[b];
+                gn[b] += g[3 * a + b] * n[a];
+            }
+        for (int b = 0; b < 3; ++b) ngn += gn[b] * n[b];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                g[3 * a + b] -= ng[a] * n[b];
+                g[3 * a + b] -= n[a] * gn[b];
+                g[3 * a + b] += 2.0 * n[a] * n[b] * ngn;
+            }
+    }
+}
+
 void or_gradient(or_case *c, int which)
 {
-    if (which == 0) or_gradient_field(c, c->U, c->gradU, 3, c->prm.u_shifted);
-    else if (which == 1) or_gradient_field(c, c->p, c->gradP, 1, c->prm.p_shifted);
+    if (which == 0) {
+        or_gradient_field(c, c->U, c->gradU, 3, c->prm.u_shifted);
+        if (c->has_sym) symmetry_correct(c, c->gradU, 3);
+    } else if (which == 1) {
+        or_gradient_field(c, c->p, c->gradP, 1, c->prm.p_shifted);
+        if (c->has_sym) symmetry_correct(c, c->gradP, 1);
+    }
     else if (which == 2 && c->k) or_gradient_field(c, c->k, c->gradK, 1, c->sst.k_shifted);
     else if (which == 3 && c->w) or_gradient_field(c, c->w, c->gradW, 1, c->sst.w_shifted);
 }
@@ -312,6 +392,7 @@ void or_gradient_field(or_case *c, const double *phi, double *grad, int N, int s
     double fsf[16];
     or_quad_face_shape(shifted, fsf);
     for (int f = 0; f < c->nbf; ++f) {
+        if (c->bc_type[c->bf_bc[f]] == OR_BC_SYMMETRY) continue;
         const int *en = &c->conn[8 * c->bf_elem[f]];
         int fn[4];
         for (int k = 0; k < 4; ++k) fn[k] = en[sn[4 * c->bf_ord[f] + k]];
@@ -428,12 +509,49 @@ void or_assemble_momentum(or_case *c)
         or_apply_coeff(3, 8, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
     }
 
-    /* no-slip walls, laminar (navierStokesAssemblerElemBoundaryConditions.cpp:2659-2884) */
+    /* no-slip walls (navierStokesAssemblerElemBoundaryConditions.cpp:2659-2884)
+     * and symmetry planes (:2082-2375, IP_FULL_STRESS__ :296-343 with the
+     * incompressible divU term zero) */
     const int *sn = or_hex_side_nodes();
     double fsf[16];
     or_quad_face_shape(P->u_shifted, fsf);
     for (int f = 0; f < c->nbf; ++f) {
         const int e = c->bf_elem[f], ord = c->bf_ord[f];
+        if (c->bc_type[c->bf_bc[f]] == OR_BC_SYMMETRY) {
+            const int *en = &c->conn[8 * e];
+            memset(lhs, 0, sizeof(lhs));
+            memset(rhs, 0, sizeof(rhs));
+            double muf[4];
+            for (int a = 0; a < 8; ++a) nodes[a] = en[a];
+            for (int k = 0; k < 4; ++k) muf[k] = c->mu[en[sn[4 * ord + k]]];
+            gather_coords(c, e, co);
+            double fdn[96], fdet[4];
+            or_hex_face_gradop(P->ugrad_shifted, ord, co, fdn, fdet);
+            const double *aw = &c->bareav[12 * f];
+            for (int ip = 0; ip < 4; ++ip) {
+                const int nearest = sn[4 * ord + ip];
+                const double *ax = &aw[3 * ip];
+                double mub = 0;
+                for (int ic = 0; ic < 4; ++ic) mub += fsf[4 * ip + ic] * muf[ic];
+                for (int ic = 0; ic < 8; ++ic) {
+                    const double *dn = &fdn[24 * ip + 3 * ic];
+                    const double *u = &c->U[3 * en[ic]];
+                    for (int i = 0; i < 3; ++i) {
+                        const int iR = 3 * nearest + i, rowR = iR * 24;
+                        for (int j = 0; j < 3; ++j) {
+                            double lf = -mub * dn[j] * ax[j];
+                            lhs[rowR + 3 * ic + i] += lf;
+                            rhs[iR] -= lf * u[i];
+                            lf = -mub * dn[i] * ax[j];
+                            lhs[rowR + 3 * ic + j] += lf;
+                            rhs[iR] -= lf * u[j];
+                        }
+                    }
+                }
+            }
+            or_apply_coeff(3, 8, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
+            continue;
+        }
         const int *en = &c->conn[8 * e];
         memset(lhs, 0, sizeof(lhs));
         memset(rhs, 0, sizeof(rhs));
@@ -505,6 +623,29 @@ void or_assemble_momentum(or_case *c)
     for (int i = 0; i < nn; ++i)
         if (c->bnode[i])
             for (int k = 0; k < 3; ++k) c->b3[3 * i + k] *= P->bnd_relax;
+    /* applySymmetryConditions_ (navierStokesAssembler.cpp:387-471): the rhs
+     * loses its component along the assembled symmetry normal */
+    for (int i = 0; i < nn; ++i) {
+        double n[3];
+        if (!sym_normal(c, i, n)) continue;
+        double *b = &c->b3[3 * i], dot = 0;
+        for (int j = 0; j < 3; ++j) dot += b[j] * n[j];
+        for (int j = 0; j < 3; ++j) b[j] -= dot * n[j];
+    }
+}
+
+void or_normal_relaxation(or_case *c)
+{
+    const double urf = 0.75;
+    for (int i = 0; i < c->nn; ++i) {
+        double n[3];
+        if (!sym_normal(c, i, n)) continue;
+        double *d = &c->A3[9 * ((size_t)c->row_ptr[i] + c->diag_off[i])], mod[9] = {0};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int k = 0; k < 3; ++k) mod[3 * a + b] += d[3 * a + k] * n[k] * n[b];
+        for (int k = 0; k < 9; ++k) d[k] += (1.0 - urf) * mod[k];
+    }
 }
 
 void or_compute_du(or_case *c)
@@ -782,6 +923,7 @@ void or_outer_iteration(or_case *c, const double *ctl, double *hist)
     /* momentum predictor (navierStokesEquation::solve :157-222) */
     or_assemble_momentum(c);
     or_compute_du(c);
+    or_normal_relaxation(c);
     or_scaled_residual(c, 3, r3);
     memset(c->x3, 0, sizeof(double) * 3 * (size_t)c->nn);
     const int itU = or_solve(c, 3, 0, (int)ctl[0], ctl[1], 1e-30, 30, res);

@@ -24,7 +24,11 @@ struct or_case {
     /* fields */
     double *U, *p, *gradU, *gradP, *du, *mdot, *div, *vol, *rho, *mu, *beta;
     double *bareav; /* exposed area vectors [nbf][4][3] */
-    int *bnode;     /* node is on a wall boundary */
+    int *bnode;     /* node is on a boundary face (any type) */
+    int *bc_type;   /* physical type of each boundary id (OR_BC_*) */
+    int *wnode;     /* node is on a no-slip wall face */
+    double *symA;   /* assembled symmetry area vector [nn][3] (0 off symmetry) */
+    int has_sym;
     double U_scale, p_scale;
     /* k-omega SST (sst.c); NULL until or_sst_enable */
     or_sst_params sst;

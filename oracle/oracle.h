@@ -150,6 +150,13 @@ void or_dual_volume(or_case *c);
 /* systems.  *_assemble fill the case's own matrix (vals/b) for N=3 or N=1. */
 void or_assemble_momentum(or_case *c);   /* navierStokesEquation::solve :159-173 */
 void or_compute_du(or_case *c);          /* navierStokesAssembler.cpp:84-232     */
+/* assembleNormalRelaxation (navierStokesAssembler.cpp:473-696, called with
+ * urf 0.75 by navierStokesEquation.cpp:169): symmetry-node diagonal blocks
+ * += (1 - 0.75) D n n^T, after the du coefficients */
+void or_normal_relaxation(or_case *c);
+/* boundary physical types per boundary id: wall (no-slip) or symmetry */
+enum { OR_BC_WALL = 0, OR_BC_SYMMETRY = 3 };
+int or_case_set_bc_types(or_case *c, const int *type);
 void or_assemble_pcorr(or_case *c);      /* pressureCorrectionEquation::solve :89-107 */
 double *or_case_vals(or_case *c, int N); /* [nnz][N*N] */
 double *or_case_rhs(or_case *c, int N);  /* [nn][N]     */

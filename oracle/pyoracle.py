@@ -131,6 +131,8 @@ def lib():
             getattr(L, f).argtypes = [C.c_void_p, C.c_int]
         L.or_sst_iteration.argtypes = [C.c_void_p, _dp, _dp]
         L.or_update_blending.argtypes = [C.c_void_p, C.c_int]
+        L.or_normal_relaxation.argtypes = [C.c_void_p]
+        L.or_case_set_bc_types.argtypes = [C.c_void_p, _ip]
         L.or_hex_side_nodes.restype = _ip
         _lib = L
     return _lib
@@ -186,7 +188,7 @@ class Case:
         return self._view(lib().or_case_diag_off(self.h), self.nn)
 
     _sizes = {"U": 3, "p": 1, "gradU": 9, "gradP": 3, "du": 3, "div": 1, "vol": 1,
-              "rho": 1, "mu": 1, "beta": 3, "xyz": 3, "k": 1, "omega": 1, "gradK": 3,
+              "rho": 1, "mu": 1, "beta": 3, "symA": 3, "xyz": 3, "k": 1, "omega": 1, "gradK": 3,
               "gradW": 3, "ymin": 1, "mut": 1, "F1": 1, "Pk": 1, "Pk_wall": 1, "kprev": 1,
               "beta_k": 1, "beta_w": 1}
 
@@ -217,6 +219,16 @@ class Case:
 
     def compute_du(self):
         lib().or_compute_du(self.h)
+
+    def normal_relaxation(self):
+        lib().or_normal_relaxation(self.h)
+
+    BC_WALL, BC_SYMMETRY = 0, 3
+
+    def set_bc_types(self, types):
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        if lib().or_case_set_bc_types(self.h, t.ctypes.data_as(_ip)) != 0:
+            raise ValueError("unsupported boundary type")
 
     def assemble_pcorr(self):
         lib().or_assemble_pcorr(self.h)

@@ -112,7 +112,7 @@ static double pk_by_mut(const double *g)
 void or_sst_update_Pk(or_case *c)
 {
     for (int i = 0; i < c->nn; ++i)
-        c->Pk[i] = c->bnode[i] ? c->Pk_wall[i] : fmax(c->mut[i] * pk_by_mut(&c->gradU[9 * i]), 0.0);
+        c->Pk[i] = c->wnode[i] ? c->Pk_wall[i] : fmax(c->mut[i] * pk_by_mut(&c->gradU[9 * i]), 0.0);
 }
 
 static void node_terms(or_case *c, int which, int i, double *lhs, double *rhs)
@@ -241,7 +241,7 @@ void or_assemble_scalar(or_case *c, int which)
         if (c->bnode[i]) c->b1[i] *= 0.75;
     if (which == 1)
         for (int i = 0; i < nn; ++i)
-            if (c->bnode[i]) {
+            if (c->wnode[i]) {  /* omega rows fixed at no-slip wall nodes */
                 for (int s = c->row_ptr[i]; s < c->row_ptr[i + 1]; ++s)
                     if (s - c->row_ptr[i] != c->diag_off[i]) c->A1[s] = 0.0;
                 c->b1[i] = 0.0;
