@@ -1,0 +1,362 @@
+/*
+ * bc.c -- oracle restatement of the velocity-inlet and static-pressure-outlet
+ * boundary conditions of the segregated flow and k-omega SST path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Inlet: specified velocity (per boundary id, the bc_U row), zero-gradient
+ * pressure, specified k / omega.  Outlet: specified static pressure,
+ * zero-gradient velocity and k / omega, with the flow-reversal flag of each
+ * face ip.  Side mass flow rates live per boundary face ip ([nbf][4]).
+ *
+ * References (src/):
+ *   side mass flow init   model/flow/flowModel.cpp:3802-4997
+ *     (inlet specifiedValue: rho U_bc . A; outlet default: max(0, rho U_ip . A))
+ *   side mass flow update flowModel.cpp:6470-6598 (inlet), :7323-7690 (outlet,
+ *     Rhie-Chow at the face ip with the face gradient operator and the
+ *     specified pressure on the face nodes)
+ *   reversal flags        flowModel.cpp:15226-15454 (outlet staticPressure)
+ *   mass divergence       flowModel.cpp:14237-14500 (side flows to the ip's
+ *     nearest node)
+ *   momentum              assemble/flow/segregatedFlow/navierStokesAssembler/
+ *     navierStokesAssemblerElemBoundaryConditions.cpp:791-1124 (inlet:
+ *     IP_EXPLICIT_ADVECTIVE_FLUX__, IP_FULL_STRESS_FIXED_VEL__ :344-392),
+ *     :1468-1775 (outlet: IP_IMPLICIT_UPWIND_ADVECTIVE_FLUX__,
+ *     IP_ZERO_NORMAL_STRESS__ :440-500); incompressible, so no divU terms
+ *   pressure correction   pressureCorrectionAssembler/
+ *     pressureCorrectionAssemblerElemBoundaryConditions.cpp:923-1393 (inlet),
+ *     :1895-2530 (outlet)
+ *   k / omega             assemble/phiAssembler/phiAssemblerElemBoundaryConditions.hpp:
+ *     894-1229 (inlet fixed value), 1231-1437 (outlet zero gradient)
+ */
+#include "case_internal.h"
+#include "oracle.h"
+
+#include <math.h>
+#include <string.h>
+
+
+
+static int face_type(const or_case *c, int f) { return c->bc_type[c->bf_bc[f]]; }
+
+/* element nodes, face nodes (side-ordinal order) and coordinates */
+static void face_gather(const or_case *c, int f, const int **en, int fn[4], double co[24])
+{
+    const int *sn = or_hex_side_nodes();
+    *en = &c->conn[8 * c->bf_elem[f]];
+    for (int k = 0; k < 4; ++k) fn[k] = (*en)[sn[4 * c->bf_ord[f] + k]];
+    or_gather_coords(c, c->bf_elem[f], co);
+}
+
+/* ------------------------------------------------------ side mass flow */
+void or_bc_mdot_init(or_case *c)
+{
+    double fsf[16];
+    or_quad_face_shape(c->prm.u_shifted, fsf);
+    for (int f = 0; f < c->nbf; ++f) {
+        const int t = face_type(c, f);
+        if (t != OR_BC_INLET && t != OR_BC_OUTLET) continue;
+        const int *en;
+        int fn[4];
+        double co[24];
+        face_gather(c, f, &en, fn, co);
+        const double *Ubc = &c->bc_U[3 * c->bf_bc[f]];
+        for (int ip = 0; ip < 4; ++ip) {
+            const double *ax = &c->bareav[12 * f + 3 * ip];
+            double rip = 0, u[3] = {0, 0, 0};
+            for (int ic = 0; ic < 4; ++ic) {
+                const double r = fsf[4 * ip + ic];
+                rip += r * c->rho[fn[ic]];
+                for (int j = 0; j < 3; ++j) u[j] += r * c->U[3 * fn[ic] + j];
+            }
+            double m = 0;
+            for (int j = 0; j < 3; ++j) m += rip * (t == OR_BC_INLET ? Ubc[j] : u[j]) * ax[j];
+            c->msd[4 * f + ip] = t == OR_BC_INLET ? m : fmax(0.0, m);
+        }
+    }
+}
+
+/* Rhie-Chow mass flow at outlet face ip `ip` (duRhs = du: SIMPLE) */
+static double outlet_rhie_chow(const or_case *c, int f, int ip, const double *fdn,
+                               const double *fsf, const int *en, const int fn[4])
+{
+    const int *sn = or_hex_side_nodes();
+    const int ord = c->bf_ord[f];
+    const double pbc = c->bc_p[c->bf_bc[f]];
+    double p8[8];
+    for (int a = 0; a < 8; ++a) p8[a] = c->p[en[a]];
+    for (int k = 0; k < 4; ++k) p8[sn[4 * ord + k]] = pbc;  /* nodal side pressure */
+    double rip = 0, u[3] = {0, 0, 0}, du[3] = {0, 0, 0}, Gp[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
+    for (int ic = 0; ic < 4; ++ic) {
+        const double r = fsf[4 * ip + ic];
+        rip += r * c->rho[fn[ic]];
+        for (int j = 0; j < 3; ++j) {
+            u[j] += r * c->U[3 * fn[ic] + j];
+            du[j] += r * c->du[3 * fn[ic] + j];
+            Gp[j] += 0.25 * c->gradP[3 * fn[ic] + j];
+        }
+    }
+    for (int ic = 0; ic < 8; ++ic)
+        for (int j = 0; j < 3; ++j) dp[j] += fdn[24 * ip + 3 * ic + j] * p8[ic];
+    const double *ax = &c->bareav[12 * f + 3 * ip];
+    double m = 0;
+    for (int j = 0; j < 3; ++j) m += (rip * u[j] - rip * du[j] * (dp[j] - Gp[j])) * ax[j];
+    return m;
+}
+
+void or_bc_mdot_update(or_case *c)
+{
+    const double urf = c->prm.urf_mdot;
+    double fsf[16];
+    or_quad_face_shape(c->prm.u_shifted, fsf);
+    for (int f = 0; f < c->nbf; ++f) {
+        const int t = face_type(c, f);
+        if (t != OR_BC_INLET && t != OR_BC_OUTLET) continue;
+        const int *en;
+        int fn[4];
+        double co[24], fdn[96], fdet[4];
+        face_gather(c, f, &en, fn, co);
+        if (t == OR_BC_OUTLET) or_hex_face_gradop(c->prm.pgrad_shifted, c->bf_ord[f], co, fdn, fdet);
+        const double *Ubc = &c->bc_U[3 * c->bf_bc[f]];
+        for (int ip = 0; ip < 4; ++ip) {
+            double *md = &c->msd[4 * f + ip];
+            if (t == OR_BC_INLET) {
+                const double *ax = &c->bareav[12 * f + 3 * ip];
+                double rip = 0;
+                for (int ic = 0; ic < 4; ++ic) rip += fsf[4 * ip + ic] * c->rho[fn[ic]];
+                double m = 0;
+                for (int j = 0; j < 3; ++j) m += (rip * Ubc[j]) * ax[j];
+                *md = urf * m + (1.0 - urf) * *md;
+            } else {
+                if (c->rfl[4 * f + ip] == 1) {  /* slip wall while reversed */
+                    *md = 0.0;
+                    continue;
+                }
+                const double m = outlet_rhie_chow(c, f, ip, fdn, fsf, en, fn);
+                *md = urf * m + (1.0 - urf) * *md;
+            }
+        }
+    }
+}
+
+void or_bc_reversal(or_case *c)
+{
+    for (int f = 0; f < c->nbf; ++f) {
+        if (face_type(c, f) != OR_BC_OUTLET) continue;
+        const int *en;
+        int fn[4];
+        double co[24];
+        face_gather(c, f, &en, fn, co);
+        double *md = &c->msd[4 * f];
+        int *rf = &c->rfl[4 * f];
+        double side = 0;
+        for (int ip = 0; ip < 4; ++ip) {
+            side += md[ip];
+            md[ip] = fmax(md[ip], 0.0);
+        }
+        const double pbc = c->bc_p[c->bf_bc[f]];
+        for (int ip = 0; ip < 4; ++ip) {
+            const double *ax = &c->bareav[12 * f + 3 * ip];
+            const double amag = sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+            double pavg = 0, uavg[3] = {0, 0, 0};
+            for (int ic = 0; ic < 4; ++ic) {
+                pavg += 0.25 * c->p[fn[ic]];
+                for (int j = 0; j < 3; ++j) uavg[j] += 0.25 * c->U[3 * fn[ic] + j];
+            }
+            double vn = 0;
+            for (int j = 0; j < 3; ++j) vn += uavg[j] * ax[j] / amag;
+            if (rf[ip] == 0) {
+                if (side < -SMALL) {
+                    rf[ip] = 1;
+                    md[ip] = 0.0;
+                }
+            } else if (vn >= 0.0 && pbc <= pavg) {
+                rf[ip] = 0;
+            }
+        }
+    }
+}
+
+void or_bc_divergence(or_case *c)
+{
+    const int *sn = or_hex_side_nodes();
+    for (int f = 0; f < c->nbf; ++f) {
+        const int t = face_type(c, f);
+        if (t != OR_BC_INLET && t != OR_BC_OUTLET) continue;
+        const int *en = &c->conn[8 * c->bf_elem[f]];
+        for (int ip = 0; ip < 4; ++ip) c->div[en[sn[4 * c->bf_ord[f] + ip]]] += c->msd[4 * f + ip];
+    }
+}
+
+/* ------------------------------------------------------------ momentum */
+void or_bc_momentum(or_case *c, int f, double *lhs, double *rhs)
+{
+    const or_params *P = &c->prm;
+    const int *sn = or_hex_side_nodes();
+    const int ord = c->bf_ord[f], t = face_type(c, f);
+    const int *en;
+    int fn[4];
+    double co[24], fdn[96], fdet[4], fsf[16], Uw[24], bcm[8], muf[4];
+    face_gather(c, f, &en, fn, co);
+    or_quad_face_shape(P->u_shifted, fsf);
+    or_hex_face_gradop(P->ugrad_shifted, ord, co, fdn, fdet);
+    const double *Ubc = &c->bc_U[3 * c->bf_bc[f]];
+    for (int a = 0; a < 8; ++a) {
+        bcm[a] = 1.0;
+        for (int d = 0; d < 3; ++d) Uw[3 * a + d] = c->U[3 * en[a] + d];
+    }
+    for (int k = 0; k < 4; ++k) {
+        muf[k] = c->mu[fn[k]];
+        if (t == OR_BC_INLET) {
+            const int ic = sn[4 * ord + k];
+            bcm[ic] = 0.0;
+            for (int d = 0; d < 3; ++d) Uw[3 * ic + d] = Ubc[d];
+        }
+    }
+    for (int ip = 0; ip < 4; ++ip) {
+        const int nearest = sn[4 * ord + ip];
+        const double *ax = &c->bareav[12 * f + 3 * ip];
+        const double tm = c->msd[4 * f + ip];
+        double mub = 0;
+        for (int ic = 0; ic < 4; ++ic) mub += fsf[4 * ip + ic] * muf[ic];
+        if (t == OR_BC_INLET) {
+            for (int i = 0; i < 3; ++i) rhs[3 * nearest + i] -= tm * Ubc[i];
+            for (int ic = 0; ic < 8; ++ic) {
+                const double *dn = &fdn[24 * ip + 3 * ic];
+                for (int i = 0; i < 3; ++i) {
+                    const int iR = 3 * nearest + i, rowR = iR * 24;
+                    for (int j = 0; j < 3; ++j) {
+                        double lf = -mub * dn[j] * ax[j];
+                        lhs[rowR + 3 * ic + i] += lf * bcm[ic];
+                        rhs[iR] -= lf * Uw[3 * ic + i];
+                        lf = -mub * dn[i] * ax[j];
+                        lhs[rowR + 3 * ic + j] += lf * bcm[ic];
+                        rhs[iR] -= lf * Uw[3 * ic + j];
+                    }
+                }
+            }
+            continue;
+        }
+        if (c->rfl[4 * f + ip] == 1) continue;  /* reversed: slip wall, nothing */
+        for (int i = 0; i < 3; ++i) {
+            const int iR = 3 * nearest + i;
+            rhs[iR] -= tm * Uw[3 * nearest + i];
+            lhs[iR * 24 + 3 * nearest + i] += tm;
+        }
+        const double amag = sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+        double nx[3];
+        for (int i = 0; i < 3; ++i) nx[i] = ax[i] / amag;
+        for (int ic = 0; ic < 8; ++ic) {
+            const double *dn = &fdn[24 * ip + 3 * ic];
+            for (int j = 0; j < 3; ++j) {
+                const double axj = ax[j], dnj = dn[j], uxj = Uw[3 * ic + j];
+                for (int i = 0; i < 3; ++i) {
+                    const int iR = 3 * nearest + i, rowR = iR * 24;
+                    const double dni = dn[i], uxi = Uw[3 * ic + i], nxi = nx[i];
+                    const double om = 1.0 - nxi * nxi;
+                    double lf = -mub * dnj * axj * om;
+                    lhs[rowR + 3 * ic + i] += lf;
+                    rhs[iR] -= lf * uxi;
+                    lf = -mub * dni * axj * om;
+                    lhs[rowR + 3 * ic + j] += lf;
+                    rhs[iR] -= lf * uxj;
+                    for (int l = 0; l < 3; ++l) {
+                        if (l == i) continue;
+                        const double nn_ = nxi * nx[l];
+                        lf = mub * dnj * axj * nn_;
+                        lhs[rowR + 3 * ic + l] += lf;
+                        rhs[iR] -= lf * Uw[3 * ic + l];
+                        lf = mub * dn[l] * axj * nn_;
+                        lhs[rowR + 3 * ic + j] += lf;
+                        rhs[iR] -= lf * uxj;
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* --------------------------------------------------- pressure correction */
+void or_bc_pcorr(or_case *c, int f, double *lhs, double *rhs)
+{
+    const or_params *P = &c->prm;
+    const int *sn = or_hex_side_nodes();
+    const int ord = c->bf_ord[f], t = face_type(c, f);
+    const int *en;
+    int fn[4];
+    double co[24], fdn[96], fdet[4], fsf[16], bcm[8];
+    face_gather(c, f, &en, fn, co);
+    or_quad_face_shape(P->u_shifted, fsf);
+    const double *Ubc = &c->bc_U[3 * c->bf_bc[f]];
+    if (t == OR_BC_OUTLET) or_hex_face_gradop(P->pgrad_shifted, ord, co, fdn, fdet);
+    for (int a = 0; a < 8; ++a) bcm[a] = 1.0;
+    for (int k = 0; k < 4; ++k) bcm[sn[4 * ord + k]] = 0.0;
+    for (int ip = 0; ip < 4; ++ip) {
+        const int nearest = sn[4 * ord + ip];
+        const double *ax = &c->bareav[12 * f + 3 * ip];
+        double rip = 0;
+        for (int ic = 0; ic < 4; ++ic) rip += fsf[4 * ip + ic] * c->rho[fn[ic]];
+        if (t == OR_BC_INLET) {
+            double m = 0;
+            for (int j = 0; j < 3; ++j) m += (rip * Ubc[j]) * ax[j];
+            rhs[nearest] -= m;
+            continue;
+        }
+        if (c->rfl[4 * f + ip] != 0) continue;  /* reversed: no mass flux (static mesh) */
+        double du[3] = {0, 0, 0};
+        for (int ic = 0; ic < 4; ++ic)
+            for (int j = 0; j < 3; ++j) du[j] += fsf[4 * ip + ic] * c->du[3 * fn[ic] + j];
+        for (int ic = 0; ic < 8; ++ic) {
+            double lf = 0;
+            for (int j = 0; j < 3; ++j) lf += -rip * du[j] * fdn[24 * ip + 3 * ic + j] * ax[j];
+            lhs[8 * nearest + ic] += lf * bcm[ic];
+        }
+        rhs[nearest] -= outlet_rhie_chow(c, f, ip, fdn, fsf, en, fn);
+    }
+}
+
+/* -------------------------------------------------------- k and omega */
+void or_bc_scalar(or_case *c, int which, int f, const double *G8, double *lhs, double *rhs)
+{
+    const or_sst_params *S = &c->sst;
+    const int *sn = or_hex_side_nodes();
+    const int ord = c->bf_ord[f], t = face_type(c, f);
+    const double *phiF = which == 0 ? c->k : c->w;
+    const int *en;
+    int fn[4];
+    double co[24], fdn[96], fdet[4], fsf[16], bcm[8], ph[8];
+    face_gather(c, f, &en, fn, co);
+    const double pbc = which == 0 ? c->bc_k[c->bf_bc[f]] : c->bc_w[c->bf_bc[f]];
+    for (int a = 0; a < 8; ++a) {
+        bcm[a] = 1.0;
+        ph[a] = phiF[en[a]];
+    }
+    for (int k = 0; k < 4; ++k) {
+        bcm[sn[4 * ord + k]] = 0.0;
+        ph[sn[4 * ord + k]] = pbc;
+    }
+    if (t == OR_BC_INLET) {
+        or_quad_face_shape(which == 0 ? S->k_shifted : S->w_shifted, fsf);
+        or_hex_face_gradop(which == 0 ? S->kgrad_shifted : S->wgrad_shifted, ord, co, fdn, fdet);
+    }
+    for (int ip = 0; ip < 4; ++ip) {
+        if (c->rfl[4 * f + ip] != 0) continue;
+        const int nearest = sn[4 * ord + ip];
+        const double tm = c->msd[4 * f + ip];
+        if (t == OR_BC_OUTLET) {
+            rhs[nearest] -= tm * phiF[en[nearest]];
+            lhs[8 * nearest + nearest] += tm;
+            continue;
+        }
+        rhs[nearest] -= tm * pbc;
+        double Gb = 0;
+        for (int ic = 0; ic < 4; ++ic) Gb += fsf[4 * ip + ic] * G8[sn[4 * ord + ic]];
+        const double *ax = &c->bareav[12 * f + 3 * ip];
+        for (int ic = 0; ic < 8; ++ic)
+            for (int j = 0; j < 3; ++j) {
+                const double lf = -Gb * fdn[24 * ip + 3 * ic + j] * ax[j];
+                lhs[8 * nearest + ic] += lf * bcm[ic];
+                rhs[nearest] -= lf * ph[ic];
+            }
+    }
+}

@@ -91,6 +91,11 @@ or_case *or_case_create(int nn, int ne, const double *xyz, const int *conn, int 
     c->wnode = (int *)xcalloc((size_t)nn, sizeof(int));
     c->symA = (double *)xcalloc((size_t)nn * 3, sizeof(double));
     c->bc_type = (int *)xcalloc((size_t)nbc, sizeof(int));
+    c->bc_p = (double *)xcalloc((size_t)nbc, sizeof(double));
+    c->bc_k = (double *)xcalloc((size_t)nbc, sizeof(double));
+    c->bc_w = (double *)xcalloc((size_t)nbc, sizeof(double));
+    c->msd = (double *)xcalloc((size_t)nbf * 4, sizeof(double));
+    c->rfl = (int *)xcalloc((size_t)nbf * 4, sizeof(int));
     for (int i = 0; i < nn; ++i) {
         c->rho[i] = prm->rho;
         c->mu[i] = prm->mu;
@@ -121,7 +126,8 @@ void or_case_destroy(or_case *c)
                     c->row_ptr, c->cols, c->diag_off, c->A3, c->b3, c->x3,
                     c->A1, c->b1, c->x1, c->U, c->p, c->gradU, c->gradP, c->du,
                     c->mdot, c->div, c->vol, c->rho, c->mu, c->beta, c->bareav,
-                    c->bnode, c->wnode, c->symA, c->bc_type,
+                    c->bnode, c->wnode, c->symA, c->bc_type, c->bc_p, c->bc_k, c->bc_w,
+                    c->msd, c->rfl,
                     c->k, c->w, c->gradK, c->gradW, c->ymin, c->mut, c->F1,
                     c->Pk, c->Pk_wall, c->kprev, c->beta_k, c->beta_w};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
@@ -135,11 +141,14 @@ void or_case_destroy(or_case *c)
 int or_case_set_bc_types(or_case *c, const int *type)
 {
     for (int b = 0; b < c->nbc; ++b)
-        if (type[b] != OR_BC_WALL && type[b] != OR_BC_SYMMETRY) return 1;
+        if (type[b] < OR_BC_WALL || type[b] > OR_BC_SYMMETRY) return 1;
     memcpy(c->bc_type, type, sizeof(int) * (size_t)c->nbc);
     memset(c->wnode, 0, sizeof(int) * (size_t)c->nn);
     memset(c->symA, 0, sizeof(double) * 3 * (size_t)c->nn);
     c->has_sym = 0;
+    c->has_outlet = 0;
+    for (int f = 0; f < c->nbf; ++f)
+        if (c->bc_type[c->bf_bc[f]] == OR_BC_OUTLET) c->has_outlet = 1;
     const int *sn = or_hex_side_nodes();
     for (int f = 0; f < c->nbf; ++f) {
         const int *en = &c->conn[8 * c->bf_elem[f]];
@@ -155,6 +164,18 @@ int or_case_set_bc_types(or_case *c, const int *type)
     }
     return 0;
 }
+
+void or_case_set_bc_values(or_case *c, const double *p, const double *k, const double *w)
+{
+    for (int b = 0; b < c->nbc; ++b) {
+        c->bc_p[b] = p ? p[b] : 0.0;
+        c->bc_k[b] = k ? k[b] : 0.0;
+        c->bc_w[b] = w ? w[b] : 0.0;
+    }
+}
+
+const int *or_case_reversal(const or_case *c) { return c->rfl; }
+void or_update_reversal(or_case *c) { or_bc_reversal(c); }
 
 static int sym_normal(const or_case *c, int i, double n[3])
 {
@@ -197,6 +218,7 @@ double *or_case_field(or_case *c, const char *name)
     if (!strcmp(name, "Pk_wall")) return c->Pk_wall;
     if (!strcmp(name, "kprev")) return c->kprev;
     if (!strcmp(name, "symA")) return c->symA;
+    if (!strcmp(name, "msd")) return c->msd;
     if (!strcmp(name, "beta_k")) return c->beta_k;
     if (!strcmp(name, "beta_w")) return c->beta_w;
     return NULL;
@@ -253,6 +275,7 @@ static void mdot_init(or_case *c)
             c->mdot[12 * e + ip] = t;
         }
     }
+    or_bc_mdot_init(c);
 }
 
 void or_update_mdot(or_case *c)
@@ -297,6 +320,7 @@ void or_update_mdot(or_case *c)
             md[ip] = urf * t + (1.0 - urf) * md[ip];
         }
     }
+    or_bc_mdot_update(c);
 }
 
 void or_mass_divergence(or_case *c)
@@ -311,7 +335,9 @@ void or_mass_divergence(or_case *c)
             c->div[en[lr[2 * ip + 1]]] -= m;
         }
     }
-    /* wall boundaries carry zero side mass flow: no boundary contribution */
+    /* side flows of inlets and outlets to the ip's nearest node; static walls
+     * and symmetry planes carry none */
+    or_bc_divergence(c);
 }
 
 /* ------------------------------------------------------------- gradients */
@@ -517,7 +543,16 @@ void or_assemble_momentum(or_case *c)
     or_quad_face_shape(P->u_shifted, fsf);
     for (int f = 0; f < c->nbf; ++f) {
         const int e = c->bf_elem[f], ord = c->bf_ord[f];
-        if (c->bc_type[c->bf_bc[f]] == OR_BC_SYMMETRY) {
+        const int btype = c->bc_type[c->bf_bc[f]];
+        if (btype == OR_BC_INLET || btype == OR_BC_OUTLET) {
+            memset(lhs, 0, sizeof(lhs));
+            memset(rhs, 0, sizeof(rhs));
+            for (int a = 0; a < 8; ++a) nodes[a] = c->conn[8 * e + a];
+            or_bc_momentum(c, f, lhs, rhs);
+            or_apply_coeff(3, 8, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
+            continue;
+        }
+        if (btype == OR_BC_SYMMETRY) {
             const int *en = &c->conn[8 * e];
             memset(lhs, 0, sizeof(lhs));
             memset(rhs, 0, sizeof(rhs));
@@ -712,9 +747,20 @@ void or_assemble_pcorr(or_case *c)
         }
         or_apply_coeff(1, 8, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A1, c->b1);
     }
-    /* no-slip static walls add nothing to p' (no frame/mesh motion) */
-    /* pressure reference: zero the row, keep the diagonal, b = 0 */
-    const int r = P->p_ref_node;
+    /* no-slip static walls and symmetry planes add nothing to p' (no
+     * frame/mesh motion); inlets and outlets do */
+    for (int f = 0; f < c->nbf; ++f) {
+        const int btype = c->bc_type[c->bf_bc[f]];
+        if (btype != OR_BC_INLET && btype != OR_BC_OUTLET) continue;
+        memset(lhs, 0, sizeof(lhs));
+        memset(rhs, 0, sizeof(rhs));
+        for (int a = 0; a < 8; ++a) nodes[a] = c->conn[8 * c->bf_elem[f] + a];
+        or_bc_pcorr(c, f, lhs, rhs);
+        or_apply_coeff(1, 8, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A1, c->b1);
+    }
+    /* pressure reference (only without a pressure-specified boundary,
+     * domain::pressureLevelRequired): zero the row, keep the diagonal, b = 0 */
+    const int r = c->has_outlet ? -1 : P->p_ref_node;
     if (r >= 0 && r < nn) {
         const double d = c->A1[c->row_ptr[r] + c->diag_off[r]];
         for (int k = c->row_ptr[r]; k < c->row_ptr[r + 1]; ++k) c->A1[k] = 0.0;
@@ -939,6 +985,7 @@ void or_outer_iteration(or_case *c, const double *ctl, double *hist)
     or_correct_p(c, P->urf_p);
     /* segregatedFlowEquations.cpp:106-285 */
     or_update_mdot(c);
+    or_bc_reversal(c);
     or_mass_divergence(c);
     or_velocity_correction(c, 1);
     or_gradient(c, 1);

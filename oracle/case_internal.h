@@ -28,7 +28,10 @@ struct or_case {
     int *bc_type;   /* physical type of each boundary id (OR_BC_*) */
     int *wnode;     /* node is on a no-slip wall face */
     double *symA;   /* assembled symmetry area vector [nn][3] (0 off symmetry) */
-    int has_sym;
+    int has_sym, has_outlet;
+    double *bc_p, *bc_k, *bc_w; /* outlet static pressure, inlet k / omega per id */
+    double *msd;    /* side mass flow rate per boundary face ip [nbf][4] */
+    int *rfl;       /* flow-reversal flag per boundary face ip [nbf][4] */
     double U_scale, p_scale;
     /* k-omega SST (sst.c); NULL until or_sst_enable */
     or_sst_params sst;
@@ -48,6 +51,15 @@ static inline void or_gather_coords(const or_case *c, int e, double *co)
 void or_gradient_field(or_case *c, const double *phi, double *grad, int N, int shifted);
 /* scale of an N-component field, nodeField::updateScale (case.c) */
 void or_field_scale(const double *v, int n, int N, double *scale);
+
+/* inlet / outlet boundary conditions (bc.c) */
+void or_bc_mdot_init(or_case *c);
+void or_bc_mdot_update(or_case *c);
+void or_bc_reversal(or_case *c);
+void or_bc_divergence(or_case *c);
+void or_bc_momentum(or_case *c, int f, double *lhs, double *rhs);
+void or_bc_pcorr(or_case *c, int f, double *lhs, double *rhs);
+void or_bc_scalar(or_case *c, int which, int f, const double *G8, double *lhs, double *rhs);
 
 
 #endif

@@ -154,9 +154,18 @@ void or_compute_du(or_case *c);          /* navierStokesAssembler.cpp:84-232    
  * urf 0.75 by navierStokesEquation.cpp:169): symmetry-node diagonal blocks
  * += (1 - 0.75) D n n^T, after the du coefficients */
 void or_normal_relaxation(or_case *c);
-/* boundary physical types per boundary id: wall (no-slip) or symmetry */
-enum { OR_BC_WALL = 0, OR_BC_SYMMETRY = 3 };
+/* boundary physical types per boundary id: no-slip wall, velocity inlet
+ * (the bc_U row; zero-gradient pressure), static-pressure outlet
+ * (zero-gradient velocity), symmetry plane */
+enum { OR_BC_WALL = 0, OR_BC_INLET = 1, OR_BC_OUTLET = 2, OR_BC_SYMMETRY = 3 };
 int or_case_set_bc_types(or_case *c, const int *type);
+/* per boundary id: outlet static pressure, inlet k and omega (NULL: zero) */
+void or_case_set_bc_values(or_case *c, const double *p, const double *k, const double *w);
+/* flow-reversal flags of the boundary face ips [nbf][4] */
+const int *or_case_reversal(const or_case *c);
+/* flowModel::updateFlowReversalFlag (flowModel.cpp:14949-15454) for the
+ * static-pressure outlets: clip the side mass flow, set / clear flags */
+void or_update_reversal(or_case *c);
 void or_assemble_pcorr(or_case *c);      /* pressureCorrectionEquation::solve :89-107 */
 double *or_case_vals(or_case *c, int N); /* [nnz][N*N] */
 double *or_case_rhs(or_case *c, int N);  /* [nn][N]     */

@@ -133,6 +133,10 @@ def lib():
         L.or_update_blending.argtypes = [C.c_void_p, C.c_int]
         L.or_normal_relaxation.argtypes = [C.c_void_p]
         L.or_case_set_bc_types.argtypes = [C.c_void_p, _ip]
+        L.or_case_set_bc_values.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.or_case_reversal.argtypes = [C.c_void_p]
+        L.or_case_reversal.restype = _ip
+        L.or_update_reversal.argtypes = [C.c_void_p]
         L.or_hex_side_nodes.restype = _ip
         _lib = L
     return _lib
@@ -157,6 +161,7 @@ class Case:
         self.nn = mesh["xyz"].shape[0]
         self.ne = mesh["conn"].shape[0]
         bc = np.ascontiguousarray(bc_U, dtype=np.float64)
+        self._nbc = bc.shape[0]
         self._keep = [mesh["xyz"], mesh["conn"], mesh["bf_elem"], mesh["bf_ord"],
                       mesh["bf_bc"], bc]
         self.h = L.or_case_create(self.nn, self.ne, dptr(mesh["xyz"]), iptr(mesh["conn"]),
@@ -196,6 +201,8 @@ class Case:
         ptr = lib().or_case_field(self.h, name.encode())
         if name == "mdot":
             return self._view(ptr, self.ne * 12).reshape(self.ne, 12)
+        if name == "msd":
+            return self._view(ptr, self.mesh["bf_elem"].size * 4).reshape(-1, 4)
         if name == "bareav":
             return self._view(ptr, self.mesh["bf_elem"].size * 12).reshape(-1, 4, 3)
         k = self._sizes[name]
@@ -223,7 +230,23 @@ class Case:
     def normal_relaxation(self):
         lib().or_normal_relaxation(self.h)
 
-    BC_WALL, BC_SYMMETRY = 0, 3
+    BC_WALL, BC_INLET, BC_OUTLET, BC_SYMMETRY = 0, 1, 2, 3
+
+    def set_bc_values(self, p=None, k=None, w=None):
+        """Per boundary id: outlet static pressure, inlet k and omega."""
+        nb = self._nbc
+        arr = [np.ascontiguousarray(np.zeros(nb) if v is None else v, dtype=np.float64)
+               for v in (p, k, w)]
+        self._bcv = arr
+        lib().or_case_set_bc_values(self.h, *[a.ctypes.data_as(_dp) for a in arr])
+
+    def update_reversal(self):
+        lib().or_update_reversal(self.h)
+
+    def reversal(self):
+        nbf = self.mesh["bf_elem"].size
+        ptr = lib().or_case_reversal(self.h)
+        return np.ctypeslib.as_array(ptr, shape=(nbf * 4,)).reshape(nbf, 4).copy()
 
     def set_bc_types(self, types):
         t = np.ascontiguousarray(types, dtype=np.int32)
