@@ -146,17 +146,23 @@ int oagpu_element_geometry(int topology, int shifted, int n_elems, const double 
 
 /* --------------------------------------------------------------- flow */
 /* boundary physical types (the zone boundaries' types of the reference,
- * boundaryPhysicalType): no-slip wall, symmetry plane */
+ * boundaryPhysicalType): no-slip wall, velocity inlet, static-pressure
+ * outlet, symmetry plane */
 #define OAGPU_BC_WALL 0
+#define OAGPU_BC_INLET 1
+#define OAGPU_BC_OUTLET 2
 #define OAGPU_BC_SYMMETRY 3
 
-/* bc_U[n_bc][3]: wall velocity of each boundary id (no-slip walls). */
+/* bc_U[n_bc][3]: velocity of each boundary id (no-slip walls: the wall
+ * velocity; velocity inlets: the specified inlet velocity). */
 int oagpu_flow_create(oagpu_mesh *m, const oagpu_flow_params *prm, int n_bc,
                       const double *bc_U, oagpu_flow **out);
 int oagpu_flow_destroy(oagpu_flow *f);
 /* fields: "U" [nn][3], "p" [nn], "gradU" [nn][9], "gradP" [nn][3],
  * "du" [nn][3], "mdot" [ne][12], "div" [nn], "mu" [nn] (effective
- * viscosity), and the SST fields once enabled */
+ * viscosity), the SST fields once enabled, and with inlets or outlets
+ * "msd" [n_bfaces][4] (side mass flow rate per boundary face ip, outward
+ * positive) */
 int oagpu_field_upload(oagpu_flow *f, const char *name, const double *host, size_t count);
 int oagpu_field_download(oagpu_flow *f, const char *name, double *host, size_t count);
 
@@ -189,8 +195,23 @@ int oagpu_update_scales(oagpu_flow *f, double *U_scale, double *p_scale); /* nod
  * symmetry projection and normal relaxation navierStokesAssembler.cpp:387-696,
  * gradient corrections nodeVectorField.cpp:46 / nodeScalarField.cpp:45, the
  * assembled symmetry area meshGeometry.cpp:1787) carry no mass flow and no
- * scalar flux.  Other types are rejected (OAGPU_EINVAL). */
+ * scalar flux.  Velocity inlets (the bc_U row; zero-gradient pressure) and
+ * static-pressure outlets (zero-gradient velocity, k, omega) carry side mass
+ * flows (flowModel.cpp:3802-4997 initial, :6470-6598 inlet and :7323-7690
+ * outlet Rhie-Chow updates, :15226-15454 outlet reversal flags); with an
+ * outlet the p' system has no pressure-reference row.  Other types are
+ * rejected (OAGPU_EINVAL).  Resets the side mass flows and reversal flags. */
 int oagpu_flow_set_bc_types(oagpu_flow *f, int n_bc, const int *type);
+/* Per boundary id: p[n_bc] outlet static pressure, k[n_bc] / w[n_bc] inlet
+ * turbulent kinetic energy / eddy frequency (the specified values of the
+ * outlet's pressure and the inlet's k and omega boundary conditions);
+ * NULL leaves that table unchanged; all zero until set. */
+int oagpu_flow_set_bc_values(oagpu_flow *f, int n_bc, const double *p, const double *k,
+                             const double *w);
+/* updateFlowReversalFlag (flowModel.cpp:15226-15454) on its own, and the
+ * flags [n_bfaces][4] (1: the outlet face ip is reversed, a slip wall) */
+int oagpu_update_reversal(oagpu_flow *f);
+int oagpu_reversal_download(oagpu_flow *f, int *flags, size_t count);
 /* nodeField::updateBlendingFactorField (nodeField.hpp:2069): 0 U, 2 k, 3 omega */
 int oagpu_update_blending(oagpu_flow *f, int which);
 /* control residual source: RMS over rows of b / a_diag per component */

@@ -33,7 +33,7 @@ LIB_PATH = os.path.join(HERE, "liboagpu.so")
 OAGPU_FP64, OAGPU_MIXED = 0, 1
 OAGPU_STENCIL_FULL, OAGPU_STENCIL_REDUCED = 0, 1
 OAGPU_GMRES, OAGPU_CG, OAGPU_BICGSTAB, OAGPU_CG_AMG = 0, 1, 2, 3
-OAGPU_BC_WALL, OAGPU_BC_SYMMETRY = 0, 3
+OAGPU_BC_WALL, OAGPU_BC_INLET, OAGPU_BC_OUTLET, OAGPU_BC_SYMMETRY = 0, 1, 2, 3
 OAGPU_HEX8, OAGPU_TET4, OAGPU_WED6, OAGPU_PYR5 = 0, 1, 2, 3
 TOPOLOGY_SIZES = {OAGPU_HEX8: (8, 12), OAGPU_TET4: (4, 6), OAGPU_WED6: (6, 9), OAGPU_PYR5: (5, 12)}
 
@@ -111,6 +111,9 @@ _SIGS = {
     "oagpu_element_geometry": [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp],
     "oagpu_update_blending": [C.c_void_p, C.c_int],
     "oagpu_flow_set_bc_types": [C.c_void_p, C.c_int, _ip],
+    "oagpu_flow_set_bc_values": [C.c_void_p, C.c_int, _dp, _dp, _dp],
+    "oagpu_update_reversal": [C.c_void_p],
+    "oagpu_reversal_download": [C.c_void_p, _ip, C.c_size_t],
     "oagpu_sst_enable": [C.c_void_p, C.POINTER(SstParams)],
     "oagpu_sst_update": [C.c_void_p],
     "oagpu_assemble_scalar": [C.c_void_p, C.c_int],
@@ -201,6 +204,7 @@ class DeviceMesh:
         _check(L.oagpu_mesh_create(xyz.shape[0], _d(xyz), conn.shape[0], _i(conn), bfe.size,
                                    _i(bfe), _i(bfo), _i(bfb), stencil, C.byref(h)))
         self.h = h
+        self.nbf = bfe.size
         nn, ne, nnz, ninc, nbinc = (C.c_int() for _ in range(5))
         _check(L.oagpu_mesh_sizes(h, C.byref(nn), C.byref(ne), C.byref(nnz), C.byref(ninc),
                                   C.byref(nbinc)))
@@ -254,6 +258,8 @@ class FlowCase:
     def _size(self, name):
         if name == "mdot":
             return self.mesh.ne * 12
+        if name == "msd":
+            return self.mesh.nbf * 4
         return self.mesh.nn * self._comp[name]
 
     def upload(self, name, arr):
@@ -266,6 +272,8 @@ class FlowCase:
         _check(load_library().oagpu_field_download(self.h, name.encode(), _d(a), n))
         if name == "mdot":
             return a.reshape(self.mesh.ne, 12)
+        if name == "msd":
+            return a.reshape(self.mesh.nbf, 4)
         k = self._comp[name]
         return a if k == 1 else a.reshape(self.mesh.nn, k)
 
@@ -304,9 +312,27 @@ class FlowCase:
         _check(load_library().oagpu_gradient(self.h, {"U": 0, "p": 1, "k": 2, "omega": 3}[which]))
 
     def set_bc_types(self, types):
-        """Physical type per boundary id: OAGPU_BC_WALL or OAGPU_BC_SYMMETRY."""
+        """Physical type per boundary id: OAGPU_BC_WALL, OAGPU_BC_INLET,
+        OAGPU_BC_OUTLET or OAGPU_BC_SYMMETRY."""
         t = np.ascontiguousarray(types, dtype=np.int32)
         _check(load_library().oagpu_flow_set_bc_types(self.h, len(t), _i(t)))
+
+    def set_bc_values(self, p=None, k=None, w=None):
+        """Per boundary id: outlet static pressure p, inlet k and omega (None:
+        unchanged)."""
+        arrs = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (p, k, w)]
+        n = next(a.size for a in arrs if a is not None)
+        ptr = [None if a is None else _d(a) for a in arrs]
+        _check(load_library().oagpu_flow_set_bc_values(self.h, n, *ptr))
+
+    def update_reversal(self):
+        _check(load_library().oagpu_update_reversal(self.h))
+
+    def reversal(self):
+        """Reversal flag per boundary face ip [n_bfaces][4]."""
+        r = np.zeros(self.mesh.nbf * 4, np.int32)
+        _check(load_library().oagpu_reversal_download(self.h, _i(r), r.size))
+        return r.reshape(self.mesh.nbf, 4)
 
     def update_blending(self, which):
         _check(load_library().oagpu_update_blending(self.h, {"U": 0, "k": 2, "omega": 3}[which]))

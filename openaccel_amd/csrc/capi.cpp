@@ -48,7 +48,7 @@ void ensure_init() {
 }
 
 double* field_ptr(oagpu::Flow& f, const char* name, size_t* count) {
-    const size_t nn = size_t(f.m->nn), ne = size_t(f.m->ne);
+    const size_t nn = size_t(f.m->nn), ne = size_t(f.m->ne), nbf = size_t(f.m->nbf);
     struct {
         const char* n;
         oagpu::DBuf<double>* b;
@@ -61,10 +61,12 @@ double* field_ptr(oagpu::Flow& f, const char* name, size_t* count) {
                {"F1", &f.F1, nn},            {"Pk", &f.Pk, nn},     {"Pk_wall", &f.Pk_wall, nn},
                {"kprev", &f.kprev, nn},      {"beta", &f.beta, nn * 3},
                {"beta_k", &f.beta_k, nn},    {"beta_w", &f.beta_w, nn},
-               {"symA", &f.symA, nn * 3}};
+               {"symA", &f.symA, nn * 3},    {"msd", &f.msd, nbf * 4}};
     for (auto& t : tab)
         if (!std::strcmp(name, t.n)) {
-            OAGPU_REQUIRE(t.b->p, std::string("field ") + name + " not allocated (SST, high resolution or symmetry off)");
+            OAGPU_REQUIRE(t.b->p, std::string("field ") + name +
+                                          " not allocated (SST, high resolution, symmetry or"
+                                          " inlets / outlets off)");
             *count = t.c;
             return t.b->p;
         }
@@ -224,9 +226,6 @@ int oagpu_flow_create(oagpu_mesh* m, const oagpu_flow_params* prm, int n_bc, con
         std::vector<double> bc(bc_U, bc_U + 3 * size_t(n_bc));
         if (bc.empty()) bc.assign(3, 0.0);
         f.bc_U.upload(bc);
-        for (int fc = 0; fc < m->m.nbf; ++fc) {
-            // validated lazily: boundary ids must index the table
-        }
         const size_t nn = size_t(m->m.nn), ne = size_t(m->m.ne), nsell = m->m.nsell;
         f.U.alloc(nn * 3);
         f.p.alloc(nn);
@@ -454,15 +453,98 @@ int oagpu_flow_set_bc_types(oagpu_flow* h, int n_bc, const int* type) {
         OAGPU_REQUIRE(n_bc == f.nbc, "one type per boundary id");
         bool sym = false;
         for (int b = 0; b < n_bc; ++b) {
-            OAGPU_REQUIRE(type[b] == OAGPU_BC_WALL || type[b] == OAGPU_BC_SYMMETRY,
-                          "unsupported boundary type (walls and symmetry planes)");
+            OAGPU_REQUIRE(type[b] >= OAGPU_BC_WALL && type[b] <= OAGPU_BC_SYMMETRY,
+                          "unsupported boundary type (wall, inlet, outlet, symmetry)");
             sym = sym || type[b] == OAGPU_BC_SYMMETRY;
         }
+        OAGPU_HIP(hipStreamSynchronize(f.stream));
         f.h_bc_type.assign(type, type + n_bc);
         f.bc_type.upload(f.h_bc_type);
         f.has_sym = sym;
         if (sym && !f.symA.p) f.symA.alloc(size_t(f.m->nn) * 3);
+        // inlet and outlet faces, in face order
+        const int nbf = f.m->nbf;
+        std::vector<int> ids(size_t(nbf > 0 ? nbf : 1)), io;
+        if (nbf) f.m->bf_bc.download(ids.data(), size_t(nbf));
+        OAGPU_HIP(hipDeviceSynchronize());
+        f.has_outlet = false;
+        for (int i = 0; i < nbf; ++i) {
+            const int t = type[ids[i]];
+            if (t == OAGPU_BC_INLET || t == OAGPU_BC_OUTLET) io.push_back(i);
+            f.has_outlet = f.has_outlet || t == OAGPU_BC_OUTLET;
+        }
+        f.nio = int(io.size());
+        f.has_io = f.nio > 0;
+        if (f.has_io) {
+            f.io_face.upload(io);
+            // rows nearest to an inlet / outlet face ip, ascending
+            const oagpu::DevMesh& dm = *f.m;
+            std::vector<int> rows;
+            for (int r = 0; r < dm.nn; ++r)
+                for (int k = dm.h_binc_ptr[r]; k < dm.h_binc_ptr[r + 1]; ++k) {
+                    const int t = type[ids[dm.h_binc[k] >> 2]];
+                    if (t == OAGPU_BC_INLET || t == OAGPU_BC_OUTLET) {
+                        rows.push_back(r);
+                        break;
+                    }
+                }
+            f.nio_row = int(rows.size());
+            f.io_row.upload(rows);
+            if (!f.msd.p) {
+                f.msd.alloc(size_t(nbf) * 4);
+                f.rfl.alloc(size_t(nbf) * 4);
+            }
+            f.msd.zero();
+            f.rfl.zero();
+            if (!f.bc_p.p) {
+                for (auto* v : {&f.h_bc_p, &f.h_bc_k, &f.h_bc_w}) v->assign(size_t(n_bc), 0.0);
+                f.bc_p.upload(f.h_bc_p);
+                f.bc_k.upload(f.h_bc_k);
+                f.bc_w.upload(f.h_bc_w);
+            }
+        }
         oagpu::launch_boundary_types(f);
+        OAGPU_HIP(hipDeviceSynchronize());
+    });
+}
+
+int oagpu_flow_set_bc_values(oagpu_flow* h, int n_bc, const double* p, const double* k,
+                             const double* w) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h, "null flow");
+        oagpu::Flow& f = h->f;
+        OAGPU_REQUIRE(n_bc == f.nbc, "one value per boundary id");
+        if (f.h_bc_p.empty())
+            for (auto* v : {&f.h_bc_p, &f.h_bc_k, &f.h_bc_w}) v->assign(size_t(n_bc), 0.0);
+        if (p) f.h_bc_p.assign(p, p + n_bc);
+        if (k) f.h_bc_k.assign(k, k + n_bc);
+        if (w) f.h_bc_w.assign(w, w + n_bc);
+        OAGPU_HIP(hipStreamSynchronize(f.stream));
+        f.bc_p.upload(f.h_bc_p);
+        f.bc_k.upload(f.h_bc_k);
+        f.bc_w.upload(f.h_bc_w);
+        OAGPU_HIP(hipDeviceSynchronize());
+    });
+}
+
+int oagpu_update_reversal(oagpu_flow* h) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h, "null flow");
+        if (h->f.has_outlet) oagpu::launch_reversal(h->f);
+        OAGPU_HIP(hipStreamSynchronize(h->f.stream));
+    });
+}
+
+int oagpu_reversal_download(oagpu_flow* h, int* flags, size_t count) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && flags, "null argument");
+        oagpu::Flow& f = h->f;
+        OAGPU_REQUIRE(count == size_t(f.m->nbf) * 4, "flags: [n_bfaces][4]");
+        if (!f.rfl.p) {
+            std::fill(flags, flags + count, 0);
+            return;
+        }
+        f.rfl.download(flags, count, f.stream);
         OAGPU_HIP(hipStreamSynchronize(f.stream));
     });
 }
@@ -662,6 +744,7 @@ int oagpu_outer_iteration(oagpu_flow* h, const double* ctl, double* hist) {
         oagpu::launch_correct(f, 1, f.prm.urf_p);
         // mass flow, divergence, velocity correction, gradients (:106-285)
         oagpu::launch_update_mdot(f);
+        if (f.has_outlet) oagpu::launch_reversal(f);  // segregatedFlowEquations.cpp:121
         oagpu::launch_mass_divergence(f);
         oagpu::launch_velocity_correction(f, 1);
         oagpu::launch_gradient(f, 1);

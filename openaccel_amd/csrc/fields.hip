@@ -53,9 +53,14 @@ void launch_mesh_geometry(DevMesh& m, hipStream_t s) {
     OAGPU_HIP(hipGetLastError());
 }
 
-// updateMassDivergenceField_ interior ips (flowModel.cpp:13733-13772): divL += mdot, divR -= mdot
+// updateMassDivergenceField_ interior ips (flowModel.cpp:13733-13772): divL += mdot,
+// divR -= mdot; then the side mass flows of inlet and outlet face ips whose
+// nearest node this is (flowModel.cpp:14237-14500; msd null without them)
 __global__ void __launch_bounds__(256) k_mass_divergence(int nn, const int* inc_ptr, const int* inc,
-                                                         const double* mdot, double* div) {
+                                                         const double* mdot, const int* binc_ptr,
+                                                         const int* binc, const int* bf_bc,
+                                                         const int* bc_type, const double* msd,
+                                                         double* div) {
     const int a = xcd_block() * blockDim.x + threadIdx.x;
     if (a >= nn) return;
     double s = 0.0;
@@ -66,14 +71,20 @@ __global__ void __launch_bounds__(256) k_mass_divergence(int nn, const int* inc_
             if (c_node_ip_sign[la][q] > 0) s += m; else s -= m;
         }
     }
+    if (msd)
+        for (int k = binc_ptr[a]; k < binc_ptr[a + 1]; ++k) {
+            const int f = binc[k] >> 2, t = bc_type[bf_bc[f]];
+            if (t == OAGPU_BC_INLET || t == OAGPU_BC_OUTLET) s += msd[4 * size_t(f) + (binc[k] & 3)];
+        }
     div[a] = s;
 }
 
 void launch_mass_divergence(Flow& f) {
     DevMesh& m = *f.m;
     const unsigned B = 256;
-    k_mass_divergence<<<xcd_grid(m.nn, B), B, 0, f.stream>>>(m.nn, m.inc_ptr.p, m.inc.p, f.mdot.p,
-                                                             f.div.p);
+    k_mass_divergence<<<xcd_grid(m.nn, B), B, 0, f.stream>>>(
+        m.nn, m.inc_ptr.p, m.inc.p, f.mdot.p, m.binc_ptr.p, m.binc.p, m.bf_bc.p, f.bc_type.p,
+        f.has_io ? f.msd.p : nullptr, f.div.p);
     OAGPU_HIP(hipGetLastError());
 }
 

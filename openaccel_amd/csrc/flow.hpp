@@ -87,6 +87,17 @@ struct Flow {
     DBuf<int> bc_type, wnode;
     DBuf<double> symA;
     bool has_sym = false;
+    // velocity inlets and static-pressure outlets: the boundary faces of
+    // either type, the outlet static pressure and inlet k / omega per
+    // boundary id, and per face ip the side mass flow rate and (outlets) the
+    // flow-reversal flag; allocated only when such a boundary exists
+    bool has_io = false, has_outlet = false;
+    int nio = 0, nio_row = 0;
+    DBuf<int> io_face, io_row;  // inlet / outlet faces; nodes nearest to one of their ips
+    std::vector<double> h_bc_p, h_bc_k, h_bc_w;
+    DBuf<double> bc_p, bc_k, bc_w;
+    DBuf<double> msd;  // [nbf][4]
+    DBuf<int> rfl;     // [nbf][4]
     // k-omega SST state (oagpu_sst_enable)
     bool sst_on = false;
     oagpu_sst_params sst{};
@@ -102,6 +113,10 @@ void launch_mass_divergence(Flow& f);
 void launch_gradient(Flow& f, int which);
 void launch_blending(Flow& f, int which);   // 0 U, 2 k, 3 omega
 void launch_boundary_types(Flow& f);        // wall-node flags and symmetry areas
+// bc.hip: inlet / outlet side mass flow rates (init: from U; update: inlet
+// specified, outlet Rhie-Chow, relaxed) and the outlets' reversal flags
+void launch_side_mdot(Flow& f, bool init);
+void launch_reversal(Flow& f);
 void launch_assemble_momentum(Flow& f);
 void launch_assemble_pcorr(Flow& f);
 void launch_correct(Flow& f, int N, double urf);

@@ -430,6 +430,8 @@ struct MomArgs {
     const double* gradU;
     const int* bc_type;    // physical type per boundary id
     const double* symA;    // assembled symmetry area [nn][3], null without symmetry
+    const double* msd;     // side mass flow per boundary face ip, null without inlets / outlets
+    const int* rfl;        // outlet reversal flag per boundary face ip
     double rho, mu, dt, urf_inv, bnd_relax;
     int ugrad_shifted, us;
     double* vals;
@@ -604,50 +606,45 @@ __device__ __forceinline__ void mom_incidence(const MomArgs& a, int la, int e, u
     }
 }
 
-// laminar no-slip wall: the face ip of side (f, fk) nearest to this row
-// (navierStokesAssemblerElemBoundaryConditions.cpp:2659-2884,
-// IP_ZERO_NORMAL_STRESS_FIXED_VEL__ :592-654)
-template <typename T, typename LT>
-__device__ __forceinline__ void mom_wall(const MomArgs& a, int f, int fk, uint64_t sl,
+// viscous face terms of the face ip of side (f, fk) nearest to this row.
+// ZNS: zero normal stress (the stress projected off the face normal:
+// IP_ZERO_NORMAL_STRESS_FIXED_VEL__ / IP_ZERO_NORMAL_STRESS__,
+// navierStokesAssemblerElemBoundaryConditions.cpp:592-654 / :440-500),
+// else the full stress (IP_FULL_STRESS_FIXED_VEL__ / IP_FULL_STRESS__
+// :344-392 / :296-343; incompressible, so no divU term).  fixed: the side's
+// nodes carry the boundary velocity bc_U and their columns are dropped.
+//   no-slip wall      ZNS, fixed   (:2659-2884)
+//   velocity inlet    full, fixed  (:791-1124)
+//   pressure outlet   ZNS, free    (:1468-1775)
+//   symmetry plane    full, free   (:2082-2375)
+template <typename T, bool ZNS, typename LT>
+__device__ __forceinline__ void mom_face(const MomArgs& a, int f, int fk, bool fixed, uint64_t sl,
                                          const SellRow& sr, int dof, const double xr[3],
                                          double D[9], double R[3], int& bad, LT* lt) {
     const int e = a.bf_elem[f], ord = a.bf_ord[f], bc = a.bf_bc[f];
     const int* sn = hex8::c_side_nodes[ord];
-    // face ip location: exposed-face ip, or the side node itself when shifted
-    T s[3];
-    for (int d = 0; d < 3; ++d)
-        s[d] = a.ugrad_shifted ? T(hex8::c_node_loc[sn[fk]][d]) : T(hex8::c_face_loc[ord][fk][d]);
-    T Ji[9], A[3];
-    {
-        int nd[8];
-        load_conn(a.conn, e, nd);
-        T c[8][3];
-        gather_rel(a.xyz, nd, xr, c);
-        jac_inv(s[0], s[1], s[2], c, Ji, &bad);
-        T fp[4][3];
-        for (int q = 0; q < 4; ++q) {
-            const size_t o = 3 * size_t(a.conn[8 * size_t(e) + sn[q]]);
-            for (int d = 0; d < 3; ++d) fp[q][d] = T(a.xyz[o + d] - xr[d]);
-        }
-        hex8::quad_face_areav(fp, fk, A);
+    const int* ce = a.conn + 8 * size_t(e);
+    T s[3], Ji[9], A[3];
+    face_ip_geom(a.xyz, ce, ord, fk, a.ugrad_shifted, xr, s, Ji, A, bad);
+    T nx[3] = {0, 0, 0};
+    if constexpr (ZNS) {
+        const T amag = sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
+        for (int d = 0; d < 3; ++d) nx[d] = A[d] / amag;
     }
-    const T amag = sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
-    const T nx[3] = {A[0] / amag, A[1] / amag, A[2] / amag};
     T mu = T(a.mu);
     if (a.mueff) {
         mu = T(0);
-        for (int q = 0; q < 4; ++q)
-            mu += T(hex8::c_quad_shape[a.us][fk][q]) * T(a.mueff[a.conn[8 * size_t(e) + sn[q]]]);
+        for (int q = 0; q < 4; ++q) mu += T(hex8::c_quad_shape[a.us][fk][q]) * T(a.mueff[ce[sn[q]]]);
     }
     const T ub[3] = {T(a.bc_U[3 * bc]), T(a.bc_U[3 * bc + 1]), T(a.bc_U[3 * bc + 2])};
     T rl[3] = {0, 0, 0};
 #pragma unroll 1
     for (int ic = 0; ic < 8; ++ic) {
-        const bool onface = ic == sn[0] || ic == sn[1] || ic == sn[2] || ic == sn[3];
-        const T bcm = onface ? T(0) : T(1);
-        const size_t o = 3 * size_t(a.conn[8 * size_t(e) + ic]);
+        const bool mask = fixed && on_side(ord, ic);
+        const T bcm = mask ? T(0) : T(1);
+        const size_t o = 3 * size_t(ce[ic]);
         T uc[3];
-        for (int d = 0; d < 3; ++d) uc[d] = onface ? ub[d] : T(a.U[o + d]);
+        for (int d = 0; d < 3; ++d) uc[d] = mask ? ub[d] : T(a.U[o + d]);
         T g[3];
         grad_N(ic, s[0], s[1], s[2], Ji, g);
         T acc[9];
@@ -656,22 +653,24 @@ __device__ __forceinline__ void mom_wall(const MomArgs& a, int f, int fk, uint64
             const T axj = A[j], dnj = g[j], uxj = uc[j];
             for (int i = 0; i < 3; ++i) {
                 const T dni = g[i], uxi = uc[i], nxi = nx[i];
-                const T om = T(1) - nxi * nxi;
+                const T om = ZNS ? T(1) - nxi * nxi : T(1);
                 T lf = -mu * dnj * axj * om;
                 acc[3 * i + i] += lf * bcm;
                 rl[i] -= lf * uxi;
                 lf = -mu * dni * axj * om;
                 acc[3 * i + j] += lf * bcm;
                 rl[i] -= lf * uxj;
-                for (int l = 0; l < 3; ++l) {
-                    if (l == i) continue;
-                    const T nn_ = nxi * nx[l];
-                    lf = mu * dnj * axj * nn_;
-                    acc[3 * i + l] += lf * bcm;
-                    rl[i] -= lf * uc[l];
-                    lf = mu * g[l] * axj * nn_;
-                    acc[3 * i + j] += lf * bcm;
-                    rl[i] -= lf * uxj;
+                if constexpr (ZNS) {
+                    for (int l = 0; l < 3; ++l) {
+                        if (l == i) continue;
+                        const T nn_ = nxi * nx[l];
+                        lf = mu * dnj * axj * nn_;
+                        acc[3 * i + l] += lf * bcm;
+                        rl[i] -= lf * uc[l];
+                        lf = mu * g[l] * axj * nn_;
+                        acc[3 * i + j] += lf * bcm;
+                        rl[i] -= lf * uxj;
+                    }
                 }
             }
         }
@@ -680,63 +679,40 @@ __device__ __forceinline__ void mom_wall(const MomArgs& a, int f, int fk, uint64
     for (int i = 0; i < 3; ++i) R[i] += double(rl[i]);
 }
 
-// symmetry plane: the face ip of side (f, fk) nearest to this row
-// (navierStokesAssemblerElemBoundaryConditions.cpp:2082-2375,
-// IP_FULL_STRESS__ :296-343; incompressible, so no divU term)
+// the boundary face ip (f, fk) nearest to this row, by the physical type of
+// its boundary; inlets add the explicit advective flux msd U_bc
+// (IP_EXPLICIT_ADVECTIVE_FLUX__), outlets the implicit upwind one
+// (IP_IMPLICIT_UPWIND_ADVECTIVE_FLUX__: msd on the diagonal, msd U_row on
+// the rhs) unless the ip is reversed, when it takes no terms (a slip wall)
 template <typename T, typename LT>
-__device__ __forceinline__ void mom_sym(const MomArgs& a, int f, int fk, uint64_t sl,
-                                        const SellRow& sr, int dof, const double xr[3],
-                                        double D[9], double R[3], int& bad, LT* lt) {
-    const int e = a.bf_elem[f], ord = a.bf_ord[f];
-    const int* sn = hex8::c_side_nodes[ord];
-    T s[3];
-    for (int d = 0; d < 3; ++d)
-        s[d] = a.ugrad_shifted ? T(hex8::c_node_loc[sn[fk]][d]) : T(hex8::c_face_loc[ord][fk][d]);
-    T Ji[9], A[3];
-    {
-        int nd[8];
-        load_conn(a.conn, e, nd);
-        T c[8][3];
-        gather_rel(a.xyz, nd, xr, c);
-        jac_inv(s[0], s[1], s[2], c, Ji, &bad);
-        T fp[4][3];
-        for (int q = 0; q < 4; ++q) {
-            const size_t o = 3 * size_t(a.conn[8 * size_t(e) + sn[q]]);
-            for (int d = 0; d < 3; ++d) fp[q][d] = T(a.xyz[o + d] - xr[d]);
+__device__ __forceinline__ void mom_boundary(const MomArgs& a, int pk, uint64_t sl,
+                                             const SellRow& sr, int dof, int row,
+                                             const double xr[3], double D[9], double R[3],
+                                             int& bad, LT* lt) {
+    const int f = pk >> 2, fk = pk & 3;
+    const int t = (a.symA || a.msd) ? a.bc_type[a.bf_bc[f]] : OAGPU_BC_WALL;
+    if (t == OAGPU_BC_SYMMETRY) {
+        mom_face<T, false>(a, f, fk, false, sl, sr, dof, xr, D, R, bad, lt);
+    } else if (t == OAGPU_BC_INLET) {
+        const double tm = a.msd[4 * size_t(f) + fk];
+        const int bc = a.bf_bc[f];
+        for (int i = 0; i < 3; ++i) R[i] -= tm * a.bc_U[3 * bc + i];
+        mom_face<T, false>(a, f, fk, true, sl, sr, dof, xr, D, R, bad, lt);
+    } else if (t == OAGPU_BC_OUTLET) {
+        if (a.rfl[4 * size_t(f) + fk] == 1) return;
+        const double tm = a.msd[4 * size_t(f) + fk];
+        for (int i = 0; i < 3; ++i) {
+            R[i] -= tm * a.U[3 * size_t(row) + i];
+            D[4 * i] += tm;
         }
-        hex8::quad_face_areav(fp, fk, A);
+        mom_face<T, true>(a, f, fk, false, sl, sr, dof, xr, D, R, bad, lt);
+    } else {
+        mom_face<T, true>(a, f, fk, true, sl, sr, dof, xr, D, R, bad, lt);
     }
-    T mu = T(a.mu);
-    if (a.mueff) {
-        mu = T(0);
-        for (int q = 0; q < 4; ++q)
-            mu += T(hex8::c_quad_shape[a.us][fk][q]) * T(a.mueff[a.conn[8 * size_t(e) + sn[q]]]);
-    }
-    T rl[3] = {0, 0, 0};
-#pragma unroll 1
-    for (int ic = 0; ic < 8; ++ic) {
-        const size_t o = 3 * size_t(a.conn[8 * size_t(e) + ic]);
-        const T uc[3] = {T(a.U[o]), T(a.U[o + 1]), T(a.U[o + 2])};
-        T g[3];
-        grad_N(ic, s[0], s[1], s[2], Ji, g);
-        T acc[9];
-        for (int k = 0; k < 9; ++k) acc[k] = T(0);
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) {
-                T lf = -mu * g[j] * A[j];
-                acc[3 * i + i] += lf;
-                rl[i] -= lf * uc[i];
-                lf = -mu * g[i] * A[j];
-                acc[3 * i + j] += lf;
-                rl[i] -= lf * uc[j];
-            }
-        scatter_block(a.vals, sr, dof, sl, ic, acc, D, lt);
-    }
-    for (int i = 0; i < 3; ++i) R[i] += double(rl[i]);
 }
 
 // navierStokesAssembler for one row: node terms (NodeTerms.cpp:16-149),
-// interior element terms, no-slip walls, relaxation (phiAssembler.h:966-1006),
+// interior element terms, boundary face ips, relaxation (phiAssembler.h:966-1006),
 // boundary rhs relaxation (phiAssembler.h:1008-1152, navierStokesAssembler.cpp:238)
 // and du = V/(a_ii + SMALL) (navierStokesAssembler.cpp:188-190)
 // LDS: 0 = off-diagonal blocks read-modify-written in HBM (256-lane blocks);
@@ -796,13 +772,8 @@ __global__ void __launch_bounds__(LDS ? 64 : 256) k_momentum_rows(const MomArgs 
         mom_incidence<T, SH, HR>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, D, R,
                                  bad, lt);
 #pragma unroll 1
-    for (int k = a.binc_ptr[row]; k < a.binc_ptr[row + 1]; ++k) {
-        const int pk = a.binc[k];
-        if (a.symA && a.bc_type[a.bf_bc[pk >> 2]] == OAGPU_BC_SYMMETRY)
-            mom_sym<T>(a, pk >> 2, pk & 3, a.binc_slots[k], sr, dof, xr, D, R, bad, lt);
-        else
-            mom_wall<T>(a, pk >> 2, pk & 3, a.binc_slots[k], sr, dof, xr, D, R, bad, lt);
-    }
+    for (int k = a.binc_ptr[row]; k < a.binc_ptr[row + 1]; ++k)
+        mom_boundary<T>(a, a.binc[k], a.binc_slots[k], sr, dof, row, xr, D, R, bad, lt);
     if constexpr (LDS != 0) {
         const int ns = sr.width < kLdsSlots ? sr.width : kLdsSlots;
 #pragma unroll 1
@@ -887,6 +858,8 @@ void launch_assemble_momentum(Flow& f) {
     a.gradU = f.gradU.p;
     a.bc_type = f.bc_type.p;
     a.symA = f.has_sym ? f.symA.p : nullptr;
+    a.msd = f.has_io ? f.msd.p : nullptr;
+    a.rfl = f.has_io ? f.rfl.p : nullptr;
     a.vals = f.A3.p;
     a.b = f.b3.p;
     a.du = f.du.p;
@@ -929,6 +902,17 @@ struct PcArgs {
     const double* du;
     const double* p;
     const double* gradP;
+    // inlet / outlet face ips (boundary-row kernel only)
+    const int* binc_ptr;
+    const int* binc;
+    const uint64_t* binc_slots;
+    const int* bf_elem;
+    const int* bf_ord;
+    const int* bf_bc;
+    const int* bc_type;
+    const double* bc_U;
+    const double* bc_p;
+    const int* rfl;
     double rho;
     int u_shifted, p_ref;
     double* vals;
@@ -1078,8 +1062,81 @@ __device__ __forceinline__ void pc_incidence(const PcArgs& a, int la, int e, uin
     }
 }
 
+// inlet / outlet face ip (f, fk) nearest to this p' row, in FP64 in both
+// forms.  Inlet (pressureCorrectionAssemblerElemBoundaryConditions.cpp:
+// 923-1393): the specified mass flow rho U_bc . A leaves the rhs.  Outlet
+// (:1895-2530), unless reversed: the Rhie-Chow pressure term
+// -rho du_ip grad N . A for the columns off the side (the side's pressure is
+// specified) and the Rhie-Chow mass flow on the rhs.
+template <int SP>
+__device__ __forceinline__ void pc_boundary(const PcArgs& a, int pk, uint64_t sl,
+                                            const SellRow& sr, int dof, const double xr[3],
+                                            double& D, double& R, int& bad) {
+    const int f = pk >> 2, fk = pk & 3;
+    const int bc = a.bf_bc[f], t = a.bc_type[bc];
+    if (t != OAGPU_BC_INLET && t != OAGPU_BC_OUTLET) return;
+    if (t == OAGPU_BC_OUTLET && a.rfl[4 * size_t(f) + fk] != 0) return;
+    const int ord = a.bf_ord[f];
+    const int* ce = a.conn + 8 * size_t(a.bf_elem[f]);
+    double s[3], Ji[9], A[3];
+    face_ip_geom(a.xyz, ce, ord, fk, t == OAGPU_BC_OUTLET ? SP : a.u_shifted, xr, s, Ji, A, bad);
+    const int us = a.u_shifted ? 1 : 0;
+    double rip = 0;
+    for (int q = 0; q < 4; ++q) rip += hex8::c_quad_shape[us][fk][q] * a.rho;
+    if (t == OAGPU_BC_INLET) {
+        double m = 0;
+        for (int j = 0; j < 3; ++j) m += (rip * a.bc_U[3 * bc + j]) * A[j];
+        R -= m;
+        return;
+    }
+    const int* sn = hex8::c_side_nodes[ord];
+    double du[3] = {0, 0, 0};
+    for (int q = 0; q < 4; ++q) {
+        const double r = hex8::c_quad_shape[us][fk][q];
+        for (int j = 0; j < 3; ++j) du[j] += r * a.du[3 * size_t(ce[sn[q]]) + j];
+    }
+#pragma unroll 1
+    for (int ic = 0; ic < 8; ++ic) {
+        if (on_side(ord, ic)) continue;
+        double g[3];
+        grad_N(ic, s[0], s[1], s[2], Ji, g);
+        double lf = 0;
+        for (int j = 0; j < 3; ++j) lf += -rip * du[j] * g[j] * A[j];
+        const int sp = slot_of(sl, ic);
+        if (sp == 0xFF || sp == dof) D += lf;
+        else *sell_val<1>(a.vals, sr, sp, 0) += lf;
+    }
+    R -= outlet_rhie_chow(ce, ord, fk, us, s, Ji, A, a.U, a.du, a.p, a.gradP, a.rho, a.bc_p[bc]);
+}
+
+// The inlet / outlet face ips of the p' rows in io_row (each row one lane,
+// its stored row updated in place), after k_pcorr_rows.  A pressure-
+// reference row exists only without outlets, where the only face term is
+// the inlet rhs, which the reference row drops: it is skipped.
+template <int SP>
+__global__ void __launch_bounds__(256) k_pcorr_boundary(const PcArgs a, int nrow,
+                                                        const int* io_row) {
+    const int i = int(xcd_block() * blockDim.x + threadIdx.x);
+    if (i >= nrow) return;
+    const int row = io_row[i];
+    if (row == a.p_ref) return;
+    const SellRow sr = sell_row<1>(a.slice_ptr, a.slice_w, row);
+    const int dof = a.diag_off[row];
+    const double xr[3] = {a.xyz[3 * size_t(row)], a.xyz[3 * size_t(row) + 1],
+                          a.xyz[3 * size_t(row) + 2]};
+    int bad = 0;
+    double D = 0.0, R = 0.0;
+#pragma unroll 1
+    for (int k = a.binc_ptr[row]; k < a.binc_ptr[row + 1]; ++k)
+        pc_boundary<SP>(a, a.binc[k], a.binc_slots[k], sr, dof, xr, D, R, bad);
+    *sell_val<1>(a.vals, sr, dof, 0) += D;
+    a.b[row] += R;
+    if (bad) *a.err = 1;
+}
+
 // pressureCorrectionAssembler interior terms for one row, with the
-// pressure-reference row (pressureCorrectionAssembler.cpp:19-63)
+// pressure-reference row (pressureCorrectionAssembler.cpp:19-63; none when
+// an outlet specifies the pressure level)
 template <typename T, int SP>
 __global__ void __launch_bounds__(64) k_pcorr_rows(const PcArgs a) {
     __shared__ double tile[kLdsSlots * 64];
@@ -1135,9 +1192,19 @@ void launch_assemble_pcorr(Flow& f) {
     a.du = f.du.p;
     a.p = f.p.p;
     a.gradP = f.gradP.p;
+    a.binc_ptr = m.binc_ptr.p;
+    a.binc = m.binc.p;
+    a.binc_slots = m.binc_slots.p;
+    a.bf_elem = m.bf_elem.p;
+    a.bf_ord = m.bf_ord.p;
+    a.bf_bc = m.bf_bc.p;
+    a.bc_type = f.bc_type.p;
+    a.bc_U = f.bc_U.p;
+    a.bc_p = f.bc_p.p;
+    a.rfl = f.rfl.p;
     a.rho = f.prm.rho;
     a.u_shifted = f.prm.u_shifted;
-    a.p_ref = f.prm.p_ref_node;
+    a.p_ref = f.has_outlet ? -1 : f.prm.p_ref_node;  // domain::pressureLevelRequired
     a.vals = f.A1.p;
     a.b = f.b1.p;
     a.err = m.err.p;
@@ -1151,6 +1218,12 @@ void launch_assemble_pcorr(Flow& f) {
         else k_pcorr_rows<double, 0><<<G, B, 0, f.stream>>>(a);
     }
     OAGPU_HIP(hipGetLastError());
+    if (f.has_io) {
+        const unsigned GB = xcd_grid(size_t(f.nio_row), 256);
+        if (f.prm.pgrad_shifted) k_pcorr_boundary<1><<<GB, 256, 0, f.stream>>>(a, f.nio_row, f.io_row.p);
+        else k_pcorr_boundary<0><<<GB, 256, 0, f.stream>>>(a, f.nio_row, f.io_row.p);
+        OAGPU_HIP(hipGetLastError());
+    }
 }
 
 // ------------------------------------------------------ k / omega (SST)
@@ -1191,6 +1264,17 @@ struct ScArgs {
     const double* Pk;
     const double* beta;   // blending factor of phi (high resolution) or null
     const double* gphi;   // grad phi [nn][3]
+    // inlet / outlet face ips (boundary-row kernel only)
+    const int* binc_ptr;
+    const int* binc;
+    const uint64_t* binc_slots;
+    const int* bf_elem;
+    const int* bf_ord;
+    const int* bf_bc;
+    const int* bc_type;
+    const double* bc_phi;  // inlet k or omega per boundary id
+    const double* msd;
+    const int* rfl;
     double rho, mu, dt, urf_inv, sig1, sig2;
     int which, us;
     double* vals;
@@ -1378,6 +1462,84 @@ __device__ __forceinline__ void sc_incidence(const ScArgs& a, int la, int e, uin
     }
 }
 
+// inlet / outlet face ip (f, fk) nearest to this k / omega row, FP64 in
+// both forms, skipped while the ip is reversed.  Inlet
+// (phiAssemblerElemBoundaryConditions.hpp:894-1229, fixed value): the
+// advective flux msd phi_bc and the diffusive flux with phi_bc on the
+// side's nodes (their columns dropped), Gamma interpolated from the side's
+// nodes.  Outlet (:1231-1437, zero gradient): implicit upwind, msd on the
+// diagonal and msd phi_row on the rhs.
+template <int SG>
+__device__ __forceinline__ void sc_boundary(const ScArgs& a, int pk, uint64_t sl,
+                                            const SellRow& sr, int dof, int row,
+                                            const double xr[3], bool offd, double& D, double& R,
+                                            int& bad) {
+    const int f = pk >> 2, fk = pk & 3;
+    const int bc = a.bf_bc[f], t = a.bc_type[bc];
+    if (t != OAGPU_BC_INLET && t != OAGPU_BC_OUTLET) return;
+    if (a.rfl[4 * size_t(f) + fk] != 0) return;
+    const double tm = a.msd[4 * size_t(f) + fk];
+    if (t == OAGPU_BC_OUTLET) {
+        R -= tm * a.phi[row];
+        D += tm;
+        return;
+    }
+    const double pbc = a.bc_phi[bc];
+    R -= tm * pbc;
+    const int ord = a.bf_ord[f];
+    const int* sn = hex8::c_side_nodes[ord];
+    const int* ce = a.conn + 8 * size_t(a.bf_elem[f]);
+    double s[3], Ji[9], A[3];
+    face_ip_geom(a.xyz, ce, ord, fk, SG, xr, s, Ji, A, bad);
+    double Gb = 0;
+    for (int q = 0; q < 4; ++q) {
+        const int n = ce[sn[q]];
+        const double F1 = a.F1[n];
+        Gb += hex8::c_quad_shape[a.us][fk][q] * (a.mu + a.mut[n] * (F1 * a.sig1 + (1.0 - F1) * a.sig2));
+    }
+#pragma unroll 1
+    for (int ic = 0; ic < 8; ++ic) {
+        double g[3];
+        grad_N(ic, s[0], s[1], s[2], Ji, g);
+        double lf = 0;
+        for (int j = 0; j < 3; ++j) lf += -Gb * g[j] * A[j];
+        if (on_side(ord, ic)) {
+            R -= lf * pbc;
+            continue;
+        }
+        R -= lf * a.phi[ce[ic]];
+        const int sp = slot_of(sl, ic);
+        if (sp == 0xFF || sp == dof) D += lf;
+        else if (offd) *sell_val<1>(a.vals, sr, sp, 0) += lf;
+    }
+}
+
+// The inlet / outlet face ips of the k / omega rows in io_row (each row one
+// lane, its stored row updated in place), after k_scalar_rows: the diagonal
+// part relaxed by urf, the rhs part by the boundary-row factor 0.75 (the
+// rows are boundary rows).  Omega rows fixed at no-slip wall nodes keep
+// only the diagonal part (their off-diagonals and rhs are zero).
+template <int SG>
+__global__ void __launch_bounds__(256) k_scalar_boundary(const ScArgs a, int nrow,
+                                                         const int* io_row) {
+    const int i = int(xcd_block() * blockDim.x + threadIdx.x);
+    if (i >= nrow) return;
+    const int row = io_row[i];
+    const bool fixed = a.which == 1 && a.wnode[row];
+    const SellRow sr = sell_row<1>(a.slice_ptr, a.slice_w, row);
+    const int dof = a.diag_off[row];
+    const double xr[3] = {a.xyz[3 * size_t(row)], a.xyz[3 * size_t(row) + 1],
+                          a.xyz[3 * size_t(row) + 2]};
+    int bad = 0;
+    double D = 0.0, R = 0.0;
+#pragma unroll 1
+    for (int k = a.binc_ptr[row]; k < a.binc_ptr[row + 1]; ++k)
+        sc_boundary<SG>(a, a.binc[k], a.binc_slots[k], sr, dof, row, xr, !fixed, D, R, bad);
+    *sell_val<1>(a.vals, sr, dof, 0) += D * a.urf_inv;
+    if (!fixed) a.b[row] += R * 0.75;
+    if (bad) *a.err = 1;
+}
+
 // one k / omega row: node terms and sources, interior terms, relaxation
 // (phiAssembler.h:966-1006), boundary rhs relaxation x 0.75
 // (turbulentKineticEnergyAssembler.h:44-48, phiAssembler.h:1008-1152), and
@@ -1454,6 +1616,16 @@ void launch_assemble_scalar(Flow& f, int which) {
     a.Pk = f.Pk.p;
     a.beta = f.prm.high_res ? (which == 0 ? f.beta_k.p : f.beta_w.p) : nullptr;
     a.gphi = which == 0 ? f.gradK.p : f.gradW.p;
+    a.binc_ptr = m.binc_ptr.p;
+    a.binc = m.binc.p;
+    a.binc_slots = m.binc_slots.p;
+    a.bf_elem = m.bf_elem.p;
+    a.bf_ord = m.bf_ord.p;
+    a.bf_bc = m.bf_bc.p;
+    a.bc_type = f.bc_type.p;
+    a.bc_phi = which == 0 ? f.bc_k.p : f.bc_w.p;
+    a.msd = f.msd.p;
+    a.rfl = f.rfl.p;
     a.rho = f.prm.rho;
     a.mu = f.prm.mu;
     a.dt = f.prm.dt;
@@ -1478,6 +1650,12 @@ void launch_assemble_scalar(Flow& f, int which) {
         else go(hr ? k_scalar_rows<double, 0, true> : k_scalar_rows<double, 0, false>);
     }
     OAGPU_HIP(hipGetLastError());
+    if (f.has_io) {
+        const unsigned GB = xcd_grid(size_t(f.nio_row), 256);
+        if (gs) k_scalar_boundary<1><<<GB, 256, 0, f.stream>>>(a, f.nio_row, f.io_row.p);
+        else k_scalar_boundary<0><<<GB, 256, 0, f.stream>>>(a, f.nio_row, f.io_row.p);
+        OAGPU_HIP(hipGetLastError());
+    }
 }
 
 // --------------------------------------------------------------- mass flow
@@ -1593,6 +1771,7 @@ void launch_mdot_init(Flow& f) {
     else
         k_mdot_init<0><<<G, B, 0, f.stream>>>(m.ne, m.xyz.p, m.conn.p, f.U.p, f.prm.rho, f.mdot.p);
     OAGPU_HIP(hipGetLastError());
+    launch_side_mdot(f, true);
 }
 
 void launch_update_mdot(Flow& f) {
@@ -1609,6 +1788,7 @@ void launch_update_mdot(Flow& f) {
     else if (sp) args(k_mdot_update<0, 1>);
     else args(k_mdot_update<0, 0>);
     OAGPU_HIP(hipGetLastError());
+    launch_side_mdot(f, false);
 }
 
 // -------------------------------------------------------------- gradients
