@@ -7,6 +7,7 @@ update, p' assembly + AMG-preconditioned CG solve, corrections, gradients)
 over the whole mesh, inputs resident in HBM.  One JSON line is printed.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--n 128] [--perturb 0]
+    python bench.py --workload bfs [--bfs-scale 1.0]   # 8M-element step, k-omega SST
 """
 import argparse
 import json
@@ -88,6 +89,139 @@ def run_gpu(oa, mesh, precision, steps, warmup, p_method):
             "nn": dm.nn, "ne": dm.ne, "nnz": dm.nnz}
 
 
+# backward-facing step with k-omega SST: inlet U = 1 with 1 % intensity and
+# viscosity ratio 10, rho = 1, mu = 2e-5, pitzDaily controls of the
+# reference's examples (relaxation 0.7 / 0.3 / 0.9, linear_linear velocity
+# interpolation, linear solves to rtol 1e-2)
+BFS_DIMS = (30, 210, 96, 112, 170)   # nx_up, nx_dn, ny_low, ny_high, nz: 7 996 800 elements
+BFS_MU, BFS_KIN = 2.0e-5, 1.5e-4
+BFS_WIN = BFS_KIN / (10.0 * BFS_MU)
+BFS_CTL, BFS_SCTL = [20, 1.0e-2, 200, 1.0e-2], [20, 1.0e-2]
+
+
+def bfs_setup(oa_or_oracle, mesh, case, device=True):
+    """Boundary types and values, SST and the initial state of the
+    backward-facing step on a FlowCase (device) or an oracle Case."""
+    from openaccel_amd import meshgen
+    case.set_bc_types([1, 2, 0, 3])   # inlet, outlet, no-slip walls, symmetry sides
+    case.set_bc_values(p=[0.0] * 4, k=[BFS_KIN, 0, 0, 0], w=[BFS_WIN, 0, 0, 0])
+    sst = oa_or_oracle.default_sst(urf_k=0.9, urf_w=0.9)
+    case.sst_enable(sst)
+    nn = mesh["xyz"].shape[0]
+    U = np.zeros((nn, 3))
+    U[:, 0] = 1.0
+    y = meshgen.bfs_wall_distance(mesh["xyz"])
+    w = np.full(nn, BFS_WIN)
+    # omega at no-slip wall nodes: 10 * 6 nu / (beta1 y1^2), y1 the wall cells' thickness
+    w[y < 1e-12] = 60.0 * BFS_MU / (0.075 * mesh["first_dy"] ** 2)
+    vals = {"U": U, "p": np.zeros(nn), "k": np.full(nn, BFS_KIN), "omega": w, "ymin": y}
+    for name, v in vals.items():
+        if device:
+            case.upload(name, v)
+        else:
+            case.field(name)[:] = v
+    case.initialize()
+
+
+def bfs_params(mod, precision=None):
+    kw = dict(mu=BFS_MU, urf_U=0.7, urf_p=0.3, u_shifted=1)
+    if precision is None:
+        return mod.default_params(**kw)
+    return mod.default_params(precision, p_method=mod.OAGPU_CG_AMG, **kw)
+
+
+def run_bfs(oa, dm, mesh, precision, steps, warmup):
+    bc = np.array([[1.0, 0.0, 0.0], [0.0] * 3, [0.0] * 3, [0.0] * 3])
+    f = oa.FlowCase(dm, bfs_params(oa, precision), bc)
+    bfs_setup(oa, mesh, f)
+    hist = []
+    for _ in range(warmup):
+        hist.append(np.concatenate([f.outer_iteration(BFS_CTL), f.sst_iteration(BFS_SCTL)]))
+    oa.load_library().oagpu_synchronize()
+    ms = {k: 0.0 for k in ("momentum", "pcorr", "mdot", "gradU")}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hist.append(np.concatenate([f.outer_iteration(BFS_CTL), f.sst_iteration(BFS_SCTL)]))
+        for k in ms:
+            ms[k] += f.kernel_ms(k) / steps
+    oa.load_library().oagpu_synchronize()
+    t = time.perf_counter() - t0
+    f.close()
+    return {"seconds": t, "kernel_ms": ms, "hist": np.array(hist)}
+
+
+def bfs_cpu_baseline(dims):
+    """Oracle on a bounded sample of the backward-facing step: the same
+    geometry and controls on a coarser grid, one flow + SST outer iteration."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from openaccel_amd import meshgen
+    mesh = meshgen.bfs_hex(*dims)
+    bc = np.array([[1.0, 0.0, 0.0], [0.0] * 3, [0.0] * 3, [0.0] * 3])
+    c = pyoracle.Case(mesh, bfs_params(pyoracle), bc)
+    bfs_setup(pyoracle, mesh, c, device=False)
+    t0 = time.perf_counter()
+    c.outer_iteration(BFS_CTL)
+    c.sst_iteration(BFS_SCTL)
+    return {"outer_s": time.perf_counter() - t0, "ne": c.ne}
+
+
+def bench_bfs(args):
+    import openaccel_amd as oa
+    from openaccel_amd import meshgen
+    oa.init(0)
+    s = args.bfs_scale
+    dims = tuple(max(2, int(round(d * s))) for d in BFS_DIMS)
+    mesh = meshgen.bfs_hex(*dims)
+    ne, nn = mesh["conn"].shape[0], mesh["xyz"].shape[0]
+    print(f"[bench] backward-facing step {dims}: {ne} elements, first dy {mesh['first_dy']:.2e}",
+          file=sys.stderr, flush=True)
+    dm = oa.DeviceMesh(mesh)
+    r64 = run_bfs(oa, dm, mesh, oa.OAGPU_FP64, args.steps, args.warmup)
+    print(f"[bench] fp64 {args.steps / r64['seconds']:.3f} it/s", file=sys.stderr, flush=True)
+    rmx = run_bfs(oa, dm, mesh, oa.OAGPU_MIXED, args.steps, args.warmup)
+    print(f"[bench] mixed {args.steps / rmx['seconds']:.3f} it/s", file=sys.stderr, flush=True)
+    it64, itmx = args.steps / r64["seconds"], args.steps / rmx["seconds"]
+    cpu = None
+    if not args.no_cpu:
+        cd = tuple(max(2, d // 3) for d in BFS_DIMS)
+        cb = bfs_cpu_baseline(cd)
+        scale = cb["ne"] / ne
+        cpu = {"value": round(scale / cb["outer_s"], 6), "unit": "outer iterations/s", "cores": 1,
+               "kind": "port",
+               "sample": f"oracle C restatement, single thread, 1 flow + SST outer iteration of the "
+                         f"same step at grid {cd} ({cb['ne']} elements, {cb['outer_s']:.2f} s) scaled "
+                         f"by the element ratio {scale:.5f}"}
+
+    def hist(r):
+        return {"flow_res_U_p": [[float(x) for x in h[:4]] for h in r["hist"]],
+                "sst_res_k_w": [[float(x) for x in h[6:8]] for h in r["hist"]],
+                "krylov_iters": [[int(h[4]), int(h[5]), int(h[8]), int(h[9])] for h in r["hist"]]}
+
+    line = {
+        "metric": METRIC,
+        "value": round(itmx, 4),
+        "unit": "outer iterations/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * rmx["seconds"] / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 local / f64 global", "data": "synthetic",
+        "config": {"workload": f"backward-facing step k-omega SST, channel 30 x 3 x 4, step 1, "
+                               f"{ne} hex8 ({nn} nodes), wall cells {mesh['first_dy']:.1e} graded "
+                               f"1.1; inlet U=1 (k {BFS_KIN}, omega {BFS_WIN}), outlet p=0, "
+                               f"symmetry sides, rho=1, mu={BFS_MU}; one step = flow outer "
+                               f"iteration + SST iteration; mixed form",
+                   "dims": dims, "elements": ne, "nodes": nn},
+        "fp64": {"value": round(it64, 4), "ms_per_step": round(1e3 * r64["seconds"] / args.steps, 3),
+                 "kernel_ms": {k: round(v, 4) for k, v in r64["kernel_ms"].items()}, **hist(r64)},
+        "mixed_over_fp64": round(itmx / it64, 3),
+        "kernel_ms": {k: round(v, 4) for k, v in rmx["kernel_ms"].items()},
+        **hist(rmx),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
 def cpu_baseline(n_sample, perturb):
     """Oracle (single-thread C restatement) on a bounded sample of the same
     workload: the same cavity at n_sample^3, one outer iteration timed after
@@ -119,6 +253,11 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=64, help="oracle sample grid (elements scale)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once (calibrates HBM byte counters)")
+    ap.add_argument("--workload", choices=("cavity", "bfs"), default="cavity",
+                    help="cavity: the lid-driven cavity (default); bfs: the 8M-element "
+                         "backward-facing step with k-omega SST")
+    ap.add_argument("--bfs-scale", type=float, default=1.0,
+                    help="scale of the step's cell counts per direction (1: 8M elements)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X only: the multi-GPU path (element partition + "
@@ -135,6 +274,10 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": nb}))
+        return
+
+    if args.workload == "bfs":
+        bench_bfs(args)
         return
 
     import openaccel_amd as oa

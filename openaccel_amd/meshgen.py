@@ -66,3 +66,98 @@ def initial_velocity(n_nodes: int, seed: int, sigma: float = 1.0e-3) -> np.ndarr
     """Seeded N(0, sigma) initial velocity [n_nodes][3]."""
     rng = np.random.default_rng(seed)
     return np.ascontiguousarray(rng.normal(0.0, sigma, size=(n_nodes, 3)))
+
+
+# boundary ids of the backward-facing-step channel
+BFS_INLET, BFS_OUTLET, BFS_WALL, BFS_SIDE = 0, 1, 2, 3
+
+
+def graded(a: float, b: float, n: int, ratio: float) -> np.ndarray:
+    """n cells on [a, b], refined towards both ends: cell sizes grow by
+    `ratio` from each end to the middle (n even; ratio 1: uniform)."""
+    if ratio == 1.0 or n < 2:
+        return np.linspace(a, b, n + 1)
+    half = n // 2
+    h = ratio ** np.arange(half)
+    sizes = np.concatenate([h, [ratio ** half] if n % 2 else [], h[::-1]])
+    x = np.concatenate([[0.0], np.cumsum(sizes)])
+    return a + (b - a) * x / x[-1]
+
+
+def bfs_hex(nx_up: int, nx_dn: int, ny_low: int, ny_high: int, nz: int, step: float = 1.0,
+            length_up: float = 5.0, length_dn: float = 25.0, height: float = 3.0,
+            width: float = 4.0, ratio: float = 1.1, ratio_z: float = 1.0):
+    """Backward-facing step: a channel of height `height` and width `width`
+    with a step of height `step` at x = 0; the inlet section x in
+    [-length_up, 0] spans y in [step, height], the section behind the step
+    x in [0, length_dn] the full height.  Structured hex8 blocks: nx_up and
+    nx_dn cells in x (uniform), ny_low cells in y below the step edge and
+    ny_high above it, nz cells in z, the y cells graded by `ratio` from each
+    wall (y = 0, step, height), the z cells by ratio_z from each side.
+
+    Boundary ids: BFS_INLET (x = -length_up), BFS_OUTLET (x = length_dn),
+    BFS_WALL (floor, step face, roof), BFS_SIDE (z = 0 and z = width).
+    Returns the mesh dict of cavity_hex plus "first_dy" (the wall cells'
+    thickness)."""
+    xs = np.concatenate([np.linspace(-length_up, 0.0, nx_up + 1),
+                         np.linspace(0.0, length_dn, nx_dn + 1)[1:]])
+    ys = np.concatenate([graded(0.0, step, ny_low, ratio),
+                         graded(step, height, ny_high, ratio)[1:]])
+    zs = graded(0.0, width, nz, ratio_z)
+    NX, NY, NZ = nx_up + nx_dn, ny_low + ny_high, nz
+    # cells of the box; the block below the step edge upstream is removed
+    k, j, i = np.meshgrid(np.arange(NZ), np.arange(NY), np.arange(NX), indexing="ij")
+    alive = ~((i < nx_up) & (j < ny_low))
+    # node (i, j, k) is used iff one of its up to 8 cells is alive
+    used = np.zeros((NZ + 1, NY + 1, NX + 1), dtype=bool)
+    for dk in (0, 1):
+        for dj in (0, 1):
+            for di in (0, 1):
+                used[dk:dk + NZ, dj:dj + NY, di:di + NX] |= alive
+    nid = np.full(used.shape, -1, dtype=np.int64)
+    nid[used] = np.arange(int(used.sum()))
+    kk, jj, ii = np.nonzero(used)  # lexicographic, x fastest
+    xyz = np.stack([xs[ii], ys[jj], zs[kk]], axis=1)
+    ci, cj, ck = i[alive], j[alive], k[alive]
+    corners = ((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1))
+    conn = np.stack([nid[ck + dk, cj + dj, ci + di] for di, dj, dk in corners], axis=1)
+    eid = np.full(alive.shape, -1, dtype=np.int64)
+    eid[alive] = np.arange(int(alive.sum()))
+    pad = np.zeros((NZ + 2, NY + 2, NX + 2), dtype=bool)
+    pad[1:-1, 1:-1, 1:-1] = alive
+    faces = []
+    # side ordinal -> neighbour offset (di, dj, dk)
+    for ordv, (di, dj, dk) in ((0, (0, -1, 0)), (1, (1, 0, 0)), (2, (0, 1, 0)),
+                               (3, (-1, 0, 0)), (4, (0, 0, -1)), (5, (0, 0, 1))):
+        nb = pad[1 + dk:1 + dk + NZ, 1 + dj:1 + dj + NY, 1 + di:1 + di + NX]
+        sel = alive & ~nb
+        e = eid[sel]
+        if ordv == 3:
+            bc = np.where(i[sel] == 0, BFS_INLET, BFS_WALL)
+        elif ordv == 1:
+            bc = np.full(e.shape, BFS_OUTLET)
+        elif ordv in (4, 5):
+            bc = np.full(e.shape, BFS_SIDE)
+        else:
+            bc = np.full(e.shape, BFS_WALL)
+        faces.append(np.stack([e, np.full_like(e, ordv), bc], axis=1))
+    bf = np.concatenate(faces).astype(np.int32)
+    return {
+        "xyz": np.ascontiguousarray(xyz),
+        "conn": np.ascontiguousarray(conn.astype(np.int32)),
+        "bf_elem": np.ascontiguousarray(bf[:, 0]),
+        "bf_ord": np.ascontiguousarray(bf[:, 1]),
+        "bf_bc": np.ascontiguousarray(bf[:, 2]),
+        "first_dy": float(min(ys[1] - ys[0], ys[ny_low + 1] - ys[ny_low])),
+    }
+
+
+def bfs_wall_distance(xyz: np.ndarray, step: float = 1.0, height: float = 3.0) -> np.ndarray:
+    """Distance to the nearest no-slip wall of bfs_hex (floor, step face,
+    roof; the sides are not walls)."""
+    x, y = xyz[:, 0], xyz[:, 1]
+    roof = height - y
+    floor = np.where(x < 0.0, y - step, y)
+    # the step face x = 0, y in [0, step], seen from downstream
+    face = np.where(x >= 0.0, np.hypot(x, np.maximum(y - step, 0.0)), np.inf)
+    return np.minimum(np.minimum(roof, floor), face)

@@ -2,7 +2,8 @@
 against the oracle on identical seeded inputs: a channel, the unit cube with
 a velocity inlet at x = 0 (id 0), a static-pressure outlet at x = 1 (id 1)
 and symmetry planes on the four other faces (id 2); one variant puts
-no-slip walls on y = 0 and y = 1 instead (id 3).
+no-slip walls on y = 0 and y = 1 instead (id 3); and a coarse
+backward-facing step with the k-omega SST model.
 
 Bars: side mass flows (initial and Rhie-Chow updated), reversal flags and
 mass divergence within 1e-12 relative / exact for the flags; FP64 momentum,
@@ -224,3 +225,51 @@ def test_systems_mixed(oracle, oa, meshgen):
     for N in (3, 1):
         assert rel_err(out[1, N][0], out[0, N][0]) < 2e-5
         assert rel_err(out[1, N][1], out[0, N][1]) < 2e-5
+
+
+def bfs_case(oracle, oa, meshgen, precision=0):
+    """A coarse backward-facing step (inlet, outlet, no-slip floor / step /
+    roof, symmetry sides) with the pitzDaily controls of the reference's
+    examples (velocity / pressure / turbulence relaxation 0.7 / 0.3 / 0.9,
+    linear_linear velocity interpolation)."""
+    m = meshgen.bfs_hex(3, 8, 4, 4, 3, ratio=1.2)
+    print(f"[case] backward-facing step {m['conn'].shape[0]} elements precision={precision}",
+          flush=True)
+    BC = np.array([[1.0, 0.0, 0.0], [0.0] * 3, [0.0] * 3, [0.0] * 3])
+    types = [INLET, OUTLET, WALL, SYM]
+    kin, win = 1.5e-4, 0.75
+    bcv = dict(p=[0.0] * 4, k=[kin, 0, 0, 0], w=[win, 0, 0, 0])
+    kw = dict(mu=1e-3, urf_U=0.7, urf_p=0.3, u_shifted=1)
+    c = oracle.Case(m, oracle.default_params(**kw), BC)
+    c.set_bc_types(types)
+    c.set_bc_values(**bcv)
+    c.sst_enable(oracle.default_sst(urf_k=0.9, urf_w=0.9))
+    dm = oa.DeviceMesh(m)
+    f = oa.FlowCase(dm, oa.default_params(precision, **kw), BC)
+    f.set_bc_types(types)
+    f.set_bc_values(**bcv)
+    f.sst_enable(oa.default_sst(urf_k=0.9, urf_w=0.9))
+    U = np.zeros((c.nn, 3))
+    U[:, 0] = 1.0
+    y = meshgen.bfs_wall_distance(m["xyz"])
+    init = {"U": U, "p": np.zeros(c.nn), "k": np.full(c.nn, kin), "omega": np.full(c.nn, win),
+            "ymin": y}
+    for name, v in init.items():
+        both(c, f, name, v)
+    c.initialize()
+    f.initialize()
+    return c, dm, f, m
+
+
+def test_bfs_sst_iterations(oracle, oa, meshgen):
+    c, dm, f, m = bfs_case(oracle, oa, meshgen)
+    ctl, sctl = [20, 1e-2, 200, 1e-2], [20, 1e-2]
+    for it in range(3):
+        # v and w start at zero: their first residuals are rounding (1e-20)
+        ho, hg = c.outer_iteration(ctl), f.outer_iteration(ctl)
+        np.testing.assert_allclose(hg[:4], ho[:4], rtol=1e-8, atol=1e-14)
+        so, sg = c.sst_iteration(sctl), f.sst_iteration(sctl)
+        np.testing.assert_allclose(sg[:2], so[:2], rtol=1e-8, atol=1e-14)
+    for name in ("U", "p", "k", "omega", "mut", "msd"):
+        assert rel_err(f.download(name), c.field(name)) < 1e-8, name
+    np.testing.assert_array_equal(f.reversal(), c.reversal())
