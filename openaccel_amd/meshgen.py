@@ -161,3 +161,140 @@ def bfs_wall_distance(xyz: np.ndarray, step: float = 1.0, height: float = 3.0) -
     # the step face x = 0, y in [0, step], seen from downstream
     face = np.where(x >= 0.0, np.hypot(x, np.maximum(y - step, 0.0)), np.inf)
     return np.minimum(np.minimum(roof, floor), face)
+
+
+# ------------------------------------------------------------------ mixed
+# element topology codes (the C ABI's OAGPU_HEX8 / TET4 / WED6 / PYR5)
+HEX8, TET4, WED6, PYR5 = 0, 1, 2, 3
+DUCT_INLET, DUCT_OUTLET, DUCT_WALL = 0, 1, 2
+
+# hex8 sides (outward, counter-clockwise seen from outside), as above
+_HEX_SIDES = np.array([[0, 1, 5, 4], [1, 2, 6, 5], [2, 3, 7, 6], [0, 4, 7, 3], [0, 3, 2, 1],
+                       [4, 5, 6, 7]])
+# cube corner offsets (di, dj, dk) of the hex8 node order
+_CORNERS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1],
+                     [0, 1, 1]])
+# side -> neighbour cube offset
+_SIDE_NB = np.array([[0, -1, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, 0, -1], [0, 0, 1]])
+
+
+def duct_mixed(m: int, layers=(187, 8, 25, 75, 25), length: float = 1.0, width: float = 0.2,
+               jitter: float = 0.1, seed: int = 0):
+    """Mixed-topology duct [0, length] x [0, width]^2 of cubes (m across,
+    sum(layers) along x with length / sum(layers) == width / m): along x the
+    cube layers are hex8 (inlet side), pyramids (each cube 6 pyramids about
+    a centre node), tets, wedges, tets and hex8 again (outlet side), layers =
+    (hex total, pyramid, tet, wedge, tet); the outer ring of cubes is hex8
+    everywhere, so every boundary face is a quad side of a hex8.
+
+    Tet cubes: each side split into two triangles (the diagonal through its
+    lowest node id, the same from both cubes sharing it), each triangle
+    joined to the cube's centre node; a side facing a hex8 / pyramid cube or
+    a wedge cube's quad side stays a quad and makes a pyramid instead.
+    Wedge cubes: two prisms along x, split by the diagonal that the lowest-id
+    rule gives on their x sides.  Node ordering per topology: tet4 base
+    (counter-clockwise seen from the apex) + apex, wedge6 bottom triangle +
+    top triangle, pyramid5 base + apex (positive Jacobians).
+
+    Interior grid nodes move by N(0, jitter*h) per component, clipped at
+    +-2.5 jitter*h; boundary nodes stay; centre nodes sit at their cube's
+    corner mean.  Boundary ids DUCT_INLET (x = 0), DUCT_OUTLET (x = length),
+    DUCT_WALL.  Returns xyz, conn [E][8] (-1 padded), etype, bf_elem,
+    bf_ord, bf_bc."""
+    H, P, T, W = 0, 3, 1, 2
+    nh, npyr, nt1, nw, nt2 = layers
+    nx, ny, nz = nh + npyr + nt1 + nw + nt2, m, m
+    h = width / m
+    assert abs(length / nx - h) < 1e-12 * h, "cubes: length / nx must equal width / m"
+    nh1 = nh // 2
+    xtype = np.array([H] * nh1 + [P] * npyr + [T] * nt1 + [W] * nw + [T] * nt2 + [H] * (nh - nh1))
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ctype = xtype[i]
+    ring = (j == 0) | (j == ny - 1) | (k == 0) | (k == nz - 1)
+    ctype = np.where(ring, H, ctype)
+    # grid nodes
+    g1 = nx + 1
+    gid = lambda ii, jj, kk: ii + g1 * (jj + (ny + 1) * kk)
+    kk_, jj_, ii_ = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    xyz = np.stack([ii_.ravel() * h, jj_.ravel() * h, kk_.ravel() * h], axis=1)
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        d = np.clip(rng.normal(0.0, jitter * h, size=xyz.shape), -2.5 * jitter * h, 2.5 * jitter * h)
+        on_b = ((ii_ == 0) | (ii_ == nx) | (jj_ == 0) | (jj_ == ny) | (kk_ == 0) | (kk_ == nz)).ravel()
+        d[on_b] = 0.0
+        xyz = xyz + d
+    ng = xyz.shape[0]
+    ci, cj, ck = i.ravel(), j.ravel(), k.ravel()
+    ct = ctype.ravel()
+    corner = np.stack([gid(ci + a, cj + b, ck + c) for a, b, c in _CORNERS], axis=1)  # [ncube][8]
+    # centre nodes of pyramid and tet cubes
+    has_c = (ct == P) | (ct == T)
+    cnode = np.full(ct.size, -1, dtype=np.int64)
+    cnode[has_c] = ng + np.arange(int(has_c.sum()))
+    xyz = np.concatenate([xyz, xyz[corner[has_c]].mean(axis=1)])
+    elems, types = [], []
+
+    def add(conn, t):
+        c = np.full((conn.shape[0], 8), -1, dtype=np.int64)
+        c[:, :conn.shape[1]] = conn
+        elems.append(c)
+        types.append(np.full(conn.shape[0], t, dtype=np.int32))
+
+    # hex cubes (their element ids come first, in cube order)
+    hsel = np.where(ct == H)[0]
+    add(corner[hsel], HEX8)
+    hex_eid = np.full(ct.size, -1, dtype=np.int64)
+    hex_eid[hsel] = np.arange(hsel.size)
+    # wedge cubes
+    ws = np.where(ct == W)[0]
+    cw = corner[ws]
+    add(cw[:, [0, 3, 7, 1, 2, 6]], WED6)
+    add(cw[:, [0, 7, 4, 1, 6, 5]], WED6)
+    # neighbour types per side (-1 outside)
+    padded = np.full((nz + 2, ny + 2, nx + 2), -1)
+    padded[1:-1, 1:-1, 1:-1] = ctype
+    nbt = np.stack([padded[1 + ck + dk, 1 + cj + dj, 1 + ci + di] for di, dj, dk in _SIDE_NB], axis=1)
+    for cube_t in (P, T):
+        sel = np.where(ct == cube_t)[0]
+        for side in range(6):
+            q = corner[sel][:, _HEX_SIDES[side]]          # outward quad
+            apex = cnode[sel]
+            if cube_t == T:
+                nb = nbt[sel, side]
+                tri = (nb == T) | ((nb == W) & ((side == 1) | (side == 3)))
+            else:
+                tri = np.zeros(sel.size, dtype=bool)
+            pq = q[~tri]
+            add(np.stack([pq[:, 3], pq[:, 2], pq[:, 1], pq[:, 0], apex[~tri]], axis=1), PYR5)
+            tq, ta = q[tri], apex[tri]
+            if tq.size:
+                first = np.argmin(tq, axis=1) % 2 == 0      # diagonal q0-q2, else q1-q3
+                t1 = np.where(first[:, None], tq[:, [0, 1, 2]], tq[:, [1, 2, 3]])
+                t2 = np.where(first[:, None], tq[:, [0, 2, 3]], tq[:, [1, 3, 0]])
+                for tr in (t1, t2):
+                    add(np.stack([tr[:, 0], tr[:, 2], tr[:, 1], ta], axis=1), TET4)
+    conn = np.concatenate(elems).astype(np.int32)
+    etype = np.concatenate(types)
+    # boundary faces: sides of hex cubes on the domain boundary
+    faces = []
+    for side, (di, dj, dk) in enumerate(_SIDE_NB):
+        out = nbt[:, side] == -1
+        e = hex_eid[out]
+        assert np.all(e >= 0)
+        if side == 3:
+            bc = np.full(e.shape, DUCT_INLET)
+        elif side == 1:
+            bc = np.full(e.shape, DUCT_OUTLET)
+        else:
+            bc = np.full(e.shape, DUCT_WALL)
+        faces.append(np.stack([e, np.full_like(e, side), bc], axis=1))
+    bf = np.concatenate(faces).astype(np.int32)
+    return {
+        "xyz": np.ascontiguousarray(xyz),
+        "conn": np.ascontiguousarray(conn),
+        "etype": np.ascontiguousarray(etype),
+        "bf_elem": np.ascontiguousarray(bf[:, 0]),
+        "bf_ord": np.ascontiguousarray(bf[:, 1]),
+        "bf_bc": np.ascontiguousarray(bf[:, 2]),
+        "h": h,
+    }

@@ -34,10 +34,16 @@ static void *xcalloc(size_t n, size_t s)
     return p;
 }
 
-or_case *or_case_create(int nn, int ne, const double *xyz, const int *conn, int nbf,
-                        const int *bf_elem, const int *bf_ord, const int *bf_bc,
-                        int nbc, const double *bc_U, const or_params *prm)
+or_case *or_case_create(int nn, int ne, const double *xyz, const int *conn,
+                        const int *etype, int nbf, const int *bf_elem, const int *bf_ord,
+                        const int *bf_bc, int nbc, const double *bc_U, const or_params *prm)
 {
+    if (etype) {
+        for (int e = 0; e < ne; ++e)
+            if (etype[e] < OR_HEX8 || etype[e] > OR_PYR5) return NULL;
+        for (int f = 0; f < nbf; ++f)
+            if (etype[bf_elem[f]] != OR_HEX8) return NULL; /* quad sides of hex8 only */
+    }
     or_case *c = (or_case *)xcalloc(1, sizeof(or_case));
     c->nn = nn;
     c->ne = ne;
@@ -48,6 +54,10 @@ or_case *or_case_create(int nn, int ne, const double *xyz, const int *conn, int 
     memcpy(c->xyz, xyz, sizeof(double) * 3 * (size_t)nn);
     c->conn = (int *)xcalloc((size_t)ne * 8, sizeof(int));
     memcpy(c->conn, conn, sizeof(int) * 8 * (size_t)ne);
+    if (etype) {
+        c->etype = (int *)xcalloc((size_t)ne, sizeof(int));
+        memcpy(c->etype, etype, sizeof(int) * (size_t)ne);
+    }
     c->bf_elem = (int *)xcalloc((size_t)nbf, sizeof(int));
     c->bf_ord = (int *)xcalloc((size_t)nbf, sizeof(int));
     c->bf_bc = (int *)xcalloc((size_t)nbf, sizeof(int));
@@ -60,7 +70,7 @@ or_case *or_case_create(int nn, int ne, const double *xyz, const int *conn, int 
     if (nbc) memcpy(c->bc_U, bc_U, sizeof(double) * 3 * (size_t)nbc);
 
     c->row_ptr = (int *)xcalloc((size_t)nn + 1, sizeof(int));
-    if (or_graph_build(nn, ne, conn, prm->reduced, c->row_ptr, &c->cols) != 0) {
+    if (or_graph_build(nn, ne, conn, etype, prm->reduced, c->row_ptr, &c->cols) != 0) {
         or_case_destroy(c);
         return NULL;
     }
@@ -122,7 +132,7 @@ or_case *or_case_create(int nn, int ne, const double *xyz, const int *conn, int 
 void or_case_destroy(or_case *c)
 {
     if (!c) return;
-    void *ptrs[] = {c->xyz, c->conn, c->bf_elem, c->bf_ord, c->bf_bc, c->bc_U,
+    void *ptrs[] = {c->xyz, c->conn, c->etype, c->bf_elem, c->bf_ord, c->bf_bc, c->bc_U,
                     c->row_ptr, c->cols, c->diag_off, c->A3, c->b3, c->x3,
                     c->A1, c->b1, c->x1, c->U, c->p, c->gradU, c->gradP, c->du,
                     c->mdot, c->div, c->vol, c->rho, c->mu, c->beta, c->bareav,
@@ -235,21 +245,24 @@ void or_dual_volume(or_case *c)
     memset(c->vol, 0, sizeof(double) * (size_t)c->nn);
     double co[24], v[8];
     for (int e = 0; e < c->ne; ++e) {
+        const int t = or_etype(c, e), npe = or_topo_npe(t);
         gather_coords(c, e, co);
-        or_hex_scv_vol(co, v);
-        for (int a = 0; a < 8; ++a) c->vol[c->conn[8 * e + a]] += v[a];
+        or_elem_scv(t, co, v);
+        for (int a = 0; a < npe; ++a) c->vol[c->conn[8 * e + a]] += v[a];
     }
 }
 
-/* SCV weights of the element-centre average (pressureCorrectionAssemblerElemTerms.cpp:346-368) */
-static void scv_weights(const double *co, double *w)
+/* SCV weights of the element-centre average (pressureCorrectionAssemblerElemTerms.cpp:346-368;
+ * the SCV ipNodeMap of every topology is the identity) */
+static void scv_weights(int topo, const double *co, double *w)
 {
+    const int npe = or_topo_npe(topo);
     double v[8], V = 0.0;
-    or_hex_scv_vol(co, v);
-    for (int k = 0; k < 8; ++k) V += v[k];
+    or_elem_scv(topo, co, v);
+    for (int k = 0; k < npe; ++k) V += v[k];
     const double inv = 1.0 / V;
-    for (int k = 0; k < 8; ++k) w[k] = 0.0;
-    for (int k = 0; k < 8; ++k) w[k] += v[k] * inv; /* hex SCV ipNodeMap is identity */
+    for (int k = 0; k < npe; ++k) w[k] = 0.0;
+    for (int k = 0; k < npe; ++k) w[k] += v[k] * inv;
 }
 
 /* ------------------------------------------------------------ mass flow */
@@ -257,22 +270,27 @@ static void mdot_init(or_case *c)
 {
     /* initializeMassFlowRateInterior_: shape functions shifted iff the
      * velocity interpolation is linear_linear (flowModel.cpp:3310-3316) */
-    double sf[96], co[24], av[36];
-    or_hex_scs_shape(c->prm.u_shifted, sf);
+    or_tables tab[4];
+    or_tables_init(tab);
+    double co[24], av[36];
     for (int e = 0; e < c->ne; ++e) {
+        const int t = or_etype(c, e);
+        const or_tables *T = &tab[t];
+        const int npe = T->npe;
+        const double *sf = T->sf[c->prm.u_shifted ? 1 : 0];
         const int *en = &c->conn[8 * e];
         gather_coords(c, e, co);
-        or_hex_scs_areav(co, av);
-        for (int ip = 0; ip < 12; ++ip) {
+        or_elem_areav(t, co, av);
+        for (int ip = 0; ip < T->nscs; ++ip) {
             double u[3] = {0, 0, 0}, rip = 0;
-            for (int ic = 0; ic < 8; ++ic) {
-                const double r = sf[8 * ip + ic];
+            for (int ic = 0; ic < npe; ++ic) {
+                const double r = sf[npe * ip + ic];
                 rip += r * c->rho[en[ic]];
                 for (int j = 0; j < 3; ++j) u[j] += r * c->U[3 * en[ic] + j];
             }
-            double t = 0;
-            for (int j = 0; j < 3; ++j) t += rip * u[j] * av[3 * ip + j];
-            c->mdot[12 * e + ip] = t;
+            double tt = 0;
+            for (int j = 0; j < 3; ++j) tt += rip * u[j] * av[3 * ip + j];
+            c->mdot[12 * e + ip] = tt;
         }
     }
     or_bc_mdot_init(c);
@@ -280,30 +298,33 @@ static void mdot_init(or_case *c)
 
 void or_update_mdot(or_case *c)
 {
-    const int *lr = or_hex_lrscv();
-    double sfu[96], sfc[96], co[24], av[36], dndx[288], detj[12], w[8];
-    or_hex_scs_shape(c->prm.u_shifted, sfu);
-    or_hex_scs_shape(0, sfc);
+    or_tables tab[4];
+    or_tables_init(tab);
+    double co[24], av[36], dndx[288], detj[12], w[8];
     const double urf = c->prm.urf_mdot;
     for (int e = 0; e < c->ne; ++e) {
+        const int t = or_etype(c, e);
+        const or_tables *T = &tab[t];
+        const int npe = T->npe, *lr = T->lr;
+        const double *sfu = T->sf[c->prm.u_shifted ? 1 : 0], *sfc = T->sf[0];
         const int *en = &c->conn[8 * e];
         gather_coords(c, e, co);
-        or_hex_scs_areav(co, av);
-        scv_weights(co, w);
-        or_hex_scs_gradop(c->prm.pgrad_shifted, co, dndx, detj);
+        or_elem_areav(t, co, av);
+        scv_weights(t, co, w);
+        or_elem_gradop(t, c->prm.pgrad_shifted, co, dndx, detj);
         double *md = &c->mdot[12 * e];
-        for (int ip = 0; ip < 12; ++ip) {
+        for (int ip = 0; ip < T->nscs; ++ip) {
             const int il = lr[2 * ip], ir = lr[2 * ip + 1];
             double xip[3] = {0, 0, 0}, uip[3] = {0, 0, 0}, Gp[3] = {0, 0, 0};
             double dp[3] = {0, 0, 0}, duip[3] = {0, 0, 0}, rip = 0;
-            for (int ic = 0; ic < 8; ++ic) {
+            for (int ic = 0; ic < npe; ++ic) {
                 const int n = en[ic];
-                const double rv = sfu[8 * ip + ic], rc = sfc[8 * ip + ic], ws = w[ic];
+                const double rv = sfu[npe * ip + ic], rc = sfc[npe * ip + ic], ws = w[ic];
                 rip += rv * c->rho[n];
                 for (int j = 0; j < 3; ++j) {
                     duip[j] += rv * c->du[3 * n + j];
                     uip[j] += rv * c->U[3 * n + j];
-                    dp[j] += dndx[24 * ip + 3 * ic + j] * c->p[n];
+                    dp[j] += dndx[3 * npe * ip + 3 * ic + j] * c->p[n];
                     xip[j] += rc * co[3 * ic + j];
                     Gp[j] += ws * c->gradP[3 * n + j];
                 }
@@ -312,12 +333,12 @@ void or_update_mdot(or_case *c)
             /* deferred correction of rho vanishes (blending factor 0 for
              * incompressible upwind); rhoHR = upwind rho */
             const double rhoHR = ur > 0 ? c->rho[en[il]] : c->rho[en[ir]];
-            double t = 0;
+            double tt = 0;
             for (int j = 0; j < 3; ++j) {
-                t += rhoHR * uip[j] * av[3 * ip + j];
-                t -= rhoHR * duip[j] * (dp[j] - Gp[j]) * av[3 * ip + j];
+                tt += rhoHR * uip[j] * av[3 * ip + j];
+                tt -= rhoHR * duip[j] * (dp[j] - Gp[j]) * av[3 * ip + j];
             }
-            md[ip] = urf * t + (1.0 - urf) * md[ip];
+            md[ip] = urf * tt + (1.0 - urf) * md[ip];
         }
     }
     or_bc_mdot_update(c);
@@ -325,11 +346,12 @@ void or_update_mdot(or_case *c)
 
 void or_mass_divergence(or_case *c)
 {
-    const int *lr = or_hex_lrscv();
     memset(c->div, 0, sizeof(double) * (size_t)c->nn);
     for (int e = 0; e < c->ne; ++e) {
+        const int t = or_etype(c, e);
         const int *en = &c->conn[8 * e];
-        for (int ip = 0; ip < 12; ++ip) {
+        const int *lr = t == OR_HEX8 ? or_hex_lrscv() : or_topo_lrscv(t);
+        for (int ip = 0; ip < or_topo_nscs(t); ++ip) {
             const double m = c->mdot[12 * e + ip];
             c->div[en[lr[2 * ip]]] += m;
             c->div[en[lr[2 * ip + 1]]] -= m;
@@ -389,21 +411,25 @@ void or_gradient(or_case *c, int which)
 void or_gradient_field(or_case *c, const double *phi, double *grad, int N, int shifted)
 {
     const double gurf = c->prm.grad_urf;
-    const int *lr = or_hex_lrscv();
     const int *sn = or_hex_side_nodes();
     /* relax the existing gradient (nodeField.hpp:3268, relax :6362) */
     for (size_t k = 0; k < (size_t)c->nn * 3 * N; ++k) grad[k] *= (1.0 - gurf);
-    double sf[96], co[24], av[36];
-    or_hex_scs_shape(shifted, sf);
+    or_tables tab[4];
+    or_tables_init(tab);
+    double co[24], av[36];
     for (int e = 0; e < c->ne; ++e) {
+        const int t = or_etype(c, e);
+        const or_tables *T = &tab[t];
+        const int npe = T->npe, *lr = T->lr;
+        const double *sf = T->sf[shifted ? 1 : 0];
         const int *en = &c->conn[8 * e];
         gather_coords(c, e, co);
-        or_hex_scs_areav(co, av);
-        for (int ip = 0; ip < 12; ++ip) {
+        or_elem_areav(t, co, av);
+        for (int ip = 0; ip < T->nscs; ++ip) {
             const int L = en[lr[2 * ip]], R = en[lr[2 * ip + 1]];
             double pip[3] = {0, 0, 0};
-            for (int ic = 0; ic < 8; ++ic)
-                for (int j = 0; j < N; ++j) pip[j] += sf[8 * ip + ic] * phi[N * en[ic] + j];
+            for (int ic = 0; ic < npe; ++ic)
+                for (int j = 0; j < N; ++j) pip[j] += sf[npe * ip + ic] * phi[N * en[ic] + j];
             const double ivL = 1.0 / c->vol[L], ivR = 1.0 / c->vol[R];
             for (int i = 0; i < N; ++i)
                 for (int j = 0; j < 3; ++j) {
@@ -463,31 +489,34 @@ void or_assemble_momentum(or_case *c)
     }
 
     /* interior element terms */
-    const int *lr = or_hex_lrscv();
-    double sfu[96], sfc[96], co[24], av[36], dndx[288], detj[12];
-    or_hex_scs_shape(P->u_shifted, sfu);
-    or_hex_scs_shape(0, sfc);
+    or_tables tab[4];
+    or_tables_init(tab);
+    double co[24], av[36], dndx[288], detj[12];
     for (int e = 0; e < c->ne; ++e) {
+        const int t = or_etype(c, e);
+        const or_tables *T = &tab[t];
+        const int npe = T->npe, *lr = T->lr, S = 3 * npe; /* local row stride */
+        const double *sfu = T->sf[P->u_shifted ? 1 : 0], *sfc = T->sf[0];
         const int *en = &c->conn[8 * e];
         memset(lhs, 0, sizeof(lhs));
         memset(rhs, 0, sizeof(rhs));
         double Uw[24], mu[8];
-        for (int a = 0; a < 8; ++a) {
+        for (int a = 0; a < npe; ++a) {
             nodes[a] = en[a];
             mu[a] = c->mu[en[a]];
             for (int d = 0; d < 3; ++d) Uw[3 * a + d] = c->U[3 * en[a] + d];
         }
         gather_coords(c, e, co);
-        or_hex_scs_areav(co, av);
-        or_hex_scs_gradop(P->ugrad_shifted, co, dndx, detj);
+        or_elem_areav(t, co, av);
+        or_elem_gradop(t, P->ugrad_shifted, co, dndx, detj);
         const double *md = &c->mdot[12 * e];
-        for (int ip = 0; ip < 12; ++ip) {
+        for (int ip = 0; ip < T->nscs; ++ip) {
             const int il = lr[2 * ip], ir = lr[2 * ip + 1];
             const double tm = md[ip];
             double muIp = 0, xip[3] = {0, 0, 0};
-            for (int ic = 0; ic < 8; ++ic) {
-                muIp += sfu[8 * ip + ic] * mu[ic];
-                for (int i = 0; i < 3; ++i) xip[i] += sfc[8 * ip + ic] * co[3 * ic + i];
+            for (int ic = 0; ic < npe; ++ic) {
+                muIp += sfu[npe * ip + ic] * mu[ic];
+                for (int i = 0; i < 3; ++i) xip[i] += sfc[npe * ip + ic] * co[3 * ic + i];
             }
             for (int i = 0; i < 3; ++i) {
                 double uup, dcorr = 0;
@@ -498,7 +527,7 @@ void or_assemble_momentum(or_case *c)
                              c->gradU[9 * en[iu] + 3 * i + j];
                 const double aflux = tm * (uup + dcorr);
                 const int iL = 3 * il + i, iR = 3 * ir + i;
-                const int rowL = iL * 24, rowR = iR * 24;
+                const int rowL = iL * S, rowR = iR * S;
                 rhs[iL] -= aflux;
                 rhs[iR] += aflux;
                 const double aL = 0.5 * (tm + fabs(tm)), aR = 0.5 * (tm - fabs(tm));
@@ -507,12 +536,12 @@ void or_assemble_momentum(or_case *c)
                 lhs[rowR + 3 * ir + i] -= aR;
                 lhs[rowL + 3 * ir + i] += aR;
             }
-            for (int ic = 0; ic < 8; ++ic) {
+            for (int ic = 0; ic < npe; ++ic) {
                 const int icN = 3 * ic;
                 for (int i = 0; i < 3; ++i) {
                     const int iL = 3 * il + i, iR = 3 * ir + i;
-                    const int rowL = iL * 24, rowR = iR * 24;
-                    const double *dn = &dndx[24 * ip + icN];
+                    const int rowL = iL * S, rowR = iR * S;
+                    const double *dn = &dndx[S * ip + icN];
                     double lhs_riC_i = 0;
                     for (int j = 0; j < 3; ++j) {
                         const double axj = av[3 * ip + j];
@@ -532,7 +561,7 @@ void or_assemble_momentum(or_case *c)
                 }
             }
         }
-        or_apply_coeff(3, 8, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
+        or_apply_coeff(3, npe, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
     }
 
     /* no-slip walls (navierStokesAssemblerElemBoundaryConditions.cpp:2659-2884)
@@ -698,42 +727,46 @@ void or_assemble_pcorr(or_case *c)
     const or_params *P = &c->prm;
     memset(c->A1, 0, sizeof(double) * (size_t)c->nnz);
     memset(c->b1, 0, sizeof(double) * (size_t)nn);
-    const int *lr = or_hex_lrscv();
-    double sfu[96], sfc[96], co[24], av[36], dndx[288], detj[12], w[8];
+    or_tables tab[4];
+    or_tables_init(tab);
+    double co[24], av[36], dndx[288], detj[12], w[8];
     double lhs[64], rhs[8];
     int nodes[8];
-    or_hex_scs_shape(P->u_shifted, sfu);
-    or_hex_scs_shape(0, sfc);
     for (int e = 0; e < c->ne; ++e) {
+        const int t = or_etype(c, e);
+        const or_tables *T = &tab[t];
+        const int npe = T->npe, *lr = T->lr;
+        const double *sfu = T->sf[P->u_shifted ? 1 : 0];
         const int *en = &c->conn[8 * e];
         memset(lhs, 0, sizeof(lhs));
         memset(rhs, 0, sizeof(rhs));
-        for (int a = 0; a < 8; ++a) nodes[a] = en[a];
+        for (int a = 0; a < npe; ++a) nodes[a] = en[a];
         gather_coords(c, e, co);
-        or_hex_scs_areav(co, av);
-        scv_weights(co, w);
-        or_hex_scs_gradop(P->pgrad_shifted, co, dndx, detj);
+        or_elem_areav(t, co, av);
+        scv_weights(t, co, w);
+        or_elem_gradop(t, P->pgrad_shifted, co, dndx, detj);
         const double *md = &c->mdot[12 * e];
-        for (int ip = 0; ip < 12; ++ip) {
+        for (int ip = 0; ip < T->nscs; ++ip) {
             const int il = lr[2 * ip], ir = lr[2 * ip + 1];
             const double tm = md[ip];
-            const int rowL = il * 8, rowR = ir * 8;
+            const int rowL = il * npe, rowR = ir * npe;
+            const double *dn = &dndx[3 * npe * ip];
             double uip[3] = {0, 0, 0}, duip[3] = {0, 0, 0}, dp[3] = {0, 0, 0}, Gp[3] = {0, 0, 0};
-            for (int ic = 0; ic < 8; ++ic) {
+            for (int ic = 0; ic < npe; ++ic) {
                 const int n = en[ic];
-                const double rv = sfu[8 * ip + ic], ws = w[ic];
+                const double rv = sfu[npe * ip + ic], ws = w[ic];
                 for (int j = 0; j < 3; ++j) {
                     uip[j] += rv * c->U[3 * n + j];
                     duip[j] += rv * c->du[3 * n + j];
-                    dp[j] += dndx[24 * ip + 3 * ic + j] * c->p[n];
+                    dp[j] += dn[3 * ic + j] * c->p[n];
                     Gp[j] += ws * c->gradP[3 * n + j];
                 }
             }
             const double rhoHR = tm > 0 ? c->rho[en[il]] : c->rho[en[ir]];
-            for (int ic = 0; ic < 8; ++ic) {
+            for (int ic = 0; ic < npe; ++ic) {
                 double lf = 0;
                 for (int j = 0; j < 3; ++j)
-                    lf += -rhoHR * duip[j] * dndx[24 * ip + 3 * ic + j] * av[3 * ip + j];
+                    lf += -rhoHR * duip[j] * dn[3 * ic + j] * av[3 * ip + j];
                 lhs[rowL + ic] += lf;
                 lhs[rowR + ic] -= lf;
             }
@@ -745,7 +778,7 @@ void or_assemble_pcorr(or_case *c)
             rhs[il] -= m;
             rhs[ir] += m;
         }
-        or_apply_coeff(1, 8, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A1, c->b1);
+        or_apply_coeff(1, npe, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A1, c->b1);
     }
     /* no-slip static walls and symmetry planes add nothing to p' (no
      * frame/mesh motion); inlets and outlets do */
@@ -880,10 +913,12 @@ void or_update_blending(or_case *c, int which)
     const size_t n = (size_t)c->nn * N;
     double *mn = (double *)xcalloc(n, sizeof(double)), *mx = (double *)xcalloc(n, sizeof(double));
     for (size_t k = 0; k < n; ++k) mn[k] = mx[k] = phi[k];
-    const int *lr = or_hex_lrscv();
+    or_tables tab[4];
+    or_tables_init(tab);
     for (int e = 0; e < c->ne; ++e) {
-        const int *en = &c->conn[8 * e];
-        for (int ip = 0; ip < 12; ++ip) {
+        const or_tables *T = &tab[or_etype(c, e)];
+        const int *en = &c->conn[8 * e], *lr = T->lr;
+        for (int ip = 0; ip < T->nscs; ++ip) {
             const int L = en[lr[2 * ip]], R = en[lr[2 * ip + 1]];
             for (int i = 0; i < N; ++i) {
                 mn[N * L + i] = fmin(mn[N * L + i], phi[N * R + i]);
@@ -894,15 +929,17 @@ void or_update_blending(or_case *c, int which)
         }
     }
     for (size_t k = 0; k < n; ++k) beta[k] = 1.0; /* blendingFactorMax_ */
-    double sfc[96], co[24];
-    or_hex_scs_shape(0, sfc); /* coordinates: standard shape functions */
+    double co[24];
     for (int e = 0; e < c->ne; ++e) {
+        const or_tables *T = &tab[or_etype(c, e)];
+        const int npe = T->npe, *lr = T->lr;
+        const double *sfc = T->sf[0]; /* coordinates: standard shape functions */
         const int *en = &c->conn[8 * e];
         gather_coords(c, e, co);
-        for (int ip = 0; ip < 12; ++ip) {
+        for (int ip = 0; ip < T->nscs; ++ip) {
             double xip[3] = {0, 0, 0};
-            for (int ic = 0; ic < 8; ++ic)
-                for (int j = 0; j < 3; ++j) xip[j] += sfc[8 * ip + ic] * co[3 * ic + j];
+            for (int ic = 0; ic < npe; ++ic)
+                for (int j = 0; j < 3; ++j) xip[j] += sfc[npe * ip + ic] * co[3 * ic + j];
             for (int side = 0; side < 2; ++side) {
                 const int a = lr[2 * ip + side], node = en[a];
                 double dx[3];

@@ -12,7 +12,8 @@
 struct or_case {
     int nn, ne, nbf, nbc;
     double *xyz;
-    int *conn;
+    int *conn;      /* [ne][8], unused trailing entries -1 */
+    int *etype;     /* topology per element (OR_*), NULL: every element hex8 */
     int *bf_elem, *bf_ord, *bf_bc;
     double *bc_U;
     or_params prm;
@@ -40,12 +41,26 @@ struct or_case {
     int scalar_terms;   /* or_set_scalar_terms mask (test hook), default all */
 };
 
+static inline int or_etype(const or_case *c, int e) { return c->etype ? c->etype[e] : OR_HEX8; }
+
 static inline void or_gather_coords(const or_case *c, int e, double *co)
 {
     const int *en = &c->conn[8 * e];
-    for (int a = 0; a < 8; ++a)
+    const int npe = or_topo_npe(or_etype(c, e));
+    for (int a = 0; a < npe; ++a)
         for (int d = 0; d < 3; ++d) co[3 * a + d] = c->xyz[3 * en[a] + d];
 }
+
+/* master-element tables of one topology (elem.c); sized for hex8 */
+typedef struct {
+    int npe, nscs;
+    const int *lr;       /* [nscs][2] */
+    double sf[2][96];    /* SCS shape functions [nscs][npe], standard / shifted ips */
+} or_tables;
+void or_tables_init(or_tables tab[4]);
+void or_elem_areav(int topo, const double *co, double *av);
+int or_elem_gradop(int topo, int shifted, const double *co, double *g, double *detj);
+void or_elem_scv(int topo, const double *co, double *v);
 
 /* Green-Gauss gradient of an N-component nodal field (case.c) */
 void or_gradient_field(or_case *c, const double *phi, double *grad, int N, int shifted);

@@ -17,27 +17,27 @@ static int cmp_int(const void *a, const void *b)
     return (x > y) - (x < y);
 }
 
-int or_graph_build(int nn, int ne, const int *conn, int reduced, int *row_ptr,
-                   int **cols_out)
+int or_graph_build(int nn, int ne, const int *conn, const int *etype, int reduced,
+                   int *row_ptr, int **cols_out)
 {
     /* candidate (row, col) insertions, counted then filled; std::set per row
-     * in the reference (nodeGraph.cpp:36) == sort + unique per row here */
-    const int *lr = or_hex_lrscv();
-    const int per_elem = reduced ? 12 * 4 : 64;
+     * in the reference (nodeGraph.cpp:36) == sort + unique per row here.
+     * Elements of any topology: conn [ne][8], npe leading entries used. */
     int *cnt = (int *)calloc((size_t)nn + 1, sizeof(int));
     if (!cnt) return -1;
     for (int e = 0; e < ne; ++e) {
-        const int *en = &conn[8 * e];
+        const int t = etype ? etype[e] : OR_HEX8;
+        const int *en = &conn[8 * e], npe = or_topo_npe(t), nscs = or_topo_nscs(t);
+        const int *lr = t == OR_HEX8 ? or_hex_lrscv() : or_topo_lrscv(t);
         if (reduced) {
-            for (int ip = 0; ip < 12; ++ip) {
+            for (int ip = 0; ip < nscs; ++ip) {
                 cnt[en[lr[2 * ip]]] += 2;
                 cnt[en[lr[2 * ip + 1]]] += 2;
             }
         } else {
-            for (int a = 0; a < 8; ++a) cnt[en[a]] += 8;
+            for (int a = 0; a < npe; ++a) cnt[en[a]] += npe;
         }
     }
-    (void)per_elem;
     size_t *start = (size_t *)malloc(((size_t)nn + 1) * sizeof(size_t));
     if (!start) { free(cnt); return -1; }
     start[0] = 0;
@@ -46,9 +46,11 @@ int or_graph_build(int nn, int ne, const int *conn, int reduced, int *row_ptr,
     if (!cand) { free(cnt); free(start); return -1; }
     memset(cnt, 0, sizeof(int) * ((size_t)nn + 1));
     for (int e = 0; e < ne; ++e) {
-        const int *en = &conn[8 * e];
+        const int t = etype ? etype[e] : OR_HEX8;
+        const int *en = &conn[8 * e], npe = or_topo_npe(t), nscs = or_topo_nscs(t);
+        const int *lr = t == OR_HEX8 ? or_hex_lrscv() : or_topo_lrscv(t);
         if (reduced) {
-            for (int ip = 0; ip < 12; ++ip) {
+            for (int ip = 0; ip < nscs; ++ip) {
                 const int L = en[lr[2 * ip]], R = en[lr[2 * ip + 1]];
                 cand[start[L] + cnt[L]++] = L;
                 cand[start[L] + cnt[L]++] = R;
@@ -56,9 +58,9 @@ int or_graph_build(int nn, int ne, const int *conn, int reduced, int *row_ptr,
                 cand[start[R] + cnt[R]++] = R;
             }
         } else {
-            for (int a = 0; a < 8; ++a) {
+            for (int a = 0; a < npe; ++a) {
                 const int r = en[a];
-                for (int c = 0; c < 8; ++c) cand[start[r] + cnt[r]++] = en[c];
+                for (int c = 0; c < npe; ++c) cand[start[r] + cnt[r]++] = en[c];
             }
         }
     }

@@ -77,13 +77,16 @@ int or_topo_gradop(int npe, int npts, const double *deriv, const double *coords,
 /* SCS gradient operator [nscs][npe][3] at the standard or shifted ips */
 int or_topo_scs_gradop(int topo, int shifted, const double *coords, double *gradop,
                        double *detj);
+/* tet_shape_fcn, wedge_shape_fcn, pyr_shape_fcn / shifted_pyr_shape_fcn at
+ * npts parametric points: shape [npts][npe] (elem.c) */
+void or_topo_shape(int topo, int shifted, int npts, const double *par, double *shape);
 
 /* ------------------------------------------------------------- CRS graph */
 /* nodeGraph::buildGraph_ (mesh/graph/nodeGraph.cpp:29-392): reduced stencil
  * (SCS-ip pairs, :68-96) when reduced != 0, else full element stencil
  * (:98-118).  Owned rows 0..nn-1, sorted local columns.  row_ptr[nn+1] must
  * be allocated by the caller; cols is allocated here (free with or_free). */
-int or_graph_build(int nn, int ne, const int *conn, int reduced,
+int or_graph_build(int nn, int ne, const int *conn, const int *etype, int reduced,
                    int *row_ptr, int **cols_out);
 void or_free(void *p);
 /* position of `col` in row `row`, or -1 (the sorted search of
@@ -123,8 +126,10 @@ typedef struct or_case or_case;
 
 /* Build the oracle state for a hex8 mesh.  Boundary faces: parent element,
  * face ordinal and boundary id; bc_U[nbc][3] wall velocities. */
+/* conn [ne][8] (unused trailing entries -1), etype [ne] topology per
+ * element (NULL: all hex8); boundary faces are quad sides of hex8 elements */
 or_case *or_case_create(int nn, int ne, const double *xyz, const int *conn,
-                        int nbf, const int *bf_elem, const int *bf_ord,
+                        const int *etype, int nbf, const int *bf_elem, const int *bf_ord,
                         const int *bf_bc, int nbc, const double *bc_U,
                         const or_params *prm);
 void or_case_destroy(or_case *c);

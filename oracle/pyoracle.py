@@ -62,7 +62,7 @@ def lib():
             raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
         L = C.CDLL(LIB_PATH)
         L.or_case_create.restype = C.c_void_p
-        L.or_case_create.argtypes = [C.c_int, C.c_int, _dp, _ip, C.c_int, _ip, _ip, _ip,
+        L.or_case_create.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, C.c_int, _ip, _ip, _ip,
                                      C.c_int, _dp, C.POINTER(Params)]
         L.or_case_destroy.argtypes = [C.c_void_p]
         for f in ("or_case_nn", "or_case_nnz"):
@@ -90,7 +90,8 @@ def lib():
         L.or_solve.restype = C.c_int
         L.or_scaled_residual.argtypes = [C.c_void_p, C.c_int, _dp]
         L.or_outer_iteration.argtypes = [C.c_void_p, _dp, _dp]
-        L.or_graph_build.argtypes = [C.c_int, C.c_int, _ip, C.c_int, _ip, C.POINTER(_ip)]
+        L.or_graph_build.argtypes = [C.c_int, C.c_int, _ip, _ip, C.c_int, _ip, C.POINTER(_ip)]
+        L.or_topo_shape.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.or_graph_build.restype = C.c_int
         L.or_free.argtypes = [C.c_void_p]
         L.or_graph_find.argtypes = [_ip, _ip, C.c_int, C.c_int]
@@ -162,9 +163,12 @@ class Case:
         self.ne = mesh["conn"].shape[0]
         bc = np.ascontiguousarray(bc_U, dtype=np.float64)
         self._nbc = bc.shape[0]
+        et = mesh.get("etype")
+        et = None if et is None else np.ascontiguousarray(et, dtype=np.int32)
         self._keep = [mesh["xyz"], mesh["conn"], mesh["bf_elem"], mesh["bf_ord"],
-                      mesh["bf_bc"], bc]
+                      mesh["bf_bc"], bc, et]
         self.h = L.or_case_create(self.nn, self.ne, dptr(mesh["xyz"]), iptr(mesh["conn"]),
+                                  None if et is None else iptr(et),
                                   mesh["bf_elem"].size, iptr(mesh["bf_elem"]),
                                   iptr(mesh["bf_ord"]), iptr(mesh["bf_bc"]),
                                   bc.shape[0], dptr(bc), C.byref(params))

@@ -175,18 +175,22 @@ void or_assemble_scalar(or_case *c, int which)
         or_apply_coeff(1, 1, &i, &l, &r, nn, c->row_ptr, c->cols, c->diag_off, c->A1, c->b1);
     }
     /* diffusivity Gamma = mu + sigma(F1) mut (computeGamma_ / setup lambdas) */
-    const int *lr = or_hex_lrscv();
-    double sf[96], sfc[96], co[24], av[36], dndx[288], detj[12], lhs[64], rhs[8];
-    or_hex_scs_shape(shifted, sf);
-    or_hex_scs_shape(0, sfc); /* coordinates: standard shape functions */
+    or_tables tab[4];
+    or_tables_init(tab);
+    double co[24], av[36], dndx[288], detj[12], lhs[64], rhs[8];
     /* high-resolution advection: limited deferred correction (blending
      * factor beta_k / beta_w, phiAssemblerElemTerms.hpp:291-340) */
     const double *beta = c->prm.high_res ? (which == 0 ? c->beta_k : c->beta_w) : NULL;
     const double *gphi = which == 0 ? c->gradK : c->gradW;
     for (int e = 0; e < ((c->scalar_terms & 4) ? c->ne : 0); ++e) {
+        const int t = or_etype(c, e);
+        const or_tables *T = &tab[t];
+        const int npe = T->npe, *lr = T->lr;
+        const double *sf = T->sf[shifted ? 1 : 0];
+        const double *sfc = T->sf[0]; /* coordinates: standard shape functions */
         const int *en = &c->conn[8 * e];
         double phi[8], G[8];
-        for (int a = 0; a < 8; ++a) {
+        for (int a = 0; a < npe; ++a) {
             const int n = en[a];
             phi[a] = phiF[n];
             const double F1 = c->F1[n];
@@ -195,20 +199,21 @@ void or_assemble_scalar(or_case *c, int which)
         memset(lhs, 0, sizeof(lhs));
         memset(rhs, 0, sizeof(rhs));
         or_gather_coords(c, e, co);
-        or_hex_scs_areav(co, av);
-        or_hex_scs_gradop(gshift, co, dndx, detj);
-        for (int ip = 0; ip < 12; ++ip) {
+        or_elem_areav(t, co, av);
+        or_elem_gradop(t, gshift, co, dndx, detj);
+        for (int ip = 0; ip < T->nscs; ++ip) {
             const int il = lr[2 * ip], ir = lr[2 * ip + 1];
             const double tm = c->mdot[12 * e + ip];
+            const double *dn = &dndx[3 * npe * ip];
             double Gip = 0.0;
-            for (int ic = 0; ic < 8; ++ic) Gip += sf[8 * ip + ic] * G[ic];
+            for (int ic = 0; ic < npe; ++ic) Gip += sf[npe * ip + ic] * G[ic];
             /* upwind advection (+ deferred correction) */
             const int iu = tm > 0 ? il : ir;
             double dcorr = 0.0;
             if (beta) {
                 double xip[3] = {0, 0, 0};
-                for (int ic = 0; ic < 8; ++ic)
-                    for (int j = 0; j < 3; ++j) xip[j] += sfc[8 * ip + ic] * co[3 * ic + j];
+                for (int ic = 0; ic < npe; ++ic)
+                    for (int j = 0; j < 3; ++j) xip[j] += sfc[npe * ip + ic] * co[3 * ic + j];
                 for (int j = 0; j < 3; ++j)
                     dcorr += beta[en[iu]] * (xip[j] - co[3 * iu + j]) * gphi[3 * en[iu] + j];
             }
@@ -216,21 +221,21 @@ void or_assemble_scalar(or_case *c, int which)
             rhs[il] -= aflux;
             rhs[ir] += aflux;
             const double aL = 0.5 * (tm + fabs(tm)), aR = 0.5 * (tm - fabs(tm));
-            lhs[8 * il + il] += aL;
-            lhs[8 * ir + il] -= aL;
-            lhs[8 * ir + ir] -= aR;
-            lhs[8 * il + ir] += aR;
+            lhs[npe * il + il] += aL;
+            lhs[npe * ir + il] -= aL;
+            lhs[npe * ir + ir] -= aR;
+            lhs[npe * il + ir] += aR;
             /* diffusion */
-            for (int ic = 0; ic < 8; ++ic) {
+            for (int ic = 0; ic < npe; ++ic) {
                 double l = 0.0;
-                for (int j = 0; j < 3; ++j) l += -Gip * dndx[24 * ip + 3 * ic + j] * av[3 * ip + j];
-                lhs[8 * il + ic] += l;
-                lhs[8 * ir + ic] -= l;
+                for (int j = 0; j < 3; ++j) l += -Gip * dn[3 * ic + j] * av[3 * ip + j];
+                lhs[npe * il + ic] += l;
+                lhs[npe * ir + ic] -= l;
                 rhs[il] -= l * phi[ic];
                 rhs[ir] += l * phi[ic];
             }
         }
-        or_apply_coeff(1, 8, en, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A1, c->b1);
+        or_apply_coeff(1, npe, en, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A1, c->b1);
     }
     /* inlets (fixed value) and outlets (zero gradient); nothing at walls (k
      * zero-gradient, omega rows fixed below) or symmetry planes */

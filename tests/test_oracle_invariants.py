@@ -44,7 +44,7 @@ def test_graph_closed_form(oracle, meshgen, n):
     for reduced, expect in ((1, V + 6 * n * (n + 1) ** 2), (0, (3 * n + 1) ** 3)):
         rp = np.zeros(V + 1, np.int32)
         cols = oracle._ip()
-        assert L.or_graph_build(V, n ** 3, oracle.iptr(m["conn"]), reduced, oracle.iptr(rp),
+        assert L.or_graph_build(V, n ** 3, oracle.iptr(m["conn"]), None, reduced, oracle.iptr(rp),
                                 oracle.C.byref(cols)) == 0
         nnz = rp[-1]
         c = np.ctypeslib.as_array(cols, shape=(nnz,)).copy()
@@ -64,7 +64,7 @@ def test_apply_coeff_lumping_preserves_row_sums(oracle, meshgen):
     for reduced in (1, 0):
         rp = np.zeros(V + 1, np.int32)
         cp = oracle._ip()
-        L.or_graph_build(V, E, oracle.iptr(m["conn"]), reduced, oracle.iptr(rp), oracle.C.byref(cp))
+        L.or_graph_build(V, E, oracle.iptr(m["conn"]), None, reduced, oracle.iptr(rp), oracle.C.byref(cp))
         cols = np.ctypeslib.as_array(cp, shape=(rp[-1],)).copy()
         L.or_free(cp)
         doff = np.array([np.searchsorted(cols[rp[i]:rp[i + 1]], i) for i in range(V)], np.int32)
