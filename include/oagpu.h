@@ -112,15 +112,19 @@ const char *oagpu_last_error(void);
 int oagpu_synchronize(void);
 
 /* --------------------------------------------------------------- mesh */
-/* Device mirror of a hex8 mesh: SoA-free interleaved coordinates, element
- * connectivity in the nalu hex8 order, boundary sides (parent element, side
- * ordinal 0..5, boundary id).  Builds the CRS node graph
+/* Device mirror of a mesh: interleaved coordinates [n_nodes][3], element
+ * connectivity conn [n_elems][8] in the nalu node order of each element's
+ * topology (etype [n_elems]: OAGPU_HEX8 / TET4 / WED6 / PYR5, the unused
+ * trailing entries of smaller elements ignored; etype NULL: all hex8, the
+ * element buckets of the reference's blocks flattened into one list),
+ * boundary sides (parent element, side ordinal 0..5, boundary id), which
+ * must be quad sides of hex8 elements.  Builds the CRS node graph
  * (nodeGraph::buildGraph_, src/mesh/graph/nodeGraph.cpp:29-392) with the
  * given stencil, the node->element incidences and element->CRS scatter maps,
  * the dual nodal volumes (mesh::updateDualNodalVolumeField_,
  * src/mesh/meshGeometry.cpp:1277) and per-element SCV weights. */
 int oagpu_mesh_create(int n_nodes, const double *xyz, int n_elems, const int *conn,
-                      int n_bfaces, const int *bf_elem, const int *bf_ord,
+                      const int *etype, int n_bfaces, const int *bf_elem, const int *bf_ord,
                       const int *bf_bc, int stencil, oagpu_mesh **out);
 int oagpu_mesh_destroy(oagpu_mesh *m);
 int oagpu_mesh_sizes(const oagpu_mesh *m, int *n_nodes, int *n_elems, int *nnz,

@@ -81,7 +81,7 @@ _SIGS = {
     "oagpu_init": [C.c_int],
     "oagpu_last_error": [],
     "oagpu_synchronize": [],
-    "oagpu_mesh_create": [C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, _ip, _ip, C.c_int,
+    "oagpu_mesh_create": [C.c_int, _dp, C.c_int, _ip, _ip, C.c_int, _ip, _ip, _ip, C.c_int,
                           C.POINTER(C.c_void_p)],
     "oagpu_mesh_destroy": [C.c_void_p],
     "oagpu_mesh_sizes": [C.c_void_p, _ip, _ip, _ip, _ip, _ip],
@@ -191,7 +191,8 @@ def default_params(precision: int = OAGPU_FP64, **kw) -> FlowParams:
 
 
 class DeviceMesh:
-    """Device mirror of a hex8 mesh (oagpu_mesh_create)."""
+    """Device mirror of a mesh (oagpu_mesh_create); mesh["etype"] gives the
+    element topologies of a mixed mesh (absent: all hex8)."""
 
     def __init__(self, mesh: dict, stencil: int = OAGPU_STENCIL_REDUCED):
         L = load_library()
@@ -200,8 +201,11 @@ class DeviceMesh:
         bfe = np.ascontiguousarray(mesh["bf_elem"], dtype=np.int32)
         bfo = np.ascontiguousarray(mesh["bf_ord"], dtype=np.int32)
         bfb = np.ascontiguousarray(mesh["bf_bc"], dtype=np.int32)
+        et = mesh.get("etype")
+        self._etype = None if et is None else np.ascontiguousarray(et, dtype=np.int32)
         h = C.c_void_p()
-        _check(L.oagpu_mesh_create(xyz.shape[0], _d(xyz), conn.shape[0], _i(conn), bfe.size,
+        _check(L.oagpu_mesh_create(xyz.shape[0], _d(xyz), conn.shape[0], _i(conn),
+                                   None if self._etype is None else _i(self._etype), bfe.size,
                                    _i(bfe), _i(bfo), _i(bfb), stencil, C.byref(h)))
         self.h = h
         self.nbf = bfe.size

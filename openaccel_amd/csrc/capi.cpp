@@ -113,9 +113,9 @@ int oagpu_synchronize(void) {
     return guarded([&] { OAGPU_HIP(hipDeviceSynchronize()); });
 }
 
-int oagpu_mesh_create(int n_nodes, const double* xyz, int n_elems, const int* conn, int n_bfaces,
-                      const int* bf_elem, const int* bf_ord, const int* bf_bc, int stencil,
-                      oagpu_mesh** out) {
+int oagpu_mesh_create(int n_nodes, const double* xyz, int n_elems, const int* conn,
+                      const int* etype, int n_bfaces, const int* bf_elem, const int* bf_ord,
+                      const int* bf_bc, int stencil, oagpu_mesh** out) {
     return guarded([&] {
         OAGPU_REQUIRE(out, "null output handle");
         OAGPU_REQUIRE(xyz && conn, "null mesh arrays");
@@ -128,6 +128,7 @@ int oagpu_mesh_create(int n_nodes, const double* xyz, int n_elems, const int* co
         hm.nbf = n_bfaces;
         hm.xyz = xyz;
         hm.conn = conn;
+        hm.etype = etype;
         hm.bf_elem = bf_elem;
         hm.bf_ord = bf_ord;
         hm.bf_bc = bf_bc;

@@ -3,17 +3,24 @@
 // element-centre weights), mass flow rate initialisation and Rhie-Chow
 // update, mass divergence, corrections, field scales and control residuals.
 #include "device_util.hpp"
+#include "topo_tables.hpp"
 
 namespace oagpu {
 
 using namespace hex8;
 
 // ---------------------------------------------------------------- geometry
-// SCV volumes of every element (hex_scv_det), written raw to scvw
+// SCV volumes of every hex8 element (hex_scv_det), written raw to scvw;
+// other elements (etype non-null) get zeros here and their volumes from
+// launch_scv_generic
 __global__ void __launch_bounds__(256) k_scv_volumes(int ne, const double* xyz, const int* conn,
-                                                     double* scvw) {
+                                                     const int* etype, double* scvw) {
     const int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= ne) return;
+    if (etype && etype[e] != OAGPU_HEX8) {
+        for (int n = 0; n < 8; ++n) scvw[8 * size_t(e) + n] = 0.0;
+        return;
+    }
     int nd[8];
     load_conn(conn, e, nd);
     double c[8][3];
@@ -47,7 +54,9 @@ __global__ void __launch_bounds__(256) k_scv_weights(int ne, double* scvw) {
 
 void launch_mesh_geometry(DevMesh& m, hipStream_t s) {
     const unsigned B = 256;
-    k_scv_volumes<<<xcd_grid(m.ne, B), B, 0, s>>>(m.ne, m.xyz.p, m.conn.p, m.scvw.p);
+    k_scv_volumes<<<xcd_grid(m.ne, B), B, 0, s>>>(m.ne, m.xyz.p, m.conn.p,
+                                                  m.mixed ? m.etype.p : nullptr, m.scvw.p);
+    if (m.mixed) launch_scv_generic(m, s);
     k_dual_volume<<<xcd_grid(m.nn, B), B, 0, s>>>(m.nn, m.inc_ptr.p, m.inc.p, m.scvw.p, m.vol.p);
     k_scv_weights<<<xcd_grid(m.ne, B), B, 0, s>>>(m.ne, m.scvw.p);
     OAGPU_HIP(hipGetLastError());
@@ -60,12 +69,23 @@ __global__ void __launch_bounds__(256) k_mass_divergence(int nn, const int* inc_
                                                          const double* mdot, const int* binc_ptr,
                                                          const int* binc, const int* bf_bc,
                                                          const int* bc_type, const double* msd,
-                                                         double* div) {
+                                                         const int* etype, double* div) {
     const int a = xcd_block() * blockDim.x + threadIdx.x;
     if (a >= nn) return;
     double s = 0.0;
     for (int k = inc_ptr[a]; k < inc_ptr[a + 1]; ++k) {
         const int e = inc[k] >> 3, la = inc[k] & 7;
+        const int t = etype ? etype[e] : OAGPU_HEX8;
+        if (t != OAGPU_HEX8) {  // the element's SCS ips with this node on either side
+            const int ns = t == OAGPU_TET4 ? 6 : t == OAGPU_WED6 ? 9 : 12;
+            for (int ip = 0; ip < ns; ++ip) {
+                const int* lr = t == OAGPU_TET4 ? topo::c_tet_lr[ip]
+                                : t == OAGPU_WED6 ? topo::c_wed_lr[ip] : topo::c_pyr_lr[ip];
+                if (lr[0] == la) s += mdot[12 * size_t(e) + ip];
+                else if (lr[1] == la) s -= mdot[12 * size_t(e) + ip];
+            }
+            continue;
+        }
         for (int q = 0; q < 3; ++q) {
             const double m = mdot[12 * size_t(e) + c_node_ip[la][q]];
             if (c_node_ip_sign[la][q] > 0) s += m; else s -= m;
@@ -84,7 +104,7 @@ void launch_mass_divergence(Flow& f) {
     const unsigned B = 256;
     k_mass_divergence<<<xcd_grid(m.nn, B), B, 0, f.stream>>>(
         m.nn, m.inc_ptr.p, m.inc.p, f.mdot.p, m.binc_ptr.p, m.binc.p, m.bf_bc.p, f.bc_type.p,
-        f.has_io ? f.msd.p : nullptr, f.div.p);
+        f.has_io ? f.msd.p : nullptr, m.mixed ? m.etype.p : nullptr, f.div.p);
     OAGPU_HIP(hipGetLastError());
 }
 

@@ -9,6 +9,7 @@
 // element column in a row and its lumping of absent columns into the diagonal
 // block (src/assemble/assembler.h:380-445).
 #include "mesh.hpp"
+#include "oagpu.h"
 #include "sell.hpp"
 
 #include <algorithm>
@@ -19,15 +20,28 @@ namespace oagpu {
 
 static const int kLrscv[12][2] = {{0, 1}, {1, 2}, {2, 3}, {0, 3}, {4, 5}, {5, 6},
                                   {6, 7}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+// SCS left / right nodes of tet4, wedge6, pyramid5 (Tet4CVFEM.C, Wed6CVFEM.C,
+// Pyr5CVFEM.C lrscv_)
+static const int kTetLr[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+static const int kWedLr[9][2] = {{0, 1}, {1, 2}, {0, 2}, {3, 4}, {4, 5}, {3, 5}, {0, 3}, {1, 4}, {2, 5}};
+static const int kPyrLr[12][2] = {{0, 1}, {1, 2}, {2, 3}, {0, 3}, {0, 4}, {0, 4},
+                                  {1, 4}, {1, 4}, {2, 4}, {2, 4}, {3, 4}, {3, 4}};
+
+static int topo_npe(int t) { return t == OAGPU_TET4 ? 4 : t == OAGPU_WED6 ? 6 : t == OAGPU_PYR5 ? 5 : 8; }
+static int topo_nscs(int t) { return t == OAGPU_TET4 ? 6 : t == OAGPU_WED6 ? 9 : 12; }
+static const int (*topo_lr(int t))[2] {
+    return t == OAGPU_TET4 ? kTetLr : t == OAGPU_WED6 ? kWedLr : t == OAGPU_PYR5 ? kPyrLr : kLrscv;
+}
 static const int kSideNodes[6][4] = {{0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6},
                                      {0, 4, 7, 3}, {0, 3, 2, 1}, {4, 5, 6, 7}};
 
 static uint64_t row_slots(const std::vector<int>& rp, const std::vector<int>& cols, int row,
-                          const int* en, int* max_pos) {
-    uint64_t s = 0;
+                          const int* en, int* max_pos, int npe = 8) {
+    uint64_t s = ~uint64_t(0);
     const int* b = cols.data() + rp[row];
     const int len = rp[row + 1] - rp[row];
-    for (int c = 0; c < 8; ++c) {
+    for (int c = 0; c < npe; ++c) {
+        s &= ~(uint64_t(0xFF) << (8 * c));
         const int* it = std::lower_bound(b, b + len, en[c]);
         uint64_t pos = 0xFF;
         if (it != b + len && *it == en[c]) {
@@ -43,10 +57,18 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
     OAGPU_REQUIRE(hm.nn > 0 && hm.ne > 0, "empty mesh");
     OAGPU_REQUIRE(hm.ne < (1 << 28), "element count exceeds the incidence encoding");
     const int nn = hm.nn, ne = hm.ne;
-    for (size_t k = 0; k < size_t(ne) * 8; ++k)
-        OAGPU_REQUIRE(hm.conn[k] >= 0 && hm.conn[k] < nn, "connectivity out of range");
+    auto etype = [&](int e) { return hm.etype ? hm.etype[e] : int(OAGPU_HEX8); };
+    for (int e = 0; e < ne; ++e) {
+        OAGPU_REQUIRE(etype(e) >= OAGPU_HEX8 && etype(e) <= OAGPU_PYR5, "unknown element topology");
+        for (int k = 0; k < topo_npe(etype(e)); ++k)
+            OAGPU_REQUIRE(hm.conn[8 * size_t(e) + k] >= 0 && hm.conn[8 * size_t(e) + k] < nn,
+                          "connectivity out of range");
+        dm.mixed = dm.mixed || etype(e) != OAGPU_HEX8;
+    }
     for (int f = 0; f < hm.nbf; ++f) {
         OAGPU_REQUIRE(hm.bf_elem[f] >= 0 && hm.bf_elem[f] < ne, "boundary face element out of range");
+        OAGPU_REQUIRE(etype(hm.bf_elem[f]) == OAGPU_HEX8,
+                      "boundary faces must be quad sides of hex8 elements");
         OAGPU_REQUIRE(hm.bf_ord[f] >= 0 && hm.bf_ord[f] < 6, "boundary face ordinal out of range");
         OAGPU_REQUIRE(hm.bf_bc[f] >= 0, "negative boundary id");
     }
@@ -59,13 +81,15 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
     std::vector<int> cnt(size_t(nn) + 1, 0);
     for (int e = 0; e < ne; ++e) {
         const int* en = hm.conn + 8 * size_t(e);
+        const int t = etype(e), npe = topo_npe(t);
+        const int(*lr)[2] = topo_lr(t);
         if (reduced) {
-            for (int ip = 0; ip < 12; ++ip) {
-                cnt[en[kLrscv[ip][0]]] += 2;
-                cnt[en[kLrscv[ip][1]]] += 2;
+            for (int ip = 0; ip < topo_nscs(t); ++ip) {
+                cnt[en[lr[ip][0]]] += 2;
+                cnt[en[lr[ip][1]]] += 2;
             }
         } else {
-            for (int a = 0; a < 8; ++a) cnt[en[a]] += 8;
+            for (int a = 0; a < npe; ++a) cnt[en[a]] += npe;
         }
     }
     std::vector<size_t> start(size_t(nn) + 1, 0);
@@ -74,18 +98,20 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
     std::fill(cnt.begin(), cnt.end(), 0);
     for (int e = 0; e < ne; ++e) {
         const int* en = hm.conn + 8 * size_t(e);
+        const int t = etype(e), npe = topo_npe(t);
+        const int(*lr)[2] = topo_lr(t);
         if (reduced) {
-            for (int ip = 0; ip < 12; ++ip) {
-                const int L = en[kLrscv[ip][0]], R = en[kLrscv[ip][1]];
+            for (int ip = 0; ip < topo_nscs(t); ++ip) {
+                const int L = en[lr[ip][0]], R = en[lr[ip][1]];
                 cand[start[L] + cnt[L]++] = L;
                 cand[start[L] + cnt[L]++] = R;
                 cand[start[R] + cnt[R]++] = L;
                 cand[start[R] + cnt[R]++] = R;
             }
         } else {
-            for (int a = 0; a < 8; ++a) {
+            for (int a = 0; a < npe; ++a) {
                 const int r = en[a];
-                for (int c = 0; c < 8; ++c) cand[start[r] + cnt[r]++] = en[c];
+                for (int c = 0; c < npe; ++c) cand[start[r] + cnt[r]++] = en[c];
             }
         }
     }
@@ -173,8 +199,10 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
         dm.slice_order.upload(so);
         const std::vector<int> eo = order_groups(ne, 8, [&](int e, double x[3]) {
             for (int d = 0; d < 3; ++d) x[d] = 0.0;
-            for (int k = 0; k < 8; ++k)
-                for (int d = 0; d < 3; ++d) x[d] += 0.125 * hm.xyz[3 * size_t(hm.conn[8 * size_t(e) + k]) + d];
+            const int npe = topo_npe(etype(e));
+            for (int k = 0; k < npe; ++k)
+                for (int d = 0; d < 3; ++d)
+                    x[d] += hm.xyz[3 * size_t(hm.conn[8 * size_t(e) + k]) + d] / npe;
         });
         dm.negroup = int(eo.size());
         dm.egroup_order.upload(eo);
@@ -182,7 +210,8 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
 
     // ---- node -> element incidences in ascending element order
     dm.h_inc_ptr.assign(size_t(nn) + 1, 0);
-    for (size_t k = 0; k < size_t(ne) * 8; ++k) dm.h_inc_ptr[hm.conn[k] + 1]++;
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < topo_npe(etype(e)); ++a) dm.h_inc_ptr[hm.conn[8 * size_t(e) + a] + 1]++;
     for (int i = 0; i < nn; ++i) dm.h_inc_ptr[i + 1] += dm.h_inc_ptr[i];
     dm.ninc = dm.h_inc_ptr[nn];
     dm.h_inc.resize(dm.ninc);
@@ -192,11 +221,12 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
         int max_pos = 0;
         for (int e = 0; e < ne; ++e) {
             const int* en = hm.conn + 8 * size_t(e);
-            for (int a = 0; a < 8; ++a) {
+            const int npe = topo_npe(etype(e));
+            for (int a = 0; a < npe; ++a) {
                 const int r = en[a];
                 const int k = fill[r]++;
                 dm.h_inc[k] = (e << 3) | a;
-                dm.h_inc_slots[k] = row_slots(dm.h_row_ptr, dm.h_cols, r, en, &max_pos);
+                dm.h_inc_slots[k] = row_slots(dm.h_row_ptr, dm.h_cols, r, en, &max_pos, npe);
             }
         }
     }
@@ -235,6 +265,15 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
     dm.xyz.upload(hm.xyz, size_t(nn) * 3);
     dm.conn.alloc(size_t(ne) * 8);
     dm.conn.upload(hm.conn, size_t(ne) * 8);
+    if (dm.mixed) {
+        dm.h_etype.assign(hm.etype, hm.etype + ne);
+        dm.etype.upload(dm.h_etype);
+        std::vector<int> gl;
+        for (int e = 0; e < ne; ++e)
+            if (etype(e) != OAGPU_HEX8) gl.push_back(e);
+        dm.nglist = int(gl.size());
+        dm.glist.upload(gl);
+    }
     dm.row_ptr.upload(dm.h_row_ptr);
     dm.cols.upload(dm.h_cols);
     dm.diag_off.upload(dm.h_diag_off);
@@ -244,11 +283,16 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
         // local-node major copy of the incidences for the row kernels
         std::vector<int> inc_la(size_t(8) * nn, -1);
         std::vector<uint64_t> slots_la(size_t(8) * nn, ~uint64_t(0));
-        std::vector<int> ext_ptr(size_t(nn) + 1, 0), ext;
-        std::vector<uint64_t> ext_slots;
+        std::vector<int> ext_ptr(size_t(nn) + 1, 0), ext, gen_ptr(size_t(nn) + 1, 0), gen;
+        std::vector<uint64_t> ext_slots, gen_slots;
         for (int r = 0; r < nn; ++r) {
             for (int k = dm.h_inc_ptr[r]; k < dm.h_inc_ptr[r + 1]; ++k) {
                 const int la = dm.h_inc[k] & 7;
+                if (etype(dm.h_inc[k] >> 3) != OAGPU_HEX8) {
+                    gen.push_back(dm.h_inc[k]);
+                    gen_slots.push_back(dm.h_inc_slots[k]);
+                    continue;
+                }
                 const size_t o = size_t(la) * nn + r;
                 if (inc_la[o] < 0) {
                     inc_la[o] = dm.h_inc[k] >> 3;
@@ -259,8 +303,17 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
                 }
             }
             ext_ptr[r + 1] = int(ext.size());
+            gen_ptr[r + 1] = int(gen.size());
         }
         dm.next = int(ext.size());
+        dm.ngen = int(gen.size());
+        dm.gen_ptr.upload(gen_ptr);
+        if (gen.empty()) {
+            gen.push_back(0);
+            gen_slots.push_back(0);
+        }
+        dm.gen.upload(gen);
+        dm.gen_slots.upload(gen_slots);
         dm.inc_la.upload(inc_la);
         dm.slots_la.upload(slots_la);
         dm.ext_ptr.upload(ext_ptr);

@@ -11,7 +11,8 @@ namespace oagpu {
 struct HostMesh {
     int nn = 0, ne = 0, nbf = 0;
     const double* xyz = nullptr;
-    const int* conn = nullptr;
+    const int* conn = nullptr;      // [ne][8], trailing entries of smaller elements unused
+    const int* etype = nullptr;     // [ne] OAGPU_HEX8 / TET4 / WED6 / PYR5; null: all hex8
     const int* bf_elem = nullptr;
     const int* bf_ord = nullptr;
     const int* bf_bc = nullptr;
@@ -33,11 +34,22 @@ struct HostMesh {
 //   (assembler.h:423-445).
 // * node -> wall-face incidences (face << 2 | face ip whose nearest node is
 //   the row), with their own scatter maps.
+// * meshes with tet4 / wedge6 / pyramid5 elements: inc_la / ext hold the
+//   hex8 incidences only; the others (elem << 3 | local node) with their
+//   scatter maps (one byte per element node) form a CSR list gen_* per row,
+//   walked by the mixed-mesh variants of the row kernels; glist lists the
+//   non-hex elements for the element-owned kernels.
 struct DevMesh {
     int nn = 0, ne = 0, nbf = 0, nnz = 0, ninc = 0, nbinc = 0, max_row = 0, next = 0;
     int reduced = 1;
+    bool mixed = false;        // some element is not hex8
+    int ngen = 0, nglist = 0;
     DBuf<double> xyz;          // [nn][3]
     DBuf<int> conn;            // [ne][8]
+    DBuf<int> etype;           // [ne] (mixed meshes only)
+    DBuf<int> gen_ptr, gen;    // [nn+1], [ngen]
+    DBuf<uint64_t> gen_slots;  // [ngen]
+    DBuf<int> glist;           // [nglist] non-hex elements, ascending
     DBuf<int> row_ptr, cols, diag_off;
     // Values of the CRS matrix are stored sliced-ELL: slices of 64 rows (one
     // wavefront), slot-major and lane-contiguous, so a wavefront's lanes (one
@@ -71,7 +83,7 @@ struct DevMesh {
     DBuf<int> err;             // device error word (non-positive Jacobians)
 
     // host copies kept for the bit-exact graph/scatter-map checks
-    std::vector<int> h_row_ptr, h_cols, h_diag_off, h_inc_ptr, h_inc, h_binc_ptr, h_binc;
+    std::vector<int> h_row_ptr, h_cols, h_diag_off, h_inc_ptr, h_inc, h_binc_ptr, h_binc, h_etype;
     std::vector<int> h_slice_ptr, h_slice_w;
     std::vector<uint64_t> h_inc_slots, h_binc_slots;
 };

@@ -20,6 +20,7 @@
 // (coordinates are shifted to the row node in FP64 before rounding, so FP32
 // sees element-sized numbers).  Global coefficients are summed in FP64.
 #include "device_util.hpp"
+#include "topo_tables.hpp"
 #include "tables.hpp"
 
 #include <type_traits>
@@ -432,6 +433,10 @@ struct MomArgs {
     const double* symA;    // assembled symmetry area [nn][3], null without symmetry
     const double* msd;     // side mass flow per boundary face ip, null without inlets / outlets
     const int* rfl;        // outlet reversal flag per boundary face ip
+    const int* gen_ptr;    // tet4 / wedge6 / pyramid5 incidences (mixed meshes)
+    const int* gen;
+    const uint64_t* gen_slots;
+    const int* etype;
     double rho, mu, dt, urf_inv, bnd_relax;
     int ugrad_shifted, us;
     double* vals;
@@ -606,6 +611,76 @@ __device__ __forceinline__ void mom_incidence(const MomArgs& a, int la, int e, u
     }
 }
 
+// the SCS ips of a tet4 / wedge6 / pyramid5 element touching the momentum
+// row of its local node la (navierStokesAssemblerElemTerms.cpp:256-435 per
+// ip): area vector and mass flow taken from the row's side (the row node
+// on the left), upwind advection with the limited deferred correction on
+// the diagonal and the other node's column, the viscous blocks
+// -mu (grad N . A I + grad N (x) A) of every column with their explicit
+// flux on the rhs
+template <typename T, int TOPO, bool HR, typename LT>
+__device__ void mom_generic(const MomArgs& a, int la, int e, uint64_t sl, const SellRow& sr, int dof,
+                            const double xr[3], double D[9], double R[3], int& bad, LT* lt) {
+    constexpr int npe = topo::Traits<TOPO>::npe;
+    const int* ce = a.conn + 8 * size_t(e);
+    T c[npe][3];
+    for (int k = 0; k < npe; ++k)
+        for (int d = 0; d < 3; ++d) c[k][d] = T(a.xyz[3 * size_t(ce[k]) + d] - xr[d]);
+#pragma unroll 1
+    for (int ip = 0; ip < topo::Traits<TOPO>::nscs; ++ip) {
+        const int* lr = topo::lr<TOPO>(ip);
+        if (lr[0] != la && lr[1] != la) continue;
+        const bool left = lr[0] == la;
+        const int o = left ? lr[1] : lr[0];
+        const T sg = left ? T(1) : T(-1);
+        T A[3], g[npe][3], Nc[npe], Nu[npe];
+        topo::scs_ip<TOPO>(ip, a.ugrad_shifted, a.us, c, A, g, Nc, Nu, bad);
+        for (int d = 0; d < 3; ++d) A[d] *= sg;
+        const T m = sg * T(a.mdot[12 * size_t(e) + ip]);
+        T mu = T(a.mu);
+        if (a.mueff) {
+            mu = T(0);
+            for (int k = 0; k < npe; ++k) mu += Nu[k] * T(a.mueff[ce[k]]);
+        }
+        const int ul = m > T(0) ? la : o;
+        const size_t nu = size_t(ce[ul]);
+        T up[3] = {T(a.U[3 * nu]), T(a.U[3 * nu + 1]), T(a.U[3 * nu + 2])};
+        if constexpr (HR) {
+            T dx[3], cu[3];
+            topo::pick_row<npe>(c, ul, cu);
+            for (int d = 0; d < 3; ++d) {
+                T xi = T(0);
+                for (int k = 0; k < npe; ++k) xi += Nc[k] * c[k][d];
+                dx[d] = xi - cu[d];
+            }
+            for (int i = 0; i < 3; ++i) {
+                T dc = T(0);
+                for (int j = 0; j < 3; ++j) dc += T(a.beta[3 * nu + i]) * dx[j] * T(a.gradU[9 * nu + 3 * i + j]);
+                up[i] += dc;
+            }
+        }
+        const T aS = T(0.5) * (m + fabs(m)), aO = T(0.5) * (m - fabs(m));
+        for (int i = 0; i < 3; ++i) {
+            R[i] -= double(m * up[i]);
+            D[4 * i] += double(aS);
+        }
+        T rl[3] = {0, 0, 0};
+#pragma unroll
+        for (int ic = 0; ic < npe; ++ic) {
+            T acc[9] = {};
+            visc_block(mu, g[ic], A, acc);
+            const size_t oc = 3 * size_t(ce[ic]);
+            const T uc[3] = {T(a.U[oc]), T(a.U[oc + 1]), T(a.U[oc + 2])};
+            for (int i = 0; i < 3; ++i)
+                rl[i] -= acc[3 * i] * uc[0] + acc[3 * i + 1] * uc[1] + acc[3 * i + 2] * uc[2];
+            if (ic == o)
+                for (int i = 0; i < 3; ++i) acc[4 * i] += aO;
+            scatter_block(a.vals, sr, dof, sl, ic, acc, D, lt);
+        }
+        for (int i = 0; i < 3; ++i) R[i] += double(rl[i]);
+    }
+}
+
 // viscous face terms of the face ip of side (f, fk) nearest to this row.
 // ZNS: zero normal stress (the stress projected off the face normal:
 // IP_ZERO_NORMAL_STRESS_FIXED_VEL__ / IP_ZERO_NORMAL_STRESS__,
@@ -718,7 +793,7 @@ __device__ __forceinline__ void mom_boundary(const MomArgs& a, int pk, uint64_t 
 // LDS: 0 = off-diagonal blocks read-modify-written in HBM (256-lane blocks);
 // 1 / 2 = summed in a per-wavefront LDS tile in float / double and written
 // once (64-lane blocks, one wavefront each).
-template <typename T, int SH, int LDS, bool HR>
+template <typename T, int SH, int LDS, bool HR, bool MX>
 __global__ void __launch_bounds__(LDS ? 64 : 256) k_momentum_rows(const MomArgs a) {
     using LT = std::conditional_t<LDS == 0, void, std::conditional_t<LDS == 2, double, float>>;
     using LS = std::conditional_t<LDS == 2, double, float>;
@@ -771,6 +846,18 @@ __global__ void __launch_bounds__(LDS ? 64 : 256) k_momentum_rows(const MomArgs 
     for (int k = a.ext_ptr[row]; k < a.ext_ptr[row + 1]; ++k)
         mom_incidence<T, SH, HR>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, D, R,
                                  bad, lt);
+    if constexpr (MX) {
+#pragma unroll 1
+        for (int k = a.gen_ptr[row]; k < a.gen_ptr[row + 1]; ++k) {
+            const int e = a.gen[k] >> 3, la = a.gen[k] & 7, t = a.etype[e];
+            if (t == OAGPU_TET4)
+                mom_generic<T, topo::kTet, HR>(a, la, e, a.gen_slots[k], sr, dof, xr, D, R, bad, lt);
+            else if (t == OAGPU_WED6)
+                mom_generic<T, topo::kWed, HR>(a, la, e, a.gen_slots[k], sr, dof, xr, D, R, bad, lt);
+            else
+                mom_generic<T, topo::kPyr, HR>(a, la, e, a.gen_slots[k], sr, dof, xr, D, R, bad, lt);
+        }
+    }
 #pragma unroll 1
     for (int k = a.binc_ptr[row]; k < a.binc_ptr[row + 1]; ++k)
         mom_boundary<T>(a, a.binc[k], a.binc_slots[k], sr, dof, row, xr, D, R, bad, lt);
@@ -860,6 +947,10 @@ void launch_assemble_momentum(Flow& f) {
     a.symA = f.has_sym ? f.symA.p : nullptr;
     a.msd = f.has_io ? f.msd.p : nullptr;
     a.rfl = f.has_io ? f.rfl.p : nullptr;
+    a.gen_ptr = m.gen_ptr.p;
+    a.gen = m.gen.p;
+    a.gen_slots = m.gen_slots.p;
+    a.etype = m.etype.p;
     a.vals = f.A3.p;
     a.b = f.b3.p;
     a.du = f.du.p;
@@ -873,13 +964,18 @@ void launch_assemble_momentum(Flow& f) {
         kern<<<xcd_grid(m.nn, B), B, 0, f.stream>>>(a);
     };
     const bool hr = a.beta != nullptr;
-    if (a.ugrad_shifted) {
-        if (mixed) go(hr ? k_momentum_rows<float, 1, 1, true> : k_momentum_rows<float, 1, 1, false>, 64);
-        else go(hr ? k_momentum_rows<double, 1, 2, true> : k_momentum_rows<double, 1, 2, false>, 64);
-    } else {
-        if (mixed) go(hr ? k_momentum_rows<float, 0, 1, true> : k_momentum_rows<float, 0, 1, false>, 64);
-        else go(hr ? k_momentum_rows<double, 0, 2, true> : k_momentum_rows<double, 0, 2, false>, 64);
-    }
+    auto pick = [&](auto mx) {
+        constexpr bool MX = decltype(mx)::value;
+        if (a.ugrad_shifted) {
+            if (mixed) go(hr ? k_momentum_rows<float, 1, 1, true, MX> : k_momentum_rows<float, 1, 1, false, MX>, 64);
+            else go(hr ? k_momentum_rows<double, 1, 2, true, MX> : k_momentum_rows<double, 1, 2, false, MX>, 64);
+        } else {
+            if (mixed) go(hr ? k_momentum_rows<float, 0, 1, true, MX> : k_momentum_rows<float, 0, 1, false, MX>, 64);
+            else go(hr ? k_momentum_rows<double, 0, 2, true, MX> : k_momentum_rows<double, 0, 2, false, MX>, 64);
+        }
+    };
+    if (m.mixed) pick(std::true_type{});
+    else pick(std::false_type{});
     OAGPU_HIP(hipGetLastError());
 }
 
@@ -913,6 +1009,10 @@ struct PcArgs {
     const double* bc_U;
     const double* bc_p;
     const int* rfl;
+    const int* gen_ptr;    // tet4 / wedge6 / pyramid5 incidences (mixed meshes)
+    const int* gen;
+    const uint64_t* gen_slots;
+    const int* etype;
     double rho;
     int u_shifted, p_ref;
     double* vals;
@@ -1062,6 +1162,61 @@ __device__ __forceinline__ void pc_incidence(const PcArgs& a, int la, int e, uin
     }
 }
 
+// the SCS ips of a tet4 / wedge6 / pyramid5 element touching the p' row of
+// its local node la (pressureCorrectionAssemblerElemTerms.cpp:390-610 per
+// ip), from the row's side: -rho du_ip grad N . A per column, the
+// Rhie-Chow mass flow on the rhs (pressures relative to the row's)
+template <typename T, int TOPO>
+__device__ void pc_generic(const PcArgs& a, int la, int e, uint64_t sl, const SellRow& sr, int dof,
+                           const double xr[3], double pr, int sp, double& D, double& R, int& bad,
+                           double* lt) {
+    constexpr int npe = topo::Traits<TOPO>::npe;
+    const int* ce = a.conn + 8 * size_t(e);
+    T c[npe][3], Gp[3] = {0, 0, 0};
+    for (int k = 0; k < npe; ++k) {
+        const size_t o = 3 * size_t(ce[k]);
+        const T w = T(a.scvw[8 * size_t(e) + k]);
+        for (int d = 0; d < 3; ++d) {
+            c[k][d] = T(a.xyz[o + d] - xr[d]);
+            Gp[d] += w * T(a.gradP[o + d]);
+        }
+    }
+    const T rho = T(a.rho);
+#pragma unroll 1
+    for (int ip = 0; ip < topo::Traits<TOPO>::nscs; ++ip) {
+        const int* lr = topo::lr<TOPO>(ip);
+        if (lr[0] != la && lr[1] != la) continue;
+        const T sg = lr[0] == la ? T(1) : T(-1);
+        T A[3], g[npe][3], Nc[npe], Nu[npe];
+        topo::scs_ip<TOPO>(ip, sp, a.u_shifted, c, A, g, Nc, Nu, bad);
+        for (int d = 0; d < 3; ++d) A[d] *= sg;
+        T uip[3] = {0, 0, 0}, duip[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
+        for (int k = 0; k < npe; ++k) {
+            const size_t o = 3 * size_t(ce[k]);
+            const T pn = T(a.p[ce[k]] - pr);
+            for (int j = 0; j < 3; ++j) {
+                uip[j] += Nu[k] * T(a.U[o + j]);
+                duip[j] += Nu[k] * T(a.du[o + j]);
+                dp[j] += g[k][j] * pn;
+            }
+        }
+        T md = T(0);
+        for (int j = 0; j < 3; ++j) {
+            md += rho * uip[j] * A[j];
+            md -= rho * duip[j] * (dp[j] - Gp[j]) * A[j];
+        }
+        R -= double(md);
+#pragma unroll
+        for (int ic = 0; ic < npe; ++ic) {
+            T lf = T(0);
+            for (int j = 0; j < 3; ++j) lf += -rho * duip[j] * g[ic][j] * A[j];
+            const int s = slot_of(sl, ic);
+            if (s == 0xFF || s == dof) D += double(lf);
+            else scalar_scatter(a.vals, sr, lt, s, double(lf));
+        }
+    }
+}
+
 // inlet / outlet face ip (f, fk) nearest to this p' row, in FP64 in both
 // forms.  Inlet (pressureCorrectionAssemblerElemBoundaryConditions.cpp:
 // 923-1393): the specified mass flow rho U_bc . A leaves the rhs.  Outlet
@@ -1137,7 +1292,7 @@ __global__ void __launch_bounds__(256) k_pcorr_boundary(const PcArgs a, int nrow
 // pressureCorrectionAssembler interior terms for one row, with the
 // pressure-reference row (pressureCorrectionAssembler.cpp:19-63; none when
 // an outlet specifies the pressure level)
-template <typename T, int SP>
+template <typename T, int SP, bool MX>
 __global__ void __launch_bounds__(64) k_pcorr_rows(const PcArgs a) {
     __shared__ double tile[kLdsSlots * 64];
     double* lt = tile + threadIdx.x;
@@ -1161,6 +1316,18 @@ __global__ void __launch_bounds__(64) k_pcorr_rows(const PcArgs a) {
     for (int k = a.ext_ptr[row]; k < a.ext_ptr[row + 1]; ++k)
         pc_incidence<T, SP>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, pr, D, R,
                             bad, lt);
+    if constexpr (MX) {
+#pragma unroll 1
+        for (int k = a.gen_ptr[row]; k < a.gen_ptr[row + 1]; ++k) {
+            const int e = a.gen[k] >> 3, la = a.gen[k] & 7, t = a.etype[e];
+            if (t == OAGPU_TET4)
+                pc_generic<T, topo::kTet>(a, la, e, a.gen_slots[k], sr, dof, xr, pr, SP, D, R, bad, lt);
+            else if (t == OAGPU_WED6)
+                pc_generic<T, topo::kWed>(a, la, e, a.gen_slots[k], sr, dof, xr, pr, SP, D, R, bad, lt);
+            else
+                pc_generic<T, topo::kPyr>(a, la, e, a.gen_slots[k], sr, dof, xr, pr, SP, D, R, bad, lt);
+        }
+    }
     if (row == a.p_ref) {
         scalar_row_zero(a.vals, sr, lt, dof);
         R = 0.0;
@@ -1202,6 +1369,10 @@ void launch_assemble_pcorr(Flow& f) {
     a.bc_U = f.bc_U.p;
     a.bc_p = f.bc_p.p;
     a.rfl = f.rfl.p;
+    a.gen_ptr = m.gen_ptr.p;
+    a.gen = m.gen.p;
+    a.gen_slots = m.gen_slots.p;
+    a.etype = m.etype.p;
     a.rho = f.prm.rho;
     a.u_shifted = f.prm.u_shifted;
     a.p_ref = f.has_outlet ? -1 : f.prm.p_ref_node;  // domain::pressureLevelRequired
@@ -1210,13 +1381,18 @@ void launch_assemble_pcorr(Flow& f) {
     a.err = m.err.p;
     const unsigned B = 64, G = xcd_grid(m.nn, B);
     const bool mixed = f.prm.precision == OAGPU_MIXED;
-    if (f.prm.pgrad_shifted) {
-        if (mixed) k_pcorr_rows<float, 1><<<G, B, 0, f.stream>>>(a);
-        else k_pcorr_rows<double, 1><<<G, B, 0, f.stream>>>(a);
-    } else {
-        if (mixed) k_pcorr_rows<float, 0><<<G, B, 0, f.stream>>>(a);
-        else k_pcorr_rows<double, 0><<<G, B, 0, f.stream>>>(a);
-    }
+    auto pick = [&](auto mx) {
+        constexpr bool MX = decltype(mx)::value;
+        if (f.prm.pgrad_shifted) {
+            if (mixed) k_pcorr_rows<float, 1, MX><<<G, B, 0, f.stream>>>(a);
+            else k_pcorr_rows<double, 1, MX><<<G, B, 0, f.stream>>>(a);
+        } else {
+            if (mixed) k_pcorr_rows<float, 0, MX><<<G, B, 0, f.stream>>>(a);
+            else k_pcorr_rows<double, 0, MX><<<G, B, 0, f.stream>>>(a);
+        }
+    };
+    if (m.mixed) pick(std::true_type{});
+    else pick(std::false_type{});
     OAGPU_HIP(hipGetLastError());
     if (f.has_io) {
         const unsigned GB = xcd_grid(size_t(f.nio_row), 256);
@@ -1275,6 +1451,10 @@ struct ScArgs {
     const double* bc_phi;  // inlet k or omega per boundary id
     const double* msd;
     const int* rfl;
+    const int* gen_ptr;    // tet4 / wedge6 / pyramid5 incidences (mixed meshes)
+    const int* gen;
+    const uint64_t* gen_slots;
+    const int* etype;
     double rho, mu, dt, urf_inv, sig1, sig2;
     int which, us;
     double* vals;
@@ -1462,6 +1642,66 @@ __device__ __forceinline__ void sc_incidence(const ScArgs& a, int la, int e, uin
     }
 }
 
+// the SCS ips of a tet4 / wedge6 / pyramid5 element touching the k / omega
+// row of its local node la (phiAssemblerElemTerms.hpp:171-406 per ip), from
+// the row's side: upwind advection with the limited deferred correction,
+// diffusion -Gamma_ip grad N . A per column with its explicit flux on the rhs
+template <typename T, int TOPO, bool HR>
+__device__ void sc_generic(const ScArgs& a, int la, int e, uint64_t sl, const SellRow& sr, int dof,
+                           const double xr[3], int sg_shift, double& D, double& R, int& bad,
+                           double* lt) {
+    constexpr int npe = topo::Traits<TOPO>::npe;
+    const int* ce = a.conn + 8 * size_t(e);
+    T c[npe][3], ph[npe], gm[npe];
+    for (int k = 0; k < npe; ++k) {
+        for (int d = 0; d < 3; ++d) c[k][d] = T(a.xyz[3 * size_t(ce[k]) + d] - xr[d]);
+        ph[k] = T(a.phi[ce[k]]);
+        const double F1 = a.F1[ce[k]];
+        gm[k] = T(a.mu + a.mut[ce[k]] * (F1 * a.sig1 + (1.0 - F1) * a.sig2));
+    }
+#pragma unroll 1
+    for (int ip = 0; ip < topo::Traits<TOPO>::nscs; ++ip) {
+        const int* lr = topo::lr<TOPO>(ip);
+        if (lr[0] != la && lr[1] != la) continue;
+        const bool left = lr[0] == la;
+        const int o = left ? lr[1] : lr[0];
+        const T sgn = left ? T(1) : T(-1);
+        T A[3], g[npe][3], Nc[npe], Nu[npe];
+        topo::scs_ip<TOPO>(ip, sg_shift, a.us, c, A, g, Nc, Nu, bad);
+        for (int d = 0; d < 3; ++d) A[d] *= sgn;
+        const T m = sgn * T(a.mdot[12 * size_t(e) + ip]);
+        T Gip = T(0);
+        for (int k = 0; k < npe; ++k) Gip += Nu[k] * gm[k];
+        const int ul = m > T(0) ? la : o;
+        T up = topo::pick<npe>(ph, ul);
+        if constexpr (HR) {
+            const size_t nu = size_t(ce[ul]);
+            T dc = T(0), cu[3];
+            topo::pick_row<npe>(c, ul, cu);
+            for (int d = 0; d < 3; ++d) {
+                T xi = T(0);
+                for (int k = 0; k < npe; ++k) xi += Nc[k] * c[k][d];
+                dc += T(a.beta[nu]) * (xi - cu[d]) * T(a.gphi[3 * nu + d]);
+            }
+            up += dc;
+        }
+        const T aS = T(0.5) * (m + fabs(m)), aO = T(0.5) * (m - fabs(m));
+        T rl = -(m * up), lD = aS;
+#pragma unroll
+        for (int ic = 0; ic < npe; ++ic) {
+            T lf = T(0);
+            for (int j = 0; j < 3; ++j) lf += -Gip * g[ic][j] * A[j];
+            rl -= lf * ph[ic];
+            if (ic == o) lf += aO;
+            const int s = slot_of(sl, ic);
+            if (s == 0xFF || s == dof) lD += lf;
+            else scalar_scatter(a.vals, sr, lt, s, double(lf));
+        }
+        D += double(lD);
+        R += double(rl);
+    }
+}
+
 // inlet / outlet face ip (f, fk) nearest to this k / omega row, FP64 in
 // both forms, skipped while the ip is reversed.  Inlet
 // (phiAssemblerElemBoundaryConditions.hpp:894-1229, fixed value): the
@@ -1544,7 +1784,7 @@ __global__ void __launch_bounds__(256) k_scalar_boundary(const ScArgs a, int nro
 // (phiAssembler.h:966-1006), boundary rhs relaxation x 0.75
 // (turbulentKineticEnergyAssembler.h:44-48, phiAssembler.h:1008-1152), and
 // for omega the no-slip wall rows reduced to their diagonal with b = 0
-template <typename T, int SG, bool HR>
+template <typename T, int SG, bool HR, bool MX>
 __global__ void __launch_bounds__(64) k_scalar_rows(const ScArgs a) {
     __shared__ double tile[kLdsSlots * 64];
     double* lt = tile + threadIdx.x;
@@ -1568,6 +1808,18 @@ __global__ void __launch_bounds__(64) k_scalar_rows(const ScArgs a) {
     for (int k = a.ext_ptr[row]; k < a.ext_ptr[row + 1]; ++k)
         sc_incidence<T, SG, HR>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, D, R,
                                 bad, lt);
+    if constexpr (MX) {
+#pragma unroll 1
+        for (int k = a.gen_ptr[row]; k < a.gen_ptr[row + 1]; ++k) {
+            const int e = a.gen[k] >> 3, la = a.gen[k] & 7, t = a.etype[e];
+            if (t == OAGPU_TET4)
+                sc_generic<T, topo::kTet, HR>(a, la, e, a.gen_slots[k], sr, dof, xr, SG, D, R, bad, lt);
+            else if (t == OAGPU_WED6)
+                sc_generic<T, topo::kWed, HR>(a, la, e, a.gen_slots[k], sr, dof, xr, SG, D, R, bad, lt);
+            else
+                sc_generic<T, topo::kPyr, HR>(a, la, e, a.gen_slots[k], sr, dof, xr, SG, D, R, bad, lt);
+        }
+    }
     D *= a.urf_inv;
     if (a.bnode[row]) R *= 0.75;
     if (a.which == 1 && a.wnode[row]) {  // omega rows fixed at no-slip wall nodes
@@ -1626,6 +1878,10 @@ void launch_assemble_scalar(Flow& f, int which) {
     a.bc_phi = which == 0 ? f.bc_k.p : f.bc_w.p;
     a.msd = f.msd.p;
     a.rfl = f.rfl.p;
+    a.gen_ptr = m.gen_ptr.p;
+    a.gen = m.gen.p;
+    a.gen_slots = m.gen_slots.p;
+    a.etype = m.etype.p;
     a.rho = f.prm.rho;
     a.mu = f.prm.mu;
     a.dt = f.prm.dt;
@@ -1642,13 +1898,18 @@ void launch_assemble_scalar(Flow& f, int which) {
     const bool gs = (which == 0 ? f.sst.kgrad_shifted : f.sst.wgrad_shifted) != 0;
     const bool hr = a.beta != nullptr;
     auto go = [&](auto kern) { kern<<<G, B, 0, f.stream>>>(a); };
-    if (gs) {
-        if (mixed) go(hr ? k_scalar_rows<float, 1, true> : k_scalar_rows<float, 1, false>);
-        else go(hr ? k_scalar_rows<double, 1, true> : k_scalar_rows<double, 1, false>);
-    } else {
-        if (mixed) go(hr ? k_scalar_rows<float, 0, true> : k_scalar_rows<float, 0, false>);
-        else go(hr ? k_scalar_rows<double, 0, true> : k_scalar_rows<double, 0, false>);
-    }
+    auto pick = [&](auto mx) {
+        constexpr bool MX = decltype(mx)::value;
+        if (gs) {
+            if (mixed) go(hr ? k_scalar_rows<float, 1, true, MX> : k_scalar_rows<float, 1, false, MX>);
+            else go(hr ? k_scalar_rows<double, 1, true, MX> : k_scalar_rows<double, 1, false, MX>);
+        } else {
+            if (mixed) go(hr ? k_scalar_rows<float, 0, true, MX> : k_scalar_rows<float, 0, false, MX>);
+            else go(hr ? k_scalar_rows<double, 0, true, MX> : k_scalar_rows<double, 0, false, MX>);
+        }
+    };
+    if (m.mixed) pick(std::true_type{});
+    else pick(std::false_type{});
     OAGPU_HIP(hipGetLastError());
     if (f.has_io) {
         const unsigned GB = xcd_grid(size_t(f.nio_row), 256);
@@ -1662,9 +1923,11 @@ void launch_assemble_scalar(Flow& f, int which) {
 // initializeMassFlowRateInterior_ (flowModel.cpp:3303-3470): rho_ip U_ip . A
 template <int US>
 __global__ void __launch_bounds__(256) k_mdot_init(int ne, const double* xyz, const int* conn,
-                                                   const double* U, double rho, double* mdot) {
+                                                   const int* etype, const double* U, double rho,
+                                                   double* mdot) {
     const int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= ne) return;
+    if (etype && etype[e] != OAGPU_HEX8) return;  // k_mdot_generic
     int nd[8];
     load_conn(conn, e, nd);
     const double xr[3] = {xyz[3 * size_t(nd[0])], xyz[3 * size_t(nd[0]) + 1],
@@ -1695,6 +1958,7 @@ __global__ void __launch_bounds__(256) k_mdot_init(int ne, const double* xyz, co
 // reads and writes instead of 8-byte accesses at a 96-byte lane stride.
 template <int US, int SP>
 __global__ void __launch_bounds__(256) k_mdot_update(int ne, int negroup, const int* egroup_order,
+                                                     const int* etype,
                                                      const double* xyz, const int* conn,
                                                      const double* scvw, const double* U,
                                                      const double* du, const double* p,
@@ -1712,7 +1976,7 @@ __global__ void __launch_bounds__(256) k_mdot_update(int ne, int negroup, const 
     for (int k = l; k < 12 * cnt; k += 64) st[k] = mdot[12 * size_t(e0) + k];
     __syncthreads();
     int bad = 0;
-    if (l < cnt) {
+    if (l < cnt && !(etype && etype[e0 + l] != OAGPU_HEX8)) {  // others: k_mdot_generic
         const int e = e0 + l;
         int nd[8];
         load_conn(conn, e, nd);
@@ -1763,14 +2027,93 @@ __global__ void __launch_bounds__(256) k_mdot_update(int ne, int negroup, const 
     if (bad) *err = 1;
 }
 
+// mass flow rates of the tet4 / wedge6 / pyramid5 elements of a mixed mesh,
+// one lane per listed element of topology TOPO: init rho U_ip . A
+// (initializeMassFlowRateInterior_), else the Rhie-Chow rate relaxed with
+// urf (updateMassFlowRateInterior_), as k_mdot_init / k_mdot_update
+template <int TOPO>
+__global__ void __launch_bounds__(256) k_mdot_generic(int n, const int* glist, const int* etype,
+                                                      const double* xyz, const int* conn,
+                                                      const double* scvw, const double* U,
+                                                      const double* du, const double* p,
+                                                      const double* gradP, double rho, double urf,
+                                                      int us, int sp, int init, double* mdot,
+                                                      int* err) {
+    using Tr = topo::Traits<TOPO>;
+    constexpr int npe = Tr::npe;
+    const int i = int(xcd_block() * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const int e = glist[i];
+    if (etype[e] != TOPO) return;
+    int nd[npe];
+    for (int k = 0; k < npe; ++k) nd[k] = conn[8 * size_t(e) + k];
+    const double xr[3] = {xyz[3 * size_t(nd[0])], xyz[3 * size_t(nd[0]) + 1], xyz[3 * size_t(nd[0]) + 2]};
+    double c[npe][3];
+    for (int k = 0; k < npe; ++k)
+        for (int d = 0; d < 3; ++d) c[k][d] = xyz[3 * size_t(nd[k]) + d] - xr[d];
+    double Gp[3] = {0, 0, 0};
+    const double pr = p[nd[0]];
+    if (!init)
+        for (int k = 0; k < npe; ++k)
+            for (int d = 0; d < 3; ++d) Gp[d] += scvw[8 * size_t(e) + k] * gradP[3 * size_t(nd[k]) + d];
+    int bad = 0;
+#pragma unroll 1
+    for (int ip = 0; ip < Tr::nscs; ++ip) {
+        double A[3], g[npe][3], Nc[npe], Nu[npe];
+        topo::scs_ip<TOPO>(ip, sp, us, c, A, g, Nc, Nu, bad);
+        double uip[3] = {0, 0, 0}, duip[3] = {0, 0, 0}, dp[3] = {0, 0, 0}, rip = 0;
+        for (int k = 0; k < npe; ++k) {
+            const size_t o = 3 * size_t(nd[k]);
+            rip += Nu[k] * rho;
+            const double pn = p[nd[k]] - pr;
+            for (int j = 0; j < 3; ++j) {
+                uip[j] += Nu[k] * U[o + j];
+                duip[j] += Nu[k] * du[o + j];
+                dp[j] += g[k][j] * pn;
+            }
+        }
+        double t = 0;
+        if (init) {
+            for (int j = 0; j < 3; ++j) t += rip * uip[j] * A[j];
+            mdot[12 * size_t(e) + ip] = t;
+        } else {
+            for (int j = 0; j < 3; ++j) {
+                t += rho * uip[j] * A[j];
+                t -= rho * duip[j] * (dp[j] - Gp[j]) * A[j];
+            }
+            double& md = mdot[12 * size_t(e) + ip];
+            md = urf * t + (1.0 - urf) * md;
+        }
+    }
+    if (bad) *err = 1;
+}
+
+static void launch_mdot_generic(Flow& f, bool init) {
+    DevMesh& m = *f.m;
+    if (!m.nglist) return;
+    const unsigned B = 256, G = xcd_grid(size_t(m.nglist), B);
+    auto go = [&](auto kern) {
+        kern<<<G, B, 0, f.stream>>>(m.nglist, m.glist.p, m.etype.p, m.xyz.p, m.conn.p, m.scvw.p, f.U.p,
+                                    f.du.p, f.p.p, f.gradP.p, f.prm.rho, f.prm.urf_mdot,
+                                    f.prm.u_shifted ? 1 : 0, f.prm.pgrad_shifted ? 1 : 0,
+                                    init ? 1 : 0, f.mdot.p, m.err.p);
+    };
+    go(k_mdot_generic<topo::kTet>);
+    go(k_mdot_generic<topo::kWed>);
+    go(k_mdot_generic<topo::kPyr>);
+    OAGPU_HIP(hipGetLastError());
+}
+
 void launch_mdot_init(Flow& f) {
     DevMesh& m = *f.m;
     const unsigned B = 256, G = xcd_grid(m.ne, B);
+    const int* et = m.mixed ? m.etype.p : nullptr;
     if (f.prm.u_shifted)
-        k_mdot_init<1><<<G, B, 0, f.stream>>>(m.ne, m.xyz.p, m.conn.p, f.U.p, f.prm.rho, f.mdot.p);
+        k_mdot_init<1><<<G, B, 0, f.stream>>>(m.ne, m.xyz.p, m.conn.p, et, f.U.p, f.prm.rho, f.mdot.p);
     else
-        k_mdot_init<0><<<G, B, 0, f.stream>>>(m.ne, m.xyz.p, m.conn.p, f.U.p, f.prm.rho, f.mdot.p);
+        k_mdot_init<0><<<G, B, 0, f.stream>>>(m.ne, m.xyz.p, m.conn.p, et, f.U.p, f.prm.rho, f.mdot.p);
     OAGPU_HIP(hipGetLastError());
+    launch_mdot_generic(f, true);
     launch_side_mdot(f, true);
 }
 
@@ -1778,7 +2121,8 @@ void launch_update_mdot(Flow& f) {
     DevMesh& m = *f.m;
     const unsigned B = 256, G = xcd_grid(m.ne, B);
     auto args = [&](auto kern) {
-        kern<<<G, B, 0, f.stream>>>(m.ne, m.negroup, m.egroup_order.p, m.xyz.p, m.conn.p, m.scvw.p,
+        kern<<<G, B, 0, f.stream>>>(m.ne, m.negroup, m.egroup_order.p, m.mixed ? m.etype.p : nullptr,
+                                    m.xyz.p, m.conn.p, m.scvw.p,
                                     f.U.p, f.du.p, f.p.p, f.gradP.p, f.prm.rho, f.prm.urf_mdot,
                                     f.mdot.p, m.err.p);
     };
@@ -1788,6 +2132,7 @@ void launch_update_mdot(Flow& f) {
     else if (sp) args(k_mdot_update<0, 1>);
     else args(k_mdot_update<0, 0>);
     OAGPU_HIP(hipGetLastError());
+    launch_mdot_generic(f, false);
     launch_side_mdot(f, false);
 }
 
@@ -1830,7 +2175,37 @@ __device__ __forceinline__ void grad_incidence(const double* xyz, const int* con
     }
 }
 
-template <int N, int SH>
+// the same for a tet4 / wedge6 / pyramid5 element: its SCS ips with the row
+// node (local la) on either side, area vectors oriented away from the row
+template <int N, int SH, int TOPO>
+__device__ void grad_generic(const double* xyz, const int* conn, const double* phi, int la, int e,
+                             const double xr[3], const double pa[N], double iv, double gurf,
+                             double G[3 * N]) {
+    constexpr int npe = topo::Traits<TOPO>::npe;
+    int nd[npe];
+    double c[npe][3];
+    for (int k = 0; k < npe; ++k) {
+        nd[k] = conn[8 * size_t(e) + k];
+        for (int d = 0; d < 3; ++d) c[k][d] = xyz[3 * size_t(nd[k]) + d] - xr[d];
+    }
+#pragma unroll 1
+    for (int ip = 0; ip < topo::Traits<TOPO>::nscs; ++ip) {
+        const int* lr = topo::lr<TOPO>(ip);
+        if (lr[0] != la && lr[1] != la) continue;
+        const double sg = lr[0] == la ? 1.0 : -1.0;
+        double A[3], Nv[npe];
+        topo::scs_area<TOPO>(ip, c, A);
+        const double* l = topo::ip_loc<TOPO>(SH, ip);
+        topo::shape<TOPO>(SH, l[0], l[1], l[2], Nv);
+        for (int i = 0; i < N; ++i) {
+            double pip = 0.0;
+            for (int k = 0; k < npe; ++k) pip += Nv[k] * phi[N * size_t(nd[k]) + i];
+            for (int j = 0; j < 3; ++j) G[3 * i + j] += gurf * (pip - pa[i]) * (sg * A[j]) * iv;
+        }
+    }
+}
+
+template <int N, int SH, bool MX>
 __global__ void __launch_bounds__(256) k_gradient(int nn, int nslice, const int* slice_order,
                                                   const double* xyz, const int* conn,
                                                   const int* inc_la, const uint64_t* slots_la,
@@ -1840,7 +2215,8 @@ __global__ void __launch_bounds__(256) k_gradient(int nn, int nslice, const int*
                                                   const int* bf_ord, const int* bf_bc,
                                                   const int* bc_type, const double* symA,
                                                   const double* vol, const double* phi,
-                                                  double gurf, double* grad) {
+                                                  const int* gen_ptr, const int* gen,
+                                                  const int* etype, double gurf, double* grad) {
     const int a = ordered_item(slice_order, nslice, nn);
     if (a < 0) return;
     double G[3 * N];
@@ -1857,6 +2233,14 @@ __global__ void __launch_bounds__(256) k_gradient(int nn, int nslice, const int*
 #pragma unroll 1
     for (int k = ext_ptr[a]; k < ext_ptr[a + 1]; ++k)
         grad_incidence<N, SH>(xyz, conn, phi, ext[k] & 7, ext[k] >> 3, xr, pa, iv, gurf, G);
+    if constexpr (MX)
+#pragma unroll 1
+        for (int k = gen_ptr[a]; k < gen_ptr[a + 1]; ++k) {
+            const int e = gen[k] >> 3, la = gen[k] & 7, t = etype[e];
+            if (t == OAGPU_TET4) grad_generic<N, SH, topo::kTet>(xyz, conn, phi, la, e, xr, pa, iv, gurf, G);
+            else if (t == OAGPU_WED6) grad_generic<N, SH, topo::kWed>(xyz, conn, phi, la, e, xr, pa, iv, gurf, G);
+            else grad_generic<N, SH, topo::kPyr>(xyz, conn, phi, la, e, xr, pa, iv, gurf, G);
+        }
 #pragma unroll 1
     for (int k = binc_ptr[a]; k < binc_ptr[a + 1]; ++k) {
         const int f = binc[k] >> 2, fk = binc[k] & 3;
@@ -1921,13 +2305,48 @@ __device__ __forceinline__ void bj_limit(double& beta, double mx, double mn, dou
     if (rgrad > 100 * kSmall || rgrad < -100 * kSmall) beta = fmin(beta, y);
 }
 
-template <int N>
+// a tet4 / wedge6 / pyramid5 incidence of the blending pass: pass 0 folds
+// the SCS neighbours' values into mn / mx, pass 1 limits with the offset of
+// each SCS ip touching the node
+template <int N, int TOPO>
+__device__ void blend_generic(const double* xyz, const int* conn, const double* phi, int la, int e,
+                              bool limit, const double xa[3], const double pa[N], double mn[N],
+                              double mx[N], double b[N], double g[N][3]) {
+    constexpr int npe = topo::Traits<TOPO>::npe;
+#pragma unroll 1
+    for (int ip = 0; ip < topo::Traits<TOPO>::nscs; ++ip) {
+        const int* lr = topo::lr<TOPO>(ip);
+        if (lr[0] != la && lr[1] != la) continue;
+        if (!limit) {
+            const int other = conn[8 * size_t(e) + (lr[0] == la ? lr[1] : lr[0])];
+            for (int i = 0; i < N; ++i) {
+                const double v = phi[N * size_t(other) + i];
+                mn[i] = fmin(mn[i], v);
+                mx[i] = fmax(mx[i], v);
+            }
+            continue;
+        }
+        double Nv[npe];
+        const double* l = topo::ip_loc<TOPO>(0, ip);
+        topo::shape<TOPO>(0, l[0], l[1], l[2], Nv);
+        double dx[3];
+        for (int d = 0; d < 3; ++d) {
+            double xi = 0.0;
+            for (int n = 0; n < npe; ++n) xi += Nv[n] * xyz[3 * size_t(conn[8 * size_t(e) + n]) + d];
+            dx[d] = xi - xa[d];
+        }
+        for (int i = 0; i < N; ++i) bj_limit(b[i], mx[i], mn[i], pa[i], g[i], dx);
+    }
+}
+
+template <int N, bool MX>
 __global__ void __launch_bounds__(256) k_blending(int nn, const double* xyz, const int* conn,
                                                   const int* inc_la, const int* ext_ptr,
                                                   const int* ext, const int* binc_ptr,
                                                   const int* binc, const int* bf_elem,
                                                   const int* bf_ord, const double* phi,
-                                                  const double* grad, double* beta) {
+                                                  const double* grad, const int* gen_ptr,
+                                                  const int* gen, const int* etype, double* beta) {
     const int a = xcd_block() * blockDim.x + threadIdx.x;
     if (a >= nn) return;
     double pa[N], mn[N], mx[N], b[N], g[N][3];
@@ -1968,6 +2387,14 @@ __global__ void __launch_bounds__(256) k_blending(int nn, const double* xyz, con
         }
 #pragma unroll 1
         for (int k = ext_ptr[a]; k < ext_ptr[a + 1]; ++k) incidence(ext[k] & 7, ext[k] >> 3, pass == 1);
+        if constexpr (MX)
+#pragma unroll 1
+            for (int k = gen_ptr[a]; k < gen_ptr[a + 1]; ++k) {
+                const int e = gen[k] >> 3, la = gen[k] & 7, t = etype[e];
+                if (t == OAGPU_TET4) blend_generic<N, topo::kTet>(xyz, conn, phi, la, e, pass == 1, xa, pa, mn, mx, b, g);
+                else if (t == OAGPU_WED6) blend_generic<N, topo::kWed>(xyz, conn, phi, la, e, pass == 1, xa, pa, mn, mx, b, g);
+                else blend_generic<N, topo::kPyr>(xyz, conn, phi, la, e, pass == 1, xa, pa, mn, mx, b, g);
+            }
     }
 #pragma unroll 1
     for (int k = binc_ptr[a]; k < binc_ptr[a + 1]; ++k) {
@@ -1987,17 +2414,18 @@ __global__ void __launch_bounds__(256) k_blending(int nn, const double* xyz, con
 void launch_blending(Flow& f, int which) {
     DevMesh& m = *f.m;
     const unsigned B = 256, G = xcd_grid(m.nn, B);
+    const int* gp = m.mixed ? m.gen_ptr.p : nullptr;
     if (which == 0) {
         OAGPU_REQUIRE(f.beta.p, "blending factors not allocated (high resolution off)");
-        k_blending<3><<<G, B, 0, f.stream>>>(m.nn, m.xyz.p, m.conn.p, m.inc_la.p, m.ext_ptr.p, m.ext.p,
-                                             m.binc_ptr.p, m.binc.p, m.bf_elem.p, m.bf_ord.p, f.U.p,
-                                             f.gradU.p, f.beta.p);
+        (m.mixed ? k_blending<3, true> : k_blending<3, false>)<<<G, B, 0, f.stream>>>(
+            m.nn, m.xyz.p, m.conn.p, m.inc_la.p, m.ext_ptr.p, m.ext.p, m.binc_ptr.p, m.binc.p,
+            m.bf_elem.p, m.bf_ord.p, f.U.p, f.gradU.p, gp, m.gen.p, m.etype.p, f.beta.p);
     } else {
         OAGPU_REQUIRE((which == 2 || which == 3) && f.sst_on && f.beta_k.p, "bad blending request");
-        k_blending<1><<<G, B, 0, f.stream>>>(
+        (m.mixed ? k_blending<1, true> : k_blending<1, false>)<<<G, B, 0, f.stream>>>(
             m.nn, m.xyz.p, m.conn.p, m.inc_la.p, m.ext_ptr.p, m.ext.p, m.binc_ptr.p, m.binc.p,
             m.bf_elem.p, m.bf_ord.p, which == 2 ? f.k.p : f.om.p, which == 2 ? f.gradK.p : f.gradW.p,
-            which == 2 ? f.beta_k.p : f.beta_w.p);
+            gp, m.gen.p, m.etype.p, which == 2 ? f.beta_k.p : f.beta_w.p);
     }
     OAGPU_HIP(hipGetLastError());
 }
@@ -2054,11 +2482,16 @@ void launch_gradient(Flow& f, int which) {
         kern<<<G, B, 0, f.stream>>>(m.nn, m.nslice, m.slice_order.p, m.xyz.p, m.conn.p, m.inc_la.p,
                                     m.slots_la.p, m.ext_ptr.p, m.ext.p, m.ext_slots.p, m.binc_ptr.p,
                                     m.binc.p, m.bf_elem.p, m.bf_ord.p, m.bf_bc.p, types, sym,
-                                    m.vol.p, phi, f.prm.grad_urf, grad);
+                                    m.vol.p, phi, m.mixed ? m.gen_ptr.p : nullptr, m.gen.p,
+                                    m.etype.p, f.prm.grad_urf, grad);
+    };
+    auto pick = [&](auto kern_hex, auto kern_mixed, const double* phi, double* grad) {
+        if (m.mixed) go(kern_mixed, phi, grad);
+        else go(kern_hex, phi, grad);
     };
     if (which == 0) {
-        if (f.prm.u_shifted) go(k_gradient<3, 1>, f.U.p, f.gradU.p);
-        else go(k_gradient<3, 0>, f.U.p, f.gradU.p);
+        if (f.prm.u_shifted) pick(k_gradient<3, 1, false>, k_gradient<3, 1, true>, f.U.p, f.gradU.p);
+        else pick(k_gradient<3, 0, false>, k_gradient<3, 0, true>, f.U.p, f.gradU.p);
     } else {
         const double* phi = f.p.p;
         double* grad = f.gradP.p;
@@ -2069,8 +2502,8 @@ void launch_gradient(Flow& f, int which) {
             grad = which == 2 ? f.gradK.p : f.gradW.p;
             sh = which == 2 ? f.sst.k_shifted : f.sst.w_shifted;
         }
-        if (sh) go(k_gradient<1, 1>, phi, grad);
-        else go(k_gradient<1, 0>, phi, grad);
+        if (sh) pick(k_gradient<1, 1, false>, k_gradient<1, 1, true>, phi, grad);
+        else pick(k_gradient<1, 0, false>, k_gradient<1, 0, true>, phi, grad);
     }
     OAGPU_HIP(hipGetLastError());
 }

@@ -1018,7 +1018,7 @@ void or_outer_iteration(or_case *c, const double *ctl, double *hist)
     or_assemble_pcorr(c);
     or_scaled_residual(c, 1, &r1);
     memset(c->x1, 0, sizeof(double) * (size_t)c->nn);
-    const int itP = or_solve(c, 1, 1, (int)ctl[2], ctl[3], 1e-30, 30, res);
+    const int itP = or_solve(c, 1, P->p_method, (int)ctl[2], ctl[3], 1e-30, 30, res);
     or_correct_p(c, P->urf_p);
     /* segregatedFlowEquations.cpp:106-285 */
     or_update_mdot(c);

@@ -120,6 +120,8 @@ typedef struct or_params {
     int p_ref_node;          /* pressure-level node (-1: none)            */
     int high_res;            /* advection_scheme high_resolution: limited
                                 deferred correction (else upwind)         */
+    int p_method;            /* p' solver of or_outer_iteration: 0 GMRES,
+                                1 Jacobi-CG, 2 BiCGStab                   */
 } or_params;
 
 typedef struct or_case or_case;

@@ -25,7 +25,8 @@ class Params(C.Structure):
                 ("grad_urf", C.c_double), ("bnd_relax", C.c_double),
                 ("u_shifted", C.c_int), ("ugrad_shifted", C.c_int),
                 ("p_shifted", C.c_int), ("pgrad_shifted", C.c_int),
-                ("reduced", C.c_int), ("p_ref_node", C.c_int), ("high_res", C.c_int)]
+                ("reduced", C.c_int), ("p_ref_node", C.c_int), ("high_res", C.c_int),
+                ("p_method", C.c_int)]
 
 
 class SstParams(C.Structure):
@@ -146,7 +147,7 @@ def lib():
 def default_params(**kw) -> Params:
     p = Params(rho=1.0, mu=0.01, dt=1.0, urf_U=0.9, urf_p=0.1, urf_mdot=1.0,
                grad_urf=0.5625, bnd_relax=0.75, u_shifted=0, ugrad_shifted=1,
-               p_shifted=1, pgrad_shifted=1, reduced=1, p_ref_node=0, high_res=0)
+               p_shifted=1, pgrad_shifted=1, reduced=1, p_ref_node=0, high_res=0, p_method=1)
     for k, v in kw.items():
         setattr(p, k, v)
     return p

@@ -239,7 +239,8 @@ def bfs_case(oracle, oa, meshgen, precision=0):
     types = [INLET, OUTLET, WALL, SYM]
     kin, win = 1.5e-4, 0.75
     bcv = dict(p=[0.0] * 4, k=[kin, 0, 0, 0], w=[win, 0, 0, 0])
-    kw = dict(mu=1e-3, urf_U=0.7, urf_p=0.3, u_shifted=1)
+    # p' by BiCGStab: the outlet makes the p' matrix nonsymmetric
+    kw = dict(mu=1e-3, urf_U=0.7, urf_p=0.3, u_shifted=1, p_method=2)
     c = oracle.Case(m, oracle.default_params(**kw), BC)
     c.set_bc_types(types)
     c.set_bc_values(**bcv)
@@ -262,8 +263,14 @@ def bfs_case(oracle, oa, meshgen, precision=0):
 
 
 def test_bfs_sst_iterations(oracle, oa, meshgen):
+    """Tight linear solves, p' by BiCGStab: the step is mirror-symmetric in
+    z, and an unconverged Krylov solve (Jacobi-CG stalls on this
+    nonsymmetric p' system) leaves a z-antisymmetric part driven by rounding
+    whose sign differs between the two implementations (their states come
+    out as near mirror images with equal residual norms); solved tightly,
+    both give the same solution."""
     c, dm, f, m = bfs_case(oracle, oa, meshgen)
-    ctl, sctl = [20, 1e-2, 200, 1e-2], [20, 1e-2]
+    ctl, sctl = [60, 1e-8, 1000, 1e-10], [60, 1e-10]
     for it in range(3):
         # v and w start at zero: their first residuals are rounding (1e-20)
         ho, hg = c.outer_iteration(ctl), f.outer_iteration(ctl)
