@@ -8,6 +8,7 @@ over the whole mesh, inputs resident in HBM.  One JSON line is printed.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--n 128] [--perturb 0]
     python bench.py --workload bfs [--bfs-scale 1.0]   # 8M-element step, k-omega SST
+    python bench.py --workload duct [--duct-scale 1.0] # 3.9M-element mixed-topology duct
 """
 import argparse
 import json
@@ -222,6 +223,115 @@ def bench_bfs(args):
     print(json.dumps(line))
 
 
+# mixed-topology duct 1 x 0.2 x 0.2: plug inflow U = 1, static-pressure
+# outlet p = 0, no-slip walls, rho = 1, mu = 1e-3, high-resolution advection
+DUCT_M, DUCT_LAYERS = 64, (187, 8, 25, 75, 25)   # 3.87M elements: 60 % tet4, 21 % hex8, 15 % wedge6, 5 % pyramid5
+DUCT_MU = 1.0e-3
+DUCT_CTL = [20, 1.0e-2, 200, 1.0e-2]
+
+
+def duct_setup(case, mesh, device=True):
+    case.set_bc_types([1, 2, 0])      # inlet, outlet, no-slip walls
+    case.set_bc_values(p=[0.0] * 3)
+    nn = mesh["xyz"].shape[0]
+    U = np.zeros((nn, 3))
+    U[:, 0] = 1.0
+    for name, v in (("U", U), ("p", np.zeros(nn))):
+        if device:
+            case.upload(name, v)
+        else:
+            case.field(name)[:] = v
+    case.initialize()
+
+
+def run_duct(oa, dm, mesh, precision, steps, warmup):
+    bc = np.array([[1.0, 0.0, 0.0], [0.0] * 3, [0.0] * 3])
+    f = oa.FlowCase(dm, oa.default_params(precision, p_method=oa.OAGPU_CG_AMG, mu=DUCT_MU,
+                                          high_res=1), bc)
+    duct_setup(f, mesh)
+    hist = [f.outer_iteration(DUCT_CTL) for _ in range(warmup)]
+    oa.load_library().oagpu_synchronize()
+    ms = {k: 0.0 for k in ("momentum", "pcorr", "mdot", "gradU")}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hist.append(f.outer_iteration(DUCT_CTL))
+        for k in ms:
+            ms[k] += f.kernel_ms(k) / steps
+    oa.load_library().oagpu_synchronize()
+    t = time.perf_counter() - t0
+    f.close()
+    return {"seconds": t, "kernel_ms": ms, "hist": np.array(hist)}
+
+
+def duct_cpu_baseline(m):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from openaccel_amd import meshgen
+    layers = tuple(max(1, round(x * m / DUCT_M)) for x in DUCT_LAYERS)
+    mesh = meshgen.duct_mixed(m, layers=layers, length=sum(layers) * 0.2 / m, seed=1)
+    bc = np.array([[1.0, 0.0, 0.0], [0.0] * 3, [0.0] * 3])
+    c = pyoracle.Case(mesh, pyoracle.default_params(mu=DUCT_MU, high_res=1), bc)
+    duct_setup(c, mesh, device=False)
+    t0 = time.perf_counter()
+    c.update_blending("U")
+    c.outer_iteration(DUCT_CTL)
+    return {"outer_s": time.perf_counter() - t0, "ne": c.ne, "layers": layers}
+
+
+def bench_duct(args):
+    import openaccel_amd as oa
+    from openaccel_amd import meshgen
+    oa.init(0)
+    m = max(8, int(round(DUCT_M * args.duct_scale)))
+    layers = tuple(max(1, round(x * m / DUCT_M)) for x in DUCT_LAYERS)
+    mesh = meshgen.duct_mixed(m, layers=layers, length=sum(layers) * 0.2 / m, seed=1)
+    ne, nn = mesh["conn"].shape[0], mesh["xyz"].shape[0]
+    counts = np.bincount(mesh["etype"], minlength=4)
+    print(f"[bench] mixed duct m={m} layers={layers}: {ne} elements "
+          f"(hex {counts[0]}, tet {counts[1]}, wedge {counts[2]}, pyramid {counts[3]})",
+          file=sys.stderr, flush=True)
+    dm = oa.DeviceMesh(mesh)
+    r64 = run_duct(oa, dm, mesh, oa.OAGPU_FP64, args.steps, args.warmup)
+    print(f"[bench] fp64 {args.steps / r64['seconds']:.3f} it/s", file=sys.stderr, flush=True)
+    rmx = run_duct(oa, dm, mesh, oa.OAGPU_MIXED, args.steps, args.warmup)
+    print(f"[bench] mixed {args.steps / rmx['seconds']:.3f} it/s", file=sys.stderr, flush=True)
+    it64, itmx = args.steps / r64["seconds"], args.steps / rmx["seconds"]
+    cpu = None
+    if not args.no_cpu:
+        cb = duct_cpu_baseline(32)
+        scale = cb["ne"] / ne
+        cpu = {"value": round(scale / cb["outer_s"], 6), "unit": "outer iterations/s", "cores": 1,
+               "kind": "port",
+               "sample": f"oracle C restatement, single thread, 1 outer iteration of the same duct "
+                         f"at m=16 layers {cb['layers']} ({cb['ne']} elements, {cb['outer_s']:.2f} s) "
+                         f"scaled by the element ratio {scale:.5f}"}
+    line = {
+        "metric": METRIC,
+        "value": round(itmx, 4),
+        "unit": "outer iterations/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * rmx["seconds"] / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 local / f64 global", "data": "synthetic",
+        "config": {"workload": f"mixed-topology duct 1 x 0.2 x 0.2, {ne} elements ({counts[1]} tet4, "
+                               f"{counts[0]} hex8, {counts[2]} wedge6, {counts[3]} pyramid5), node "
+                               f"jitter N(0, 0.1 h); plug inflow U=1, outlet p=0, rho=1, "
+                               f"mu={DUCT_MU}, high-resolution advection; mixed form",
+                   "m": m, "layers": layers, "elements": ne, "nodes": nn},
+        "fp64": {"value": round(it64, 4), "ms_per_step": round(1e3 * r64["seconds"] / args.steps, 3),
+                 "kernel_ms": {k: round(v, 4) for k, v in r64["kernel_ms"].items()},
+                 "res_U_p": [[float(x) for x in h[:4]] for h in r64["hist"]],
+                 "krylov_iters": [[int(h[4]), int(h[5])] for h in r64["hist"]]},
+        "mixed_over_fp64": round(itmx / it64, 3),
+        "kernel_ms": {k: round(v, 4) for k, v in rmx["kernel_ms"].items()},
+        "kernel_Melem_s": {k: round(ne / (v * 1e-3) / 1e6, 1) for k, v in rmx["kernel_ms"].items() if v > 0},
+        "res_U_p": [[float(x) for x in h[:4]] for h in rmx["hist"]],
+        "krylov_iters": [[int(h[4]), int(h[5])] for h in rmx["hist"]],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
 def cpu_baseline(n_sample, perturb):
     """Oracle (single-thread C restatement) on a bounded sample of the same
     workload: the same cavity at n_sample^3, one outer iteration timed after
@@ -253,9 +363,12 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=64, help="oracle sample grid (elements scale)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once (calibrates HBM byte counters)")
-    ap.add_argument("--workload", choices=("cavity", "bfs"), default="cavity",
+    ap.add_argument("--workload", choices=("cavity", "bfs", "duct"), default="cavity",
                     help="cavity: the lid-driven cavity (default); bfs: the 8M-element "
-                         "backward-facing step with k-omega SST")
+                         "backward-facing step with k-omega SST; duct: the 4M-element "
+                         "mixed-topology duct")
+    ap.add_argument("--duct-scale", type=float, default=1.0,
+                    help="scale of the duct's cube counts per direction (1: 3.9M elements)")
     ap.add_argument("--bfs-scale", type=float, default=1.0,
                     help="scale of the step's cell counts per direction (1: 8M elements)")
     args = ap.parse_args()
@@ -278,6 +391,9 @@ def main():
 
     if args.workload == "bfs":
         bench_bfs(args)
+        return
+    if args.workload == "duct":
+        bench_duct(args)
         return
 
     import openaccel_amd as oa

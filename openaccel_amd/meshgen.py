@@ -198,7 +198,9 @@ def duct_mixed(m: int, layers=(187, 8, 25, 75, 25), length: float = 1.0, width: 
 
     Interior grid nodes move by N(0, jitter*h) per component, clipped at
     +-2.5 jitter*h; boundary nodes stay; centre nodes sit at their cube's
-    corner mean.  Boundary ids DUCT_INLET (x = 0), DUCT_OUTLET (x = length),
+    corner mean.  Where that leaves a tet or pyramid with a volume below
+    h^3 / 100, the moves of its grid nodes are halved until none is left.
+    Boundary ids DUCT_INLET (x = 0), DUCT_OUTLET (x = length),
     DUCT_WALL.  Returns xyz, conn [E][8] (-1 padded), etype, bf_elem,
     bf_ord, bf_bc."""
     H, P, T, W = 0, 3, 1, 2
@@ -216,13 +218,14 @@ def duct_mixed(m: int, layers=(187, 8, 25, 75, 25), length: float = 1.0, width: 
     g1 = nx + 1
     gid = lambda ii, jj, kk: ii + g1 * (jj + (ny + 1) * kk)
     kk_, jj_, ii_ = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
-    xyz = np.stack([ii_.ravel() * h, jj_.ravel() * h, kk_.ravel() * h], axis=1)
+    xyz0 = np.stack([ii_.ravel() * h, jj_.ravel() * h, kk_.ravel() * h], axis=1)
+    d = np.zeros_like(xyz0)
     if jitter > 0:
         rng = np.random.default_rng(seed)
-        d = np.clip(rng.normal(0.0, jitter * h, size=xyz.shape), -2.5 * jitter * h, 2.5 * jitter * h)
+        d = np.clip(rng.normal(0.0, jitter * h, size=xyz0.shape), -2.5 * jitter * h, 2.5 * jitter * h)
         on_b = ((ii_ == 0) | (ii_ == nx) | (jj_ == 0) | (jj_ == ny) | (kk_ == 0) | (kk_ == nz)).ravel()
         d[on_b] = 0.0
-        xyz = xyz + d
+    xyz = xyz0 + d
     ng = xyz.shape[0]
     ci, cj, ck = i.ravel(), j.ravel(), k.ravel()
     ct = ctype.ravel()
@@ -275,6 +278,24 @@ def duct_mixed(m: int, layers=(187, 8, 25, 75, 25), length: float = 1.0, width: 
                     add(np.stack([tr[:, 0], tr[:, 2], tr[:, 1], ta], axis=1), TET4)
     conn = np.concatenate(elems).astype(np.int32)
     etype = np.concatenate(types)
+
+    def tet_vol(a, b, cc, dd):
+        return np.einsum("ij,ij->i", np.cross(b - a, cc - a), dd - a) / 6.0
+
+    for _ in range(20):   # halve the moves around too-thin tets and pyramids
+        ct4, cp5 = conn[etype == TET4], conn[etype == PYR5]
+        v = [tet_vol(*(xyz[ct4[:, k]] for k in range(4)))]
+        q = [xyz[cp5[:, k]] for k in range(5)]
+        v += [tet_vol(q[0], q[1], q[2], q[4]), tet_vol(q[0], q[2], q[3], q[4]),
+              tet_vol(q[0], q[1], q[3], q[4]), tet_vol(q[1], q[2], q[3], q[4])]
+        bad = [ct4[v[0] < 1e-2 * h ** 3]] + [cp5[vk < 0.5e-2 * h ** 3] for vk in v[1:]]
+        nodes = np.unique(np.concatenate([b.ravel() for b in bad]))
+        nodes = nodes[nodes < ng]
+        if nodes.size == 0:
+            break
+        d[nodes] *= 0.5
+        xyz[:ng] = xyz0 + d
+        xyz[ng:] = xyz[corner[has_c]].mean(axis=1)
     # boundary faces: sides of hex cubes on the domain boundary
     faces = []
     for side, (di, dj, dk) in enumerate(_SIDE_NB):
