@@ -69,8 +69,8 @@ typedef struct oagpu_flow_params {
     int pgrad_shifted;     /* pressure gradient interpolation linear_linear */
     int p_ref_node;        /* pressure-level node id, -1 for none           */
     int precision;         /* OAGPU_FP64 or OAGPU_MIXED                     */
-    int p_method;          /* p' solver of oagpu_outer_iteration: OAGPU_CG or
-                              OAGPU_CG_AMG                                  */
+    int p_method;          /* p' solver of oagpu_outer_iteration: OAGPU_CG,
+                              OAGPU_CG_AMG, OAGPU_BICGSTAB(_AMG), OAGPU_GMRES */
     int use_graphs;        /* 1: CG iterations replayed from a captured HIP
                               graph; 0: launched one by one (same results)  */
     int high_res;          /* advection_scheme high_resolution: limited
@@ -105,6 +105,10 @@ typedef struct oagpu_control_data {
 #define OAGPU_CG_AMG 3   /* CG preconditioned by one aggregation-AMG V-cycle
                             (scalar systems; the role of HYPRE BoomerAMG in
                             liblinsolve's hypre family, linearSystem.h:331) */
+#define OAGPU_BICGSTAB_AMG 4 /* BiCGStab right-preconditioned by one AMG
+                            V-cycle (scalar systems that are not symmetric:
+                            p' with static-pressure outlets); in the mixed
+                            form the V-cycle runs on the FP32 hierarchy */
 
 /* ---------------------------------------------------------------- runtime */
 int oagpu_init(int device);

@@ -20,6 +20,7 @@
 
 #include <array>
 #include <cmath>
+#include <type_traits>
 
 namespace oagpu {
 
@@ -649,6 +650,127 @@ static int bicgstab(Flow& f, int max_it, double rtol, double atol, oagpu_control
     return it;
 }
 
+// ------------------------------------------------------ AMG-BiCGStab
+__global__ void k_to_double(size_t n, const float* x, double* y) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = double(x[i]);
+}
+
+__global__ void k_axpby_add(size_t n, double a, const double* x, double b, const double* y,
+                            double* out) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += a * x[i] + b * y[i];
+}
+
+// z = M^-1 r, one V-cycle on the hierarchy of value type V (FP32: r and z
+// converted at the boundary); the V-cycle's r.z partials go to a spare slot
+template <typename V>
+static void amg_apply(Flow& f, const V* A, const V* dinv, const double* r, double* z) {
+    const size_t n = size_t(f.m->nn);
+    double* part = f.part.p + 8 * kRedGrid;
+    if constexpr (std::is_same_v<V, double>) {
+        amg_vcycle<double>(f, A, dinv, r, z, part, nullptr);
+    } else {
+        const unsigned B = 256, G = unsigned((n + B - 1) / B);
+        k_to_float<<<G, B, 0, f.stream>>>(n, 1.0, r, f.fr.p);
+        amg_vcycle<float>(f, A, dinv, f.fr.p, f.fz.p, part, nullptr);
+        k_to_double<<<G, B, 0, f.stream>>>(n, f.fz.p, z);
+    }
+    OAGPU_HIP(hipGetLastError());
+}
+
+// BiCGStab (Saad Alg. 7.7) right-preconditioned by one AMG V-cycle: the
+// p' systems of flows with static-pressure outlets are not symmetric, where
+// AMG-CG stalls.  FP64 vectors and FP64 matrix products; the V-cycle on the
+// FP64 hierarchy, or in the mixed form on the FP32 one.
+template <typename V>
+static int bicgstab_amg(Flow& f, const V* Ah, const V* dinvh, int max_it, double rtol, double atol,
+                        oagpu_control_data* cd) {
+    DevMesh& m = *f.m;
+    const size_t n = size_t(m.nn);
+    const double* A = f.A1.p;
+    const double* b = f.b1.p;
+    double* x = f.x1.p;
+    double *ph = f.z.p, *sh = f.w.p;   // M^-1 p, M^-1 s
+    const unsigned B = 256, G = unsigned((n + B - 1) / B);
+    k_clear_flag<<<1, 1, 0, f.stream>>>(f.flag.p);  // the V-cycle kernels honour the CG stop flag
+    OAGPU_HIP(hipMemsetAsync(x, 0, n * sizeof(double), f.stream));
+    OAGPU_HIP(hipMemcpyAsync(f.r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, f.stream));
+    OAGPU_HIP(hipMemcpyAsync(f.r0.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, f.stream));
+    OAGPU_HIP(hipMemsetAsync(f.pk.p, 0, n * sizeof(double), f.stream));
+    OAGPU_HIP(hipMemsetAsync(f.q.p, 0, n * sizeof(double), f.stream));  // v
+    double rn;
+    {
+        const double* xs[1] = {f.r.p};
+        dots(f, n, 1, xs, xs, &rn);
+        rn = std::sqrt(rn);
+    }
+    cd->solver_initial_res = rn;
+    const double tol = std::max(rtol * rn, atol);
+    double rho = 1, alpha = 1, omega = 1;
+    int it = 0;
+    while (it < max_it && rn > tol) {
+        double rho1;
+        {
+            const double* xs[1] = {f.r0.p};
+            const double* ys[1] = {f.r.p};
+            dots(f, n, 1, xs, ys, &rho1);
+        }
+        const double beta = (rho1 / rho) * (alpha / omega);
+        rho = rho1;
+        k_bicg_p<<<G, B, 0, f.stream>>>(n, beta, omega, f.r.p, f.q.p, f.pk.p);
+        amg_apply<V>(f, Ah, dinvh, f.pk.p, ph);
+        spmv<1>(f, A, ph, nullptr, f.q.p);  // v = A M^-1 p
+        double r0v;
+        {
+            const double* xs[1] = {f.r0.p};
+            const double* ys[1] = {f.q.p};
+            dots(f, n, 1, xs, ys, &r0v);
+        }
+        alpha = rho / r0v;
+        k_lin2<<<G, B, 0, f.stream>>>(n, 1.0, f.r.p, -alpha, f.q.p, f.s.p);
+        amg_apply<V>(f, Ah, dinvh, f.s.p, sh);
+        spmv<1>(f, A, sh, nullptr, f.t.p);  // t = A M^-1 s
+        double tt_ts[2];
+        {
+            const double* xs[2] = {f.t.p, f.t.p};
+            const double* ys[2] = {f.t.p, f.s.p};
+            dots(f, n, 2, xs, ys, tt_ts);
+        }
+        omega = tt_ts[0] > 0 ? tt_ts[1] / tt_ts[0] : 0.0;
+        k_axpby_add<<<G, B, 0, f.stream>>>(n, alpha, ph, omega, sh, x);
+        k_lin2<<<G, B, 0, f.stream>>>(n, 1.0, f.s.p, -omega, f.t.p, f.r.p);
+        ++it;
+        const double* xs[1] = {f.r.p};
+        dots(f, n, 1, xs, xs, &rn);
+        rn = std::sqrt(rn);
+        if (omega == 0.0) break;
+    }
+    f.amg_vcycles += 2 * it;
+    cd->solver_final_res = rn;
+    return it;
+}
+
+static int bicgstab_amg_solve(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
+    DevMesh& m = *f.m;
+    const size_t n = size_t(m.nn);
+    if (f.prm.precision == OAGPU_MIXED) {
+        const size_t nv = m.nsell;
+        if (f.Af.n < nv) f.Af.alloc(nv);
+        for (auto* v : {&f.fb, &f.fx, &f.fr, &f.fz, &f.fp, &f.fq, &f.fdinv})
+            if (v->n < n) v->alloc(n);
+        const unsigned B = 256;
+        k_to_float<<<unsigned((nv + B - 1) / B), B, 0, f.stream>>>(nv, 1.0, f.A1.p, f.Af.p);
+        k_to_float<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, 1.0, f.dinv.p, f.fdinv.p);
+        amg_setup<float>(f, f.Af.p, f.fdinv.p);
+        if (f.amg.lv.size() > 1) return bicgstab_amg<float>(f, f.Af.p, f.fdinv.p, max_it, rtol, atol, cd);
+    } else {
+        amg_setup<double>(f, f.A1.p, f.dinv.p);
+        if (f.amg.lv.size() > 1) return bicgstab_amg<double>(f, f.A1.p, f.dinv.p, max_it, rtol, atol, cd);
+    }
+    return bicgstab<1>(f, max_it, rtol, atol, cd);  // no coarse level (tiny system): Jacobi
+}
+
 int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double atol, int restart,
                  oagpu_control_data* cd) {
     DevMesh& m = *f.m;
@@ -676,7 +798,10 @@ int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double ato
     }
     else if (method == OAGPU_BICGSTAB)
         it = N == 3 ? bicgstab<3>(f, max_it, rtol, atol, cd) : bicgstab<1>(f, max_it, rtol, atol, cd);
-    else
+    else if (method == OAGPU_BICGSTAB_AMG) {
+        OAGPU_REQUIRE(N == 1, "AMG preconditioning is for scalar systems (N = 1)");
+        it = bicgstab_amg_solve(f, max_it, rtol, atol, cd);
+    } else
         throw Error(OAGPU_EINVAL, "unknown Krylov method");
     cd->n_iterations = it;
     return it;

@@ -160,3 +160,44 @@ def test_graph_replay_matches_eager(oa, meshgen, precision, method):
     for (i0, x0), (i1, x1) in zip(out[0], out[1]):
         assert i0 == i1
         assert np.array_equal(x0, x1)
+
+
+def outlet_pcorr_case(oa, meshgen, n, perturb, seed, precision=0):
+    """p' system of a channel with a velocity inlet and a static-pressure
+    outlet: not symmetric (the outlet rows drop their face columns)."""
+    print(f"[case] amg outlet n={n} perturb={perturb} seed={seed} precision={precision}", flush=True)
+    m = meshgen.cavity_hex(n, perturb=perturb, seed=seed)
+    bc = np.full(m["bf_bc"].shape, 2, dtype=np.int32)
+    bc[m["bf_ord"] == 3] = 0
+    bc[m["bf_ord"] == 1] = 1
+    m["bf_bc"] = bc
+    dm = oa.DeviceMesh(m)
+    f = oa.FlowCase(dm, oa.default_params(precision), np.array([[1.0, 0, 0], [0.0] * 3, [0.0] * 3]))
+    f.set_bc_types([oa.OAGPU_BC_INLET, oa.OAGPU_BC_OUTLET, oa.OAGPU_BC_WALL])
+    f.set_bc_values(p=[0.0, 0.0, 0.0])
+    rng = np.random.default_rng(seed + 200)
+    U = rng.normal(scale=0.2, size=(dm.nn, 3))
+    U[:, 0] += 1.0
+    f.upload("U", U)
+    f.initialize()
+    f.upload("du", rng.uniform(1e-3, 2e-3, size=(dm.nn, 3)))
+    f.update_mdot()
+    f.assemble_pcorr()
+    rp, cols, _ = dm.graph()
+    vals, b = f.system(1)
+    A = sp.csr_matrix((vals.ravel(), cols, rp), shape=(dm.nn, dm.nn))
+    return dm, f, A, b
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_amg_bicgstab_nonsymmetric(oa, meshgen, precision):
+    dm, f, A, b = outlet_pcorr_case(oa, meshgen, 16, 0.15, 7, precision)
+    assert abs(A - A.T).max() > 1e-3 * abs(A).max()        # really not symmetric
+    xs = spla.spsolve(A.tocsc(), b)
+    cd = f.solve(1, oa.OAGPU_BICGSTAB_AMG, 200, 1e-10)
+    x = f.solution(1)
+    assert cd.solver_final_res <= 1e-10 * cd.solver_initial_res * 1.0001
+    assert np.linalg.norm(b - A @ x) <= 1.01e-10 * np.linalg.norm(b)
+    assert rel_err(x, xs) < 1e-7
+    cdj = f.solve(1, oa.OAGPU_BICGSTAB, 3000, 1e-10)      # Jacobi-BiCGStab
+    assert cd.n_iterations * 3 < cdj.n_iterations, (cd.n_iterations, cdj.n_iterations)
