@@ -9,6 +9,7 @@ over the whole mesh, inputs resident in HBM.  One JSON line is printed.
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--n 128] [--perturb 0]
     python bench.py --workload bfs [--bfs-scale 1.0]   # 8M-element step, k-omega SST
     python bench.py --workload duct [--duct-scale 1.0] # 3.9M-element mixed-topology duct
+    python bench.py --workload batch                   # 96 independent cavities, batched
 """
 import argparse
 import json
@@ -333,6 +334,88 @@ def bench_duct(args):
     print(json.dumps(line))
 
 
+# batched independent cavities: sizes from {16, 24, 32, 48, 64}^3, Re
+# log-uniform in [10, 1000] (mu = 1 / Re, unit lid speed and cube), the lid
+# velocity a random unit vector in the lid plane, all seeded
+BATCH_N, BATCH_SIZES, BATCH_ITERS = 96, (16, 24, 32, 48, 64), 20
+
+
+def batch_cases(n_cases, seed=7):
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice(BATCH_SIZES, size=n_cases)
+    re = 10.0 ** rng.uniform(1.0, 3.0, size=n_cases)
+    th = rng.uniform(0.0, 2.0 * np.pi, size=n_cases)
+    return [dict(n=int(n), mu=float(1.0 / r), lid=[float(np.cos(t)), 0.0, float(np.sin(t))])
+            for n, r, t in zip(sizes, re, th)]
+
+
+def bench_batch(args):
+    import openaccel_amd as oa
+    from openaccel_amd import meshgen
+    oa.init(0)
+    cases = batch_cases(args.batch_cases)
+    meshes = {n: oa.DeviceMesh(meshgen.cavity_hex(n)) for n in sorted({c["n"] for c in cases})}
+    elems = sum(c["n"] ** 3 for c in cases)
+    print(f"[bench] batch of {len(cases)} cavities, {elems} elements in all", file=sys.stderr, flush=True)
+
+    def flow(c, precision):
+        dm = meshes[c["n"]]
+        f = oa.FlowCase(dm, oa.default_params(precision, mu=c["mu"], p_method=oa.OAGPU_CG_AMG),
+                        np.array([c["lid"], [0.0, 0.0, 0.0]]))
+        f.upload("U", meshgen.initial_velocity(dm.nn, c["n"]))
+        f.initialize()
+        return f
+
+    iters = args.batch_iters
+    # FP64, one case at a time
+    seq = np.zeros(len(cases))
+    oa.load_library().oagpu_synchronize()
+    t0 = time.perf_counter()
+    for k, c in enumerate(cases):
+        f = flow(c, oa.OAGPU_FP64)
+        t1 = time.perf_counter()
+        for _ in range(iters):
+            f.outer_iteration(CTL)
+        seq[k] = time.perf_counter() - t1
+        f.close()
+    t_seq = seq.sum()
+    print(f"[bench] fp64 one at a time: {t_seq:.2f} s of iterations ({time.perf_counter() - t0:.1f} s "
+          f"with set-up)", file=sys.stderr, flush=True)
+    # mixed, batched
+    flows = [flow(c, oa.OAGPU_MIXED) for c in cases]
+    oa.load_library().oagpu_synchronize()
+    t0 = time.perf_counter()
+    hist, done, sec = oa.batch_outer_iterations(flows, CTL, iters, threads=16)
+    t_batch = time.perf_counter() - t0
+    print(f"[bench] mixed batched: {t_batch:.2f} s", file=sys.stderr, flush=True)
+    res = np.array([h[-1][:4].max() for h in hist])
+    line = {
+        "metric": METRIC,
+        "value": round(len(cases) / t_batch, 4),
+        "unit": "cases/s",
+        "n_gpus": 1, "steps": iters, "warmup": 0,
+        "ms_per_step": round(1e3 * t_batch / iters, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 local / f64 global", "data": "synthetic",
+        "config": {"workload": f"{len(cases)} independent lid-driven cavities, sizes from "
+                               f"{BATCH_SIZES}^3 hex8, Re log-uniform in [10, 1000], lid velocity a "
+                               f"random unit vector in the lid plane (seeded); {iters} outer "
+                               f"iterations each; mixed form batched on 16 host threads (one "
+                               f"stream per case) vs FP64 one case at a time",
+                   "cases": len(cases), "elements": elems, "iterations_per_case": iters},
+        "fp64_one_at_a_time": {"value": round(len(cases) / t_seq, 4), "unit": "cases/s",
+                               "per_case_s_p50": round(float(np.percentile(seq, 50)), 4),
+                               "per_case_s_p99": round(float(np.percentile(seq, 99)), 4)},
+        "batched_over_one_at_a_time": round(t_seq / t_batch, 3),
+        "per_case_s_p50": round(float(np.percentile(sec, 50)), 4),
+        "per_case_s_p99": round(float(np.percentile(sec, 99)), 4),
+        "residual_after_iters_p50": float(np.percentile(res, 50)),
+        "residual_after_iters_max": float(res.max()),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
 def cpu_baseline(n_sample, perturb):
     """Oracle (single-thread C restatement) on a bounded sample of the same
     workload: the same cavity at n_sample^3, one outer iteration timed after
@@ -364,7 +447,9 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=64, help="oracle sample grid (elements scale)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once (calibrates HBM byte counters)")
-    ap.add_argument("--workload", choices=("cavity", "bfs", "duct"), default="cavity",
+    ap.add_argument("--batch-cases", type=int, default=BATCH_N)
+    ap.add_argument("--batch-iters", type=int, default=BATCH_ITERS)
+    ap.add_argument("--workload", choices=("cavity", "bfs", "duct", "batch"), default="cavity",
                     help="cavity: the lid-driven cavity (default); bfs: the 8M-element "
                          "backward-facing step with k-omega SST; duct: the 4M-element "
                          "mixed-topology duct")
@@ -395,6 +480,9 @@ def main():
         return
     if args.workload == "duct":
         bench_duct(args)
+        return
+    if args.workload == "batch":
+        bench_batch(args)
         return
 
     import openaccel_amd as oa

@@ -247,6 +247,20 @@ int oagpu_amg_level_copy(oagpu_flow *f, int level, int *agg, int *row_ptr, int *
  * hist[6] = U, V, W, P control residuals, momentum / p' Krylov iterations. */
 int oagpu_outer_iteration(oagpu_flow *f, const double *ctl, double *hist);
 
+/* Batched independent cases (many small flows of any meshes, sizes and
+ * parameters): each of the n flows is advanced by up to `iters` outer
+ * iterations (oagpu_outer_iteration, in its own HIP stream) on a pool of
+ * `threads` host threads (<= 0: min(n, 16)), so that the cases' kernels
+ * overlap on the GPU.  A case stops early once all four scaled residuals
+ * it reports fall below tol (tol <= 0: never).  Outputs (NULL: not wanted):
+ * hist [n][iters][6] as oagpu_outer_iteration's, done [n] iterations run,
+ * seconds [n] wall time of each case.  Every case runs; the first failing
+ * case's status and message are returned.  No counterpart call in the
+ * reference, which runs one case per process. */
+int oagpu_batch_outer_iterations(oagpu_flow *const *flows, int n, const double *ctl, int iters,
+                                 double tol, int threads, double *hist, int *done,
+                                 double *seconds);
+
 /* ---------------------------------------------------------- k-omega SST */
 /* allocate the SST fields: "k", "omega", "gradK", "gradW", "ymin", "mut",
  * "F1", "Pk", "Pk_wall", "kprev"; momentum then uses "mu" = mu + mu_t */

@@ -113,6 +113,8 @@ _SIGS = {
     "oagpu_flow_set_bc_types": [C.c_void_p, C.c_int, _ip],
     "oagpu_flow_set_bc_values": [C.c_void_p, C.c_int, _dp, _dp, _dp],
     "oagpu_update_reversal": [C.c_void_p],
+    "oagpu_batch_outer_iterations": [C.POINTER(C.c_void_p), C.c_int, _dp, C.c_int, C.c_double,
+                                     C.c_int, _dp, _ip, _dp],
     "oagpu_reversal_download": [C.c_void_p, _ip, C.c_size_t],
     "oagpu_sst_enable": [C.c_void_p, C.POINTER(SstParams)],
     "oagpu_sst_update": [C.c_void_p],
@@ -188,6 +190,21 @@ def default_params(precision: int = OAGPU_FP64, **kw) -> FlowParams:
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+def batch_outer_iterations(flows, ctl, iters, tol=0.0, threads=0):
+    """Advance independent FlowCases concurrently (oagpu_batch_outer_iterations):
+    returns hist [n][iters][6], iterations run [n], wall seconds [n]."""
+    n = len(flows)
+    hs = (C.c_void_p * max(n, 1))(*[f.h.value if hasattr(f.h, "value") else f.h for f in flows])
+    c = np.ascontiguousarray(ctl, dtype=np.float64)
+    hist = np.zeros((max(n, 1), max(iters, 1), 6))
+    done = np.zeros(max(n, 1), np.int32)
+    sec = np.zeros(max(n, 1))
+    _check(load_library().oagpu_batch_outer_iterations(
+        C.cast(hs, C.POINTER(C.c_void_p)), n, _d(c), iters, tol, threads, _d(hist), _i(done),
+        _d(sec)))
+    return hist[:n, :iters], done[:n], sec[:n]
 
 
 class DeviceMesh:

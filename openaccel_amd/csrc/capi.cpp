@@ -3,9 +3,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "flow.hpp"
 #include "sell.hpp"
@@ -721,45 +726,104 @@ int oagpu_amg_level_copy(oagpu_flow* h, int level, int* agg, int* row_ptr, int* 
     });
 }
 
+namespace {
 // simulation::assembleAndSolveSystems -> segregatedFlowEquations::solve
 // (segregatedFlowEquations.cpp:76-297) for one p' sub-iteration
+void run_outer_iteration(oagpu::Flow& f, const double* ctl, double* hist) {
+    const double us0 = f.U_scale, ps0 = f.p_scale;
+    const double kSmall = 2.220446049250313e-16;
+    oagpu_control_data cdU{}, cdP{};
+    double rU[3], rP[1];
+    // momentum predictor (navierStokesEquation::solve :157-222)
+    f.ms_momentum = timed(f, [&] { oagpu::launch_assemble_momentum(f); });
+    oagpu::scaled_residual(f, 3, rU);
+    oagpu::krylov_solve(f, 3, OAGPU_GMRES, int(ctl[0]), ctl[1], 1e-30, 30, &cdU);
+    oagpu::launch_correct(f, 3, 1.0);
+    f.ms_mdot = timed(f, [&] { oagpu::launch_update_mdot(f); });
+    oagpu::field_scales(f, nullptr, nullptr);
+    // pressure correction (pressureCorrectionEquation::solve :89-164)
+    f.ms_pcorr = timed(f, [&] { oagpu::launch_assemble_pcorr(f); });
+    oagpu::scaled_residual(f, 1, rP);
+    oagpu::krylov_solve(f, 1, f.prm.p_method, int(ctl[2]), ctl[3], 1e-30, 30, &cdP);
+    oagpu::launch_correct(f, 1, f.prm.urf_p);
+    // mass flow, divergence, velocity correction, gradients (:106-285)
+    oagpu::launch_update_mdot(f);
+    if (f.has_outlet) oagpu::launch_reversal(f);  // segregatedFlowEquations.cpp:121
+    oagpu::launch_mass_divergence(f);
+    oagpu::launch_velocity_correction(f, 1);
+    oagpu::launch_gradient(f, 1);
+    oagpu::launch_velocity_correction(f, 2);
+    oagpu::field_scales(f, nullptr, nullptr);
+    f.ms_grad = timed(f, [&] { oagpu::launch_gradient(f, 0); });
+    if (f.prm.high_res) oagpu::launch_blending(f, 0);  // segregatedFlowEquations.cpp:284-288
+    check_err(f);
+    if (hist) {
+        for (int k = 0; k < 3; ++k) hist[k] = rU[k] / (us0 + kSmall);
+        hist[3] = rP[0] / (ps0 + kSmall);
+        hist[4] = cdU.n_iterations;
+        hist[5] = cdP.n_iterations;
+    }
+}
+}  // namespace
+
 int oagpu_outer_iteration(oagpu_flow* h, const double* ctl, double* hist) {
     return guarded([&] {
         OAGPU_REQUIRE(h && ctl, "null argument");
-        oagpu::Flow& f = h->f;
-        const double us0 = f.U_scale, ps0 = f.p_scale;
-        const double kSmall = 2.220446049250313e-16;
-        oagpu_control_data cdU{}, cdP{};
-        double rU[3], rP[1];
-        // momentum predictor (navierStokesEquation::solve :157-222)
-        f.ms_momentum = timed(f, [&] { oagpu::launch_assemble_momentum(f); });
-        oagpu::scaled_residual(f, 3, rU);
-        oagpu::krylov_solve(f, 3, OAGPU_GMRES, int(ctl[0]), ctl[1], 1e-30, 30, &cdU);
-        oagpu::launch_correct(f, 3, 1.0);
-        f.ms_mdot = timed(f, [&] { oagpu::launch_update_mdot(f); });
-        oagpu::field_scales(f, nullptr, nullptr);
-        // pressure correction (pressureCorrectionEquation::solve :89-164)
-        f.ms_pcorr = timed(f, [&] { oagpu::launch_assemble_pcorr(f); });
-        oagpu::scaled_residual(f, 1, rP);
-        oagpu::krylov_solve(f, 1, f.prm.p_method, int(ctl[2]), ctl[3], 1e-30, 30, &cdP);
-        oagpu::launch_correct(f, 1, f.prm.urf_p);
-        // mass flow, divergence, velocity correction, gradients (:106-285)
-        oagpu::launch_update_mdot(f);
-        if (f.has_outlet) oagpu::launch_reversal(f);  // segregatedFlowEquations.cpp:121
-        oagpu::launch_mass_divergence(f);
-        oagpu::launch_velocity_correction(f, 1);
-        oagpu::launch_gradient(f, 1);
-        oagpu::launch_velocity_correction(f, 2);
-        oagpu::field_scales(f, nullptr, nullptr);
-        f.ms_grad = timed(f, [&] { oagpu::launch_gradient(f, 0); });
-        if (f.prm.high_res) oagpu::launch_blending(f, 0);  // segregatedFlowEquations.cpp:284-288
-        check_err(f);
-        if (hist) {
-            for (int k = 0; k < 3; ++k) hist[k] = rU[k] / (us0 + kSmall);
-            hist[3] = rP[0] / (ps0 + kSmall);
-            hist[4] = cdU.n_iterations;
-            hist[5] = cdP.n_iterations;
-        }
+        run_outer_iteration(h->f, ctl, hist);
+    });
+}
+
+int oagpu_batch_outer_iterations(oagpu_flow* const* flows, int n, const double* ctl, int iters,
+                                 double tol, int threads, double* hist, int* done,
+                                 double* seconds) {
+    return guarded([&] {
+        OAGPU_REQUIRE(flows && ctl && n >= 0 && iters >= 0, "bad batch arguments");
+        for (int c = 0; c < n; ++c) OAGPU_REQUIRE(flows[c], "null flow in batch");
+        for (int c = 0; c < n; ++c)
+            for (int d = 0; d < c; ++d) OAGPU_REQUIRE(flows[c] != flows[d], "flow listed twice");
+        const int nt = std::max(1, std::min(threads > 0 ? threads : 16, n));
+        std::atomic<int> next{0};
+        std::mutex mu;
+        std::string first_error;
+        int first_code = OAGPU_OK;
+        auto worker = [&] {
+            for (int c = next++; c < n; c = next++) {
+                oagpu::Flow& f = flows[c]->f;
+                const auto t0 = std::chrono::steady_clock::now();
+                int it = 0;
+                try {
+                    for (; it < iters; ++it) {
+                        double hb[6];
+                        run_outer_iteration(f, ctl, hb);
+                        if (hist) std::copy(hb, hb + 6, hist + (size_t(c) * iters + it) * 6);
+                        if (tol > 0 && std::max({hb[0], hb[1], hb[2], hb[3]}) < tol) {
+                            ++it;
+                            break;
+                        }
+                    }
+                } catch (const oagpu::Error& e) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (first_code == OAGPU_OK) {
+                        first_code = e.code;
+                        first_error = "case " + std::to_string(c) + ": " + e.what();
+                    }
+                } catch (const std::exception& e) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (first_code == OAGPU_OK) {
+                        first_code = OAGPU_EINTERNAL;
+                        first_error = "case " + std::to_string(c) + ": " + e.what();
+                    }
+                }
+                const std::chrono::duration<double> dt = std::chrono::steady_clock::now() - t0;
+                if (done) done[c] = it;
+                if (seconds) seconds[c] = dt.count();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+        worker();
+        for (auto& t : pool) t.join();
+        if (first_code != OAGPU_OK) throw oagpu::Error(first_code, first_error);
     });
 }
 
