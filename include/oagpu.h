@@ -107,8 +107,8 @@ typedef struct oagpu_control_data {
                             liblinsolve's hypre family, linearSystem.h:331) */
 #define OAGPU_BICGSTAB_AMG 4 /* BiCGStab right-preconditioned by one AMG
                             V-cycle (scalar systems that are not symmetric:
-                            p' with static-pressure outlets); in the mixed
-                            form the V-cycle runs on the FP32 hierarchy */
+                            p' with static-pressure outlets or on distorted
+                            meshes); FP64 in both forms */
 
 /* ---------------------------------------------------------------- runtime */
 int oagpu_init(int device);
