@@ -56,7 +56,10 @@ void launch_mesh_geometry(DevMesh& m, hipStream_t s) {
     const unsigned B = 256;
     k_scv_volumes<<<xcd_grid(m.ne, B), B, 0, s>>>(m.ne, m.xyz.p, m.conn.p,
                                                   m.mixed ? m.etype.p : nullptr, m.scvw.p);
-    if (m.mixed) launch_scv_generic(m, s);
+    if (m.mixed) {
+        launch_scv_generic(m, s);
+        launch_geo_generic(m, s);
+    }
     k_dual_volume<<<xcd_grid(m.nn, B), B, 0, s>>>(m.nn, m.inc_ptr.p, m.inc.p, m.scvw.p, m.vol.p);
     k_scv_weights<<<xcd_grid(m.ne, B), B, 0, s>>>(m.ne, m.scvw.p);
     OAGPU_HIP(hipGetLastError());

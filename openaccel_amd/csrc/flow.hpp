@@ -133,6 +133,7 @@ void launch_correct_scalar(Flow& f, int which);   // phi += x, clipped; field sc
 // (0 hex8, 1 tet4, 2 wedge6, 3 pyramid5), device arrays
 int topology_npe(int topo);
 void launch_scv_generic(DevMesh& m, hipStream_t s);  // SCV volumes of non-hex elements
+void launch_geo_generic(DevMesh& m, hipStream_t s);  // static ip geometry of non-hex elements
 int topology_nscs(int topo);
 void element_geometry(int topo, int shifted, int n, const double* coords, double* A, double* V,
                       double* G, double* detj, int* err, hipStream_t s);

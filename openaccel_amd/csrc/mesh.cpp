@@ -273,6 +273,16 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
             if (etype(e) != OAGPU_HEX8) gl.push_back(e);
         dm.nglist = int(gl.size());
         dm.glist.upload(gl);
+        // static ip geometry: per ip 3 + 6 npe doubles (area, dN/dx standard and shifted)
+        std::vector<long long> off(size_t(ne), -1);
+        long long tot = 0;
+        for (int e : gl) {
+            off[e] = tot;
+            const int t = etype(e), npe = topo_npe(t);
+            tot += (long long)topo_nscs(t) * (3 + 6 * npe);
+        }
+        dm.geo_off.upload(off);
+        dm.geo.alloc(size_t(tot));
     }
     dm.row_ptr.upload(dm.h_row_ptr);
     dm.cols.upload(dm.h_cols);

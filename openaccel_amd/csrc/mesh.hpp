@@ -50,6 +50,8 @@ struct DevMesh {
     DBuf<int> gen_ptr, gen;    // [nn+1], [ngen]
     DBuf<uint64_t> gen_slots;  // [ngen]
     DBuf<int> glist;           // [nglist] non-hex elements, ascending
+    DBuf<double> geo;          // static ip geometry of the non-hex elements (topo::ip_geom)
+    DBuf<long long> geo_off;   // [ne] offset of an element's geometry in geo (-1: hex8)
     DBuf<int> row_ptr, cols, diag_off;
     // Values of the CRS matrix are stored sliced-ELL: slices of 64 rows (one
     // wavefront), slot-major and lane-contiguous, so a wavefront's lanes (one

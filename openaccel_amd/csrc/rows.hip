@@ -436,6 +436,8 @@ struct MomArgs {
     const int* gen_ptr;    // tet4 / wedge6 / pyramid5 incidences (mixed meshes)
     const int* gen;
     const uint64_t* gen_slots;
+    const double* geo;          // static ip geometry of the non-hex elements
+    const long long* geo_off;   // [ne] offset into geo
     const int* etype;
     double rho, mu, dt, urf_inv, bnd_relax;
     int ugrad_shifted, us;
@@ -623,9 +625,11 @@ __device__ void mom_generic(const MomArgs& a, int la, int e, uint64_t sl, const 
                             const double xr[3], double D[9], double R[3], int& bad, LT* lt) {
     constexpr int npe = topo::Traits<TOPO>::npe;
     const int* ce = a.conn + 8 * size_t(e);
+    const double* geo = a.geo + a.geo_off[e];
     T c[npe][3];
-    for (int k = 0; k < npe; ++k)
-        for (int d = 0; d < 3; ++d) c[k][d] = T(a.xyz[3 * size_t(ce[k]) + d] - xr[d]);
+    if constexpr (HR)
+        for (int k = 0; k < npe; ++k)
+            for (int d = 0; d < 3; ++d) c[k][d] = T(a.xyz[3 * size_t(ce[k]) + d] - xr[d]);
 #pragma unroll 1
     for (int ip = 0; ip < topo::Traits<TOPO>::nscs; ++ip) {
         const int* lr = topo::lr<TOPO>(ip);
@@ -634,7 +638,7 @@ __device__ void mom_generic(const MomArgs& a, int la, int e, uint64_t sl, const 
         const int o = left ? lr[1] : lr[0];
         const T sg = left ? T(1) : T(-1);
         T A[3], g[npe][3], Nc[npe], Nu[npe];
-        topo::scs_ip<TOPO>(ip, a.ugrad_shifted, a.us, c, A, g, Nc, Nu, bad);
+        topo::ip_geom<TOPO>(geo, ip, a.ugrad_shifted, a.us, A, g, Nc, Nu);
         for (int d = 0; d < 3; ++d) A[d] *= sg;
         const T m = sg * T(a.mdot[12 * size_t(e) + ip]);
         T mu = T(a.mu);
@@ -950,6 +954,8 @@ void launch_assemble_momentum(Flow& f) {
     a.gen_ptr = m.gen_ptr.p;
     a.gen = m.gen.p;
     a.gen_slots = m.gen_slots.p;
+    a.geo = m.geo.p;
+    a.geo_off = m.geo_off.p;
     a.etype = m.etype.p;
     a.vals = f.A3.p;
     a.b = f.b3.p;
@@ -1012,6 +1018,8 @@ struct PcArgs {
     const int* gen_ptr;    // tet4 / wedge6 / pyramid5 incidences (mixed meshes)
     const int* gen;
     const uint64_t* gen_slots;
+    const double* geo;          // static ip geometry of the non-hex elements
+    const long long* geo_off;   // [ne] offset into geo
     const int* etype;
     double rho;
     int u_shifted, p_ref;
@@ -1172,14 +1180,12 @@ __device__ void pc_generic(const PcArgs& a, int la, int e, uint64_t sl, const Se
                            double* lt) {
     constexpr int npe = topo::Traits<TOPO>::npe;
     const int* ce = a.conn + 8 * size_t(e);
-    T c[npe][3], Gp[3] = {0, 0, 0};
+    const double* geo = a.geo + a.geo_off[e];
+    T Gp[3] = {0, 0, 0};
     for (int k = 0; k < npe; ++k) {
         const size_t o = 3 * size_t(ce[k]);
         const T w = T(a.scvw[8 * size_t(e) + k]);
-        for (int d = 0; d < 3; ++d) {
-            c[k][d] = T(a.xyz[o + d] - xr[d]);
-            Gp[d] += w * T(a.gradP[o + d]);
-        }
+        for (int d = 0; d < 3; ++d) Gp[d] += w * T(a.gradP[o + d]);
     }
     const T rho = T(a.rho);
 #pragma unroll 1
@@ -1188,7 +1194,7 @@ __device__ void pc_generic(const PcArgs& a, int la, int e, uint64_t sl, const Se
         if (lr[0] != la && lr[1] != la) continue;
         const T sg = lr[0] == la ? T(1) : T(-1);
         T A[3], g[npe][3], Nc[npe], Nu[npe];
-        topo::scs_ip<TOPO>(ip, sp, a.u_shifted, c, A, g, Nc, Nu, bad);
+        topo::ip_geom<TOPO>(geo, ip, sp, a.u_shifted, A, g, Nc, Nu);
         for (int d = 0; d < 3; ++d) A[d] *= sg;
         T uip[3] = {0, 0, 0}, duip[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
         for (int k = 0; k < npe; ++k) {
@@ -1372,6 +1378,8 @@ void launch_assemble_pcorr(Flow& f) {
     a.gen_ptr = m.gen_ptr.p;
     a.gen = m.gen.p;
     a.gen_slots = m.gen_slots.p;
+    a.geo = m.geo.p;
+    a.geo_off = m.geo_off.p;
     a.etype = m.etype.p;
     a.rho = f.prm.rho;
     a.u_shifted = f.prm.u_shifted;
@@ -1454,6 +1462,8 @@ struct ScArgs {
     const int* gen_ptr;    // tet4 / wedge6 / pyramid5 incidences (mixed meshes)
     const int* gen;
     const uint64_t* gen_slots;
+    const double* geo;          // static ip geometry of the non-hex elements
+    const long long* geo_off;   // [ne] offset into geo
     const int* etype;
     double rho, mu, dt, urf_inv, sig1, sig2;
     int which, us;
@@ -1652,9 +1662,11 @@ __device__ void sc_generic(const ScArgs& a, int la, int e, uint64_t sl, const Se
                            double* lt) {
     constexpr int npe = topo::Traits<TOPO>::npe;
     const int* ce = a.conn + 8 * size_t(e);
+    const double* geo = a.geo + a.geo_off[e];
     T c[npe][3], ph[npe], gm[npe];
     for (int k = 0; k < npe; ++k) {
-        for (int d = 0; d < 3; ++d) c[k][d] = T(a.xyz[3 * size_t(ce[k]) + d] - xr[d]);
+        if constexpr (HR)
+            for (int d = 0; d < 3; ++d) c[k][d] = T(a.xyz[3 * size_t(ce[k]) + d] - xr[d]);
         ph[k] = T(a.phi[ce[k]]);
         const double F1 = a.F1[ce[k]];
         gm[k] = T(a.mu + a.mut[ce[k]] * (F1 * a.sig1 + (1.0 - F1) * a.sig2));
@@ -1667,7 +1679,7 @@ __device__ void sc_generic(const ScArgs& a, int la, int e, uint64_t sl, const Se
         const int o = left ? lr[1] : lr[0];
         const T sgn = left ? T(1) : T(-1);
         T A[3], g[npe][3], Nc[npe], Nu[npe];
-        topo::scs_ip<TOPO>(ip, sg_shift, a.us, c, A, g, Nc, Nu, bad);
+        topo::ip_geom<TOPO>(geo, ip, sg_shift, a.us, A, g, Nc, Nu);
         for (int d = 0; d < 3; ++d) A[d] *= sgn;
         const T m = sgn * T(a.mdot[12 * size_t(e) + ip]);
         T Gip = T(0);
@@ -1881,6 +1893,8 @@ void launch_assemble_scalar(Flow& f, int which) {
     a.gen_ptr = m.gen_ptr.p;
     a.gen = m.gen.p;
     a.gen_slots = m.gen_slots.p;
+    a.geo = m.geo.p;
+    a.geo_off = m.geo_off.p;
     a.etype = m.etype.p;
     a.rho = f.prm.rho;
     a.mu = f.prm.mu;

@@ -222,6 +222,54 @@ __global__ void __launch_bounds__(256) k_scv_generic(int n, const int* glist, co
     topo::scv_volumes<TOPO>(p, scvw + 8 * size_t(e));
 }
 
+// static ip geometry of the listed elements of topology TOPO (coordinates
+// relative to the element's node 0): area vector, dN/dx at the standard and
+// at the shifted ip; a non-positive Jacobian sets *err
+template <int TOPO>
+__global__ void __launch_bounds__(256) k_geo_generic(int n, const int* glist, const int* etype,
+                                                     const int* conn, const double* xyz,
+                                                     const long long* off, double* geo, int* err) {
+    using Tr = topo::Traits<TOPO>;
+    constexpr int npe = Tr::npe;
+    const int i = int(xcd_block() * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const int e = glist[i];
+    if (etype[e] != TOPO) return;
+    double c[npe][3];
+    const size_t o0 = 3 * size_t(conn[8 * size_t(e)]);
+    for (int j = 0; j < npe; ++j)
+        for (int k = 0; k < 3; ++k) c[j][k] = xyz[3 * size_t(conn[8 * size_t(e) + j]) + k] - xyz[o0 + k];
+    int bad = 0;
+    double* q = geo + off[e];
+#pragma unroll 1
+    for (int ip = 0; ip < Tr::nscs; ++ip, q += topo::geo_stride<TOPO>()) {
+        double A[3];
+        topo::scs_area<TOPO>(ip, c, A);
+        for (int d = 0; d < 3; ++d) q[d] = A[d];
+        for (int sh = 0; sh < 2; ++sh) {
+            const double* l = topo::ip_loc<TOPO>(sh, ip);
+            double dd[npe][3], g[npe][3];
+            topo::deriv<TOPO>(sh, l[0], l[1], l[2], dd);
+            topo::gradop<npe>(dd, c, g, &bad);
+            for (int j = 0; j < npe; ++j)
+                for (int k = 0; k < 3; ++k) q[3 + 3 * npe * sh + 3 * j + k] = g[j][k];
+        }
+    }
+    if (bad) *err = 1;
+}
+
+void launch_geo_generic(DevMesh& m, hipStream_t s) {
+    if (!m.nglist) return;
+    const unsigned B = 256, G = xcd_grid(size_t(m.nglist), B);
+    k_geo_generic<topo::kTet><<<G, B, 0, s>>>(m.nglist, m.glist.p, m.etype.p, m.conn.p, m.xyz.p,
+                                              m.geo_off.p, m.geo.p, m.err.p);
+    k_geo_generic<topo::kWed><<<G, B, 0, s>>>(m.nglist, m.glist.p, m.etype.p, m.conn.p, m.xyz.p,
+                                              m.geo_off.p, m.geo.p, m.err.p);
+    k_geo_generic<topo::kPyr><<<G, B, 0, s>>>(m.nglist, m.glist.p, m.etype.p, m.conn.p, m.xyz.p,
+                                              m.geo_off.p, m.geo.p, m.err.p);
+    OAGPU_HIP(hipGetLastError());
+}
+
 void launch_scv_generic(DevMesh& m, hipStream_t s) {
     if (!m.nglist) return;
     const unsigned B = 256, G = xcd_grid(size_t(m.nglist), B);

@@ -314,5 +314,28 @@ __device__ inline void scs_ip(int ip, int gshift, int us, const T c[][3], T A[3]
     shape<TOPO>(us, T(lu[0]), T(lu[1]), T(lu[2]), Nu);
 }
 
+// Static ip geometry of a tet4 / wedge6 / pyramid5 element, precomputed in
+// FP64 once per mesh (k_geo_generic): per SCS ip the area vector and dN/dx
+// at the standard and at the shifted ip, [nscs][3 + 2 * npe * 3] from geo.
+template <int TOPO>
+__device__ constexpr int geo_stride() {
+    return 3 + 6 * Traits<TOPO>::npe;
+}
+
+template <int TOPO, typename T>
+__device__ inline void ip_geom(const double* geo, int ip, int gshift, int us, T A[3], T g[][3],
+                               T Nc[], T Nu[]) {
+    constexpr int npe = Traits<TOPO>::npe;
+    const double* q = geo + size_t(ip) * geo_stride<TOPO>();
+    for (int d = 0; d < 3; ++d) A[d] = T(q[d]);
+    const double* gq = q + 3 + (gshift ? 3 * npe : 0);
+    for (int n = 0; n < npe; ++n)
+        for (int d = 0; d < 3; ++d) g[n][d] = T(gq[3 * n + d]);
+    const double* l0 = ip_loc<TOPO>(0, ip);
+    shape<TOPO>(0, T(l0[0]), T(l0[1]), T(l0[2]), Nc);
+    const double* lu = ip_loc<TOPO>(us, ip);
+    shape<TOPO>(us, T(lu[0]), T(lu[1]), T(lu[2]), Nu);
+}
+
 }  // namespace topo
 }  // namespace oagpu
