@@ -16,14 +16,24 @@
 //   rows in a fixed order (no atomics, bitwise reproducible).
 // * Cycle: V(1,1) with damped Jacobi (omega = 2/3) and the coarse correction
 //   scaled by alpha = 2 (unsmoothed aggregation under-corrects), symmetric, so
-//   it preconditions CG.  The coarsest level (<= 64 rows) is solved with its
-//   dense inverse, computed per solve by Gauss-Jordan in one workgroup's LDS.
+//   it preconditions CG and BiCGStab.  The coarsest level (<= 64 rows) is
+//   solved with its dense inverse, computed per solve by Gauss-Jordan in one
+//   workgroup's LDS.  Under (flexible) CG the coarse levels above the
+//   one-workgroup tail are K-cycle levels instead: two flexible-CG steps
+//   each, preconditioned by the cycle below, with no alpha scaling; the
+//   iteration count then stays nearly flat with the number of levels (on a
+//   7-point Laplacian, 128^3: 22 iterations to 1e-6 against 47 for V(1,1);
+//   on the 128^3 cavity's p' systems 25 against 63 in FP64).  Each K-cycle
+//   launches ~35 kernels against ~12 for a V-cycle, most of them on levels
+//   small enough to be latency-bound, so the gain in time is smaller than in
+//   iterations (FP64 4 %, mixed form within 1 % on that cavity).
 // * Values and vectors of every level are of the Krylov type V: double for
 //   the FP64 form, float for the FP32 inner solve of the mixed form.
 #include "device_util.hpp"
 #include "flow.hpp"
 #include "sell.hpp"
 
+#include <algorithm>
 #include <numeric>
 #include <type_traits>
 
@@ -216,25 +226,62 @@ __device__ inline double amg_block_sum(double v, double* sh) {
     return r;
 }
 
+// ------------------------------------------------------------ K-cycle
+// A coarse level c of the K-cycle (Notay and Vassilevski, Numer. Linear
+// Algebra Appl. 15 (2008)) is solved by two flexible-CG steps, each
+// preconditioned by the cycle of level c:
+//   c1 = B rc, v1 = A c1, rho1 = c1.v1, a1 = c1.rc, r2 = rc - (a1/rho1) v1,
+//   c2 = B r2, gam = c2.v1, bet = c2.A c2, a2 = c2.r2, rho2 = bet - gam^2/rho1,
+//   e = (a1/rho1 - gam a2/(rho1 rho2)) c1 + (a2/rho2) c2.
+// Dot products are block partials over a fixed grid of G blocks; the last
+// block to finish reduces them in a fixed order and leaves the step
+// coefficients in coef[] (0: a1/rho1, 1: rho1, 2: a1, 3: ca, 4: cb, with
+// e = ca c1 + cb c2).  r2 is formed inside the pre-smoothing kernel of the
+// second visit, e inside the prolongation that consumes it (on the coarse
+// levels; for the fine level by k_kc_combine).
+struct KcRef {
+    const double* coef = nullptr;
+};
+
+__device__ inline double kc_reduce(const double* kp, int k, int G, double* sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < G; i += blockDim.x) v += kp[size_t(k) * G + i];
+    return amg_block_sum(v, sh);
+}
+
 // Pre-smoothing from a zero guess fused with the residual:
 //   x = omega D^-1 r,  res = r - A x  (x of the neighbours formed on the fly)
+// With kc.coef (second visit of a K-cycle level) the right-hand side is
+// r - (a1/rho1) v1, formed on the fly and written to rout.
 // FINE only names the fine-level instance separately (profiles, timing).
 template <typename V, bool FINE>
 __global__ void __launch_bounds__(256) k_amg_pre_residual(SellView s, const V* A, const V* dinv,
                                                           V omega, const V* r, V* x, V* res,
-                                                          const int* flag) {
+                                                          const int* flag, KcRef kc = {},
+                                                          const V* v1 = nullptr,
+                                                          V* rout = nullptr) {
     if (flag[0]) return;
+    const V sc = kc.coef ? V(kc.coef[0]) : V(0);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n) return;
     int w;
     const size_t b = sv_base(s, i, w);
     V acc = V(0);
+    if (kc.coef) {
 #pragma unroll 4
-    for (int q = 0; q < w; ++q) {
-        const int j = s.cols[b + size_t(q) * 64];
-        acc += A[b + size_t(q) * 64] * (dinv[j] * r[j]);
+        for (int q = 0; q < w; ++q) {
+            const int j = s.cols[b + size_t(q) * 64];
+            acc += A[b + size_t(q) * 64] * (dinv[j] * (r[j] - sc * v1[j]));
+        }
+    } else {
+#pragma unroll 4
+        for (int q = 0; q < w; ++q) {
+            const int j = s.cols[b + size_t(q) * 64];
+            acc += A[b + size_t(q) * 64] * (dinv[j] * r[j]);
+        }
     }
-    const V ri = r[i];
+    const V ri = kc.coef ? r[i] - sc * v1[i] : r[i];
+    if (kc.coef) rout[i] = ri;
     x[i] = omega * dinv[i] * ri;
     res[i] = ri - omega * acc;
 }
@@ -250,30 +297,125 @@ __global__ void __launch_bounds__(256) k_amg_prolong_smooth(SellView s, const V*
                                                             V omega, const V* r, const V* x,
                                                             const int* agg, V alpha, const V* ec,
                                                             V* out, double* part, V* p,
-                                                            const int* flag) {
+                                                            const int* flag, const V* qv = nullptr,
+                                                            double* qpart = nullptr, KcRef kc = {},
+                                                            const V* ec2 = nullptr) {
     __shared__ double sh[256];
     if (flag[0]) return;
-    double rz = 0.0;
+    // K-cycle correction: alpha P (ca c1 + cb c2), c1 = ec, c2 = ec2
+    const V ca = kc.coef ? V(kc.coef[3]) : alpha, cb = kc.coef ? V(kc.coef[4]) : V(0);
+    double rz = 0.0, qz = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
         int w;
         const size_t b = sv_base(s, i, w);
         V acc = V(0);
+        if (ec2) {
 #pragma unroll 4
-        for (int q = 0; q < w; ++q) {
-            const int j = s.cols[b + size_t(q) * 64];
-            acc += A[b + size_t(q) * 64] * (x[j] + alpha * ec[agg[j]]);
+            for (int q = 0; q < w; ++q) {
+                const int j = s.cols[b + size_t(q) * 64];
+                const int J = agg[j];
+                acc += A[b + size_t(q) * 64] * (x[j] + (ca * ec[J] + cb * ec2[J]));
+            }
+        } else {
+#pragma unroll 4
+            for (int q = 0; q < w; ++q) {
+                const int j = s.cols[b + size_t(q) * 64];
+                acc += A[b + size_t(q) * 64] * (x[j] + ca * ec[agg[j]]);
+            }
         }
         const V ri = r[i];
-        const V ti = x[i] + alpha * ec[agg[i]];
+        const int I = agg[i];
+        const V ti = x[i] + (ec2 ? ca * ec[I] + cb * ec2[I] : ca * ec[I]);
         const V oi = ti + omega * dinv[i] * (ri - acc);
         out[i] = oi;
         if (p) p[i] = oi;
         rz += double(ri) * double(oi);
+        if (qv) qz += double(qv[i]) * double(oi);
     }
     if (part) {
         const double t = amg_block_sum(rz, sh);
         if (threadIdx.x == 0) part[blockIdx.x] = t;
     }
+    if (qpart) {
+        const double t = amg_block_sum(qz, sh);
+        if (threadIdx.x == 0) qpart[blockIdx.x] = t;
+    }
+}
+
+// v = A c (v may be null); partials of c.(A c), c.y1 and c.y2 (y2 may be
+// null) into kp[k*G+b]; the last block reduces them: step 1 (c = c1, y1 =
+// rc) leaves a1/rho1, rho1, a1 in coef[0..2]; step 2 (c = c2, y1 = v1,
+// y2 = r2) the combination ca, cb in coef[3..4].  *cnt counts finished
+// blocks and is back at 0 when the kernel ends.
+template <typename V>
+__global__ void __launch_bounds__(256) k_kc_dots(SellView s, const V* A, const V* c, V* v,
+                                                 const V* y1, const V* y2, double* kp,
+                                                 double* coef, unsigned* cnt, int step,
+                                                 const int* flag) {
+    __shared__ double sh[256];
+    __shared__ int last;
+    if (flag[0]) return;
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+        int w;
+        const size_t b = sv_base(s, i, w);
+        V acc = V(0);
+#pragma unroll 4
+        for (int q = 0; q < w; ++q) acc += A[b + size_t(q) * 64] * c[s.cols[b + size_t(q) * 64]];
+        if (v) v[i] = acc;
+        const double ci = double(c[i]);
+        d0 += ci * double(acc);
+        d1 += ci * double(y1[i]);
+        if (y2) d2 += ci * double(y2[i]);
+    }
+    const int G = gridDim.x;
+    d0 = amg_block_sum(d0, sh);
+    d1 = amg_block_sum(d1, sh);
+    d2 = amg_block_sum(d2, sh);
+    if (threadIdx.x == 0) {
+        kp[blockIdx.x] = d0;
+        kp[G + blockIdx.x] = d1;
+        kp[2 * G + blockIdx.x] = d2;
+        __threadfence();
+        last = atomicAdd(cnt, 1u) == unsigned(G - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const double e0 = kc_reduce(kp, 0, G, sh), e1 = kc_reduce(kp, 1, G, sh);
+    const double e2 = kc_reduce(kp, 2, G, sh);
+    if (threadIdx.x == 0) {
+        if (step == 1) {
+            coef[0] = e0 != 0.0 ? e1 / e0 : 0.0;
+            coef[1] = e0;
+            coef[2] = e1;
+        } else {
+            const double rho1 = coef[1], a1 = coef[2], bet = e0, gam = e1, a2 = e2;
+            double ca = 0.0, cb = 0.0;
+            if (rho1 > 0.0) {
+                ca = a1 / rho1;
+                const double rho2 = bet - gam * gam / rho1;
+                if (rho2 > 1e-12 * bet) {
+                    cb = a2 / rho2;
+                    ca -= gam * cb / rho1;
+                }
+            }
+            coef[3] = ca;
+            coef[4] = cb;
+        }
+        atomicExch(cnt, 0u);
+    }
+}
+
+// c1 = ca c1 + cb c2: the correction e of a K-cycle level formed in place,
+// for the fine-level prolongation (gathering both vectors there costs more
+// than this pass over the coarse level)
+template <typename V>
+__global__ void __launch_bounds__(256) k_kc_combine(int n, const double* coef, V* c1, const V* c2,
+                                                    const int* flag) {
+    if (flag[0]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c1[i] = V(coef[3]) * c1[i] + V(coef[4]) * c2[i];
 }
 
 // x = Minv r on the coarsest level (one workgroup; Minv stored transposed)
@@ -538,8 +680,10 @@ void amg_build(Flow& f) {
         cl.fA.alloc(cl.nsell);
         cl.A.zero(f.stream);
         cl.fA.zero(f.stream);
-        for (auto* b : {&cl.dinv, &cl.X, &cl.T, &cl.R, &cl.S}) b->alloc(nc);
-        for (auto* b : {&cl.fdinv, &cl.fX, &cl.fT, &cl.fR, &cl.fS}) b->alloc(nc);
+        for (auto* b : {&cl.dinv, &cl.X, &cl.T, &cl.R, &cl.S, &cl.C1, &cl.C2, &cl.V1, &cl.R2})
+            b->alloc(nc);
+        for (auto* b : {&cl.fdinv, &cl.fX, &cl.fT, &cl.fR, &cl.fS, &cl.fC1, &cl.fC2, &cl.fV1, &cl.fR2})
+            b->alloc(nc);
         fine_slice_ptr = sh.slice_ptr;
         fine_slice_w = sh.slice_w;
         fine_nsell = sh.nsell;
@@ -550,6 +694,10 @@ void amg_build(Flow& f) {
     for (auto* b : {&l0.T, &l0.S}) b->alloc(m.nn);
     for (auto* b : {&l0.fT, &l0.fS}) b->alloc(m.nn);
     const int nl = int(amg.lv.size());
+    amg.kc_coef.alloc(8 * size_t(kMaxLevels));
+    amg.kc_coef.zero(f.stream);
+    amg.kc_cnt.alloc(kMaxLevels);
+    amg.kc_cnt.zero(f.stream);
     amg.dense_n = nl > 1 && amg.lv[nl - 1].n <= kDenseMax ? amg.lv[nl - 1].n : 0;
     if (amg.dense_n) {
         amg.dense.alloc(size_t(amg.dense_n) * amg.dense_n);
@@ -565,12 +713,12 @@ static V* lvA(AmgLevel& l) {
     else return l.A.p;
 }
 template <typename V>
-static V* lvv(AmgLevel& l, int which) {  // 0 dinv, 1 X, 2 T, 3 R, 4 S
+static V* lvv(AmgLevel& l, int which) {  // 0 dinv, 1 X, 2 T, 3 R, 4 S, 5 C1, 6 C2, 7 V1, 8 R2
     if constexpr (std::is_same_v<V, float>) {
-        DBuf<float>* b[5] = {&l.fdinv, &l.fX, &l.fT, &l.fR, &l.fS};
+        DBuf<float>* b[9] = {&l.fdinv, &l.fX, &l.fT, &l.fR, &l.fS, &l.fC1, &l.fC2, &l.fV1, &l.fR2};
         return b[which]->p;
     } else {
-        DBuf<double>* b[5] = {&l.dinv, &l.X, &l.T, &l.R, &l.S};
+        DBuf<double>* b[9] = {&l.dinv, &l.X, &l.T, &l.R, &l.S, &l.C1, &l.C2, &l.V1, &l.R2};
         return b[which]->p;
     }
 }
@@ -602,21 +750,130 @@ void amg_setup(Flow& f, const V* A0, const V* dinv0) {
     OAGPU_HIP(hipGetLastError());
 }
 
+// Host driver of one preconditioner application: the fine level and the
+// levels above the tail run as grid-wide kernels, levels from the tail down
+// inside one workgroup.  Every level's vectors are its own, so the two visits
+// of a K-cycle level reuse the deeper levels' buffers one after the other.
+template <typename V>
+struct AmgCycle {
+    Flow& f;
+    Amg& amg;
+    const int* flag;
+    V om, al;
+    int t, last;
+    bool kcycle;
+    const V* A0;
+    const V* dinv0;
+
+    SellView view(int l) const { return l == 0 ? view_of(*f.m) : view_of(amg.lv[l]); }
+    const V* A(int l) const { return l == 0 ? A0 : lvA<V>(amg.lv[l]); }
+    const V* D(int l) const { return l == 0 ? dinv0 : lvv<V>(amg.lv[l], 0); }
+    V* X(int l) const { return lvv<V>(amg.lv[l], l == 0 ? 2 : 1); }  // T0 on level 0
+
+    // correction on level l + 1 of the residual S_l of level l, applied by
+    // the prolongation as alpha P ec, or for a K-cycle level as
+    // P (ca c1 + cb c2) with the coefficients from kc
+    struct Corr {
+        const V* ec;
+        const V* ec2;
+        V alpha;
+        KcRef kc;
+    };
+
+    Corr coarse(int l) {
+        const int c = l + 1;
+        AmgLevel& cl = amg.lv[c];
+        const V* S = lvv<V>(amg.lv[l], 4);
+        if (kcycle && c < t && c < last && 16 + 3 * (2 * c + 2) <= 64) {
+            V* rc = lvv<V>(cl, 3);
+            V* c1 = lvv<V>(cl, 5);
+            V* c2 = lvv<V>(cl, 6);
+            V* v1 = lvv<V>(cl, 7);
+            V* r2 = lvv<V>(cl, 8);
+            const int G = int(std::min<unsigned>(blocks(cl.n), unsigned(kRedGrid)));
+            double* kp = f.part.p + size_t(16 + 6 * c) * kRedGrid;
+            double* coef = amg.kc_coef.p + 8 * c;
+            unsigned* cnt = amg.kc_cnt.p + c;
+            k_amg_restrict<V><<<blocks(cl.n), 256, 0, f.stream>>>(cl.n, cl.mem_ptr.p, cl.mem.p, S, om,
+                                                                  nullptr, rc, nullptr, flag);
+            level(c, rc, c1);
+            k_kc_dots<V><<<G, 256, 0, f.stream>>>(view(c), A(c), c1, v1, rc, nullptr, kp, coef, cnt, 1,
+                                                  flag);
+            KcRef k;
+            k.coef = coef;
+            level(c, rc, c2, k, v1, r2);  // right-hand side r2 = rc - (a1/rho1) v1
+            k_kc_dots<V><<<G, 256, 0, f.stream>>>(view(c), A(c), c2, nullptr, v1, r2, kp, coef, cnt, 2,
+                                                  flag);
+            if (l == 0) {
+                k_kc_combine<V><<<blocks(cl.n), 256, 0, f.stream>>>(cl.n, coef, c1, c2, flag);
+                return {c1, nullptr, V(1), {}};
+            }
+            return {c1, c2, V(1), k};
+        }
+        if (t <= last && c == t) {  // the tail restricts S_l itself
+            TailArgs<V> ta{};
+            ta.t = t;
+            ta.last = last;
+            ta.dense_n = amg.dense_n;
+            ta.omega = om;
+            ta.alpha = al;
+            ta.res_in = S;
+            if constexpr (std::is_same_v<V, float>) ta.inv = amg.dense_n ? amg.fdense.p : nullptr;
+            else ta.inv = amg.dense_n ? amg.dense.p : nullptr;
+            for (int k = t; k <= last; ++k) {
+                AmgLevel& L = amg.lv[k];
+                ta.lv[k] = {view(k), A(k), D(k), lvv<V>(L, 3), lvv<V>(L, 1), lvv<V>(L, 4), lvv<V>(L, 2),
+                            L.agg.p, L.mem_ptr.p, L.mem.p};
+            }
+            k_amg_tail<V><<<1, kTailThreads, 0, f.stream>>>(ta, flag);
+            return {t == last ? X(last) : lvv<V>(amg.lv[t], 2), nullptr, al, {}};
+        }
+        V* rc = lvv<V>(cl, 3);
+        k_amg_restrict<V><<<blocks(cl.n), 256, 0, f.stream>>>(cl.n, cl.mem_ptr.p, cl.mem.p, S, om,
+                                                              nullptr, rc, nullptr, flag);
+        if (c < last) {  // a V-cycle level above the tail
+            level(c, rc, lvv<V>(cl, 2));
+            return {lvv<V>(cl, 2), nullptr, al, {}};
+        }
+        if (amg.dense_n) {  // coarsest level
+            const V* inv;
+            if constexpr (std::is_same_v<V, float>) inv = amg.fdense.p;
+            else inv = amg.dense.p;
+            k_amg_dense_apply<V><<<1, kDenseMax, 0, f.stream>>>(cl.n, inv, rc, X(c), flag);
+        } else {  // coarsening stalled above the dense size: Jacobi steps
+            k_amg_jacobi0<V><<<blocks(cl.n), 256, 0, f.stream>>>(cl.n, om, D(c), rc, X(c), flag);
+            for (int it = 0; it < 4; ++it) {
+                k_amg_smooth<V><<<blocks(cl.n), 256, 0, f.stream>>>(view(c), A(c), D(c), om, rc, X(c),
+                                                                    lvv<V>(cl, 2), flag);
+                k_amg_smooth<V><<<blocks(cl.n), 256, 0, f.stream>>>(view(c), A(c), D(c), om, rc,
+                                                                    lvv<V>(cl, 2), X(c), flag);
+            }
+        }
+        return {X(c), nullptr, al, {}};
+    }
+
+    // out ~ A_l^-1 rhs on a level 1 <= l < min(t, last): pre-smoothing and
+    // residual, the coarse correction, prolongation and post-smoothing.
+    // With pre.coef the right-hand side is rhs - (a1/rho1) v1, written to rout.
+    void level(int l, const V* rhs, V* out, KcRef pre = {}, const V* v1 = nullptr,
+               V* rout = nullptr) {
+        AmgLevel& L = amg.lv[l];
+        k_amg_pre_residual<V, false><<<blocks(L.n), 256, 0, f.stream>>>(
+            view(l), A(l), D(l), om, rhs, X(l), lvv<V>(L, 4), flag, pre, v1, rout);
+        const V* r = pre.coef ? rout : rhs;
+        const Corr cc = coarse(l);
+        k_amg_prolong_smooth<V, false><<<blocks(L.n), 256, 0, f.stream>>>(
+            view(l), A(l), D(l), om, r, X(l), amg.lv[l + 1].agg.p, cc.alpha, cc.ec, out, nullptr,
+            nullptr, flag, nullptr, nullptr, cc.kc, cc.ec2);
+    }
+};
+
 template <typename V>
 void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* rz_part, V* p_init,
-                bool timing) {
+                bool timing, bool kcycle, const V* q, double* qz_part) {
     Amg& amg = f.amg;
     const int nl = int(amg.lv.size());
-    const int* flag = f.flag.p;
-    const V om = V(kOmega), al = V(kAlpha);
-    AmgLevel& l0 = amg.lv[0];
-    const SellView v0 = view_of(*f.m);
     OAGPU_REQUIRE(nl > 1, "AMG hierarchy without a coarse level");
-    auto R = [&](int l) -> const V* { return l == 0 ? r : lvv<V>(amg.lv[l], 3); };
-    auto X = [&](int l) -> V* { return lvv<V>(amg.lv[l], l == 0 ? 2 : 1); };  // T0 on level 0
-    auto A = [&](int l) -> const V* { return l == 0 ? A0 : lvA<V>(amg.lv[l]); };
-    auto D = [&](int l) -> const V* { return l == 0 ? dinv0 : lvv<V>(amg.lv[l], 0); };
-    auto view = [&](int l) { return l == 0 ? v0 : view_of(amg.lv[l]); };
     const int last = nl - 1;
     // first level handled by the one-workgroup tail (all deeper ones are smaller)
     int t = last + 1;
@@ -625,72 +882,17 @@ void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* 
             t = l;
             break;
         }
-    // downward: pre-smooth + residual, restrict
-    for (int l = 0; l < last && l < t; ++l) {
-        AmgLevel& fl = amg.lv[l];
-        AmgLevel& cl = amg.lv[l + 1];
-        V* S = lvv<V>(fl, 4);
-        if (l == 0) {
-            if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[0], f.stream));
-            k_amg_pre_residual<V, true><<<blocks(fl.n), 256, 0, f.stream>>>(view(l), A(l), D(l), om,
-                                                                            R(l), X(l), S, flag);
-            if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[1], f.stream));
-        } else {
-            k_amg_pre_residual<V, false><<<blocks(fl.n), 256, 0, f.stream>>>(view(l), A(l), D(l), om,
-                                                                             R(l), X(l), S, flag);
-        }
-        if (l + 1 == t) break;  // the tail restricts into level t
-        k_amg_restrict<V><<<blocks(cl.n), 256, 0, f.stream>>>(cl.n, cl.mem_ptr.p, cl.mem.p, S, om,
-                                                              nullptr, lvv<V>(cl, 3), nullptr, flag);
-    }
-    AmgLevel& c = amg.lv[last];
-    const V* dinv_inv;
-    if constexpr (std::is_same_v<V, float>) dinv_inv = amg.fdense.p;
-    else dinv_inv = amg.dense.p;
-    const V* ec = X(last);
-    if (t <= last) {
-        TailArgs<V> ta{};
-        ta.t = t;
-        ta.last = last;
-        ta.dense_n = amg.dense_n;
-        ta.omega = om;
-        ta.alpha = al;
-        ta.res_in = lvv<V>(amg.lv[t - 1], 4);
-        ta.inv = amg.dense_n ? dinv_inv : nullptr;
-        for (int l = t; l <= last; ++l) {
-            AmgLevel& L = amg.lv[l];
-            ta.lv[l] = {view(l), A(l), D(l), lvv<V>(L, 3), lvv<V>(L, 1), lvv<V>(L, 4), lvv<V>(L, 2),
-                        L.agg.p, L.mem_ptr.p, L.mem.p};
-        }
-        k_amg_tail<V><<<1, kTailThreads, 0, f.stream>>>(ta, flag);
-        ec = t == last ? X(last) : lvv<V>(amg.lv[t], 2);
-    } else {
-        if (amg.dense_n) {
-            k_amg_dense_apply<V><<<1, kDenseMax, 0, f.stream>>>(c.n, dinv_inv, R(last), X(last), flag);
-        } else {  // coarsening stalled above the dense size: Jacobi steps
-            k_amg_jacobi0<V><<<blocks(c.n), 256, 0, f.stream>>>(c.n, om, D(last), R(last), X(last), flag);
-            for (int it = 0; it < 4; ++it) {
-                k_amg_smooth<V><<<blocks(c.n), 256, 0, f.stream>>>(view(last), A(last), D(last), om,
-                                                                   R(last), X(last), lvv<V>(c, 2), flag);
-                k_amg_smooth<V><<<blocks(c.n), 256, 0, f.stream>>>(view(last), A(last), D(last), om,
-                                                                   R(last), lvv<V>(c, 2), X(last), flag);
-            }
-        }
-        t = last;
-    }
-    // upward: prolong + post-smooth, out of place (coarse levels into T_l)
-    for (int l = t - 1; l >= 1; --l) {
-        AmgLevel& fl = amg.lv[l];
-        V* out = lvv<V>(fl, 2);
-        k_amg_prolong_smooth<V, false><<<blocks(fl.n), 256, 0, f.stream>>>(
-            view(l), A(l), D(l), om, R(l), X(l), amg.lv[l + 1].agg.p, al, ec, out, nullptr, nullptr,
-            flag);
-        ec = out;
-    }
+    AmgCycle<V> cy{f, amg, f.flag.p, V(kOmega), V(kAlpha), t, last, kcycle, A0, dinv0};
+    AmgLevel& l0 = amg.lv[0];
+    if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[0], f.stream));
+    k_amg_pre_residual<V, true><<<blocks(l0.n), 256, 0, f.stream>>>(cy.view(0), A0, dinv0, cy.om, r,
+                                                                    cy.X(0), lvv<V>(l0, 4), cy.flag);
+    if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[1], f.stream));
+    const auto cc = cy.coarse(0);
     if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[2], f.stream));
-    k_amg_prolong_smooth<V, true><<<kRedGrid, 256, 0, f.stream>>>(v0, A0, dinv0, om, r, X(0),
-                                                                  amg.lv[1].agg.p, al, ec, z,
-                                                                  rz_part, p_init, flag);
+    k_amg_prolong_smooth<V, true><<<kRedGrid, 256, 0, f.stream>>>(
+        cy.view(0), A0, dinv0, cy.om, r, cy.X(0), amg.lv[1].agg.p, cc.alpha, cc.ec, z, rz_part,
+        p_init, cy.flag, q, qz_part, cc.kc, cc.ec2);
     if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[3], f.stream));
     OAGPU_HIP(hipGetLastError());
 }
@@ -698,8 +900,8 @@ void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* 
 template void amg_setup<double>(Flow&, const double*, const double*);
 template void amg_setup<float>(Flow&, const float*, const float*);
 template void amg_vcycle<double>(Flow&, const double*, const double*, const double*, double*,
-                                 double*, double*, bool);
+                                 double*, double*, bool, bool, const double*, double*);
 template void amg_vcycle<float>(Flow&, const float*, const float*, const float*, float*, double*,
-                                float*, bool);
+                                float*, bool, bool, const float*, double*);
 
 }  // namespace oagpu

@@ -24,6 +24,8 @@ struct AmgLevel {
     DBuf<uint8_t> gmap;      // [level l-1 sliced-ELL slots] -> slot of the receiving row here
     DBuf<double> A, dinv, X, T, R, S;
     DBuf<float> fA, fdinv, fX, fT, fR, fS;
+    DBuf<double> C1, C2, V1, R2;  // K-cycle vectors (levels solved by two inner FCG steps)
+    DBuf<float> fC1, fC2, fV1, fR2;
 };
 
 struct Amg {
@@ -33,6 +35,8 @@ struct Amg {
     int dense_n = 0;           // coarsest level solved by its dense inverse when > 0
     DBuf<double> dense;        // [dense_n][dense_n] inverse of the coarsest matrix
     DBuf<float> fdense;
+    DBuf<double> kc_coef;      // [level][8] K-cycle step coefficients
+    DBuf<unsigned> kc_cnt;     // [level] finished-block counters of the K-cycle dot kernels
 };
 
 // An instantiated HIP graph of a block of CG iterations and the operands it
@@ -142,10 +146,14 @@ void amg_build(Flow& f);                       // hierarchy structure (once per 
 template <typename V>
 void amg_setup(Flow& f, const V* A0, const V* dinv0);  // numeric Galerkin levels for A0
 // z = M^-1 r; the block partials of r . z go to rz_part (kRedGrid blocks) and,
-// with p_init, p = z as well
+// with p_init, p = z as well; with q, the partials of q . z go to qz_part.
+// kcycle: the coarse levels above the one-workgroup tail are solved by two
+// flexible-CG steps each (a nonlinear preconditioner: flexible CG only), else
+// one V-cycle (a fixed linear operator).
 // timing: HIP events around the fine-level kernels (eager launches only)
 template <typename V>
 void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* rz_part, V* p_init,
-                bool timing = false);
+                bool timing = false, bool kcycle = false, const V* q = nullptr,
+                double* qz_part = nullptr);
 
 }  // namespace oagpu

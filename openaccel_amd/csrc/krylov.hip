@@ -365,7 +365,10 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_update1(size_t n, const V* p, 
 
 __global__ void k_clear_flag(int* flag) { flag[0] = 0; }
 
-template <typename V>
+// FLEX: flexible CG (the preconditioner is the nonlinear K-cycle):
+// beta = -(z . q) / (p . q) keeps the new direction A-orthogonal to the last
+// one (Notay, SIAM J. Sci. Comput. 22 (2000)); else beta = r.z / r_old.z_old
+template <typename V, bool FLEX = false>
 __global__ void __launch_bounds__(kRedBlock) k_cg_update2(size_t n, const V* z, V* p,
                                                           const double* part, double* scal,
                                                           int* flag) {
@@ -376,7 +379,13 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_update2(size_t n, const V* z, 
     const double rr = reduce_partials(part, 4, sh);
     const bool conv = rr <= scal[2];
     if (!conv) {
-        const V beta = V(rz / scal[it & 1]);
+        V beta;
+        if constexpr (FLEX) {
+            const double pq = reduce_partials(part, 2, sh), qz = reduce_partials(part, 5, sh);
+            beta = V(pq != 0.0 ? -qz / pq : 0.0);
+        } else {
+            beta = V(rz / scal[it & 1]);
+        }
         for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
              i += size_t(gridDim.x) * blockDim.x)
             p[i] = z[i] + beta * p[i];
@@ -399,7 +408,7 @@ __global__ void k_cg_tick(int* flag) {
 
 // Device-resident PCG on A x = b (values V); returns iterations, fills
 // initial / final ||r||_2 in V arithmetic.  amg: preconditioner = one AMG
-// V-cycle (N = 1), else Jacobi.
+// K-cycle under flexible CG (N = 1), else Jacobi under CG.
 template <int N, typename V>
 static void cg_iteration(Flow& f, const V* A, const V* dinv, V* x, V* r, V* z, V* p, V* q, bool amg) {
     DevMesh& m = *f.m;
@@ -412,19 +421,22 @@ static void cg_iteration(Flow& f, const V* A, const V* dinv, V* x, V* r, V* z, V
             k_cg_update1<V, true><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, p, q, dinv, x, r, z,
                                                                          f.part.p, f.scal.p,
                                                                          f.flag.p);
-            amg_vcycle<V>(f, A, dinv, r, z, f.part.p + 3 * kRedGrid, nullptr);
+            amg_vcycle<V>(f, A, dinv, r, z, f.part.p + 3 * kRedGrid, nullptr, false, true, q,
+                          f.part.p + 5 * kRedGrid);
+            k_cg_update2<V, true><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, z, p, f.part.p, f.scal.p,
+                                                                         f.flag.p);
         }
     } else {
         k_cg_update1<V, false><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, p, q, dinv, x, r, z,
                                                                       f.part.p, f.scal.p, f.flag.p);
+        k_cg_update2<V><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, z, p, f.part.p, f.scal.p, f.flag.p);
     }
-    k_cg_update2<V><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, z, p, f.part.p, f.scal.p, f.flag.p);
     k_cg_tick<<<1, 1, 0, f.stream>>>(f.flag.p);
 }
 
 // Device-resident PCG on A x = b (values V); returns iterations, fills
 // initial / final ||r||_2 in V arithmetic.  amg: preconditioner = one AMG
-// V-cycle (N = 1), else Jacobi.  Every iteration reads its index and the
+// K-cycle under flexible CG (N = 1), else Jacobi under CG.  Every iteration reads its index and the
 // stop flag from device memory, so a block of kPoll iterations is the same
 // launch sequence every time: with graphs on it is captured once per set of
 // operands and replayed (one launch per block instead of ~15 per iteration);
@@ -440,7 +452,7 @@ static int cg_core(Flow& f, const V* A, const V* b, const V* dinv, V* x, V* r, V
     k_clear_flag<<<1, 1, 0, f.stream>>>(f.flag.p);  // a kernel, not a memset: capturable
     if (amg) {
         k_cg_init<V, true><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, b, dinv, x, r, z, p, f.part.p);
-        if constexpr (N == 1) amg_vcycle<V>(f, A, dinv, r, z, f.part.p, p, true);  // z, p = z, r.z
+        if constexpr (N == 1) amg_vcycle<V>(f, A, dinv, r, z, f.part.p, p, true, true);  // z, p = z, r.z
     } else {
         k_cg_init<V, false><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, b, dinv, x, r, z, p, f.part.p);
     }

@@ -13,7 +13,9 @@ reference vectors; what the mathematics fixes is checked instead:
 * the AMG-CG solution equals the direct (scipy) solution of the same system
   to the accuracy its tolerance allows, and the mixed form (FP32 AMG-CG under
   FP64 refinement) reaches the FP64 residual bar;
-* it needs several times fewer iterations than Jacobi-CG.
+* it needs several times fewer iterations than Jacobi-CG, and with K-cycle
+  levels (coarse levels above the one-workgroup tail, under flexible CG)
+  its iteration count stays nearly flat as levels are added.
 """
 import numpy as np
 import pytest
@@ -107,6 +109,27 @@ def test_amg_cg_solution_fp64(oa, meshgen, case):
     assert cd.n_iterations * 3 < cdj.n_iterations, (cd.n_iterations, cdj.n_iterations)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_amg_kcycle_levels(oa, meshgen, precision):
+    """A mesh with two levels above the one-workgroup tail (2048 rows), so
+    that level 1 is a K-cycle level under flexible CG: the residual bar of
+    the tolerance is met (the true residual, with the downloaded matrix), and the
+    iteration count stays within 1.5x that of a mesh whose coarse levels all
+    sit in the tail (the K-cycle keeps it nearly flat with the levels)."""
+    tol = 1e-10 if precision == 0 else 1e-8
+    dm, f, A, b = pcorr_case(oa, meshgen, 40, 0.1, 8, precision=precision)
+    cd = f.solve(1, oa.OAGPU_CG_AMG, 400, tol)
+    sizes = f.amg_levels()
+    assert sizes[1] > 2048 and len(sizes) >= 4, sizes
+    x = f.solution(1)
+    assert cd.solver_final_res <= tol * cd.solver_initial_res * 1.0001
+    assert np.linalg.norm(b - A @ x) <= 1.01 * tol * np.linalg.norm(b)
+    dm2, f2, A2, b2 = pcorr_case(oa, meshgen, 16, 0.1, 8, precision=precision)
+    cd2 = f2.solve(1, oa.OAGPU_CG_AMG, 400, tol)
+    assert f2.amg_levels()[1] <= 2048
+    assert cd.n_iterations <= 1.5 * cd2.n_iterations, (cd.n_iterations, cd2.n_iterations)
+
+
 def test_amg_cg_mixed_refinement(oa, meshgen):
     dm, f, A, b = pcorr_case(oa, meshgen, 16, 0.15, 6, precision=1)
     xs = spla.spsolve(A.tocsc(), b)
@@ -132,16 +155,17 @@ def test_amg_outer_iterations_converge(oa, meshgen):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("method", ["CG", "CG_AMG"])
-def test_graph_replay_matches_eager(oa, meshgen, precision, method):
+@pytest.mark.parametrize("method, n", [("CG", 14), ("CG_AMG", 14), ("CG_AMG", 40)])
+def test_graph_replay_matches_eager(oa, meshgen, precision, method, n):
     """CG iterations replayed from a captured HIP graph give bitwise the
     results of the same kernels launched one by one, on every replay, with
-    eager work (assembly, uploads, other solves) between the solves."""
-    m = meshgen.cavity_hex(14, perturb=0.1, seed=9)
+    eager work (assembly, uploads, other solves) between the solves (n = 40:
+    with a K-cycle level above the tail)."""
+    m = meshgen.cavity_hex(n, perturb=0.1, seed=9)
     meth = getattr(oa, "OAGPU_" + method)
     out = {}
     for graphs in (0, 1):
-        print(f"[case] graphs={graphs} precision={precision} method={method}", flush=True)
+        print(f"[case] graphs={graphs} precision={precision} method={method} n={n}", flush=True)
         dm = oa.DeviceMesh(m)
         f = oa.FlowCase(dm, oa.default_params(precision, use_graphs=graphs), BC)
         rng = np.random.default_rng(31)
