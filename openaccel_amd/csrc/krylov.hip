@@ -59,16 +59,31 @@ __device__ __forceinline__ void sell_row_product(int i, const int* slice_ptr, co
     }
 }
 
-// y = A (s .* x) (s = nullptr: y = A x)
+// y = A (s .* x) (s = nullptr: y = A x), one row per lane, consecutive
+// slices on one XCD (its L2 holds the neighbours' x)
 template <int N>
-__global__ void __launch_bounds__(kRedBlock) k_spmv(int nn, const int* slice_ptr, const int* slice_w,
-                                                    const int* scols, const double* A,
-                                                    const double* x, const double* s, double* y) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
-        double acc[N];
-        sell_row_product<N, double>(i, slice_ptr, slice_w, scols, A, x, s, acc);
-        for (int m = 0; m < N; ++m) y[N * size_t(i) + m] = acc[m];
+__global__ void __launch_bounds__(256) k_spmv(int nn, const int* slice_ptr, const int* slice_w,
+                                              const int* scols, const double* A, const double* x,
+                                              const double* s, double* y) {
+    const int i = int(xcd_block() * blockDim.x + threadIdx.x);
+    if (i >= nn) return;
+    const int sl = i >> 6, ln = i & 63;
+    const size_t s0 = size_t(slice_ptr[sl]);
+    const int w = slice_w[sl];
+    const int* cp = scols + s0 * 64 + ln;
+    const double* ap = A + s0 * (N * N * 64) + ln;
+    double acc[N];
+    for (int m = 0; m < N; ++m) acc[m] = 0.0;
+#pragma unroll 4
+    for (int q = 0; q < w; ++q) {
+        const size_t c = size_t(N) * cp[64 * q];
+        double xv[N];
+        for (int r = 0; r < N; ++r) xv[r] = s ? s[c + r] * x[c + r] : x[c + r];
+        const double* blk = ap + size_t(q) * (N * N * 64);
+        for (int m = 0; m < N; ++m)
+            for (int r = 0; r < N; ++r) acc[m] += blk[(N * m + r) * 64] * xv[r];
     }
+    for (int m = 0; m < N; ++m) y[N * size_t(i) + m] = acc[m];
 }
 
 __device__ inline double block_sum(double v, double* sh) {
@@ -165,8 +180,8 @@ __global__ void k_residual_copy(size_t n, const double* b, const double* Ax, dou
 template <int N>
 static void spmv(Flow& f, const double* A, const double* x, const double* s, double* y) {
     DevMesh& m = *f.m;
-    k_spmv<N><<<kRedGrid, kRedBlock, 0, f.stream>>>(m.nn, m.slice_ptr.p, m.slice_w.p, m.sell_cols.p,
-                                                     A, x, s, y);
+    k_spmv<N><<<xcd_grid(size_t(m.nn), 256), 256, 0, f.stream>>>(m.nn, m.slice_ptr.p, m.slice_w.p,
+                                                                  m.sell_cols.p, A, x, s, y);
     OAGPU_HIP(hipGetLastError());
 }
 
