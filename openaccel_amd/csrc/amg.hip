@@ -12,8 +12,8 @@
 // * The hierarchy structure (aggregates, coarse patterns, the fine-slot ->
 //   coarse-slot map) depends only on the mesh graph and is built once, on the
 //   host.  Every solve recomputes the coarse values on the device from the
-//   current fine matrix: one lane per coarse row sums the slots of its member
-//   rows in a fixed order (no atomics, bitwise reproducible).
+//   current fine matrix: one lane per coarse sliced-ELL slot sums the fine
+//   slots listed for it in a fixed order (no atomics, bitwise reproducible).
 // * Cycle: V(1,1) with damped Jacobi (omega = 2/3) and the coarse correction
 //   scaled by alpha = 2 (unsmoothed aggregation under-corrects), symmetric, so
 //   it preconditions CG and BiCGStab.  The coarsest level (<= 64 rows) is
@@ -59,24 +59,19 @@ __device__ __forceinline__ size_t sv_base(const SellView& s, int i, int& w) {
 }
 
 // --------------------------------------------------------------- set-up
+// Galerkin coarse values: one lane per coarse sliced-ELL slot sums the fine
+// slots listed for it (members ascending, then their slots ascending; an
+// empty list is padding or an absent entry and gives 0), so the sum order is
+// fixed and the result bitwise reproducible.
 template <typename V>
-__global__ void __launch_bounds__(256) k_amg_galerkin(SellView c, const int* mem_ptr, const int* mem,
-                                                      SellView fl, const uint8_t* gmap, const V* Af,
-                                                      V* Ac) {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= c.n) return;
-    int cw;
-    const size_t cb = sv_base(c, I, cw);
-    for (int q = 0; q < cw; ++q) Ac[cb + size_t(q) * 64] = V(0);
-    for (int k = mem_ptr[I]; k < mem_ptr[I + 1]; ++k) {
-        const int i = mem[k];
-        int fw;
-        const size_t fb = sv_base(fl, i, fw);
-        for (int q = 0; q < fw; ++q) {
-            const int g = gmap[fb + size_t(q) * 64];
-            if (g != 0xFF) Ac[cb + size_t(g) * 64] += Af[fb + size_t(q) * 64];
-        }
-    }
+__global__ void __launch_bounds__(256) k_amg_galerkin(size_t nsell, const int* gptr,
+                                                      const int* glist, const V* Af, V* Ac) {
+    const size_t p = size_t(xcd_block()) * blockDim.x + threadIdx.x;
+    if (p >= nsell) return;
+    V s = V(0);
+    const int k1 = gptr[p + 1];
+    for (int k = gptr[p]; k < k1; ++k) s += Af[glist[k]];
+    Ac[p] = s;
 }
 
 template <typename V>
@@ -642,19 +637,7 @@ void amg_build(Flow& f) {
             ccols.insert(ccols.end(), row.begin(), row.end());
             crp[I + 1] = int(ccols.size());
         }
-        if (int(ccols.size()) != crp[nc] || crp[nc] == 0) break;  // a row too long for 8-bit slots
-        // fine slot -> coarse slot
-        std::vector<uint8_t> gmap(fine_nsell, 0xFF);
-        for (int i = 0; i < nf; ++i) {
-            const int I = agg[i];
-            const int* cb = ccols.data() + crp[I];
-            const int clen = crp[I + 1] - crp[I];
-            for (int q = fl.h_row_ptr[i]; q < fl.h_row_ptr[i + 1]; ++q) {
-                const int J = agg[fl.h_cols[q]];
-                const int pos = int(std::lower_bound(cb, cb + clen, J) - cb);
-                gmap[sell_index(fine_slice_ptr, i, q - fl.h_row_ptr[i], 1, 0)] = uint8_t(pos);
-            }
-        }
+        if (int(ccols.size()) != crp[nc] || crp[nc] == 0) break;  // a coarse row over 255 entries
         AmgLevel cl;
         cl.n = nc;
         cl.h_row_ptr = crp;
@@ -663,6 +646,37 @@ void amg_build(Flow& f) {
         SellHost sh = make_sell(nc, crp, ccols);
         cl.nslice = sh.nslice;
         cl.nsell = sh.nsell;
+        // coarse slot <- fine slots, in the order of the members and their slots
+        {
+            std::vector<int> gptr(sh.nsell + 1, 0), glist;
+            std::vector<size_t> target;
+            target.reserve(size_t(fl.h_row_ptr[nf]));
+            for (int I = 0; I < nc; ++I) {
+                const int* cb = ccols.data() + crp[I];
+                const int clen = crp[I + 1] - crp[I];
+                for (int k = mem_ptr[I]; k < mem_ptr[I + 1]; ++k) {
+                    const int i = mem[k];
+                    for (int q = fl.h_row_ptr[i]; q < fl.h_row_ptr[i + 1]; ++q) {
+                        const int g = int(std::lower_bound(cb, cb + clen, agg[fl.h_cols[q]]) - cb);
+                        target.push_back(sell_index(sh.slice_ptr, I, g, 1, 0));
+                        ++gptr[target.back() + 1];
+                    }
+                }
+            }
+            std::partial_sum(gptr.begin(), gptr.end(), gptr.begin());
+            glist.resize(target.size());
+            std::vector<int> pos(gptr.begin(), gptr.end() - 1);
+            size_t t = 0;
+            for (int I = 0; I < nc; ++I)
+                for (int k = mem_ptr[I]; k < mem_ptr[I + 1]; ++k) {
+                    const int i = mem[k];
+                    for (int q = fl.h_row_ptr[i]; q < fl.h_row_ptr[i + 1]; ++q)
+                        glist[pos[target[t++]]++] =
+                            int(sell_index(fine_slice_ptr, i, q - fl.h_row_ptr[i], 1, 0));
+                }
+            cl.gptr.upload(gptr);
+            cl.glist.upload(glist);
+        }
         std::vector<int> doff(nc);
         for (int I = 0; I < nc; ++I) {
             const int* b = ccols.data() + crp[I];
@@ -675,7 +689,6 @@ void amg_build(Flow& f) {
         cl.agg.upload(agg);
         cl.mem_ptr.upload(mem_ptr);
         cl.mem.upload(mem);
-        cl.gmap.upload(gmap);
         cl.A.alloc(cl.nsell);
         cl.fA.alloc(cl.nsell);
         cl.A.zero(f.stream);
@@ -734,10 +747,9 @@ void amg_setup(Flow& f, const V* A0, const V* dinv0) {
     const int nl = int(amg.lv.size());
     for (int l = 1; l < nl; ++l) {
         AmgLevel& c = amg.lv[l];
-        const SellView fv = l == 1 ? view_of(*f.m) : view_of(amg.lv[l - 1]);
         const V* Af = l == 1 ? A0 : lvA<V>(amg.lv[l - 1]);
-        k_amg_galerkin<V><<<blocks(c.n), 256, 0, f.stream>>>(view_of(c), c.mem_ptr.p, c.mem.p, fv,
-                                                             c.gmap.p, Af, lvA<V>(c));
+        k_amg_galerkin<V><<<xcd_grid(c.nsell, 256), 256, 0, f.stream>>>(c.nsell, c.gptr.p, c.glist.p,
+                                                                        Af, lvA<V>(c));
         k_amg_dinv<V><<<blocks(c.n), 256, 0, f.stream>>>(view_of(c), lvA<V>(c), lvv<V>(c, 0));
     }
     if (amg.dense_n && nl > 1) {

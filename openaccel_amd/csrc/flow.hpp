@@ -21,7 +21,7 @@ struct AmgLevel {
     std::vector<int> h_row_ptr, h_cols, h_agg;  // host pattern and aggregation
     DBuf<int> agg;           // [rows of level l-1] -> row of level l
     DBuf<int> mem_ptr, mem;  // rows of level l-1 aggregated into each row (ascending)
-    DBuf<uint8_t> gmap;      // [level l-1 sliced-ELL slots] -> slot of the receiving row here
+    DBuf<int> gptr, glist;   // per sliced-ELL slot here: the level l-1 slots summed into it, in order
     DBuf<double> A, dinv, X, T, R, S;
     DBuf<float> fA, fdinv, fX, fT, fR, fS;
     DBuf<double> C1, C2, V1, R2;  // K-cycle vectors (levels solved by two inner FCG steps)
