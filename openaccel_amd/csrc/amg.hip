@@ -228,14 +228,17 @@ __device__ inline double amg_block_sum(double v, double* sh) {
 //   c1 = B rc, v1 = A c1, rho1 = c1.v1, a1 = c1.rc, r2 = rc - (a1/rho1) v1,
 //   c2 = B r2, gam = c2.v1, bet = c2.A c2, a2 = c2.r2, rho2 = bet - gam^2/rho1,
 //   e = (a1/rho1 - gam a2/(rho1 rho2)) c1 + (a2/rho2) c2.
-// Dot products are block partials over a fixed grid of G blocks; the last
-// block to finish reduces them in a fixed order and leaves the step
-// coefficients in coef[] (0: a1/rho1, 1: rho1, 2: a1, 3: ca, 4: cb, with
-// e = ca c1 + cb c2).  r2 is formed inside the pre-smoothing kernel of the
-// second visit, e inside the prolongation that consumes it (on the coarse
-// levels; for the fine level by k_kc_combine).
+// Dot products are block partials over a fixed grid of G blocks
+// (kp[k*G+b]); every block that needs a step coefficient reduces them in the
+// same fixed order (no inter-block signalling; bitwise reproducible).  r2 is
+// formed inside the pre-smoothing kernel of the second visit, e inside the
+// prolongation that consumes it (on the coarse levels; for the fine level by
+// k_kc_combine, as every one of the fine prolongation's blocks reducing the
+// partials would cost more than that pass).
 struct KcRef {
-    const double* coef = nullptr;
+    const double* kp1 = nullptr;  // rho1, a1
+    const double* kp2 = nullptr;  // bet, gam, a2
+    int G = 0;
 };
 
 __device__ inline double kc_reduce(const double* kp, int k, int G, double* sh) {
@@ -244,9 +247,32 @@ __device__ inline double kc_reduce(const double* kp, int k, int G, double* sh) {
     return amg_block_sum(v, sh);
 }
 
+// a1 / rho1 (0 when c1 = 0)
+__device__ inline double kc_step1(const KcRef& k, double* sh) {
+    const double rho1 = kc_reduce(k.kp1, 0, k.G, sh), a1 = kc_reduce(k.kp1, 1, k.G, sh);
+    return rho1 != 0.0 ? a1 / rho1 : 0.0;
+}
+
+// e = ca c1 + cb c2
+__device__ inline void kc_coeffs(const KcRef& k, double* sh, double& ca, double& cb) {
+    const double rho1 = kc_reduce(k.kp1, 0, k.G, sh), a1 = kc_reduce(k.kp1, 1, k.G, sh);
+    const double bet = kc_reduce(k.kp2, 0, k.G, sh), gam = kc_reduce(k.kp2, 1, k.G, sh);
+    const double a2 = kc_reduce(k.kp2, 2, k.G, sh);
+    ca = 0.0;
+    cb = 0.0;
+    if (rho1 > 0.0) {
+        ca = a1 / rho1;
+        const double rho2 = bet - gam * gam / rho1;
+        if (rho2 > 1e-12 * bet) {
+            cb = a2 / rho2;
+            ca -= gam * cb / rho1;
+        }
+    }
+}
+
 // Pre-smoothing from a zero guess fused with the residual:
 //   x = omega D^-1 r,  res = r - A x  (x of the neighbours formed on the fly)
-// With kc.coef (second visit of a K-cycle level) the right-hand side is
+// With kc.kp1 (second visit of a K-cycle level) the right-hand side is
 // r - (a1/rho1) v1, formed on the fly and written to rout.
 // FINE only names the fine-level instance separately (profiles, timing).
 template <typename V, bool FINE>
@@ -255,14 +281,15 @@ __global__ void __launch_bounds__(256) k_amg_pre_residual(SellView s, const V* A
                                                           const int* flag, KcRef kc = {},
                                                           const V* v1 = nullptr,
                                                           V* rout = nullptr) {
+    __shared__ double sh[256];
     if (flag[0]) return;
-    const V sc = kc.coef ? V(kc.coef[0]) : V(0);
+    const V sc = kc.kp1 ? V(kc_step1(kc, sh)) : V(0);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n) return;
     int w;
     const size_t b = sv_base(s, i, w);
     V acc = V(0);
-    if (kc.coef) {
+    if (kc.kp1) {
 #pragma unroll 4
         for (int q = 0; q < w; ++q) {
             const int j = s.cols[b + size_t(q) * 64];
@@ -275,8 +302,8 @@ __global__ void __launch_bounds__(256) k_amg_pre_residual(SellView s, const V* A
             acc += A[b + size_t(q) * 64] * (dinv[j] * r[j]);
         }
     }
-    const V ri = kc.coef ? r[i] - sc * v1[i] : r[i];
-    if (kc.coef) rout[i] = ri;
+    const V ri = kc.kp1 ? r[i] - sc * v1[i] : r[i];
+    if (kc.kp1) rout[i] = ri;
     x[i] = omega * dinv[i] * ri;
     res[i] = ri - omega * acc;
 }
@@ -298,7 +325,13 @@ __global__ void __launch_bounds__(256) k_amg_prolong_smooth(SellView s, const V*
     __shared__ double sh[256];
     if (flag[0]) return;
     // K-cycle correction: alpha P (ca c1 + cb c2), c1 = ec, c2 = ec2
-    const V ca = kc.coef ? V(kc.coef[3]) : alpha, cb = kc.coef ? V(kc.coef[4]) : V(0);
+    V ca = alpha, cb = V(0);
+    if (kc.kp2) {
+        double a, b;
+        kc_coeffs(kc, sh, a, b);
+        ca = V(a);
+        cb = V(b);
+    }
     double rz = 0.0, qz = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
         int w;
@@ -337,18 +370,13 @@ __global__ void __launch_bounds__(256) k_amg_prolong_smooth(SellView s, const V*
     }
 }
 
-// v = A c (v may be null); partials of c.(A c), c.y1 and c.y2 (y2 may be
-// null) into kp[k*G+b]; the last block reduces them: step 1 (c = c1, y1 =
-// rc) leaves a1/rho1, rho1, a1 in coef[0..2]; step 2 (c = c2, y1 = v1,
-// y2 = r2) the combination ca, cb in coef[3..4].  *cnt counts finished
-// blocks and is back at 0 when the kernel ends.
+// v = A c (v may be null); block partials of c.(A c), c.y1 and c.y2 (y2 may
+// be null) into kp[k*G+b]
 template <typename V>
 __global__ void __launch_bounds__(256) k_kc_dots(SellView s, const V* A, const V* c, V* v,
                                                  const V* y1, const V* y2, double* kp,
-                                                 double* coef, unsigned* cnt, int step,
                                                  const int* flag) {
     __shared__ double sh[256];
-    __shared__ int last;
     if (flag[0]) return;
     double d0 = 0.0, d1 = 0.0, d2 = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
@@ -371,34 +399,6 @@ __global__ void __launch_bounds__(256) k_kc_dots(SellView s, const V* A, const V
         kp[blockIdx.x] = d0;
         kp[G + blockIdx.x] = d1;
         kp[2 * G + blockIdx.x] = d2;
-        __threadfence();
-        last = atomicAdd(cnt, 1u) == unsigned(G - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    const double e0 = kc_reduce(kp, 0, G, sh), e1 = kc_reduce(kp, 1, G, sh);
-    const double e2 = kc_reduce(kp, 2, G, sh);
-    if (threadIdx.x == 0) {
-        if (step == 1) {
-            coef[0] = e0 != 0.0 ? e1 / e0 : 0.0;
-            coef[1] = e0;
-            coef[2] = e1;
-        } else {
-            const double rho1 = coef[1], a1 = coef[2], bet = e0, gam = e1, a2 = e2;
-            double ca = 0.0, cb = 0.0;
-            if (rho1 > 0.0) {
-                ca = a1 / rho1;
-                const double rho2 = bet - gam * gam / rho1;
-                if (rho2 > 1e-12 * bet) {
-                    cb = a2 / rho2;
-                    ca -= gam * cb / rho1;
-                }
-            }
-            coef[3] = ca;
-            coef[4] = cb;
-        }
-        atomicExch(cnt, 0u);
     }
 }
 
@@ -406,11 +406,14 @@ __global__ void __launch_bounds__(256) k_kc_dots(SellView s, const V* A, const V
 // for the fine-level prolongation (gathering both vectors there costs more
 // than this pass over the coarse level)
 template <typename V>
-__global__ void __launch_bounds__(256) k_kc_combine(int n, const double* coef, V* c1, const V* c2,
+__global__ void __launch_bounds__(256) k_kc_combine(int n, KcRef kc, V* c1, const V* c2,
                                                     const int* flag) {
+    __shared__ double sh[256];
     if (flag[0]) return;
+    double a, b;
+    kc_coeffs(kc, sh, a, b);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) c1[i] = V(coef[3]) * c1[i] + V(coef[4]) * c2[i];
+    if (i < n) c1[i] = V(a) * c1[i] + V(b) * c2[i];
 }
 
 // x = Minv r on the coarsest level (one workgroup; Minv stored transposed)
@@ -707,10 +710,6 @@ void amg_build(Flow& f) {
     for (auto* b : {&l0.T, &l0.S}) b->alloc(m.nn);
     for (auto* b : {&l0.fT, &l0.fS}) b->alloc(m.nn);
     const int nl = int(amg.lv.size());
-    amg.kc_coef.alloc(8 * size_t(kMaxLevels));
-    amg.kc_coef.zero(f.stream);
-    amg.kc_cnt.alloc(kMaxLevels);
-    amg.kc_cnt.zero(f.stream);
     amg.dense_n = nl > 1 && amg.lv[nl - 1].n <= kDenseMax ? amg.lv[nl - 1].n : 0;
     if (amg.dense_n) {
         amg.dense.alloc(size_t(amg.dense_n) * amg.dense_n);
@@ -803,21 +802,20 @@ struct AmgCycle {
             V* v1 = lvv<V>(cl, 7);
             V* r2 = lvv<V>(cl, 8);
             const int G = int(std::min<unsigned>(blocks(cl.n), unsigned(kRedGrid)));
-            double* kp = f.part.p + size_t(16 + 6 * c) * kRedGrid;
-            double* coef = amg.kc_coef.p + 8 * c;
-            unsigned* cnt = amg.kc_cnt.p + c;
+            double* kp1 = f.part.p + size_t(16 + 6 * c) * kRedGrid;
+            double* kp2 = kp1 + 3 * size_t(kRedGrid);
             k_amg_restrict<V><<<blocks(cl.n), 256, 0, f.stream>>>(cl.n, cl.mem_ptr.p, cl.mem.p, S, om,
                                                                   nullptr, rc, nullptr, flag);
             level(c, rc, c1);
-            k_kc_dots<V><<<G, 256, 0, f.stream>>>(view(c), A(c), c1, v1, rc, nullptr, kp, coef, cnt, 1,
-                                                  flag);
+            k_kc_dots<V><<<G, 256, 0, f.stream>>>(view(c), A(c), c1, v1, rc, nullptr, kp1, flag);
             KcRef k;
-            k.coef = coef;
+            k.kp1 = kp1;
+            k.G = G;
             level(c, rc, c2, k, v1, r2);  // right-hand side r2 = rc - (a1/rho1) v1
-            k_kc_dots<V><<<G, 256, 0, f.stream>>>(view(c), A(c), c2, nullptr, v1, r2, kp, coef, cnt, 2,
-                                                  flag);
+            k_kc_dots<V><<<G, 256, 0, f.stream>>>(view(c), A(c), c2, nullptr, v1, r2, kp2, flag);
+            k.kp2 = kp2;
             if (l == 0) {
-                k_kc_combine<V><<<blocks(cl.n), 256, 0, f.stream>>>(cl.n, coef, c1, c2, flag);
+                k_kc_combine<V><<<blocks(cl.n), 256, 0, f.stream>>>(cl.n, k, c1, c2, flag);
                 return {c1, nullptr, V(1), {}};
             }
             return {c1, c2, V(1), k};
@@ -866,13 +864,13 @@ struct AmgCycle {
 
     // out ~ A_l^-1 rhs on a level 1 <= l < min(t, last): pre-smoothing and
     // residual, the coarse correction, prolongation and post-smoothing.
-    // With pre.coef the right-hand side is rhs - (a1/rho1) v1, written to rout.
+    // With pre.kp1 the right-hand side is rhs - (a1/rho1) v1, written to rout.
     void level(int l, const V* rhs, V* out, KcRef pre = {}, const V* v1 = nullptr,
                V* rout = nullptr) {
         AmgLevel& L = amg.lv[l];
         k_amg_pre_residual<V, false><<<blocks(L.n), 256, 0, f.stream>>>(
             view(l), A(l), D(l), om, rhs, X(l), lvv<V>(L, 4), flag, pre, v1, rout);
-        const V* r = pre.coef ? rout : rhs;
+        const V* r = pre.kp1 ? rout : rhs;
         const Corr cc = coarse(l);
         k_amg_prolong_smooth<V, false><<<blocks(L.n), 256, 0, f.stream>>>(
             view(l), A(l), D(l), om, r, X(l), amg.lv[l + 1].agg.p, cc.alpha, cc.ec, out, nullptr,

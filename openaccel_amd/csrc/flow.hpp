@@ -35,8 +35,6 @@ struct Amg {
     int dense_n = 0;           // coarsest level solved by its dense inverse when > 0
     DBuf<double> dense;        // [dense_n][dense_n] inverse of the coarsest matrix
     DBuf<float> fdense;
-    DBuf<double> kc_coef;      // [level][8] K-cycle step coefficients
-    DBuf<unsigned> kc_cnt;     // [level] finished-block counters of the K-cycle dot kernels
 };
 
 // An instantiated HIP graph of a block of CG iterations and the operands it
