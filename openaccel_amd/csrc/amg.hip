@@ -274,13 +274,16 @@ __device__ inline void kc_coeffs(const KcRef& k, double* sh, double& ca, double&
 //   x = omega D^-1 r,  res = r - A x  (x of the neighbours formed on the fly)
 // With kc.kp1 (second visit of a K-cycle level) the right-hand side is
 // r - (a1/rho1) v1, formed on the fly and written to rout.
+// With spos the residual is stored at spos[i] (member order of the next
+// level, for the one-workgroup tail's restriction).
 // FINE only names the fine-level instance separately (profiles, timing).
 template <typename V, bool FINE>
 __global__ void __launch_bounds__(256) k_amg_pre_residual(SellView s, const V* A, const V* dinv,
                                                           V omega, const V* r, V* x, V* res,
                                                           const int* flag, KcRef kc = {},
                                                           const V* v1 = nullptr,
-                                                          V* rout = nullptr) {
+                                                          V* rout = nullptr,
+                                                          const int* spos = nullptr) {
     __shared__ double sh[256];
     if (flag[0]) return;
     const V sc = kc.kp1 ? V(kc_step1(kc, sh)) : V(0);
@@ -305,7 +308,7 @@ __global__ void __launch_bounds__(256) k_amg_pre_residual(SellView s, const V* A
     const V ri = kc.kp1 ? r[i] - sc * v1[i] : r[i];
     if (kc.kp1) rout[i] = ri;
     x[i] = omega * dinv[i] * ri;
-    res[i] = ri - omega * acc;
+    res[spos ? spos[i] : i] = ri - omega * acc;
 }
 
 // Prolongation fused with post-smoothing (out of place):
@@ -449,23 +452,27 @@ struct TailLevel {
     const int* agg;      // level l-1 row -> level l row
     const int* mem_ptr;  // members (level l-1 rows) of each level l row
     const int* mem;
+    const int* mpos;     // level l-1 row -> its position in mem (residuals are stored there)
 };
 
 template <typename V>
 struct TailArgs {
     int t, last, dense_n;
     V omega, alpha;
-    const V* res_in;  // residual of level t-1
+    const V* res_in;  // residual of level t-1, stored in member order of level t
     const V* inv;     // dense inverse of the coarsest level (transposed) or null
     TailLevel<V> lv[kMaxLevels];
 };
 
 template <typename V>
+// restriction from a residual stored in member order: each aggregate sums a
+// contiguous range (no index gather; the same order as over mem)
 __device__ __forceinline__ void tail_restrict(const TailLevel<V>& c, const V* res) {
     for (int I = threadIdx.x; I < c.s.n; I += blockDim.x) {
         V s = V(0);
-#pragma unroll 4
-        for (int k = c.mem_ptr[I]; k < c.mem_ptr[I + 1]; ++k) s += res[c.mem[k]];
+        const int k1 = c.mem_ptr[I + 1];
+#pragma unroll 8
+        for (int k = c.mem_ptr[I]; k < k1; ++k) s += res[k];
         c.R[I] = s;
     }
 }
@@ -489,7 +496,7 @@ __global__ void __launch_bounds__(kTailThreads) k_amg_tail(const TailArgs<V> a, 
             }
             const V ri = L.R[i];
             L.X[i] = om * L.dinv[i] * ri;
-            L.S[i] = ri - om * acc;
+            L.S[a.lv[l + 1].mpos[i]] = ri - om * acc;
         }
         __syncthreads();
         tail_restrict(a.lv[l + 1], L.S);
@@ -692,6 +699,11 @@ void amg_build(Flow& f) {
         cl.agg.upload(agg);
         cl.mem_ptr.upload(mem_ptr);
         cl.mem.upload(mem);
+        {
+            std::vector<int> mpos(nf);
+            for (int k = 0; k < nf; ++k) mpos[mem[k]] = k;
+            cl.mpos.upload(mpos);
+        }
         cl.A.alloc(cl.nsell);
         cl.fA.alloc(cl.nsell);
         cl.A.zero(f.stream);
@@ -777,6 +789,8 @@ struct AmgCycle {
     const V* dinv0;
 
     SellView view(int l) const { return l == 0 ? view_of(*f.m) : view_of(amg.lv[l]); }
+    // the residual of the level above the tail is stored in the tail's member order
+    const int* spos(int l) const { return l + 1 == t && t <= last ? amg.lv[t].mpos.p : nullptr; }
     const V* A(int l) const { return l == 0 ? A0 : lvA<V>(amg.lv[l]); }
     const V* D(int l) const { return l == 0 ? dinv0 : lvv<V>(amg.lv[l], 0); }
     V* X(int l) const { return lvv<V>(amg.lv[l], l == 0 ? 2 : 1); }  // T0 on level 0
@@ -833,7 +847,7 @@ struct AmgCycle {
             for (int k = t; k <= last; ++k) {
                 AmgLevel& L = amg.lv[k];
                 ta.lv[k] = {view(k), A(k), D(k), lvv<V>(L, 3), lvv<V>(L, 1), lvv<V>(L, 4), lvv<V>(L, 2),
-                            L.agg.p, L.mem_ptr.p, L.mem.p};
+                            L.agg.p, L.mem_ptr.p, L.mem.p, L.mpos.p};
             }
             k_amg_tail<V><<<1, kTailThreads, 0, f.stream>>>(ta, flag);
             return {t == last ? X(last) : lvv<V>(amg.lv[t], 2), nullptr, al, {}};
@@ -869,7 +883,7 @@ struct AmgCycle {
                V* rout = nullptr) {
         AmgLevel& L = amg.lv[l];
         k_amg_pre_residual<V, false><<<blocks(L.n), 256, 0, f.stream>>>(
-            view(l), A(l), D(l), om, rhs, X(l), lvv<V>(L, 4), flag, pre, v1, rout);
+            view(l), A(l), D(l), om, rhs, X(l), lvv<V>(L, 4), flag, pre, v1, rout, spos(l));
         const V* r = pre.kp1 ? rout : rhs;
         const Corr cc = coarse(l);
         k_amg_prolong_smooth<V, false><<<blocks(L.n), 256, 0, f.stream>>>(
@@ -895,8 +909,9 @@ void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* 
     AmgCycle<V> cy{f, amg, f.flag.p, V(kOmega), V(kAlpha), t, last, kcycle, A0, dinv0};
     AmgLevel& l0 = amg.lv[0];
     if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[0], f.stream));
-    k_amg_pre_residual<V, true><<<blocks(l0.n), 256, 0, f.stream>>>(cy.view(0), A0, dinv0, cy.om, r,
-                                                                    cy.X(0), lvv<V>(l0, 4), cy.flag);
+    k_amg_pre_residual<V, true><<<blocks(l0.n), 256, 0, f.stream>>>(
+        cy.view(0), A0, dinv0, cy.om, r, cy.X(0), lvv<V>(l0, 4), cy.flag, {}, nullptr, nullptr,
+        cy.spos(0));
     if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[1], f.stream));
     const auto cc = cy.coarse(0);
     if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[2], f.stream));

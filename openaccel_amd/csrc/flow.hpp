@@ -21,6 +21,7 @@ struct AmgLevel {
     std::vector<int> h_row_ptr, h_cols, h_agg;  // host pattern and aggregation
     DBuf<int> agg;           // [rows of level l-1] -> row of level l
     DBuf<int> mem_ptr, mem;  // rows of level l-1 aggregated into each row (ascending)
+    DBuf<int> mpos;          // [rows of level l-1] -> position in mem
     DBuf<int> gptr, glist;   // per sliced-ELL slot here: the level l-1 slots summed into it, in order
     DBuf<double> A, dinv, X, T, R, S;
     DBuf<float> fA, fdinv, fX, fT, fR, fS;
