@@ -169,9 +169,9 @@ __global__ void __launch_bounds__(256) k_amg_residual(SellView s, const V* A, co
     res[i] = r[i] - acc;
 }
 
-// rc[I] = sum of res over the members of I; xc = omega dinv_c rc (first
-// smoothing step of the coarse level from a zero guess; skipped when
-// dinv_c is null)
+// rc[I] = sum of res over the members of I (res stored in member order: a
+// contiguous range); xc = omega dinv_c rc (first smoothing step of the
+// coarse level from a zero guess; skipped when dinv_c is null)
 template <typename V>
 __global__ void __launch_bounds__(256) k_amg_restrict(int nc, const int* mem_ptr, const int* mem,
                                                       const V* res, V omega, const V* dinvc, V* rc,
@@ -180,8 +180,9 @@ __global__ void __launch_bounds__(256) k_amg_restrict(int nc, const int* mem_ptr
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= nc) return;
     V s = V(0);
-#pragma unroll 4
-    for (int k = mem_ptr[I]; k < mem_ptr[I + 1]; ++k) s += res[mem[k]];
+    const int k1 = mem_ptr[I + 1];
+#pragma unroll 8
+    for (int k = mem_ptr[I]; k < k1; ++k) s += res[k];  // res in member order
     rc[I] = s;
     if (dinvc) xc[I] = omega * dinvc[I] * s;
 }
@@ -789,8 +790,9 @@ struct AmgCycle {
     const V* dinv0;
 
     SellView view(int l) const { return l == 0 ? view_of(*f.m) : view_of(amg.lv[l]); }
-    // the residual of the level above the tail is stored in the tail's member order
-    const int* spos(int l) const { return l + 1 == t && t <= last ? amg.lv[t].mpos.p : nullptr; }
+    // a level's residual is stored in the member order of the next level, so
+    // that every restriction sums contiguous ranges
+    const int* spos(int l) const { return amg.lv[l + 1].mpos.p; }
     const V* A(int l) const { return l == 0 ? A0 : lvA<V>(amg.lv[l]); }
     const V* D(int l) const { return l == 0 ? dinv0 : lvv<V>(amg.lv[l], 0); }
     V* X(int l) const { return lvv<V>(amg.lv[l], l == 0 ? 2 : 1); }  // T0 on level 0
