@@ -169,9 +169,9 @@ __global__ void __launch_bounds__(256) k_amg_residual(SellView s, const V* A, co
     res[i] = r[i] - acc;
 }
 
-// rc[I] = sum of res over the members of I (res stored in member order: a
-// contiguous range); xc = omega dinv_c rc (first smoothing step of the
-// coarse level from a zero guess; skipped when dinv_c is null)
+// rc[I] = sum of res over the members of I (mem == nullptr: res stored in
+// member order, a contiguous range); xc = omega dinv_c rc (first smoothing
+// step of the coarse level from a zero guess; skipped when dinv_c is null)
 template <typename V>
 __global__ void __launch_bounds__(256) k_amg_restrict(int nc, const int* mem_ptr, const int* mem,
                                                       const V* res, V omega, const V* dinvc, V* rc,
@@ -181,8 +181,13 @@ __global__ void __launch_bounds__(256) k_amg_restrict(int nc, const int* mem_ptr
     if (I >= nc) return;
     V s = V(0);
     const int k1 = mem_ptr[I + 1];
+    if (mem) {
+#pragma unroll 4
+        for (int k = mem_ptr[I]; k < k1; ++k) s += res[mem[k]];
+    } else {
 #pragma unroll 8
-    for (int k = mem_ptr[I]; k < k1; ++k) s += res[k];  // res in member order
+        for (int k = mem_ptr[I]; k < k1; ++k) s += res[k];  // res in member order
+    }
     rc[I] = s;
     if (dinvc) xc[I] = omega * dinvc[I] * s;
 }
@@ -790,9 +795,12 @@ struct AmgCycle {
     const V* dinv0;
 
     SellView view(int l) const { return l == 0 ? view_of(*f.m) : view_of(amg.lv[l]); }
-    // a level's residual is stored in the member order of the next level, so
-    // that every restriction sums contiguous ranges
-    const int* spos(int l) const { return amg.lv[l + 1].mpos.p; }
+    // a coarse level's residual is stored in the member order of the next
+    // level, so that its restriction sums contiguous ranges; the fine level's
+    // only when the tail restricts it (scattered stores cost more there than
+    // the gather of the restriction saves)
+    const int* spos(int l) const { return l > 0 || t == 1 ? amg.lv[l + 1].mpos.p : nullptr; }
+    const int* rmem(int l) const { return spos(l) ? nullptr : amg.lv[l + 1].mem.p; }
     const V* A(int l) const { return l == 0 ? A0 : lvA<V>(amg.lv[l]); }
     const V* D(int l) const { return l == 0 ? dinv0 : lvv<V>(amg.lv[l], 0); }
     V* X(int l) const { return lvv<V>(amg.lv[l], l == 0 ? 2 : 1); }  // T0 on level 0
@@ -820,7 +828,7 @@ struct AmgCycle {
             const int G = int(std::min<unsigned>(blocks(cl.n), unsigned(kRedGrid)));
             double* kp1 = f.part.p + size_t(16 + 6 * c) * kRedGrid;
             double* kp2 = kp1 + 3 * size_t(kRedGrid);
-            k_amg_restrict<V><<<blocks(cl.n), 256, 0, f.stream>>>(cl.n, cl.mem_ptr.p, cl.mem.p, S, om,
+            k_amg_restrict<V><<<blocks(cl.n), 256, 0, f.stream>>>(cl.n, cl.mem_ptr.p, rmem(l), S, om,
                                                                   nullptr, rc, nullptr, flag);
             level(c, rc, c1);
             k_kc_dots<V><<<G, 256, 0, f.stream>>>(view(c), A(c), c1, v1, rc, nullptr, kp1, flag);
@@ -855,7 +863,7 @@ struct AmgCycle {
             return {t == last ? X(last) : lvv<V>(amg.lv[t], 2), nullptr, al, {}};
         }
         V* rc = lvv<V>(cl, 3);
-        k_amg_restrict<V><<<blocks(cl.n), 256, 0, f.stream>>>(cl.n, cl.mem_ptr.p, cl.mem.p, S, om,
+        k_amg_restrict<V><<<blocks(cl.n), 256, 0, f.stream>>>(cl.n, cl.mem_ptr.p, rmem(l), S, om,
                                                               nullptr, rc, nullptr, flag);
         if (c < last) {  // a V-cycle level above the tail
             level(c, rc, lvv<V>(cl, 2));
