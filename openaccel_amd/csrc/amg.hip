@@ -916,6 +916,9 @@ void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* 
             t = l;
             break;
         }
+    // K-cycle levels only with at least two levels above the tail: with one,
+    // the extra launches cost more than the iterations they save (32^3 cavity)
+    kcycle = kcycle && t >= 3;
     AmgCycle<V> cy{f, amg, f.flag.p, V(kOmega), V(kAlpha), t, last, kcycle, A0, dinv0};
     AmgLevel& l0 = amg.lv[0];
     if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[0], f.stream));
