@@ -18,6 +18,7 @@
 #include "device_util.hpp"
 #include "flow.hpp"
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <type_traits>
@@ -132,15 +133,42 @@ __global__ void __launch_bounds__(kRedBlock) k_reduce(int np, const double* part
     if (threadIdx.x == 0) out[k] = s;
 }
 
+// x_k . y for up to 8 vectors x_k against one y (a Gram-Schmidt step): y is
+// read once per element for all of them; partials of pair k0 + k to slot k0 + k
+__global__ void __launch_bounds__(kRedBlock) k_dots_y(size_t n, int k0, int nk, DotPairs P,
+                                                      const double* y, double* part) {
+    __shared__ double sh[kRedBlock];
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x) {
+        const double yi = y[i];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < nk) v[k] += P.x[k0 + k][i] * yi;
+    }
+    for (int k = 0; k < nk; ++k) {
+        const double s = block_sum(v[k], sh);
+        if (threadIdx.x == 0) part[size_t(k0 + k) * kRedGrid + blockIdx.x] = s;
+    }
+}
+
 // host-side: np dot products in one pass, results copied to the host
 static void dots(Flow& f, size_t n, int np, const double* const* xs, const double* const* ys,
                  double* out) {
     DotPairs P;
+    bool same_y = np > 1;
     for (int k = 0; k < np; ++k) {
         P.x[k] = xs[k];
         P.y[k] = ys[k];
+        same_y = same_y && ys[k] == ys[0];
     }
-    k_dots<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, np, P, f.part.p);
+    if (same_y) {
+        for (int k0 = 0; k0 < np; k0 += 8)
+            k_dots_y<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, k0, std::min(8, np - k0), P, ys[0],
+                                                           f.part.p);
+    } else {
+        k_dots<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, np, P, f.part.p);
+    }
     k_reduce<<<np, kRedBlock, 0, f.stream>>>(np, f.part.p, f.red.p);
     OAGPU_HIP(hipGetLastError());
     f.red.download(out, np, f.stream);
@@ -155,6 +183,25 @@ __global__ void k_gs_update(size_t n, int nk, const double* V, const double* h, 
     double s = 0.0;
     for (int k = 0; k < nk; ++k) s += h[k] * V[size_t(k) * n + i];
     w[i] -= s;
+}
+
+// w -= sum_k h[k] V_k fused with the block partials of w . w (the norm that
+// follows the second Gram-Schmidt pass)
+__global__ void __launch_bounds__(kRedBlock) k_gs_update_norm(size_t n, int nk, const double* V,
+                                                              const double* h, double* w,
+                                                              double* part) {
+    __shared__ double sh[kRedBlock];
+    double ww = 0.0;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x) {
+        double s = 0.0;
+        for (int k = 0; k < nk; ++k) s += h[k] * V[size_t(k) * n + i];
+        const double wi = w[i] - s;
+        w[i] = wi;
+        ww += wi * wi;
+    }
+    const double t = block_sum(ww, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
 __global__ void k_scale_to(size_t n, double a, const double* x, double* y) {
@@ -229,14 +276,15 @@ static int gmres(Flow& f, int max_it, double rtol, double atol, int mres, oagpu_
             k_gs_update<<<G, B, 0, f.stream>>>(n, j + 1, f.V.p, f.red.p, f.w.p);
             dots(f, n, j + 1, xs.data(), ys.data(), h2.data());
             f.red.upload(h2.data(), j + 1, f.stream);
-            k_gs_update<<<G, B, 0, f.stream>>>(n, j + 1, f.V.p, f.red.p, f.w.p);
+            k_gs_update_norm<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, j + 1, f.V.p, f.red.p, f.w.p,
+                                                                   f.part.p);
+            k_reduce<<<1, kRedBlock, 0, f.stream>>>(1, f.part.p, f.red.p);
+            OAGPU_HIP(hipGetLastError());
             for (int k = 0; k <= j; ++k) h[k] += h2[k];
             double hn;
-            {
-                const double* ws[1] = {f.w.p};
-                dots(f, n, 1, ws, ws, &hn);
-                hn = std::sqrt(hn);
-            }
+            f.red.download(&hn, 1, f.stream);
+            OAGPU_HIP(hipStreamSynchronize(f.stream));
+            hn = std::sqrt(hn);
             for (int k = 0; k <= j; ++k) H[size_t(mres) * k + j] = h[k];
             H[size_t(mres) * (j + 1) + j] = hn;
             if (hn > 0) k_scale_to<<<G, B, 0, f.stream>>>(n, 1.0 / hn, f.w.p, f.V.p + size_t(j + 1) * n);
