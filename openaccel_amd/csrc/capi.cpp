@@ -249,7 +249,7 @@ int oagpu_flow_create(oagpu_mesh* m, const oagpu_flow_params* prm, int n_bc, con
         for (auto* b : {&f.dinv, &f.r, &f.z, &f.pk, &f.q, &f.w, &f.t, &f.s, &f.r0, &f.ph, &f.sh})
             b->alloc(nn * 3);
         f.part.alloc(size_t(64) * oagpu::kRedGrid);
-        f.red.alloc(64);
+        f.red.alloc(192);
         f.scal.alloc(16);
         f.flag.alloc(4);
         for (auto* b : {&f.U, &f.p, &f.gradU, &f.gradP, &f.du, &f.mdot, &f.div, &f.A3, &f.b3, &f.x3,

@@ -185,6 +185,32 @@ __global__ void k_gs_update(size_t n, int nk, const double* V, const double* h, 
     w[i] -= s;
 }
 
+// w -= sum_k h[k] V_k fused with the block partials of V_k . w for the
+// updated w (the second Gram-Schmidt pass), nk <= MAXK
+template <int MAXK>
+__global__ void __launch_bounds__(kRedBlock) k_gs_update_dots(size_t n, int nk, const double* V,
+                                                              const double* h, double* w,
+                                                              double* part) {
+    __shared__ double sh[kRedBlock];
+    double acc[MAXK];
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) acc[k] = 0.0;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x) {
+        double s = 0.0;
+        for (int k = 0; k < nk; ++k) s += h[k] * V[size_t(k) * n + i];
+        const double wi = w[i] - s;
+        w[i] = wi;
+#pragma unroll
+        for (int k = 0; k < MAXK; ++k)
+            if (k < nk) acc[k] += V[size_t(k) * n + i] * wi;
+    }
+    for (int k = 0; k < nk; ++k) {
+        const double t = block_sum(acc[k], sh);
+        if (threadIdx.x == 0) part[size_t(k) * kRedGrid + blockIdx.x] = t;
+    }
+}
+
 // w -= sum_k h[k] V_k fused with the block partials of w . w (the norm that
 // follows the second Gram-Schmidt pass)
 __global__ void __launch_bounds__(kRedBlock) k_gs_update_norm(size_t n, int nk, const double* V,
@@ -268,23 +294,44 @@ static int gmres(Flow& f, int max_it, double rtol, double atol, int mres, oagpu_
         int jend = -1;
         for (int j = 0; j < mres && it < max_it; ++j) {
             spmv<N>(f, A, f.V.p + size_t(j) * n, f.dinv.p, f.w.p);
+            double hn;
             std::vector<const double*> xs(j + 1), ys(j + 1, f.w.p);
             for (int k = 0; k <= j; ++k) xs[k] = f.V.p + size_t(k) * n;
             // classical Gram-Schmidt, twice
-            dots(f, n, j + 1, xs.data(), ys.data(), h.data());
-            f.red.upload(h.data(), j + 1, f.stream);
-            k_gs_update<<<G, B, 0, f.stream>>>(n, j + 1, f.V.p, f.red.p, f.w.p);
-            dots(f, n, j + 1, xs.data(), ys.data(), h2.data());
-            f.red.upload(h2.data(), j + 1, f.stream);
-            k_gs_update_norm<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, j + 1, f.V.p, f.red.p, f.w.p,
-                                                                   f.part.p);
-            k_reduce<<<1, kRedBlock, 0, f.stream>>>(1, f.part.p, f.red.p);
-            OAGPU_HIP(hipGetLastError());
-            for (int k = 0; k <= j; ++k) h[k] += h2[k];
-            double hn;
-            f.red.download(&hn, 1, f.stream);
-            OAGPU_HIP(hipStreamSynchronize(f.stream));
-            hn = std::sqrt(hn);
+            if (j + 1 <= 16) {  // device-resident step: one host synchronisation
+                DotPairs P;
+                for (int k = 0; k <= j; ++k) P.x[k] = xs[k];
+                for (int k0 = 0; k0 <= j; k0 += 8)
+                    k_dots_y<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, k0, std::min(8, j + 1 - k0), P,
+                                                                   f.w.p, f.part.p);
+                k_reduce<<<j + 1, kRedBlock, 0, f.stream>>>(j + 1, f.part.p, f.red.p);
+                k_gs_update_dots<16><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, j + 1, f.V.p, f.red.p,
+                                                                            f.w.p, f.part.p);
+                k_reduce<<<j + 1, kRedBlock, 0, f.stream>>>(j + 1, f.part.p, f.red.p + 64);
+                k_gs_update_norm<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, j + 1, f.V.p, f.red.p + 64,
+                                                                       f.w.p, f.part.p);
+                k_reduce<<<1, kRedBlock, 0, f.stream>>>(1, f.part.p, f.red.p + 128);
+                OAGPU_HIP(hipGetLastError());
+                double hr[129];
+                f.red.download(hr, 129, f.stream);
+                OAGPU_HIP(hipStreamSynchronize(f.stream));
+                for (int k = 0; k <= j; ++k) h[k] = hr[k] + hr[64 + k];
+                hn = std::sqrt(hr[128]);
+            } else {
+                dots(f, n, j + 1, xs.data(), ys.data(), h.data());
+                f.red.upload(h.data(), j + 1, f.stream);
+                k_gs_update<<<G, B, 0, f.stream>>>(n, j + 1, f.V.p, f.red.p, f.w.p);
+                dots(f, n, j + 1, xs.data(), ys.data(), h2.data());
+                f.red.upload(h2.data(), j + 1, f.stream);
+                k_gs_update_norm<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, j + 1, f.V.p, f.red.p, f.w.p,
+                                                                       f.part.p);
+                k_reduce<<<1, kRedBlock, 0, f.stream>>>(1, f.part.p, f.red.p);
+                OAGPU_HIP(hipGetLastError());
+                for (int k = 0; k <= j; ++k) h[k] += h2[k];
+                f.red.download(&hn, 1, f.stream);
+                OAGPU_HIP(hipStreamSynchronize(f.stream));
+                hn = std::sqrt(hn);
+            }
             for (int k = 0; k <= j; ++k) H[size_t(mres) * k + j] = h[k];
             H[size_t(mres) * (j + 1) + j] = hn;
             if (hn > 0) k_scale_to<<<G, B, 0, f.stream>>>(n, 1.0 / hn, f.w.p, f.V.p + size_t(j + 1) * n);
