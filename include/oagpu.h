@@ -102,8 +102,11 @@ typedef struct oagpu_control_data {
 #define OAGPU_GMRES 0
 #define OAGPU_CG 1
 #define OAGPU_BICGSTAB 2
-#define OAGPU_CG_AMG 3   /* CG preconditioned by one aggregation-AMG V-cycle
-                            (scalar systems; the role of HYPRE BoomerAMG in
+#define OAGPU_CG_AMG 3   /* flexible CG preconditioned by one aggregation-AMG
+                            cycle: a K-cycle (two inner flexible-CG steps per
+                            coarse level) when two or more levels lie above
+                            the one-workgroup tail, else a V-cycle (scalar
+                            systems; the role of HYPRE BoomerAMG in
                             liblinsolve's hypre family, linearSystem.h:331) */
 #define OAGPU_BICGSTAB_AMG 4 /* BiCGStab right-preconditioned by one AMG
                             V-cycle (scalar systems that are not symmetric:
