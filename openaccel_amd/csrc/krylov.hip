@@ -61,7 +61,9 @@ __device__ __forceinline__ void sell_row_product(int i, const int* slice_ptr, co
 }
 
 // y = A (s .* x) (s = nullptr: y = A x), one row per lane, consecutive
-// slices on one XCD (its L2 holds the neighbours' x)
+// slices on one XCD (its L2 holds the neighbours' x); the block values and
+// column indices, read once per product and larger than the caches, are
+// loaded non-temporally so that they do not evict x
 template <int N>
 __global__ void __launch_bounds__(256) k_spmv(int nn, const int* slice_ptr, const int* slice_w,
                                               const int* scols, const double* A, const double* x,
@@ -77,12 +79,13 @@ __global__ void __launch_bounds__(256) k_spmv(int nn, const int* slice_ptr, cons
     for (int m = 0; m < N; ++m) acc[m] = 0.0;
 #pragma unroll 4
     for (int q = 0; q < w; ++q) {
-        const size_t c = size_t(N) * cp[64 * q];
+        const size_t c = size_t(N) * __builtin_nontemporal_load(cp + 64 * q);
         double xv[N];
         for (int r = 0; r < N; ++r) xv[r] = s ? s[c + r] * x[c + r] : x[c + r];
         const double* blk = ap + size_t(q) * (N * N * 64);
         for (int m = 0; m < N; ++m)
-            for (int r = 0; r < N; ++r) acc[m] += blk[(N * m + r) * 64] * xv[r];
+            for (int r = 0; r < N; ++r)
+                acc[m] += __builtin_nontemporal_load(blk + (N * m + r) * 64) * xv[r];
     }
     for (int m = 0; m < N; ++m) y[N * size_t(i) + m] = acc[m];
 }
