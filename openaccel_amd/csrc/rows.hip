@@ -870,8 +870,9 @@ __global__ void __launch_bounds__(LDS ? 64 : 256) k_momentum_rows(const MomArgs 
 #pragma unroll 1
         for (int s = 0; s < ns; ++s)
             if (s != dof)
-                for (int k = 0; k < 9; ++k)
-                    *sell_val<9>(a.vals, sr, s, k) = double(lt[(s * 9 + k) * 64]);
+                for (int k = 0; k < 9; ++k)  // streamed out: kept out of the caches
+                    __builtin_nontemporal_store(double(lt[(s * 9 + k) * 64]),
+                                                sell_val<9>(a.vals, sr, s, k));
     }
     for (int k = 0; k < 3; ++k) D[4 * k] *= a.urf_inv;
     if (a.bnode[row])
@@ -903,7 +904,7 @@ __global__ void __launch_bounds__(LDS ? 64 : 256) k_momentum_rows(const MomArgs 
             }
         for (int k = 0; k < 9; ++k) D[k] += (1.0 - 0.75) * mod[k];
     }
-    for (int kk = 0; kk < 9; ++kk) *sell_val<9>(a.vals, sr, dof, kk) = D[kk];
+    for (int kk = 0; kk < 9; ++kk) __builtin_nontemporal_store(D[kk], sell_val<9>(a.vals, sr, dof, kk));
     for (int k = 0; k < 3; ++k) a.b[3 * size_t(row) + k] = R[k];
     if (bad) *a.err = 1;
 }
