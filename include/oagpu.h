@@ -155,6 +155,21 @@ int oagpu_mesh_volume_copy(const oagpu_mesh *m, double *vol);
 int oagpu_element_geometry(int topology, int shifted, int n_elems, const double *coords,
                            double *areav, double *scv, double *dndx, double *detj);
 
+/* The same geometry on caller-provided DEVICE arrays (same layouts), with no
+ * device or host allocation: for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  The kernels' error word lives in a
+ * fixed 256-byte device array of the library, one per GPU, held until the
+ * process ends; it is cleared by a kernel at the start of each call, so
+ * calls that overlap on one GPU race on it.  stream: a hipStream_t (NULL:
+ * the null stream); the call synchronizes it.  Batches above
+ * OAGPU_NOALLOC_MAX_ELEMS elements are rejected (OAGPU_EINVAL); a
+ * non-positive Jacobian returns OAGPU_EGEOM, the outputs still written. */
+#define OAGPU_NOALLOC_MAX_ELEMS (1 << 30)
+int oagpu_element_geometry_noalloc(int topology, int shifted, int n_elems, const double *coords,
+                                   double *areav, double *scv, double *dndx, double *detj,
+                                   void *stream);
+
 /* --------------------------------------------------------------- flow */
 /* boundary physical types (the zone boundaries' types of the reference,
  * boundaryPhysicalType): no-slip wall, velocity inlet, static-pressure

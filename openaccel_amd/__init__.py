@@ -109,6 +109,8 @@ _SIGS = {
     "oagpu_solve": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                     C.POINTER(ControlData)],
     "oagpu_element_geometry": [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp],
+    "oagpu_element_geometry_noalloc": [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "oagpu_update_blending": [C.c_void_p, C.c_int],
     "oagpu_flow_set_bc_types": [C.c_void_p, C.c_int, _ip],
     "oagpu_flow_set_bc_values": [C.c_void_p, C.c_int, _dp, _dp, _dp],

@@ -108,6 +108,8 @@ int oagpu_init(int device) {
         OAGPU_HIP(hipGetDeviceCount(&n));
         OAGPU_REQUIRE(device >= 0 && device < n, "device index out of range");
         OAGPU_HIP(hipSetDevice(device));
+        // device address of this GPU's scratch array (allocation-free entry points)
+        (void)oagpu::scratch_device_ptr();
         g_init = true;
     });
 }
@@ -588,6 +590,23 @@ int oagpu_element_geometry(int topology, int shifted, int n_elems, const double*
         if (dndx) G.download(dndx, G.n);
         if (detj) D.download(detj, D.n);
         OAGPU_HIP(hipDeviceSynchronize());
+        if (e) throw oagpu::Error(OAGPU_EGEOM, "non-positive Jacobian determinant in a kernel");
+    });
+}
+
+int oagpu_element_geometry_noalloc(int topology, int shifted, int n_elems, const double* coords,
+                                   double* areav, double* scv, double* dndx, double* detj,
+                                   void* stream) {
+    return guarded([&] {
+        OAGPU_REQUIRE(topology >= OAGPU_HEX8 && topology <= OAGPU_PYR5, "unknown topology");
+        OAGPU_REQUIRE(n_elems >= 0 && (coords || n_elems == 0), "bad element batch");
+        // one lane per element, int element ids: a batch fits the grid
+        OAGPU_REQUIRE(n_elems <= OAGPU_NOALLOC_MAX_ELEMS,
+                      "batch larger than OAGPU_NOALLOC_MAX_ELEMS elements");
+        ensure_init();
+        const int e = oagpu::element_geometry_noalloc(topology, shifted ? 1 : 0, n_elems, coords,
+                                                      areav, scv, dndx, detj,
+                                                      static_cast<hipStream_t>(stream));
         if (e) throw oagpu::Error(OAGPU_EGEOM, "non-positive Jacobian determinant in a kernel");
     });
 }

@@ -140,6 +140,15 @@ void launch_geo_generic(DevMesh& m, hipStream_t s);  // static ip geometry of no
 int topology_nscs(int topo);
 void element_geometry(int topo, int shifted, int n, const double* coords, double* A, double* V,
                       double* G, double* detj, int* err, hipStream_t s);
+// The same on caller-provided device arrays, allocating nothing: the error
+// word lives in the file-scope scratch array of topo.hip (one per GPU, shared
+// by every call on that GPU: overlapping calls race).  Synchronizes stream s;
+// returns the error word (non-zero: a non-positive Jacobian).
+int element_geometry_noalloc(int topo, int shifted, int n, const double* coords, double* A,
+                             double* V, double* G, double* detj, hipStream_t s);
+// device address of the current GPU's scratch array (hipGetSymbolAddress,
+// looked up once per device and cached)
+int* scratch_device_ptr();
 // amg.hip
 void amg_build(Flow& f);                       // hierarchy structure (once per mesh)
 template <typename V>

@@ -15,6 +15,8 @@
 #include "flow.hpp"
 #include "topo_tables.hpp"
 
+#include <mutex>
+
 namespace oagpu {
 
 namespace topo {
@@ -281,6 +283,54 @@ void launch_scv_generic(DevMesh& m, hipStream_t s) {
 
 int topology_npe(int topo) { return topo == 1 ? 4 : topo == 2 ? 6 : topo == 3 ? 5 : 8; }
 int topology_nscs(int topo) { return topo == 1 ? 6 : topo == 2 ? 9 : 12; }
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free geometry entry point: a file-scope device
+// array, one copy per GPU, held until the process ends.  Word 0 is the
+// error word of the geometry kernels (non-zero: a non-positive Jacobian).
+// Host code never passes `scratch` itself to a kernel (in host code the name
+// is a stand-in, not a device address); it passes the address that
+// hipGetSymbolAddress returned for the current device.
+constexpr int kScratchWords = 64;  // 256 bytes
+__device__ __align__(256) int scratch[kScratchWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kMaxDevices] = {};
+
+__global__ void k_fill_int(int n, int v, int* p) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = v;
+}
+}  // namespace
+
+int* scratch_device_ptr() {
+    int dev = 0;
+    OAGPU_HIP(hipGetDevice(&dev));
+    OAGPU_REQUIRE(dev >= 0 && dev < kMaxDevices, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        OAGPU_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        OAGPU_REQUIRE(p, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<int*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+int element_geometry_noalloc(int topo, int shifted, int n, const double* coords, double* A,
+                             double* V, double* G, double* detj, hipStream_t s) {
+    int* err = scratch_device_ptr();
+    // clear the error word with a kernel (no memset node if the caller captures)
+    k_fill_int<<<1, 64, 0, s>>>(1, 0, err);
+    OAGPU_HIP(hipGetLastError());
+    element_geometry(topo, shifted, n, coords, A, V, G, detj, err, s);
+    int e = 0;
+    OAGPU_HIP(hipMemcpyAsync(&e, err, sizeof(int), hipMemcpyDeviceToHost, s));
+    OAGPU_HIP(hipStreamSynchronize(s));
+    return e;
+}
 
 void element_geometry(int topo, int shifted, int n, const double* coords, double* A, double* V,
                       double* G, double* detj, int* err, hipStream_t s) {
