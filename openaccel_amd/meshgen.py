@@ -62,6 +62,64 @@ def cavity_hex(n: int, length: float = 1.0, perturb: float = 0.0, seed: int = 0)
     }
 
 
+def cube_rotations() -> np.ndarray:
+    """The 24 proper rotations of the reference cube as permutations of the
+    hex8 local nodes: row r maps new local node k to old local node
+    perm[r][k] (node k of the relabelled element sits where node perm[r][k]
+    of the original was)."""
+    v = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1],
+                  [1, 1, 1], [-1, 1, 1]])
+    perms = []
+    for axes in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
+        for signs in np.ndindex(2, 2, 2):
+            R = np.zeros((3, 3), dtype=int)
+            for i, (a, s) in enumerate(zip(axes, signs)):
+                R[i, a] = 1 if s == 0 else -1
+            if round(np.linalg.det(R)) != 1:
+                continue
+            w = v @ R.T
+            perms.append([int(np.flatnonzero((v == w[k]).all(axis=1))[0]) for k in range(8)])
+    return np.array(perms, dtype=np.int64)
+
+
+def relabel_hex(mesh: dict, seed: int) -> dict:
+    """The same hex8 mesh with each element's local node order changed by a
+    random proper cube rotation (positive Jacobians kept; node ids, node
+    coordinates and element ids unchanged), boundary side ordinals remapped
+    to the same physical faces.  A node then is the same local node of
+    several elements, as in meshes from unstructured meshers."""
+    rots = cube_rotations()
+    rng = np.random.default_rng(seed)
+    conn = np.asarray(mesh["conn"])
+    ne = conn.shape[0]
+    pick = rng.integers(0, len(rots), size=ne)
+    perm = rots[pick]                                   # [ne][8]
+    new = np.take_along_axis(conn[:, :8], perm, axis=1)
+    # side s of the original (local nodes S) is the side of the relabelled
+    # element whose local nodes map onto S
+    side_sets = [frozenset(s) for s in _HEX_SIDES.tolist()]
+    side_map = np.zeros((len(rots), 6), dtype=np.int64)
+    for r, p in enumerate(rots):
+        for s_new in range(6):
+            image = frozenset(int(p[k]) for k in _HEX_SIDES[s_new])
+            side_map[r, side_sets.index(image)] = s_new
+    out = dict(mesh)
+    out["conn"] = np.ascontiguousarray(new.astype(np.int32))
+    bfe = np.asarray(mesh["bf_elem"])
+    out["bf_ord"] = np.ascontiguousarray(
+        side_map[pick[bfe], np.asarray(mesh["bf_ord"])].astype(np.int32))
+    return out
+
+
+def repeated_local_nodes(conn: np.ndarray, n_nodes: int) -> int:
+    """Number of (node, local node) pairs held by more than one hex8 element:
+    the incidences the row kernels visit through their extra list."""
+    c = np.asarray(conn)[:, :8]
+    key = c.astype(np.int64) * 8 + np.arange(8)[None, :]
+    counts = np.bincount(key.ravel(), minlength=8 * n_nodes)
+    return int(np.maximum(counts - 1, 0).sum())
+
+
 def initial_velocity(n_nodes: int, seed: int, sigma: float = 1.0e-3) -> np.ndarray:
     """Seeded N(0, sigma) initial velocity [n_nodes][3]."""
     rng = np.random.default_rng(seed)
