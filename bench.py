@@ -1,19 +1,32 @@
-"""Benchmark: steady segregated outer iterations of the lid-driven cavity on
-one MI355X, mixed form vs FP64 form, with assembly kernel roofline figures
-and the oracle's CPU time on the same workload.
+"""Benchmark of the CVFEM assemble-and-solve path on one MI355X: outer
+iterations/s of the mixed form against the FP64 form, per-kernel device time
+and HBM roofline, the mixed form's coefficient error against the FP64 form,
+and the oracle's CPU time on a bounded sample of the same workload.
 
-A "step" is one outer iteration (momentum assembly + GMRES solve, mass flow
-update, p' assembly + AMG-preconditioned CG solve, corrections, gradients)
-over the whole mesh, inputs resident in HBM.  One JSON line is printed.
+A "step" is one outer iteration over the whole mesh with every input resident
+in HBM: for the backward-facing step (the default workload, 8M hex8 cells,
+k-omega SST) the segregated flow iteration (momentum assembly + Jacobi-GMRES,
+mass flow update, p' assembly + AMG-BiCGStab, corrections, gradients) and
+the SST iteration (model update, k and omega assembly + GMRES, corrections,
+gradients).  One JSON line is printed.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--n 128] [--perturb 0]
-    python bench.py --workload bfs [--bfs-scale 1.0]   # 8M-element step, k-omega SST
-    python bench.py --workload duct [--duct-scale 1.0] # 3.9M-element mixed-topology duct
-    python bench.py --workload batch                   # 96 independent cavities, batched
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2]          # backward-facing step
+    python bench.py --workload cavity [--n 128] [--perturb 0.2 --p-method bicgstab_amg]
+    python bench.py --workload cavity --to-residual 1e-8          # time to a scaled residual
+    python bench.py --workload duct                               # 3.9M-cell mixed topologies
+    python bench.py --workload batch                              # 96 independent cavities
+
+Kernel device times come from the library's HIP-event clocks on the flow's
+stream (oagpu_profile / oagpu_kernel_stats): during the warm-up steps every
+main kernel is clocked and the one with the largest device time per step is
+picked; the timed steps then clock only that kernel (its average launch time
+gives roofline.achieved = algorithmic bytes per launch / launch time).
 """
 import argparse
 import json
 import os
+import platform
+import subprocess
 import sys
 import time
 
@@ -25,33 +38,40 @@ sys.path.insert(0, ROOT)
 METRIC = "outer iters/s + assembly Melem/s (%HBM peak), mixed/batched vs FP64 on 1 GPU"
 HBM_PEAK_GBS = 8000.0
 BC = np.array([[1.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
-CTL = [20, 0.1, 200, 1.0e-6]   # momentum GMRES max_it/rtol, p' CG max_it/rtol
+CTL = [20, 0.1, 200, 1.0e-6]   # momentum GMRES max_it/rtol, p' max_it/rtol
 
 
-def algorithmic_bytes(nn, ne, nnz, amg_value_bytes=4):
-    """Compulsory HBM bytes of one launch of each row-owned kernel (each array
-    read once, each output written once; FP64 values, int32 indices; the AMG
-    hierarchy in FP32 for the mixed form, FP64 for the FP64 form)."""
+# ------------------------------------------------------------------ bytes
+def algorithmic_bytes(nn, ne, nnz, value_bytes, vector_bytes, high_res=False):
+    """Compulsory HBM bytes of one launch of each clocked kernel on a hex8
+    mesh: each array read once, each output written once, FP64 fields, int32
+    indices.  value_bytes / vector_bytes: the AMG fine level's matrix values
+    and vectors (FP64 form 8 / 8; mixed form with BiCGStab 4 / 8, with the
+    FP32 inner CG 4 / 4)."""
     conn, mdot, slots = 4 * 8, 8 * 12, 4 * 32
-    mom = (nn * (24 + 24 + 8) + ne * (conn + mdot + slots) + nnz * 72 + nn * 24
-           + nn * (8 + 8 + 24) + nn * 24)          # fused node terms + du write
-    pc = nn * (24 + 24 + 8 + 24 + 24 + 8) + ne * (conn + mdot + slots) + nnz * 8 + nn * 8
+    elem = ne * (conn + mdot + slots)
+    hr = nn * (72 + 24) if high_res else 0
+    mom = (nn * (24 + 24 + 8) + elem + nnz * 72 + nn * 24    # x, U, mu_eff; A, b
+           + nn * (8 + 8 + 24) + nn * 24 + hr)                # V, div, grad p node terms; du
+    pc = nn * (24 + 24 + 8 + 24 + 24 + 8) + elem + nnz * 8 + nn * 8
+    sc = nn * (24 + 8 + 8 + 8 + 8 + 8) + elem + nnz * 8 + nn * 8   # x, V, div, phi, mu_t, F1
     md = nn * 112 + ne * (conn + 16 * 12)
-    gu = nn * (24 + 24 + 8 + 72) + ne * conn
-    # fine-level AMG kernels of the V-cycle: sliced-ELL values + column
-    # indices of the true nonzeros, row vectors
-    v = amg_value_bytes
-    amg_pre = nnz * (v + 4) + nn * v * 4           # r, dinv in; x, residual out
-    amg_post = nnz * (v + 4) + nn * (v * 5 + 4)    # x, agg, r, dinv, coarse e in; z out
-    return {"momentum": mom, "pcorr": pc, "mdot": md, "gradU": gu, "amg_pre": amg_pre,
-            "amg_post": amg_post}
+    gu = nn * (24 + 24 + 8 + 72 + 72) + ne * conn           # x, U, V, grad U in and out
+    gp = nn * (24 + 8 + 8 + 24 + 24) + ne * conn
+    v, w = value_bytes, vector_bytes
+    amg_pre = nnz * (v + 4) + nn * w * 4                     # A, cols; r, dinv in; x, res out
+    amg_post = nnz * (v + 4) + nn * (w * 5 + 4)              # A, cols; x, r, dinv, e, z; agg
+    spmv1 = nnz * (8 + 4) + nn * 16
+    spmv3 = nnz * (72 + 4) + nn * 48
+    return {"momentum": mom, "pcorr": pc, "scalar": sc, "mdot": md, "gradU": gu, "gradP": gp,
+            "amg_pre": amg_pre, "amg_post": amg_post, "spmv1": spmv1, "spmv3": spmv3}
 
 
-def measured_traffic(kernel, n, precision):
-    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc FETCH_SIZE
-    and WRITE_SIZE on this workload (profiles/traffic_<kernel>_<n>.json, with
-    the copy-kernel calibration factors applied), or None if not measured."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{n}.json")
+def measured_traffic(kernel, workload, precision):
+    """HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE (profiles/traffic_<kernel>_<workload>.json, copy-kernel
+    calibration applied), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -62,35 +82,109 @@ def measured_traffic(kernel, n, precision):
     return int(rec["fetch_kib"] * 1024 * t["fetch_scale"] + rec["write_kib"] * 1024 * t["write_scale"])
 
 
-def run_gpu(oa, mesh, precision, steps, warmup, p_method):
-    dm = oa.DeviceMesh(mesh)
-    f = oa.FlowCase(dm, oa.default_params(precision, p_method=p_method), BC)
-    from openaccel_amd import meshgen
-    f.upload("U", meshgen.initial_velocity(dm.nn, 0))
-    f.initialize()
-    for _ in range(warmup):
-        f.outer_iteration(CTL)
+# ----------------------------------------------------------------- driver
+def run_form(oa, f, step, steps, warmup, dominant=None):
+    """Warm-up with every kernel clocked, then `steps` timed steps clocking
+    only `dominant` (or the warm-up's largest device time per step).
+    Returns wall seconds, step outputs, per-kernel warm-up ms/step and the
+    dominant kernel's (launches, total ms) over the timed steps."""
+    f.profile("all")
+    hist = [step() for _ in range(warmup)]
+    warm = {}
+    for k in f.CLOCKS:
+        ms, n = f.kernel_stats(k)
+        if n:
+            warm[k] = {"ms_per_step": ms / max(warmup, 1), "launches_per_step": n / max(warmup, 1)}
+    if dominant is None:
+        dominant = max(warm, key=lambda k: warm[k]["ms_per_step"])
+    f.profile(dominant)
     oa.load_library().oagpu_synchronize()
-    ms = {k: 0.0 for k in ("momentum", "pcorr", "mdot", "gradU", "amg_pre", "amg_post")}
-    hist = []
-    v0 = f.kernel_ms("amg_vcycles")
     t0 = time.perf_counter()
     for _ in range(steps):
-        hist.append(f.outer_iteration(CTL))
-        for k in ms:
-            ms[k] += f.kernel_ms(k)
+        hist.append(step())
     oa.load_library().oagpu_synchronize()
     t = time.perf_counter() - t0
-    for k in ms:
-        ms[k] /= steps
-    # launches per step: mdot is updated twice per outer iteration, the
-    # fine-level AMG kernels once per V-cycle
-    vps = (f.kernel_ms("amg_vcycles") - v0) / steps
-    per_step = {"momentum": 1, "pcorr": 1, "mdot": 2, "gradU": 1, "amg_pre": vps, "amg_post": vps}
-    return {"seconds": t, "kernel_ms": ms, "launches": per_step, "hist": np.array(hist),
-            "nn": dm.nn, "ne": dm.ne, "nnz": dm.nnz}
+    ms, n = f.kernel_stats(dominant)
+    f.profile("none")
+    return {"seconds": t, "hist": np.array(hist), "warm": warm, "dominant": dominant,
+            "dom_ms": ms, "dom_launches": n}
 
 
+def kernel_table(r, ab, ne):
+    out = {}
+    for k, v in r["warm"].items():
+        e = {"ms_per_step": round(v["ms_per_step"], 3),
+             "launches_per_step": round(v["launches_per_step"], 2)}
+        if k in ab and v["launches_per_step"] > 0:
+            per = v["ms_per_step"] / v["launches_per_step"]
+            gbs = ab[k] / (per * 1e-3) / 1e9
+            e.update({"ms_per_launch": round(per, 4), "bytes": int(ab[k]), "GB_s": round(gbs, 1),
+                      "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)})
+            if k in ("momentum", "pcorr", "scalar", "mdot", "gradU"):
+                e["Melem_s"] = round(ne / (per * 1e-3) / 1e6, 1)
+        out[k] = e
+    return out
+
+
+def roofline(r, ab, workload, precision):
+    k = r["dominant"]
+    per = r["dom_ms"] / max(r["dom_launches"], 1)
+    gbs = ab[k] / (per * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": k, "launch_ms": round(per, 4), "launches": r["dom_launches"],
+            "bytes_per_launch": int(ab[k]), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(k, workload, precision)}
+
+
+def coefficient_error(oa, dm, make_flow, setup):
+    """Mixed-form momentum and p' coefficients and rhs against the FP64 form
+    assembled from the same state: largest error of a matrix row relative to
+    that row's largest coefficient, and of the rhs relative to its scale."""
+    out = {}
+    systems = {}
+    for prec in (oa.OAGPU_FP64, oa.OAGPU_MIXED):
+        f = make_flow(prec)
+        setup(f)
+        f.assemble_momentum()
+        a3, b3 = f.system(3)
+        f.assemble_pcorr()
+        a1, b1 = f.system(1)
+        systems[prec] = (a3, b3, a1, b1)
+        f.close()
+    rp, _, _ = dm.graph()
+    starts = rp[:-1]
+    for name, k in (("momentum", 0), ("pcorr", 2)):
+        a, ref = systems[1][k], systems[0][k]
+        d = np.abs(a - ref).reshape(a.shape[0], -1).max(axis=1)
+        s = np.abs(ref).reshape(a.shape[0], -1).max(axis=1)
+        err = np.maximum.reduceat(d, starts) / np.maximum(np.maximum.reduceat(s, starts), 1e-300)
+        b, bref = systems[1][k + 1], systems[0][k + 1]
+        out[name] = {"matrix_row_rel_max": float(err.max()),
+                     "rhs_rel_max": float(np.abs(b - bref).max() / max(np.abs(bref).max(), 1e-300))}
+    return out
+
+
+def host_info():
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    model = platform.processor() or ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return cores, model
+
+
+def oracle_module(threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.set_threads(threads)
+    return pyoracle
+
+
+# ---------------------------------------------------- backward-facing step
 # backward-facing step with k-omega SST: inlet U = 1 with 1 % intensity and
 # viscosity ratio 10, rho = 1, mu = 2e-5, pitzDaily controls of the
 # reference's examples (relaxation 0.7 / 0.3 / 0.9, linear_linear velocity
@@ -99,6 +193,7 @@ BFS_DIMS = (30, 210, 96, 112, 170)   # nx_up, nx_dn, ny_low, ny_high, nz: 7 996 
 BFS_MU, BFS_KIN = 2.0e-5, 1.5e-4
 BFS_WIN = BFS_KIN / (10.0 * BFS_MU)
 BFS_CTL, BFS_SCTL = [20, 1.0e-2, 200, 1.0e-2], [20, 1.0e-2]
+BFS_BC = np.array([[1.0, 0.0, 0.0], [0.0] * 3, [0.0] * 3, [0.0] * 3])
 
 
 def bfs_setup(oa_or_oracle, mesh, case, device=True):
@@ -133,36 +228,17 @@ def bfs_params(mod, precision=None):
     return mod.default_params(precision, p_method=mod.OAGPU_BICGSTAB_AMG, **kw)
 
 
-def run_bfs(oa, dm, mesh, precision, steps, warmup):
-    bc = np.array([[1.0, 0.0, 0.0], [0.0] * 3, [0.0] * 3, [0.0] * 3])
-    f = oa.FlowCase(dm, bfs_params(oa, precision), bc)
-    bfs_setup(oa, mesh, f)
-    hist = []
-    for _ in range(warmup):
-        hist.append(np.concatenate([f.outer_iteration(BFS_CTL), f.sst_iteration(BFS_SCTL)]))
-    oa.load_library().oagpu_synchronize()
-    ms = {k: 0.0 for k in ("momentum", "pcorr", "mdot", "gradU")}
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        hist.append(np.concatenate([f.outer_iteration(BFS_CTL), f.sst_iteration(BFS_SCTL)]))
-        for k in ms:
-            ms[k] += f.kernel_ms(k) / steps
-    oa.load_library().oagpu_synchronize()
-    t = time.perf_counter() - t0
-    f.close()
-    return {"seconds": t, "kernel_ms": ms, "hist": np.array(hist)}
-
-
-def bfs_cpu_baseline(dims):
+def bfs_cpu_baseline(dims, threads):
     """Oracle on a bounded sample of the backward-facing step: the same
-    geometry and controls on a coarser grid, one flow + SST outer iteration."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
+    geometry and controls on a coarser grid, one flow + SST outer iteration
+    after one untimed iteration."""
+    pyoracle = oracle_module(threads)
     from openaccel_amd import meshgen
     mesh = meshgen.bfs_hex(*dims)
-    bc = np.array([[1.0, 0.0, 0.0], [0.0] * 3, [0.0] * 3, [0.0] * 3])
-    c = pyoracle.Case(mesh, bfs_params(pyoracle), bc)
+    c = pyoracle.Case(mesh, bfs_params(pyoracle), BFS_BC)
     bfs_setup(pyoracle, mesh, c, device=False)
+    c.outer_iteration(BFS_CTL)
+    c.sst_iteration(BFS_SCTL)
     t0 = time.perf_counter()
     c.outer_iteration(BFS_CTL)
     c.sst_iteration(BFS_SCTL)
@@ -180,21 +256,43 @@ def bench_bfs(args):
     print(f"[bench] backward-facing step {dims}: {ne} elements, first dy {mesh['first_dy']:.2e}",
           file=sys.stderr, flush=True)
     dm = oa.DeviceMesh(mesh)
-    r64 = run_bfs(oa, dm, mesh, oa.OAGPU_FP64, args.steps, args.warmup)
-    print(f"[bench] fp64 {args.steps / r64['seconds']:.3f} it/s", file=sys.stderr, flush=True)
-    rmx = run_bfs(oa, dm, mesh, oa.OAGPU_MIXED, args.steps, args.warmup)
-    print(f"[bench] mixed {args.steps / rmx['seconds']:.3f} it/s", file=sys.stderr, flush=True)
+    nnz = dm.nnz
+
+    def make_flow(prec):
+        return oa.FlowCase(dm, bfs_params(oa, prec), BFS_BC)
+
+    res = {}
+    for prec in (oa.OAGPU_MIXED, oa.OAGPU_FP64):
+        f = make_flow(prec)
+        bfs_setup(oa, mesh, f)
+
+        def step():
+            return np.concatenate([f.outer_iteration(BFS_CTL), f.sst_iteration(BFS_SCTL)])
+
+        dom = None if prec == oa.OAGPU_MIXED else res[oa.OAGPU_MIXED]["dominant"]
+        res[prec] = run_form(oa, f, step, args.steps, args.warmup, dom)
+        f.close()
+        print(f"[bench] {'mixed' if prec else 'fp64'} {args.steps / res[prec]['seconds']:.3f} it/s, "
+              f"dominant {res[prec]['dominant']}", file=sys.stderr, flush=True)
+    rmx, r64 = res[oa.OAGPU_MIXED], res[oa.OAGPU_FP64]
+    ab_mx = algorithmic_bytes(nn, ne, nnz, 4, 8)
+    ab_64 = algorithmic_bytes(nn, ne, nnz, 8, 8)
     it64, itmx = args.steps / r64["seconds"], args.steps / rmx["seconds"]
+    print("[bench] coefficient error of the mixed form", file=sys.stderr, flush=True)
+    cerr = coefficient_error(oa, dm, make_flow, lambda f: bfs_setup(oa, mesh, f))
     cpu = None
     if not args.no_cpu:
-        cd = tuple(max(2, d // 3) for d in BFS_DIMS)
-        cb = bfs_cpu_baseline(cd)
+        cores, model = host_info()
+        threads = args.cpu_threads or cores
+        cd = tuple(max(2, int(round(d * args.cpu_scale))) for d in BFS_DIMS)
+        print(f"[bench] cpu baseline (oracle, {threads} threads) at {cd}", file=sys.stderr, flush=True)
+        cb = bfs_cpu_baseline(cd, threads)
         scale = cb["ne"] / ne
-        cpu = {"value": round(scale / cb["outer_s"], 6), "unit": "outer iterations/s", "cores": 1,
+        cpu = {"value": round(scale / cb["outer_s"], 6), "unit": "outer iterations/s", "cores": threads,
                "kind": "port",
-               "sample": f"oracle C restatement, single thread, 1 flow + SST outer iteration of the "
-                         f"same step at grid {cd} ({cb['ne']} elements, {cb['outer_s']:.2f} s) scaled "
-                         f"by the element ratio {scale:.5f}"}
+               "sample": f"oracle C restatement (OpenMP, {threads} threads on {model}), one flow + SST "
+                         f"outer iteration of the same step at grid {cd} ({cb['ne']} elements, "
+                         f"{cb['outer_s']:.2f} s) scaled by the element ratio {scale:.5f}"}
 
     def hist(r):
         return {"flow_res_U_p": [[float(x) for x in h[:4]] for h in r["hist"]],
@@ -211,25 +309,168 @@ def bench_bfs(args):
         "dtype": "f32 local / f64 global", "data": "synthetic",
         "config": {"workload": f"backward-facing step k-omega SST, channel 30 x 3 x 4, step 1, "
                                f"{ne} hex8 ({nn} nodes), wall cells {mesh['first_dy']:.1e} graded "
-                               f"1.1; inlet U=1 (k {BFS_KIN}, omega {BFS_WIN}), outlet p=0, "
+                               f"1.1; inlet U=1 (k {BFS_KIN}, omega {BFS_WIN:g}), outlet p=0, "
                                f"symmetry sides, rho=1, mu={BFS_MU}; one step = flow outer "
-                               f"iteration + SST iteration; mixed form",
-                   "dims": dims, "elements": ne, "nodes": nn},
+                               f"iteration + SST iteration; mixed form: FP32 local matrices into "
+                               f"FP64 CRS, AMG-BiCGStab p' with FP32 fine-level values",
+                   "dims": dims, "elements": ne, "nodes": nn, "nnz": nnz},
+        "roofline": roofline(rmx, ab_mx, "bfs", 1),
+        "kernels": kernel_table(rmx, ab_mx, ne),
         "fp64": {"value": round(it64, 4), "ms_per_step": round(1e3 * r64["seconds"] / args.steps, 3),
-                 "kernel_ms": {k: round(v, 4) for k, v in r64["kernel_ms"].items()}, **hist(r64)},
+                 "roofline": roofline(r64, ab_64, "bfs", 0), "kernels": kernel_table(r64, ab_64, ne),
+                 **hist(r64)},
         "mixed_over_fp64": round(itmx / it64, 3),
-        "kernel_ms": {k: round(v, 4) for k, v in rmx["kernel_ms"].items()},
+        "coef_err_vs_fp64": cerr,
         **hist(rmx),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
 
 
+# ------------------------------------------------------------------ cavity
+def cavity_cpu_baseline(n_sample, perturb, threads, p_method):
+    """Oracle on a bounded sample of the same workload: the same cavity at
+    n_sample^3, one outer iteration after one untimed one."""
+    pyoracle = oracle_module(threads)
+    from openaccel_amd import meshgen
+    mesh = meshgen.cavity_hex(n_sample, perturb=perturb, seed=1)
+    c = pyoracle.Case(mesh, pyoracle.default_params(p_method=p_method), BC)
+    c.field("U")[:] = meshgen.initial_velocity(c.nn, 0)
+    c.initialize()
+    c.outer_iteration(CTL)
+    t0 = time.perf_counter()
+    c.assemble_momentum()
+    t_mom = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    c.outer_iteration(CTL)
+    return {"outer_s": time.perf_counter() - t0, "mom_s": t_mom, "ne": c.ne}
+
+
+def bench_cavity(args):
+    import openaccel_amd as oa
+    from openaccel_amd import meshgen
+    oa.init(0)
+    mesh = meshgen.cavity_hex(args.n, perturb=args.perturb, seed=1)
+    ne = mesh["conn"].shape[0]
+    print(f"[bench] cavity {args.n}^3 perturb={args.perturb}: {ne} elements", file=sys.stderr,
+          flush=True)
+    pm = oa.OAGPU_CG_AMG if args.p_method == "cg_amg" else oa.OAGPU_BICGSTAB_AMG
+    dm = oa.DeviceMesh(mesh)
+    nn, nnz = dm.nn, dm.nnz
+
+    def make_flow(prec):
+        return oa.FlowCase(dm, oa.default_params(prec, p_method=pm), BC)
+
+    def setup(f):
+        f.upload("U", meshgen.initial_velocity(dm.nn, 0))
+        f.initialize()
+
+    if args.to_residual:
+        return bench_cavity_to_residual(args, oa, dm, make_flow, setup)
+    res = {}
+    for prec in (oa.OAGPU_MIXED, oa.OAGPU_FP64):
+        f = make_flow(prec)
+        setup(f)
+        dom = None if prec == oa.OAGPU_MIXED else res[oa.OAGPU_MIXED]["dominant"]
+        res[prec] = run_form(oa, f, lambda: f.outer_iteration(CTL), args.steps, args.warmup, dom)
+        f.close()
+        print(f"[bench] {'mixed' if prec else 'fp64'} {args.steps / res[prec]['seconds']:.3f} it/s, "
+              f"dominant {res[prec]['dominant']}", file=sys.stderr, flush=True)
+    rmx, r64 = res[oa.OAGPU_MIXED], res[oa.OAGPU_FP64]
+    # AMG-CG's mixed form solves FP32 systems (values and vectors); AMG-BiCGStab
+    # reads FP32 fine-level values under FP64 vectors
+    ab_mx = algorithmic_bytes(nn, ne, nnz, 4, 4 if pm == oa.OAGPU_CG_AMG else 8)
+    ab_64 = algorithmic_bytes(nn, ne, nnz, 8, 8)
+    itmx, it64 = args.steps / rmx["seconds"], args.steps / r64["seconds"]
+    cerr = coefficient_error(oa, dm, make_flow, setup)
+    cpu = None
+    if not args.no_cpu:
+        cores, model = host_info()
+        threads = args.cpu_threads or cores
+        cn = min(args.cpu_n, args.n)
+        print(f"[bench] cpu baseline (oracle, {threads} threads) at {cn}^3", file=sys.stderr, flush=True)
+        cb = cavity_cpu_baseline(cn, args.perturb, threads, 1 if pm == oa.OAGPU_CG_AMG else 2)
+        scale = (cn / args.n) ** 3
+        cpu = {"value": round(scale / cb["outer_s"], 6), "unit": "outer iterations/s",
+               "cores": threads, "kind": "port",
+               "sample": f"oracle C restatement (OpenMP, {threads} threads on {model}), one outer "
+                         f"iteration of the same cavity at {cn}^3 ({cb['ne']} elements, "
+                         f"{cb['outer_s']:.2f} s) scaled by the element ratio {scale:.4f}; its "
+                         f"momentum assembly alone runs at {cb['ne'] / cb['mom_s'] / 1e6:.3f} Melem/s"}
+    wl = "cavity" if not args.perturb else "cavity_perturbed"
+    line = {
+        "metric": METRIC, "value": round(itmx, 4), "unit": "outer iterations/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * rmx["seconds"] / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 local / f64 global", "data": "synthetic",
+        "config": {"workload": f"lid-driven cavity, uniform{' perturbed' if args.perturb else ''} "
+                               f"{args.n}^3 hex8 ({ne} elements, {nn} nodes), Re=100, mixed form; "
+                               f"Jacobi-GMRES(20 it, rtol 0.1) momentum, aggregation-AMG-"
+                               f"{'CG' if args.p_method == 'cg_amg' else 'BiCGStab'} "
+                               f"(200 it, rtol 1e-6) p'",
+                   "n": args.n, "elements": ne, "nodes": nn, "nnz": nnz},
+        "roofline": roofline(rmx, ab_mx, f"{wl}_{args.n}", 1),
+        "kernels": kernel_table(rmx, ab_mx, ne),
+        "fp64": {"value": round(it64, 4), "ms_per_step": round(1e3 * r64["seconds"] / args.steps, 3),
+                 "roofline": roofline(r64, ab_64, f"{wl}_{args.n}", 0),
+                 "kernels": kernel_table(r64, ab_64, ne),
+                 "krylov_iters_mean": [round(float(v), 1) for v in r64["hist"][:, 4:6].mean(axis=0)]},
+        "krylov_iters_mean": [round(float(v), 1) for v in rmx["hist"][:, 4:6].mean(axis=0)],
+        "mixed_over_fp64": round(itmx / it64, 3),
+        "coef_err_vs_fp64": cerr,
+        "residuals": [[float(x) for x in h[:4]] for h in rmx["hist"]],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def bench_cavity_to_residual(args, oa, dm, make_flow, setup):
+    """Wall time until every scaled control residual of an outer iteration
+    falls below the target, mixed and FP64 (p' solved to 1e-6 as in the
+    reference's cavity controls)."""
+    target, cap = args.to_residual, args.max_iters
+    out = {}
+    for prec in (oa.OAGPU_MIXED, oa.OAGPU_FP64):
+        f = make_flow(prec)
+        setup(f)
+        oa.load_library().oagpu_synchronize()
+        t0 = time.perf_counter()
+        hist = []
+        for _ in range(cap):
+            h = f.outer_iteration(CTL)
+            hist.append(h)
+            if h[:4].max() < target:
+                break
+        oa.load_library().oagpu_synchronize()
+        t = time.perf_counter() - t0
+        hist = np.array(hist)
+        out[prec] = {"seconds": round(t, 3), "iterations": len(hist),
+                     "reached": bool(hist[-1, :4].max() < target),
+                     "final_residuals": [float(x) for x in hist[-1, :4]],
+                     "history_every_50": [[float(x) for x in h[:4]] for h in hist[::50]]}
+        f.close()
+        print(f"[bench] {'mixed' if prec else 'fp64'}: {len(hist)} iterations, {t:.2f} s",
+              file=sys.stderr, flush=True)
+    mx, f64 = out[oa.OAGPU_MIXED], out[oa.OAGPU_FP64]
+    line = {"metric": f"time to scaled residual {target:g}", "value": mx["seconds"], "unit": "s",
+            "n_gpus": 1, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+            "dtype": "f32 local / f64 global", "data": "synthetic",
+            "config": {"workload": f"lid-driven cavity {args.n}^3 hex8, Re=100, lid (1,0,0), "
+                                   f"U0 ~ N(0, 1e-3), outer iterations until every scaled control "
+                                   f"residual < {target:g} (cap {cap})", "n": args.n},
+            "mixed": mx, "fp64": f64,
+            "mixed_speedup": round(f64["seconds"] / mx["seconds"], 3) if mx["seconds"] > 0 else None}
+    print(json.dumps(line))
+
+
+# -------------------------------------------------------------------- duct
 # mixed-topology duct 1 x 0.2 x 0.2: plug inflow U = 1, static-pressure
 # outlet p = 0, no-slip walls, rho = 1, mu = 1e-3, high-resolution advection
-DUCT_M, DUCT_LAYERS = 64, (187, 8, 25, 75, 25)   # 3.87M elements: 60 % tet4, 21 % hex8, 15 % wedge6, 5 % pyramid5
+DUCT_M, DUCT_LAYERS = 64, (187, 8, 25, 75, 25)   # 3.87M elements
 DUCT_MU = 1.0e-3
 DUCT_CTL = [20, 1.0e-2, 200, 1.0e-2]
+DUCT_BC = np.array([[1.0, 0.0, 0.0], [0.0] * 3, [0.0] * 3])
 
 
 def duct_setup(case, mesh, device=True):
@@ -246,71 +487,57 @@ def duct_setup(case, mesh, device=True):
     case.initialize()
 
 
-def run_duct(oa, dm, mesh, precision, steps, warmup):
-    bc = np.array([[1.0, 0.0, 0.0], [0.0] * 3, [0.0] * 3])
-    f = oa.FlowCase(dm, oa.default_params(precision, p_method=oa.OAGPU_BICGSTAB_AMG, mu=DUCT_MU,
-                                          high_res=1), bc)
-    duct_setup(f, mesh)
-    hist = [f.outer_iteration(DUCT_CTL) for _ in range(warmup)]
-    oa.load_library().oagpu_synchronize()
-    ms = {k: 0.0 for k in ("momentum", "pcorr", "mdot", "gradU")}
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        hist.append(f.outer_iteration(DUCT_CTL))
-        for k in ms:
-            ms[k] += f.kernel_ms(k) / steps
-    oa.load_library().oagpu_synchronize()
-    t = time.perf_counter() - t0
-    f.close()
-    return {"seconds": t, "kernel_ms": ms, "hist": np.array(hist)}
-
-
-def duct_cpu_baseline(m):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
+def duct_mesh(m):
     from openaccel_amd import meshgen
     layers = tuple(max(1, round(x * m / DUCT_M)) for x in DUCT_LAYERS)
-    mesh = meshgen.duct_mixed(m, layers=layers, length=sum(layers) * 0.2 / m, seed=1)
-    bc = np.array([[1.0, 0.0, 0.0], [0.0] * 3, [0.0] * 3])
-    c = pyoracle.Case(mesh, pyoracle.default_params(mu=DUCT_MU, high_res=1), bc)
-    duct_setup(c, mesh, device=False)
-    t0 = time.perf_counter()
-    c.update_blending("U")
-    c.outer_iteration(DUCT_CTL)
-    return {"outer_s": time.perf_counter() - t0, "ne": c.ne, "layers": layers}
+    return meshgen.duct_mixed(m, layers=layers, length=sum(layers) * 0.2 / m, seed=1), layers
 
 
 def bench_duct(args):
     import openaccel_amd as oa
-    from openaccel_amd import meshgen
     oa.init(0)
     m = max(8, int(round(DUCT_M * args.duct_scale)))
-    layers = tuple(max(1, round(x * m / DUCT_M)) for x in DUCT_LAYERS)
-    mesh = meshgen.duct_mixed(m, layers=layers, length=sum(layers) * 0.2 / m, seed=1)
+    mesh, layers = duct_mesh(m)
     ne, nn = mesh["conn"].shape[0], mesh["xyz"].shape[0]
     counts = np.bincount(mesh["etype"], minlength=4)
     print(f"[bench] mixed duct m={m} layers={layers}: {ne} elements "
           f"(hex {counts[0]}, tet {counts[1]}, wedge {counts[2]}, pyramid {counts[3]})",
           file=sys.stderr, flush=True)
     dm = oa.DeviceMesh(mesh)
-    r64 = run_duct(oa, dm, mesh, oa.OAGPU_FP64, args.steps, args.warmup)
-    print(f"[bench] fp64 {args.steps / r64['seconds']:.3f} it/s", file=sys.stderr, flush=True)
-    rmx = run_duct(oa, dm, mesh, oa.OAGPU_MIXED, args.steps, args.warmup)
-    print(f"[bench] mixed {args.steps / rmx['seconds']:.3f} it/s", file=sys.stderr, flush=True)
-    it64, itmx = args.steps / r64["seconds"], args.steps / rmx["seconds"]
+    res = {}
+    for prec in (oa.OAGPU_MIXED, oa.OAGPU_FP64):
+        f = oa.FlowCase(dm, oa.default_params(prec, p_method=oa.OAGPU_BICGSTAB_AMG, mu=DUCT_MU,
+                                              high_res=1), DUCT_BC)
+        duct_setup(f, mesh)
+        res[prec] = run_form(oa, f, lambda: f.outer_iteration(DUCT_CTL), args.steps, args.warmup)
+        f.close()
+        print(f"[bench] {'mixed' if prec else 'fp64'} {args.steps / res[prec]['seconds']:.3f} it/s",
+              file=sys.stderr, flush=True)
+    rmx, r64 = res[oa.OAGPU_MIXED], res[oa.OAGPU_FP64]
+    itmx, it64 = args.steps / rmx["seconds"], args.steps / r64["seconds"]
     cpu = None
     if not args.no_cpu:
-        cb = duct_cpu_baseline(32)
-        scale = cb["ne"] / ne
-        cpu = {"value": round(scale / cb["outer_s"], 6), "unit": "outer iterations/s", "cores": 1,
+        cores, model = host_info()
+        threads = args.cpu_threads or cores
+        pyoracle = oracle_module(threads)
+        cm = 24
+        cmesh, clayers = duct_mesh(cm)
+        c = pyoracle.Case(cmesh, pyoracle.default_params(mu=DUCT_MU, high_res=1, p_method=2), DUCT_BC)
+        duct_setup(c, cmesh, device=False)
+        c.update_blending("U")
+        c.outer_iteration(DUCT_CTL)
+        t0 = time.perf_counter()
+        c.outer_iteration(DUCT_CTL)
+        ts = time.perf_counter() - t0
+        scale = c.ne / ne
+        cpu = {"value": round(scale / ts, 6), "unit": "outer iterations/s", "cores": threads,
                "kind": "port",
-               "sample": f"oracle C restatement, single thread, 1 outer iteration of the same duct "
-                         f"at m=32 layers {cb['layers']} ({cb['ne']} elements, {cb['outer_s']:.2f} s) "
-                         f"scaled by the element ratio {scale:.5f}"}
+               "sample": f"oracle C restatement (OpenMP, {threads} threads on {model}), one outer "
+                         f"iteration of the same duct at m={cm} ({c.ne} elements, {ts:.2f} s) scaled "
+                         f"by the element ratio {scale:.5f}"}
+    ms_step = {k: round(v["ms_per_step"], 3) for k, v in rmx["warm"].items()}
     line = {
-        "metric": METRIC,
-        "value": round(itmx, 4),
-        "unit": "outer iterations/s",
+        "metric": METRIC, "value": round(itmx, 4), "unit": "outer iterations/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * rmx["seconds"] / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -321,19 +548,18 @@ def bench_duct(args):
                                f"mu={DUCT_MU}, high-resolution advection; mixed form",
                    "m": m, "layers": layers, "elements": ne, "nodes": nn},
         "fp64": {"value": round(it64, 4), "ms_per_step": round(1e3 * r64["seconds"] / args.steps, 3),
-                 "kernel_ms": {k: round(v, 4) for k, v in r64["kernel_ms"].items()},
-                 "res_U_p": [[float(x) for x in h[:4]] for h in r64["hist"]],
-                 "krylov_iters": [[int(h[4]), int(h[5])] for h in r64["hist"]]},
+                 "kernel_ms_per_step": {k: round(v["ms_per_step"], 3) for k, v in r64["warm"].items()}},
         "mixed_over_fp64": round(itmx / it64, 3),
-        "kernel_ms": {k: round(v, 4) for k, v in rmx["kernel_ms"].items()},
-        "kernel_Melem_s": {k: round(ne / (v * 1e-3) / 1e6, 1) for k, v in rmx["kernel_ms"].items() if v > 0},
-        "res_U_p": [[float(x) for x in h[:4]] for h in rmx["hist"]],
-        "krylov_iters": [[int(h[4]), int(h[5])] for h in rmx["hist"]],
+        "kernel_ms_per_step": ms_step,
+        "assembly_Melem_s": {k: round(ne / (v["ms_per_step"] / v["launches_per_step"] * 1e-3) / 1e6, 1)
+                             for k, v in rmx["warm"].items() if k in ("momentum", "pcorr", "mdot")},
+        "residuals": [[float(x) for x in h[:4]] for h in rmx["hist"]],
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
 
 
+# ------------------------------------------------------------------- batch
 # batched independent cavities: sizes from {16, 24, 32, 48, 64}^3, Re
 # log-uniform in [10, 1000] (mu = 1 / Re, unit lid speed and cube), the lid
 # velocity a random unit vector in the lid plane, all seeded
@@ -366,74 +592,56 @@ def bench_batch(args):
         f.initialize()
         return f
 
-    iters = args.batch_iters
-    # FP64, one case at a time
+    iters, tol = args.batch_iters, args.batch_tol
+    # FP64, one case at a time, each to the residual target (or the cap)
     seq = np.zeros(len(cases))
-    oa.load_library().oagpu_synchronize()
-    t0 = time.perf_counter()
+    seq_it = np.zeros(len(cases), dtype=int)
     for k, c in enumerate(cases):
         f = flow(c, oa.OAGPU_FP64)
+        oa.load_library().oagpu_synchronize()
         t1 = time.perf_counter()
-        for _ in range(iters):
-            f.outer_iteration(CTL)
+        for it in range(iters):
+            h = f.outer_iteration(CTL)
+            if tol > 0 and h[:4].max() < tol:
+                break
+        oa.load_library().oagpu_synchronize()
         seq[k] = time.perf_counter() - t1
+        seq_it[k] = it + 1
         f.close()
     t_seq = seq.sum()
-    print(f"[bench] fp64 one at a time: {t_seq:.2f} s of iterations ({time.perf_counter() - t0:.1f} s "
-          f"with set-up)", file=sys.stderr, flush=True)
-    # mixed, batched
+    print(f"[bench] fp64 one at a time: {t_seq:.2f} s", file=sys.stderr, flush=True)
     flows = [flow(c, oa.OAGPU_MIXED) for c in cases]
     oa.load_library().oagpu_synchronize()
     t0 = time.perf_counter()
-    hist, done, sec = oa.batch_outer_iterations(flows, CTL, iters, threads=16)
+    hist, done, sec = oa.batch_outer_iterations(flows, CTL, iters, tol=tol, threads=16)
     t_batch = time.perf_counter() - t0
     print(f"[bench] mixed batched: {t_batch:.2f} s", file=sys.stderr, flush=True)
-    res = np.array([h[-1][:4].max() for h in hist])
+    last = np.array([hist[k][done[k] - 1][:4].max() for k in range(len(cases))])
     line = {
-        "metric": METRIC,
-        "value": round(len(cases) / t_batch, 4),
-        "unit": "cases/s",
+        "metric": METRIC, "value": round(len(cases) / t_batch, 4), "unit": "cases/s",
         "n_gpus": 1, "steps": iters, "warmup": 0,
-        "ms_per_step": round(1e3 * t_batch / iters, 3),
+        "ms_per_step": round(1e3 * t_batch / max(done.max(), 1), 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 local / f64 global", "data": "synthetic",
         "config": {"workload": f"{len(cases)} independent lid-driven cavities, sizes from "
                                f"{BATCH_SIZES}^3 hex8, Re log-uniform in [10, 1000], lid velocity a "
-                               f"random unit vector in the lid plane (seeded); {iters} outer "
-                               f"iterations each; mixed form batched on 16 host threads (one "
-                               f"stream per case) vs FP64 one case at a time",
-                   "cases": len(cases), "elements": elems, "iterations_per_case": iters},
+                               f"random unit vector in the lid plane (seeded); each case runs until "
+                               f"its scaled residuals are below {tol:g} or {iters} outer "
+                               f"iterations; mixed form batched (one stream per case, 16 host "
+                               f"threads) vs FP64 one case at a time",
+                   "cases": len(cases), "elements": elems, "iteration_cap": iters},
         "fp64_one_at_a_time": {"value": round(len(cases) / t_seq, 4), "unit": "cases/s",
                                "per_case_s_p50": round(float(np.percentile(seq, 50)), 4),
-                               "per_case_s_p99": round(float(np.percentile(seq, 99)), 4)},
+                               "per_case_s_p99": round(float(np.percentile(seq, 99)), 4),
+                               "reached": int((seq_it < iters).sum())},
         "batched_over_one_at_a_time": round(t_seq / t_batch, 3),
         "per_case_s_p50": round(float(np.percentile(sec, 50)), 4),
         "per_case_s_p99": round(float(np.percentile(sec, 99)), 4),
-        "residual_after_iters_p50": float(np.percentile(res, 50)),
-        "residual_after_iters_max": float(res.max()),
+        "reached": int((last < tol).sum()) if tol > 0 else None,
+        "iterations_p50": float(np.percentile(done, 50)),
         "cpu_baseline": None,
     }
     print(json.dumps(line))
-
-
-def cpu_baseline(n_sample, perturb):
-    """Oracle (single-thread C restatement) on a bounded sample of the same
-    workload: the same cavity at n_sample^3, one outer iteration timed after
-    set-up, and its momentum assembly alone."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-    from openaccel_amd import meshgen
-    mesh = meshgen.cavity_hex(n_sample, perturb=perturb, seed=1)
-    c = pyoracle.Case(mesh, pyoracle.default_params(), BC)
-    c.field("U")[:] = meshgen.initial_velocity(c.nn, 0)
-    c.initialize()
-    t0 = time.perf_counter()
-    c.assemble_momentum()
-    t_mom = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    c.outer_iteration(CTL)
-    t_it = time.perf_counter() - t0
-    return {"outer_s": t_it, "mom_s": t_mom, "ne": c.ne}
 
 
 def main():
@@ -441,32 +649,34 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=("bfs", "cavity", "duct", "batch"), default="bfs",
+                    help="bfs: the 8M-cell backward-facing step with k-omega SST (default); "
+                         "cavity: the lid-driven cavity; duct: the 3.9M-cell mixed-topology "
+                         "duct; batch: 96 independent cavities")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: all host cores)")
+    ap.add_argument("--cpu-scale", type=float, default=0.5,
+                    help="step: oracle sample grid as a fraction of the cell counts per direction")
+    ap.add_argument("--cpu-n", type=int, default=64, help="cavity: oracle sample grid")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--perturb", type=float, default=0.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=64, help="oracle sample grid (elements scale)")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="only copy MIB MiB device-to-device once (calibrates HBM byte counters)")
-    ap.add_argument("--p-method", choices=("cg_amg", "bicgstab_amg"), default="cg_amg",
-                    help="cavity p' solver: AMG-CG (default), or AMG-BiCGStab for the p' systems "
-                         "of distorted meshes, which are not symmetric")
+    ap.add_argument("--p-method", choices=("cg_amg", "bicgstab_amg"), default="cg_amg")
+    ap.add_argument("--to-residual", type=float, default=0.0,
+                    help="cavity: time the outer iterations until every scaled residual is below this")
+    ap.add_argument("--max-iters", type=int, default=4000)
     ap.add_argument("--batch-cases", type=int, default=BATCH_N)
     ap.add_argument("--batch-iters", type=int, default=BATCH_ITERS)
-    ap.add_argument("--workload", choices=("cavity", "bfs", "duct", "batch"), default="cavity",
-                    help="cavity: the lid-driven cavity (default); bfs: the 8M-element "
-                         "backward-facing step with k-omega SST; duct: the 4M-element "
-                         "mixed-topology duct")
-    ap.add_argument("--duct-scale", type=float, default=1.0,
-                    help="scale of the duct's cube counts per direction (1: 3.9M elements)")
+    ap.add_argument("--batch-tol", type=float, default=0.0)
+    ap.add_argument("--duct-scale", type=float, default=1.0)
     ap.add_argument("--bfs-scale", type=float, default=1.0,
                     help="scale of the step's cell counts per direction (1: 8M elements)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device once (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X only: the multi-GPU path (element partition + "
-              "halo exchange) is designed but not implemented; run with --gpus 1.",
-              file=sys.stderr)
+              "halo exchange) is designed but not built; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
-
     if args.copy_probe:
         import torch
         nb = args.copy_probe << 20
@@ -477,94 +687,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": nb}))
         return
-
-    if args.workload == "bfs":
-        bench_bfs(args)
-        return
-    if args.workload == "duct":
-        bench_duct(args)
-        return
-    if args.workload == "batch":
-        bench_batch(args)
-        return
-
-    import openaccel_amd as oa
-    from openaccel_amd import meshgen
-    oa.init(0)
-    mesh = meshgen.cavity_hex(args.n, perturb=args.perturb, seed=1)
-    print(f"[bench] mesh {args.n}^3 perturb={args.perturb}: {mesh['conn'].shape[0]} elements",
-          file=sys.stderr, flush=True)
-    pm = oa.OAGPU_CG_AMG if args.p_method == "cg_amg" else oa.OAGPU_BICGSTAB_AMG
-    r64 = run_gpu(oa, mesh, oa.OAGPU_FP64, args.steps, args.warmup, pm)
-    print(f"[bench] fp64 {args.steps / r64['seconds']:.3f} it/s {r64['kernel_ms']}", file=sys.stderr,
-          flush=True)
-    rmx = run_gpu(oa, mesh, oa.OAGPU_MIXED, args.steps, args.warmup, pm)
-    print(f"[bench] mixed {args.steps / rmx['seconds']:.3f} it/s {rmx['kernel_ms']}", file=sys.stderr,
-          flush=True)
-    nn, ne, nnz = rmx["nn"], rmx["ne"], rmx["nnz"]
-
-    def kern(r):
-        ab = algorithmic_bytes(nn, ne, nnz, 4 if r is rmx else 8)
-        out = {}
-        for k, ms in r["kernel_ms"].items():
-            gbs = ab[k] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-            out[k] = {"ms": round(ms, 4), "GB_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-                      "bytes": int(ab[k]), "launches_per_step": round(r["launches"][k], 2),
-                      "ms_per_step": round(ms * r["launches"][k], 3)}
-            if not k.startswith("amg"):
-                out[k]["Melem_s"] = round(ne / (ms * 1e-3) / 1e6, 1) if ms > 0 else 0
-        return out
-
-    k64, kmx = kern(r64), kern(rmx)
-    # the dominant kernel: largest device time per outer iteration
-    dom = max(kmx, key=lambda k: kmx[k]["ms_per_step"])
-    it_s_mx = args.steps / rmx["seconds"]
-    it_s_64 = args.steps / r64["seconds"]
-    cpu = None
-    if not args.no_cpu:
-        print("[bench] cpu baseline (oracle, 1 thread) ...", file=sys.stderr, flush=True)
-        cn = min(args.cpu_n, args.n)
-        cb = cpu_baseline(cn, args.perturb)
-        scale = (cn / args.n) ** 3
-        cpu = {"value": round(scale / cb["outer_s"], 6), "unit": "outer iterations/s", "cores": 1,
-               "kind": "port",
-               "sample": f"oracle C restatement, single thread, 1 outer iteration of the same cavity "
-                         f"at {cn}^3 ({cb['ne']} elements, {cb['outer_s']:.2f} s) scaled by the "
-                         f"element ratio {scale:.4f} to {args.n}^3; its momentum assembly alone "
-                         f"runs at {cb['ne'] / cb['mom_s'] / 1e6:.3f} Melem/s"}
-    line = {
-        "metric": METRIC,
-        "value": round(it_s_mx, 4),
-        "unit": "outer iterations/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * rmx["seconds"] / args.steps, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 local / f64 global",
-        "data": "synthetic",
-        "config": {"workload": f"lid-driven cavity, uniform{' perturbed' if args.perturb else ''} "
-                               f"{args.n}^3 hex8 ({ne} elements, {nn} nodes), Re=100, mixed form; "
-                               f"Jacobi-GMRES(20 it, rtol 0.1) momentum, aggregation-AMG-"
-                               f"{'CG' if args.p_method == 'cg_amg' else 'BiCGStab'} "
-                               f"(200 it, rtol 1e-6) p'",
-                   "n": args.n, "elements": ne, "nodes": nn, "nnz": nnz},
-        "fp64": {"value": round(it_s_64, 4), "ms_per_step": round(1e3 * r64["seconds"] / args.steps, 3),
-                 "krylov_iters_mean": [round(float(v), 1) for v in r64["hist"][:, 4:6].mean(axis=0)],
-                 "kernels": k64},
-        "krylov_iters_mean": [round(float(v), 1) for v in rmx["hist"][:, 4:6].mean(axis=0)],
-        "mixed_over_fp64": round(it_s_mx / it_s_64, 3),
-        "kernels": kmx,
-        "roofline": {"bound": "hbm", "kernel": dom, "launch_ms": kmx[dom]["ms"],
-                     "achieved": kmx[dom]["GB_s"],
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kmx[dom]["hbm_frac"],
-                     "traffic": measured_traffic(dom, args.n, 1) if not args.perturb else None},
-        "cpu_baseline": cpu,
-        "residuals_last": [float(x) for x in rmx["hist"][-1][:4]],
-    }
-    print(json.dumps(line))
+    {"bfs": bench_bfs, "cavity": bench_cavity, "duct": bench_duct, "batch": bench_batch}[args.workload](args)
 
 
 if __name__ == "__main__":

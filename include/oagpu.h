@@ -32,6 +32,7 @@ extern "C" {
 #define OAGPU_EHIP 3     /* HIP runtime failure                        */
 #define OAGPU_ENOMEM 4   /* allocation failure                         */
 #define OAGPU_EGEOM 5    /* non-positive Jacobian met by a kernel      */
+#define OAGPU_ENONFINITE 6 /* a linear solve produced a non-finite residual */
 #define OAGPU_EINTERNAL 9
 
 /* graph layout, mirroring linearSolver::GraphLayout (src/mesh/graph/nodeGraph.h,
@@ -301,12 +302,19 @@ int oagpu_correct_scalar(oagpu_flow *f, int which);
  * hist[4] = k, omega control residuals, k, omega iterations */
 int oagpu_sst_iteration(oagpu_flow *f, const double *ctl, double *hist);
 
-/* device time of the last launch of a kernel, milliseconds (HIP events):
- * "momentum", "pcorr", "mdot", "gradU" (assembly and field kernels of the last
- * call), "amg_pre" / "amg_post" (fine-level AMG pre-smooth + residual and
- * prolongation + post-smooth of the V-cycle that started the last AMG-CG
- * solve); "amg_vcycles" returns the V-cycle count of the flow so far. */
-int oagpu_last_kernel_ms(oagpu_flow *f, const char *which, double *ms);
+/* Device time of the library's main kernels, measured with HIP events on the
+ * flow's stream (no counterpart in the reference, whose only instruments are
+ * the per-iteration CPU time and a profiler region around each solve).
+ * oagpu_profile(f, which) clocks "all", "none" or one of: "momentum",
+ * "pcorr", "scalar" (k / omega rows), "mdot", "gradU", "gradP" (every scalar
+ * gradient), "amg_pre" / "amg_post" (the fine level's pre-smoothing +
+ * residual and prolongation + post-smoothing kernels), "amg_tail" (the
+ * one-workgroup coarse levels), "spmv1" / "spmv3" (Krylov products), and
+ * resets the totals.  Launches replayed from captured HIP graphs are not
+ * clocked.  Nothing synchronises until oagpu_kernel_stats, which returns the
+ * total device time (ms) and the launch count since the last oagpu_profile. */
+int oagpu_profile(oagpu_flow *f, const char *which);
+int oagpu_kernel_stats(oagpu_flow *f, const char *which, double *total_ms, long long *launches);
 
 #ifdef __cplusplus
 }

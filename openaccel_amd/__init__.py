@@ -126,7 +126,8 @@ _SIGS = {
     "oagpu_amg_info": [C.c_void_p, _ip, _ip],
     "oagpu_amg_level_copy": [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp],
     "oagpu_outer_iteration": [C.c_void_p, _dp, _dp],
-    "oagpu_last_kernel_ms": [C.c_void_p, C.c_char_p, _dp],
+    "oagpu_profile": [C.c_void_p, C.c_char_p],
+    "oagpu_kernel_stats": [C.c_void_p, C.c_char_p, _dp, C.POINTER(C.c_longlong)],
 }
 
 _lib = None
@@ -427,10 +428,18 @@ class FlowCase:
         _check(load_library().oagpu_outer_iteration(self.h, _d(c), _d(h)))
         return h
 
-    def kernel_ms(self, which):
-        v = C.c_double()
-        _check(load_library().oagpu_last_kernel_ms(self.h, which.encode(), C.byref(v)))
-        return v.value
+    CLOCKS = ("momentum", "pcorr", "scalar", "mdot", "gradU", "gradP", "amg_pre", "amg_post",
+              "amg_tail", "spmv1", "spmv3")
+
+    def profile(self, which="all"):
+        """Clock "all", "none" or one named kernel (HIP events); resets totals."""
+        _check(load_library().oagpu_profile(self.h, which.encode()))
+
+    def kernel_stats(self, which):
+        """(total device ms, launches) of a clocked kernel since profile()."""
+        ms, n = C.c_double(), C.c_longlong()
+        _check(load_library().oagpu_kernel_stats(self.h, which.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -5,13 +5,14 @@
 // family (BoomerAMG inside GMRES/CG, linearSystem.h:309-358); the library is
 // not part of the reference tree, so this is a design of its own, chosen for
 // the MI355X: plain (unsmoothed) aggregation (Vanek, Mandel, Brezina,
-// Computing 56 (1996), phase 1-3 greedy aggregation on the matrix graph) with
-// piecewise-constant prolongation, so that the Galerkin product P^T A P is a
-// plain sum of fine coefficients into their aggregate's row.
+// Computing 56 (1996), phase 1-3 greedy aggregation over the strong
+// couplings) with piecewise-constant prolongation, so that the Galerkin
+// product P^T A P is a plain sum of fine coefficients into their aggregate's
+// row.
 //
 // * The hierarchy structure (aggregates, coarse patterns, the fine-slot ->
-//   coarse-slot map) depends only on the mesh graph and is built once, on the
-//   host.  Every solve recomputes the coarse values on the device from the
+//   coarse-slot map) depends on the mesh graph and on the coupling strengths
+//   of the first matrix solved, and is built once, on the host.  Every solve recomputes the coarse values on the device from the
 //   current fine matrix: one lane per coarse sliced-ELL slot sums the fine
 //   slots listed for it in a fixed order (no atomics, bitwise reproducible).
 // * Cycle: V(1,1) with damped Jacobi (omega = 2/3) and the coarse correction
@@ -34,6 +35,7 @@
 #include "sell.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 #include <type_traits>
 
@@ -43,6 +45,14 @@ constexpr int kDenseMax = 64;      // largest coarsest level inverted densely
 constexpr int kMaxLevels = 16;
 constexpr double kOmega = 2.0 / 3.0;
 constexpr double kAlpha = 2.0;
+// strength thresholds: the fine level's separates the wall-normal couplings of
+// stretched cells; the coarse levels keep nearly all couplings (on the 128^3
+// cavity a coarse threshold of 0.04 and above grew the K-cycle's coarse
+// levels by half for no fewer iterations; on the 8M-cell step 0.02 gave
+// 24-27 AMG-BiCGStab iterations per p' solve where the graph-only
+// aggregation took 85)
+constexpr double kTheta = 0.08;
+constexpr double kThetaCoarse = 0.02;
 
 struct SellView {
     int n;
@@ -283,8 +293,10 @@ __device__ inline void kc_coeffs(const KcRef& k, double* sh, double& ca, double&
 // With spos the residual is stored at spos[i] (member order of the next
 // level, for the one-workgroup tail's restriction).
 // FINE only names the fine-level instance separately (profiles, timing).
-template <typename V, bool FINE>
-__global__ void __launch_bounds__(256) k_amg_pre_residual(SellView s, const V* A, const V* dinv,
+// S: storage type of the matrix values (float values under double vectors:
+// the fine level of the mixed form's BiCGStab preconditioner).
+template <typename V, bool FINE, typename S = V>
+__global__ void __launch_bounds__(256) k_amg_pre_residual(SellView s, const S* A, const V* dinv,
                                                           V omega, const V* r, V* x, V* res,
                                                           const int* flag, KcRef kc = {},
                                                           const V* v1 = nullptr,
@@ -302,13 +314,13 @@ __global__ void __launch_bounds__(256) k_amg_pre_residual(SellView s, const V* A
 #pragma unroll 4
         for (int q = 0; q < w; ++q) {
             const int j = s.cols[b + size_t(q) * 64];
-            acc += A[b + size_t(q) * 64] * (dinv[j] * (r[j] - sc * v1[j]));
+            acc += V(A[b + size_t(q) * 64]) * (dinv[j] * (r[j] - sc * v1[j]));
         }
     } else {
 #pragma unroll 4
         for (int q = 0; q < w; ++q) {
             const int j = s.cols[b + size_t(q) * 64];
-            acc += A[b + size_t(q) * 64] * (dinv[j] * r[j]);
+            acc += V(A[b + size_t(q) * 64]) * (dinv[j] * r[j]);
         }
     }
     const V ri = kc.kp1 ? r[i] - sc * v1[i] : r[i];
@@ -323,8 +335,8 @@ __global__ void __launch_bounds__(256) k_amg_pre_residual(SellView s, const V* A
 // Grid-stride; with part != nullptr the block also writes its partial of
 // r . out (a fixed partition of gridDim blocks) and, with p != nullptr,
 // copies out into p (first CG iteration).
-template <typename V, bool FINE>
-__global__ void __launch_bounds__(256) k_amg_prolong_smooth(SellView s, const V* A, const V* dinv,
+template <typename V, bool FINE, typename S = V>
+__global__ void __launch_bounds__(256) k_amg_prolong_smooth(SellView s, const S* A, const V* dinv,
                                                             V omega, const V* r, const V* x,
                                                             const int* agg, V alpha, const V* ec,
                                                             V* out, double* part, V* p,
@@ -351,13 +363,13 @@ __global__ void __launch_bounds__(256) k_amg_prolong_smooth(SellView s, const V*
             for (int q = 0; q < w; ++q) {
                 const int j = s.cols[b + size_t(q) * 64];
                 const int J = agg[j];
-                acc += A[b + size_t(q) * 64] * (x[j] + (ca * ec[J] + cb * ec2[J]));
+                acc += V(A[b + size_t(q) * 64]) * (x[j] + (ca * ec[J] + cb * ec2[J]));
             }
         } else {
 #pragma unroll 4
             for (int q = 0; q < w; ++q) {
                 const int j = s.cols[b + size_t(q) * 64];
-                acc += A[b + size_t(q) * 64] * (x[j] + ca * ec[agg[j]]);
+                acc += V(A[b + size_t(q) * 64]) * (x[j] + ca * ec[agg[j]]);
             }
         }
         const V ri = r[i];
@@ -562,12 +574,42 @@ static SellView view_of(const AmgLevel& l) {
     return {l.n, l.slice_ptr.p, l.slice_w.p, l.cols.p, l.diag_off.p};
 }
 
-// greedy aggregation on the graph of a CRS pattern (rows include their diagonal)
+// Strong couplings of a CRS matrix (Vanek, Mandel, Brezina 1996):
+// j != i is a strong neighbour of i when |a_ij| >= theta sqrt(|a_ii a_jj|).
+// On the anisotropic cells of wall-resolved meshes (aspect ratios in the
+// hundreds) the wall-normal couplings of the pressure-correction operator
+// exceed the tangential ones by the square of the aspect ratio; aggregating
+// along the strong couplings only gives the semi-coarsening that damped
+// Jacobi needs there.  On isotropic meshes every neighbour is strong.
+static void strong_graph(int n, const std::vector<int>& rp, const std::vector<int>& ci,
+                         const std::vector<double>& a, double theta, std::vector<int>& srp,
+                         std::vector<int>& sci) {
+    std::vector<double> d(n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] == i) d[i] = std::fabs(a[k]);
+    srp.assign(size_t(n) + 1, 0);
+    sci.clear();
+    sci.reserve(ci.size());
+    for (int i = 0; i < n; ++i) {
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            const int j = ci[k];
+            if (j != i && std::fabs(a[k]) >= theta * std::sqrt(d[i] * d[j])) sci.push_back(j);
+        }
+        srp[i + 1] = int(sci.size());
+    }
+}
+
+// Greedy aggregation over the strong-coupling graph (rows list neighbours
+// only).  Phase 1: a node whose strong neighbours are all free roots an
+// aggregate of itself and them (a node without strong neighbours stays a
+// singleton); phase 2: a node left over joins the aggregate of its strongest
+// aggregated neighbour; phase 3: what is still free groups with its free
+// strong neighbours.
 static int aggregate(int n, const std::vector<int>& rp, const std::vector<int>& ci,
-                     std::vector<int>& agg) {
+                     const std::vector<double>& w, std::vector<int>& agg) {
     agg.assign(n, -1);
     int na = 0;
-    // phase 1: a node whose whole neighbourhood is free roots a new aggregate
     for (int i = 0; i < n; ++i) {
         if (agg[i] >= 0) continue;
         bool free = true;
@@ -577,17 +619,16 @@ static int aggregate(int n, const std::vector<int>& rp, const std::vector<int>& 
         agg[i] = na;
         ++na;
     }
-    // phase 2: remaining nodes join a neighbouring phase-1 aggregate
     std::vector<int> a2 = agg;
     for (int i = 0; i < n; ++i) {
         if (agg[i] >= 0) continue;
+        double best = -1.0;
         for (int k = rp[i]; k < rp[i + 1]; ++k)
-            if (agg[ci[k]] >= 0) {
+            if (agg[ci[k]] >= 0 && w[k] > best) {
+                best = w[k];
                 a2[i] = agg[ci[k]];
-                break;
             }
     }
-    // phase 3: what is still free forms aggregates with its free neighbours
     for (int i = 0; i < n; ++i) {
         if (a2[i] >= 0) continue;
         for (int k = rp[i]; k < rp[i + 1]; ++k)
@@ -597,6 +638,19 @@ static int aggregate(int n, const std::vector<int>& rp, const std::vector<int>& 
     }
     agg.swap(a2);
     return na;
+}
+
+// |a_ij| of each strong coupling, in the strong graph's order
+static std::vector<double> strong_weights(int n, const std::vector<int>& rp,
+                                          const std::vector<int>& ci, const std::vector<double>& a,
+                                          const std::vector<int>& srp, const std::vector<int>& sci) {
+    std::vector<double> w(sci.size());
+    for (int i = 0; i < n; ++i) {
+        int q = srp[i];
+        for (int k = rp[i]; k < rp[i + 1] && q < srp[i + 1]; ++k)
+            if (ci[k] == sci[q]) w[q++] = std::fabs(a[k]);
+    }
+    return w;
 }
 
 template <typename T>
@@ -619,11 +673,24 @@ void amg_build(Flow& f) {
     }
     std::vector<int> fine_slice_ptr = m.h_slice_ptr, fine_slice_w = m.h_slice_w;
     size_t fine_nsell = m.nsell;
+    // values of the scalar system in CRS order: the strength of the couplings
+    // (the hierarchy is built once, from the first matrix solved)
+    std::vector<double> fvals(size_t(m.nnz));
+    {
+        std::vector<double> sell(m.nsell);
+        f.A1.download(sell.data(), sell.size(), f.stream);
+        OAGPU_HIP(hipStreamSynchronize(f.stream));
+        sell_to_crs(m, 1, sell.data(), fvals.data());
+    }
+    double theta = kTheta;
     while (int(amg.lv.size()) < kMaxLevels && amg.lv.back().n > kDenseMax) {
         AmgLevel& fl = amg.lv.back();
         const int nf = fl.n;
-        std::vector<int> agg;
-        const int nc = aggregate(nf, fl.h_row_ptr, fl.h_cols, agg);
+        std::vector<int> agg, srp, sci;
+        strong_graph(nf, fl.h_row_ptr, fl.h_cols, fvals, theta, srp, sci);
+        const std::vector<double> sw = strong_weights(nf, fl.h_row_ptr, fl.h_cols, fvals, srp, sci);
+        const int nc = aggregate(nf, srp, sci, sw, agg);
+        theta = kThetaCoarse;
         if (nc >= nf || nc * 5 > nf * 4) break;  // coarsening has stalled
         // members of each aggregate, ascending
         std::vector<int> mem_ptr(size_t(nc) + 1, 0), mem(nf);
@@ -654,6 +721,23 @@ void amg_build(Flow& f) {
             crp[I + 1] = int(ccols.size());
         }
         if (int(ccols.size()) != crp[nc] || crp[nc] == 0) break;  // a coarse row over 255 entries
+        // coarse values P^T A P (sums of the members' coefficients): the next
+        // level's coupling strengths
+        {
+            std::vector<double> cv(ccols.size(), 0.0);
+            for (int I = 0; I < nc; ++I) {
+                const int* cb = ccols.data() + crp[I];
+                const int clen = crp[I + 1] - crp[I];
+                for (int k = mem_ptr[I]; k < mem_ptr[I + 1]; ++k) {
+                    const int i = mem[k];
+                    for (int q = fl.h_row_ptr[i]; q < fl.h_row_ptr[i + 1]; ++q) {
+                        const int g = int(std::lower_bound(cb, cb + clen, agg[fl.h_cols[q]]) - cb);
+                        cv[size_t(crp[I]) + g] += fvals[q];
+                    }
+                }
+            }
+            fvals.swap(cv);
+        }
         AmgLevel cl;
         cl.n = nc;
         cl.h_row_ptr = crp;
@@ -793,6 +877,7 @@ struct AmgCycle {
     bool kcycle;
     const V* A0;
     const V* dinv0;
+    bool timing = false;  // clock the tail (eager launches only)
 
     SellView view(int l) const { return l == 0 ? view_of(*f.m) : view_of(amg.lv[l]); }
     // a coarse level's residual is stored in the member order of the next
@@ -859,7 +944,9 @@ struct AmgCycle {
                 ta.lv[k] = {view(k), A(k), D(k), lvv<V>(L, 3), lvv<V>(L, 1), lvv<V>(L, 4), lvv<V>(L, 2),
                             L.agg.p, L.mem_ptr.p, L.mem.p, L.mpos.p};
             }
+            if (timing) clock_begin(f, kClkAmgTail);
             k_amg_tail<V><<<1, kTailThreads, 0, f.stream>>>(ta, flag);
+            if (timing) clock_end(f, kClkAmgTail);
             return {t == last ? X(last) : lvv<V>(amg.lv[t], 2), nullptr, al, {}};
         }
         V* rc = lvv<V>(cl, 3);
@@ -904,7 +991,7 @@ struct AmgCycle {
 
 template <typename V>
 void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* rz_part, V* p_init,
-                bool timing, bool kcycle, const V* q, double* qz_part) {
+                bool timing, bool kcycle, const V* q, double* qz_part, const float* A0f) {
     Amg& amg = f.amg;
     const int nl = int(amg.lv.size());
     OAGPU_REQUIRE(nl > 1, "AMG hierarchy without a coarse level");
@@ -921,25 +1008,38 @@ void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* 
     kcycle = kcycle && t >= 3;
     AmgCycle<V> cy{f, amg, f.flag.p, V(kOmega), V(kAlpha), t, last, kcycle, A0, dinv0};
     AmgLevel& l0 = amg.lv[0];
-    if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[0], f.stream));
-    k_amg_pre_residual<V, true><<<blocks(l0.n), 256, 0, f.stream>>>(
-        cy.view(0), A0, dinv0, cy.om, r, cy.X(0), lvv<V>(l0, 4), cy.flag, {}, nullptr, nullptr,
-        cy.spos(0));
-    if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[1], f.stream));
+    const bool fine_f32 = std::is_same_v<V, double> && A0f != nullptr;
+    if (timing) clock_begin(f, kClkAmgPre);
+    if (fine_f32)
+        k_amg_pre_residual<V, true, float><<<blocks(l0.n), 256, 0, f.stream>>>(
+            cy.view(0), A0f, dinv0, cy.om, r, cy.X(0), lvv<V>(l0, 4), cy.flag, {}, nullptr, nullptr,
+            cy.spos(0));
+    else
+        k_amg_pre_residual<V, true><<<blocks(l0.n), 256, 0, f.stream>>>(
+            cy.view(0), A0, dinv0, cy.om, r, cy.X(0), lvv<V>(l0, 4), cy.flag, {}, nullptr, nullptr,
+            cy.spos(0));
+    if (timing) clock_end(f, kClkAmgPre);
+    cy.timing = timing;
     const auto cc = cy.coarse(0);
-    if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[2], f.stream));
-    k_amg_prolong_smooth<V, true><<<kRedGrid, 256, 0, f.stream>>>(
-        cy.view(0), A0, dinv0, cy.om, r, cy.X(0), amg.lv[1].agg.p, cc.alpha, cc.ec, z, rz_part,
-        p_init, cy.flag, q, qz_part, cc.kc, cc.ec2);
-    if (timing) OAGPU_HIP(hipEventRecord(f.ev_amg[3], f.stream));
+    if (timing) clock_begin(f, kClkAmgPost);
+    if (fine_f32)
+        k_amg_prolong_smooth<V, true, float><<<kRedGrid, 256, 0, f.stream>>>(
+            cy.view(0), A0f, dinv0, cy.om, r, cy.X(0), amg.lv[1].agg.p, cc.alpha, cc.ec, z, rz_part,
+            p_init, cy.flag, q, qz_part, cc.kc, cc.ec2);
+    else
+        k_amg_prolong_smooth<V, true><<<kRedGrid, 256, 0, f.stream>>>(
+            cy.view(0), A0, dinv0, cy.om, r, cy.X(0), amg.lv[1].agg.p, cc.alpha, cc.ec, z, rz_part,
+            p_init, cy.flag, q, qz_part, cc.kc, cc.ec2);
+    if (timing) clock_end(f, kClkAmgPost);
     OAGPU_HIP(hipGetLastError());
 }
 
 template void amg_setup<double>(Flow&, const double*, const double*);
 template void amg_setup<float>(Flow&, const float*, const float*);
 template void amg_vcycle<double>(Flow&, const double*, const double*, const double*, double*,
-                                 double*, double*, bool, bool, const double*, double*);
+                                 double*, double*, bool, bool, const double*, double*,
+                                 const float*);
 template void amg_vcycle<float>(Flow&, const float*, const float*, const float*, float*, double*,
-                                float*, bool, bool, const float*, double*);
+                                float*, bool, bool, const float*, double*, const float*);
 
 }  // namespace oagpu

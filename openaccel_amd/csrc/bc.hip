@@ -95,7 +95,7 @@ SideArgs side_args(Flow& f) {
     a.init = 0;
     a.rfl = f.rfl.p;
     a.msd = f.msd.p;
-    a.err = m.err.p;
+    a.err = f.err.p;
     return a;
 }
 

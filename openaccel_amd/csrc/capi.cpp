@@ -78,26 +78,18 @@ double* field_ptr(oagpu::Flow& f, const char* name, size_t* count) {
     throw oagpu::Error(OAGPU_EINVAL, std::string("unknown field ") + name);
 }
 
+// Each flow has its own error word: flows that share a mesh (batched cases)
+// run concurrently on their own streams and never see each other's flags.
 void check_err(oagpu::Flow& f) {
     int e = 0;
-    f.m->err.download(&e, 1, f.stream);
+    f.err.download(&e, 1, f.stream);
     OAGPU_HIP(hipStreamSynchronize(f.stream));
     if (e) {
-        f.m->err.zero(f.stream);
+        f.err.zero(f.stream);
         throw oagpu::Error(OAGPU_EGEOM, "non-positive Jacobian determinant in a kernel");
     }
 }
 
-template <typename F>
-double timed(oagpu::Flow& f, F&& fn) {
-    OAGPU_HIP(hipEventRecord(f.ev0, f.stream));
-    fn();
-    OAGPU_HIP(hipEventRecord(f.ev1, f.stream));
-    OAGPU_HIP(hipEventSynchronize(f.ev1));
-    float ms = 0.f;
-    OAGPU_HIP(hipEventElapsedTime(&ms, f.ev0, f.ev1));
-    return double(ms);
-}
 }  // namespace
 
 extern "C" {
@@ -141,7 +133,10 @@ int oagpu_mesh_create(int n_nodes, const double* xyz, int n_elems, const int* co
         hm.bf_bc = bf_bc;
         oagpu::build_mesh(hm, stencil == OAGPU_STENCIL_FULL ? 0 : 1, h->m);
         oagpu::launch_mesh_geometry(h->m, nullptr);
+        int e = 0;
+        h->m.err.download(&e, 1);
         OAGPU_HIP(hipDeviceSynchronize());
+        if (e) throw oagpu::Error(OAGPU_EGEOM, "non-positive Jacobian determinant in the mesh");
         *out = h.release();
     });
 }
@@ -254,6 +249,8 @@ int oagpu_flow_create(oagpu_mesh* m, const oagpu_flow_params* prm, int n_bc, con
         f.red.alloc(192);
         f.scal.alloc(16);
         f.flag.alloc(4);
+        f.err.alloc(1);
+        f.err.zero();
         for (auto* b : {&f.U, &f.p, &f.gradU, &f.gradP, &f.du, &f.mdot, &f.div, &f.A3, &f.b3, &f.x3,
                         &f.A1, &f.b1, &f.x1})
             b->zero();
@@ -267,9 +264,6 @@ int oagpu_flow_create(oagpu_mesh* m, const oagpu_flow_params* prm, int n_bc, con
         f.wnode.alloc(nn);
         OAGPU_HIP(hipMemcpy(f.wnode.p, m->m.bnode.p, nn * sizeof(int), hipMemcpyDeviceToDevice));
         OAGPU_HIP(hipStreamCreate(&f.stream));
-        OAGPU_HIP(hipEventCreate(&f.ev0));
-        OAGPU_HIP(hipEventCreate(&f.ev1));
-        for (auto& e : f.ev_amg) OAGPU_HIP(hipEventCreate(&e));
         // boundary ids must index the velocity table
         {
             std::vector<int> ids(m->m.nbf ? m->m.nbf : 1);
@@ -289,10 +283,6 @@ int oagpu_flow_destroy(oagpu_flow* f) {
     return guarded([&] {
         if (!f) return;
         if (f->f.stream) (void)hipStreamSynchronize(f->f.stream);
-        if (f->f.ev0) (void)hipEventDestroy(f->f.ev0);
-        if (f->f.ev1) (void)hipEventDestroy(f->f.ev1);
-        for (auto& e : f->f.ev_amg)
-            if (e) (void)hipEventDestroy(e);
         if (f->f.stream) (void)hipStreamDestroy(f->f.stream);
         delete f;
     });
@@ -339,7 +329,7 @@ int oagpu_initialize(oagpu_flow* h) {
 int oagpu_assemble_momentum(oagpu_flow* h) {
     return guarded([&] {
         OAGPU_REQUIRE(h, "null flow");
-        h->f.ms_momentum = timed(h->f, [&] { oagpu::launch_assemble_momentum(h->f); });
+        oagpu::launch_assemble_momentum(h->f);
         check_err(h->f);
     });
 }
@@ -347,7 +337,7 @@ int oagpu_assemble_momentum(oagpu_flow* h) {
 int oagpu_assemble_pcorr(oagpu_flow* h) {
     return guarded([&] {
         OAGPU_REQUIRE(h, "null flow");
-        h->f.ms_pcorr = timed(h->f, [&] { oagpu::launch_assemble_pcorr(h->f); });
+        oagpu::launch_assemble_pcorr(h->f);
         check_err(h->f);
     });
 }
@@ -394,7 +384,7 @@ int oagpu_solution_download(oagpu_flow* h, int N, double* x) {
 int oagpu_update_mdot(oagpu_flow* h) {
     return guarded([&] {
         OAGPU_REQUIRE(h, "null flow");
-        h->f.ms_mdot = timed(h->f, [&] { oagpu::launch_update_mdot(h->f); });
+        oagpu::launch_update_mdot(h->f);
         check_err(h->f);
     });
 }
@@ -410,7 +400,8 @@ int oagpu_mass_divergence(oagpu_flow* h) {
 int oagpu_gradient(oagpu_flow* h, int which) {
     return guarded([&] {
         OAGPU_REQUIRE(h && which >= 0 && which <= 3, "bad argument");
-        h->f.ms_grad = timed(h->f, [&] { oagpu::launch_gradient(h->f, which); });
+        oagpu::launch_gradient(h->f, which);
+        OAGPU_HIP(hipStreamSynchronize(h->f.stream));
     });
 }
 
@@ -754,14 +745,14 @@ void run_outer_iteration(oagpu::Flow& f, const double* ctl, double* hist) {
     oagpu_control_data cdU{}, cdP{};
     double rU[3], rP[1];
     // momentum predictor (navierStokesEquation::solve :157-222)
-    f.ms_momentum = timed(f, [&] { oagpu::launch_assemble_momentum(f); });
+    oagpu::launch_assemble_momentum(f);
     oagpu::scaled_residual(f, 3, rU);
     oagpu::krylov_solve(f, 3, OAGPU_GMRES, int(ctl[0]), ctl[1], 1e-30, 30, &cdU);
     oagpu::launch_correct(f, 3, 1.0);
-    f.ms_mdot = timed(f, [&] { oagpu::launch_update_mdot(f); });
+    oagpu::launch_update_mdot(f);
     oagpu::field_scales(f, nullptr, nullptr);
     // pressure correction (pressureCorrectionEquation::solve :89-164)
-    f.ms_pcorr = timed(f, [&] { oagpu::launch_assemble_pcorr(f); });
+    oagpu::launch_assemble_pcorr(f);
     oagpu::scaled_residual(f, 1, rP);
     oagpu::krylov_solve(f, 1, f.prm.p_method, int(ctl[2]), ctl[3], 1e-30, 30, &cdP);
     oagpu::launch_correct(f, 1, f.prm.urf_p);
@@ -773,7 +764,7 @@ void run_outer_iteration(oagpu::Flow& f, const double* ctl, double* hist) {
     oagpu::launch_gradient(f, 1);
     oagpu::launch_velocity_correction(f, 2);
     oagpu::field_scales(f, nullptr, nullptr);
-    f.ms_grad = timed(f, [&] { oagpu::launch_gradient(f, 0); });
+    oagpu::launch_gradient(f, 0);
     if (f.prm.high_res) oagpu::launch_blending(f, 0);  // segregatedFlowEquations.cpp:284-288
     check_err(f);
     if (hist) {
@@ -846,18 +837,40 @@ int oagpu_batch_outer_iterations(oagpu_flow* const* flows, int n, const double* 
     });
 }
 
-int oagpu_last_kernel_ms(oagpu_flow* h, const char* which, double* ms) {
+int oagpu_profile(oagpu_flow* h, const char* which) {
     return guarded([&] {
-        OAGPU_REQUIRE(h && which && ms, "null argument");
-        const oagpu::Flow& f = h->f;
-        if (!std::strcmp(which, "momentum")) *ms = f.ms_momentum;
-        else if (!std::strcmp(which, "pcorr")) *ms = f.ms_pcorr;
-        else if (!std::strcmp(which, "mdot")) *ms = f.ms_mdot;
-        else if (!std::strcmp(which, "gradU")) *ms = f.ms_grad;
-        else if (!std::strcmp(which, "amg_pre")) *ms = f.ms_amg_pre;
-        else if (!std::strcmp(which, "amg_post")) *ms = f.ms_amg_post;
-        else if (!std::strcmp(which, "amg_vcycles")) *ms = double(f.amg_vcycles);
-        else throw oagpu::Error(OAGPU_EINVAL, "unknown kernel name");
+        OAGPU_REQUIRE(h && which, "null argument");
+        oagpu::Flow& f = h->f;
+        unsigned mask = 0;
+        if (!std::strcmp(which, "all")) {
+            mask = (1u << oagpu::kClkCount) - 1;
+        } else if (std::strcmp(which, "none") != 0) {
+            for (int k = 0; k < oagpu::kClkCount; ++k)
+                if (!std::strcmp(which, oagpu::clock_name(k))) mask = 1u << k;
+            OAGPU_REQUIRE(mask, std::string("unknown kernel name ") + which);
+        }
+        OAGPU_HIP(hipStreamSynchronize(f.stream));
+        for (auto& c : f.clk) {
+            c.used = 0;
+            c.total_ms = 0.0;
+            c.launches = 0;
+        }
+        f.prof_mask = mask;
+    });
+}
+
+int oagpu_kernel_stats(oagpu_flow* h, const char* which, double* total_ms, long long* launches) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && which && total_ms && launches, "null argument");
+        oagpu::Flow& f = h->f;
+        int id = -1;
+        for (int k = 0; k < oagpu::kClkCount; ++k)
+            if (!std::strcmp(which, oagpu::clock_name(k))) id = k;
+        OAGPU_REQUIRE(id >= 0, std::string("unknown kernel name ") + which);
+        OAGPU_HIP(hipStreamSynchronize(f.stream));
+        f.clk[id].fold();
+        *total_ms = f.clk[id].total_ms;
+        *launches = f.clk[id].launches;
     });
 }
 

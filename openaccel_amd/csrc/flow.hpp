@@ -5,6 +5,7 @@
 #include "mesh.hpp"
 
 #include <array>
+#include <vector>
 
 namespace oagpu {
 
@@ -38,6 +39,39 @@ struct Amg {
     DBuf<float> fdense;
 };
 
+// Kernels whose device time the library can measure (oagpu_profile /
+// oagpu_kernel_stats): each clocked launch records a pair of HIP events on
+// the flow's stream from a per-kernel pool; nothing waits on them until the
+// host asks for the totals.  Launches captured into a HIP graph are not
+// clocked.
+enum ClockId {
+    kClkMomentum, kClkPcorr, kClkScalar, kClkMdot, kClkGradU, kClkGradP, kClkAmgPre,
+    kClkAmgPost, kClkAmgTail, kClkSpmv1, kClkSpmv3, kClkCount
+};
+inline const char* clock_name(int id) {
+    static const char* names[kClkCount] = {"momentum", "pcorr", "scalar", "mdot", "gradU",
+                                           "gradP", "amg_pre", "amg_post", "amg_tail", "spmv1",
+                                           "spmv3"};
+    return names[id];
+}
+struct KernelClock {
+    std::vector<hipEvent_t> ev;  // pairs (start, stop)
+    size_t used = 0;             // pairs recorded and not yet folded in
+    double total_ms = 0.0;
+    long long launches = 0;
+    void fold() {
+        for (size_t k = 0; k < used; ++k) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]) == hipSuccess) total_ms += ms;
+        }
+        launches += (long long)used;
+        used = 0;
+    }
+    ~KernelClock() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+
 // An instantiated HIP graph of a block of CG iterations and the operands it
 // was captured with.
 struct CgGraph {
@@ -67,16 +101,13 @@ struct Flow {
     DBuf<double> red;                    // reduced values [64]
     DBuf<double> scal;                   // device-resident Krylov scalars
     DBuf<int> flag;                      // device-resident convergence flag / counters
+    DBuf<int> err;                       // this flow's device error word (non-positive Jacobians)
     int V_cols = 0;
     double U_scale = 1.0, p_scale = 1.0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double ms_momentum = 0, ms_pcorr = 0, ms_mdot = 0, ms_grad = 0;
-    // fine-level AMG kernels, timed on the eager V-cycle that starts each CG
-    // solve: [0..1] pre-smooth+residual, [2..3] prolong+post-smooth
-    hipEvent_t ev_amg[4] = {nullptr, nullptr, nullptr, nullptr};
-    double ms_amg_pre = 0, ms_amg_post = 0;
-    long amg_vcycles = 0;   // V-cycles run (counts launches of the fine kernels)
+    // device time of the clocked kernels (profiling on only)
+    unsigned prof_mask = 0;
+    KernelClock clk[kClkCount];
     Amg amg;
     CgGraph cg_graph;
     // effective viscosity mu + mu_t per node (mu alone while laminar)
@@ -107,6 +138,30 @@ struct Flow {
     DBuf<double> k, om, gradK, gradW, ymin, mut, F1, Pk, Pk_wall, kprev;  // om: omega
     double k_scale = 1.0, w_scale = 1.0;
 };
+
+// start / stop of a clocked launch on the flow's stream (no-ops unless the
+// flow's profiling covers the kernel); a pool of 4096 pairs is folded in
+// (after a stream synchronisation) when full.
+inline void clock_begin(Flow& f, int id) {
+    if (!(f.prof_mask & (1u << id))) return;
+    KernelClock& c = f.clk[id];
+    if (c.used == 4096) {
+        OAGPU_HIP(hipStreamSynchronize(f.stream));
+        c.fold();
+    }
+    while (c.ev.size() < 2 * (c.used + 1)) {
+        hipEvent_t e = nullptr;
+        OAGPU_HIP(hipEventCreate(&e));
+        c.ev.push_back(e);
+    }
+    OAGPU_HIP(hipEventRecord(c.ev[2 * c.used], f.stream));
+}
+inline void clock_end(Flow& f, int id) {
+    if (!(f.prof_mask & (1u << id))) return;
+    KernelClock& c = f.clk[id];
+    OAGPU_HIP(hipEventRecord(c.ev[2 * c.used + 1], f.stream));
+    ++c.used;
+}
 
 // kernels.hip
 void launch_mesh_geometry(DevMesh& m, hipStream_t s);
@@ -159,9 +214,11 @@ void amg_setup(Flow& f, const V* A0, const V* dinv0);  // numeric Galerkin level
 // flexible-CG steps each (a nonlinear preconditioner: flexible CG only), else
 // one V-cycle (a fixed linear operator).
 // timing: HIP events around the fine-level kernels (eager launches only)
+// A0f (V = double only): the fine matrix's values rounded to FP32, read by
+// the fine-level kernels instead of A0 (vectors and coarse levels stay V).
 template <typename V>
 void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* rz_part, V* p_init,
                 bool timing = false, bool kcycle = false, const V* q = nullptr,
-                double* qz_part = nullptr);
+                double* qz_part = nullptr, const float* A0f = nullptr);
 
 }  // namespace oagpu

@@ -256,8 +256,10 @@ __global__ void k_residual_copy(size_t n, const double* b, const double* Ax, dou
 template <int N>
 static void spmv(Flow& f, const double* A, const double* x, const double* s, double* y) {
     DevMesh& m = *f.m;
+    clock_begin(f, N == 1 ? kClkSpmv1 : kClkSpmv3);
     k_spmv<N><<<xcd_grid(size_t(m.nn), 256), 256, 0, f.stream>>>(m.nn, m.slice_ptr.p, m.slice_w.p,
                                                                   m.sell_cols.p, A, x, s, y);
+    clock_end(f, N == 1 ? kClkSpmv1 : kClkSpmv3);
     OAGPU_HIP(hipGetLastError());
 }
 
@@ -601,14 +603,6 @@ static int cg_core(Flow& f, const V* A, const V* b, const V* dinv, V* x, V* r, V
     double sc[5];
     f.scal.download(sc, 5, f.stream);
     OAGPU_HIP(hipStreamSynchronize(f.stream));
-    if (amg) {
-        float t0 = 0.f, t1 = 0.f;
-        OAGPU_HIP(hipEventElapsedTime(&t0, f.ev_amg[0], f.ev_amg[1]));
-        OAGPU_HIP(hipEventElapsedTime(&t1, f.ev_amg[2], f.ev_amg[3]));
-        f.ms_amg_pre = t0;
-        f.ms_amg_post = t1;
-        f.amg_vcycles += 1 + hflag[1];
-    }
     *r0 = std::sqrt(sc[4]);
     *r1 = std::sqrt(sc[3]);
     return hflag[1];
@@ -713,6 +707,22 @@ __global__ void k_bicg_x(size_t n, double alpha, const double* p, double omega, 
     if (i < n) x[i] += dinv[i] * (alpha * p[i] + omega * s[i]);
 }
 
+// Breakdown of BiCGStab: an inner product with the shadow residual r0 that
+// vanishes against ||r0|| ||r|| (or is not finite) leaves alpha or beta
+// undefined; the iteration then restarts with r0 = r (Saad 2003, 7.4).
+static bool bicg_breakdown(double d, double r0n, double rn) {
+    return !std::isfinite(d) || std::fabs(d) <= 1e-14 * r0n * rn;
+}
+
+static void bicg_restart(Flow& f, size_t n, double* rho, double* alpha, double* omega, double* r0n,
+                         double rn) {
+    OAGPU_HIP(hipMemcpyAsync(f.r0.p, f.r.p, n * sizeof(double), hipMemcpyDeviceToDevice, f.stream));
+    OAGPU_HIP(hipMemsetAsync(f.pk.p, 0, n * sizeof(double), f.stream));
+    OAGPU_HIP(hipMemsetAsync(f.q.p, 0, n * sizeof(double), f.stream));
+    *rho = *alpha = *omega = 1.0;
+    *r0n = rn;
+}
+
 template <int N>
 static int bicgstab(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
     DevMesh& m = *f.m;
@@ -734,7 +744,7 @@ static int bicgstab(Flow& f, int max_it, double rtol, double atol, oagpu_control
     }
     cd->solver_initial_res = rn;
     const double tol = std::max(rtol * rn, atol);
-    double rho = 1, alpha = 1, omega = 1;
+    double rho = 1, alpha = 1, omega = 1, r0n = rn;
     int it = 0;
     while (it < max_it && rn > tol) {
         double rho1;
@@ -742,6 +752,10 @@ static int bicgstab(Flow& f, int max_it, double rtol, double atol, oagpu_control
             const double* xs[1] = {f.r0.p};
             const double* ys[1] = {f.r.p};
             dots(f, n, 1, xs, ys, &rho1);
+        }
+        if (bicg_breakdown(rho1, r0n, rn)) {  // r0 . r ~ 0: restart from r
+            bicg_restart(f, n, &rho, &alpha, &omega, &r0n, rn);
+            continue;
         }
         const double beta = (rho1 / rho) * (alpha / omega);
         rho = rho1;
@@ -752,6 +766,11 @@ static int bicgstab(Flow& f, int max_it, double rtol, double atol, oagpu_control
             const double* xs[1] = {f.r0.p};
             const double* ys[1] = {f.q.p};
             dots(f, n, 1, xs, ys, &r0v);
+        }
+        if (bicg_breakdown(r0v, r0n, rn)) {  // r0 . v ~ 0: alpha undefined
+            bicg_restart(f, n, &rho, &alpha, &omega, &r0n, rn);
+            ++it;
+            continue;
         }
         alpha = rho / r0v;
         k_lin2<<<G, B, 0, f.stream>>>(n, 1.0, f.r.p, -alpha, f.q.p, f.s.p);
@@ -782,18 +801,23 @@ __global__ void k_axpby_add(size_t n, double a, const double* x, double b, const
     if (i < n) out[i] += a * x[i] + b * y[i];
 }
 
-// z = M^-1 r, one V-cycle on the FP64 hierarchy; the V-cycle's r.z partials
-// go to a spare slot
-static void amg_apply(Flow& f, const double* A, const double* dinv, const double* r, double* z) {
-    amg_vcycle<double>(f, A, dinv, r, z, f.part.p + 8 * kRedGrid, nullptr);
+// z = M^-1 r, one V-cycle on the FP64 hierarchy (Af: the fine level's
+// values in FP32, or null); the V-cycle's r.z partials go to a spare slot
+static void amg_apply(Flow& f, const double* A, const double* dinv, const double* r, double* z,
+                      const float* Af) {
+    amg_vcycle<double>(f, A, dinv, r, z, f.part.p + 8 * kRedGrid, nullptr, true, false, nullptr,
+                       nullptr, Af);
     OAGPU_HIP(hipGetLastError());
 }
 
 // BiCGStab (Saad Alg. 7.7) right-preconditioned by one AMG V-cycle: the
 // p' systems of flows with static-pressure outlets, or on distorted meshes,
-// are not symmetric, and AMG-CG stalls on them.  FP64 throughout, in both
-// forms: with the V-cycle on the FP32 hierarchy BiCGStab needed 15-40 % more
-// iterations on the backward-facing step, which cancelled the cheaper cycles.
+// are not symmetric, and AMG-CG stalls on them.  Krylov vectors, the
+// operator and the V-cycle's vectors are FP64 in both forms (with the whole
+// V-cycle on the FP32 hierarchy BiCGStab needed 15-40 % more iterations on
+// the backward-facing step, which cancelled the cheaper cycles); the mixed
+// form's V-cycle reads the fine level's values rounded to FP32 (the fine
+// level's two kernels move a third fewer bytes).
 static int bicgstab_amg(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
     DevMesh& m = *f.m;
     const size_t n = size_t(m.nn);
@@ -802,6 +826,12 @@ static int bicgstab_amg(Flow& f, int max_it, double rtol, double atol, oagpu_con
     double* x = f.x1.p;
     double *ph = f.z.p, *sh = f.w.p;   // M^-1 p, M^-1 s
     const unsigned B = 256, G = unsigned((n + B - 1) / B);
+    const float* Af = nullptr;
+    if (f.prm.precision == OAGPU_MIXED) {
+        if (f.Af.n < m.nsell) f.Af.alloc(m.nsell);
+        k_to_float<<<unsigned((m.nsell + B - 1) / B), B, 0, f.stream>>>(m.nsell, 1.0, A, f.Af.p);
+        Af = f.Af.p;
+    }
     k_clear_flag<<<1, 1, 0, f.stream>>>(f.flag.p);  // the V-cycle kernels honour the CG stop flag
     OAGPU_HIP(hipMemsetAsync(x, 0, n * sizeof(double), f.stream));
     OAGPU_HIP(hipMemcpyAsync(f.r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, f.stream));
@@ -816,7 +846,7 @@ static int bicgstab_amg(Flow& f, int max_it, double rtol, double atol, oagpu_con
     }
     cd->solver_initial_res = rn;
     const double tol = std::max(rtol * rn, atol);
-    double rho = 1, alpha = 1, omega = 1;
+    double rho = 1, alpha = 1, omega = 1, r0n = rn;
     int it = 0;
     while (it < max_it && rn > tol) {
         double rho1;
@@ -825,10 +855,14 @@ static int bicgstab_amg(Flow& f, int max_it, double rtol, double atol, oagpu_con
             const double* ys[1] = {f.r.p};
             dots(f, n, 1, xs, ys, &rho1);
         }
+        if (bicg_breakdown(rho1, r0n, rn)) {  // r0 . r ~ 0: restart from r
+            bicg_restart(f, n, &rho, &alpha, &omega, &r0n, rn);
+            continue;
+        }
         const double beta = (rho1 / rho) * (alpha / omega);
         rho = rho1;
         k_bicg_p<<<G, B, 0, f.stream>>>(n, beta, omega, f.r.p, f.q.p, f.pk.p);
-        amg_apply(f, A, f.dinv.p, f.pk.p, ph);
+        amg_apply(f, A, f.dinv.p, f.pk.p, ph, Af);
         spmv<1>(f, A, ph, nullptr, f.q.p);  // v = A M^-1 p
         double r0v;
         {
@@ -836,9 +870,14 @@ static int bicgstab_amg(Flow& f, int max_it, double rtol, double atol, oagpu_con
             const double* ys[1] = {f.q.p};
             dots(f, n, 1, xs, ys, &r0v);
         }
+        if (bicg_breakdown(r0v, r0n, rn)) {  // r0 . v ~ 0: alpha undefined
+            bicg_restart(f, n, &rho, &alpha, &omega, &r0n, rn);
+            ++it;
+            continue;
+        }
         alpha = rho / r0v;
         k_lin2<<<G, B, 0, f.stream>>>(n, 1.0, f.r.p, -alpha, f.q.p, f.s.p);
-        amg_apply(f, A, f.dinv.p, f.s.p, sh);
+        amg_apply(f, A, f.dinv.p, f.s.p, sh, Af);
         spmv<1>(f, A, sh, nullptr, f.t.p);  // t = A M^-1 s
         double tt_ts[2];
         {
@@ -855,7 +894,6 @@ static int bicgstab_amg(Flow& f, int max_it, double rtol, double atol, oagpu_con
         rn = std::sqrt(rn);
         if (omega == 0.0) break;
     }
-    f.amg_vcycles += 2 * it;
     cd->solver_final_res = rn;
     return it;
 }
@@ -899,6 +937,8 @@ int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double ato
     } else
         throw Error(OAGPU_EINVAL, "unknown Krylov method");
     cd->n_iterations = it;
+    if (!std::isfinite(cd->solver_final_res) || !std::isfinite(cd->solver_initial_res))
+        throw Error(OAGPU_ENONFINITE, "the linear solve produced a non-finite residual");
     return it;
 }
 

@@ -961,14 +961,16 @@ void launch_assemble_momentum(Flow& f) {
     a.vals = f.A3.p;
     a.b = f.b3.p;
     a.du = f.du.p;
-    a.err = m.err.p;
+    a.err = f.err.p;
     // off-diagonal blocks summed in a per-wavefront LDS tile: double in the
     // FP64 form, float in the mixed form (the element contributions are FP32
     // there; the at most four of them per block are summed in FP32 before the
     // single FP64 store)
     const bool mixed = f.prm.precision == OAGPU_MIXED;
     auto go = [&](auto kern, unsigned B) {
+        clock_begin(f, kClkMomentum);
         kern<<<xcd_grid(m.nn, B), B, 0, f.stream>>>(a);
+        clock_end(f, kClkMomentum);
     };
     const bool hr = a.beta != nullptr;
     auto pick = [&](auto mx) {
@@ -1390,7 +1392,7 @@ void launch_assemble_pcorr(Flow& f) {
     a.p_ref = f.has_outlet ? -1 : f.prm.p_ref_node;  // domain::pressureLevelRequired
     a.vals = f.A1.p;
     a.b = f.b1.p;
-    a.err = m.err.p;
+    a.err = f.err.p;
     const unsigned B = 64, G = xcd_grid(m.nn, B);
     const bool mixed = f.prm.precision == OAGPU_MIXED;
     auto pick = [&](auto mx) {
@@ -1403,8 +1405,10 @@ void launch_assemble_pcorr(Flow& f) {
             else k_pcorr_rows<double, 0, MX><<<G, B, 0, f.stream>>>(a);
         }
     };
+    clock_begin(f, kClkPcorr);
     if (m.mixed) pick(std::true_type{});
     else pick(std::false_type{});
+    clock_end(f, kClkPcorr);
     OAGPU_HIP(hipGetLastError());
     if (f.has_io) {
         const unsigned GB = xcd_grid(size_t(f.nio_row), 256);
@@ -1910,12 +1914,16 @@ void launch_assemble_scalar(Flow& f, int which) {
     a.us = (which == 0 ? f.sst.k_shifted : f.sst.w_shifted) ? 1 : 0;
     a.vals = f.A1.p;
     a.b = f.b1.p;
-    a.err = m.err.p;
+    a.err = f.err.p;
     const unsigned B = 64, G = xcd_grid(m.nn, B);
     const bool mixed = f.prm.precision == OAGPU_MIXED;
     const bool gs = (which == 0 ? f.sst.kgrad_shifted : f.sst.wgrad_shifted) != 0;
     const bool hr = a.beta != nullptr;
-    auto go = [&](auto kern) { kern<<<G, B, 0, f.stream>>>(a); };
+    auto go = [&](auto kern) {
+        clock_begin(f, kClkScalar);
+        kern<<<G, B, 0, f.stream>>>(a);
+        clock_end(f, kClkScalar);
+    };
     auto pick = [&](auto mx) {
         constexpr bool MX = decltype(mx)::value;
         if (gs) {
@@ -2114,7 +2122,7 @@ static void launch_mdot_generic(Flow& f, bool init) {
         kern<<<G, B, 0, f.stream>>>(m.nglist, m.glist.p, m.etype.p, m.xyz.p, m.conn.p, m.scvw.p, f.U.p,
                                     f.du.p, f.p.p, f.gradP.p, f.prm.rho, f.prm.urf_mdot,
                                     f.prm.u_shifted ? 1 : 0, f.prm.pgrad_shifted ? 1 : 0,
-                                    init ? 1 : 0, f.mdot.p, m.err.p);
+                                    init ? 1 : 0, f.mdot.p, f.err.p);
     };
     go(k_mdot_generic<topo::kTet>);
     go(k_mdot_generic<topo::kWed>);
@@ -2142,13 +2150,15 @@ void launch_update_mdot(Flow& f) {
         kern<<<G, B, 0, f.stream>>>(m.ne, m.negroup, m.egroup_order.p, m.mixed ? m.etype.p : nullptr,
                                     m.xyz.p, m.conn.p, m.scvw.p,
                                     f.U.p, f.du.p, f.p.p, f.gradP.p, f.prm.rho, f.prm.urf_mdot,
-                                    f.mdot.p, m.err.p);
+                                    f.mdot.p, f.err.p);
     };
     const int us = f.prm.u_shifted ? 1 : 0, sp = f.prm.pgrad_shifted ? 1 : 0;
+    clock_begin(f, kClkMdot);
     if (us && sp) args(k_mdot_update<1, 1>);
     else if (us) args(k_mdot_update<1, 0>);
     else if (sp) args(k_mdot_update<0, 1>);
     else args(k_mdot_update<0, 0>);
+    clock_end(f, kClkMdot);
     OAGPU_HIP(hipGetLastError());
     launch_mdot_generic(f, false);
     launch_side_mdot(f, false);
@@ -2496,12 +2506,15 @@ void launch_gradient(Flow& f, int which) {
     // (velocity.cpp:52, pressure.cpp:48), not to k / omega (default off)
     const double* sym = f.has_sym && which <= 1 ? f.symA.p : nullptr;
     const int* types = f.has_sym ? f.bc_type.p : nullptr;  // symmetry faces add no term
+    const int clk = which == 0 ? kClkGradU : kClkGradP;
     auto go = [&](auto kern, const double* phi, double* grad) {
+        clock_begin(f, clk);
         kern<<<G, B, 0, f.stream>>>(m.nn, m.nslice, m.slice_order.p, m.xyz.p, m.conn.p, m.inc_la.p,
                                     m.slots_la.p, m.ext_ptr.p, m.ext.p, m.ext_slots.p, m.binc_ptr.p,
                                     m.binc.p, m.bf_elem.p, m.bf_ord.p, m.bf_bc.p, types, sym,
                                     m.vol.p, phi, m.mixed ? m.gen_ptr.p : nullptr, m.gen.p,
                                     m.etype.p, f.prm.grad_urf, grad);
+        clock_end(f, clk);
     };
     auto pick = [&](auto kern_hex, auto kern_mixed, const double* phi, double* grad) {
         if (m.mixed) go(kern_mixed, phi, grad);

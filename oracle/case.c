@@ -34,6 +34,65 @@ static void *xcalloc(size_t n, size_t s)
     return p;
 }
 
+int or_nthreads = 1;
+
+void or_set_threads(int n) { or_nthreads = n > 0 ? n : 1; }
+
+/* Greedy colouring of the elements by shared nodes (with one thread: one
+ * class, element order, so single-threaded results keep their summation
+ * order). */
+void or_color_elements(or_case *c)
+{
+    const int want = or_nthreads > 1 ? 2 : 1;
+    if (c->color_ptr && c->color_threads == want) return;
+    free(c->color_ptr);
+    free(c->color_elem);
+    c->color_threads = want;
+    c->color_elem = (int *)xcalloc((size_t)c->ne, sizeof(int));
+    if (want == 1) {
+        c->ncolor = 1;
+        c->color_ptr = (int *)xcalloc(2, sizeof(int));
+        c->color_ptr[1] = c->ne;
+        for (int e = 0; e < c->ne; ++e) c->color_elem[e] = e;
+        return;
+    }
+    int *col = (int *)xcalloc((size_t)c->ne, sizeof(int));
+    unsigned long long *used = (unsigned long long *)xcalloc((size_t)c->nn, sizeof(unsigned long long));
+    int ncol = 0, base = 0;
+    /* classes of at most 64 colours per sweep; elements that find none wait
+     * for the next sweep */
+    int left = c->ne;
+    for (int e = 0; e < c->ne; ++e) col[e] = -1;
+    while (left > 0) {
+        memset(used, 0, sizeof(unsigned long long) * (size_t)c->nn);
+        int top = 0;
+        for (int e = 0; e < c->ne; ++e) {
+            if (col[e] >= 0) continue;
+            const int npe = or_topo_npe(or_etype(c, e));
+            unsigned long long m = 0;
+            for (int a = 0; a < npe; ++a) m |= used[c->conn[8 * e + a]];
+            if (m == ~0ULL) continue;
+            int k = 0;
+            while (m & (1ULL << k)) ++k;
+            for (int a = 0; a < npe; ++a) used[c->conn[8 * e + a]] |= 1ULL << k;
+            col[e] = base + k;
+            if (k + 1 > top) top = k + 1;
+            --left;
+        }
+        base += top;
+        ncol = base;
+    }
+    c->ncolor = ncol;
+    c->color_ptr = (int *)xcalloc((size_t)ncol + 1, sizeof(int));
+    for (int e = 0; e < c->ne; ++e) c->color_ptr[col[e] + 1]++;
+    for (int k = 0; k < ncol; ++k) c->color_ptr[k + 1] += c->color_ptr[k];
+    int *fill = (int *)xcalloc((size_t)ncol, sizeof(int));
+    for (int e = 0; e < c->ne; ++e) c->color_elem[c->color_ptr[col[e]] + fill[col[e]]++] = e;
+    free(fill);
+    free(col);
+    free(used);
+}
+
 or_case *or_case_create(int nn, int ne, const double *xyz, const int *conn,
                         const int *etype, int nbf, const int *bf_elem, const int *bf_ord,
                         const int *bf_bc, int nbc, const double *bc_U, const or_params *prm)
@@ -131,6 +190,10 @@ or_case *or_case_create(int nn, int ne, const double *xyz, const int *conn,
 
 void or_case_destroy(or_case *c)
 {
+    if (c) {
+        free(c->color_ptr);
+        free(c->color_elem);
+    }
     if (!c) return;
     void *ptrs[] = {c->xyz, c->conn, c->etype, c->bf_elem, c->bf_ord, c->bf_bc, c->bc_U,
                     c->row_ptr, c->cols, c->diag_off, c->A3, c->b3, c->x3,
@@ -272,8 +335,9 @@ static void mdot_init(or_case *c)
      * velocity interpolation is linear_linear (flowModel.cpp:3310-3316) */
     or_tables tab[4];
     or_tables_init(tab);
-    double co[24], av[36];
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
     for (int e = 0; e < c->ne; ++e) {
+        double co[24], av[36];
         const int t = or_etype(c, e);
         const or_tables *T = &tab[t];
         const int npe = T->npe;
@@ -300,9 +364,10 @@ void or_update_mdot(or_case *c)
 {
     or_tables tab[4];
     or_tables_init(tab);
-    double co[24], av[36], dndx[288], detj[12], w[8];
     const double urf = c->prm.urf_mdot;
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
     for (int e = 0; e < c->ne; ++e) {
+        double co[24], av[36], dndx[288], detj[12], w[8];
         const int t = or_etype(c, e);
         const or_tables *T = &tab[t];
         const int npe = T->npe, *lr = T->lr;
@@ -347,7 +412,9 @@ void or_update_mdot(or_case *c)
 void or_mass_divergence(or_case *c)
 {
     memset(c->div, 0, sizeof(double) * (size_t)c->nn);
-    for (int e = 0; e < c->ne; ++e) {
+    or_color_elements(c);
+    OR_PARALLEL
+    OR_FOR_COLORED(c, e) {
         const int t = or_etype(c, e);
         const int *en = &c->conn[8 * e];
         const int *lr = t == OR_HEX8 ? or_hex_lrscv() : or_topo_lrscv(t);
@@ -416,8 +483,11 @@ void or_gradient_field(or_case *c, const double *phi, double *grad, int N, int s
     for (size_t k = 0; k < (size_t)c->nn * 3 * N; ++k) grad[k] *= (1.0 - gurf);
     or_tables tab[4];
     or_tables_init(tab);
+    or_color_elements(c);
+    OR_PARALLEL
+    {
     double co[24], av[36];
-    for (int e = 0; e < c->ne; ++e) {
+    OR_FOR_COLORED(c, e) {
         const int t = or_etype(c, e);
         const or_tables *T = &tab[t];
         const int npe = T->npe, *lr = T->lr;
@@ -439,6 +509,7 @@ void or_gradient_field(or_case *c, const double *phi, double *grad, int N, int s
                         gurf * (pip[i] - phi[N * R + i]) * av[3 * ip + j] * ivR;
                 }
         }
+    }
     }
     /* boundary ip contribution (walls; symmetry planes are skipped) */
     double fsf[16];
@@ -474,6 +545,7 @@ void or_assemble_momentum(or_case *c)
     int nodes[8];
 
     /* node terms, steady (navierStokesAssemblerNodeTerms.cpp:16-149) */
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
     for (int i = 0; i < nn; ++i) {
         double l9[9] = {0}, r3[3] = {0};
         const double rho = c->rho[i], vol = c->vol[i], div = c->div[i];
@@ -491,8 +563,12 @@ void or_assemble_momentum(or_case *c)
     /* interior element terms */
     or_tables tab[4];
     or_tables_init(tab);
-    double co[24], av[36], dndx[288], detj[12];
-    for (int e = 0; e < c->ne; ++e) {
+    or_color_elements(c);
+    OR_PARALLEL
+    {
+    double co[24], av[36], dndx[288], detj[12], lhs[576], rhs[24];
+    int nodes[8];
+    OR_FOR_COLORED(c, e) {
         const int t = or_etype(c, e);
         const or_tables *T = &tab[t];
         const int npe = T->npe, *lr = T->lr, S = 3 * npe; /* local row stride */
@@ -563,6 +639,7 @@ void or_assemble_momentum(or_case *c)
         }
         or_apply_coeff(3, npe, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
     }
+    }
 
     /* no-slip walls (navierStokesAssemblerElemBoundaryConditions.cpp:2659-2884)
      * and symmetry planes (:2082-2375, IP_FULL_STRESS__ :296-343 with the
@@ -570,6 +647,7 @@ void or_assemble_momentum(or_case *c)
     const int *sn = or_hex_side_nodes();
     double fsf[16];
     or_quad_face_shape(P->u_shifted, fsf);
+    double co[24];
     for (int f = 0; f < c->nbf; ++f) {
         const int e = c->bf_elem[f], ord = c->bf_ord[f];
         const int btype = c->bc_type[c->bf_bc[f]];
@@ -729,10 +807,14 @@ void or_assemble_pcorr(or_case *c)
     memset(c->b1, 0, sizeof(double) * (size_t)nn);
     or_tables tab[4];
     or_tables_init(tab);
-    double co[24], av[36], dndx[288], detj[12], w[8];
     double lhs[64], rhs[8];
     int nodes[8];
-    for (int e = 0; e < c->ne; ++e) {
+    or_color_elements(c);
+    OR_PARALLEL
+    {
+    double co[24], av[36], dndx[288], detj[12], w[8], lhs[64], rhs[8];
+    int nodes[8];
+    OR_FOR_COLORED(c, e) {
         const int t = or_etype(c, e);
         const or_tables *T = &tab[t];
         const int npe = T->npe, *lr = T->lr;
@@ -779,6 +861,7 @@ void or_assemble_pcorr(or_case *c)
             rhs[ir] += m;
         }
         or_apply_coeff(1, npe, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A1, c->b1);
+    }
     }
     /* no-slip static walls and symmetry planes add nothing to p' (no
      * frame/mesh motion); inlets and outlets do */

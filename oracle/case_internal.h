@@ -39,7 +39,24 @@ struct or_case {
     double *k, *w, *gradK, *gradW, *ymin, *mut, *F1, *Pk, *Pk_wall, *kprev, *beta_k, *beta_w;
     double k_scale, w_scale;
     int scalar_terms;   /* or_set_scalar_terms mask (test hook), default all */
+    /* element colour classes (no two elements of a class share a node) for
+     * the scattering loops when several threads run them; one class in
+     * element order with one thread */
+    int ncolor, color_threads;
+    int *color_ptr, *color_elem;
 };
+
+/* threads of the oracle's loops (or_set_threads, default 1) */
+extern int or_nthreads;
+void or_color_elements(or_case *c);
+#define OR_PARALLEL _Pragma("omp parallel num_threads(or_nthreads)")
+/* inside OR_PARALLEL: every element once, one colour class after the other,
+ * each class shared out among the threads */
+#define OR_FOR_COLORED(c, e)                                                    \
+    for (int col_ = 0; col_ < (c)->ncolor; ++col_)                              \
+        _Pragma("omp for schedule(static)")                                     \
+        for (int k_ = (c)->color_ptr[col_]; k_ < (c)->color_ptr[col_ + 1]; ++k_) \
+            for (int e = (c)->color_elem[k_], once_ = 1; once_; once_ = 0)
 
 static inline int or_etype(const or_case *c, int e) { return c->etype ? c->etype[e] : OR_HEX8; }
 

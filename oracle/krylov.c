@@ -10,7 +10,7 @@
  * PCG (Alg. 9.1) and BiCGStab (Alg. 7.7, right preconditioned), in the same
  * form the HIP path runs, so that the two can be compared iterate by iterate.
  */
-#include "oracle.h"
+#include "case_internal.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -26,6 +26,7 @@ typedef struct {
 static void spmv(const sys_t *s, const double *x, double *y)
 {
     const int N = s->N, NN = N * N;
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
     for (int i = 0; i < s->n; ++i) {
         double acc[3] = {0, 0, 0};
         for (int k = s->rp[i]; k < s->rp[i + 1]; ++k) {
@@ -41,6 +42,7 @@ static void spmv(const sys_t *s, const double *x, double *y)
 static double dot(int n, const double *a, const double *b)
 {
     double s = 0;
+#pragma omp parallel for num_threads(or_nthreads) reduction(+ : s) schedule(static)
     for (int i = 0; i < n; ++i) s += a[i] * b[i];
     return s;
 }
@@ -48,6 +50,7 @@ static double dot(int n, const double *a, const double *b)
 static void jacobi_inv(const sys_t *s, double *dinv)
 {
     const int N = s->N, NN = N * N;
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
     for (int i = 0; i < s->n; ++i) {
         const double *d = &s->A[(size_t)NN * (s->rp[i] + s->doff[i])];
         for (int m = 0; m < N; ++m) dinv[N * i + m] = 1.0 / d[(N + 1) * m];
@@ -70,6 +73,7 @@ static int gmres(const sys_t *s, const double *b, double *x, int max_it, double 
     double *dinv = (double *)calloc((size_t)n, sizeof(double));
     jacobi_inv(s, dinv);
     spmv(s, x, w);
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
     for (int i = 0; i < n; ++i) w[i] = b[i] - w[i];
     double beta = sqrt(dot(n, w, w));
     const double r0 = beta;
@@ -77,11 +81,13 @@ static int gmres(const sys_t *s, const double *b, double *x, int max_it, double 
     res[0] = r0;
     int it = 0;
     while (it < max_it && beta > tol) {
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
         for (int i = 0; i < n; ++i) V[i] = w[i] / beta;
         memset(g, 0, sizeof(double) * (m + 1));
         g[0] = beta;
         int jend = -1;
         for (int j = 0; j < m && it < max_it; ++j) {
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
             for (int i = 0; i < n; ++i) z[i] = dinv[i] * V[(size_t)n * j + i];
             spmv(s, z, w);
             /* classical Gram-Schmidt, twice */
@@ -91,6 +97,7 @@ static int gmres(const sys_t *s, const double *b, double *x, int max_it, double 
                 for (int k = 0; k <= j; ++k) hk[k] = dot(n, &V[(size_t)n * k], w);
                 for (int k = 0; k <= j; ++k) {
                     const double *vk = &V[(size_t)n * k];
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
                     for (int i = 0; i < n; ++i) w[i] -= hk[k] * vk[i];
                     h[k] += hk[k];
                 }
@@ -99,6 +106,7 @@ static int gmres(const sys_t *s, const double *b, double *x, int max_it, double 
             for (int k = 0; k <= j; ++k) H[(size_t)m * k + j] = h[k];
             H[(size_t)m * (j + 1) + j] = hn;
             if (hn > 0)
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
                 for (int i = 0; i < n; ++i) V[(size_t)n * (j + 1) + i] = w[i] / hn;
             /* previous rotations on column j */
             for (int k = 0; k < j; ++k) {
@@ -127,10 +135,13 @@ static int gmres(const sys_t *s, const double *b, double *x, int max_it, double 
         memset(z, 0, sizeof(double) * n);
         for (int k = 0; k <= jend; ++k) {
             const double *vk = &V[(size_t)n * k];
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
             for (int i = 0; i < n; ++i) z[i] += y[k] * vk[i];
         }
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
         for (int i = 0; i < n; ++i) x[i] += dinv[i] * z[i];
         spmv(s, x, w);
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
         for (int i = 0; i < n; ++i) w[i] = b[i] - w[i];
         beta = sqrt(dot(n, w, w));
     }
@@ -150,6 +161,7 @@ static int pcg(const sys_t *s, const double *b, double *x, int max_it, double rt
     double *dinv = (double *)calloc((size_t)n, sizeof(double));
     jacobi_inv(s, dinv);
     spmv(s, x, q);
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
     for (int i = 0; i < n; ++i) {
         r[i] = b[i] - q[i];
         z[i] = dinv[i] * r[i];
@@ -163,6 +175,7 @@ static int pcg(const sys_t *s, const double *b, double *x, int max_it, double rt
     while (it < max_it && rn > tol) {
         spmv(s, p, q);
         const double alpha = rz / dot(n, p, q);
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
         for (int i = 0; i < n; ++i) {
             x[i] += alpha * p[i];
             r[i] -= alpha * q[i];
@@ -170,10 +183,12 @@ static int pcg(const sys_t *s, const double *b, double *x, int max_it, double rt
         ++it;
         rn = sqrt(dot(n, r, r));
         if (rn <= tol) break;
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
         for (int i = 0; i < n; ++i) z[i] = dinv[i] * r[i];
         const double rz1 = dot(n, r, z);
         const double beta = rz1 / rz;
         rz = rz1;
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
         for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
     }
     res[1] = rn;
@@ -196,6 +211,7 @@ static int bicgstab(const sys_t *s, const double *b, double *x, int max_it, doub
     double *dinv = (double *)calloc((size_t)n, sizeof(double));
     jacobi_inv(s, dinv);
     spmv(s, x, v);
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
     for (int i = 0; i < n; ++i) {
         r[i] = b[i] - v[i];
         r0[i] = r[i];
@@ -211,15 +227,20 @@ static int bicgstab(const sys_t *s, const double *b, double *x, int max_it, doub
         const double rho1 = dot(n, r0, r);
         const double beta = (rho1 / rho) * (alpha / omega);
         rho = rho1;
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
         for (int i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
         for (int i = 0; i < n; ++i) ph[i] = dinv[i] * p[i];
         spmv(s, ph, v);
         alpha = rho / dot(n, r0, v);
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
         for (int i = 0; i < n; ++i) sv[i] = r[i] - alpha * v[i];
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
         for (int i = 0; i < n; ++i) sh[i] = dinv[i] * sv[i];
         spmv(s, sh, t);
         const double tt = dot(n, t, t);
         omega = tt > 0 ? dot(n, t, sv) / tt : 0.0;
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
         for (int i = 0; i < n; ++i) {
             x[i] += alpha * ph[i] + omega * sh[i];
             r[i] = sv[i] - omega * t[i];

@@ -208,6 +208,11 @@ void or_scaled_residual(or_case *c, int N, double *out /*[N]*/);
  * segregatedFlowEquations::solve :76-297); ctl[0..3] = momentum solver max_it,
  * rtol, p' solver max_it, rtol; hist[0..4] = U,V,W,P control residuals,
  * hist[5..6] = momentum / p' solver iterations. */
+/* threads of the oracle's loops (OpenMP; default 1).  With one thread every
+ * loop runs in element / row order; with more, the loops that scatter into
+ * nodes run over colour classes of elements that share no node, so sums are
+ * formed in another order (results agree to rounding). */
+void or_set_threads(int n);
 void or_outer_iteration(or_case *c, const double *ctl, double *hist);
 
 /* ------------------------------------------------------- k-omega SST */

@@ -140,8 +140,14 @@ def lib():
         L.or_case_reversal.restype = _ip
         L.or_update_reversal.argtypes = [C.c_void_p]
         L.or_hex_side_nodes.restype = _ip
+        L.or_set_threads.argtypes = [C.c_int]
         _lib = L
     return _lib
+
+
+def set_threads(n: int):
+    """Threads of the oracle's loops (OpenMP colour classes); 1 by default."""
+    lib().or_set_threads(int(n))
 
 
 def default_params(**kw) -> Params:

@@ -169,6 +169,7 @@ void or_assemble_scalar(or_case *c, int which)
     memset(c->A1, 0, sizeof(double) * (size_t)c->nnz);
     memset(c->b1, 0, sizeof(double) * (size_t)nn);
     /* node terms first (phiAssembler::assemble_ :754-758) */
+#pragma omp parallel for num_threads(or_nthreads) schedule(static)
     for (int i = 0; i < nn; ++i) {
         double l, r;
         node_terms(c, which, i, &l, &r);
@@ -177,12 +178,17 @@ void or_assemble_scalar(or_case *c, int which)
     /* diffusivity Gamma = mu + sigma(F1) mut (computeGamma_ / setup lambdas) */
     or_tables tab[4];
     or_tables_init(tab);
-    double co[24], av[36], dndx[288], detj[12], lhs[64], rhs[8];
+    double lhs[64], rhs[8];
     /* high-resolution advection: limited deferred correction (blending
      * factor beta_k / beta_w, phiAssemblerElemTerms.hpp:291-340) */
     const double *beta = c->prm.high_res ? (which == 0 ? c->beta_k : c->beta_w) : NULL;
     const double *gphi = which == 0 ? c->gradK : c->gradW;
-    for (int e = 0; e < ((c->scalar_terms & 4) ? c->ne : 0); ++e) {
+    or_color_elements(c);
+    if (c->scalar_terms & 4)
+    OR_PARALLEL
+    {
+    double co[24], av[36], dndx[288], detj[12], lhs[64], rhs[8];
+    OR_FOR_COLORED(c, e) {
         const int t = or_etype(c, e);
         const or_tables *T = &tab[t];
         const int npe = T->npe, *lr = T->lr;
@@ -236,6 +242,7 @@ void or_assemble_scalar(or_case *c, int which)
             }
         }
         or_apply_coeff(1, npe, en, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A1, c->b1);
+    }
     }
     /* inlets (fixed value) and outlets (zero gradient); nothing at walls (k
      * zero-gradient, omega rows fixed below) or symmetry planes */
