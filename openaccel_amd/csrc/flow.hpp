@@ -97,6 +97,8 @@ struct Flow {
     DBuf<double> dinv, r, z, pk, q, w, t, s, r0, ph, sh;
     DBuf<double> V;                      // GMRES basis [(m+1)][n]
     DBuf<float> Af, fb, fx, fr, fz, fp, fq, fdinv;  // FP32 inner solve of the mixed form
+    DBuf<float> fV, fw;                  // FP32 GMRES basis and work vector
+    DBuf<float> fr0, fs, ft, fsh;        // FP32 AMG-BiCGStab vectors
     DBuf<double> part;                   // reduction partials [64][kRedGrid]
     DBuf<double> red;                    // reduced values [64]
     DBuf<double> scal;                   // device-resident Krylov scalars

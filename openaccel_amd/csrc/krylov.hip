@@ -64,25 +64,25 @@ __device__ __forceinline__ void sell_row_product(int i, const int* slice_ptr, co
 // slices on one XCD (its L2 holds the neighbours' x); the block values and
 // column indices, read once per product and larger than the caches, are
 // loaded non-temporally so that they do not evict x
-template <int N>
+template <int N, typename T = double>
 __global__ void __launch_bounds__(256) k_spmv(int nn, const int* slice_ptr, const int* slice_w,
-                                              const int* scols, const double* A, const double* x,
-                                              const double* s, double* y) {
+                                              const int* scols, const T* A, const T* x,
+                                              const T* s, T* y) {
     const int i = int(xcd_block() * blockDim.x + threadIdx.x);
     if (i >= nn) return;
     const int sl = i >> 6, ln = i & 63;
     const size_t s0 = size_t(slice_ptr[sl]);
     const int w = slice_w[sl];
     const int* cp = scols + s0 * 64 + ln;
-    const double* ap = A + s0 * (N * N * 64) + ln;
-    double acc[N];
-    for (int m = 0; m < N; ++m) acc[m] = 0.0;
+    const T* ap = A + s0 * (N * N * 64) + ln;
+    T acc[N];
+    for (int m = 0; m < N; ++m) acc[m] = T(0);
 #pragma unroll 4
     for (int q = 0; q < w; ++q) {
         const size_t c = size_t(N) * __builtin_nontemporal_load(cp + 64 * q);
-        double xv[N];
+        T xv[N];
         for (int r = 0; r < N; ++r) xv[r] = s ? s[c + r] * x[c + r] : x[c + r];
-        const double* blk = ap + size_t(q) * (N * N * 64);
+        const T* blk = ap + size_t(q) * (N * N * 64);
         for (int m = 0; m < N; ++m)
             for (int r = 0; r < N; ++r)
                 acc[m] += __builtin_nontemporal_load(blk + (N * m + r) * 64) * xv[r];
@@ -109,10 +109,12 @@ __device__ inline double reduce_partials(const double* part, int k, double* sh) 
     return block_sum(v, sh);
 }
 
-struct DotPairs {
-    const double* x[64];
-    const double* y[64];
+template <typename T>
+struct DotPairsT {
+    const T* x[64];
+    const T* y[64];
 };
+using DotPairs = DotPairsT<double>;
 
 __global__ void __launch_bounds__(kRedBlock) k_dots(size_t n, int np, DotPairs P, double* part) {
     __shared__ double sh[kRedBlock];
@@ -138,16 +140,17 @@ __global__ void __launch_bounds__(kRedBlock) k_reduce(int np, const double* part
 
 // x_k . y for up to 8 vectors x_k against one y (a Gram-Schmidt step): y is
 // read once per element for all of them; partials of pair k0 + k to slot k0 + k
-__global__ void __launch_bounds__(kRedBlock) k_dots_y(size_t n, int k0, int nk, DotPairs P,
-                                                      const double* y, double* part) {
+template <typename T = double>
+__global__ void __launch_bounds__(kRedBlock) k_dots_y(size_t n, int k0, int nk, DotPairsT<T> P,
+                                                      const T* y, double* part) {
     __shared__ double sh[kRedBlock];
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
          i += size_t(gridDim.x) * blockDim.x) {
-        const double yi = y[i];
+        const double yi = double(y[i]);
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            if (k < nk) v[k] += P.x[k0 + k][i] * yi;
+            if (k < nk) v[k] += double(P.x[k0 + k][i]) * yi;
     }
     for (int k = 0; k < nk; ++k) {
         const double s = block_sum(v[k], sh);
@@ -167,8 +170,8 @@ static void dots(Flow& f, size_t n, int np, const double* const* xs, const doubl
     }
     if (same_y) {
         for (int k0 = 0; k0 < np; k0 += 8)
-            k_dots_y<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, k0, std::min(8, np - k0), P, ys[0],
-                                                           f.part.p);
+            k_dots_y<double><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, k0, std::min(8, np - k0), P,
+                                                                   ys[0], f.part.p);
     } else {
         k_dots<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, np, P, f.part.p);
     }
@@ -180,19 +183,20 @@ static void dots(Flow& f, size_t n, int np, const double* const* xs, const doubl
 
 // ------------------------------------------------------------------ GMRES
 // w -= sum_k h[k] V_k
-__global__ void k_gs_update(size_t n, int nk, const double* V, const double* h, double* w) {
+template <typename T = double>
+__global__ void k_gs_update(size_t n, int nk, const T* V, const double* h, T* w) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s = 0.0;
-    for (int k = 0; k < nk; ++k) s += h[k] * V[size_t(k) * n + i];
-    w[i] -= s;
+    for (int k = 0; k < nk; ++k) s += h[k] * double(V[size_t(k) * n + i]);
+    w[i] = T(double(w[i]) - s);
 }
 
 // w -= sum_k h[k] V_k fused with the block partials of V_k . w for the
 // updated w (the second Gram-Schmidt pass), nk <= MAXK
-template <int MAXK>
-__global__ void __launch_bounds__(kRedBlock) k_gs_update_dots(size_t n, int nk, const double* V,
-                                                              const double* h, double* w,
+template <int MAXK, typename T = double>
+__global__ void __launch_bounds__(kRedBlock) k_gs_update_dots(size_t n, int nk, const T* V,
+                                                              const double* h, T* w,
                                                               double* part) {
     __shared__ double sh[kRedBlock];
     double acc[MAXK];
@@ -201,12 +205,13 @@ __global__ void __launch_bounds__(kRedBlock) k_gs_update_dots(size_t n, int nk, 
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
          i += size_t(gridDim.x) * blockDim.x) {
         double s = 0.0;
-        for (int k = 0; k < nk; ++k) s += h[k] * V[size_t(k) * n + i];
-        const double wi = w[i] - s;
-        w[i] = wi;
+        for (int k = 0; k < nk; ++k) s += h[k] * double(V[size_t(k) * n + i]);
+        const T wt = T(double(w[i]) - s);
+        w[i] = wt;
+        const double wi = double(wt);
 #pragma unroll
         for (int k = 0; k < MAXK; ++k)
-            if (k < nk) acc[k] += V[size_t(k) * n + i] * wi;
+            if (k < nk) acc[k] += double(V[size_t(k) * n + i]) * wi;
     }
     for (int k = 0; k < nk; ++k) {
         const double t = block_sum(acc[k], sh);
@@ -216,105 +221,119 @@ __global__ void __launch_bounds__(kRedBlock) k_gs_update_dots(size_t n, int nk, 
 
 // w -= sum_k h[k] V_k fused with the block partials of w . w (the norm that
 // follows the second Gram-Schmidt pass)
-__global__ void __launch_bounds__(kRedBlock) k_gs_update_norm(size_t n, int nk, const double* V,
-                                                              const double* h, double* w,
+template <typename T = double>
+__global__ void __launch_bounds__(kRedBlock) k_gs_update_norm(size_t n, int nk, const T* V,
+                                                              const double* h, T* w,
                                                               double* part) {
     __shared__ double sh[kRedBlock];
     double ww = 0.0;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
          i += size_t(gridDim.x) * blockDim.x) {
         double s = 0.0;
-        for (int k = 0; k < nk; ++k) s += h[k] * V[size_t(k) * n + i];
-        const double wi = w[i] - s;
-        w[i] = wi;
+        for (int k = 0; k < nk; ++k) s += h[k] * double(V[size_t(k) * n + i]);
+        const T wt = T(double(w[i]) - s);
+        w[i] = wt;
+        const double wi = double(wt);
         ww += wi * wi;
     }
     const double t = block_sum(ww, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-__global__ void k_scale_to(size_t n, double a, const double* x, double* y) {
+template <typename T = double>
+__global__ void k_scale_to(size_t n, double a, const T* x, T* y) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = a * x[i];
+    if (i < n) y[i] = T(a * double(x[i]));
 }
 
 // x += dinv .* (V y)
-__global__ void k_gmres_x(size_t n, int nk, const double* V, const double* y, const double* dinv,
-                          double* x) {
+template <typename T = double>
+__global__ void k_gmres_x(size_t n, int nk, const T* V, const double* y, const T* dinv, T* x) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s = 0.0;
-    for (int k = 0; k < nk; ++k) s += y[k] * V[size_t(k) * n + i];
-    x[i] += dinv[i] * s;
+    for (int k = 0; k < nk; ++k) s += y[k] * double(V[size_t(k) * n + i]);
+    x[i] = T(double(x[i]) + double(dinv[i]) * s);
 }
 
-__global__ void k_residual_copy(size_t n, const double* b, const double* Ax, double* r) {
+template <typename T = double>
+__global__ void k_residual_copy(size_t n, const T* b, const T* Ax, T* r) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) r[i] = b[i] - Ax[i];
 }
 
-template <int N>
-static void spmv(Flow& f, const double* A, const double* x, const double* s, double* y) {
+// np dot products of T vectors (accumulated in double) to the host
+template <typename T>
+static void dots_t(Flow& f, size_t n, int np, const T* const* xs, const T* y, double* out) {
+    DotPairsT<T> P;
+    for (int k = 0; k < np; ++k) P.x[k] = xs[k];
+    for (int k0 = 0; k0 < np; k0 += 8)
+        k_dots_y<T><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, k0, std::min(8, np - k0), P, y, f.part.p);
+    k_reduce<<<np, kRedBlock, 0, f.stream>>>(np, f.part.p, f.red.p);
+    OAGPU_HIP(hipGetLastError());
+    f.red.download(out, np, f.stream);
+    OAGPU_HIP(hipStreamSynchronize(f.stream));
+}
+
+template <int N, typename T = double>
+static void spmv(Flow& f, const T* A, const T* x, const std::type_identity_t<T>* s, T* y) {
     DevMesh& m = *f.m;
     clock_begin(f, N == 1 ? kClkSpmv1 : kClkSpmv3);
-    k_spmv<N><<<xcd_grid(size_t(m.nn), 256), 256, 0, f.stream>>>(m.nn, m.slice_ptr.p, m.slice_w.p,
-                                                                  m.sell_cols.p, A, x, s, y);
+    k_spmv<N, T><<<xcd_grid(size_t(m.nn), 256), 256, 0, f.stream>>>(m.nn, m.slice_ptr.p, m.slice_w.p,
+                                                                     m.sell_cols.p, A, x, s, y);
     clock_end(f, N == 1 ? kClkSpmv1 : kClkSpmv3);
     OAGPU_HIP(hipGetLastError());
 }
 
-template <int N>
-static int gmres(Flow& f, int max_it, double rtol, double atol, int mres, oagpu_control_data* cd) {
+// GMRES(m) right-preconditioned by Jacobi (x = D^-1 V y), restarted from the
+// true residual, on values and vectors of type T (double; float for the
+// inner solve of the mixed form), Gram-Schmidt coefficients and dot
+// products in double.  Starts from x = 0; returns the iterations, r0 / r1
+// the initial / final residual norms (the final one from b - A x).
+template <int N, typename T>
+static int gmres_core(Flow& f, const T* A, const T* b, const T* dinv, T* x, T* Vb, T* w, T* r,
+                      int max_it, double rtol, double atol, int mres, double* r0out,
+                      double* r1out) {
     DevMesh& m = *f.m;
     const size_t n = size_t(m.nn) * N;
-    const double* A = N == 3 ? f.A3.p : f.A1.p;
-    const double* b = N == 3 ? f.b3.p : f.b1.p;
-    double* x = N == 3 ? f.x3.p : f.x1.p;
     const unsigned B = 256, G = unsigned((n + B - 1) / B);
-    if (mres < 1) mres = 1;
-    if (mres > 60) mres = 60;
-    if (f.V_cols < mres + 1 || f.V.n < n * size_t(mres + 1)) {
-        f.V.alloc(n * size_t(mres + 1));
-        f.V_cols = mres + 1;
-    }
-    OAGPU_HIP(hipMemsetAsync(x, 0, n * sizeof(double), f.stream));
-    // x0 = 0: r0 = b
+    OAGPU_HIP(hipMemsetAsync(x, 0, n * sizeof(T), f.stream));
     double beta;
     {
-        const double* xs[1] = {b};
-        dots(f, n, 1, xs, xs, &beta);
+        const T* xs[1] = {b};
+        dots_t<T>(f, n, 1, xs, b, &beta);
         beta = std::sqrt(beta);
     }
     const double r0 = beta;
     const double tol = std::max(rtol * r0, atol);
-    cd->solver_initial_res = r0;
+    *r0out = r0;
     std::vector<double> H(size_t(mres + 1) * mres), cs(mres), sn(mres), g(mres + 1), y(mres),
         h(mres + 1), h2(mres + 1);
-    const double* rvec = b;
+    const T* rvec = b;
     int it = 0;
     while (it < max_it && beta > tol) {
-        k_scale_to<<<G, B, 0, f.stream>>>(n, 1.0 / beta, rvec, f.V.p);
+        k_scale_to<T><<<G, B, 0, f.stream>>>(n, 1.0 / beta, rvec, Vb);
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
         int jend = -1;
         for (int j = 0; j < mres && it < max_it; ++j) {
-            spmv<N>(f, A, f.V.p + size_t(j) * n, f.dinv.p, f.w.p);
+            spmv<N, T>(f, A, Vb + size_t(j) * n, dinv, w);
             double hn;
-            std::vector<const double*> xs(j + 1), ys(j + 1, f.w.p);
-            for (int k = 0; k <= j; ++k) xs[k] = f.V.p + size_t(k) * n;
+            std::vector<const T*> xs(j + 1);
+            for (int k = 0; k <= j; ++k) xs[k] = Vb + size_t(k) * n;
             // classical Gram-Schmidt, twice
             if (j + 1 <= 16) {  // device-resident step: one host synchronisation
-                DotPairs P;
+                DotPairsT<T> P;
                 for (int k = 0; k <= j; ++k) P.x[k] = xs[k];
                 for (int k0 = 0; k0 <= j; k0 += 8)
-                    k_dots_y<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, k0, std::min(8, j + 1 - k0), P,
-                                                                   f.w.p, f.part.p);
+                    k_dots_y<T><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, k0, std::min(8, j + 1 - k0),
+                                                                      P, w, f.part.p);
                 k_reduce<<<j + 1, kRedBlock, 0, f.stream>>>(j + 1, f.part.p, f.red.p);
-                k_gs_update_dots<16><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, j + 1, f.V.p, f.red.p,
-                                                                            f.w.p, f.part.p);
+                k_gs_update_dots<16, T><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, j + 1, Vb, f.red.p,
+                                                                               w, f.part.p);
                 k_reduce<<<j + 1, kRedBlock, 0, f.stream>>>(j + 1, f.part.p, f.red.p + 64);
-                k_gs_update_norm<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, j + 1, f.V.p, f.red.p + 64,
-                                                                       f.w.p, f.part.p);
+                k_gs_update_norm<T><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, j + 1, Vb, f.red.p + 64,
+                                                                          w, f.part.p);
                 k_reduce<<<1, kRedBlock, 0, f.stream>>>(1, f.part.p, f.red.p + 128);
                 OAGPU_HIP(hipGetLastError());
                 double hr[129];
@@ -323,13 +342,13 @@ static int gmres(Flow& f, int max_it, double rtol, double atol, int mres, oagpu_
                 for (int k = 0; k <= j; ++k) h[k] = hr[k] + hr[64 + k];
                 hn = std::sqrt(hr[128]);
             } else {
-                dots(f, n, j + 1, xs.data(), ys.data(), h.data());
+                dots_t<T>(f, n, j + 1, xs.data(), w, h.data());
                 f.red.upload(h.data(), j + 1, f.stream);
-                k_gs_update<<<G, B, 0, f.stream>>>(n, j + 1, f.V.p, f.red.p, f.w.p);
-                dots(f, n, j + 1, xs.data(), ys.data(), h2.data());
+                k_gs_update<T><<<G, B, 0, f.stream>>>(n, j + 1, Vb, f.red.p, w);
+                dots_t<T>(f, n, j + 1, xs.data(), w, h2.data());
                 f.red.upload(h2.data(), j + 1, f.stream);
-                k_gs_update_norm<<<kRedGrid, kRedBlock, 0, f.stream>>>(n, j + 1, f.V.p, f.red.p, f.w.p,
-                                                                       f.part.p);
+                k_gs_update_norm<T><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, j + 1, Vb, f.red.p, w,
+                                                                          f.part.p);
                 k_reduce<<<1, kRedBlock, 0, f.stream>>>(1, f.part.p, f.red.p);
                 OAGPU_HIP(hipGetLastError());
                 for (int k = 0; k <= j; ++k) h[k] += h2[k];
@@ -339,7 +358,7 @@ static int gmres(Flow& f, int max_it, double rtol, double atol, int mres, oagpu_
             }
             for (int k = 0; k <= j; ++k) H[size_t(mres) * k + j] = h[k];
             H[size_t(mres) * (j + 1) + j] = hn;
-            if (hn > 0) k_scale_to<<<G, B, 0, f.stream>>>(n, 1.0 / hn, f.w.p, f.V.p + size_t(j + 1) * n);
+            if (hn > 0) k_scale_to<T><<<G, B, 0, f.stream>>>(n, 1.0 / hn, w, Vb + size_t(j + 1) * n);
             for (int k = 0; k < j; ++k) {
                 const double a0 = H[size_t(mres) * k + j], b0 = H[size_t(mres) * (k + 1) + j];
                 H[size_t(mres) * k + j] = cs[k] * a0 + sn[k] * b0;
@@ -363,16 +382,34 @@ static int gmres(Flow& f, int max_it, double rtol, double atol, int mres, oagpu_
             y[k] = t / H[size_t(mres) * k + k];
         }
         f.red.upload(y.data(), jend + 1, f.stream);
-        k_gmres_x<<<G, B, 0, f.stream>>>(n, jend + 1, f.V.p, f.red.p, f.dinv.p, x);
-        spmv<N>(f, A, x, nullptr, f.w.p);
-        k_residual_copy<<<G, B, 0, f.stream>>>(n, b, f.w.p, f.r.p);
-        rvec = f.r.p;
-        const double* rs[1] = {f.r.p};
-        dots(f, n, 1, rs, rs, &beta);
+        k_gmres_x<T><<<G, B, 0, f.stream>>>(n, jend + 1, Vb, f.red.p, dinv, x);
+        spmv<N, T>(f, A, x, nullptr, w);
+        k_residual_copy<T><<<G, B, 0, f.stream>>>(n, b, w, r);
+        rvec = r;
+        const T* rs[1] = {r};
+        dots_t<T>(f, n, 1, rs, r, &beta);
         beta = std::sqrt(beta);
     }
-    cd->solver_final_res = beta;
+    *r1out = beta;
     return it;
+}
+
+static int gmres_restart_len(int mres) { return std::min(std::max(mres, 1), 60); }
+
+template <int N>
+static int gmres(Flow& f, int max_it, double rtol, double atol, int mres, oagpu_control_data* cd) {
+    DevMesh& m = *f.m;
+    const size_t n = size_t(m.nn) * N;
+    mres = gmres_restart_len(mres);
+    if (f.V_cols < mres + 1 || f.V.n < n * size_t(mres + 1)) {
+        f.V.alloc(n * size_t(mres + 1));
+        f.V_cols = mres + 1;
+    }
+    const double* A = N == 3 ? f.A3.p : f.A1.p;
+    const double* b = N == 3 ? f.b3.p : f.b1.p;
+    double* x = N == 3 ? f.x3.p : f.x1.p;
+    return gmres_core<N, double>(f, A, b, f.dinv.p, x, f.V.p, f.w.p, f.r.p, max_it, rtol, atol, mres,
+                                 &cd->solver_initial_res, &cd->solver_final_res);
 }
 
 // -------------------------------------------------------------------- CG
@@ -689,6 +726,57 @@ static int cg_mixed(Flow& f, int max_it, double rtol, double atol, oagpu_control
     return used;
 }
 
+// Mixed form of GMRES: FP32 GMRES(m) on A d = r / ||r|| (A rounded to FP32,
+// FP32 basis), x += ||r|| d and r = b - A x in FP64, until ||r|| <= max(rtol
+// ||b||, atol) or the iteration budget is spent.  The momentum and k / omega
+// systems are solved to 1e-1 / 1e-2 of their right-hand side, which one FP32
+// inner solve reaches; the FP64 residual confirms it.
+template <int N>
+static int gmres_mixed(Flow& f, int max_it, double rtol, double atol, int mres,
+                       oagpu_control_data* cd) {
+    DevMesh& m = *f.m;
+    const size_t n = size_t(m.nn) * N, nv = m.nsell * N * N;
+    mres = gmres_restart_len(mres);
+    const double* A = N == 3 ? f.A3.p : f.A1.p;
+    const double* b = N == 3 ? f.b3.p : f.b1.p;
+    double* x = N == 3 ? f.x3.p : f.x1.p;
+    if (f.Af.n < nv) f.Af.alloc(nv);
+    for (auto* v : {&f.fb, &f.fx, &f.fr, &f.fw, &f.fdinv})
+        if (v->n < n) v->alloc(n);
+    if (f.fV.n < n * size_t(mres + 1)) f.fV.alloc(n * size_t(mres + 1));
+    const unsigned B = 256;
+    k_to_float<<<unsigned((nv + B - 1) / B), B, 0, f.stream>>>(nv, 1.0, A, f.Af.p);
+    k_to_float<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, 1.0, f.dinv.p, f.fdinv.p);
+    OAGPU_HIP(hipMemsetAsync(x, 0, n * sizeof(double), f.stream));
+    OAGPU_HIP(hipMemcpyAsync(f.r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, f.stream));
+    double rn;
+    {
+        const double* xs[1] = {f.r.p};
+        dots(f, n, 1, xs, xs, &rn);
+        rn = std::sqrt(rn);
+    }
+    cd->solver_initial_res = rn;
+    const double tol = std::max(rtol * rn, atol);
+    int used = 0;
+    while (used < max_it && rn > tol) {
+        const double inner = std::max(tol / rn, 1.0e-5);
+        k_to_float<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, 1.0 / rn, f.r.p, f.fb.p);
+        double i0, i1;
+        const int it = gmres_core<N, float>(f, f.Af.p, f.fb.p, f.fdinv.p, f.fx.p, f.fV.p, f.fw.p,
+                                            f.fr.p, max_it - used, inner, 0.0, mres, &i0, &i1);
+        used += std::max(it, 1);
+        k_refine<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, rn, f.fx.p, x);
+        spmv<N>(f, A, x, nullptr, f.w.p);
+        k_residual_copy<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, b, f.w.p, f.r.p);
+        const double* xs[1] = {f.r.p};
+        dots(f, n, 1, xs, xs, &rn);
+        rn = std::sqrt(rn);
+        if (it == 0) break;
+    }
+    cd->solver_final_res = rn;
+    return used;
+}
+
 // --------------------------------------------------------------- BiCGStab
 __global__ void k_bicg_p(size_t n, double beta, double omega, const double* r, const double* v,
                          double* p) {
@@ -795,49 +883,136 @@ static int bicgstab(Flow& f, int max_it, double rtol, double atol, oagpu_control
 }
 
 // ------------------------------------------------------ AMG-BiCGStab
-__global__ void k_axpby_add(size_t n, double a, const double* x, double b, const double* y,
-                            double* out) {
+template <typename V>
+__global__ void k_axpby_add(size_t n, double a, const V* x, double b, const V* y, V* out) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) out[i] += a * x[i] + b * y[i];
+    if (i < n) out[i] = V(double(out[i]) + a * double(x[i]) + b * double(y[i]));
 }
 
-// z = M^-1 r, one V-cycle on the FP64 hierarchy (Af: the fine level's
-// values in FP32, or null); the V-cycle's r.z partials go to a spare slot
-static void amg_apply(Flow& f, const double* A, const double* dinv, const double* r, double* z,
-                      const float* Af) {
-    amg_vcycle<double>(f, A, dinv, r, z, f.part.p + 8 * kRedGrid, nullptr, true, false, nullptr,
-                       nullptr, Af);
-    OAGPU_HIP(hipGetLastError());
+template <typename V>
+__global__ void k_bicg_p_t(size_t n, double beta, double omega, const V* r, const V* v, V* p) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = V(double(r[i]) + beta * (double(p[i]) - omega * double(v[i])));
 }
 
-// BiCGStab (Saad Alg. 7.7) right-preconditioned by one AMG V-cycle: the
-// p' systems of flows with static-pressure outlets, or on distorted meshes,
-// are not symmetric, and AMG-CG stalls on them.  Krylov vectors, the
-// operator and the V-cycle's vectors are FP64 in both forms (with the whole
-// V-cycle on the FP32 hierarchy BiCGStab needed 15-40 % more iterations on
-// the backward-facing step, which cancelled the cheaper cycles); the mixed
-// form's V-cycle reads the fine level's values rounded to FP32 (the fine
-// level's two kernels move a third fewer bytes).
-static int bicgstab_amg(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
+template <typename V>
+__global__ void k_lin2_t(size_t n, double a, const V* x, double b, const V* y, V* out) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = V(a * double(x[i]) + b * double(y[i]));
+}
+
+template <typename V>
+struct BicgVecs {
+    V *r, *r0, *p, *v, *s, *t, *ph, *sh;  // ph = M^-1 p, sh = M^-1 s
+};
+
+// BiCGStab (Saad Alg. 7.7) right-preconditioned by one AMG V-cycle on
+// values and vectors of type V, scalars and dot products in double: the p'
+// systems of flows with static-pressure outlets, or on distorted meshes, are
+// not symmetric, and AMG-CG stalls on them.  Af (V = double only): the fine
+// level's values in FP32, read by the V-cycle's fine-level kernels.  Starts
+// from x = 0; a vanishing inner product with the shadow residual restarts
+// it from the current residual.
+template <typename V>
+static int bicgstab_amg_core(Flow& f, const V* A, const float* Af, const V* b, const V* dinv, V* x,
+                             const BicgVecs<V>& w, int max_it, double rtol, double atol,
+                             double* r0out, double* r1out) {
     DevMesh& m = *f.m;
     const size_t n = size_t(m.nn);
+    const unsigned B = 256, G = unsigned((n + B - 1) / B);
+    k_clear_flag<<<1, 1, 0, f.stream>>>(f.flag.p);  // the V-cycle kernels honour the CG stop flag
+    OAGPU_HIP(hipMemsetAsync(x, 0, n * sizeof(V), f.stream));
+    OAGPU_HIP(hipMemcpyAsync(w.r, b, n * sizeof(V), hipMemcpyDeviceToDevice, f.stream));
+    auto restart = [&](double* rho, double* alpha, double* omega, double* r0n, double rn) {
+        OAGPU_HIP(hipMemcpyAsync(w.r0, w.r, n * sizeof(V), hipMemcpyDeviceToDevice, f.stream));
+        OAGPU_HIP(hipMemsetAsync(w.p, 0, n * sizeof(V), f.stream));
+        OAGPU_HIP(hipMemsetAsync(w.v, 0, n * sizeof(V), f.stream));
+        *rho = *alpha = *omega = 1.0;
+        *r0n = rn;
+    };
+    auto dot = [&](const V* a, const V* c) {
+        double d;
+        const V* xs[1] = {a};
+        dots_t<V>(f, n, 1, xs, c, &d);
+        return d;
+    };
+    auto apply = [&](const V* r, V* z) {
+        amg_vcycle<V>(f, A, dinv, r, z, f.part.p + 8 * kRedGrid, nullptr, true, false, nullptr,
+                      nullptr, Af);
+        OAGPU_HIP(hipGetLastError());
+    };
+    double rn = std::sqrt(dot(w.r, w.r));
+    *r0out = rn;
+    const double tol = std::max(rtol * rn, atol);
+    double rho = 1, alpha = 1, omega = 1, r0n = rn;
+    restart(&rho, &alpha, &omega, &r0n, rn);
+    int it = 0;
+    while (it < max_it && rn > tol) {
+        const double rho1 = dot(w.r0, w.r);
+        if (bicg_breakdown(rho1, r0n, rn)) {  // r0 . r ~ 0: restart from r
+            restart(&rho, &alpha, &omega, &r0n, rn);
+            continue;
+        }
+        const double beta = (rho1 / rho) * (alpha / omega);
+        rho = rho1;
+        k_bicg_p_t<V><<<G, B, 0, f.stream>>>(n, beta, omega, w.r, w.v, w.p);
+        apply(w.p, w.ph);
+        spmv<1, V>(f, A, w.ph, nullptr, w.v);  // v = A M^-1 p
+        const double r0v = dot(w.r0, w.v);
+        if (bicg_breakdown(r0v, r0n, rn)) {  // r0 . v ~ 0: alpha undefined
+            restart(&rho, &alpha, &omega, &r0n, rn);
+            ++it;
+            continue;
+        }
+        alpha = rho / r0v;
+        k_lin2_t<V><<<G, B, 0, f.stream>>>(n, 1.0, w.r, -alpha, w.v, w.s);
+        apply(w.s, w.sh);
+        spmv<1, V>(f, A, w.sh, nullptr, w.t);  // t = A M^-1 s
+        double tt_ts[2];
+        {
+            const V* xs[1] = {w.t};
+            dots_t<V>(f, n, 1, xs, w.t, &tt_ts[0]);
+            const V* ys[1] = {w.s};
+            dots_t<V>(f, n, 1, ys, w.t, &tt_ts[1]);
+        }
+        omega = tt_ts[0] > 0 ? tt_ts[1] / tt_ts[0] : 0.0;
+        k_axpby_add<V><<<G, B, 0, f.stream>>>(n, alpha, w.ph, omega, w.sh, x);
+        k_lin2_t<V><<<G, B, 0, f.stream>>>(n, 1.0, w.s, -omega, w.t, w.r);
+        ++it;
+        rn = std::sqrt(dot(w.r, w.r));
+        if (omega == 0.0) break;
+    }
+    *r1out = rn;
+    return it;
+}
+
+// The mixed form's p' solve: FP32 AMG-BiCGStab on A d = r / ||r|| with the
+// FP32 hierarchy, x += ||r|| d and r = b - A x in FP64, until ||r|| <=
+// max(rtol ||b||, atol) or the iteration budget is spent (kMixedBicgInner =
+// false: FP64 BiCGStab whose V-cycle reads the fine level's values in FP32).
+constexpr bool kMixedBicgInner = true;
+
+static int bicgstab_amg_mixed(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
+    DevMesh& m = *f.m;
+    const size_t n = size_t(m.nn);
+    const unsigned B = 256;
     const double* A = f.A1.p;
     const double* b = f.b1.p;
     double* x = f.x1.p;
-    double *ph = f.z.p, *sh = f.w.p;   // M^-1 p, M^-1 s
-    const unsigned B = 256, G = unsigned((n + B - 1) / B);
-    const float* Af = nullptr;
-    if (f.prm.precision == OAGPU_MIXED) {
-        if (f.Af.n < m.nsell) f.Af.alloc(m.nsell);
-        k_to_float<<<unsigned((m.nsell + B - 1) / B), B, 0, f.stream>>>(m.nsell, 1.0, A, f.Af.p);
-        Af = f.Af.p;
+    if (f.Af.n < m.nsell) f.Af.alloc(m.nsell);
+    k_to_float<<<unsigned((m.nsell + B - 1) / B), B, 0, f.stream>>>(m.nsell, 1.0, A, f.Af.p);
+    if (!kMixedBicgInner) {
+        amg_setup<double>(f, A, f.dinv.p);
+        BicgVecs<double> w{f.r.p, f.r0.p, f.pk.p, f.q.p, f.s.p, f.t.p, f.z.p, f.w.p};
+        return bicgstab_amg_core<double>(f, A, f.Af.p, b, f.dinv.p, x, w, max_it, rtol, atol,
+                                         &cd->solver_initial_res, &cd->solver_final_res);
     }
-    k_clear_flag<<<1, 1, 0, f.stream>>>(f.flag.p);  // the V-cycle kernels honour the CG stop flag
+    for (auto* v : {&f.fb, &f.fx, &f.fr, &f.fz, &f.fp, &f.fq, &f.fdinv, &f.fr0, &f.fs, &f.ft, &f.fsh})
+        if (v->n < n) v->alloc(n);
+    k_to_float<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, 1.0, f.dinv.p, f.fdinv.p);
+    amg_setup<float>(f, f.Af.p, f.fdinv.p);
     OAGPU_HIP(hipMemsetAsync(x, 0, n * sizeof(double), f.stream));
     OAGPU_HIP(hipMemcpyAsync(f.r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, f.stream));
-    OAGPU_HIP(hipMemcpyAsync(f.r0.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, f.stream));
-    OAGPU_HIP(hipMemsetAsync(f.pk.p, 0, n * sizeof(double), f.stream));
-    OAGPU_HIP(hipMemsetAsync(f.q.p, 0, n * sizeof(double), f.stream));  // v
     double rn;
     {
         const double* xs[1] = {f.r.p};
@@ -846,62 +1021,35 @@ static int bicgstab_amg(Flow& f, int max_it, double rtol, double atol, oagpu_con
     }
     cd->solver_initial_res = rn;
     const double tol = std::max(rtol * rn, atol);
-    double rho = 1, alpha = 1, omega = 1, r0n = rn;
-    int it = 0;
-    while (it < max_it && rn > tol) {
-        double rho1;
-        {
-            const double* xs[1] = {f.r0.p};
-            const double* ys[1] = {f.r.p};
-            dots(f, n, 1, xs, ys, &rho1);
-        }
-        if (bicg_breakdown(rho1, r0n, rn)) {  // r0 . r ~ 0: restart from r
-            bicg_restart(f, n, &rho, &alpha, &omega, &r0n, rn);
-            continue;
-        }
-        const double beta = (rho1 / rho) * (alpha / omega);
-        rho = rho1;
-        k_bicg_p<<<G, B, 0, f.stream>>>(n, beta, omega, f.r.p, f.q.p, f.pk.p);
-        amg_apply(f, A, f.dinv.p, f.pk.p, ph, Af);
-        spmv<1>(f, A, ph, nullptr, f.q.p);  // v = A M^-1 p
-        double r0v;
-        {
-            const double* xs[1] = {f.r0.p};
-            const double* ys[1] = {f.q.p};
-            dots(f, n, 1, xs, ys, &r0v);
-        }
-        if (bicg_breakdown(r0v, r0n, rn)) {  // r0 . v ~ 0: alpha undefined
-            bicg_restart(f, n, &rho, &alpha, &omega, &r0n, rn);
-            ++it;
-            continue;
-        }
-        alpha = rho / r0v;
-        k_lin2<<<G, B, 0, f.stream>>>(n, 1.0, f.r.p, -alpha, f.q.p, f.s.p);
-        amg_apply(f, A, f.dinv.p, f.s.p, sh, Af);
-        spmv<1>(f, A, sh, nullptr, f.t.p);  // t = A M^-1 s
-        double tt_ts[2];
-        {
-            const double* xs[2] = {f.t.p, f.t.p};
-            const double* ys[2] = {f.t.p, f.s.p};
-            dots(f, n, 2, xs, ys, tt_ts);
-        }
-        omega = tt_ts[0] > 0 ? tt_ts[1] / tt_ts[0] : 0.0;
-        k_axpby_add<<<G, B, 0, f.stream>>>(n, alpha, ph, omega, sh, x);
-        k_lin2<<<G, B, 0, f.stream>>>(n, 1.0, f.s.p, -omega, f.t.p, f.r.p);
-        ++it;
+    BicgVecs<float> w{f.fr.p, f.fr0.p, f.fp.p, f.fq.p, f.fs.p, f.ft.p, f.fz.p, f.fsh.p};
+    int used = 0;
+    while (used < max_it && rn > tol) {
+        const double inner = std::max(tol / rn, 1.0e-4);
+        k_to_float<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, 1.0 / rn, f.r.p, f.fb.p);
+        double i0, i1;
+        const int it = bicgstab_amg_core<float>(f, f.Af.p, nullptr, f.fb.p, f.fdinv.p, f.fx.p, w,
+                                                max_it - used, inner, 0.0, &i0, &i1);
+        used += std::max(it, 1);
+        k_refine<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, rn, f.fx.p, x);
+        spmv<1>(f, A, x, nullptr, f.w.p);
+        k_residual_copy<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, b, f.w.p, f.r.p);
         const double* xs[1] = {f.r.p};
         dots(f, n, 1, xs, xs, &rn);
         rn = std::sqrt(rn);
-        if (omega == 0.0) break;
+        if (it == 0) break;
     }
     cd->solver_final_res = rn;
-    return it;
+    return used;
 }
 
 static int bicgstab_amg_solve(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
+    amg_build(f);
+    if (f.amg.lv.size() <= 1) return bicgstab<1>(f, max_it, rtol, atol, cd);  // tiny: Jacobi
+    if (f.prm.precision == OAGPU_MIXED) return bicgstab_amg_mixed(f, max_it, rtol, atol, cd);
     amg_setup<double>(f, f.A1.p, f.dinv.p);
-    if (f.amg.lv.size() > 1) return bicgstab_amg(f, max_it, rtol, atol, cd);
-    return bicgstab<1>(f, max_it, rtol, atol, cd);  // no coarse level (tiny system): Jacobi
+    BicgVecs<double> w{f.r.p, f.r0.p, f.pk.p, f.q.p, f.s.p, f.t.p, f.z.p, f.w.p};
+    return bicgstab_amg_core<double>(f, f.A1.p, nullptr, f.b1.p, f.dinv.p, f.x1.p, w, max_it, rtol,
+                                     atol, &cd->solver_initial_res, &cd->solver_final_res);
 }
 
 int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double atol, int restart,
@@ -917,9 +1065,14 @@ int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double ato
                                                                  m.diag_off.p, f.A1.p, f.dinv.p);
     OAGPU_HIP(hipGetLastError());
     int it = 0;
-    if (method == OAGPU_GMRES)
-        it = N == 3 ? gmres<3>(f, max_it, rtol, atol, restart, cd)
-                    : gmres<1>(f, max_it, rtol, atol, restart, cd);
+    if (method == OAGPU_GMRES) {
+        if (f.prm.precision == OAGPU_MIXED)
+            it = N == 3 ? gmres_mixed<3>(f, max_it, rtol, atol, restart, cd)
+                        : gmres_mixed<1>(f, max_it, rtol, atol, restart, cd);
+        else
+            it = N == 3 ? gmres<3>(f, max_it, rtol, atol, restart, cd)
+                        : gmres<1>(f, max_it, rtol, atol, restart, cd);
+    }
     else if (method == OAGPU_CG || method == OAGPU_CG_AMG) {
         const bool amg = method == OAGPU_CG_AMG;
         OAGPU_REQUIRE(!amg || N == 1, "AMG preconditioning is for scalar systems (N = 1)");
