@@ -309,7 +309,8 @@ int oagpu_sst_iteration(oagpu_flow *f, const double *ctl, double *hist);
  * "pcorr", "scalar" (k / omega rows), "mdot", "gradU", "gradP" (every scalar
  * gradient), "amg_pre" / "amg_post" (the fine level's pre-smoothing +
  * residual and prolongation + post-smoothing kernels), "amg_tail" (the
- * one-workgroup coarse levels), "spmv1" / "spmv3" (Krylov products), and
+ * one-workgroup coarse levels), "spmv1" / "spmv3" (Krylov products in
+ * FP64), "spmv1_f32" / "spmv3_f32" (FP32 inner solves of the mixed form), and
  * resets the totals.  Launches replayed from captured HIP graphs are not
  * clocked.  Nothing synchronises until oagpu_kernel_stats, which returns the
  * total device time (ms) and the launch count since the last oagpu_profile. */

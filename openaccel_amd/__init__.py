@@ -429,7 +429,7 @@ class FlowCase:
         return h
 
     CLOCKS = ("momentum", "pcorr", "scalar", "mdot", "gradU", "gradP", "amg_pre", "amg_post",
-              "amg_tail", "spmv1", "spmv3")
+              "amg_tail", "spmv1", "spmv3", "spmv1_f32", "spmv3_f32")
 
     def profile(self, which="all"):
         """Clock "all", "none" or one named kernel (HIP events); resets totals."""

@@ -278,10 +278,12 @@ static void dots_t(Flow& f, size_t n, int np, const T* const* xs, const T* y, do
 template <int N, typename T = double>
 static void spmv(Flow& f, const T* A, const T* x, const std::type_identity_t<T>* s, T* y) {
     DevMesh& m = *f.m;
-    clock_begin(f, N == 1 ? kClkSpmv1 : kClkSpmv3);
+    constexpr bool f32 = std::is_same_v<T, float>;
+    const int clk = N == 1 ? (f32 ? kClkSpmv1f : kClkSpmv1) : (f32 ? kClkSpmv3f : kClkSpmv3);
+    clock_begin(f, clk);
     k_spmv<N, T><<<xcd_grid(size_t(m.nn), 256), 256, 0, f.stream>>>(m.nn, m.slice_ptr.p, m.slice_w.p,
                                                                      m.sell_cols.p, A, x, s, y);
-    clock_end(f, N == 1 ? kClkSpmv1 : kClkSpmv3);
+    clock_end(f, clk);
     OAGPU_HIP(hipGetLastError());
 }
 
@@ -941,16 +943,25 @@ static int bicgstab_amg_core(Flow& f, const V* A, const float* Af, const V* b, c
                       nullptr, Af);
         OAGPU_HIP(hipGetLastError());
     };
+    // r . r and r0 . r in one pass over r (r0 = r at a start or restart)
+    auto norms = [&](double* rr, double* r0r) {
+        double d[2];
+        const V* xs[2] = {w.r, w.r0};
+        dots_t<V>(f, n, 2, xs, w.r, d);
+        *rr = d[0];
+        *r0r = d[1];
+    };
     double rn = std::sqrt(dot(w.r, w.r));
     *r0out = rn;
     const double tol = std::max(rtol * rn, atol);
     double rho = 1, alpha = 1, omega = 1, r0n = rn;
     restart(&rho, &alpha, &omega, &r0n, rn);
+    double rho1 = rn * rn;  // r0 . r
     int it = 0;
     while (it < max_it && rn > tol) {
-        const double rho1 = dot(w.r0, w.r);
         if (bicg_breakdown(rho1, r0n, rn)) {  // r0 . r ~ 0: restart from r
             restart(&rho, &alpha, &omega, &r0n, rn);
+            rho1 = rn * rn;
             continue;
         }
         const double beta = (rho1 / rho) * (alpha / omega);
@@ -961,6 +972,7 @@ static int bicgstab_amg_core(Flow& f, const V* A, const float* Af, const V* b, c
         const double r0v = dot(w.r0, w.v);
         if (bicg_breakdown(r0v, r0n, rn)) {  // r0 . v ~ 0: alpha undefined
             restart(&rho, &alpha, &omega, &r0n, rn);
+            rho1 = rn * rn;
             ++it;
             continue;
         }
@@ -970,16 +982,16 @@ static int bicgstab_amg_core(Flow& f, const V* A, const float* Af, const V* b, c
         spmv<1, V>(f, A, w.sh, nullptr, w.t);  // t = A M^-1 s
         double tt_ts[2];
         {
-            const V* xs[1] = {w.t};
-            dots_t<V>(f, n, 1, xs, w.t, &tt_ts[0]);
-            const V* ys[1] = {w.s};
-            dots_t<V>(f, n, 1, ys, w.t, &tt_ts[1]);
+            const V* xs[2] = {w.t, w.s};  // t . t and s . t in one pass over t
+            dots_t<V>(f, n, 2, xs, w.t, tt_ts);
         }
         omega = tt_ts[0] > 0 ? tt_ts[1] / tt_ts[0] : 0.0;
         k_axpby_add<V><<<G, B, 0, f.stream>>>(n, alpha, w.ph, omega, w.sh, x);
         k_lin2_t<V><<<G, B, 0, f.stream>>>(n, 1.0, w.s, -omega, w.t, w.r);
         ++it;
-        rn = std::sqrt(dot(w.r, w.r));
+        double rr;
+        norms(&rr, &rho1);
+        rn = std::sqrt(rr);
         if (omega == 0.0) break;
     }
     *r1out = rn;
