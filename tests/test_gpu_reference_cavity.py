@@ -123,3 +123,40 @@ def test_mixed_form_tracks_fp64(oa, mesh):
         f.initialize()
         hist[prec] = np.array([f.outer_iteration([20, 0.1, 400, 1e-8])[:4] for _ in range(8)])
     np.testing.assert_allclose(hist[1], hist[0], rtol=1e-3)
+
+
+def test_history_and_system_dump(oracle, oa, mesh, tmp_path):
+    """The device path's residual histories (momentum and p', the reference's
+    writeHistory_ format) and a write_system dump of the last momentum system,
+    read back: equal to the outer iterations' control residuals (6 digits) and
+    to the oracle's system assembled from the same state (1e-12)."""
+    from openaccel_amd import sysio
+    c, dm, f = pair(oracle, oa, mesh)
+    c.initialize()
+    f.initialize()
+    rdir = str(tmp_path / "0" / "residuals")
+    hu = sysio.ResidualHistory(rdir, "momentum equations", ["U-momentum", "V-momentum", "W-momentum"])
+    hp = sysio.ResidualHistory(rdir, "pressure correction", ["P-correction"])
+    hist = []
+    for it in range(1, 4):
+        h = f.outer_iteration(CTL)
+        c.outer_iteration(CTL)
+        hu.write(it, it, int(h[4]), 0.0, h[:3])
+        hp.write(it, it, int(h[5]), 0.0, h[3:4])
+        hist.append(h)
+    hist = np.array(hist)
+    for hf, cols in ((hu, slice(0, 3)), (hp, slice(3, 4))):
+        names, rows = sysio.read_history(hf.path)
+        np.testing.assert_allclose(rows[:, 5:], hist[:, cols], rtol=1e-6)
+        np.testing.assert_array_equal(rows[:, 0], [1, 2, 3])
+    for name in ("U", "p", "mdot", "gradU", "gradP", "div"):
+        f.upload(name, c.field(name))
+    c.assemble_momentum()
+    c.compute_du()
+    c.normal_relaxation()
+    f.assemble_momentum()
+    a_path, b_path = sysio.write_flow_system(f, 3, str(tmp_path / "momentum_equations"), 4)
+    rp, cols, vals, b = sysio.read_system(a_path, b_path)
+    assert np.array_equal(rp, c.row_ptr) and np.array_equal(cols, c.cols)
+    assert row_rel_err(vals, c.vals(3), c.row_ptr) < 1e-12
+    assert entry_rel_err(b, c.rhs(3)) < 1e-12
