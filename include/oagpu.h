@@ -280,6 +280,22 @@ int oagpu_batch_outer_iterations(oagpu_flow *const *flows, int n, const double *
                                  double tol, int threads, double *hist, int *done,
                                  double *seconds);
 
+/* ------------------------------------------------------------ transient */
+#define OAGPU_BDF1 1
+#define OAGPU_BDF2 2
+/* Start a physical time step of length dt (timestep loop of
+ * simulation::run with transient_scheme first / second order backward
+ * Euler): U, and k and omega with SST, move back one time level (fields
+ * "Uold" <- "U", "Uold2" <- "Uold", likewise "kold", "wold"), and the
+ * momentum and k / omega assemblies that follow add the BDF time terms
+ * (navierStokesAssemblerNodeTerms.cpp:151-400, phiAssemblerNodeTerms.hpp:
+ * 150-404) in place of the steady pseudo-time term: c0 rho V / dt on the
+ * diagonal, -(c0 phi + c1 phi_old + c2 phi_old2) rho V / dt in the rhs, with
+ * BDF1 {1, -1} and BDF2 {(1+2w)/(1+w), -(1+w), w^2/(1+w)}, w = dt / dt_prev
+ * (BDF1 coefficients on the first step), rho constant in time.  The outer
+ * iterations of the step then run as in the steady case. */
+int oagpu_time_step(oagpu_flow *f, int scheme, double dt);
+
 /* ---------------------------------------------------------- k-omega SST */
 /* allocate the SST fields: "k", "omega", "gradK", "gradW", "ymin", "mut",
  * "F1", "Pk", "Pk_wall", "kprev"; momentum then uses "mu" = mu + mu_t */

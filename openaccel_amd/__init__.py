@@ -35,6 +35,7 @@ OAGPU_STENCIL_FULL, OAGPU_STENCIL_REDUCED = 0, 1
 OAGPU_GMRES, OAGPU_CG, OAGPU_BICGSTAB, OAGPU_CG_AMG, OAGPU_BICGSTAB_AMG = 0, 1, 2, 3, 4
 OAGPU_BC_WALL, OAGPU_BC_INLET, OAGPU_BC_OUTLET, OAGPU_BC_SYMMETRY = 0, 1, 2, 3
 OAGPU_HEX8, OAGPU_TET4, OAGPU_WED6, OAGPU_PYR5 = 0, 1, 2, 3
+OAGPU_BDF1, OAGPU_BDF2 = 1, 2
 TOPOLOGY_SIZES = {OAGPU_HEX8: (8, 12), OAGPU_TET4: (4, 6), OAGPU_WED6: (6, 9), OAGPU_PYR5: (5, 12)}
 
 _dp = C.POINTER(C.c_double)
@@ -126,6 +127,7 @@ _SIGS = {
     "oagpu_amg_info": [C.c_void_p, _ip, _ip],
     "oagpu_amg_level_copy": [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp],
     "oagpu_outer_iteration": [C.c_void_p, _dp, _dp],
+    "oagpu_time_step": [C.c_void_p, C.c_int, C.c_double],
     "oagpu_profile": [C.c_void_p, C.c_char_p],
     "oagpu_kernel_stats": [C.c_void_p, C.c_char_p, _dp, C.POINTER(C.c_longlong)],
 }
@@ -275,7 +277,8 @@ class FlowCase:
                                                 C.byref(h)))
         self.h = h
 
-    _comp = {"U": 3, "p": 1, "gradU": 9, "gradP": 3, "du": 3, "div": 1, "mu": 1, "k": 1,
+    _comp = {"Uold": 3, "Uold2": 3, "kold": 1, "kold2": 1, "wold": 1, "wold2": 1,
+             "U": 3, "p": 1, "gradU": 9, "gradP": 3, "du": 3, "div": 1, "mu": 1, "k": 1,
              "omega": 1, "gradK": 3, "gradW": 3, "ymin": 1, "mut": 1, "F1": 1, "Pk": 1,
              "Pk_wall": 1, "kprev": 1, "beta": 3, "beta_k": 1, "beta_w": 1, "symA": 3}
 
@@ -421,6 +424,10 @@ class FlowCase:
         vals = np.zeros(int(rp[-1]))
         _check(load_library().oagpu_amg_level_copy(self.h, level, None, None, _i(cols), _d(vals)))
         return agg, rp, cols, vals
+
+    def time_step(self, scheme, dt):
+        """Start a physical time step: scheme OAGPU_BDF1 or OAGPU_BDF2."""
+        _check(load_library().oagpu_time_step(self.h, scheme, dt))
 
     def outer_iteration(self, ctl):
         c = np.ascontiguousarray(ctl, dtype=np.float64)

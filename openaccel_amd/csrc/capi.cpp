@@ -66,7 +66,10 @@ double* field_ptr(oagpu::Flow& f, const char* name, size_t* count) {
                {"F1", &f.F1, nn},            {"Pk", &f.Pk, nn},     {"Pk_wall", &f.Pk_wall, nn},
                {"kprev", &f.kprev, nn},      {"beta", &f.beta, nn * 3},
                {"beta_k", &f.beta_k, nn},    {"beta_w", &f.beta_w, nn},
-               {"symA", &f.symA, nn * 3},    {"msd", &f.msd, nbf * 4}};
+               {"symA", &f.symA, nn * 3},    {"msd", &f.msd, nbf * 4},
+               {"Uold", &f.Uold, nn * 3},    {"Uold2", &f.Uold2, nn * 3},
+               {"kold", &f.kold, nn},        {"kold2", &f.kold2, nn},
+               {"wold", &f.wold, nn},        {"wold2", &f.wold2, nn}};
     for (auto& t : tab)
         if (!std::strcmp(name, t.n)) {
             OAGPU_REQUIRE(t.b->p, std::string("field ") + name +
@@ -683,6 +686,38 @@ int oagpu_sst_iteration(oagpu_flow* h, const double* ctl, double* hist) {
             hist[2] = cdk.n_iterations;
             hist[3] = cdw.n_iterations;
         }
+    });
+}
+
+int oagpu_time_step(oagpu_flow* h, int scheme, double dt) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h, "null flow");
+        OAGPU_REQUIRE(scheme == OAGPU_BDF1 || scheme == OAGPU_BDF2, "scheme must be BDF1 or BDF2");
+        OAGPU_REQUIRE(dt > 0.0, "time step must be positive");
+        oagpu::Flow& f = h->f;
+        const size_t n3 = size_t(f.m->nn) * 3, n1 = size_t(f.m->nn);
+        auto shift = [&](oagpu::DBuf<double>& old2, oagpu::DBuf<double>& old,
+                         const oagpu::DBuf<double>& cur, size_t n) {
+            if (!old.p) {
+                old.alloc(n);
+                old2.alloc(n);
+                old.zero(f.stream);
+            }
+            OAGPU_HIP(hipMemcpyAsync(old2.p, old.p, n * sizeof(double), hipMemcpyDeviceToDevice,
+                                     f.stream));
+            OAGPU_HIP(hipMemcpyAsync(old.p, cur.p, n * sizeof(double), hipMemcpyDeviceToDevice,
+                                     f.stream));
+        };
+        shift(f.Uold2, f.Uold, f.U, n3);
+        if (f.sst_on) {
+            shift(f.kold2, f.kold, f.k, n1);
+            shift(f.wold2, f.wold, f.om, n1);
+        }
+        f.tdt_prev = f.tsteps > 0 ? f.tdt : 0.0;
+        f.tdt = dt;
+        f.tscheme = scheme;
+        ++f.tsteps;
+        OAGPU_HIP(hipStreamSynchronize(f.stream));
     });
 }
 

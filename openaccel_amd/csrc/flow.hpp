@@ -139,7 +139,27 @@ struct Flow {
     oagpu_sst_params sst{};
     DBuf<double> k, om, gradK, gradW, ymin, mut, F1, Pk, Pk_wall, kprev;  // om: omega
     double k_scale = 1.0, w_scale = 1.0;
+    // transient (oagpu_time_step): scheme 0 steady, 1 BDF1, 2 BDF2; the time
+    // step and the previous one (0 on the first step); U, k and omega of the
+    // previous two time levels
+    int tscheme = 0, tsteps = 0;
+    double tdt = 0.0, tdt_prev = 0.0;
+    DBuf<double> Uold, Uold2, kold, kold2, wold, wold2;
 };
+
+// BDF coefficients of the current time step (types.cpp:392-411: BDF1 {1, -1};
+// BDF2 with omega = dt / dt_prev; BDF1 on the first step of BDF2)
+inline void bdf_coeff(const Flow& f, double c[3]) {
+    c[0] = 1.0;
+    c[1] = -1.0;
+    c[2] = 0.0;
+    if (f.tscheme == 2 && f.tdt_prev > 0.0) {
+        const double om = f.tdt / f.tdt_prev;
+        c[0] = (1.0 + 2.0 * om) / (1.0 + om);
+        c[1] = -(1.0 + om);
+        c[2] = om * om / (1.0 + om);
+    }
+}
 
 // start / stop of a clocked launch on the flow's stream (no-ops unless the
 // flow's profiling covers the kernel); a pool of 4096 pairs is folded in

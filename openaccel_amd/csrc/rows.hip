@@ -441,6 +441,11 @@ struct MomArgs {
     const int* etype;
     double rho, mu, dt, urf_inv, bnd_relax;
     int ugrad_shifted, us;
+    // transient: velocity of the previous two time levels (null when steady)
+    // and the BDF coefficients times rho / dt (dt above is then dt / c0)
+    const double* Uold;
+    const double* Uold2;
+    double tc[3];
     double* vals;
     double* b;
     double* du;
@@ -819,9 +824,13 @@ __global__ void __launch_bounds__(LDS ? 64 : 256) k_momentum_rows(const MomArgs 
         for (int k = 0; k < 3; ++k) {
             double l = 0.0, r = 0.0;
             if (dv < 0) l += -dv;
-            r -= -dv * a.U[3 * size_t(row) + k];
+            const double Uk = a.U[3 * size_t(row) + k];
+            r -= -dv * Uk;
             r -= a.gradP[3 * size_t(row) + k] * vol;
             l += lhsfac;
+            if (a.Uold)  // BDF1 / BDF2 time derivative of the previous levels
+                r -= vol * (a.tc[0] * Uk + a.tc[1] * a.Uold[3 * size_t(row) + k] +
+                            a.tc[2] * a.Uold2[3 * size_t(row) + k]);
             D[4 * k] = l;
             R[k] = r;
         }
@@ -941,6 +950,16 @@ void launch_assemble_momentum(Flow& f) {
     a.rho = f.prm.rho;
     a.mu = f.prm.mu;
     a.dt = f.prm.dt;
+    a.Uold = a.Uold2 = nullptr;
+    a.tc[0] = a.tc[1] = a.tc[2] = 0.0;
+    if (f.tscheme) {
+        double c[3];
+        bdf_coeff(f, c);
+        a.dt = f.tdt / c[0];
+        for (int k = 0; k < 3; ++k) a.tc[k] = c[k] * f.prm.rho / f.tdt;
+        a.Uold = f.Uold.p;
+        a.Uold2 = f.Uold2.p;
+    }
     a.urf_inv = 1.0 / f.prm.urf_U;
     a.bnd_relax = f.prm.bnd_relax;
     a.ugrad_shifted = f.prm.ugrad_shifted;
@@ -1474,6 +1493,9 @@ struct ScArgs {
     const long long* geo_off;   // [ne] offset into geo
     const int* etype;
     double rho, mu, dt, urf_inv, sig1, sig2;
+    const double* phiold;   // transient: phi of the previous two time levels
+    const double* phiold2;
+    double tc[3];
     int which, us;
     double* vals;
     double* b;
@@ -1487,6 +1509,7 @@ __device__ void sst_node_terms(const ScArgs& a, int i, double& lhs, double& rhs)
     const double rho = a.rho, vol = a.vol[i], div = a.div[i], w = a.w[i];
     const double phi = a.phi[i];
     double l = rho * vol / a.dt, r = 0.0;
+    if (a.phiold) r -= vol * (a.tc[0] * phi + a.tc[1] * a.phiold[i] + a.tc[2] * a.phiold2[i]);
     if (div < 0) l += -div;
     r -= -div * phi;
     if (a.which == 0) {
@@ -1879,6 +1902,8 @@ void launch_assemble_scalar(Flow& f, int which) {
     a.k = f.k.p;
     a.w = f.om.p;
     a.kprev = f.kprev.p;
+    a.phiold = a.phiold2 = nullptr;
+    a.tc[0] = a.tc[1] = a.tc[2] = 0.0;
     a.gradU = f.gradU.p;
     a.gradK = f.gradK.p;
     a.gradW = f.gradW.p;
@@ -1907,6 +1932,15 @@ void launch_assemble_scalar(Flow& f, int which) {
     a.rho = f.prm.rho;
     a.mu = f.prm.mu;
     a.dt = f.prm.dt;
+    if (f.tscheme) {
+        OAGPU_REQUIRE(f.kold.p, "k-omega SST enabled after the first time step");
+        double c[3];
+        bdf_coeff(f, c);
+        a.dt = f.tdt / c[0];
+        for (int k = 0; k < 3; ++k) a.tc[k] = c[k] * f.prm.rho / f.tdt;
+        a.phiold = which == 0 ? f.kold.p : f.wold.p;
+        a.phiold2 = which == 0 ? f.kold2.p : f.wold2.p;
+    }
     a.urf_inv = 1.0 / (which == 0 ? f.sst.urf_k : f.sst.urf_w);
     a.sig1 = which == 0 ? sstc::kSigK1 : sstc::kSigW1;
     a.sig2 = which == 0 ? sstc::kSigK2 : sstc::kSigW2;

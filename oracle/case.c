@@ -202,7 +202,8 @@ void or_case_destroy(or_case *c)
                     c->bnode, c->wnode, c->symA, c->bc_type, c->bc_p, c->bc_k, c->bc_w,
                     c->msd, c->rfl,
                     c->k, c->w, c->gradK, c->gradW, c->ymin, c->mut, c->F1,
-                    c->Pk, c->Pk_wall, c->kprev, c->beta_k, c->beta_w};
+                    c->Pk, c->Pk_wall, c->kprev, c->beta_k, c->beta_w,
+                    c->Uold, c->Uold2, c->kold, c->kold2, c->wold, c->wold2};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
     free(c);
 }
@@ -268,6 +269,12 @@ const int *or_case_diag_off(const or_case *c) { return c->diag_off; }
 double *or_case_field(or_case *c, const char *name)
 {
     if (!strcmp(name, "U")) return c->U;
+    if (!strcmp(name, "Uold")) return c->Uold;
+    if (!strcmp(name, "Uold2")) return c->Uold2;
+    if (!strcmp(name, "kold")) return c->kold;
+    if (!strcmp(name, "kold2")) return c->kold2;
+    if (!strcmp(name, "wold")) return c->wold;
+    if (!strcmp(name, "wold2")) return c->wold2;
     if (!strcmp(name, "p")) return c->p;
     if (!strcmp(name, "gradU")) return c->gradU;
     if (!strcmp(name, "gradP")) return c->gradP;
@@ -544,18 +551,29 @@ void or_assemble_momentum(or_case *c)
     double lhs[576], rhs[24];
     int nodes[8];
 
-    /* node terms, steady (navierStokesAssemblerNodeTerms.cpp:16-149) */
+    /* node terms, steady (navierStokesAssemblerNodeTerms.cpp:16-149: the
+     * pseudo-time term rho V / dt on the diagonal) or transient (BDF1 :151-268,
+     * BDF2 :270-400: c0 rho V / dt on the diagonal, the time derivative of the
+     * previous levels in the rhs; rho constant in time) */
+    double cf[3];
+    or_bdf_coeff(c, cf);
 #pragma omp parallel for num_threads(or_nthreads) schedule(static)
     for (int i = 0; i < nn; ++i) {
         double l9[9] = {0}, r3[3] = {0};
         const double rho = c->rho[i], vol = c->vol[i], div = c->div[i];
-        const double lhsfac = rho * vol / P->dt;
+        const double dt = c->tscheme ? c->tdt : P->dt;
+        const double lhsfac = (c->tscheme ? cf[0] : 1.0) * rho * vol / dt;
+        const double f1 = cf[1] * rho * vol / dt, f2 = cf[2] * rho * vol / dt;
         for (int k = 0; k < 3; ++k) {
             if (div < 0) l9[4 * k] += -div;
             const double Ui = c->U[3 * i + k];
             r3[k] -= -div * Ui;
             r3[k] -= c->gradP[3 * i + k] * vol;
             l9[4 * k] += lhsfac;
+            if (c->tscheme) {
+                r3[k] -= lhsfac * Ui + f1 * c->Uold[3 * i + k];
+                if (c->Uold2) r3[k] -= f2 * c->Uold2[3 * i + k];
+            }
         }
         or_apply_coeff(3, 1, &i, l9, r3, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
     }
@@ -1078,6 +1096,38 @@ void or_case_initialize(or_case *c)
 }
 
 /* -------------------------------------------------------------- outer */
+/* Start a physical time step of length dt with scheme 1 (BDF1) or 2 (BDF2):
+ * the previous levels move back (old2 <- old <- current) for U and, with
+ * SST, k and omega (the reference's prevTime fields). */
+int or_time_step(or_case *c, int scheme, double dt)
+{
+    if ((scheme != 1 && scheme != 2) || !(dt > 0)) return 1;
+    const size_t n3 = (size_t)c->nn * 3, n1 = (size_t)c->nn;
+    if (!c->Uold) {
+        c->Uold = (double *)xcalloc(n3, sizeof(double));
+        c->Uold2 = (double *)xcalloc(n3, sizeof(double));
+        if (c->k) {
+            c->kold = (double *)xcalloc(n1, sizeof(double));
+            c->kold2 = (double *)xcalloc(n1, sizeof(double));
+            c->wold = (double *)xcalloc(n1, sizeof(double));
+            c->wold2 = (double *)xcalloc(n1, sizeof(double));
+        }
+    }
+    memcpy(c->Uold2, c->Uold, n3 * sizeof(double));
+    memcpy(c->Uold, c->U, n3 * sizeof(double));
+    if (c->kold) {
+        memcpy(c->kold2, c->kold, n1 * sizeof(double));
+        memcpy(c->kold, c->k, n1 * sizeof(double));
+        memcpy(c->wold2, c->wold, n1 * sizeof(double));
+        memcpy(c->wold, c->w, n1 * sizeof(double));
+    }
+    c->tdt_prev = c->tsteps > 0 ? c->tdt : 0.0;
+    c->tdt = dt;
+    c->tscheme = scheme;
+    ++c->tsteps;
+    return 0;
+}
+
 void or_outer_iteration(or_case *c, const double *ctl, double *hist)
 {
     const or_params *P = &c->prm;

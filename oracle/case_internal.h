@@ -39,12 +39,33 @@ struct or_case {
     double *k, *w, *gradK, *gradW, *ymin, *mut, *F1, *Pk, *Pk_wall, *kprev, *beta_k, *beta_w;
     double k_scale, w_scale;
     int scalar_terms;   /* or_set_scalar_terms mask (test hook), default all */
+    /* transient (or_time_step): scheme 0 steady, 1 BDF1, 2 BDF2; physical
+     * time step and the previous one (0 on the first step); fields of the
+     * previous two time levels */
+    int tscheme, tsteps;
+    double tdt, tdt_prev;
+    double *Uold, *Uold2, *kold, *kold2, *wold, *wold2;
     /* element colour classes (no two elements of a class share a node) for
      * the scattering loops when several threads run them; one class in
      * element order with one thread */
     int ncolor, color_threads;
     int *color_ptr, *color_elem;
 };
+
+/* BDF coefficients of the current time step (types.cpp:392-411: BDF1
+ * {1, -1}; BDF2 with omega = dt / dt_prev, BDF1 on the first step) */
+static inline void or_bdf_coeff(const or_case *c, double cf[3])
+{
+    cf[0] = 1.0;
+    cf[1] = -1.0;
+    cf[2] = 0.0;
+    if (c->tscheme == 2 && c->tdt_prev > 0.0) {
+        const double om = c->tdt / c->tdt_prev;
+        cf[0] = (1.0 + 2.0 * om) / (1.0 + om);
+        cf[1] = -(1.0 + om);
+        cf[2] = om * om / (1.0 + om);
+    }
+}
 
 /* threads of the oracle's loops (or_set_threads, default 1) */
 extern int or_nthreads;

@@ -214,6 +214,8 @@ void or_scaled_residual(or_case *c, int N, double *out /*[N]*/);
  * formed in another order (results agree to rounding). */
 void or_set_threads(int n);
 void or_outer_iteration(or_case *c, const double *ctl, double *hist);
+/* start a physical time step (scheme 1 BDF1, 2 BDF2; 0 on success) */
+int or_time_step(or_case *c, int scheme, double dt);
 
 /* ------------------------------------------------------- k-omega SST */
 /* Constants: shearStressTransportModel.h:21-37 (sigma_k1 0.85, sigma_k2 1.0,

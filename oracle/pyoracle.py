@@ -141,6 +141,8 @@ def lib():
         L.or_update_reversal.argtypes = [C.c_void_p]
         L.or_hex_side_nodes.restype = _ip
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_time_step.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.or_time_step.restype = C.c_int
         _lib = L
     return _lib
 
@@ -206,7 +208,8 @@ class Case:
     _sizes = {"U": 3, "p": 1, "gradU": 9, "gradP": 3, "du": 3, "div": 1, "vol": 1,
               "rho": 1, "mu": 1, "beta": 3, "symA": 3, "xyz": 3, "k": 1, "omega": 1, "gradK": 3,
               "gradW": 3, "ymin": 1, "mut": 1, "F1": 1, "Pk": 1, "Pk_wall": 1, "kprev": 1,
-              "beta_k": 1, "beta_w": 1}
+              "beta_k": 1, "beta_w": 1, "Uold": 3, "Uold2": 3, "kold": 1, "kold2": 1,
+              "wold": 1, "wold2": 1}
 
     def field(self, name):
         ptr = lib().or_case_field(self.h, name.encode())
@@ -322,6 +325,11 @@ class Case:
         p = np.zeros(1)
         lib().or_update_scales(self.h, dptr(u), dptr(p))
         return float(u[0]), float(p[0])
+
+    def time_step(self, scheme, dt):
+        """Start a physical time step: scheme 1 BDF1, 2 BDF2."""
+        if lib().or_time_step(self.h, scheme, dt) != 0:
+            raise ValueError("bad time step")
 
     def outer_iteration(self, ctl):
         c = np.ascontiguousarray(ctl, dtype=np.float64)

@@ -121,7 +121,17 @@ static void node_terms(or_case *c, int which, int i, double *lhs, double *rhs)
     const double phi = which == 0 ? c->k[i] : w;
     double l = 0.0, r = 0.0;
     if (c->scalar_terms & 1) {
-        l += rho * vol / c->prm.dt;
+        if (c->tscheme) {  /* phiAssemblerNodeTerms.hpp:150-404 (BDF1 / BDF2) */
+            double cf[3];
+            or_bdf_coeff(c, cf);
+            const double f0 = cf[0] * rho * vol / c->tdt;
+            const double *old = which == 0 ? c->kold : c->wold;
+            const double *old2 = which == 0 ? c->kold2 : c->wold2;
+            l += f0;
+            r -= f0 * phi + cf[1] * rho * vol / c->tdt * old[i] + cf[2] * rho * vol / c->tdt * old2[i];
+        } else {
+            l += rho * vol / c->prm.dt;
+        }
         if (div < 0) l += -div;
         r -= -div * phi;
     }
