@@ -85,11 +85,13 @@ def measured_traffic(kernel, workload, precision):
 
 
 # ----------------------------------------------------------------- driver
-def run_form(oa, f, step, steps, warmup, dominant=None):
-    """Warm-up with every kernel clocked, then `steps` timed steps clocking
-    only `dominant` (or the warm-up's largest device time per step).
-    Returns wall seconds, step outputs, per-kernel warm-up ms/step and the
-    dominant kernel's (launches, total ms) over the timed steps."""
+def run_form(oa, f, step, steps, warmup, dominant=None, tune=True):
+    """Autotune the flow's launches (set-up, untimed), warm up with every
+    kernel clocked, then `steps` timed steps clocking only `dominant` (or the
+    warm-up's largest device time per step).  Returns wall seconds, step
+    outputs, per-kernel warm-up ms/step, the dominant kernel's (launches,
+    total ms) over the timed steps and the tuner's choices."""
+    tuning = {k: [float(x) for x in v] for k, v in f.autotune().items()} if tune else None
     f.profile("all")
     hist = [step() for _ in range(warmup)]
     warm = {}
@@ -109,7 +111,7 @@ def run_form(oa, f, step, steps, warmup, dominant=None):
     ms, n = f.kernel_stats(dominant)
     f.profile("none")
     return {"seconds": t, "hist": np.array(hist), "warm": warm, "dominant": dominant,
-            "dom_ms": ms, "dom_launches": n}
+            "dom_ms": ms, "dom_launches": n, "tuning": tuning}
 
 
 def kernel_table(r, ab, ne):
@@ -319,6 +321,7 @@ def bench_bfs(args):
                    "dims": dims, "elements": ne, "nodes": nn, "nnz": nnz},
         "roofline": roofline(rmx, ab_mx, "bfs", 1),
         "kernels": kernel_table(rmx, ab_mx, ne),
+        "autotune": {"mixed": rmx["tuning"], "fp64": r64["tuning"]},
         "fp64": {"value": round(it64, 4), "ms_per_step": round(1e3 * r64["seconds"] / args.steps, 3),
                  "roofline": roofline(r64, ab_64, "bfs", 0), "kernels": kernel_table(r64, ab_64, ne),
                  **hist(r64)},
@@ -414,6 +417,7 @@ def bench_cavity(args):
                    "n": args.n, "elements": ne, "nodes": nn, "nnz": nnz},
         "roofline": roofline(rmx, ab_mx, f"{wl}_{args.n}", 1),
         "kernels": kernel_table(rmx, ab_mx, ne),
+        "autotune": {"mixed": rmx["tuning"], "fp64": r64["tuning"]},
         "fp64": {"value": round(it64, 4), "ms_per_step": round(1e3 * r64["seconds"] / args.steps, 3),
                  "roofline": roofline(r64, ab_64, f"{wl}_{args.n}", 0),
                  "kernels": kernel_table(r64, ab_64, ne),

@@ -318,6 +318,24 @@ int oagpu_correct_scalar(oagpu_flow *f, int which);
  * hist[4] = k, omega control residuals, k, omega iterations */
 int oagpu_sst_iteration(oagpu_flow *f, const double *ctl, double *hist);
 
+/* Autotuner of the flow's tunable launches (no counterpart in the
+ * reference): times the candidate variants on the flow's current mesh and
+ * state and keeps the fastest for the flow's later launches -- momentum rows
+ * with their off-diagonal blocks summed in an LDS tile (64 rows per
+ * workgroup) or accumulated in HBM (256 rows per workgroup; hex-only meshes,
+ * upwind), p' rows at most 2 waves per SIMD or at the compiler's occupancy,
+ * and the Krylov SpMV at 64 / 128 / 256 rows per workgroup (FP64, and FP32
+ * for the mixed form).  Assembles the momentum and p' systems of the current
+ * state as a side effect.  report (n_report >= 18 doubles, NULL: none):
+ * per tunable -- momentum, pcorr, spmv1, spmv3, spmv1_f32, spmv3_f32 -- the
+ * chosen variant (1 tile / 0 HBM; 0 limited / 1 compiler; rows per
+ * workgroup), its time and the default variant's time in ms. */
+int oagpu_autotune(oagpu_flow *f, double *report, int n_report);
+/* Set one tunable's variant directly (e.g. from an earlier oagpu_autotune
+ * report): which = "momentum" (1 / 0), "pcorr" (0 / 1), "spmv1", "spmv3",
+ * "spmv1_f32", "spmv3_f32" (64 / 128 / 256). */
+int oagpu_tune_set(oagpu_flow *f, const char *which, int variant);
+
 /* Device time of the library's main kernels, measured with HIP events on the
  * flow's stream (no counterpart in the reference, whose only instruments are
  * the per-iteration CPU time and a profiler region around each solve).

@@ -129,6 +129,8 @@ _SIGS = {
     "oagpu_outer_iteration": [C.c_void_p, _dp, _dp],
     "oagpu_time_step": [C.c_void_p, C.c_int, C.c_double],
     "oagpu_profile": [C.c_void_p, C.c_char_p],
+    "oagpu_autotune": [C.c_void_p, _dp, C.c_int],
+    "oagpu_tune_set": [C.c_void_p, C.c_char_p, C.c_int],
     "oagpu_kernel_stats": [C.c_void_p, C.c_char_p, _dp, C.POINTER(C.c_longlong)],
 }
 
@@ -437,6 +439,18 @@ class FlowCase:
 
     CLOCKS = ("momentum", "pcorr", "scalar", "mdot", "gradU", "gradP", "amg_pre", "amg_post",
               "amg_tail", "spmv1", "spmv3", "spmv1_f32", "spmv3_f32")
+
+    TUNABLES = ("momentum", "pcorr", "spmv1", "spmv3", "spmv1_f32", "spmv3_f32")
+
+    def autotune(self):
+        """Time the tunable launches' variants on this flow and keep the
+        fastest; {kernel: (variant, ms, default ms)}."""
+        r = np.zeros(18)
+        _check(load_library().oagpu_autotune(self.h, _d(r), r.size))
+        return {k: (r[3 * i], r[3 * i + 1], r[3 * i + 2]) for i, k in enumerate(self.TUNABLES)}
+
+    def tune_set(self, which, variant):
+        _check(load_library().oagpu_tune_set(self.h, which.encode(), int(variant)))
 
     def profile(self, which="all"):
         """Clock "all", "none" or one named kernel (HIP events); resets totals."""

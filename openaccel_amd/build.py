@@ -15,7 +15,8 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "liboagpu.so")
 OBJDIR = os.path.join(ROOT, "_build")
 
-SOURCES = ["mesh.cpp", "fields.hip", "rows.hip", "bc.hip", "krylov.hip", "amg.hip", "sst.hip", "topo.hip", "capi.cpp"]
+SOURCES = ["mesh.cpp", "fields.hip", "rows.hip", "bc.hip", "krylov.hip", "amg.hip", "sst.hip", "topo.hip",
+           "tune.hip", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -872,6 +872,36 @@ int oagpu_batch_outer_iterations(oagpu_flow* const* flows, int n, const double* 
     });
 }
 
+int oagpu_autotune(oagpu_flow* h, double* report, int n_report) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h, "null flow");
+        OAGPU_REQUIRE(n_report >= 0 && (n_report == 0 || report), "bad report buffer");
+        oagpu::autotune(h->f, report, n_report);
+        check_err(h->f);
+    });
+}
+
+int oagpu_tune_set(oagpu_flow* h, const char* which, int variant) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && which, "null argument");
+        auto& t = h->f.tune;
+        const bool blk = variant == 64 || variant == 128 || variant == 256;
+        if (!std::strcmp(which, "momentum")) {
+            OAGPU_REQUIRE(variant == 0 || variant == 1, "momentum variant: 0 or 1");
+            t.mom_tile = variant;
+        } else if (!std::strcmp(which, "pcorr")) {
+            OAGPU_REQUIRE(variant == 0 || variant == 1, "pcorr variant: 0 or 1");
+            t.pc_occupancy = variant;
+        } else if (!std::strcmp(which, "spmv1") || !std::strcmp(which, "spmv3") ||
+                   !std::strcmp(which, "spmv1_f32") || !std::strcmp(which, "spmv3_f32")) {
+            OAGPU_REQUIRE(blk, "SpMV variant: 64, 128 or 256 rows per workgroup");
+            t.spmv_block[which[4] == '3'][std::strstr(which, "f32") != nullptr] = unsigned(variant);
+        } else {
+            throw oagpu::Error(OAGPU_EINVAL, std::string("unknown tunable ") + which);
+        }
+    });
+}
+
 int oagpu_profile(oagpu_flow* h, const char* which) {
     return guarded([&] {
         OAGPU_REQUIRE(h && which, "null argument");

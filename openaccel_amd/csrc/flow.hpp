@@ -107,6 +107,13 @@ struct Flow {
     int V_cols = 0;
     double U_scale = 1.0, p_scale = 1.0;
     hipStream_t stream = nullptr;
+    // launch variants chosen by oagpu_autotune (defaults: the hand-picked ones)
+    struct Tuning {
+        int mom_tile = 1;        // momentum rows: 1 LDS tile, 0 accumulate in HBM
+        int pc_occupancy = 0;    // p' rows: 0 at most 2 waves per SIMD, 1 compiler's choice
+        unsigned spmv_block[2][2] = {{256, 256}, {256, 256}};  // [N == 3][FP32]
+        bool tuned = false;
+    } tune;
     // device time of the clocked kernels (profiling on only)
     unsigned prof_mask = 0;
     KernelClock clk[kClkCount];
@@ -226,6 +233,13 @@ int element_geometry_noalloc(int topo, int shifted, int n, const double* coords,
 // device address of the current GPU's scratch array (hipGetSymbolAddress,
 // looked up once per device and cached)
 int* scratch_device_ptr();
+// autotuning (tune.hip): time the candidates of each tunable launch on the
+// flow's current state and keep the fastest; report: per tunable kernel
+// (momentum, pcorr, spmv1, spmv3, spmv1_f32, spmv3_f32) the chosen variant,
+// its time and the default's time, ms
+void autotune(Flow& f, double* report, int n_report);
+void time_spmv_variants(Flow& f, int N, bool f32, double* best_ms, double* default_ms);
+
 // amg.hip
 void amg_build(Flow& f);                       // hierarchy structure (once per mesh)
 template <typename V>

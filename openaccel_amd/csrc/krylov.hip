@@ -280,9 +280,10 @@ static void spmv(Flow& f, const T* A, const T* x, const std::type_identity_t<T>*
     DevMesh& m = *f.m;
     constexpr bool f32 = std::is_same_v<T, float>;
     const int clk = N == 1 ? (f32 ? kClkSpmv1f : kClkSpmv1) : (f32 ? kClkSpmv3f : kClkSpmv3);
+    const unsigned B = f.tune.spmv_block[N == 3][f32];
     clock_begin(f, clk);
-    k_spmv<N, T><<<xcd_grid(size_t(m.nn), 256), 256, 0, f.stream>>>(m.nn, m.slice_ptr.p, m.slice_w.p,
-                                                                     m.sell_cols.p, A, x, s, y);
+    k_spmv<N, T><<<xcd_grid(size_t(m.nn), B), B, 0, f.stream>>>(m.nn, m.slice_ptr.p, m.slice_w.p,
+                                                                 m.sell_cols.p, A, x, s, y);
     clock_end(f, clk);
     OAGPU_HIP(hipGetLastError());
 }
@@ -1062,6 +1063,58 @@ static int bicgstab_amg_solve(Flow& f, int max_it, double rtol, double atol, oag
     BicgVecs<double> w{f.r.p, f.r0.p, f.pk.p, f.q.p, f.s.p, f.t.p, f.z.p, f.w.p};
     return bicgstab_amg_core<double>(f, f.A1.p, nullptr, f.b1.p, f.dinv.p, f.x1.p, w, max_it, rtol,
                                      atol, &cd->solver_initial_res, &cd->solver_final_res);
+}
+
+// SpMV block-size candidates (64, 128, 256 rows per workgroup) timed on the
+// flow's system N (FP32: its values rounded); the fastest is kept in
+// f.tune.spmv_block.  y = A x into the Krylov work vector.
+void time_spmv_variants(Flow& f, int N, bool f32, double* best_ms, double* default_ms) {
+    DevMesh& m = *f.m;
+    const size_t n = size_t(m.nn) * N, nv = m.nsell * N * N;
+    const double* A = N == 3 ? f.A3.p : f.A1.p;
+    const double* x = N == 3 ? f.x3.p : f.x1.p;
+    const unsigned Bc = 256;
+    if (f32) {
+        if (f.Af.n < nv) f.Af.alloc(nv);
+        for (auto* v : {&f.fx, &f.fw})
+            if (v->n < n) v->alloc(n);
+        k_to_float<<<unsigned((nv + Bc - 1) / Bc), Bc, 0, f.stream>>>(nv, 1.0, A, f.Af.p);
+        k_to_float<<<unsigned((n + Bc - 1) / Bc), Bc, 0, f.stream>>>(n, 1.0, x, f.fx.p);
+    }
+    hipEvent_t e0, e1;
+    OAGPU_HIP(hipEventCreate(&e0));
+    OAGPU_HIP(hipEventCreate(&e1));
+    const unsigned cand[3] = {64, 128, 256};
+    unsigned best = 256;
+    *best_ms = 1e30;
+    for (unsigned B : cand) {
+        f.tune.spmv_block[N == 3][f32] = B;
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {  // first pass warms the caches
+            OAGPU_HIP(hipEventRecord(e0, f.stream));
+            for (int k = 0; k < 5; ++k) {
+                if (f32) {
+                    if (N == 3) spmv<3, float>(f, f.Af.p, f.fx.p, nullptr, f.fw.p);
+                    else spmv<1, float>(f, f.Af.p, f.fx.p, nullptr, f.fw.p);
+                } else {
+                    if (N == 3) spmv<3>(f, A, x, nullptr, f.w.p);
+                    else spmv<1>(f, A, x, nullptr, f.w.p);
+                }
+            }
+            OAGPU_HIP(hipEventRecord(e1, f.stream));
+            OAGPU_HIP(hipEventSynchronize(e1));
+            OAGPU_HIP(hipEventElapsedTime(&ms, e0, e1));
+        }
+        const double per = double(ms) / 5.0;
+        if (B == 256) *default_ms = per;
+        if (per < *best_ms) {
+            *best_ms = per;
+            best = B;
+        }
+    }
+    f.tune.spmv_block[N == 3][f32] = best;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
 }
 
 int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double atol, int restart,

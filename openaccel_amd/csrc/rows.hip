@@ -992,6 +992,16 @@ void launch_assemble_momentum(Flow& f) {
         clock_end(f, kClkMomentum);
     };
     const bool hr = a.beta != nullptr;
+    // the off-diagonal blocks accumulated in HBM instead of the LDS tile (an
+    // autotuning candidate; hex-only meshes, upwind advection)
+    if (f.tune.mom_tile == 0 && !hr && !m.mixed) {
+        if (a.ugrad_shifted) go(mixed ? k_momentum_rows<float, 1, 0, false, false>
+                                      : k_momentum_rows<double, 1, 0, false, false>, 256);
+        else go(mixed ? k_momentum_rows<float, 0, 0, false, false>
+                      : k_momentum_rows<double, 0, 0, false, false>, 256);
+        OAGPU_HIP(hipGetLastError());
+        return;
+    }
     auto pick = [&](auto mx) {
         constexpr bool MX = decltype(mx)::value;
         if (a.ugrad_shifted) {
@@ -1324,9 +1334,7 @@ __global__ void __launch_bounds__(256) k_pcorr_boundary(const PcArgs a, int nrow
 // their gathered nodal data in the XCD's L2 (FP32 form 1.68 -> 1.59 ms on the
 // 128^3 cavity against the compiler's 3 waves).
 template <typename T, int SP, bool MX>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) k_pcorr_rows(const PcArgs a) {
-    __shared__ double tile[kLdsSlots * 64];
-    double* lt = tile + threadIdx.x;
+__device__ __forceinline__ void pcorr_row(const PcArgs& a, double* lt) {
     const int row = ordered_item(a.slice_order, a.nslice, a.nn);
     if (row < 0) return;
     const SellRow sr = sell_row<1>(a.slice_ptr, a.slice_w, row);
@@ -1367,6 +1375,19 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
     *sell_val<1>(a.vals, sr, dof, 0) = D;
     a.b[row] = R;
     if (bad) *a.err = 1;
+}
+
+template <typename T, int SP, bool MX>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) k_pcorr_rows(const PcArgs a) {
+    __shared__ double tile[kLdsSlots * 64];
+    pcorr_row<T, SP, MX>(a, tile + threadIdx.x);
+}
+
+// the same with the compiler's occupancy (an autotuning candidate)
+template <typename T, int SP, bool MX>
+__global__ void __launch_bounds__(64) k_pcorr_rows_occ(const PcArgs a) {
+    __shared__ double tile[kLdsSlots * 64];
+    pcorr_row<T, SP, MX>(a, tile + threadIdx.x);
 }
 
 void launch_assemble_pcorr(Flow& f) {
@@ -1414,14 +1435,15 @@ void launch_assemble_pcorr(Flow& f) {
     a.err = f.err.p;
     const unsigned B = 64, G = xcd_grid(m.nn, B);
     const bool mixed = f.prm.precision == OAGPU_MIXED;
+    const bool occ = f.tune.pc_occupancy != 0;
     auto pick = [&](auto mx) {
         constexpr bool MX = decltype(mx)::value;
         if (f.prm.pgrad_shifted) {
-            if (mixed) k_pcorr_rows<float, 1, MX><<<G, B, 0, f.stream>>>(a);
-            else k_pcorr_rows<double, 1, MX><<<G, B, 0, f.stream>>>(a);
+            if (mixed) (occ ? k_pcorr_rows_occ<float, 1, MX> : k_pcorr_rows<float, 1, MX>)<<<G, B, 0, f.stream>>>(a);
+            else (occ ? k_pcorr_rows_occ<double, 1, MX> : k_pcorr_rows<double, 1, MX>)<<<G, B, 0, f.stream>>>(a);
         } else {
-            if (mixed) k_pcorr_rows<float, 0, MX><<<G, B, 0, f.stream>>>(a);
-            else k_pcorr_rows<double, 0, MX><<<G, B, 0, f.stream>>>(a);
+            if (mixed) (occ ? k_pcorr_rows_occ<float, 0, MX> : k_pcorr_rows<float, 0, MX>)<<<G, B, 0, f.stream>>>(a);
+            else (occ ? k_pcorr_rows_occ<double, 0, MX> : k_pcorr_rows<double, 0, MX>)<<<G, B, 0, f.stream>>>(a);
         }
     };
     clock_begin(f, kClkPcorr);
