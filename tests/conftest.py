@@ -57,11 +57,14 @@ def block_rel_err(a, b, k):
     return float((np.abs(a - b).max(axis=1) / scale).max())
 
 
-def rhs_rel_err(b, b_ref, vals_ref, row_ptr, cols, x):
+def rhs_rel_err(b, b_ref, vals_ref, row_ptr, cols, x, block=False):
     """Error of a block-CRS system's rhs relative, row by row, to the size of
     the terms it sums: sum_j |A_ij| |x_j| + |b_i| (x: the field the system is
     assembled about).  Near convergence b is a small difference of large
-    fluxes; this holds each row to the fluxes' scale."""
+    fluxes; this holds each row to the fluxes' scale.  block: a node's
+    components share the largest of their scales (a velocity component that
+    vanishes, as the spanwise one of a two-dimensional flow, still sums
+    cross-component viscous terms of the others' size)."""
     vals_ref = np.asarray(vals_ref, dtype=np.float64)
     nnz = vals_ref.shape[0]
     N = int(round(np.sqrt(vals_ref.size // nnz)))
@@ -73,6 +76,8 @@ def rhs_rel_err(b, b_ref, vals_ref, row_ptr, cols, x):
     scale = np.add.reduceat(terms, rp[:-1], axis=0)                   # [nn][N]
     br = np.asarray(b_ref, dtype=np.float64).reshape(nn, N)
     scale = np.maximum(scale + np.abs(br), 1e-300)
+    if block:
+        scale = np.repeat(scale.max(axis=1, keepdims=True), N, axis=1)
     err = np.abs(np.asarray(b, dtype=np.float64).reshape(nn, N) - br)
     return float((err / scale).max())
 
