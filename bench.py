@@ -17,10 +17,11 @@ gradients).  One JSON line is printed.
     python bench.py --workload batch                              # 96 independent cavities
 
 Kernel device times come from the library's HIP-event clocks on the flow's
-stream (oagpu_profile / oagpu_kernel_stats): during the warm-up steps every
-main kernel is clocked and the one with the largest device time per step is
-picked; the timed steps then clock only that kernel (its average launch time
-gives roofline.achieved = algorithmic bytes per launch / launch time).
+stream (oagpu_profile / oagpu_kernel_stats): every main kernel is clocked
+during the timed steps (events recorded without a host wait, read after the
+steps) and the one with the largest device time per step is the roofline's
+kernel (its average launch time gives roofline.achieved = algorithmic bytes
+per launch / launch time).
 """
 import argparse
 import json
@@ -85,33 +86,39 @@ def measured_traffic(kernel, workload, precision):
 
 
 # ----------------------------------------------------------------- driver
+def fp64_counterpart(clock):
+    """The FP64 form's kernel doing the work of the mixed form's `clock`
+    (its FP32 Krylov SpMV runs as the FP64 one there)."""
+    return clock[:-4] if clock.endswith("_f32") else clock
+
+
 def run_form(oa, f, step, steps, warmup, dominant=None, tune=True):
-    """Autotune the flow's launches (set-up, untimed), warm up with every
-    kernel clocked, then `steps` timed steps clocking only `dominant` (or the
-    warm-up's largest device time per step).  Returns wall seconds, step
-    outputs, per-kernel warm-up ms/step, the dominant kernel's (launches,
-    total ms) over the timed steps and the tuner's choices."""
+    """Autotune the flow's launches (set-up, untimed), warm up (the first
+    step builds the AMG hierarchy), then `steps` timed steps with every main
+    kernel clocked (events recorded on the flow's stream, read after the
+    steps).  The dominant kernel is `dominant` or the one with the largest
+    device time per timed step.  Returns wall seconds, step outputs,
+    per-kernel ms/step over the timed steps, the dominant kernel's
+    (launches, total ms) and the tuner's choices."""
     tuning = {k: [float(x) for x in v] for k, v in f.autotune().items()} if tune else None
-    f.profile("all")
     hist = [step() for _ in range(warmup)]
-    warm = {}
-    for k in f.CLOCKS:
-        ms, n = f.kernel_stats(k)
-        if n:
-            warm[k] = {"ms_per_step": ms / max(warmup, 1), "launches_per_step": n / max(warmup, 1)}
-    if dominant is None:
-        dominant = max(warm, key=lambda k: warm[k]["ms_per_step"])
-    f.profile(dominant)
+    f.profile("all")
     oa.load_library().oagpu_synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         hist.append(step())
     oa.load_library().oagpu_synchronize()
     t = time.perf_counter() - t0
-    ms, n = f.kernel_stats(dominant)
+    timed = {}
+    for k in f.CLOCKS:
+        ms, n = f.kernel_stats(k)
+        if n:
+            timed[k] = {"ms_per_step": ms / steps, "launches_per_step": n / steps, "ms": ms, "n": n}
     f.profile("none")
-    return {"seconds": t, "hist": np.array(hist), "warm": warm, "dominant": dominant,
-            "dom_ms": ms, "dom_launches": n, "tuning": tuning}
+    if dominant is None or dominant not in timed:
+        dominant = max(timed, key=lambda k: timed[k]["ms_per_step"])
+    return {"seconds": t, "hist": np.array(hist), "warm": timed, "dominant": dominant,
+            "dom_ms": timed[dominant]["ms"], "dom_launches": timed[dominant]["n"], "tuning": tuning}
 
 
 def kernel_table(r, ab, ne):
@@ -273,7 +280,7 @@ def bench_bfs(args):
         def step():
             return np.concatenate([f.outer_iteration(BFS_CTL), f.sst_iteration(BFS_SCTL)])
 
-        dom = None if prec == oa.OAGPU_MIXED else res[oa.OAGPU_MIXED]["dominant"]
+        dom = None if prec == oa.OAGPU_MIXED else fp64_counterpart(res[oa.OAGPU_MIXED]["dominant"])
         res[prec] = run_form(oa, f, step, args.steps, args.warmup, dom)
         f.close()
         print(f"[bench] {'mixed' if prec else 'fp64'} {args.steps / res[prec]['seconds']:.3f} it/s, "
@@ -377,7 +384,7 @@ def bench_cavity(args):
     for prec in (oa.OAGPU_MIXED, oa.OAGPU_FP64):
         f = make_flow(prec)
         setup(f)
-        dom = None if prec == oa.OAGPU_MIXED else res[oa.OAGPU_MIXED]["dominant"]
+        dom = None if prec == oa.OAGPU_MIXED else fp64_counterpart(res[oa.OAGPU_MIXED]["dominant"])
         res[prec] = run_form(oa, f, lambda: f.outer_iteration(CTL), args.steps, args.warmup, dom)
         f.close()
         print(f"[bench] {'mixed' if prec else 'fp64'} {args.steps / res[prec]['seconds']:.3f} it/s, "
@@ -581,6 +588,39 @@ def batch_cases(n_cases, seed=7):
             for n, r, t in zip(sizes, re, th)]
 
 
+def batch_cpu_baseline(cases, iters, tol, threads, budget_s=20.0):
+    """Oracle on a bounded sample of the batch: the cases in order, each to
+    the residual target or the iteration cap, until about budget_s of CPU
+    time; cases/s over the whole batch scaled by the elements sampled."""
+    pyoracle = oracle_module(threads)
+    from openaccel_amd import meshgen
+    done_el, t_all, k = 0, 0.0, 0
+    for c in cases:
+        if t_all > budget_s:
+            break
+        m = meshgen.cavity_hex(c["n"])
+        case = pyoracle.Case(m, pyoracle.default_params(mu=c["mu"], p_method=1),
+                             np.array([c["lid"], [0.0, 0.0, 0.0]]))
+        case.field("U")[:] = meshgen.initial_velocity(case.nn, c["n"])
+        case.initialize()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            h = case.outer_iteration(CTL)
+            if tol > 0 and h[:4].max() < tol:
+                break
+        t_all += time.perf_counter() - t0
+        done_el += c["n"] ** 3
+        k += 1
+    elems = sum(c["n"] ** 3 for c in cases)
+    scale = done_el / elems
+    _, model = host_info()
+    return {"value": round(len(cases) * scale / t_all, 6), "unit": "cases/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle C restatement (OpenMP, {threads} threads on {model}), Jacobi-CG p', "
+                      f"the first {k} of the {len(cases)} cases ({done_el} of {elems} elements, "
+                      f"{t_all:.1f} s) each run as in the batch, scaled by the element fraction"}
+
+
 def bench_batch(args):
     import openaccel_amd as oa
     from openaccel_amd import meshgen
@@ -616,13 +656,26 @@ def bench_batch(args):
         f.close()
     t_seq = seq.sum()
     print(f"[bench] fp64 one at a time: {t_seq:.2f} s", file=sys.stderr, flush=True)
-    flows = [flow(c, oa.OAGPU_MIXED) for c in cases]
-    oa.load_library().oagpu_synchronize()
-    t0 = time.perf_counter()
-    hist, done, sec = oa.batch_outer_iterations(flows, CTL, iters, tol=tol, threads=16)
-    t_batch = time.perf_counter() - t0
+
+    def batched(precision):
+        flows = [flow(c, precision) for c in cases]
+        oa.load_library().oagpu_synchronize()
+        t0 = time.perf_counter()
+        hist, done, sec = oa.batch_outer_iterations(flows, CTL, iters, tol=tol, threads=16)
+        t = time.perf_counter() - t0
+        for f in flows:
+            f.close()
+        return t, hist, done, sec
+
+    # the concurrency alone (FP64 batched), then concurrency + the mixed form
+    t_b64, hist64, done64, sec64 = batched(oa.OAGPU_FP64)
+    print(f"[bench] fp64 batched: {t_b64:.2f} s", file=sys.stderr, flush=True)
+    t_batch, hist, done, sec = batched(oa.OAGPU_MIXED)
     print(f"[bench] mixed batched: {t_batch:.2f} s", file=sys.stderr, flush=True)
     last = np.array([hist[k][done[k] - 1][:4].max() for k in range(len(cases))])
+    cpu = None
+    if not args.no_cpu:
+        cpu = batch_cpu_baseline(cases, iters, tol, args.cpu_threads or host_info()[0])
     line = {
         "metric": METRIC, "value": round(len(cases) / t_batch, 4), "unit": "cases/s",
         "n_gpus": 1, "steps": iters, "warmup": 0,
@@ -640,12 +693,18 @@ def bench_batch(args):
                                "per_case_s_p50": round(float(np.percentile(seq, 50)), 4),
                                "per_case_s_p99": round(float(np.percentile(seq, 99)), 4),
                                "reached": int((seq_it < iters).sum())},
+        "fp64_batched": {"value": round(len(cases) / t_b64, 4), "unit": "cases/s",
+                         "per_case_s_p50": round(float(np.percentile(sec64, 50)), 4),
+                         "per_case_s_p99": round(float(np.percentile(sec64, 99)), 4),
+                         "iterations_p50": float(np.percentile(done64, 50))},
         "batched_over_one_at_a_time": round(t_seq / t_batch, 3),
+        "concurrency_gain_fp64": round(t_seq / t_b64, 3),
+        "mixed_over_fp64_batched": round(t_b64 / t_batch, 3),
         "per_case_s_p50": round(float(np.percentile(sec, 50)), 4),
         "per_case_s_p99": round(float(np.percentile(sec, 99)), 4),
         "reached": int((last < tol).sum()) if tol > 0 else None,
         "iterations_p50": float(np.percentile(done, 50)),
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line))
 
