@@ -125,8 +125,13 @@ int oagpu_synchronize(void);
  * topology (etype [n_elems]: OAGPU_HEX8 / TET4 / WED6 / PYR5, the unused
  * trailing entries of smaller elements ignored; etype NULL: all hex8, the
  * element buckets of the reference's blocks flattened into one list),
- * boundary sides (parent element, side ordinal 0..5, boundary id), which
- * must be quad sides of hex8 elements.  Builds the CRS node graph
+ * boundary sides (parent element, side ordinal, boundary id) of any
+ * topology: the side ordinals of the reference's master elements (hex8
+ * 0..5, tet4 0..3, wedge6 0..4 with tri sides 3 and 4, pyramid5 0..4 with
+ * the quad base 4; sideNodeOrdinals_ of Tet4CVFEM.C:416-421,
+ * Wed6CVFEM.C:494-501, Pyr5CVFEM.C:759-765), one face ip per side node;
+ * per-side arrays of the flow (side mass flows, reversal flags) keep four
+ * entries per side, the fourth unused on tri sides.  Builds the CRS node graph
  * (nodeGraph::buildGraph_, src/mesh/graph/nodeGraph.cpp:29-392) with the
  * given stencil, the node->element incidences and element->CRS scatter maps,
  * the dual nodal volumes (mesh::updateDualNodalVolumeField_,

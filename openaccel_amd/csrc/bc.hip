@@ -1,8 +1,9 @@
 // bc.hip -- side mass flow rates and reversal flags of velocity inlets and
 // static-pressure outlets.  Face-owned: one lane per inlet / outlet face ip
 // (mass flows) or per outlet face (reversal flags), over the list of such
-// faces; each value is written by exactly one lane.
-#include "device_util.hpp"
+// faces; each value is written by exactly one lane.  Sides of every element
+// topology: a tri side's fourth ip lane does nothing.
+#include "sides.hpp"
 
 namespace oagpu {
 
@@ -26,6 +27,7 @@ struct SideArgs {
     int* rfl;
     double* msd;
     int* err;
+    SideGeo sg;
 };
 
 // init: initializeMassFlowRate side part (flowModel.cpp:3802-4997): inlet
@@ -36,36 +38,37 @@ __global__ void __launch_bounds__(256) k_side_mdot(const SideArgs a) {
     const int i = int(xcd_block() * blockDim.x + threadIdx.x);
     if (i >= 4 * a.nio) return;
     const int f = a.io_face[i >> 2], fk = i & 3;
-    const int ord = a.bf_ord[f], bc = a.bf_bc[f], t = a.bc_type[bc];
-    const int* ce = a.conn + 8 * size_t(a.bf_elem[f]);
-    const int* sn = hex8::c_side_nodes[ord];
+    const int e = a.bf_elem[f], ord = a.bf_ord[f], bc = a.bf_bc[f], t = a.bc_type[bc];
+    const int* ce = a.conn + 8 * size_t(e);
+    const int* sn = c_sides[a.sg.etype ? a.sg.etype[e] : OAGPU_HEX8][ord];
+    if (fk == 3 && sn[3] < 0) return;  // tri side: three ips
     const size_t on = 3 * size_t(ce[sn[fk]]);
     const double xr[3] = {a.xyz[on], a.xyz[on + 1], a.xyz[on + 2]};
     int bad = 0;
-    double s[3], Ji[9], A[3];
-    face_ip_geom(a.xyz, ce, ord, fk, t == OAGPU_BC_OUTLET ? a.sp : a.us, xr, s, Ji, A, bad);
+    FaceIp<double> fi;
+    face_ip<true>(a.sg, a.xyz, ce, e, f, ord, fk, t == OAGPU_BC_OUTLET ? a.sp : a.us, xr, fi, bad);
     double& md = a.msd[4 * size_t(f) + fk];
     double rip = 0;
-    for (int q = 0; q < 4; ++q) rip += hex8::c_quad_shape[a.us][fk][q] * a.rho;
+    for (int q = 0; q < fi.npf; ++q) rip += fi.shape(a.us, fk, q) * a.rho;
     if (t == OAGPU_BC_INLET) {
         double m = 0;
-        for (int j = 0; j < 3; ++j) m += (rip * a.bc_U[3 * bc + j]) * A[j];
+        for (int j = 0; j < 3; ++j) m += (rip * a.bc_U[3 * bc + j]) * fi.A[j];
         md = a.init ? m : a.urf * m + (1.0 - a.urf) * md;
     } else if (a.init) {
         double u[3] = {0, 0, 0};
-        for (int q = 0; q < 4; ++q) {
-            const double r = hex8::c_quad_shape[a.us][fk][q];
-            const size_t o = 3 * size_t(ce[sn[q]]);
+        for (int q = 0; q < fi.npf; ++q) {
+            const double r = fi.shape(a.us, fk, q);
+            const size_t o = 3 * size_t(ce[fi.sn[q]]);
             for (int j = 0; j < 3; ++j) u[j] += r * a.U[o + j];
         }
         double m = 0;
-        for (int j = 0; j < 3; ++j) m += rip * u[j] * A[j];
+        for (int j = 0; j < 3; ++j) m += rip * u[j] * fi.A[j];
         md = fmax(0.0, m);
     } else if (a.rfl[4 * size_t(f) + fk] == 1) {
         md = 0.0;  // reversed: a slip wall
     } else {
-        const double m = outlet_rhie_chow(ce, ord, fk, a.us, s, Ji, A, a.U, a.du, a.p, a.gradP,
-                                          a.rho, a.bc_p[bc]);
+        const double m = outlet_rhie_chow(fi, ce, fk, a.us, a.U, a.du, a.p, a.gradP, a.rho,
+                                          a.bc_p[bc]);
         md = a.urf * m + (1.0 - a.urf) * md;
     }
     if (bad) *a.err = 1;
@@ -96,6 +99,7 @@ SideArgs side_args(Flow& f) {
     a.rfl = f.rfl.p;
     a.msd = f.msd.p;
     a.err = f.err.p;
+    a.sg = side_geo(m);
     return a;
 }
 
@@ -117,31 +121,33 @@ __global__ void __launch_bounds__(256) k_reversal(const SideArgs a) {
     const int i = int(xcd_block() * blockDim.x + threadIdx.x);
     if (i >= a.nio) return;
     const int f = a.io_face[i];
-    const int ord = a.bf_ord[f], bc = a.bf_bc[f];
+    const int e = a.bf_elem[f], ord = a.bf_ord[f], bc = a.bf_bc[f];
     if (a.bc_type[bc] != OAGPU_BC_OUTLET) return;
-    const int* ce = a.conn + 8 * size_t(a.bf_elem[f]);
-    const int* sn = hex8::c_side_nodes[ord];
+    const int* ce = a.conn + 8 * size_t(e);
+    const int* sn = c_sides[a.sg.etype ? a.sg.etype[e] : OAGPU_HEX8][ord];
+    const int npf = sn[3] < 0 ? 3 : 4;
+    const double fw = 1.0 / npf;
     double* md = a.msd + 4 * size_t(f);
     int* rf = a.rfl + 4 * size_t(f);
     double side = 0;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < npf; ++k) {
         side += md[k];
         md[k] = fmax(md[k], 0.0);
     }
-    double pavg = 0, uavg[3] = {0, 0, 0}, fp[4][3];
+    double pavg = 0, uavg[3] = {0, 0, 0}, fp[4][3] = {};
     const size_t o0 = 3 * size_t(ce[sn[0]]);
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < npf; ++q) {
         const size_t o = 3 * size_t(ce[sn[q]]);
-        pavg += 0.25 * a.p[ce[sn[q]]];
+        pavg += fw * a.p[ce[sn[q]]];
         for (int j = 0; j < 3; ++j) {
-            uavg[j] += 0.25 * a.U[o + j];
+            uavg[j] += fw * a.U[o + j];
             fp[q][j] = a.xyz[o + j] - a.xyz[o0 + j];
         }
     }
     const double pbc = a.bc_p[bc];
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < npf; ++k) {
         double A[3];
-        hex8::quad_face_areav(fp, k, A);
+        face_areav(npf, fp, k, A);
         const double amag = sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
         double vn = 0;
         for (int j = 0; j < 3; ++j) vn += uavg[j] * A[j] / amag;

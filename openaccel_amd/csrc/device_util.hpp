@@ -153,48 +153,4 @@ __device__ inline void face_ip_geom(const double* xyz, const int* ce, int ord, i
     hex8::quad_face_areav(fp, fk, A);
 }
 
-// side node q of side `ord` is element-local node sn[q]; true if element
-// node ic lies on the side
-__device__ inline bool on_side(int ord, int ic) {
-    const int* sn = hex8::c_side_nodes[ord];
-    return ic == sn[0] || ic == sn[1] || ic == sn[2] || ic == sn[3];
-}
-
-// Rhie-Chow mass flow rate at face ip fk of a static-pressure outlet side
-// (flowModel.cpp:7323-7690, SIMPLE: duRhs = du): rho U_ip . A -
-// rho du_ip . (grad p - Gp) A, U and du interpolated with the face shape
-// functions (us: shifted), grad p from the face gradient operator at s with
-// the specified pressure pbc on the side's nodes, Gp the side nodes' mean
-// nodal pressure gradient.  Pressures enter relative to pbc (the gradient
-// operator's columns sum to zero).
-__device__ inline double outlet_rhie_chow(const int* ce, int ord, int fk, int us,
-                                          const double s[3], const double Ji[9],
-                                          const double A[3], const double* U, const double* du,
-                                          const double* p, const double* gradP, double rho,
-                                          double pbc) {
-    const int* sn = hex8::c_side_nodes[ord];
-    double rip = 0, u[3] = {0, 0, 0}, d[3] = {0, 0, 0}, Gp[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
-    for (int q = 0; q < 4; ++q) {
-        const double r = hex8::c_quad_shape[us][fk][q];
-        const size_t o = 3 * size_t(ce[sn[q]]);
-        rip += r * rho;
-        for (int j = 0; j < 3; ++j) {
-            u[j] += r * U[o + j];
-            d[j] += r * du[o + j];
-            Gp[j] += 0.25 * gradP[o + j];
-        }
-    }
-#pragma unroll 1
-    for (int ic = 0; ic < 8; ++ic) {
-        if (on_side(ord, ic)) continue;
-        double g[3];
-        grad_N(ic, s[0], s[1], s[2], Ji, g);
-        const double pv = p[ce[ic]] - pbc;
-        for (int j = 0; j < 3; ++j) dp[j] += g[j] * pv;
-    }
-    double m = 0;
-    for (int j = 0; j < 3; ++j) m += (rip * u[j] - rip * d[j] * (dp[j] - Gp[j])) * A[j];
-    return m;
-}
-
 }  // namespace oagpu

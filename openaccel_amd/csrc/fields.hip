@@ -59,6 +59,7 @@ void launch_mesh_geometry(DevMesh& m, hipStream_t s) {
     if (m.mixed) {
         launch_scv_generic(m, s);
         launch_geo_generic(m, s);
+        launch_side_geo(m, s);
     }
     k_dual_volume<<<xcd_grid(m.nn, B), B, 0, s>>>(m.nn, m.inc_ptr.p, m.inc.p, m.scvw.p, m.vol.p);
     k_scv_weights<<<xcd_grid(m.ne, B), B, 0, s>>>(m.ne, m.scvw.p);

@@ -221,6 +221,7 @@ void launch_correct_scalar(Flow& f, int which);   // phi += x, clipped; field sc
 int topology_npe(int topo);
 void launch_scv_generic(DevMesh& m, hipStream_t s);  // SCV volumes of non-hex elements
 void launch_geo_generic(DevMesh& m, hipStream_t s);  // static ip geometry of non-hex elements
+void launch_side_geo(DevMesh& m, hipStream_t s);     // face-ip geometry of their boundary sides
 int topology_nscs(int topo);
 void element_geometry(int topo, int shifted, int n, const double* coords, double* A, double* V,
                       double* G, double* detj, int* err, hipStream_t s);

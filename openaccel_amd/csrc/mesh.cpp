@@ -11,6 +11,7 @@
 #include "mesh.hpp"
 #include "oagpu.h"
 #include "sell.hpp"
+#include "side_table.hpp"
 
 #include <algorithm>
 #include <functional>
@@ -32,8 +33,9 @@ static int topo_nscs(int t) { return t == OAGPU_TET4 ? 6 : t == OAGPU_WED6 ? 9 :
 static const int (*topo_lr(int t))[2] {
     return t == OAGPU_TET4 ? kTetLr : t == OAGPU_WED6 ? kWedLr : t == OAGPU_PYR5 ? kPyrLr : kLrscv;
 }
-static const int kSideNodes[6][4] = {{0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6},
-                                     {0, 4, 7, 3}, {0, 3, 2, 1}, {4, 5, 6, 7}};
+static const int kSides[4][6][4] = OAGPU_SIDE_TABLE;
+static const int kNumSides[4] = OAGPU_NSIDES;
+static int side_npf(int t, int ord) { return kSides[t][ord][3] < 0 ? 3 : 4; }
 
 static uint64_t row_slots(const std::vector<int>& rp, const std::vector<int>& cols, int row,
                           const int* en, int* max_pos, int npe = 8) {
@@ -67,9 +69,8 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
     }
     for (int f = 0; f < hm.nbf; ++f) {
         OAGPU_REQUIRE(hm.bf_elem[f] >= 0 && hm.bf_elem[f] < ne, "boundary face element out of range");
-        OAGPU_REQUIRE(etype(hm.bf_elem[f]) == OAGPU_HEX8,
-                      "boundary faces must be quad sides of hex8 elements");
-        OAGPU_REQUIRE(hm.bf_ord[f] >= 0 && hm.bf_ord[f] < 6, "boundary face ordinal out of range");
+        OAGPU_REQUIRE(hm.bf_ord[f] >= 0 && hm.bf_ord[f] < kNumSides[etype(hm.bf_elem[f])],
+                      "boundary face ordinal out of range for its element's topology");
         OAGPU_REQUIRE(hm.bf_bc[f] >= 0, "negative boundary id");
     }
     dm.nn = nn;
@@ -236,8 +237,9 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
     dm.h_binc_ptr.assign(size_t(nn) + 1, 0);
     for (int f = 0; f < hm.nbf; ++f) {
         const int* en = hm.conn + 8 * size_t(hm.bf_elem[f]);
-        for (int k = 0; k < 4; ++k) {
-            const int nd = en[kSideNodes[hm.bf_ord[f]][k]];
+        const int t = etype(hm.bf_elem[f]), ord = hm.bf_ord[f];
+        for (int k = 0; k < side_npf(t, ord); ++k) {
+            const int nd = en[kSides[t][ord][k]];
             dm.h_binc_ptr[nd + 1]++;
             isb[nd] = 1;
         }
@@ -251,11 +253,12 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
         int max_pos = 0;
         for (int f = 0; f < hm.nbf; ++f) {
             const int* en = hm.conn + 8 * size_t(hm.bf_elem[f]);
-            for (int k = 0; k < 4; ++k) {
-                const int nd = en[kSideNodes[hm.bf_ord[f]][k]];
+            const int t = etype(hm.bf_elem[f]), ord = hm.bf_ord[f];
+            for (int k = 0; k < side_npf(t, ord); ++k) {
+                const int nd = en[kSides[t][ord][k]];
                 const int q = fill[nd]++;
                 dm.h_binc[q] = (f << 2) | k;
-                dm.h_binc_slots[q] = row_slots(dm.h_row_ptr, dm.h_cols, nd, en, &max_pos);
+                dm.h_binc_slots[q] = row_slots(dm.h_row_ptr, dm.h_cols, nd, en, &max_pos, topo_npe(t));
             }
         }
     }
@@ -283,6 +286,18 @@ void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm) {
         }
         dm.geo_off.upload(off);
         dm.geo.alloc(size_t(tot));
+        // face-ip geometry of the non-hex elements' boundary sides
+        std::vector<long long> soff(size_t(hm.nbf), -1);
+        std::vector<int> sl;
+        for (int f = 0; f < hm.nbf; ++f)
+            if (etype(hm.bf_elem[f]) != OAGPU_HEX8) {
+                soff[f] = (long long)sl.size() * kSideRecord;
+                sl.push_back(f);
+            }
+        dm.nslist = int(sl.size());
+        dm.slist.upload(sl);
+        dm.sgeo_off.upload(soff);
+        dm.sgeo.alloc(std::max<size_t>(sl.size(), 1) * kSideRecord);
     }
     dm.row_ptr.upload(dm.h_row_ptr);
     dm.cols.upload(dm.h_cols);

@@ -32,8 +32,10 @@ struct HostMesh {
 //   position in the row of each of the element's 8 nodes, 0xFF when the
 //   column is not in the (reduced) row and is lumped into the diagonal block
 //   (assembler.h:423-445).
-// * node -> wall-face incidences (face << 2 | face ip whose nearest node is
-//   the row), with their own scatter maps.
+// * node -> boundary-side incidences (side << 2 | face ip whose nearest node
+//   is the row), with their own scatter maps; sides of any topology (quad4
+//   sides of hex8 / wedge6 / pyramid5, tri3 sides of tet4 / wedge6 /
+//   pyramid5, one face ip per side node).
 // * meshes with tet4 / wedge6 / pyramid5 elements: inc_la / ext hold the
 //   hex8 incidences only; the others (elem << 3 | local node) with their
 //   scatter maps (one byte per element node) form a CSR list gen_* per row,
@@ -52,6 +54,13 @@ struct DevMesh {
     DBuf<int> glist;           // [nglist] non-hex elements, ascending
     DBuf<double> geo;          // static ip geometry of the non-hex elements (topo::ip_geom)
     DBuf<long long> geo_off;   // [ne] offset of an element's geometry in geo (-1: hex8)
+    // boundary sides of non-hex elements: per side kSideStride doubles of
+    // face-ip geometry (area vector, dN/dx standard / shifted), sgeo_off[f]
+    // the side's offset (-1: a hex8 side); slist lists those sides
+    int nslist = 0;
+    DBuf<int> slist;           // [nslist]
+    DBuf<long long> sgeo_off;  // [nbf] (mixed meshes only)
+    DBuf<double> sgeo;
     DBuf<int> row_ptr, cols, diag_off;
     // Values of the CRS matrix are stored sliced-ELL: slices of 64 rows (one
     // wavefront), slot-major and lane-contiguous, so a wavefront's lanes (one

@@ -20,6 +20,7 @@
 // (coordinates are shifted to the row node in FP64 before rounding, so FP32
 // sees element-sized numbers).  Global coefficients are summed in FP64.
 #include "device_util.hpp"
+#include "sides.hpp"
 #include "topo_tables.hpp"
 #include "tables.hpp"
 
@@ -438,6 +439,7 @@ struct MomArgs {
     const uint64_t* gen_slots;
     const double* geo;          // static ip geometry of the non-hex elements
     const long long* geo_off;   // [ne] offset into geo
+    SideGeo sg;                 // boundary sides of the non-hex elements
     const int* etype;
     double rho, mu, dt, urf_inv, bnd_relax;
     int ugrad_shifted, us;
@@ -701,36 +703,36 @@ __device__ void mom_generic(const MomArgs& a, int la, int e, uint64_t sl, const 
 //   velocity inlet    full, fixed  (:791-1124)
 //   pressure outlet   ZNS, free    (:1468-1775)
 //   symmetry plane    full, free   (:2082-2375)
-template <typename T, bool ZNS, typename LT>
+template <typename T, bool ZNS, bool MX, typename LT>
 __device__ __forceinline__ void mom_face(const MomArgs& a, int f, int fk, bool fixed, uint64_t sl,
                                          const SellRow& sr, int dof, const double xr[3],
                                          double D[9], double R[3], int& bad, LT* lt) {
     const int e = a.bf_elem[f], ord = a.bf_ord[f], bc = a.bf_bc[f];
-    const int* sn = hex8::c_side_nodes[ord];
     const int* ce = a.conn + 8 * size_t(e);
-    T s[3], Ji[9], A[3];
-    face_ip_geom(a.xyz, ce, ord, fk, a.ugrad_shifted, xr, s, Ji, A, bad);
+    FaceIp<T> fi;
+    face_ip<MX>(a.sg, a.xyz, ce, e, f, ord, fk, a.ugrad_shifted, xr, fi, bad);
     T nx[3] = {0, 0, 0};
     if constexpr (ZNS) {
-        const T amag = sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
-        for (int d = 0; d < 3; ++d) nx[d] = A[d] / amag;
+        const T amag = sqrt(fi.A[0] * fi.A[0] + fi.A[1] * fi.A[1] + fi.A[2] * fi.A[2]);
+        for (int d = 0; d < 3; ++d) nx[d] = fi.A[d] / amag;
     }
     T mu = T(a.mu);
     if (a.mueff) {
         mu = T(0);
-        for (int q = 0; q < 4; ++q) mu += T(hex8::c_quad_shape[a.us][fk][q]) * T(a.mueff[ce[sn[q]]]);
+        for (int q = 0; q < fi.npf; ++q) mu += T(fi.shape(a.us, fk, q)) * T(a.mueff[ce[fi.sn[q]]]);
     }
     const T ub[3] = {T(a.bc_U[3 * bc]), T(a.bc_U[3 * bc + 1]), T(a.bc_U[3 * bc + 2])};
+    const T* A = fi.A;
     T rl[3] = {0, 0, 0};
 #pragma unroll 1
-    for (int ic = 0; ic < 8; ++ic) {
-        const bool mask = fixed && on_side(ord, ic);
+    for (int ic = 0; ic < fi.npe; ++ic) {
+        const bool mask = fixed && fi.on_side(ic);
         const T bcm = mask ? T(0) : T(1);
         const size_t o = 3 * size_t(ce[ic]);
         T uc[3];
         for (int d = 0; d < 3; ++d) uc[d] = mask ? ub[d] : T(a.U[o + d]);
         T g[3];
-        grad_N(ic, s[0], s[1], s[2], Ji, g);
+        fi.grad(ic, g);
         T acc[9];
         for (int k = 0; k < 9; ++k) acc[k] = T(0);
         for (int j = 0; j < 3; ++j) {
@@ -768,7 +770,7 @@ __device__ __forceinline__ void mom_face(const MomArgs& a, int f, int fk, bool f
 // (IP_EXPLICIT_ADVECTIVE_FLUX__), outlets the implicit upwind one
 // (IP_IMPLICIT_UPWIND_ADVECTIVE_FLUX__: msd on the diagonal, msd U_row on
 // the rhs) unless the ip is reversed, when it takes no terms (a slip wall)
-template <typename T, typename LT>
+template <typename T, bool MX, typename LT>
 __device__ __forceinline__ void mom_boundary(const MomArgs& a, int pk, uint64_t sl,
                                              const SellRow& sr, int dof, int row,
                                              const double xr[3], double D[9], double R[3],
@@ -776,12 +778,12 @@ __device__ __forceinline__ void mom_boundary(const MomArgs& a, int pk, uint64_t 
     const int f = pk >> 2, fk = pk & 3;
     const int t = (a.symA || a.msd) ? a.bc_type[a.bf_bc[f]] : OAGPU_BC_WALL;
     if (t == OAGPU_BC_SYMMETRY) {
-        mom_face<T, false>(a, f, fk, false, sl, sr, dof, xr, D, R, bad, lt);
+        mom_face<T, false, MX>(a, f, fk, false, sl, sr, dof, xr, D, R, bad, lt);
     } else if (t == OAGPU_BC_INLET) {
         const double tm = a.msd[4 * size_t(f) + fk];
         const int bc = a.bf_bc[f];
         for (int i = 0; i < 3; ++i) R[i] -= tm * a.bc_U[3 * bc + i];
-        mom_face<T, false>(a, f, fk, true, sl, sr, dof, xr, D, R, bad, lt);
+        mom_face<T, false, MX>(a, f, fk, true, sl, sr, dof, xr, D, R, bad, lt);
     } else if (t == OAGPU_BC_OUTLET) {
         if (a.rfl[4 * size_t(f) + fk] == 1) return;
         const double tm = a.msd[4 * size_t(f) + fk];
@@ -789,9 +791,9 @@ __device__ __forceinline__ void mom_boundary(const MomArgs& a, int pk, uint64_t 
             R[i] -= tm * a.U[3 * size_t(row) + i];
             D[4 * i] += tm;
         }
-        mom_face<T, true>(a, f, fk, false, sl, sr, dof, xr, D, R, bad, lt);
+        mom_face<T, true, MX>(a, f, fk, false, sl, sr, dof, xr, D, R, bad, lt);
     } else {
-        mom_face<T, true>(a, f, fk, true, sl, sr, dof, xr, D, R, bad, lt);
+        mom_face<T, true, MX>(a, f, fk, true, sl, sr, dof, xr, D, R, bad, lt);
     }
 }
 
@@ -873,7 +875,7 @@ __global__ void __launch_bounds__(LDS ? 64 : 256) k_momentum_rows(const MomArgs 
     }
 #pragma unroll 1
     for (int k = a.binc_ptr[row]; k < a.binc_ptr[row + 1]; ++k)
-        mom_boundary<T>(a, a.binc[k], a.binc_slots[k], sr, dof, row, xr, D, R, bad, lt);
+        mom_boundary<T, MX>(a, a.binc[k], a.binc_slots[k], sr, dof, row, xr, D, R, bad, lt);
     if constexpr (LDS != 0) {
         const int ns = sr.width < kLdsSlots ? sr.width : kLdsSlots;
 #pragma unroll 1
@@ -940,6 +942,7 @@ void launch_assemble_momentum(Flow& f) {
     a.bf_elem = m.bf_elem.p;
     a.bf_ord = m.bf_ord.p;
     a.bf_bc = m.bf_bc.p;
+    a.sg = side_geo(m);
     a.bc_U = f.bc_U.p;
     a.bnode = m.bnode.p;
     a.vol = m.vol.p;
@@ -1052,6 +1055,7 @@ struct PcArgs {
     const uint64_t* gen_slots;
     const double* geo;          // static ip geometry of the non-hex elements
     const long long* geo_off;   // [ne] offset into geo
+    SideGeo sg;                 // boundary sides of the non-hex elements
     const int* etype;
     double rho;
     int u_shifted, p_ref;
@@ -1269,37 +1273,37 @@ __device__ __forceinline__ void pc_boundary(const PcArgs& a, int pk, uint64_t sl
     const int bc = a.bf_bc[f], t = a.bc_type[bc];
     if (t != OAGPU_BC_INLET && t != OAGPU_BC_OUTLET) return;
     if (t == OAGPU_BC_OUTLET && a.rfl[4 * size_t(f) + fk] != 0) return;
-    const int ord = a.bf_ord[f];
-    const int* ce = a.conn + 8 * size_t(a.bf_elem[f]);
-    double s[3], Ji[9], A[3];
-    face_ip_geom(a.xyz, ce, ord, fk, t == OAGPU_BC_OUTLET ? SP : a.u_shifted, xr, s, Ji, A, bad);
+    const int e = a.bf_elem[f], ord = a.bf_ord[f];
+    const int* ce = a.conn + 8 * size_t(e);
+    FaceIp<double> fi;
+    face_ip<true>(a.sg, a.xyz, ce, e, f, ord, fk, t == OAGPU_BC_OUTLET ? SP : a.u_shifted, xr, fi,
+                  bad);
     const int us = a.u_shifted ? 1 : 0;
     double rip = 0;
-    for (int q = 0; q < 4; ++q) rip += hex8::c_quad_shape[us][fk][q] * a.rho;
+    for (int q = 0; q < fi.npf; ++q) rip += fi.shape(us, fk, q) * a.rho;
     if (t == OAGPU_BC_INLET) {
         double m = 0;
-        for (int j = 0; j < 3; ++j) m += (rip * a.bc_U[3 * bc + j]) * A[j];
+        for (int j = 0; j < 3; ++j) m += (rip * a.bc_U[3 * bc + j]) * fi.A[j];
         R -= m;
         return;
     }
-    const int* sn = hex8::c_side_nodes[ord];
     double du[3] = {0, 0, 0};
-    for (int q = 0; q < 4; ++q) {
-        const double r = hex8::c_quad_shape[us][fk][q];
-        for (int j = 0; j < 3; ++j) du[j] += r * a.du[3 * size_t(ce[sn[q]]) + j];
+    for (int q = 0; q < fi.npf; ++q) {
+        const double r = fi.shape(us, fk, q);
+        for (int j = 0; j < 3; ++j) du[j] += r * a.du[3 * size_t(ce[fi.sn[q]]) + j];
     }
 #pragma unroll 1
-    for (int ic = 0; ic < 8; ++ic) {
-        if (on_side(ord, ic)) continue;
+    for (int ic = 0; ic < fi.npe; ++ic) {
+        if (fi.on_side(ic)) continue;
         double g[3];
-        grad_N(ic, s[0], s[1], s[2], Ji, g);
+        fi.grad(ic, g);
         double lf = 0;
-        for (int j = 0; j < 3; ++j) lf += -rip * du[j] * g[j] * A[j];
+        for (int j = 0; j < 3; ++j) lf += -rip * du[j] * g[j] * fi.A[j];
         const int sp = slot_of(sl, ic);
         if (sp == 0xFF || sp == dof) D += lf;
         else *sell_val<1>(a.vals, sr, sp, 0) += lf;
     }
-    R -= outlet_rhie_chow(ce, ord, fk, us, s, Ji, A, a.U, a.du, a.p, a.gradP, a.rho, a.bc_p[bc]);
+    R -= outlet_rhie_chow(fi, ce, fk, us, a.U, a.du, a.p, a.gradP, a.rho, a.bc_p[bc]);
 }
 
 // The inlet / outlet face ips of the p' rows in io_row (each row one lane,
@@ -1417,6 +1421,7 @@ void launch_assemble_pcorr(Flow& f) {
     a.bf_elem = m.bf_elem.p;
     a.bf_ord = m.bf_ord.p;
     a.bf_bc = m.bf_bc.p;
+    a.sg = side_geo(m);
     a.bc_type = f.bc_type.p;
     a.bc_U = f.bc_U.p;
     a.bc_p = f.bc_p.p;
@@ -1513,6 +1518,7 @@ struct ScArgs {
     const uint64_t* gen_slots;
     const double* geo;          // static ip geometry of the non-hex elements
     const long long* geo_off;   // [ne] offset into geo
+    SideGeo sg;                 // boundary sides of the non-hex elements
     const int* etype;
     double rho, mu, dt, urf_inv, sig1, sig2;
     const double* phiold;   // transient: phi of the previous two time levels
@@ -1791,24 +1797,23 @@ __device__ __forceinline__ void sc_boundary(const ScArgs& a, int pk, uint64_t sl
     }
     const double pbc = a.bc_phi[bc];
     R -= tm * pbc;
-    const int ord = a.bf_ord[f];
-    const int* sn = hex8::c_side_nodes[ord];
-    const int* ce = a.conn + 8 * size_t(a.bf_elem[f]);
-    double s[3], Ji[9], A[3];
-    face_ip_geom(a.xyz, ce, ord, fk, SG, xr, s, Ji, A, bad);
+    const int e = a.bf_elem[f], ord = a.bf_ord[f];
+    const int* ce = a.conn + 8 * size_t(e);
+    FaceIp<double> fi;
+    face_ip<true>(a.sg, a.xyz, ce, e, f, ord, fk, SG, xr, fi, bad);
     double Gb = 0;
-    for (int q = 0; q < 4; ++q) {
-        const int n = ce[sn[q]];
+    for (int q = 0; q < fi.npf; ++q) {
+        const int n = ce[fi.sn[q]];
         const double F1 = a.F1[n];
-        Gb += hex8::c_quad_shape[a.us][fk][q] * (a.mu + a.mut[n] * (F1 * a.sig1 + (1.0 - F1) * a.sig2));
+        Gb += fi.shape(a.us, fk, q) * (a.mu + a.mut[n] * (F1 * a.sig1 + (1.0 - F1) * a.sig2));
     }
 #pragma unroll 1
-    for (int ic = 0; ic < 8; ++ic) {
+    for (int ic = 0; ic < fi.npe; ++ic) {
         double g[3];
-        grad_N(ic, s[0], s[1], s[2], Ji, g);
+        fi.grad(ic, g);
         double lf = 0;
-        for (int j = 0; j < 3; ++j) lf += -Gb * g[j] * A[j];
-        if (on_side(ord, ic)) {
+        for (int j = 0; j < 3; ++j) lf += -Gb * g[j] * fi.A[j];
+        if (fi.on_side(ic)) {
             R -= lf * pbc;
             continue;
         }
@@ -1941,6 +1946,7 @@ void launch_assemble_scalar(Flow& f, int which) {
     a.bf_elem = m.bf_elem.p;
     a.bf_ord = m.bf_ord.p;
     a.bf_bc = m.bf_bc.p;
+    a.sg = side_geo(m);
     a.bc_type = f.bc_type.p;
     a.bc_phi = which == 0 ? f.bc_k.p : f.bc_w.p;
     a.msd = f.msd.p;
@@ -2330,17 +2336,19 @@ __global__ void __launch_bounds__(256) k_gradient(int nn, int nslice, const int*
         const int f = binc[k] >> 2, fk = binc[k] & 3;
         if (bc_type && bc_type[bf_bc[f]] == OAGPU_BC_SYMMETRY) continue;  // no face term
         const int e = bf_elem[f], ord = bf_ord[f];
-        double fp[4][3], pf[4][N];
-        for (int q = 0; q < 4; ++q) {
-            const int n = conn[8 * size_t(e) + hex8::c_side_nodes[ord][q]];
+        const int* sn = c_sides[MX ? etype[e] : OAGPU_HEX8][ord];
+        const int npf = sn[3] < 0 ? 3 : 4;
+        double fp[4][3] = {}, pf[4][N] = {};
+        for (int q = 0; q < npf; ++q) {
+            const int n = conn[8 * size_t(e) + sn[q]];
             for (int d = 0; d < 3; ++d) fp[q][d] = xyz[3 * size_t(n) + d] - xr[d];
             for (int i = 0; i < N; ++i) pf[q][i] = phi[N * size_t(n) + i];
         }
         double A[3];
-        hex8::quad_face_areav(fp, fk, A);
+        face_areav(npf, fp, fk, A);
         for (int i = 0; i < N; ++i) {
             double pip = 0.0;
-            for (int q = 0; q < 4; ++q) pip += hex8::c_quad_shape[SH][fk][q] * pf[q][i];
+            for (int q = 0; q < npf; ++q) pip += face_shape(npf, SH, fk, q) * pf[q][i];
             for (int j = 0; j < 3; ++j) G[3 * i + j] += gurf * (pip - pa[i]) * A[j] * iv;
         }
     }
@@ -2484,10 +2492,12 @@ __global__ void __launch_bounds__(256) k_blending(int nn, const double* xyz, con
     for (int k = binc_ptr[a]; k < binc_ptr[a + 1]; ++k) {
         const int f = binc[k] >> 2, fk = binc[k] & 3;
         const int e = bf_elem[f], ord = bf_ord[f];
+        const int* sn = c_sides[MX ? etype[e] : OAGPU_HEX8][ord];
+        const int npf = sn[3] < 0 ? 3 : 4;
         double dx[3] = {0, 0, 0};
-        for (int q = 0; q < 4; ++q) {
-            const int n = conn[8 * size_t(e) + hex8::c_side_nodes[ord][q]];
-            for (int d = 0; d < 3; ++d) dx[d] += hex8::c_quad_shape[0][fk][q] * xyz[3 * size_t(n) + d];
+        for (int q = 0; q < npf; ++q) {
+            const int n = conn[8 * size_t(e) + sn[q]];
+            for (int d = 0; d < 3; ++d) dx[d] += face_shape(npf, 0, fk, q) * xyz[3 * size_t(n) + d];
         }
         for (int d = 0; d < 3; ++d) dx[d] -= xa[d];
         for (int i = 0; i < N; ++i) bj_limit(b[i], mx[i], mn[i], pa[i], g[i], dx);
@@ -2522,7 +2532,7 @@ __global__ void __launch_bounds__(256) k_boundary_types(int nn, const double* xy
                                                         const int* binc_ptr, const int* binc,
                                                         const int* bf_elem, const int* bf_ord,
                                                         const int* bf_bc, const int* bc_type,
-                                                        int* wnode, double* symA) {
+                                                        const int* etype, int* wnode, double* symA) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nn) return;
     int w = 0;
@@ -2533,12 +2543,15 @@ __global__ void __launch_bounds__(256) k_boundary_types(int nn, const double* xy
         const int t = bc_type[bf_bc[f]];
         if (t == OAGPU_BC_WALL) w = 1;
         if (t != OAGPU_BC_SYMMETRY || !symA) continue;
-        double fp[4][3], A[3];
-        for (int q = 0; q < 4; ++q) {
-            const size_t o = 3 * size_t(conn[8 * size_t(bf_elem[f]) + hex8::c_side_nodes[bf_ord[f]][q]]);
+        const int e = bf_elem[f];
+        const int* sn = c_sides[etype ? etype[e] : OAGPU_HEX8][bf_ord[f]];
+        const int npf = sn[3] < 0 ? 3 : 4;
+        double fp[4][3] = {}, A[3];
+        for (int q = 0; q < npf; ++q) {
+            const size_t o = 3 * size_t(conn[8 * size_t(e) + sn[q]]);
             for (int d = 0; d < 3; ++d) fp[q][d] = xyz[o + d] - xr[d];
         }
-        hex8::quad_face_areav(fp, fk, A);
+        face_areav(npf, fp, fk, A);
         for (int d = 0; d < 3; ++d) s[d] += A[d];
     }
     wnode[a] = w;
@@ -2551,7 +2564,7 @@ void launch_boundary_types(Flow& f) {
     const unsigned B = 256;
     k_boundary_types<<<(m.nn + B - 1) / B, B, 0, f.stream>>>(
         m.nn, m.xyz.p, m.conn.p, m.binc_ptr.p, m.binc.p, m.bf_elem.p, m.bf_ord.p, m.bf_bc.p,
-        f.bc_type.p, f.wnode.p, f.has_sym ? f.symA.p : nullptr);
+        f.bc_type.p, m.mixed ? m.etype.p : nullptr, f.wnode.p, f.has_sym ? f.symA.p : nullptr);
     OAGPU_HIP(hipGetLastError());
 }
 

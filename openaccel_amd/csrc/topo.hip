@@ -13,6 +13,7 @@
 // locations of TetSCS, WedSCS, PyrSCS (standard and shifted).
 #include "device_util.hpp"
 #include "flow.hpp"
+#include "sides.hpp"
 #include "topo_tables.hpp"
 
 #include <mutex>
@@ -251,8 +252,13 @@ __global__ void __launch_bounds__(256) k_geo_generic(int n, const int* glist, co
         for (int sh = 0; sh < 2; ++sh) {
             const double* l = topo::ip_loc<TOPO>(sh, ip);
             double dd[npe][3], g[npe][3];
+            int bsh = 0;
             topo::deriv<TOPO>(sh, l[0], l[1], l[2], dd);
-            topo::gradop<npe>(dd, c, g, &bad);
+            topo::gradop<npe>(dd, c, g, &bsh);
+            // the shifted ip at a pyramid's apex is singular (det J = 0, a
+            // zero operator from the fallback); only the standard ips judge
+            // the element
+            if (!sh) bad |= bsh;
             for (int j = 0; j < npe; ++j)
                 for (int k = 0; k < 3; ++k) q[3 + 3 * npe * sh + 3 * j + k] = g[j][k];
         }
@@ -269,6 +275,89 @@ void launch_geo_generic(DevMesh& m, hipStream_t s) {
                                               m.geo_off.p, m.geo.p, m.err.p);
     k_geo_generic<topo::kPyr><<<G, B, 0, s>>>(m.nglist, m.glist.p, m.etype.p, m.conn.p, m.xyz.p,
                                               m.geo_off.p, m.geo.p, m.err.p);
+    OAGPU_HIP(hipGetLastError());
+}
+
+// vertex parametric locations of tet4 / wedge6 / pyramid5 (index topology - 1)
+static __constant__ double c_vpar[3][6][3] = {
+    {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, 0}, {0, 0, 0}},
+    {{0, 0, -1}, {1, 0, -1}, {0, 1, -1}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}},
+    {{-1, -1, 0}, {1, -1, 0}, {1, 1, 0}, {-1, 1, 0}, {0, 0, 1}, {0, 0, 0}}};
+
+// Face-ip geometry of the listed boundary sides whose element has topology
+// TOPO (coordinates relative to the element's node 0): per face ip k the
+// sub-face area vector (tri3d_scs_det / quad3d_scs_det over the side nodes
+// in side order) and dN/dx of the element (face_grad_op) at the exposed-face
+// ip -- on a tri side the face point with weights 11/18 on side node k and
+// 7/36 on the others, on a quad side the bilinear point with weights 9/16
+// (node k), 3/16 (its neighbours), 1/16 (the opposite node), which are the
+// reference's intgExpFace_ entries -- and at side node k itself
+// (shifted_face_grad_op, intgExpFaceShift_).  A non-positive Jacobian at a
+// standard face ip sets *err.
+template <int TOPO>
+__global__ void __launch_bounds__(256) k_side_geo(int n, const int* slist, const int* bf_elem,
+                                                  const int* bf_ord, const int* etype,
+                                                  const int* conn, const double* xyz,
+                                                  const long long* off, double* geo, int* err) {
+    constexpr int npe = topo::Traits<TOPO>::npe;
+    const int i = int(xcd_block() * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const int f = slist[i], e = bf_elem[f];
+    if (etype[e] != TOPO) return;
+    double c[npe][3];
+    const size_t o0 = 3 * size_t(conn[8 * size_t(e)]);
+    for (int j = 0; j < npe; ++j)
+        for (int k = 0; k < 3; ++k) c[j][k] = xyz[3 * size_t(conn[8 * size_t(e) + j]) + k] - xyz[o0 + k];
+    const int* sn = c_sides[TOPO][bf_ord[f]];
+    const int npf = sn[3] < 0 ? 3 : 4;
+    double fp[4][3] = {};
+    for (int q = 0; q < npf; ++q)
+        for (int k = 0; k < 3; ++k) fp[q][k] = c[sn[q]][k];
+    const double(*vp)[3] = c_vpar[TOPO - 1];
+    int bad = 0;
+    double* rec = geo + off[f];
+#pragma unroll 1
+    for (int fk = 0; fk < npf; ++fk) {
+        double* q = rec + fk * kSideIpStride;
+        double A[3];
+        face_areav(npf, fp, fk, A);
+        for (int d = 0; d < 3; ++d) q[d] = A[d];
+        for (int sh = 0; sh < 2; ++sh) {
+            double l[3] = {0, 0, 0};
+            for (int qn = 0; qn < npf; ++qn) {
+                double w;
+                if (sh) w = qn == fk ? 1.0 : 0.0;
+                else if (npf == 3) w = qn == fk ? 11.0 / 18.0 : 7.0 / 36.0;
+                else {
+                    const int dq = (qn - fk + 4) & 3;
+                    w = dq == 0 ? 9.0 / 16.0 : dq == 2 ? 1.0 / 16.0 : 3.0 / 16.0;
+                }
+                for (int d = 0; d < 3; ++d) l[d] += w * vp[sn[qn]][d];
+            }
+            double dd[npe][3], g[npe][3];
+            int bsh = 0;
+            topo::deriv<TOPO>(sh, l[0], l[1], l[2], dd);
+            topo::gradop<npe>(dd, c, g, &bsh);
+            // the shifted ip at a pyramid's apex is singular (det J = 0, a
+            // zero operator from the fallback); only the standard ips judge
+            // the element
+            if (!sh) bad |= bsh;
+            for (int j = 0; j < npe; ++j)
+                for (int k = 0; k < 3; ++k) q[3 + 18 * sh + 3 * j + k] = g[j][k];
+        }
+    }
+    if (bad) *err = 1;
+}
+
+void launch_side_geo(DevMesh& m, hipStream_t s) {
+    if (!m.nslist) return;
+    const unsigned B = 256, G = xcd_grid(size_t(m.nslist), B);
+    k_side_geo<topo::kTet><<<G, B, 0, s>>>(m.nslist, m.slist.p, m.bf_elem.p, m.bf_ord.p, m.etype.p,
+                                           m.conn.p, m.xyz.p, m.sgeo_off.p, m.sgeo.p, m.err.p);
+    k_side_geo<topo::kWed><<<G, B, 0, s>>>(m.nslist, m.slist.p, m.bf_elem.p, m.bf_ord.p, m.etype.p,
+                                           m.conn.p, m.xyz.p, m.sgeo_off.p, m.sgeo.p, m.err.p);
+    k_side_geo<topo::kPyr><<<G, B, 0, s>>>(m.nslist, m.slist.p, m.bf_elem.p, m.bf_ord.p, m.etype.p,
+                                           m.conn.p, m.xyz.p, m.sgeo_off.p, m.sgeo.p, m.err.p);
     OAGPU_HIP(hipGetLastError());
 }
 

@@ -233,17 +233,56 @@ _HEX_SIDES = np.array([[0, 1, 5, 4], [1, 2, 6, 5], [2, 3, 7, 6], [0, 4, 7, 3], [
 _CORNERS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1],
                      [0, 1, 1]])
 # side -> neighbour cube offset
+# side-node ordinals of every topology (side order, outward), the tables of
+# the reference's master elements (HexSCS, TetSCS, WedSCS, PyrSCS)
+SIDES = {HEX8: [[0, 1, 5, 4], [1, 2, 6, 5], [2, 3, 7, 6], [0, 4, 7, 3], [0, 3, 2, 1], [4, 5, 6, 7]],
+         TET4: [[0, 1, 3], [1, 2, 3], [0, 3, 2], [0, 2, 1]],
+         WED6: [[0, 1, 4, 3], [1, 2, 5, 4], [0, 3, 5, 2], [0, 2, 1], [3, 4, 5]],
+         PYR5: [[0, 1, 4], [1, 2, 4], [2, 3, 4], [0, 4, 3], [0, 3, 2, 1]]}
+
+
+def boundary_sides(conn: np.ndarray, etype=None):
+    """Sides that belong to one element only: (element, side ordinal) pairs
+    in element order, and the sides' node ids ([n][4], -1 padded)."""
+    conn = np.asarray(conn)
+    et = np.zeros(conn.shape[0], dtype=np.int32) if etype is None else np.asarray(etype)
+    keys, elems, ords, nodes = [], [], [], []
+    for t, sides in SIDES.items():
+        sel = np.where(et == t)[0]
+        if sel.size == 0:
+            continue
+        for o, sn in enumerate(sides):
+            nd = conn[sel][:, sn]
+            k = np.sort(nd, axis=1)
+            if k.shape[1] == 3:
+                k = np.concatenate([np.full((k.shape[0], 1), -1), k], axis=1)
+                nd = np.concatenate([nd, np.full((nd.shape[0], 1), -1)], axis=1)
+            keys.append(k)
+            elems.append(sel)
+            ords.append(np.full(sel.size, o))
+            nodes.append(nd)
+    keys, elems = np.concatenate(keys), np.concatenate(elems)
+    ords, nodes = np.concatenate(ords), np.concatenate(nodes)
+    _, inv, cnt = np.unique(keys, axis=0, return_inverse=True, return_counts=True)
+    once = cnt[inv.ravel()] == 1
+    order = np.lexsort((ords[once], elems[once]))
+    return elems[once][order], ords[once][order], nodes[once][order]
+
+
 _SIDE_NB = np.array([[0, -1, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, 0, -1], [0, 0, 1]])
 
 
 def duct_mixed(m: int, layers=(187, 8, 25, 75, 25), length: float = 1.0, width: float = 0.2,
-               jitter: float = 0.1, seed: int = 0):
+               jitter: float = 0.1, seed: int = 0, hex_ring: bool = False):
     """Mixed-topology duct [0, length] x [0, width]^2 of cubes (m across,
     sum(layers) along x with length / sum(layers) == width / m): along x the
     cube layers are hex8 (inlet side), pyramids (each cube 6 pyramids about
     a centre node), tets, wedges, tets and hex8 again (outlet side), layers =
-    (hex total, pyramid, tet, wedge, tet); the outer ring of cubes is hex8
-    everywhere, so every boundary face is a quad side of a hex8.
+    (hex total, pyramid, tet, wedge, tet).  The walls and the ends are sides
+    of whatever elements touch them: quad sides of hex8, wedge6 and pyramid5,
+    tri sides of tet4 (and of wedge6 where a wedge layer ends the duct).
+    hex_ring: the outer ring of cubes is hex8 everywhere instead, so every
+    boundary side is a quad side of a hex8.
 
     Tet cubes: each side split into two triangles (the diagonal through its
     lowest node id, the same from both cubes sharing it), each triangle
@@ -270,8 +309,9 @@ def duct_mixed(m: int, layers=(187, 8, 25, 75, 25), length: float = 1.0, width: 
     xtype = np.array([H] * nh1 + [P] * npyr + [T] * nt1 + [W] * nw + [T] * nt2 + [H] * (nh - nh1))
     k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     ctype = xtype[i]
-    ring = (j == 0) | (j == ny - 1) | (k == 0) | (k == nz - 1)
-    ctype = np.where(ring, H, ctype)
+    if hex_ring:
+        ring = (j == 0) | (j == ny - 1) | (k == 0) | (k == nz - 1)
+        ctype = np.where(ring, H, ctype)
     # grid nodes
     g1 = nx + 1
     gid = lambda ii, jj, kk: ii + g1 * (jj + (ny + 1) * kk)
@@ -322,7 +362,7 @@ def duct_mixed(m: int, layers=(187, 8, 25, 75, 25), length: float = 1.0, width: 
             apex = cnode[sel]
             if cube_t == T:
                 nb = nbt[sel, side]
-                tri = (nb == T) | ((nb == W) & ((side == 1) | (side == 3)))
+                tri = (nb == T) | (nb == -1) | ((nb == W) & ((side == 1) | (side == 3)))
             else:
                 tri = np.zeros(sel.size, dtype=bool)
             pq = q[~tri]
@@ -354,20 +394,13 @@ def duct_mixed(m: int, layers=(187, 8, 25, 75, 25), length: float = 1.0, width: 
         d[nodes] *= 0.5
         xyz[:ng] = xyz0 + d
         xyz[ng:] = xyz[corner[has_c]].mean(axis=1)
-    # boundary faces: sides of hex cubes on the domain boundary
-    faces = []
-    for side, (di, dj, dk) in enumerate(_SIDE_NB):
-        out = nbt[:, side] == -1
-        e = hex_eid[out]
-        assert np.all(e >= 0)
-        if side == 3:
-            bc = np.full(e.shape, DUCT_INLET)
-        elif side == 1:
-            bc = np.full(e.shape, DUCT_OUTLET)
-        else:
-            bc = np.full(e.shape, DUCT_WALL)
-        faces.append(np.stack([e, np.full_like(e, side), bc], axis=1))
-    bf = np.concatenate(faces).astype(np.int32)
+    # boundary sides: sides of one element only, typed by the plane they lie on
+    be, bo, bn = boundary_sides(conn, etype)
+    xs = np.where(bn >= 0, xyz[np.maximum(bn, 0), 0], np.nan)
+    bc = np.full(be.shape, DUCT_WALL)
+    bc[np.nanmax(np.abs(xs), axis=1) < 1e-9 * length] = DUCT_INLET
+    bc[np.nanmin(xs, axis=1) > length * (1 - 1e-9)] = DUCT_OUTLET
+    bf = np.stack([be, bo, bc], axis=1).astype(np.int32)
     return {
         "xyz": np.ascontiguousarray(xyz),
         "conn": np.ascontiguousarray(conn),

@@ -4,10 +4,14 @@
  *
  * TEST INFRASTRUCTURE ONLY (see oracle.h).
  *
+ * Sides of every topology: quad4 sides of hex8 / wedge6 / pyramid5 and tri3
+ * sides of tet4 / wedge6 / pyramid5, one face ip per side node (side order).
+ *
  * Inlet: specified velocity (per boundary id, the bc_U row), zero-gradient
  * pressure, specified k / omega.  Outlet: specified static pressure,
  * zero-gradient velocity and k / omega, with the flow-reversal flag of each
- * face ip.  Side mass flow rates live per boundary face ip ([nbf][4]).
+ * face ip.  Side mass flow rates live per boundary face ip ([nbf][4]; a tri
+ * side leaves the fourth unused).
  *
  * References (src/):
  *   side mass flow init   model/flow/flowModel.cpp:3802-4997
@@ -39,35 +43,32 @@
 
 static int face_type(const or_case *c, int f) { return c->bc_type[c->bf_bc[f]]; }
 
-/* element nodes, face nodes (side-ordinal order) and coordinates */
-static void face_gather(const or_case *c, int f, const int **en, int fn[4], double co[24])
+/* the side, its face shape functions (npf x npf) and the element coordinates */
+static void side_gather(const or_case *c, int f, or_side *sd, double co[24])
 {
-    const int *sn = or_hex_side_nodes();
-    *en = &c->conn[8 * c->bf_elem[f]];
-    for (int k = 0; k < 4; ++k) fn[k] = (*en)[sn[4 * c->bf_ord[f] + k]];
-    or_gather_coords(c, c->bf_elem[f], co);
+    or_side_get(c, f, sd);
+    or_gather_coords(c, sd->e, co);
 }
 
 /* ------------------------------------------------------ side mass flow */
 void or_bc_mdot_init(or_case *c)
 {
-    double fsf[16];
-    or_quad_face_shape(c->prm.u_shifted, fsf);
     for (int f = 0; f < c->nbf; ++f) {
         const int t = face_type(c, f);
         if (t != OR_BC_INLET && t != OR_BC_OUTLET) continue;
-        const int *en;
-        int fn[4];
-        double co[24];
-        face_gather(c, f, &en, fn, co);
+        or_side sd;
+        double co[24], fsf[16];
+        side_gather(c, f, &sd, co);
+        or_side_shape(sd.npf, c->prm.u_shifted, fsf);
+        const int npf = sd.npf;
         const double *Ubc = &c->bc_U[3 * c->bf_bc[f]];
-        for (int ip = 0; ip < 4; ++ip) {
+        for (int ip = 0; ip < npf; ++ip) {
             const double *ax = &c->bareav[12 * f + 3 * ip];
             double rip = 0, u[3] = {0, 0, 0};
-            for (int ic = 0; ic < 4; ++ic) {
-                const double r = fsf[4 * ip + ic];
-                rip += r * c->rho[fn[ic]];
-                for (int j = 0; j < 3; ++j) u[j] += r * c->U[3 * fn[ic] + j];
+            for (int ic = 0; ic < npf; ++ic) {
+                const double r = fsf[npf * ip + ic];
+                rip += r * c->rho[sd.fn[ic]];
+                for (int j = 0; j < 3; ++j) u[j] += r * c->U[3 * sd.fn[ic] + j];
             }
             double m = 0;
             for (int j = 0; j < 3; ++j) m += rip * (t == OR_BC_INLET ? Ubc[j] : u[j]) * ax[j];
@@ -76,28 +77,30 @@ void or_bc_mdot_init(or_case *c)
     }
 }
 
-/* Rhie-Chow mass flow at outlet face ip `ip` (duRhs = du: SIMPLE) */
+/* Rhie-Chow mass flow at outlet face ip `ip` (duRhs = du: SIMPLE); the
+ * face-averaged nodal pressure gradient weighs each side node 1/npf */
 static double outlet_rhie_chow(const or_case *c, int f, int ip, const double *fdn,
-                               const double *fsf, const int *en, const int fn[4])
+                               const double *fsf, const or_side *sd)
 {
-    const int *sn = or_hex_side_nodes();
-    const int ord = c->bf_ord[f];
+    const int npf = sd->npf, npe = sd->npe;
     const double pbc = c->bc_p[c->bf_bc[f]];
     double p8[8];
-    for (int a = 0; a < 8; ++a) p8[a] = c->p[en[a]];
-    for (int k = 0; k < 4; ++k) p8[sn[4 * ord + k]] = pbc;  /* nodal side pressure */
+    for (int a = 0; a < npe; ++a) p8[a] = c->p[sd->en[a]];
+    for (int k = 0; k < npf; ++k) p8[sd->sn[k]] = pbc;  /* nodal side pressure */
+    const double fw = 1.0 / npf;
     double rip = 0, u[3] = {0, 0, 0}, du[3] = {0, 0, 0}, Gp[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
-    for (int ic = 0; ic < 4; ++ic) {
-        const double r = fsf[4 * ip + ic];
-        rip += r * c->rho[fn[ic]];
+    for (int ic = 0; ic < npf; ++ic) {
+        const double r = fsf[npf * ip + ic];
+        const int n = sd->fn[ic];
+        rip += r * c->rho[n];
         for (int j = 0; j < 3; ++j) {
-            u[j] += r * c->U[3 * fn[ic] + j];
-            du[j] += r * c->du[3 * fn[ic] + j];
-            Gp[j] += 0.25 * c->gradP[3 * fn[ic] + j];
+            u[j] += r * c->U[3 * n + j];
+            du[j] += r * c->du[3 * n + j];
+            Gp[j] += fw * c->gradP[3 * n + j];
         }
     }
-    for (int ic = 0; ic < 8; ++ic)
-        for (int j = 0; j < 3; ++j) dp[j] += fdn[24 * ip + 3 * ic + j] * p8[ic];
+    for (int ic = 0; ic < npe; ++ic)
+        for (int j = 0; j < 3; ++j) dp[j] += fdn[3 * npe * ip + 3 * ic + j] * p8[ic];
     const double *ax = &c->bareav[12 * f + 3 * ip];
     double m = 0;
     for (int j = 0; j < 3; ++j) m += (rip * u[j] - rip * du[j] * (dp[j] - Gp[j])) * ax[j];
@@ -107,23 +110,22 @@ static double outlet_rhie_chow(const or_case *c, int f, int ip, const double *fd
 void or_bc_mdot_update(or_case *c)
 {
     const double urf = c->prm.urf_mdot;
-    double fsf[16];
-    or_quad_face_shape(c->prm.u_shifted, fsf);
     for (int f = 0; f < c->nbf; ++f) {
         const int t = face_type(c, f);
         if (t != OR_BC_INLET && t != OR_BC_OUTLET) continue;
-        const int *en;
-        int fn[4];
-        double co[24], fdn[96], fdet[4];
-        face_gather(c, f, &en, fn, co);
-        if (t == OR_BC_OUTLET) or_hex_face_gradop(c->prm.pgrad_shifted, c->bf_ord[f], co, fdn, fdet);
+        or_side sd;
+        double co[24], fdn[96], fdet[4], fsf[16];
+        side_gather(c, f, &sd, co);
+        or_side_shape(sd.npf, c->prm.u_shifted, fsf);
+        const int npf = sd.npf;
+        if (t == OR_BC_OUTLET) or_topo_side_gradop(sd.topo, c->prm.pgrad_shifted, sd.ord, co, fdn, fdet);
         const double *Ubc = &c->bc_U[3 * c->bf_bc[f]];
-        for (int ip = 0; ip < 4; ++ip) {
+        for (int ip = 0; ip < npf; ++ip) {
             double *md = &c->msd[4 * f + ip];
             if (t == OR_BC_INLET) {
                 const double *ax = &c->bareav[12 * f + 3 * ip];
                 double rip = 0;
-                for (int ic = 0; ic < 4; ++ic) rip += fsf[4 * ip + ic] * c->rho[fn[ic]];
+                for (int ic = 0; ic < npf; ++ic) rip += fsf[npf * ip + ic] * c->rho[sd.fn[ic]];
                 double m = 0;
                 for (int j = 0; j < 3; ++j) m += (rip * Ubc[j]) * ax[j];
                 *md = urf * m + (1.0 - urf) * *md;
@@ -132,7 +134,7 @@ void or_bc_mdot_update(or_case *c)
                     *md = 0.0;
                     continue;
                 }
-                const double m = outlet_rhie_chow(c, f, ip, fdn, fsf, en, fn);
+                const double m = outlet_rhie_chow(c, f, ip, fdn, fsf, &sd);
                 *md = urf * m + (1.0 - urf) * *md;
             }
         }
@@ -143,25 +145,25 @@ void or_bc_reversal(or_case *c)
 {
     for (int f = 0; f < c->nbf; ++f) {
         if (face_type(c, f) != OR_BC_OUTLET) continue;
-        const int *en;
-        int fn[4];
-        double co[24];
-        face_gather(c, f, &en, fn, co);
+        or_side sd;
+        or_side_get(c, f, &sd);
+        const int npf = sd.npf;
+        const double fw = 1.0 / npf;
         double *md = &c->msd[4 * f];
         int *rf = &c->rfl[4 * f];
         double side = 0;
-        for (int ip = 0; ip < 4; ++ip) {
+        for (int ip = 0; ip < npf; ++ip) {
             side += md[ip];
             md[ip] = fmax(md[ip], 0.0);
         }
         const double pbc = c->bc_p[c->bf_bc[f]];
-        for (int ip = 0; ip < 4; ++ip) {
+        for (int ip = 0; ip < npf; ++ip) {
             const double *ax = &c->bareav[12 * f + 3 * ip];
             const double amag = sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
             double pavg = 0, uavg[3] = {0, 0, 0};
-            for (int ic = 0; ic < 4; ++ic) {
-                pavg += 0.25 * c->p[fn[ic]];
-                for (int j = 0; j < 3; ++j) uavg[j] += 0.25 * c->U[3 * fn[ic] + j];
+            for (int ic = 0; ic < npf; ++ic) {
+                pavg += fw * c->p[sd.fn[ic]];
+                for (int j = 0; j < 3; ++j) uavg[j] += fw * c->U[3 * sd.fn[ic] + j];
             }
             double vn = 0;
             for (int j = 0; j < 3; ++j) vn += uavg[j] * ax[j] / amag;
@@ -179,52 +181,52 @@ void or_bc_reversal(or_case *c)
 
 void or_bc_divergence(or_case *c)
 {
-    const int *sn = or_hex_side_nodes();
     for (int f = 0; f < c->nbf; ++f) {
         const int t = face_type(c, f);
         if (t != OR_BC_INLET && t != OR_BC_OUTLET) continue;
-        const int *en = &c->conn[8 * c->bf_elem[f]];
-        for (int ip = 0; ip < 4; ++ip) c->div[en[sn[4 * c->bf_ord[f] + ip]]] += c->msd[4 * f + ip];
+        or_side sd;
+        or_side_get(c, f, &sd);
+        for (int ip = 0; ip < sd.npf; ++ip) c->div[sd.fn[ip]] += c->msd[4 * f + ip];
     }
 }
 
 /* ------------------------------------------------------------ momentum */
+/* element lhs [3 npe][3 npe] (row-major), rhs [3 npe] */
 void or_bc_momentum(or_case *c, int f, double *lhs, double *rhs)
 {
     const or_params *P = &c->prm;
-    const int *sn = or_hex_side_nodes();
-    const int ord = c->bf_ord[f], t = face_type(c, f);
-    const int *en;
-    int fn[4];
+    const int t = face_type(c, f);
+    or_side sd;
     double co[24], fdn[96], fdet[4], fsf[16], Uw[24], bcm[8], muf[4];
-    face_gather(c, f, &en, fn, co);
-    or_quad_face_shape(P->u_shifted, fsf);
-    or_hex_face_gradop(P->ugrad_shifted, ord, co, fdn, fdet);
+    side_gather(c, f, &sd, co);
+    const int npf = sd.npf, npe = sd.npe, ld = 3 * npe;
+    or_side_shape(npf, P->u_shifted, fsf);
+    or_topo_side_gradop(sd.topo, P->ugrad_shifted, sd.ord, co, fdn, fdet);
     const double *Ubc = &c->bc_U[3 * c->bf_bc[f]];
-    for (int a = 0; a < 8; ++a) {
+    for (int a = 0; a < npe; ++a) {
         bcm[a] = 1.0;
-        for (int d = 0; d < 3; ++d) Uw[3 * a + d] = c->U[3 * en[a] + d];
+        for (int d = 0; d < 3; ++d) Uw[3 * a + d] = c->U[3 * sd.en[a] + d];
     }
-    for (int k = 0; k < 4; ++k) {
-        muf[k] = c->mu[fn[k]];
+    for (int k = 0; k < npf; ++k) {
+        muf[k] = c->mu[sd.fn[k]];
         if (t == OR_BC_INLET) {
-            const int ic = sn[4 * ord + k];
+            const int ic = sd.sn[k];
             bcm[ic] = 0.0;
             for (int d = 0; d < 3; ++d) Uw[3 * ic + d] = Ubc[d];
         }
     }
-    for (int ip = 0; ip < 4; ++ip) {
-        const int nearest = sn[4 * ord + ip];
+    for (int ip = 0; ip < npf; ++ip) {
+        const int nearest = sd.sn[ip];
         const double *ax = &c->bareav[12 * f + 3 * ip];
         const double tm = c->msd[4 * f + ip];
         double mub = 0;
-        for (int ic = 0; ic < 4; ++ic) mub += fsf[4 * ip + ic] * muf[ic];
+        for (int ic = 0; ic < npf; ++ic) mub += fsf[npf * ip + ic] * muf[ic];
         if (t == OR_BC_INLET) {
             for (int i = 0; i < 3; ++i) rhs[3 * nearest + i] -= tm * Ubc[i];
-            for (int ic = 0; ic < 8; ++ic) {
-                const double *dn = &fdn[24 * ip + 3 * ic];
+            for (int ic = 0; ic < npe; ++ic) {
+                const double *dn = &fdn[3 * npe * ip + 3 * ic];
                 for (int i = 0; i < 3; ++i) {
-                    const int iR = 3 * nearest + i, rowR = iR * 24;
+                    const int iR = 3 * nearest + i, rowR = iR * ld;
                     for (int j = 0; j < 3; ++j) {
                         double lf = -mub * dn[j] * ax[j];
                         lhs[rowR + 3 * ic + i] += lf * bcm[ic];
@@ -241,17 +243,17 @@ void or_bc_momentum(or_case *c, int f, double *lhs, double *rhs)
         for (int i = 0; i < 3; ++i) {
             const int iR = 3 * nearest + i;
             rhs[iR] -= tm * Uw[3 * nearest + i];
-            lhs[iR * 24 + 3 * nearest + i] += tm;
+            lhs[iR * ld + 3 * nearest + i] += tm;
         }
         const double amag = sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
         double nx[3];
         for (int i = 0; i < 3; ++i) nx[i] = ax[i] / amag;
-        for (int ic = 0; ic < 8; ++ic) {
-            const double *dn = &fdn[24 * ip + 3 * ic];
+        for (int ic = 0; ic < npe; ++ic) {
+            const double *dn = &fdn[3 * npe * ip + 3 * ic];
             for (int j = 0; j < 3; ++j) {
                 const double axj = ax[j], dnj = dn[j], uxj = Uw[3 * ic + j];
                 for (int i = 0; i < 3; ++i) {
-                    const int iR = 3 * nearest + i, rowR = iR * 24;
+                    const int iR = 3 * nearest + i, rowR = iR * ld;
                     const double dni = dn[i], uxi = Uw[3 * ic + i], nxi = nx[i];
                     const double om = 1.0 - nxi * nxi;
                     double lf = -mub * dnj * axj * om;
@@ -277,25 +279,25 @@ void or_bc_momentum(or_case *c, int f, double *lhs, double *rhs)
 }
 
 /* --------------------------------------------------- pressure correction */
+/* element lhs [npe][npe], rhs [npe] */
 void or_bc_pcorr(or_case *c, int f, double *lhs, double *rhs)
 {
     const or_params *P = &c->prm;
-    const int *sn = or_hex_side_nodes();
-    const int ord = c->bf_ord[f], t = face_type(c, f);
-    const int *en;
-    int fn[4];
+    const int t = face_type(c, f);
+    or_side sd;
     double co[24], fdn[96], fdet[4], fsf[16], bcm[8];
-    face_gather(c, f, &en, fn, co);
-    or_quad_face_shape(P->u_shifted, fsf);
+    side_gather(c, f, &sd, co);
+    const int npf = sd.npf, npe = sd.npe;
+    or_side_shape(npf, P->u_shifted, fsf);
     const double *Ubc = &c->bc_U[3 * c->bf_bc[f]];
-    if (t == OR_BC_OUTLET) or_hex_face_gradop(P->pgrad_shifted, ord, co, fdn, fdet);
-    for (int a = 0; a < 8; ++a) bcm[a] = 1.0;
-    for (int k = 0; k < 4; ++k) bcm[sn[4 * ord + k]] = 0.0;
-    for (int ip = 0; ip < 4; ++ip) {
-        const int nearest = sn[4 * ord + ip];
+    if (t == OR_BC_OUTLET) or_topo_side_gradop(sd.topo, P->pgrad_shifted, sd.ord, co, fdn, fdet);
+    for (int a = 0; a < npe; ++a) bcm[a] = 1.0;
+    for (int k = 0; k < npf; ++k) bcm[sd.sn[k]] = 0.0;
+    for (int ip = 0; ip < npf; ++ip) {
+        const int nearest = sd.sn[ip];
         const double *ax = &c->bareav[12 * f + 3 * ip];
         double rip = 0;
-        for (int ic = 0; ic < 4; ++ic) rip += fsf[4 * ip + ic] * c->rho[fn[ic]];
+        for (int ic = 0; ic < npf; ++ic) rip += fsf[npf * ip + ic] * c->rho[sd.fn[ic]];
         if (t == OR_BC_INLET) {
             double m = 0;
             for (int j = 0; j < 3; ++j) m += (rip * Ubc[j]) * ax[j];
@@ -304,58 +306,59 @@ void or_bc_pcorr(or_case *c, int f, double *lhs, double *rhs)
         }
         if (c->rfl[4 * f + ip] != 0) continue;  /* reversed: no mass flux (static mesh) */
         double du[3] = {0, 0, 0};
-        for (int ic = 0; ic < 4; ++ic)
-            for (int j = 0; j < 3; ++j) du[j] += fsf[4 * ip + ic] * c->du[3 * fn[ic] + j];
-        for (int ic = 0; ic < 8; ++ic) {
+        for (int ic = 0; ic < npf; ++ic)
+            for (int j = 0; j < 3; ++j) du[j] += fsf[npf * ip + ic] * c->du[3 * sd.fn[ic] + j];
+        for (int ic = 0; ic < npe; ++ic) {
             double lf = 0;
-            for (int j = 0; j < 3; ++j) lf += -rip * du[j] * fdn[24 * ip + 3 * ic + j] * ax[j];
-            lhs[8 * nearest + ic] += lf * bcm[ic];
+            for (int j = 0; j < 3; ++j) lf += -rip * du[j] * fdn[3 * npe * ip + 3 * ic + j] * ax[j];
+            lhs[npe * nearest + ic] += lf * bcm[ic];
         }
-        rhs[nearest] -= outlet_rhie_chow(c, f, ip, fdn, fsf, en, fn);
+        rhs[nearest] -= outlet_rhie_chow(c, f, ip, fdn, fsf, &sd);
     }
 }
 
 /* -------------------------------------------------------- k and omega */
+/* element lhs [npe][npe], rhs [npe]; G8: the diffusivity at the element's nodes */
 void or_bc_scalar(or_case *c, int which, int f, const double *G8, double *lhs, double *rhs)
 {
     const or_sst_params *S = &c->sst;
-    const int *sn = or_hex_side_nodes();
-    const int ord = c->bf_ord[f], t = face_type(c, f);
+    const int t = face_type(c, f);
     const double *phiF = which == 0 ? c->k : c->w;
-    const int *en;
-    int fn[4];
+    or_side sd;
     double co[24], fdn[96], fdet[4], fsf[16], bcm[8], ph[8];
-    face_gather(c, f, &en, fn, co);
+    side_gather(c, f, &sd, co);
+    const int npf = sd.npf, npe = sd.npe;
     const double pbc = which == 0 ? c->bc_k[c->bf_bc[f]] : c->bc_w[c->bf_bc[f]];
-    for (int a = 0; a < 8; ++a) {
+    for (int a = 0; a < npe; ++a) {
         bcm[a] = 1.0;
-        ph[a] = phiF[en[a]];
+        ph[a] = phiF[sd.en[a]];
     }
-    for (int k = 0; k < 4; ++k) {
-        bcm[sn[4 * ord + k]] = 0.0;
-        ph[sn[4 * ord + k]] = pbc;
+    for (int k = 0; k < npf; ++k) {
+        bcm[sd.sn[k]] = 0.0;
+        ph[sd.sn[k]] = pbc;
     }
     if (t == OR_BC_INLET) {
-        or_quad_face_shape(which == 0 ? S->k_shifted : S->w_shifted, fsf);
-        or_hex_face_gradop(which == 0 ? S->kgrad_shifted : S->wgrad_shifted, ord, co, fdn, fdet);
+        or_side_shape(npf, which == 0 ? S->k_shifted : S->w_shifted, fsf);
+        or_topo_side_gradop(sd.topo, which == 0 ? S->kgrad_shifted : S->wgrad_shifted, sd.ord, co,
+                            fdn, fdet);
     }
-    for (int ip = 0; ip < 4; ++ip) {
+    for (int ip = 0; ip < npf; ++ip) {
         if (c->rfl[4 * f + ip] != 0) continue;
-        const int nearest = sn[4 * ord + ip];
+        const int nearest = sd.sn[ip];
         const double tm = c->msd[4 * f + ip];
         if (t == OR_BC_OUTLET) {
-            rhs[nearest] -= tm * phiF[en[nearest]];
-            lhs[8 * nearest + nearest] += tm;
+            rhs[nearest] -= tm * phiF[sd.en[nearest]];
+            lhs[npe * nearest + nearest] += tm;
             continue;
         }
         rhs[nearest] -= tm * pbc;
         double Gb = 0;
-        for (int ic = 0; ic < 4; ++ic) Gb += fsf[4 * ip + ic] * G8[sn[4 * ord + ic]];
+        for (int ic = 0; ic < npf; ++ic) Gb += fsf[npf * ip + ic] * G8[sd.sn[ic]];
         const double *ax = &c->bareav[12 * f + 3 * ip];
-        for (int ic = 0; ic < 8; ++ic)
+        for (int ic = 0; ic < npe; ++ic)
             for (int j = 0; j < 3; ++j) {
-                const double lf = -Gb * fdn[24 * ip + 3 * ic + j] * ax[j];
-                lhs[8 * nearest + ic] += lf * bcm[ic];
+                const double lf = -Gb * fdn[3 * npe * ip + 3 * ic + j] * ax[j];
+                lhs[npe * nearest + ic] += lf * bcm[ic];
                 rhs[nearest] -= lf * ph[ic];
             }
     }

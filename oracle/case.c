@@ -100,8 +100,11 @@ or_case *or_case_create(int nn, int ne, const double *xyz, const int *conn,
     if (etype) {
         for (int e = 0; e < ne; ++e)
             if (etype[e] < OR_HEX8 || etype[e] > OR_PYR5) return NULL;
-        for (int f = 0; f < nbf; ++f)
-            if (etype[bf_elem[f]] != OR_HEX8) return NULL; /* quad sides of hex8 only */
+    }
+    for (int f = 0; f < nbf; ++f) {
+        if (bf_elem[f] < 0 || bf_elem[f] >= ne) return NULL;
+        const int t = etype ? etype[bf_elem[f]] : OR_HEX8;
+        if (bf_ord[f] < 0 || bf_ord[f] >= or_topo_nsides(t)) return NULL;
     }
     or_case *c = (or_case *)xcalloc(1, sizeof(or_case));
     c->nn = nn;
@@ -169,17 +172,20 @@ or_case *or_case_create(int nn, int ne, const double *xyz, const int *conn,
         c->rho[i] = prm->rho;
         c->mu[i] = prm->mu;
     }
-    /* exposed area vectors from the face nodes in side-ordinal order */
-    const int *sn = or_hex_side_nodes();
+    /* exposed area vectors from the face nodes in side-ordinal order
+     * (Quad3DSCS / Tri3DSCS determinant) */
     for (int f = 0; f < nbf; ++f) {
-        const int *en = &c->conn[8 * c->bf_elem[f]];
+        or_side sd;
+        or_side_get(c, f, &sd);
         double fc[12];
-        for (int k = 0; k < 4; ++k) {
-            const int nd = en[sn[4 * c->bf_ord[f] + k]];
-            for (int d = 0; d < 3; ++d) fc[3 * k + d] = c->xyz[3 * nd + d];
-            c->bnode[nd] = 1;
+        for (int k = 0; k < sd.npf; ++k) {
+            for (int d = 0; d < 3; ++d) fc[3 * k + d] = c->xyz[3 * sd.fn[k] + d];
+            c->bnode[sd.fn[k]] = 1;
         }
-        or_quad_face_areav(fc, &c->bareav[12 * f]);
+        if (sd.npf == 3)
+            or_tri_face_areav(fc, &c->bareav[12 * f]);
+        else
+            or_quad_face_areav(fc, &c->bareav[12 * f]);
     }
     memcpy(c->wnode, c->bnode, sizeof(int) * (size_t)nn);
     or_dual_volume(c);
@@ -223,12 +229,12 @@ int or_case_set_bc_types(or_case *c, const int *type)
     c->has_outlet = 0;
     for (int f = 0; f < c->nbf; ++f)
         if (c->bc_type[c->bf_bc[f]] == OR_BC_OUTLET) c->has_outlet = 1;
-    const int *sn = or_hex_side_nodes();
     for (int f = 0; f < c->nbf; ++f) {
-        const int *en = &c->conn[8 * c->bf_elem[f]];
+        or_side sd;
+        or_side_get(c, f, &sd);
         const int t = c->bc_type[c->bf_bc[f]];
-        for (int k = 0; k < 4; ++k) {
-            const int nd = en[sn[4 * c->bf_ord[f] + k]];
+        for (int k = 0; k < sd.npf; ++k) {
+            const int nd = sd.fn[k];
             if (t == OR_BC_WALL) c->wnode[nd] = 1;
             if (t == OR_BC_SYMMETRY) {
                 for (int d = 0; d < 3; ++d) c->symA[3 * nd + d] += c->bareav[12 * f + 3 * k + d];
@@ -485,7 +491,6 @@ void or_gradient(or_case *c, int which)
 void or_gradient_field(or_case *c, const double *phi, double *grad, int N, int shifted)
 {
     const double gurf = c->prm.grad_urf;
-    const int *sn = or_hex_side_nodes();
     /* relax the existing gradient (nodeField.hpp:3268, relax :6362) */
     for (size_t k = 0; k < (size_t)c->nn * 3 * N; ++k) grad[k] *= (1.0 - gurf);
     or_tables tab[4];
@@ -519,19 +524,20 @@ void or_gradient_field(or_case *c, const double *phi, double *grad, int N, int s
     }
     }
     /* boundary ip contribution (walls; symmetry planes are skipped) */
-    double fsf[16];
-    or_quad_face_shape(shifted, fsf);
     for (int f = 0; f < c->nbf; ++f) {
         if (c->bc_type[c->bf_bc[f]] == OR_BC_SYMMETRY) continue;
-        const int *en = &c->conn[8 * c->bf_elem[f]];
-        int fn[4];
-        for (int k = 0; k < 4; ++k) fn[k] = en[sn[4 * c->bf_ord[f] + k]];
+        or_side sd;
+        or_side_get(c, f, &sd);
+        const int npf = sd.npf;
+        const int *fn = sd.fn;
+        double fsf[16];
+        or_side_shape(npf, shifted, fsf);
         const double *aw = &c->bareav[12 * f];
-        for (int ip = 0; ip < 4; ++ip) {
+        for (int ip = 0; ip < npf; ++ip) {
             const int nn_ = fn[ip];
             double pip[3] = {0, 0, 0};
-            for (int ic = 0; ic < 4; ++ic)
-                for (int j = 0; j < N; ++j) pip[j] += fsf[4 * ip + ic] * phi[N * fn[ic] + j];
+            for (int ic = 0; ic < npf; ++ic)
+                for (int j = 0; j < N; ++j) pip[j] += fsf[npf * ip + ic] * phi[N * fn[ic] + j];
             const double iv = 1.0 / c->vol[nn_];
             for (int i = 0; i < N; ++i)
                 for (int j = 0; j < 3; ++j)
@@ -661,43 +667,42 @@ void or_assemble_momentum(or_case *c)
 
     /* no-slip walls (navierStokesAssemblerElemBoundaryConditions.cpp:2659-2884)
      * and symmetry planes (:2082-2375, IP_FULL_STRESS__ :296-343 with the
-     * incompressible divU term zero) */
-    const int *sn = or_hex_side_nodes();
-    double fsf[16];
-    or_quad_face_shape(P->u_shifted, fsf);
+     * incompressible divU term zero); sides of every topology (face_grad_op
+     * of the parent element at the side's face ips) */
     double co[24];
     for (int f = 0; f < c->nbf; ++f) {
-        const int e = c->bf_elem[f], ord = c->bf_ord[f];
         const int btype = c->bc_type[c->bf_bc[f]];
+        or_side sd;
+        or_side_get(c, f, &sd);
+        const int npe = sd.npe, npf = sd.npf, ld = 3 * npe;
+        const int *en = sd.en;
+        for (int a = 0; a < npe; ++a) nodes[a] = en[a];
+        memset(lhs, 0, sizeof(lhs));
+        memset(rhs, 0, sizeof(rhs));
         if (btype == OR_BC_INLET || btype == OR_BC_OUTLET) {
-            memset(lhs, 0, sizeof(lhs));
-            memset(rhs, 0, sizeof(rhs));
-            for (int a = 0; a < 8; ++a) nodes[a] = c->conn[8 * e + a];
             or_bc_momentum(c, f, lhs, rhs);
-            or_apply_coeff(3, 8, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
+            or_apply_coeff(3, npe, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
             continue;
         }
+        double fsf[16];
+        or_side_shape(npf, P->u_shifted, fsf);
+        gather_coords(c, sd.e, co);
+        double fdn[96], fdet[4];
+        or_topo_side_gradop(sd.topo, P->ugrad_shifted, sd.ord, co, fdn, fdet);
+        const double *aw = &c->bareav[12 * f];
+        double muf[4];
+        for (int k = 0; k < npf; ++k) muf[k] = c->mu[sd.fn[k]];
         if (btype == OR_BC_SYMMETRY) {
-            const int *en = &c->conn[8 * e];
-            memset(lhs, 0, sizeof(lhs));
-            memset(rhs, 0, sizeof(rhs));
-            double muf[4];
-            for (int a = 0; a < 8; ++a) nodes[a] = en[a];
-            for (int k = 0; k < 4; ++k) muf[k] = c->mu[en[sn[4 * ord + k]]];
-            gather_coords(c, e, co);
-            double fdn[96], fdet[4];
-            or_hex_face_gradop(P->ugrad_shifted, ord, co, fdn, fdet);
-            const double *aw = &c->bareav[12 * f];
-            for (int ip = 0; ip < 4; ++ip) {
-                const int nearest = sn[4 * ord + ip];
+            for (int ip = 0; ip < npf; ++ip) {
+                const int nearest = sd.sn[ip];
                 const double *ax = &aw[3 * ip];
                 double mub = 0;
-                for (int ic = 0; ic < 4; ++ic) mub += fsf[4 * ip + ic] * muf[ic];
-                for (int ic = 0; ic < 8; ++ic) {
-                    const double *dn = &fdn[24 * ip + 3 * ic];
+                for (int ic = 0; ic < npf; ++ic) mub += fsf[npf * ip + ic] * muf[ic];
+                for (int ic = 0; ic < npe; ++ic) {
+                    const double *dn = &fdn[3 * npe * ip + 3 * ic];
                     const double *u = &c->U[3 * en[ic]];
                     for (int i = 0; i < 3; ++i) {
-                        const int iR = 3 * nearest + i, rowR = iR * 24;
+                        const int iR = 3 * nearest + i, rowR = iR * ld;
                         for (int j = 0; j < 3; ++j) {
                             double lf = -mub * dn[j] * ax[j];
                             lhs[rowR + 3 * ic + i] += lf;
@@ -709,43 +714,34 @@ void or_assemble_momentum(or_case *c)
                     }
                 }
             }
-            or_apply_coeff(3, 8, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
+            or_apply_coeff(3, npe, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
             continue;
         }
-        const int *en = &c->conn[8 * e];
-        memset(lhs, 0, sizeof(lhs));
-        memset(rhs, 0, sizeof(rhs));
-        double Uw[24], bcm[8], muf[4];
-        for (int a = 0; a < 8; ++a) {
-            nodes[a] = en[a];
+        double Uw[24], bcm[8];
+        for (int a = 0; a < npe; ++a) {
             bcm[a] = 1.0;
             for (int d = 0; d < 3; ++d) Uw[3 * a + d] = c->U[3 * en[a] + d];
         }
         const double *Ubc = &c->bc_U[3 * c->bf_bc[f]];
-        for (int k = 0; k < 4; ++k) {
-            const int ic = sn[4 * ord + k];
-            muf[k] = c->mu[en[ic]];
+        for (int k = 0; k < npf; ++k) {
+            const int ic = sd.sn[k];
             bcm[ic] = 0.0;
             for (int d = 0; d < 3; ++d) Uw[3 * ic + d] = Ubc[d];
         }
-        gather_coords(c, e, co);
-        double fdn[96], fdet[4];
-        or_hex_face_gradop(P->ugrad_shifted, ord, co, fdn, fdet);
-        const double *aw = &c->bareav[12 * f];
-        for (int ip = 0; ip < 4; ++ip) {
-            const int nearest = sn[4 * ord + ip];
+        for (int ip = 0; ip < npf; ++ip) {
+            const int nearest = sd.sn[ip];
             const double *ax = &aw[3 * ip];
             const double amag = sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
             double mub = 0;
-            for (int ic = 0; ic < 4; ++ic) mub += fsf[4 * ip + ic] * muf[ic];
+            for (int ic = 0; ic < npf; ++ic) mub += fsf[npf * ip + ic] * muf[ic];
             double nx[3];
             for (int i = 0; i < 3; ++i) nx[i] = ax[i] / amag;
-            for (int ic = 0; ic < 8; ++ic) {
-                const double *dn = &fdn[24 * ip + 3 * ic];
+            for (int ic = 0; ic < npe; ++ic) {
+                const double *dn = &fdn[3 * npe * ip + 3 * ic];
                 for (int j = 0; j < 3; ++j) {
                     const double axj = ax[j], dnj = dn[j], uxj = Uw[3 * ic + j];
                     for (int i = 0; i < 3; ++i) {
-                        const int iR = 3 * nearest + i, rowR = iR * 24;
+                        const int iR = 3 * nearest + i, rowR = iR * ld;
                         const double dni = dn[i], uxi = Uw[3 * ic + i], nxi = nx[i];
                         const double om = 1.0 - nxi * nxi;
                         double lf = -mub * dnj * axj * om;
@@ -769,7 +765,7 @@ void or_assemble_momentum(or_case *c)
                 }
             }
         }
-        or_apply_coeff(3, 8, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
+        or_apply_coeff(3, npe, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
     }
 
     /* postAssemble_: relaxation diag /= urf (phiAssembler.h:966-1006), then
@@ -888,9 +884,10 @@ void or_assemble_pcorr(or_case *c)
         if (btype != OR_BC_INLET && btype != OR_BC_OUTLET) continue;
         memset(lhs, 0, sizeof(lhs));
         memset(rhs, 0, sizeof(rhs));
-        for (int a = 0; a < 8; ++a) nodes[a] = c->conn[8 * c->bf_elem[f] + a];
+        const int npe = or_topo_npe(or_etype(c, c->bf_elem[f]));
+        for (int a = 0; a < npe; ++a) nodes[a] = c->conn[8 * c->bf_elem[f] + a];
         or_bc_pcorr(c, f, lhs, rhs);
-        or_apply_coeff(1, 8, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A1, c->b1);
+        or_apply_coeff(1, npe, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A1, c->b1);
     }
     /* pressure reference (only without a pressure-specified boundary,
      * domain::pressureLevelRequired): zero the row, keep the diagonal, b = 0 */
@@ -1052,21 +1049,19 @@ void or_update_blending(or_case *c, int which)
         }
     }
     /* boundary faces: each face ip limits its nearest node */
-    const int *sn = or_hex_side_nodes();
-    double fsf[16];
-    or_quad_face_shape(0, fsf);
     for (int f = 0; f < c->nbf; ++f) {
-        const int *en = &c->conn[8 * c->bf_elem[f]];
-        int fn[4];
-        double fc[12];
-        for (int k = 0; k < 4; ++k) {
-            fn[k] = en[sn[4 * c->bf_ord[f] + k]];
+        or_side sd;
+        or_side_get(c, f, &sd);
+        const int npf = sd.npf;
+        const int *fn = sd.fn;
+        double fc[12], fsf[16];
+        or_side_shape(npf, 0, fsf);
+        for (int k = 0; k < npf; ++k)
             for (int d = 0; d < 3; ++d) fc[3 * k + d] = c->xyz[3 * fn[k] + d];
-        }
-        for (int ip = 0; ip < 4; ++ip) {
+        for (int ip = 0; ip < npf; ++ip) {
             double xip[3] = {0, 0, 0};
-            for (int ic = 0; ic < 4; ++ic)
-                for (int j = 0; j < 3; ++j) xip[j] += fsf[4 * ip + ic] * fc[3 * ic + j];
+            for (int ic = 0; ic < npf; ++ic)
+                for (int j = 0; j < 3; ++j) xip[j] += fsf[npf * ip + ic] * fc[3 * ic + j];
             const int node = fn[ip];
             double dx[3];
             for (int j = 0; j < 3; ++j) dx[j] = xip[j] - fc[3 * ip + j];

@@ -89,6 +89,35 @@ static inline void or_gather_coords(const or_case *c, int e, double *co)
         for (int d = 0; d < 3; ++d) co[3 * a + d] = c->xyz[3 * en[a] + d];
 }
 
+/* one boundary side: its element and topology, the element's nodes, the
+ * side nodes as local ordinals and as global nodes (side order, which is
+ * the order of the side's face ips) */
+typedef struct {
+    int e, ord, topo, npe, npf;
+    const int *en;
+    int sn[4], fn[4];
+} or_side;
+
+static inline void or_side_get(const or_case *c, int f, or_side *s)
+{
+    s->e = c->bf_elem[f];
+    s->ord = c->bf_ord[f];
+    s->topo = or_etype(c, s->e);
+    s->npe = or_topo_npe(s->topo);
+    s->en = &c->conn[8 * s->e];
+    s->npf = or_topo_side_nodes(s->topo, s->ord, s->sn);
+    for (int k = 0; k < 4; ++k) s->fn[k] = k < s->npf ? s->en[s->sn[k]] : -1;
+}
+
+/* face shape functions [npf][npf] of a tri3 or quad4 side */
+static inline void or_side_shape(int npf, int shifted, double *shape)
+{
+    if (npf == 3)
+        or_tri_face_shape(shifted, shape);
+    else
+        or_quad_face_shape(shifted, shape);
+}
+
 /* master-element tables of one topology (elem.c); sized for hex8 */
 typedef struct {
     int npe, nscs;

@@ -80,6 +80,19 @@ int or_topo_scs_gradop(int topo, int shifted, const double *coords, double *grad
 /* tet_shape_fcn, wedge_shape_fcn, pyr_shape_fcn / shifted_pyr_shape_fcn at
  * npts parametric points: shape [npts][npe] (elem.c) */
 void or_topo_shape(int topo, int shifted, int npts, const double *par, double *shape);
+/* sides of any topology: count, side-node ordinals [4] (returns the side's
+ * node count, 3 or 4; unused entries -1), exposed-face ip locations
+ * [npf][3], and the face gradient operator [npf][npe][3] at the standard or
+ * shifted face ips (face_grad_op / shifted_face_grad_op) */
+int or_topo_nsides(int topo);
+int or_topo_side_nodes(int topo, int ord, int *nodes);
+void or_topo_side_ips(int topo, int ord, int shifted, double *par);
+int or_topo_side_gradop(int topo, int shifted, int ord, const double *coords, double *gradop,
+                        double *detj);
+/* tri3 exposed-face area vectors [3][3] (tri3d_scs_det) and shape
+ * functions [3][3] (Tri3DSCS) */
+void or_tri_face_areav(const double *coords /*[3][3]*/, double *areav /*[3][3]*/);
+void or_tri_face_shape(int shifted, double *shape);
 
 /* ------------------------------------------------------------- CRS graph */
 /* nodeGraph::buildGraph_ (mesh/graph/nodeGraph.cpp:29-392): reduced stencil

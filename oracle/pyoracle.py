@@ -453,6 +453,63 @@ def ref_topo_geometry(topo: int, coords, deriv):
     return a, v, g, det
 
 
+def topo_sides(topo: int):
+    """Side-node ordinals of every side of a topology (lists of 3 or 4)."""
+    L = lib()
+    out = []
+    for ord_ in range(L.or_topo_nsides(topo)):
+        nodes = (C.c_int * 4)()
+        npf = L.or_topo_side_nodes(topo, ord_, nodes)
+        out.append([nodes[k] for k in range(npf)])
+    return out
+
+
+def side_ips(topo: int, ord_: int, shifted: int):
+    """Exposed-face ip locations [npf][3] of a side in the element's parametric space."""
+    L = lib()
+    npf = len(topo_sides(topo)[ord_])
+    par = np.zeros((npf, 3))
+    L.or_topo_side_ips(topo, ord_, shifted, dptr(par))
+    return par
+
+
+def side_gradop(topo: int, shifted: int, ord_: int, coords):
+    """Face gradient operator [npf][npe][3] and det J [npf] at a side's face ips."""
+    L = lib()
+    c = np.ascontiguousarray(coords, dtype=np.float64)
+    npf = len(topo_sides(topo)[ord_])
+    npe = L.or_topo_npe(topo)
+    g = np.zeros((npf, npe, 3))
+    d = np.zeros(npf)
+    L.or_topo_side_gradop(topo, shifted, ord_, dptr(c), dptr(g), dptr(d))
+    return g, d
+
+
+def tri_face_areav(coords3):
+    L = lib()
+    c = np.ascontiguousarray(coords3, dtype=np.float64)
+    out = np.zeros((3, 3))
+    L.or_tri_face_areav(dptr(c), dptr(out))
+    return out
+
+
+def quad_face_areav(coords4):
+    L = lib()
+    c = np.ascontiguousarray(coords4, dtype=np.float64)
+    out = np.zeros((4, 3))
+    L.or_quad_face_areav(dptr(c), dptr(out))
+    return out
+
+
+def ref_tri_areav(coords3):
+    """The reference Fortran tri3d_scs_det (MasterElementWork.F:1589-1697)."""
+    R = ref_lib()
+    c = np.ascontiguousarray(coords3, dtype=np.float64)
+    out = np.zeros((3, 3))
+    R.tri3d_scs_det_(_i(1), _i(3), _i(3), dptr(c), dptr(out))
+    return out
+
+
 def ref_quad_areav(coords4):
     R = ref_lib()
     c = np.ascontiguousarray(coords4, dtype=np.float64)

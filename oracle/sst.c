@@ -260,15 +260,16 @@ void or_assemble_scalar(or_case *c, int which)
         const int btype = c->bc_type[c->bf_bc[f]];
         if (btype != OR_BC_INLET && btype != OR_BC_OUTLET) continue;
         const int *en = &c->conn[8 * c->bf_elem[f]];
+        const int npe = or_topo_npe(or_etype(c, c->bf_elem[f]));
         double G[8];
-        for (int a = 0; a < 8; ++a) {
+        for (int a = 0; a < npe; ++a) {
             const double F1 = c->F1[en[a]];
             G[a] = c->prm.mu + c->mut[en[a]] * (F1 * s1 + (1.0 - F1) * s2);
         }
         memset(lhs, 0, sizeof(lhs));
         memset(rhs, 0, sizeof(rhs));
         or_bc_scalar(c, which, f, G, lhs, rhs);
-        or_apply_coeff(1, 8, en, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A1, c->b1);
+        or_apply_coeff(1, npe, en, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A1, c->b1);
     }
     /* postAssemble_: relaxation, boundary rhs relaxation */
     if (!(c->scalar_terms & 8)) return;

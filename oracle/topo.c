@@ -20,6 +20,15 @@
  *   SCS ip locations  Tet4CVFEM.C:368-382, Wed6CVFEM.C:436-456 (WedSCS),
  *                     Pyr5CVFEM.C PyrSCS constructor (intgLoc_, intgLocShift_)
  *   L/R node tables   lrscv_ of TetSCS, WedSCS, PyrSCS
+ *   sides             sideNodeOrdinals_ of TetSCS (Tet4CVFEM.C:416-421),
+ *                     WedSCS (Wed6CVFEM.C:494-501), PyrSCS (Pyr5CVFEM.C:759-765);
+ *                     exposed-face ips intgExpFace_ (Tet4CVFEM.C:384-403,
+ *                     Wed6CVFEM.C:458-480, Pyr5CVFEM.C:737-757) and
+ *                     intgExpFaceShift_ (the side nodes); face_grad_op /
+ *                     shifted_face_grad_op (Tet4CVFEM.C:668-745,
+ *                     Wed6CVFEM.C:754-860, Pyr5CVFEM.C:1035-1160)
+ *   tri3 faces        tri3d_scs_det (MasterElementWork.F:1589-1697),
+ *                     Tri3DSCS shape_fcn / shifted_shape_fcn (MasterElement.C:610-715)
  *
  * Subdivision points are written as vertex weights: each is the average of
  * the listed vertices (the element centroid averages all of them).
@@ -381,4 +390,112 @@ int or_topo_scs_gradop(int topo, int shifted, const double *coords, double *grad
     double deriv[12 * 8 * 3];
     or_topo_deriv(topo, shifted, n, or_topo_scs_ips(topo, shifted), deriv);
     return or_topo_gradop(npe, n, deriv, coords, gradop, detj);
+}
+
+/* ------------------------------------------------------------- sides */
+/* side-node ordinals (-1: a tri side's unused fourth entry) and the
+ * vertices' parametric locations */
+static const int tet_sides[4][4] = {{0, 1, 3, -1}, {1, 2, 3, -1}, {0, 3, 2, -1}, {0, 2, 1, -1}};
+static const int wed_sides[5][4] = {{0, 1, 4, 3}, {1, 2, 5, 4}, {0, 3, 5, 2}, {0, 2, 1, -1},
+                                    {3, 4, 5, -1}};
+static const int pyr_sides[5][4] = {{0, 1, 4, -1}, {1, 2, 4, -1}, {2, 3, 4, -1}, {0, 4, 3, -1},
+                                    {0, 3, 2, 1}};
+static const double tet_vpar[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+static const double wed_vpar[6][3] = {{0, 0, -1}, {1, 0, -1}, {0, 1, -1},
+                                      {0, 0, 1},  {1, 0, 1},  {0, 1, 1}};
+static const double pyr_vpar[5][3] = {{-1, -1, 0}, {1, -1, 0}, {1, 1, 0}, {-1, 1, 0}, {0, 0, 1}};
+
+int or_topo_nsides(int topo)
+{
+    return topo == OR_HEX8 ? 6 : topo == OR_TET4 ? 4 : 5;
+}
+
+int or_topo_side_nodes(int topo, int ord, int *nodes)
+{
+    const int *sn;
+    if (topo == OR_HEX8) {
+        sn = or_hex_side_nodes() + 4 * ord;
+    } else {
+        sn = topo == OR_TET4 ? tet_sides[ord] : topo == OR_WED6 ? wed_sides[ord] : pyr_sides[ord];
+    }
+    int npf = 0;
+    for (int k = 0; k < 4; ++k) {
+        nodes[k] = sn[k];
+        if (sn[k] >= 0) npf++;
+    }
+    return npf;
+}
+
+/* exposed-face ip k of a side, in the element's parametric coordinates.
+ * Standard: the centre of face node k's subface -- on a tri side the face
+ * point (7/36, 7/36) from node k, i.e. weights 11/18 on node k and 7/36 on
+ * the other two; on a quad side the bilinear point (+-1/2, +-1/2), weights
+ * 9/16 (node k), 3/16 (its two neighbours), 1/16 (the opposite node).
+ * Shifted: face node k itself.  These reproduce intgExpFace_ entry by entry. */
+void or_topo_side_ips(int topo, int ord, int shifted, double *par)
+{
+    int sn[4];
+    const int npf = or_topo_side_nodes(topo, ord, sn);
+    const double(*vp)[3] = topo == OR_TET4 ? tet_vpar : topo == OR_WED6 ? wed_vpar : pyr_vpar;
+    for (int k = 0; k < npf; ++k)
+        for (int d = 0; d < 3; ++d) {
+            double x = 0.0;
+            if (shifted) {
+                x = vp[sn[k]][d];
+            } else if (npf == 3) {
+                for (int q = 0; q < 3; ++q) x += (q == k ? 11.0 / 18.0 : 7.0 / 36.0) * vp[sn[q]][d];
+            } else {
+                const double w[4] = {9.0 / 16.0, 3.0 / 16.0, 1.0 / 16.0, 3.0 / 16.0};
+                for (int q = 0; q < 4; ++q) x += w[(q - k + 4) % 4] * vp[sn[q]][d];
+            }
+            par[3 * k + d] = x;
+        }
+}
+
+/* face_grad_op / shifted_face_grad_op: the element's gradient operator at
+ * the side's face ips, [npf][npe][3] */
+int or_topo_side_gradop(int topo, int shifted, int ord, const double *coords, double *gradop,
+                        double *detj)
+{
+    if (topo == OR_HEX8) return or_hex_face_gradop(shifted, ord, coords, gradop, detj);
+    int sn[4];
+    const int npf = or_topo_side_nodes(topo, ord, sn), npe = or_topo_npe(topo);
+    double par[12], deriv[4 * 8 * 3];
+    or_topo_side_ips(topo, ord, shifted, par);
+    or_topo_deriv(topo, shifted, npf, par, deriv);
+    return or_topo_gradop(npe, npf, deriv, coords, gradop, detj);
+}
+
+/* tri3d_scs_det: the subface of node k is the quad (node k, mid-edge k,
+ * centroid, mid-edge k-1); its area vector is half the cross product of the
+ * diagonals (centroid - node k) x (mid-edge k-1 - mid-edge k) */
+void or_tri_face_areav(const double *p, double *area)
+{
+    double e[3][3], c[3];
+    for (int d = 0; d < 3; ++d) {
+        for (int k = 0; k < 3; ++k) e[k][d] = 0.5 * (p[3 * k + d] + p[3 * ((k + 1) % 3) + d]);
+        c[d] = (p[d] + p[3 + d] + p[6 + d]) / 3.0;
+    }
+    for (int k = 0; k < 3; ++k) {
+        const int km = (k + 2) % 3;
+        const double a[3] = {c[0] - p[3 * k], c[1] - p[3 * k + 1], c[2] - p[3 * k + 2]};
+        const double b[3] = {e[km][0] - e[k][0], e[km][1] - e[k][1], e[km][2] - e[k][2]};
+        area[3 * k + 0] = 0.5 * (a[1] * b[2] - a[2] * b[1]);
+        area[3 * k + 1] = 0.5 * (a[2] * b[0] - a[0] * b[2]);
+        area[3 * k + 2] = 0.5 * (a[0] * b[1] - a[1] * b[0]);
+    }
+}
+
+/* Tri3DSCS shape functions [3][3]: (1 - xi - eta, xi, eta) at (7/36, 7/36),
+ * (11/18, 7/36), (7/36, 11/18); shifted: at the nodes (the identity) */
+void or_tri_face_shape(int shifted, double *shape)
+{
+    const double loc[3][2] = {{7.0 / 36.0, 7.0 / 36.0}, {11.0 / 18.0, 7.0 / 36.0},
+                              {7.0 / 36.0, 11.0 / 18.0}};
+    for (int k = 0; k < 3; ++k) {
+        const double xi = shifted ? (k == 1) : loc[k][0], eta = shifted ? (k == 2) : loc[k][1];
+        shape[3 * k + 0] = 1.0 - xi - eta;
+        shape[3 * k + 1] = xi;
+        shape[3 * k + 2] = eta;
+    }
 }

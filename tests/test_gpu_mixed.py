@@ -1,7 +1,10 @@
 """GPU parity on a mixed-topology mesh (hex8, tet4, wedge6 and pyramid5
 elements in one mesh) through the C ABI against the oracle on identical
 seeded inputs: the duct of meshgen.duct_mixed with a velocity inlet, a
-static-pressure outlet and no-slip walls, nodes jittered.
+static-pressure outlet and no-slip walls, nodes jittered.  Its walls are
+sides of every element that touches them: quad sides of hex8 / wedge6 /
+pyramid5 and tri sides of tet4; a second duct ends in pyramids (inlet: quad
+bases) and wedges (outlet: tri sides).
 
 Bars: CRS graph and diagonal offsets bit-exact (reduced and full stencil);
 dual volumes, mass flows, mass divergence, gradients, blending factors and
@@ -17,6 +20,7 @@ from conftest import rel_err
 pytestmark = pytest.mark.gpu
 
 LAYERS = (4, 1, 2, 2, 2)
+LAYERS_ENDS = (0, 2, 2, 3, 0)   # pyramid cubes at the inlet, wedge cubes at the outlet
 M = 6
 INLET, OUTLET, WALL = 1, 2, 0
 BC = np.array([[1.0, 0.0, 0.0], [0.0] * 3, [0.0] * 3])
@@ -30,15 +34,16 @@ def oa():
     return oa
 
 
-def duct(meshgen, jitter=0.1, seed=1):
-    L = sum(LAYERS) * 0.2 / M
-    return meshgen.duct_mixed(M, layers=LAYERS, length=L, jitter=jitter, seed=seed)
+def duct(meshgen, jitter=0.1, seed=1, layers=LAYERS):
+    L = sum(layers) * 0.2 / M
+    return meshgen.duct_mixed(M, layers=layers, length=L, jitter=jitter, seed=seed)
 
 
-def pair(oracle, oa, meshgen, reduced=1, precision=0, sst=False, high_res=0, seed=1, io=True):
-    print(f"[case] mixed duct reduced={reduced} precision={precision} sst={sst} "
+def pair(oracle, oa, meshgen, reduced=1, precision=0, sst=False, high_res=0, seed=1, io=True,
+         layers=LAYERS):
+    print(f"[case] mixed duct layers={layers} reduced={reduced} precision={precision} sst={sst} "
           f"high_res={high_res} seed={seed} io={io}", flush=True)
-    m = duct(meshgen, seed=seed)
+    m = duct(meshgen, seed=seed, layers=layers)
     c = oracle.Case(m, oracle.default_params(reduced=reduced, high_res=high_res), BC)
     dm = oa.DeviceMesh(m, oa.OAGPU_STENCIL_REDUCED if reduced else oa.OAGPU_STENCIL_FULL)
     f = oa.FlowCase(dm, oa.default_params(precision, high_res=high_res), BC)
@@ -82,8 +87,17 @@ def test_graph_and_volumes(oracle, oa, meshgen, reduced):
     assert rel_err(dm.dual_volume(), c.field("vol")) < 1e-13
 
 
-def test_mass_flow_divergence_gradients(oracle, oa, meshgen):
-    c, dm, f, m = pair(oracle, oa, meshgen)
+def test_boundary_side_kinds(meshgen):
+    """Both ducts carry tri sides on their boundary (tet walls; wedge outlet)."""
+    for layers, want in ((LAYERS, {(1, 3)}), (LAYERS_ENDS, {(1, 3), (2, 4), (3, 4)})):
+        m = duct(meshgen, layers=layers)
+        kinds = set(zip(m["etype"][m["bf_elem"]].tolist(), m["bf_ord"].tolist()))
+        assert want <= kinds, (layers, kinds)
+
+
+@pytest.mark.parametrize("layers", [LAYERS, LAYERS_ENDS])
+def test_mass_flow_divergence_gradients(oracle, oa, meshgen, layers):
+    c, dm, f, m = pair(oracle, oa, meshgen, layers=layers)
     rng = np.random.default_rng(2)
     U = rng.normal(scale=0.3, size=(c.nn, 3))
     U[:, 0] += 1.0
@@ -106,7 +120,9 @@ def test_mass_flow_divergence_gradients(oracle, oa, meshgen):
         assert rel_err(f.download(name), c.field(name)) < 1e-12, name
 
 
-@pytest.mark.parametrize("case", [dict(reduced=1), dict(reduced=0), dict(reduced=1, high_res=1)])
+@pytest.mark.parametrize("case", [dict(reduced=1), dict(reduced=0), dict(reduced=1, high_res=1),
+                                  dict(reduced=1, layers=LAYERS_ENDS),
+                                  dict(reduced=1, high_res=1, layers=LAYERS_ENDS)])
 def test_systems_fp64(oracle, oa, meshgen, case):
     c, dm, f, m = pair(oracle, oa, meshgen, **case)
     sync_state(c, f, np.random.default_rng(3))
@@ -129,8 +145,9 @@ def test_systems_fp64(oracle, oa, meshgen, case):
 
 
 @pytest.mark.parametrize("which", ["k", "omega"])
-def test_scalar_systems_fp64(oracle, oa, meshgen, which):
-    c, dm, f, m = pair(oracle, oa, meshgen, sst=True)
+@pytest.mark.parametrize("layers", [LAYERS, LAYERS_ENDS])
+def test_scalar_systems_fp64(oracle, oa, meshgen, which, layers):
+    c, dm, f, m = pair(oracle, oa, meshgen, sst=True, layers=layers)
     rng = np.random.default_rng(5)
     sync_state(c, f, rng)
     nn = c.nn
@@ -148,8 +165,9 @@ def test_scalar_systems_fp64(oracle, oa, meshgen, which):
     assert rel_err(b, c.rhs(1)) < 1e-12
 
 
-def test_outer_iterations(oracle, oa, meshgen):
-    c, dm, f, m = pair(oracle, oa, meshgen, sst=True)
+@pytest.mark.parametrize("layers", [LAYERS, LAYERS_ENDS])
+def test_outer_iterations(oracle, oa, meshgen, layers):
+    c, dm, f, m = pair(oracle, oa, meshgen, sst=True, layers=layers)
     U = np.zeros((c.nn, 3))
     U[:, 0] = 1.0
     both(c, f, "U", U)

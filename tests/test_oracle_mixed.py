@@ -44,11 +44,14 @@ def case(oracle, m, **kw):
     return oracle.Case(m, oracle.default_params(**kw), BC)
 
 
+def side_nodes(m, e, o):
+    return m["conn"][e, FACES[int(m["etype"][e])][o]]
+
+
 def interior(m):
     on = np.zeros(m["xyz"].shape[0], bool)
-    side = np.array(FACES[mg.HEX8])
-    fn = m["conn"][m["bf_elem"]][np.arange(m["bf_elem"].size)[:, None], side[m["bf_ord"]]]
-    on[fn.ravel()] = True
+    for e, o in zip(m["bf_elem"], m["bf_ord"]):
+        on[side_nodes(m, e, o)] = True
     return ~on
 
 
@@ -63,9 +66,10 @@ def test_duct_conforming_and_volumes(oracle):
             count[key] = count.get(key, 0) + 1
     assert max(count.values()) == 2
     bnd = {k for k, v in count.items() if v == 1}
-    side = np.array(FACES[mg.HEX8])
-    decl = {tuple(sorted(conn[e, side[o]])) for e, o in zip(m["bf_elem"], m["bf_ord"])}
+    decl = {tuple(sorted(side_nodes(m, e, o))) for e, o in zip(m["bf_elem"], m["bf_ord"])}
     assert bnd == decl
+    # walls of tets meet the boundary with tri sides
+    assert any(len(FACES[int(et[e])][o]) == 3 for e, o in zip(m["bf_elem"], m["bf_ord"]))
     for e in range(conn.shape[0]):
         t = int(et[e])
         co = m["xyz"][conn[e, :NPE[t]]]
