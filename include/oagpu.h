@@ -179,11 +179,18 @@ int oagpu_element_geometry_noalloc(int topology, int shifted, int n_elems, const
 /* --------------------------------------------------------------- flow */
 /* boundary physical types (the zone boundaries' types of the reference,
  * boundaryPhysicalType): no-slip wall, velocity inlet, static-pressure
- * outlet, symmetry plane */
+ * outlet, symmetry plane, and slip wall (a wall whose velocity condition is
+ * boundaryConditionType::slip: no face terms in the momentum equations,
+ * navierStokesAssemblerElemBoundaryConditions.cpp:2376-2657; the symmetry
+ * treatment of p', pressureCorrectionAssemblerElemBoundaryConditions.cpp:49,
+ * which adds nothing on a static mesh; no side mass flow; its nodes' rhs
+ * relaxed like other boundary nodes in steady runs, navierStokesAssembler.cpp:289;
+ * k and omega zero-gradient) */
 #define OAGPU_BC_WALL 0
 #define OAGPU_BC_INLET 1
 #define OAGPU_BC_OUTLET 2
 #define OAGPU_BC_SYMMETRY 3
+#define OAGPU_BC_SLIP 4
 
 /* bc_U[n_bc][3]: velocity of each boundary id (no-slip walls: the wall
  * velocity; velocity inlets: the specified inlet velocity). */

@@ -33,7 +33,7 @@ LIB_PATH = os.path.join(HERE, "liboagpu.so")
 OAGPU_FP64, OAGPU_MIXED = 0, 1
 OAGPU_STENCIL_FULL, OAGPU_STENCIL_REDUCED = 0, 1
 OAGPU_GMRES, OAGPU_CG, OAGPU_BICGSTAB, OAGPU_CG_AMG, OAGPU_BICGSTAB_AMG = 0, 1, 2, 3, 4
-OAGPU_BC_WALL, OAGPU_BC_INLET, OAGPU_BC_OUTLET, OAGPU_BC_SYMMETRY = 0, 1, 2, 3
+OAGPU_BC_WALL, OAGPU_BC_INLET, OAGPU_BC_OUTLET, OAGPU_BC_SYMMETRY, OAGPU_BC_SLIP = 0, 1, 2, 3, 4
 OAGPU_HEX8, OAGPU_TET4, OAGPU_WED6, OAGPU_PYR5 = 0, 1, 2, 3
 OAGPU_BDF1, OAGPU_BDF2 = 1, 2
 TOPOLOGY_SIZES = {OAGPU_HEX8: (8, 12), OAGPU_TET4: (4, 6), OAGPU_WED6: (6, 9), OAGPU_PYR5: (5, 12)}
@@ -342,7 +342,7 @@ class FlowCase:
 
     def set_bc_types(self, types):
         """Physical type per boundary id: OAGPU_BC_WALL, OAGPU_BC_INLET,
-        OAGPU_BC_OUTLET or OAGPU_BC_SYMMETRY."""
+        OAGPU_BC_OUTLET, OAGPU_BC_SYMMETRY or OAGPU_BC_SLIP."""
         t = np.ascontiguousarray(types, dtype=np.int32)
         _check(load_library().oagpu_flow_set_bc_types(self.h, len(t), _i(t)))
 

@@ -453,16 +453,18 @@ int oagpu_flow_set_bc_types(oagpu_flow* h, int n_bc, const int* type) {
         OAGPU_REQUIRE(h && type, "null argument");
         oagpu::Flow& f = h->f;
         OAGPU_REQUIRE(n_bc == f.nbc, "one type per boundary id");
-        bool sym = false;
+        bool sym = false, slip = false;
         for (int b = 0; b < n_bc; ++b) {
-            OAGPU_REQUIRE(type[b] >= OAGPU_BC_WALL && type[b] <= OAGPU_BC_SYMMETRY,
-                          "unsupported boundary type (wall, inlet, outlet, symmetry)");
+            OAGPU_REQUIRE(type[b] >= OAGPU_BC_WALL && type[b] <= OAGPU_BC_SLIP,
+                          "unsupported boundary type (wall, inlet, outlet, symmetry, slip wall)");
             sym = sym || type[b] == OAGPU_BC_SYMMETRY;
+            slip = slip || type[b] == OAGPU_BC_SLIP;
         }
         OAGPU_HIP(hipStreamSynchronize(f.stream));
         f.h_bc_type.assign(type, type + n_bc);
         f.bc_type.upload(f.h_bc_type);
         f.has_sym = sym;
+        f.has_slip = slip;
         if (sym && !f.symA.p) f.symA.alloc(size_t(f.m->nn) * 3);
         // inlet and outlet faces, in face order
         const int nbf = f.m->nbf;

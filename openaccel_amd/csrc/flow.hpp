@@ -130,6 +130,7 @@ struct Flow {
     DBuf<int> bc_type, wnode;
     DBuf<double> symA;
     bool has_sym = false;
+    bool has_slip = false;     // some boundary id is a slip wall
     // velocity inlets and static-pressure outlets: the boundary faces of
     // either type, the outlet static pressure and inlet k / omega per
     // boundary id, and per face ip the side mass flow rate and (outlets) the

@@ -432,6 +432,7 @@ struct MomArgs {
     const double* gradU;
     const int* bc_type;    // physical type per boundary id
     const double* symA;    // assembled symmetry area [nn][3], null without symmetry
+    int typed;             // some side is not a no-slip wall (bc_type decides per side)
     const double* msd;     // side mass flow per boundary face ip, null without inlets / outlets
     const int* rfl;        // outlet reversal flag per boundary face ip
     const int* gen_ptr;    // tet4 / wedge6 / pyramid5 incidences (mixed meshes)
@@ -776,7 +777,8 @@ __device__ __forceinline__ void mom_boundary(const MomArgs& a, int pk, uint64_t 
                                              const double xr[3], double D[9], double R[3],
                                              int& bad, LT* lt) {
     const int f = pk >> 2, fk = pk & 3;
-    const int t = (a.symA || a.msd) ? a.bc_type[a.bf_bc[f]] : OAGPU_BC_WALL;
+    const int t = a.typed ? a.bc_type[a.bf_bc[f]] : OAGPU_BC_WALL;
+    if (t == OAGPU_BC_SLIP) return;  // slip wall: no face terms
     if (t == OAGPU_BC_SYMMETRY) {
         mom_face<T, false, MX>(a, f, fk, false, sl, sr, dof, xr, D, R, bad, lt);
     } else if (t == OAGPU_BC_INLET) {
@@ -973,6 +975,7 @@ void launch_assemble_momentum(Flow& f) {
     a.bc_type = f.bc_type.p;
     a.symA = f.has_sym ? f.symA.p : nullptr;
     a.msd = f.has_io ? f.msd.p : nullptr;
+    a.typed = f.has_sym || f.has_io || f.has_slip;
     a.rfl = f.has_io ? f.rfl.p : nullptr;
     a.gen_ptr = m.gen_ptr.p;
     a.gen = m.gen.p;

@@ -221,7 +221,7 @@ void or_case_destroy(or_case *c)
 int or_case_set_bc_types(or_case *c, const int *type)
 {
     for (int b = 0; b < c->nbc; ++b)
-        if (type[b] < OR_BC_WALL || type[b] > OR_BC_SYMMETRY) return 1;
+        if (type[b] < OR_BC_WALL || type[b] > OR_BC_SLIP) return 1;
     memcpy(c->bc_type, type, sizeof(int) * (size_t)c->nbc);
     memset(c->wnode, 0, sizeof(int) * (size_t)c->nn);
     memset(c->symA, 0, sizeof(double) * 3 * (size_t)c->nn);
@@ -672,6 +672,7 @@ void or_assemble_momentum(or_case *c)
     double co[24];
     for (int f = 0; f < c->nbf; ++f) {
         const int btype = c->bc_type[c->bf_bc[f]];
+        if (btype == OR_BC_SLIP) continue;  /* slip wall: no face terms (:2376-2657) */
         or_side sd;
         or_side_get(c, f, &sd);
         const int npe = sd.npe, npf = sd.npf, ld = 3 * npe;

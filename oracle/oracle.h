@@ -176,8 +176,10 @@ void or_compute_du(or_case *c);          /* navierStokesAssembler.cpp:84-232    
 void or_normal_relaxation(or_case *c);
 /* boundary physical types per boundary id: no-slip wall, velocity inlet
  * (the bc_U row; zero-gradient pressure), static-pressure outlet
- * (zero-gradient velocity), symmetry plane */
-enum { OR_BC_WALL = 0, OR_BC_INLET = 1, OR_BC_OUTLET = 2, OR_BC_SYMMETRY = 3 };
+ * (zero-gradient velocity), symmetry plane, slip wall (no face terms in
+ * the momentum equations, navierStokesAssemblerElemBoundaryConditions.cpp:
+ * 2376-2657; nothing in p' or the side mass flows; k, omega zero-gradient) */
+enum { OR_BC_WALL = 0, OR_BC_INLET = 1, OR_BC_OUTLET = 2, OR_BC_SYMMETRY = 3, OR_BC_SLIP = 4 };
 int or_case_set_bc_types(or_case *c, const int *type);
 /* per boundary id: outlet static pressure, inlet k and omega (NULL: zero) */
 void or_case_set_bc_values(or_case *c, const double *p, const double *k, const double *w);
