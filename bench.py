@@ -231,12 +231,15 @@ def bfs_setup(oa_or_oracle, mesh, case, device=True):
     case.initialize()
 
 
-def bfs_params(mod, precision=None):
+def bfs_params(mod, precision=None, p_method="cg_amg"):
     kw = dict(mu=BFS_MU, urf_U=0.7, urf_p=0.3, u_shifted=1)
     if precision is None:
         return mod.default_params(**kw)
-    # p' by AMG-preconditioned BiCGStab: the outlet makes it nonsymmetric
-    return mod.default_params(precision, p_method=mod.OAGPU_BICGSTAB_AMG, **kw)
+    # p' by AMG-preconditioned flexible CG with K-cycle coarse levels, or by
+    # AMG-preconditioned BiCGStab (the outlet and the graded cells make the
+    # p' matrix nonsymmetric, by a few 1e-5 of its norm on this mesh)
+    pm = mod.OAGPU_CG_AMG if p_method == "cg_amg" else mod.OAGPU_BICGSTAB_AMG
+    return mod.default_params(precision, p_method=pm, **kw)
 
 
 def bfs_cpu_baseline(dims, threads):
@@ -266,11 +269,11 @@ def bench_bfs(args):
     ne, nn = mesh["conn"].shape[0], mesh["xyz"].shape[0]
     print(f"[bench] backward-facing step {dims}: {ne} elements, first dy {mesh['first_dy']:.2e}",
           file=sys.stderr, flush=True)
-    dm = oa.DeviceMesh(mesh)
+    dm = oa.DeviceMesh(mesh, caller_order=args.caller_order)
     nnz = dm.nnz
 
     def make_flow(prec):
-        return oa.FlowCase(dm, bfs_params(oa, prec), BFS_BC)
+        return oa.FlowCase(dm, bfs_params(oa, prec, args.bfs_p_method), BFS_BC)
 
     res = {}
     for prec in (oa.OAGPU_MIXED, oa.OAGPU_FP64):
@@ -323,7 +326,8 @@ def bench_bfs(args):
                                f"1.1; inlet U=1 (k {BFS_KIN}, omega {BFS_WIN:g}), outlet p=0, "
                                f"symmetry sides, rho=1, mu={BFS_MU}; one step = flow outer "
                                f"iteration + SST iteration; mixed form: FP32 local matrices into "
-                               f"FP64 CRS, FP32 inner Krylov solves (AMG-BiCGStab p', GMRES "
+                               f"FP64 CRS, FP32 inner Krylov solves ("
+                               f"{'AMG-CG (K-cycle)' if args.bfs_p_method == 'cg_amg' else 'AMG-BiCGStab'} p', GMRES "
                                f"momentum / k / omega) under FP64 refinement",
                    "dims": dims, "elements": ne, "nodes": nn, "nnz": nnz},
         "roofline": roofline(rmx, ab_mx, "bfs", 1),
@@ -368,7 +372,7 @@ def bench_cavity(args):
     print(f"[bench] cavity {args.n}^3 perturb={args.perturb}: {ne} elements", file=sys.stderr,
           flush=True)
     pm = oa.OAGPU_CG_AMG if args.p_method == "cg_amg" else oa.OAGPU_BICGSTAB_AMG
-    dm = oa.DeviceMesh(mesh)
+    dm = oa.DeviceMesh(mesh, caller_order=args.caller_order)
     nn, nnz = dm.nn, dm.nnz
 
     def make_flow(prec):
@@ -516,7 +520,7 @@ def bench_duct(args):
     print(f"[bench] mixed duct m={m} layers={layers}: {ne} elements "
           f"(hex {counts[0]}, tet {counts[1]}, wedge {counts[2]}, pyramid {counts[3]})",
           file=sys.stderr, flush=True)
-    dm = oa.DeviceMesh(mesh)
+    dm = oa.DeviceMesh(mesh, caller_order=args.caller_order)
     res = {}
     for prec in (oa.OAGPU_MIXED, oa.OAGPU_FP64):
         f = oa.FlowCase(dm, oa.default_params(prec, p_method=oa.OAGPU_BICGSTAB_AMG, mu=DUCT_MU,
@@ -626,7 +630,7 @@ def bench_batch(args):
     from openaccel_amd import meshgen
     oa.init(0)
     cases = batch_cases(args.batch_cases)
-    meshes = {n: oa.DeviceMesh(meshgen.cavity_hex(n)) for n in sorted({c["n"] for c in cases})}
+    meshes = {n: oa.DeviceMesh(meshgen.cavity_hex(n), caller_order=args.caller_order) for n in sorted({c["n"] for c in cases})}
     elems = sum(c["n"] ** 3 for c in cases)
     print(f"[bench] batch of {len(cases)} cavities, {elems} elements in all", file=sys.stderr, flush=True)
 
@@ -719,6 +723,8 @@ def main():
                          "cavity: the lid-driven cavity; duct: the 3.9M-cell mixed-topology "
                          "duct; batch: 96 independent cavities")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--caller-order", action="store_true",
+                    help="keep the generator's node numbering on the device (no locality renumbering)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: all host cores)")
     ap.add_argument("--cpu-scale", type=float, default=0.5,
                     help="step: oracle sample grid as a fraction of the cell counts per direction")
@@ -726,6 +732,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--perturb", type=float, default=0.0)
     ap.add_argument("--p-method", choices=("cg_amg", "bicgstab_amg"), default="cg_amg")
+    ap.add_argument("--bfs-p-method", choices=("cg_amg", "bicgstab_amg"), default="cg_amg",
+                    help="backward-facing step: p' solver")
     ap.add_argument("--to-residual", type=float, default=0.0,
                     help="cavity: time the outer iterations until every scaled residual is below this")
     ap.add_argument("--max-iters", type=int, default=4000)

@@ -39,6 +39,15 @@ extern "C" {
  * controls.h:244 bandwidth_reduction) */
 #define OAGPU_STENCIL_REDUCED 1
 #define OAGPU_STENCIL_FULL 0
+/* or-ed into the stencil argument of oagpu_mesh_create: keep the caller's
+ * node and element numbering on the device.  By default the device holds the
+ * mesh renumbered for locality (nodes along a Morton curve of their per-axis
+ * layer indices, elements by their smallest node), the counterpart of the
+ * reference's solver-side reordering (expert parameter bandwidth_reduction,
+ * on by default, src/controls/controls.h:244).  Either way every array that
+ * crosses this ABI (fields, graphs, scatter maps, systems, solutions, AMG
+ * aggregates of the fine level, p_ref_node) is in the caller's numbering. */
+#define OAGPU_MESH_CALLER_ORDER 0x100
 
 /* element topologies (nalu node orders of Hex8CVFEM.C, Tet4CVFEM.C,
  * Wed6CVFEM.C, Pyr5CVFEM.C) */

@@ -32,6 +32,7 @@ LIB_PATH = os.path.join(HERE, "liboagpu.so")
 
 OAGPU_FP64, OAGPU_MIXED = 0, 1
 OAGPU_STENCIL_FULL, OAGPU_STENCIL_REDUCED = 0, 1
+OAGPU_MESH_CALLER_ORDER = 0x100
 OAGPU_GMRES, OAGPU_CG, OAGPU_BICGSTAB, OAGPU_CG_AMG, OAGPU_BICGSTAB_AMG = 0, 1, 2, 3, 4
 OAGPU_BC_WALL, OAGPU_BC_INLET, OAGPU_BC_OUTLET, OAGPU_BC_SYMMETRY, OAGPU_BC_SLIP = 0, 1, 2, 3, 4
 OAGPU_HEX8, OAGPU_TET4, OAGPU_WED6, OAGPU_PYR5 = 0, 1, 2, 3
@@ -216,9 +217,13 @@ def batch_outer_iterations(flows, ctl, iters, tol=0.0, threads=0):
 
 class DeviceMesh:
     """Device mirror of a mesh (oagpu_mesh_create); mesh["etype"] gives the
-    element topologies of a mixed mesh (absent: all hex8)."""
+    element topologies of a mixed mesh (absent: all hex8).  The device holds
+    the mesh renumbered for locality unless caller_order is set; every array
+    exchanged through the class is in the caller's numbering either way."""
 
-    def __init__(self, mesh: dict, stencil: int = OAGPU_STENCIL_REDUCED):
+    def __init__(self, mesh: dict, stencil: int = OAGPU_STENCIL_REDUCED, caller_order: bool = False):
+        if caller_order:
+            stencil |= OAGPU_MESH_CALLER_ORDER
         L = load_library()
         xyz = np.ascontiguousarray(mesh["xyz"], dtype=np.float64)
         conn = np.ascontiguousarray(mesh["conn"], dtype=np.int32)

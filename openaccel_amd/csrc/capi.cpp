@@ -52,33 +52,70 @@ void ensure_init() {
     }
 }
 
-double* field_ptr(oagpu::Flow& f, const char* name, size_t* count) {
+// Index kind of a field: per node, per element or per boundary side.
+enum FieldKind { kPerNode, kPerElem, kPerSide };
+
+double* field_ptr(oagpu::Flow& f, const char* name, size_t* count, FieldKind* kind = nullptr) {
     const size_t nn = size_t(f.m->nn), ne = size_t(f.m->ne), nbf = size_t(f.m->nbf);
     struct {
         const char* n;
         oagpu::DBuf<double>* b;
         size_t c;
-    } tab[] = {{"U", &f.U, nn * 3},          {"p", &f.p, nn},       {"gradU", &f.gradU, nn * 9},
-               {"gradP", &f.gradP, nn * 3},  {"du", &f.du, nn * 3}, {"mdot", &f.mdot, ne * 12},
-               {"div", &f.div, nn},          {"mu", &f.mueff, nn},  {"k", &f.k, nn},
-               {"omega", &f.om, nn},          {"gradK", &f.gradK, nn * 3},
-               {"gradW", &f.gradW, nn * 3},  {"ymin", &f.ymin, nn}, {"mut", &f.mut, nn},
-               {"F1", &f.F1, nn},            {"Pk", &f.Pk, nn},     {"Pk_wall", &f.Pk_wall, nn},
-               {"kprev", &f.kprev, nn},      {"beta", &f.beta, nn * 3},
-               {"beta_k", &f.beta_k, nn},    {"beta_w", &f.beta_w, nn},
-               {"symA", &f.symA, nn * 3},    {"msd", &f.msd, nbf * 4},
-               {"Uold", &f.Uold, nn * 3},    {"Uold2", &f.Uold2, nn * 3},
-               {"kold", &f.kold, nn},        {"kold2", &f.kold2, nn},
-               {"wold", &f.wold, nn},        {"wold2", &f.wold2, nn}};
+        FieldKind k;
+    } tab[] = {{"U", &f.U, nn * 3, kPerNode},         {"p", &f.p, nn, kPerNode},
+               {"gradU", &f.gradU, nn * 9, kPerNode}, {"gradP", &f.gradP, nn * 3, kPerNode},
+               {"du", &f.du, nn * 3, kPerNode},       {"mdot", &f.mdot, ne * 12, kPerElem},
+               {"div", &f.div, nn, kPerNode},         {"mu", &f.mueff, nn, kPerNode},
+               {"k", &f.k, nn, kPerNode},             {"omega", &f.om, nn, kPerNode},
+               {"gradK", &f.gradK, nn * 3, kPerNode}, {"gradW", &f.gradW, nn * 3, kPerNode},
+               {"ymin", &f.ymin, nn, kPerNode},       {"mut", &f.mut, nn, kPerNode},
+               {"F1", &f.F1, nn, kPerNode},           {"Pk", &f.Pk, nn, kPerNode},
+               {"Pk_wall", &f.Pk_wall, nn, kPerNode}, {"kprev", &f.kprev, nn, kPerNode},
+               {"beta", &f.beta, nn * 3, kPerNode},   {"beta_k", &f.beta_k, nn, kPerNode},
+               {"beta_w", &f.beta_w, nn, kPerNode},   {"symA", &f.symA, nn * 3, kPerNode},
+               {"msd", &f.msd, nbf * 4, kPerSide},    {"Uold", &f.Uold, nn * 3, kPerNode},
+               {"Uold2", &f.Uold2, nn * 3, kPerNode}, {"kold", &f.kold, nn, kPerNode},
+               {"kold2", &f.kold2, nn, kPerNode},     {"wold", &f.wold, nn, kPerNode},
+               {"wold2", &f.wold2, nn, kPerNode}};
     for (auto& t : tab)
         if (!std::strcmp(name, t.n)) {
             OAGPU_REQUIRE(t.b->p, std::string("field ") + name +
                                           " not allocated (SST, high resolution, symmetry or"
                                           " inlets / outlets off)");
             *count = t.c;
+            if (kind) *kind = t.k;
             return t.b->p;
         }
     throw oagpu::Error(OAGPU_EINVAL, std::string("unknown field ") + name);
+}
+
+// Caller <-> device order of node- or element-indexed arrays of `comp`
+// values per item (the device numbering is the mesh's locality order).
+const std::vector<int>* old_ids(const oagpu::DevMesh& m, FieldKind k) {
+    if (!m.renumbered() || k == kPerSide) return nullptr;
+    return k == kPerNode ? &m.n_old : &m.e_old;
+}
+
+template <typename T>
+void to_device_order(const std::vector<int>* old, const T* caller, T* dev, size_t count) {
+    if (!old) {
+        std::copy(caller, caller + count, dev);
+        return;
+    }
+    const size_t n = old->size(), comp = count / n;
+    for (size_t i = 0; i < n; ++i)
+        std::copy(caller + size_t((*old)[i]) * comp, caller + size_t((*old)[i]) * comp + comp, dev + i * comp);
+}
+
+template <typename T>
+void to_caller_order(const std::vector<int>* old, const T* dev, T* caller, size_t count) {
+    if (!old) {
+        std::copy(dev, dev + count, caller);
+        return;
+    }
+    const size_t n = old->size(), comp = count / n;
+    for (size_t i = 0; i < n; ++i)
+        std::copy(dev + i * comp, dev + i * comp + comp, caller + size_t((*old)[i]) * comp);
 }
 
 // Each flow has its own error word: flows that share a mesh (batched cases)
@@ -134,7 +171,11 @@ int oagpu_mesh_create(int n_nodes, const double* xyz, int n_elems, const int* co
         hm.bf_elem = bf_elem;
         hm.bf_ord = bf_ord;
         hm.bf_bc = bf_bc;
-        oagpu::build_mesh(hm, stencil == OAGPU_STENCIL_FULL ? 0 : 1, h->m);
+        OAGPU_REQUIRE((stencil & ~OAGPU_MESH_CALLER_ORDER) == OAGPU_STENCIL_FULL ||
+                          (stencil & ~OAGPU_MESH_CALLER_ORDER) == OAGPU_STENCIL_REDUCED,
+                      "stencil: OAGPU_STENCIL_REDUCED or OAGPU_STENCIL_FULL, optionally | OAGPU_MESH_CALLER_ORDER");
+        oagpu::build_mesh_ordered(hm, (stencil & ~OAGPU_MESH_CALLER_ORDER) == OAGPU_STENCIL_FULL ? 0 : 1,
+                                  !(stencil & OAGPU_MESH_CALLER_ORDER), h->m);
         oagpu::launch_mesh_geometry(h->m, nullptr);
         int e = 0;
         h->m.err.download(&e, 1);
@@ -162,9 +203,11 @@ int oagpu_mesh_sizes(const oagpu_mesh* m, int* nn, int* ne, int* nnz, int* ninc,
 int oagpu_graph_copy(const oagpu_mesh* m, int* row_ptr, int* cols, int* diag_off) {
     return guarded([&] {
         OAGPU_REQUIRE(m, "null mesh");
-        if (row_ptr) std::memcpy(row_ptr, m->m.h_row_ptr.data(), sizeof(int) * m->m.h_row_ptr.size());
-        if (cols) std::memcpy(cols, m->m.h_cols.data(), sizeof(int) * m->m.h_cols.size());
-        if (diag_off) std::memcpy(diag_off, m->m.h_diag_off.data(), sizeof(int) * m->m.h_diag_off.size());
+        std::vector<int> rp, cl, dg;
+        oagpu::caller_graph(m->m, rp, cl, dg);
+        if (row_ptr) std::copy(rp.begin(), rp.end(), row_ptr);
+        if (cols) std::copy(cl.begin(), cl.end(), cols);
+        if (diag_off) std::copy(dg.begin(), dg.end(), diag_off);
     });
 }
 
@@ -172,12 +215,11 @@ int oagpu_scatter_map_copy(const oagpu_mesh* m, int* inc_ptr, int* inc, unsigned
     return guarded([&] {
         OAGPU_REQUIRE(m, "null mesh");
         const auto& d = m->m;
-        if (inc_ptr) std::memcpy(inc_ptr, d.h_inc_ptr.data(), sizeof(int) * d.h_inc_ptr.size());
-        if (inc) std::memcpy(inc, d.h_inc.data(), sizeof(int) * d.h_inc.size());
+        const size_t nn = size_t(d.nn);
+        // device incidences' scatter maps, rebuilt from the device's
+        // local-node-major copy, so the check covers what the kernels read
+        std::vector<uint64_t> dslots;
         if (slots) {
-            // rebuilt from the device's local-node-major copy, so the check
-            // covers what the kernels read
-            const size_t nn = size_t(d.nn);
             std::vector<int> inc_la(8 * nn), ext_ptr(nn + 1), ext(std::max(d.next, 1));
             std::vector<uint64_t> slots_la(8 * nn), ext_slots(std::max(d.next, 1));
             d.inc_la.download(inc_la.data(), inc_la.size());
@@ -186,6 +228,7 @@ int oagpu_scatter_map_copy(const oagpu_mesh* m, int* inc_ptr, int* inc, unsigned
             d.ext.download(ext.data(), ext.size());
             d.ext_slots.download(ext_slots.data(), ext_slots.size());
             OAGPU_HIP(hipDeviceSynchronize());
+            dslots.resize(d.h_inc.size());
             for (size_t r = 0; r < nn; ++r) {
                 for (int k = d.h_inc_ptr[r]; k < d.h_inc_ptr[r + 1]; ++k) {
                     const int e = d.h_inc[k] >> 3, la = d.h_inc[k] & 7;
@@ -200,19 +243,70 @@ int oagpu_scatter_map_copy(const oagpu_mesh* m, int* inc_ptr, int* inc, unsigned
                             v = ext_slots[q];
                             found = true;
                         }
+                    if (!found && d.mixed) {
+                        // non-hex incidences are read from the gen list
+                        v = d.h_inc_slots[k];
+                        found = true;
+                    }
                     OAGPU_REQUIRE(found, "incidence missing from the device copy");
-                    slots[k] = v;
+                    dslots[k] = v;
                 }
             }
         }
+        if (!d.renumbered()) {
+            if (inc_ptr) std::copy(d.h_inc_ptr.begin(), d.h_inc_ptr.end(), inc_ptr);
+            if (inc) std::copy(d.h_inc.begin(), d.h_inc.end(), inc);
+            if (slots) std::copy(dslots.begin(), dslots.end(), slots);
+            return;
+        }
+        // caller numbering: element ids, incidence order (ascending element)
+        // and slot positions within the caller's sorted rows
+        std::vector<int> rp, cl, dg;
+        oagpu::caller_graph(d, rp, cl, dg);
+        size_t o = 0;
+        std::vector<std::pair<int, uint64_t>> row;
+        for (size_t r = 0; r < nn; ++r) {
+            const int p = d.n_new[r];
+            if (inc_ptr) inc_ptr[r] = int(o);
+            row.clear();
+            const int* cb = cl.data() + rp[r];
+            const int clen = rp[r + 1] - rp[r];
+            for (int k = d.h_inc_ptr[p]; k < d.h_inc_ptr[p + 1]; ++k) {
+                const int e = d.e_old[d.h_inc[k] >> 3], la = d.h_inc[k] & 7;
+                uint64_t v = ~uint64_t(0);
+                if (slots) {
+                    const uint64_t dv = dslots[k];
+                    for (int c = 0; c < 8; ++c) {
+                        const uint64_t pos = (dv >> (8 * c)) & 0xFF;
+                        uint64_t np = 0xFF;
+                        if (pos != 0xFF) {
+                            const int col = d.n_old[d.h_cols[d.h_row_ptr[p] + int(pos)]];
+                            np = uint64_t(std::lower_bound(cb, cb + clen, col) - cb);
+                        }
+                        v = (v & ~(uint64_t(0xFF) << (8 * c))) | (np << (8 * c));
+                    }
+                }
+                row.push_back({(e << 3) | la, v});
+            }
+            std::sort(row.begin(), row.end(),
+                      [](const auto& x, const auto& y) { return x.first < y.first; });
+            for (const auto& q : row) {
+                if (inc) inc[o] = q.first;
+                if (slots) slots[o] = q.second;
+                ++o;
+            }
+        }
+        if (inc_ptr) inc_ptr[nn] = int(o);
     });
 }
 
 int oagpu_mesh_volume_copy(const oagpu_mesh* m, double* vol) {
     return guarded([&] {
         OAGPU_REQUIRE(m && vol, "null argument");
-        m->m.vol.download(vol, m->m.nn);
+        std::vector<double> tmp(size_t(m->m.nn));
+        m->m.vol.download(tmp.data(), tmp.size());
         OAGPU_HIP(hipDeviceSynchronize());
+        to_caller_order(old_ids(m->m, kPerNode), tmp.data(), vol, tmp.size());
     });
 }
 
@@ -228,6 +322,7 @@ int oagpu_flow_create(oagpu_mesh* m, const oagpu_flow_params* prm, int n_bc, con
         oagpu::Flow& f = h->f;
         f.m = &m->m;
         f.prm = *prm;
+        if (prm->p_ref_node >= 0 && m->m.renumbered()) f.prm.p_ref_node = m->m.n_new[prm->p_ref_node];
         f.nbc = n_bc;
         std::vector<double> bc(bc_U, bc_U + 3 * size_t(n_bc));
         if (bc.empty()) bc.assign(3, 0.0);
@@ -295,9 +390,12 @@ int oagpu_field_upload(oagpu_flow* h, const char* name, const double* host, size
     return guarded([&] {
         OAGPU_REQUIRE(h && name && host, "null argument");
         size_t n = 0;
-        double* d = field_ptr(h->f, name, &n);
+        FieldKind kind;
+        double* d = field_ptr(h->f, name, &n, &kind);
         OAGPU_REQUIRE(count == n, std::string("size mismatch for field ") + name);
-        OAGPU_HIP(hipMemcpyAsync(d, host, n * sizeof(double), hipMemcpyHostToDevice, h->f.stream));
+        std::vector<double> tmp(n);
+        to_device_order(old_ids(*h->f.m, kind), host, tmp.data(), n);
+        OAGPU_HIP(hipMemcpyAsync(d, tmp.data(), n * sizeof(double), hipMemcpyHostToDevice, h->f.stream));
         OAGPU_HIP(hipStreamSynchronize(h->f.stream));
     });
 }
@@ -306,10 +404,13 @@ int oagpu_field_download(oagpu_flow* h, const char* name, double* host, size_t c
     return guarded([&] {
         OAGPU_REQUIRE(h && name && host, "null argument");
         size_t n = 0;
-        double* d = field_ptr(h->f, name, &n);
+        FieldKind kind;
+        double* d = field_ptr(h->f, name, &n, &kind);
         OAGPU_REQUIRE(count == n, std::string("size mismatch for field ") + name);
-        OAGPU_HIP(hipMemcpyAsync(host, d, n * sizeof(double), hipMemcpyDeviceToHost, h->f.stream));
+        std::vector<double> tmp(n);
+        OAGPU_HIP(hipMemcpyAsync(tmp.data(), d, n * sizeof(double), hipMemcpyDeviceToHost, h->f.stream));
         OAGPU_HIP(hipStreamSynchronize(h->f.stream));
+        to_caller_order(old_ids(*h->f.m, kind), tmp.data(), host, n);
     });
 }
 
@@ -354,9 +455,11 @@ int oagpu_system_download(oagpu_flow* h, int N, double* vals, double* b) {
             sell.resize(f.m->nsell * N * N);
             (N == 3 ? f.A3 : f.A1).download(sell.data(), sell.size(), f.stream);
         }
-        if (b) (N == 3 ? f.b3 : f.b1).download(b, size_t(f.m->nn) * N, f.stream);
+        std::vector<double> tb(b ? size_t(f.m->nn) * N : 0);
+        if (b) (N == 3 ? f.b3 : f.b1).download(tb.data(), tb.size(), f.stream);
         OAGPU_HIP(hipStreamSynchronize(f.stream));
-        if (vals) oagpu::sell_to_crs(*f.m, N * N, sell.data(), vals);
+        if (b) to_caller_order(old_ids(*f.m, kPerNode), tb.data(), b, tb.size());
+        if (vals) oagpu::sell_to_caller_crs(*f.m, N * N, sell.data(), vals);
     });
 }
 
@@ -367,10 +470,14 @@ int oagpu_system_upload(oagpu_flow* h, int N, const double* vals, const double* 
         std::vector<double> sell;
         if (vals) {
             sell.assign(f.m->nsell * N * N, 0.0);
-            oagpu::crs_to_sell(*f.m, N * N, vals, sell.data());
+            oagpu::caller_crs_to_sell(*f.m, N * N, vals, sell.data());
             (N == 3 ? f.A3 : f.A1).upload(sell.data(), sell.size(), f.stream);
         }
-        if (b) (N == 3 ? f.b3 : f.b1).upload(b, size_t(f.m->nn) * N, f.stream);
+        std::vector<double> tb(b ? size_t(f.m->nn) * N : 0);
+        if (b) {
+            to_device_order(old_ids(*f.m, kPerNode), b, tb.data(), tb.size());
+            (N == 3 ? f.b3 : f.b1).upload(tb.data(), tb.size(), f.stream);
+        }
         OAGPU_HIP(hipStreamSynchronize(f.stream));
     });
 }
@@ -379,8 +486,10 @@ int oagpu_solution_download(oagpu_flow* h, int N, double* x) {
     return guarded([&] {
         OAGPU_REQUIRE(h && x && (N == 1 || N == 3), "bad argument");
         oagpu::Flow& f = h->f;
-        (N == 3 ? f.x3 : f.x1).download(x, size_t(f.m->nn) * N, f.stream);
+        std::vector<double> tx(size_t(f.m->nn) * N);
+        (N == 3 ? f.x3 : f.x1).download(tx.data(), tx.size(), f.stream);
         OAGPU_HIP(hipStreamSynchronize(f.stream));
+        to_caller_order(old_ids(*f.m, kPerNode), tx.data(), x, tx.size());
     });
 }
 
@@ -740,7 +849,12 @@ int oagpu_amg_level_copy(oagpu_flow* h, int level, int* agg, int* row_ptr, int* 
         oagpu::Amg& a = h->f.amg;
         OAGPU_REQUIRE(a.built && level >= 1 && level < int(a.lv.size()), "no such AMG level");
         oagpu::AmgLevel& l = a.lv[level];
-        if (agg) std::copy(l.h_agg.begin(), l.h_agg.end(), agg);
+        if (agg) {
+            if (level == 1)
+                to_caller_order(old_ids(*h->f.m, kPerNode), l.h_agg.data(), agg, l.h_agg.size());
+            else
+                std::copy(l.h_agg.begin(), l.h_agg.end(), agg);
+        }
         if (row_ptr) std::copy(l.h_row_ptr.begin(), l.h_row_ptr.end(), row_ptr);
         if (cols) std::copy(l.h_cols.begin(), l.h_cols.end(), cols);
         if (vals) {

@@ -16,6 +16,7 @@ struct HostMesh {
     const int* bf_elem = nullptr;
     const int* bf_ord = nullptr;
     const int* bf_bc = nullptr;
+    bool ordered = false;           // numbering already in locality order
 };
 
 // Everything the row-owned kernels need, resident in HBM.
@@ -97,9 +98,30 @@ struct DevMesh {
     std::vector<int> h_row_ptr, h_cols, h_diag_off, h_inc_ptr, h_inc, h_binc_ptr, h_binc, h_etype;
     std::vector<int> h_slice_ptr, h_slice_w;
     std::vector<uint64_t> h_inc_slots, h_binc_slots;
+
+    // Caller <-> device numbering.  The device holds nodes and elements in
+    // locality order (locality_order): n_old[i] / e_old[i] is the caller's id
+    // of device node / element i, n_new / e_new the inverse maps.  Empty when
+    // the mesh keeps the caller's numbering.  Boundary sides keep the
+    // caller's order.
+    std::vector<int> n_old, n_new, e_old, e_new;
+    bool renumbered() const { return !n_old.empty(); }
 };
 
+// Node and element order for locality of the row-owned kernels: nodes along
+// a Morton curve of per-axis layer indices, elements by their smallest node.
+void locality_order(const HostMesh& hm, std::vector<int>& n_old, std::vector<int>& e_old);
+// build_mesh on the caller's mesh, renumbered by locality_order when
+// `renumber` is set (the permutations are kept in dm for the ABI's copies).
+void build_mesh_ordered(const HostMesh& hm, int reduced, bool renumber, DevMesh& dm);
 void build_mesh(const HostMesh& hm, int reduced, DevMesh& dm);
+// CRS graph in the caller's numbering (columns sorted, diagonal offsets)
+void caller_graph(const DevMesh& dm, std::vector<int>& row_ptr, std::vector<int>& cols,
+                  std::vector<int>& diag_off);
+// Value conversions between the caller's block-CRS (caller numbering) and the
+// device's sliced-ELL layout (device numbering).
+void sell_to_caller_crs(const DevMesh& dm, int nb, const double* sell, double* crs);
+void caller_crs_to_sell(const DevMesh& dm, int nb, const double* crs, double* sell);
 // value layout conversions (host): CRS blocks of nb values <-> sliced-ELL
 void sell_to_crs(const DevMesh& dm, int nb, const double* sell, double* crs);
 void crs_to_sell(const DevMesh& dm, int nb, const double* crs, double* sell);
