@@ -204,6 +204,8 @@ BFS_DIMS = (30, 210, 96, 112, 170)   # nx_up, nx_dn, ny_low, ny_high, nz: 7 996 
 BFS_MU, BFS_KIN = 2.0e-5, 1.5e-4
 BFS_WIN = BFS_KIN / (10.0 * BFS_MU)
 BFS_CTL, BFS_SCTL = [20, 1.0e-2, 200, 1.0e-2], [20, 1.0e-2]
+P_METHOD_NAMES = {"cg_amg": "AMG-CG (K-cycle)", "bicgstab_amg": "AMG-BiCGStab",
+                  "bicgstab_sa": "smoothed-aggregation AMG-BiCGStab"}
 BFS_BC = np.array([[1.0, 0.0, 0.0], [0.0] * 3, [0.0] * 3, [0.0] * 3])
 
 
@@ -238,7 +240,8 @@ def bfs_params(mod, precision=None, p_method="cg_amg"):
     # p' by AMG-preconditioned flexible CG with K-cycle coarse levels, or by
     # AMG-preconditioned BiCGStab (the outlet and the graded cells make the
     # p' matrix nonsymmetric, by a few 1e-5 of its norm on this mesh)
-    pm = mod.OAGPU_CG_AMG if p_method == "cg_amg" else mod.OAGPU_BICGSTAB_AMG
+    pm = {"cg_amg": mod.OAGPU_CG_AMG, "bicgstab_amg": mod.OAGPU_BICGSTAB_AMG,
+          "bicgstab_sa": mod.OAGPU_BICGSTAB_SA}[p_method]
     return mod.default_params(precision, p_method=pm, **kw)
 
 
@@ -327,7 +330,7 @@ def bench_bfs(args):
                                f"symmetry sides, rho=1, mu={BFS_MU}; one step = flow outer "
                                f"iteration + SST iteration; mixed form: FP32 local matrices into "
                                f"FP64 CRS, FP32 inner Krylov solves ("
-                               f"{'AMG-CG (K-cycle)' if args.bfs_p_method == 'cg_amg' else 'AMG-BiCGStab'} p', GMRES "
+                               f"{P_METHOD_NAMES[args.bfs_p_method]} p', GMRES "
                                f"momentum / k / omega) under FP64 refinement",
                    "dims": dims, "elements": ne, "nodes": nn, "nnz": nnz},
         "roofline": roofline(rmx, ab_mx, "bfs", 1),
@@ -732,7 +735,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--perturb", type=float, default=0.0)
     ap.add_argument("--p-method", choices=("cg_amg", "bicgstab_amg"), default="cg_amg")
-    ap.add_argument("--bfs-p-method", choices=("cg_amg", "bicgstab_amg"), default="cg_amg",
+    ap.add_argument("--bfs-p-method", choices=("cg_amg", "bicgstab_amg", "bicgstab_sa"),
+                    default="bicgstab_sa",
                     help="backward-facing step: p' solver")
     ap.add_argument("--to-residual", type=float, default=0.0,
                     help="cavity: time the outer iterations until every scaled residual is below this")

@@ -122,6 +122,13 @@ typedef struct oagpu_control_data {
                             V-cycle (scalar systems that are not symmetric:
                             p' with static-pressure outlets or on distorted
                             meshes); FP64 in both forms */
+#define OAGPU_BICGSTAB_SA 5  /* BiCGStab right-preconditioned by one V(1,1)-cycle
+                            of smoothed aggregation (prolongation smoothed by
+                            one filtered Jacobi step; Galerkin coarse levels
+                            recomputed on the device every solve): the p'
+                            systems of wall-resolved meshes, whose cell aspect
+                            ratios stall the unsmoothed V-cycle; FP32 inner
+                            solve under FP64 refinement in the mixed form */
 
 /* ---------------------------------------------------------------- runtime */
 int oagpu_init(int device);
@@ -280,6 +287,19 @@ int oagpu_amg_info(oagpu_flow *f, int *n_levels, int *sizes);
  * of that set-up (NULL pointers are skipped). */
 int oagpu_amg_level_copy(oagpu_flow *f, int level, int *agg, int *row_ptr, int *cols,
                          double *vals);
+
+/* Smoothed-aggregation hierarchy of a flow (built by its first
+ * OAGPU_BICGSTAB_SA solve): number of levels, rows, entries of each level's
+ * matrix and of its prolongation P (to the level above), and the smoothing
+ * weight omega_p of each P (arrays of up to 16 levels; NULL: not wanted). */
+int oagpu_sa_info(oagpu_flow *f, int *n_levels, int *sizes, long long *p_nnz, long long *a_nnz,
+                  double *omega_p);
+/* level >= 1: the aggregates of the rows of level - 1, the prolongation P
+ * from this level to level - 1 (CRS over the rows of level - 1) and this
+ * level's matrix (CRS), as of the last numeric set-up (FP64).  Rows of level
+ * 0 are the mesh nodes, in the caller's numbering. */
+int oagpu_sa_level_copy(oagpu_flow *f, int level, int *agg, int *p_row_ptr, int *p_cols,
+                        double *p_vals, int *a_row_ptr, int *a_cols, double *a_vals);
 
 /* One steady outer iteration of segregatedFlowEquations::solve.
  * ctl = {momentum max_it, momentum rtol, p' max_it, p' rtol}; momentum by

@@ -34,6 +34,7 @@ OAGPU_FP64, OAGPU_MIXED = 0, 1
 OAGPU_STENCIL_FULL, OAGPU_STENCIL_REDUCED = 0, 1
 OAGPU_MESH_CALLER_ORDER = 0x100
 OAGPU_GMRES, OAGPU_CG, OAGPU_BICGSTAB, OAGPU_CG_AMG, OAGPU_BICGSTAB_AMG = 0, 1, 2, 3, 4
+OAGPU_BICGSTAB_SA = 5
 OAGPU_BC_WALL, OAGPU_BC_INLET, OAGPU_BC_OUTLET, OAGPU_BC_SYMMETRY, OAGPU_BC_SLIP = 0, 1, 2, 3, 4
 OAGPU_HEX8, OAGPU_TET4, OAGPU_WED6, OAGPU_PYR5 = 0, 1, 2, 3
 OAGPU_BDF1, OAGPU_BDF2 = 1, 2
@@ -127,6 +128,8 @@ _SIGS = {
     "oagpu_sst_iteration": [C.c_void_p, _dp, _dp],
     "oagpu_amg_info": [C.c_void_p, _ip, _ip],
     "oagpu_amg_level_copy": [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp],
+    "oagpu_sa_info": [C.c_void_p, _ip, _ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _dp],
+    "oagpu_sa_level_copy": [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _ip, _ip, _dp],
     "oagpu_outer_iteration": [C.c_void_p, _dp, _dp],
     "oagpu_time_step": [C.c_void_p, C.c_int, C.c_double],
     "oagpu_profile": [C.c_void_p, C.c_char_p],
@@ -431,6 +434,37 @@ class FlowCase:
         vals = np.zeros(int(rp[-1]))
         _check(load_library().oagpu_amg_level_copy(self.h, level, None, None, _i(cols), _d(vals)))
         return agg, rp, cols, vals
+
+    def sa_levels(self):
+        """(rows, prolongation entries, matrix entries, omega_p) per level of the
+        smoothed-aggregation hierarchy (empty before the first SA solve)."""
+        n = C.c_int()
+        sizes = np.zeros(16, dtype=np.int32)
+        pn = np.zeros(16, dtype=np.int64)
+        an = np.zeros(16, dtype=np.int64)
+        om = np.zeros(16)
+        _check(load_library().oagpu_sa_info(self.h, C.byref(n), _i(sizes),
+                                            pn.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                            an.ctypes.data_as(C.POINTER(C.c_longlong)), _d(om)))
+        k = n.value
+        return [(int(sizes[l]), int(pn[l]), int(an[l]), float(om[l])) for l in range(k)]
+
+    def sa_level(self, level):
+        """(agg, P as (row_ptr, cols, vals) over the rows of level - 1, and the
+        matrix of level >= 1 as (row_ptr, cols, vals))."""
+        lv = self.sa_levels()
+        nf, pnnz = lv[level - 1][0], lv[level - 1][1]
+        nc, annz = lv[level][0], lv[level][2]
+        agg = np.zeros(nf, dtype=np.int32)
+        prp = np.zeros(nf + 1, dtype=np.int32)
+        pc = np.zeros(pnnz, dtype=np.int32)
+        pv = np.zeros(pnnz)
+        arp = np.zeros(nc + 1, dtype=np.int32)
+        ac = np.zeros(annz, dtype=np.int32)
+        av = np.zeros(annz)
+        _check(load_library().oagpu_sa_level_copy(self.h, level, _i(agg), _i(prp), _i(pc), _d(pv),
+                                                  _i(arp), _i(ac), _d(av)))
+        return agg, (prp, pc, pv), (arp, ac, av)
 
     def time_step(self, scheme, dt):
         """Start a physical time step: scheme OAGPU_BDF1 or OAGPU_BDF2."""

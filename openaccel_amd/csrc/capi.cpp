@@ -887,6 +887,65 @@ int oagpu_amg_level_copy(oagpu_flow* h, int level, int* agg, int* row_ptr, int* 
     });
 }
 
+int oagpu_sa_info(oagpu_flow* h, int* n_levels, int* sizes, long long* p_nnz, long long* a_nnz,
+                  double* omega_p) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && n_levels, "null argument");
+        const oagpu::Sa& sa = h->f.sa;
+        *n_levels = sa.built ? int(sa.lv.size()) : 0;
+        for (int l = 0; l < *n_levels && l < 16; ++l) {
+            const oagpu::SaLevel& L = sa.lv[l];
+            if (sizes) sizes[l] = L.n;
+            if (a_nnz) a_nnz[l] = L.h_rp.empty() ? 0 : L.h_rp.back();
+            if (p_nnz) p_nnz[l] = L.h_prp.empty() ? 0 : L.h_prp.back();
+            if (omega_p) omega_p[l] = L.omega_p;
+        }
+    });
+}
+
+int oagpu_sa_level_copy(oagpu_flow* h, int level, int* agg, int* p_row_ptr, int* p_cols,
+                        double* p_vals, int* a_row_ptr, int* a_cols, double* a_vals) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h, "null flow");
+        oagpu::Flow& f = h->f;
+        oagpu::Sa& sa = f.sa;
+        OAGPU_REQUIRE(sa.built && level >= 1 && level < int(sa.lv.size()), "no such AMG level");
+        oagpu::SaLevel& F = sa.lv[level - 1];
+        oagpu::SaLevel& C = sa.lv[level];
+        // the rows of level 0 are mesh nodes: caller numbering
+        const bool remap = level == 1 && f.m->renumbered();
+        const int nf = F.n;
+        auto dev_row = [&](int r) { return remap ? f.m->n_new[r] : r; };
+        if (agg)
+            for (int r = 0; r < nf; ++r) agg[r] = F.h_agg[dev_row(r)];
+        if (p_row_ptr || p_cols || p_vals) {
+            const oagpu::SellHost ps = oagpu::make_sell(nf, F.h_prp, F.h_pcols);
+            std::vector<double> pv;
+            if (p_vals) {
+                pv.resize(F.p_nsell);
+                F.P.download(pv.data(), pv.size(), f.stream);
+                OAGPU_HIP(hipStreamSynchronize(f.stream));
+            }
+            int o = 0;
+            if (p_row_ptr) p_row_ptr[0] = 0;
+            for (int r = 0; r < nf; ++r) {
+                const int d = dev_row(r);
+                for (int q = F.h_prp[d]; q < F.h_prp[d + 1]; ++q, ++o) {
+                    if (p_cols) p_cols[o] = F.h_pcols[q];
+                    if (p_vals) p_vals[o] = pv[oagpu::sell_index(ps.slice_ptr, d, q - F.h_prp[d], 1, 0)];
+                }
+                if (p_row_ptr) p_row_ptr[r + 1] = o;
+            }
+        }
+        if (a_row_ptr) std::copy(C.h_rp.begin(), C.h_rp.end(), a_row_ptr);
+        if (a_cols) std::copy(C.h_cols.begin(), C.h_cols.end(), a_cols);
+        if (a_vals && !C.h_cols.empty()) {
+            C.A.download(a_vals, C.h_cols.size(), f.stream);
+            OAGPU_HIP(hipStreamSynchronize(f.stream));
+        }
+    });
+}
+
 namespace {
 // simulation::assembleAndSolveSystems -> segregatedFlowEquations::solve
 // (segregatedFlowEquations.cpp:76-297) for one p' sub-iteration

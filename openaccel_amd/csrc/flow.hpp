@@ -30,6 +30,59 @@ struct AmgLevel {
     DBuf<float> fC1, fC2, fV1, fR2;
 };
 
+// One level of the smoothed-aggregation hierarchy (amg.hip: sa_build).  Level
+// l holds its matrix (levels >= 1, CSR; level 0 is the flow's scalar system),
+// the prolongation P_l from level l + 1 to level l, and the lists that
+// compute P_l, B_l = A_l P_l and A_{l+1} = P_l^T B_l on the device.
+struct SaLevel {
+    int n = 0;
+    // matrix of this level (levels >= 1): CSR, the diagonal's index per row,
+    // row kernels with `lanes` lanes per row
+    int lanes = 4;
+    DBuf<int> a_rp, a_ci, a_diag;
+    DBuf<double> A, dinv;
+    DBuf<float> fA, fdinv;
+    // P_l (levels below the last): sliced-ELL over this level's rows (padding
+    // columns -1) for the prolongation; per CSR entry of P_l its slot there,
+    // whether its column is the row's own aggregate, and the couplings of
+    // A_l (value indices) summed into it
+    double omega_p = 0.0;
+    size_t p_nsell = 0;
+    DBuf<int> p_slice_ptr, p_slice_w, p_cols, prp, ppos, pt_ptr, pt_idx;
+    DBuf<unsigned char> own;
+    DBuf<double> psum, P;
+    DBuf<float> fP;
+    // B_l = A_l P_l, CSR over this level's rows: entry e = sum over its terms
+    // (value index of A_l, slot of P_l)
+    size_t nb = 0;
+    DBuf<int> b_ptr;
+    DBuf<int2> b_terms;
+    DBuf<double> B;
+    // A_{l+1} in CSR order: entry e = sum over its terms (slot of P_l, entry of B_l)
+    size_t nac = 0;
+    DBuf<int> c_ptr;
+    DBuf<int2> c_terms;
+    // restriction P_l^T: row I of level l + 1 = sum over (row of level l,
+    // slot of P_l), r_lanes lanes per row
+    int r_lanes = 4;
+    DBuf<int> r_ptr;
+    DBuf<int2> r_terms;
+    // cycle vectors: right-hand side, pre-smoothed x, residual, prolongated t, result
+    DBuf<double> R, X, S, T, Y;
+    DBuf<float> fR, fX, fS, fT, fY;
+    // host copies (tests): CSR pattern of this level's matrix; aggregates
+    // and CSR pattern of P_l
+    std::vector<int> h_rp, h_cols, h_agg, h_prp, h_pcols;
+};
+
+struct Sa {
+    bool built = false;
+    std::vector<SaLevel> lv;
+    int dense_n = 0;           // coarsest level solved by its dense inverse when > 0
+    DBuf<double> dense;
+    DBuf<float> fdense;
+};
+
 struct Amg {
     bool built = false;
     bool last_float = false;   // precision of the last numeric set-up
@@ -118,6 +171,7 @@ struct Flow {
     unsigned prof_mask = 0;
     KernelClock clk[kClkCount];
     Amg amg;
+    Sa sa;
     CgGraph cg_graph;
     // effective viscosity mu + mu_t per node (mu alone while laminar)
     DBuf<double> mueff;
@@ -258,5 +312,13 @@ template <typename V>
 void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* rz_part, V* p_init,
                 bool timing = false, bool kcycle = false, const V* q = nullptr,
                 double* qz_part = nullptr, const float* A0f = nullptr);
+// Smoothed aggregation: hierarchy structure from the flow's first scalar
+// system (once), numeric set-up of every level from the FP64 fine values A0
+// (FP32 copies with want_float), and one V(1,1)-cycle z = M^-1 r on values
+// and vectors of type V (A0 / dinv0: the fine level's, of type V).
+void sa_build(Flow& f);
+void sa_setup(Flow& f, const double* A0, bool want_float);
+template <typename V>
+void sa_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, bool timing = false);
 
 }  // namespace oagpu

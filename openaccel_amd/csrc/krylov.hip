@@ -919,7 +919,7 @@ struct BicgVecs {
 template <typename V>
 static int bicgstab_amg_core(Flow& f, const V* A, const float* Af, const V* b, const V* dinv, V* x,
                              const BicgVecs<V>& w, int max_it, double rtol, double atol,
-                             double* r0out, double* r1out) {
+                             double* r0out, double* r1out, bool sa = false) {
     DevMesh& m = *f.m;
     const size_t n = size_t(m.nn);
     const unsigned B = 256, G = unsigned((n + B - 1) / B);
@@ -940,8 +940,11 @@ static int bicgstab_amg_core(Flow& f, const V* A, const float* Af, const V* b, c
         return d;
     };
     auto apply = [&](const V* r, V* z) {
-        amg_vcycle<V>(f, A, dinv, r, z, f.part.p + 8 * kRedGrid, nullptr, true, false, nullptr,
-                      nullptr, Af);
+        if (sa)
+            sa_vcycle<V>(f, A, dinv, r, z, true);
+        else
+            amg_vcycle<V>(f, A, dinv, r, z, f.part.p + 8 * kRedGrid, nullptr, true, false, nullptr,
+                          nullptr, Af);
         OAGPU_HIP(hipGetLastError());
     };
     // r . r and r0 . r in one pass over r (r0 = r at a start or restart)
@@ -1005,7 +1008,8 @@ static int bicgstab_amg_core(Flow& f, const V* A, const float* Af, const V* b, c
 // false: FP64 BiCGStab whose V-cycle reads the fine level's values in FP32).
 constexpr bool kMixedBicgInner = true;
 
-static int bicgstab_amg_mixed(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
+static int bicgstab_amg_mixed(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd,
+                              bool sa) {
     DevMesh& m = *f.m;
     const size_t n = size_t(m.nn);
     const unsigned B = 256;
@@ -1014,7 +1018,7 @@ static int bicgstab_amg_mixed(Flow& f, int max_it, double rtol, double atol, oag
     double* x = f.x1.p;
     if (f.Af.n < m.nsell) f.Af.alloc(m.nsell);
     k_to_float<<<unsigned((m.nsell + B - 1) / B), B, 0, f.stream>>>(m.nsell, 1.0, A, f.Af.p);
-    if (!kMixedBicgInner) {
+    if (!kMixedBicgInner && !sa) {
         amg_setup<double>(f, A, f.dinv.p);
         BicgVecs<double> w{f.r.p, f.r0.p, f.pk.p, f.q.p, f.s.p, f.t.p, f.z.p, f.w.p};
         return bicgstab_amg_core<double>(f, A, f.Af.p, b, f.dinv.p, x, w, max_it, rtol, atol,
@@ -1023,7 +1027,10 @@ static int bicgstab_amg_mixed(Flow& f, int max_it, double rtol, double atol, oag
     for (auto* v : {&f.fb, &f.fx, &f.fr, &f.fz, &f.fp, &f.fq, &f.fdinv, &f.fr0, &f.fs, &f.ft, &f.fsh})
         if (v->n < n) v->alloc(n);
     k_to_float<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, 1.0, f.dinv.p, f.fdinv.p);
-    amg_setup<float>(f, f.Af.p, f.fdinv.p);
+    if (sa)
+        sa_setup(f, A, true);  // Galerkin products in FP64, FP32 copies for the inner solve
+    else
+        amg_setup<float>(f, f.Af.p, f.fdinv.p);
     OAGPU_HIP(hipMemsetAsync(x, 0, n * sizeof(double), f.stream));
     OAGPU_HIP(hipMemcpyAsync(f.r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, f.stream));
     double rn;
@@ -1041,7 +1048,7 @@ static int bicgstab_amg_mixed(Flow& f, int max_it, double rtol, double atol, oag
         k_to_float<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, 1.0 / rn, f.r.p, f.fb.p);
         double i0, i1;
         const int it = bicgstab_amg_core<float>(f, f.Af.p, nullptr, f.fb.p, f.fdinv.p, f.fx.p, w,
-                                                max_it - used, inner, 0.0, &i0, &i1);
+                                                max_it - used, inner, 0.0, &i0, &i1, sa);
         used += std::max(it, 1);
         k_refine<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, rn, f.fx.p, x);
         spmv<1>(f, A, x, nullptr, f.w.p);
@@ -1055,14 +1062,19 @@ static int bicgstab_amg_mixed(Flow& f, int max_it, double rtol, double atol, oag
     return used;
 }
 
-static int bicgstab_amg_solve(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd) {
-    amg_build(f);
-    if (f.amg.lv.size() <= 1) return bicgstab<1>(f, max_it, rtol, atol, cd);  // tiny: Jacobi
-    if (f.prm.precision == OAGPU_MIXED) return bicgstab_amg_mixed(f, max_it, rtol, atol, cd);
-    amg_setup<double>(f, f.A1.p, f.dinv.p);
+// sa: the smoothed-aggregation hierarchy (OAGPU_BICGSTAB_SA), else the
+// unsmoothed one (OAGPU_BICGSTAB_AMG)
+static int bicgstab_amg_solve(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd,
+                              bool sa) {
+    if (sa) sa_build(f);
+    else amg_build(f);
+    if ((sa ? f.sa.lv.size() : f.amg.lv.size()) <= 1) return bicgstab<1>(f, max_it, rtol, atol, cd);  // tiny: Jacobi
+    if (f.prm.precision == OAGPU_MIXED) return bicgstab_amg_mixed(f, max_it, rtol, atol, cd, sa);
+    if (sa) sa_setup(f, f.A1.p, false);
+    else amg_setup<double>(f, f.A1.p, f.dinv.p);
     BicgVecs<double> w{f.r.p, f.r0.p, f.pk.p, f.q.p, f.s.p, f.t.p, f.z.p, f.w.p};
     return bicgstab_amg_core<double>(f, f.A1.p, nullptr, f.b1.p, f.dinv.p, f.x1.p, w, max_it, rtol,
-                                     atol, &cd->solver_initial_res, &cd->solver_final_res);
+                                     atol, &cd->solver_initial_res, &cd->solver_final_res, sa);
 }
 
 // SpMV block-size candidates (64, 128, 256 rows per workgroup) timed on the
@@ -1149,9 +1161,9 @@ int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double ato
     }
     else if (method == OAGPU_BICGSTAB)
         it = N == 3 ? bicgstab<3>(f, max_it, rtol, atol, cd) : bicgstab<1>(f, max_it, rtol, atol, cd);
-    else if (method == OAGPU_BICGSTAB_AMG) {
+    else if (method == OAGPU_BICGSTAB_AMG || method == OAGPU_BICGSTAB_SA) {
         OAGPU_REQUIRE(N == 1, "AMG preconditioning is for scalar systems (N = 1)");
-        it = bicgstab_amg_solve(f, max_it, rtol, atol, cd);
+        it = bicgstab_amg_solve(f, max_it, rtol, atol, cd, method == OAGPU_BICGSTAB_SA);
     } else
         throw Error(OAGPU_EINVAL, "unknown Krylov method");
     cd->n_iterations = it;
