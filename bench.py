@@ -43,11 +43,13 @@ CTL = [20, 0.1, 200, 1.0e-6]   # momentum GMRES max_it/rtol, p' max_it/rtol
 
 
 # ------------------------------------------------------------------ bytes
-def algorithmic_bytes(nn, ne, nnz, value_bytes, vector_bytes, high_res=False):
+def algorithmic_bytes(nn, ne, nnz, value_bytes, vector_bytes, high_res=False, sa=None):
     """Compulsory HBM bytes of one launch of each clocked kernel on a hex8
     mesh: each array read once, each output written once, FP64 fields, int32
     indices.  value_bytes / vector_bytes: the AMG fine level's matrix values
-    and vectors (FP64 form 8 / 8; the mixed form's FP32 inner solves 4 / 4)."""
+    and vectors (FP64 form 8 / 8; the mixed form's FP32 inner solves 4 / 4).
+    sa = (prolongation entries of the fine level, rows of level 1): the
+    smoothed-aggregation cycle, whose fine-level post-smoothing reads P."""
     conn, mdot, slots = 4 * 8, 8 * 12, 4 * 32
     elem = ne * (conn + mdot + slots)
     hr = nn * (72 + 24) if high_res else 0
@@ -61,6 +63,9 @@ def algorithmic_bytes(nn, ne, nnz, value_bytes, vector_bytes, high_res=False):
     v, w = value_bytes, vector_bytes
     amg_pre = nnz * (v + 4) + nn * w * 4                     # A, cols; r, dinv in; x, res out
     amg_post = nnz * (v + 4) + nn * (w * 5 + 4)              # A, cols; x, r, dinv, e, z; agg
+    if sa:
+        npf, n1 = sa   # t = x + P e (P values, columns; e; x in, t out), then smoothing of t
+        amg_post = npf * (v + 4) + n1 * w + nn * 2 * w + nnz * (v + 4) + nn * 4 * w
     spmv1 = nnz * (8 + 4) + nn * 16
     spmv3 = nnz * (72 + 4) + nn * 48
     spmv1f = nnz * (4 + 4) + nn * 8
@@ -83,6 +88,157 @@ def measured_traffic(kernel, workload, precision):
     if not rec:
         return None
     return int(rec["fetch_kib"] * 1024 * t["fetch_scale"] + rec["write_kib"] * 1024 * t["write_scale"])
+
+
+# ---------------------------------------------------------------- profiles
+# rocprofv3 output of the default workload -> the summaries under profiles/
+# (python bench.py --summarize-profiles DIR, DIR holding the runs that
+# bench.py --profile-commands prints)
+PROFILE_REGEX = ("k_scalar_rows|k_mdot_update|k_momentum_rows|k_pcorr_rows|k_gradient|k_spmv|"
+                 "k_amg_pre_residual|k_sa_prolong|k_amg_smooth|copyBuffer")
+PROFILE_COMMANDS = [
+    "cd /tmp && export TMPDIR=/tmp && timeout -k 10 600 rocprofv3 --kernel-trace --stats --output-format csv "
+    "-d {dir}/trace -o trace -- python3 /root/repo/bench.py --no-cpu --steps 3 --warmup 2",
+    "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 300 rocprofv3 --pmc FETCH_SIZE --output-format csv "
+    f"--kernel-include-regex \"{PROFILE_REGEX}\" -d {{dir}}/fetch -o fetch -- python3 /root/repo/bench.py "
+    "--steps 1 --warmup 1 --no-cpu",
+    "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 300 rocprofv3 --pmc WRITE_SIZE --output-format csv "
+    f"--kernel-include-regex \"{PROFILE_REGEX}\" -d {{dir}}/write -o write -- python3 /root/repo/bench.py "
+    "--steps 1 --warmup 1 --no-cpu",
+    "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 60 rocprofv3 --pmc FETCH_SIZE --output-format csv "
+    "-d {dir}/fetchcopy -o fetchcopy -- python3 /root/repo/bench.py --copy-probe 1024",
+    "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 60 rocprofv3 --pmc WRITE_SIZE --output-format csv "
+    "-d {dir}/writecopy -o writecopy -- python3 /root/repo/bench.py --copy-probe 1024",
+]
+# clock -> (mixed form's kernel symbol, FP64 form's); None: the same symbol in
+# both forms, told apart by dispatch time (the mixed form runs first)
+PROFILE_SYMBOLS = {
+    "scalar": (r"k_scalar_rows<float", r"k_scalar_rows<double"),
+    "momentum": (r"k_momentum_rows<float", r"k_momentum_rows<double"),
+    "pcorr": (r"k_pcorr_rows(_occ)?<float", r"k_pcorr_rows(_occ)?<double"),
+    "mdot": (r"k_mdot_update<", None),
+    "gradU": (r"k_gradient<3,", None),
+    "gradP": (r"k_gradient<1,", None),
+    "spmv1_f32": (r"k_spmv<1, float>", None),
+    "spmv1": (None, r"k_spmv<1, double>"),
+    "spmv3_f32": (r"k_spmv<3, float>", None),
+    "spmv3": (r"k_spmv<3, double>", r"k_spmv<3, double>"),
+    "amg_pre": (r"k_amg_pre_residual<float, true", r"k_amg_pre_residual<double, true"),
+}
+
+
+def _csv_rows(path):
+    import csv
+    with open(path) as fh:
+        return list(csv.DictReader(fh))
+
+
+def _find_csv(d, suffix):
+    for root, _, files in os.walk(d):
+        for fn in files:
+            if fn.endswith(suffix):
+                return os.path.join(root, fn)
+    raise FileNotFoundError(f"no *{suffix} under {d}")
+
+
+def summarize_traffic(d, workload_desc):
+    """profiles/traffic_<clock>_bfs.json from the FETCH_SIZE / WRITE_SIZE runs:
+    per form, the mean counter value (KiB per dispatch) over the dispatches
+    of the fine-level grid, and the copy-kernel calibration."""
+    import re
+    cal = {}
+    for cnt in ("fetch", "write"):
+        rows = _csv_rows(_find_csv(os.path.join(d, cnt + "copy"), "counter_collection.csv"))
+        big = max(rows, key=lambda r: float(r["Counter_Value"]))
+        cal[cnt] = (float(big["Counter_Value"]), big["Kernel_Name"])
+    # 1 GiB read and written by the copy: bytes = scale * KiB * 1024
+    fscale, wscale = (1 << 20) / cal["fetch"][0], (1 << 20) / cal["write"][0]
+    data = {}
+    for cnt in ("fetch", "write"):
+        rows = _csv_rows(_find_csv(os.path.join(d, cnt), "counter_collection.csv"))
+        t64 = min((int(r["Start_Timestamp"]) for r in rows if "k_momentum_rows<double" in r["Kernel_Name"]),
+                  default=1 << 62)
+        data[cnt] = (rows, t64)
+    written = []
+    for clock, pats in PROFILE_SYMBOLS.items():
+        rec = {"kernel": clock, "workload": workload_desc,
+               "counters": "FETCH_SIZE and WRITE_SIZE in KiB per dispatch, mean over the dispatches "
+                           "of the fine-level grid in one run each",
+               "fetch_scale": round(fscale, 5), "write_scale": round(wscale, 5),
+               "calibration": f"{cal['fetch'][1].split('(')[0]} of 1 GiB (bench.py --copy-probe 1024): "
+                              f"FETCH_SIZE {cal['fetch'][0]:.1f} KiB, WRITE_SIZE {cal['write'][0]:.1f} KiB",
+               "commands": [c.format(dir="gpurun_out/final") for c in PROFILE_COMMANDS[1:]]}
+        for form, k in (("mixed", 0), ("fp64", 1)):
+            pat = pats[k] if pats[k] is not None else pats[1 - k]
+            if pat is None:
+                continue
+            vals = {}
+            for cnt in ("fetch", "write"):
+                rows, t64 = data[cnt]
+                sel = [r for r in rows if re.search(pat, r["Kernel_Name"])]
+                if pats[1 - k] is None or pats[k] is None or pats[0] == pats[1]:
+                    sel = [r for r in sel if (int(r["Start_Timestamp"]) >= t64) == (form == "fp64")]
+                if not sel:
+                    continue
+                g = max(int(r["Grid_Size"]) for r in sel)
+                sel = [r for r in sel if int(r["Grid_Size"]) == g]
+                vals[cnt] = (sum(float(r["Counter_Value"]) for r in sel) / len(sel), len(sel), g,
+                             sel[0]["Kernel_Name"].split("(")[0].replace("void ", ""))
+            if len(vals) == 2:
+                rec[form] = {"symbol": vals["fetch"][3], "grids": [vals["fetch"][2]],
+                             "dispatches": [vals["fetch"][1], vals["write"][1]],
+                             "fetch_kib": round(vals["fetch"][0], 1), "write_kib": round(vals["write"][0], 1)}
+        if "mixed" in rec or "fp64" in rec:
+            path = os.path.join(ROOT, "profiles", f"traffic_{clock}_bfs.json")
+            with open(path, "w") as fh:
+                json.dump(rec, fh, indent=1)
+            written.append(path)
+    return written
+
+
+def summarize_trace(d, steps=3):
+    """Per-iteration kernel table of the timed outer iterations of each form
+    (windows from one momentum assembly launch to the next)."""
+    import re
+    rows = sorted(_csv_rows(_find_csv(os.path.join(d, "trace"), "kernel_trace.csv")),
+                  key=lambda r: int(r["Start_Timestamp"]))
+    short = lambda n: re.sub(r"\(.*", "", n).replace("void ", "")
+    marks = [i for i, r in enumerate(rows) if "k_momentum_rows" in r["Kernel_Name"]]
+    wins = {"mixed": [], "fp64": []}
+    for a, b in zip(marks, marks[1:]):
+        form = "mixed" if "k_momentum_rows<float" in rows[a]["Kernel_Name"] else "fp64"
+        busy = sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in rows[a:b]) / 1e6
+        wall = (int(rows[b]["Start_Timestamp"]) - int(rows[a]["Start_Timestamp"])) / 1e6
+        if b - a > 100 and wall < 2 * busy:
+            wins[form].append((a, b, busy, wall))
+    out = []
+    for form, ws in wins.items():
+        ws = ws[-steps:]
+        W = len(ws)
+        agg = {}
+        for a, b, _, _ in ws:
+            for r in rows[a:b]:
+                k = short(r["Kernel_Name"])
+                e = agg.setdefault(k, [0, 0.0, r["VGPR_Count"], r["Accum_VGPR_Count"], r["LDS_Block_Size"], 0])
+                e[0] += 1
+                e[1] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+                e[5] = max(e[5], int(r["Grid_Size_X"]))
+        busy = sum(w[2] for w in ws) / W
+        wall = sum(w[3] for w in ws) / W
+        nl = sum(v[0] for v in agg.values()) / W
+        out.append(f"## {form}: {nl:.0f} launches / iteration, device busy {busy:.2f} ms of {wall:.2f} ms wall")
+        out.append("| kernel | launches / it | avg us | ms / it | % busy | VGPR | AGPR | LDS B | largest grid |")
+        out.append("|---|---|---|---|---|---|---|---|---|")
+        items = sorted(agg.items(), key=lambda x: -x[1][1])
+        rest = items[28:]
+        for k, v in items[:28]:
+            out.append(f"| `{k}` | {v[0] / W:.1f} | {v[1] / v[0]:.1f} | {v[1] / W / 1e3:.3f} | "
+                       f"{100 * v[1] / W / 1e3 / busy:.1f} | {v[2]} | {v[3]} | {v[4]} | {v[5]} |")
+        if rest:
+            out.append(f"| ({len(rest)} more kernels) | {sum(v[0] for _, v in rest) / W:.1f} | | "
+                       f"{sum(v[1] for _, v in rest) / W / 1e3:.3f} | | | | | |")
+        out.append("")
+    return "\n".join(out)
 
 
 # ----------------------------------------------------------------- driver
@@ -288,12 +444,15 @@ def bench_bfs(args):
 
         dom = None if prec == oa.OAGPU_MIXED else fp64_counterpart(res[oa.OAGPU_MIXED]["dominant"])
         res[prec] = run_form(oa, f, step, args.steps, args.warmup, dom)
+        res[prec]["sa_levels"] = f.sa_levels() if args.bfs_p_method == "bicgstab_sa" else []
         f.close()
         print(f"[bench] {'mixed' if prec else 'fp64'} {args.steps / res[prec]['seconds']:.3f} it/s, "
               f"dominant {res[prec]['dominant']}", file=sys.stderr, flush=True)
     rmx, r64 = res[oa.OAGPU_MIXED], res[oa.OAGPU_FP64]
-    ab_mx = algorithmic_bytes(nn, ne, nnz, 4, 4)
-    ab_64 = algorithmic_bytes(nn, ne, nnz, 8, 8)
+    lv = rmx["sa_levels"]
+    sa = (lv[0][1], lv[1][0]) if len(lv) > 1 else None
+    ab_mx = algorithmic_bytes(nn, ne, nnz, 4, 4, sa=sa)
+    ab_64 = algorithmic_bytes(nn, ne, nnz, 8, 8, sa=sa)
     it64, itmx = args.steps / r64["seconds"], args.steps / rmx["seconds"]
     print("[bench] coefficient error of the mixed form", file=sys.stderr, flush=True)
     cerr = coefficient_error(oa, dm, make_flow, lambda f: bfs_setup(oa, mesh, f))
@@ -341,6 +500,8 @@ def bench_bfs(args):
                  **hist(r64)},
         "mixed_over_fp64": round(itmx / it64, 3),
         "coef_err_vs_fp64": cerr,
+        "p_amg_levels": [{"rows": l[0], "entries": l[2], "p_entries": l[1], "omega_p": round(l[3], 4)}
+                         for l in rmx["sa_levels"]],
         **hist(rmx),
         "cpu_baseline": cpu,
     }
@@ -747,6 +908,11 @@ def main():
     ap.add_argument("--duct-scale", type=float, default=1.0)
     ap.add_argument("--bfs-scale", type=float, default=1.0,
                     help="scale of the step's cell counts per direction (1: 8M elements)")
+    ap.add_argument("--summarize-profiles", metavar="DIR", default="",
+                    help="write profiles/traffic_*_bfs.json and print the per-iteration kernel table "
+                         "from the rocprofv3 runs under DIR (--profile-commands lists them)")
+    ap.add_argument("--profile-commands", action="store_true",
+                    help="print the rocprofv3 commands whose output --summarize-profiles reads")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -754,6 +920,16 @@ def main():
         print("bench.py measures one MI355X only: the multi-GPU path (element partition + "
               "halo exchange) is designed but not built; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.profile_commands:
+        print("\n".join(c.format(dir="gpurun_out/final") for c in PROFILE_COMMANDS))
+        return
+    if args.summarize_profiles:
+        for p in summarize_traffic(args.summarize_profiles,
+                                   "backward-facing step k-omega SST, 7996800 hex8 (8120619 nodes), "
+                                   "bench.py defaults"):
+            print(p, file=sys.stderr)
+        print(summarize_trace(args.summarize_profiles))
+        return
     if args.copy_probe:
         import torch
         nb = args.copy_probe << 20

@@ -368,13 +368,15 @@ int oagpu_sst_iteration(oagpu_flow *f, const double *ctl, double *hist);
  * and the Krylov SpMV at 64 / 128 / 256 rows per workgroup (FP64, and FP32
  * for the mixed form).  Assembles the momentum and p' systems of the current
  * state as a side effect.  report (n_report >= 18 doubles, NULL: none):
- * per tunable -- momentum, pcorr, spmv1, spmv3, spmv1_f32, spmv3_f32 -- the
- * chosen variant (1 tile / 0 HBM; 0 limited / 1 compiler; rows per
- * workgroup), its time and the default variant's time in ms. */
+ * per tunable -- momentum, pcorr, spmv1, spmv3, spmv1_f32, spmv3_f32,
+ * scalar, mdot, grad -- the chosen variant (1 tile / 0 HBM; 0 limited / 1
+ * compiler; rows per workgroup; resident waves per CU, 0 uncapped), its time
+ * and the default variant's time in ms. */
 int oagpu_autotune(oagpu_flow *f, double *report, int n_report);
 /* Set one tunable's variant directly (e.g. from an earlier oagpu_autotune
  * report): which = "momentum" (1 / 0), "pcorr" (0 / 1), "spmv1", "spmv3",
- * "spmv1_f32", "spmv3_f32" (64 / 128 / 256). */
+ * "spmv1_f32", "spmv3_f32" (64 / 128 / 256), "scalar", "mdot", "grad"
+ * (resident waves per CU: 0, 8 or 16). */
 int oagpu_tune_set(oagpu_flow *f, const char *which, int variant);
 
 /* Device time of the library's main kernels, measured with HIP events on the

@@ -479,12 +479,12 @@ class FlowCase:
     CLOCKS = ("momentum", "pcorr", "scalar", "mdot", "gradU", "gradP", "amg_pre", "amg_post",
               "amg_tail", "spmv1", "spmv3", "spmv1_f32", "spmv3_f32")
 
-    TUNABLES = ("momentum", "pcorr", "spmv1", "spmv3", "spmv1_f32", "spmv3_f32")
+    TUNABLES = ("momentum", "pcorr", "spmv1", "spmv3", "spmv1_f32", "spmv3_f32", "scalar", "mdot", "grad")
 
     def autotune(self):
         """Time the tunable launches' variants on this flow and keep the
         fastest; {kernel: (variant, ms, default ms)}."""
-        r = np.zeros(18)
+        r = np.zeros(3 * len(self.TUNABLES))
         _check(load_library().oagpu_autotune(self.h, _d(r), r.size))
         return {k: (r[3 * i], r[3 * i + 1], r[3 * i + 2]) for i, k in enumerate(self.TUNABLES)}
 

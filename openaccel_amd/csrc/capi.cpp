@@ -1071,6 +1071,11 @@ int oagpu_tune_set(oagpu_flow* h, const char* which, int variant) {
                    !std::strcmp(which, "spmv1_f32") || !std::strcmp(which, "spmv3_f32")) {
             OAGPU_REQUIRE(blk, "SpMV variant: 64, 128 or 256 rows per workgroup");
             t.spmv_block[which[4] == '3'][std::strstr(which, "f32") != nullptr] = unsigned(variant);
+        } else if (!std::strcmp(which, "scalar") || !std::strcmp(which, "mdot") ||
+                   !std::strcmp(which, "grad")) {
+            OAGPU_REQUIRE(variant == 0 || variant == 8 || variant == 16,
+                          "occupancy cap: 0, 8 or 16 waves per CU");
+            (which[0] == 's' ? t.cap_scalar : which[0] == 'm' ? t.cap_mdot : t.cap_grad) = variant;
         } else {
             throw oagpu::Error(OAGPU_EINVAL, std::string("unknown tunable ") + which);
         }

@@ -1,6 +1,8 @@
 // flow.hpp -- device state of a segregated pressure-velocity case.
 #pragma once
 
+#include <algorithm>
+
 #include "../../include/oagpu.h"
 #include "mesh.hpp"
 
@@ -165,6 +167,11 @@ struct Flow {
         int mom_tile = 1;        // momentum rows: 1 LDS tile, 0 accumulate in HBM
         int pc_occupancy = 0;    // p' rows: 0 at most 2 waves per SIMD, 1 compiler's choice
         unsigned spmv_block[2][2] = {{256, 256}, {256, 256}};  // [N == 3][FP32]
+        // resident waves per CU of the k / omega rows, the mass-flow update and
+        // the gradients (0: as many as registers and LDS allow), capped by
+        // reserving LDS per workgroup: a smaller window of rows in flight per
+        // XCD keeps more of their gathered node and element data in its L2
+        int cap_scalar = 0, cap_mdot = 0, cap_grad = 0;
         bool tuned = false;
     } tune;
     // device time of the clocked kernels (profiling on only)
@@ -221,6 +228,15 @@ inline void bdf_coeff(const Flow& f, double c[3]) {
         c[1] = -(1.0 + om);
         c[2] = om * om / (1.0 + om);
     }
+}
+
+// Dynamic LDS bytes per workgroup of `block` threads (with `static_lds` bytes
+// of its own) that hold a CU's 160 KiB at `cap` resident waves (0: none).
+inline unsigned occupancy_lds(unsigned block, unsigned static_lds, int cap) {
+    if (cap <= 0) return 0;
+    const unsigned wg_per_cu = std::max(1u, unsigned(cap) * 64u / block);
+    const unsigned per = 163840u / wg_per_cu;
+    return per > static_lds + 256 ? std::min(per - static_lds - 256, 64512u - static_lds) : 0;
 }
 
 // start / stop of a clocked launch on the flow's stream (no-ops unless the

@@ -1984,9 +1984,10 @@ void launch_assemble_scalar(Flow& f, int which) {
     const bool mixed = f.prm.precision == OAGPU_MIXED;
     const bool gs = (which == 0 ? f.sst.kgrad_shifted : f.sst.wgrad_shifted) != 0;
     const bool hr = a.beta != nullptr;
+    const unsigned lds = occupancy_lds(B, kLdsSlots * 64 * sizeof(double), f.tune.cap_scalar);
     auto go = [&](auto kern) {
         clock_begin(f, kClkScalar);
-        kern<<<G, B, 0, f.stream>>>(a);
+        kern<<<G, B, lds, f.stream>>>(a);
         clock_end(f, kClkScalar);
     };
     auto pick = [&](auto mx) {
@@ -2211,8 +2212,9 @@ void launch_mdot_init(Flow& f) {
 void launch_update_mdot(Flow& f) {
     DevMesh& m = *f.m;
     const unsigned B = 256, G = xcd_grid(m.ne, B);
+    const unsigned lds = occupancy_lds(B, 4 * 64 * 12 * sizeof(double), f.tune.cap_mdot);
     auto args = [&](auto kern) {
-        kern<<<G, B, 0, f.stream>>>(m.ne, m.negroup, m.egroup_order.p, m.mixed ? m.etype.p : nullptr,
+        kern<<<G, B, lds, f.stream>>>(m.ne, m.negroup, m.egroup_order.p, m.mixed ? m.etype.p : nullptr,
                                     m.xyz.p, m.conn.p, m.scvw.p,
                                     f.U.p, f.du.p, f.p.p, f.gradP.p, f.prm.rho, f.prm.urf_mdot,
                                     f.mdot.p, f.err.p);
@@ -2579,9 +2581,10 @@ void launch_gradient(Flow& f, int which) {
     const double* sym = f.has_sym && which <= 1 ? f.symA.p : nullptr;
     const int* types = f.has_sym ? f.bc_type.p : nullptr;  // symmetry faces add no term
     const int clk = which == 0 ? kClkGradU : kClkGradP;
+    const unsigned lds = occupancy_lds(B, 0, f.tune.cap_grad);
     auto go = [&](auto kern, const double* phi, double* grad) {
         clock_begin(f, clk);
-        kern<<<G, B, 0, f.stream>>>(m.nn, m.nslice, m.slice_order.p, m.xyz.p, m.conn.p, m.inc_la.p,
+        kern<<<G, B, lds, f.stream>>>(m.nn, m.nslice, m.slice_order.p, m.xyz.p, m.conn.p, m.inc_la.p,
                                     m.slots_la.p, m.ext_ptr.p, m.ext.p, m.ext_slots.p, m.binc_ptr.p,
                                     m.binc.p, m.bf_elem.p, m.bf_ord.p, m.bf_bc.p, types, sym,
                                     m.vol.p, phi, m.mixed ? m.gen_ptr.p : nullptr, m.gen.p,

@@ -10,7 +10,11 @@
 // * p' rows: at most 2 waves per SIMD or the compiler's occupancy;
 // * sliced-ELL SpMV of the Krylov solvers (momentum and scalar systems, FP64
 //   and the mixed form's FP32 inner solves): 64, 128 or 256 rows per
-//   workgroup.
+//   workgroup;
+// * k / omega rows, mass-flow update and gradients: resident waves per CU
+//   capped at 8, 16 or not at all (LDS reserved per workgroup).  The
+//   mass-flow update relaxes the stored rates in place, so its candidates run
+//   on a saved copy of them that is put back afterwards.
 #include "flow.hpp"
 
 namespace oagpu {
@@ -35,7 +39,7 @@ double time_ms(Flow& f, F&& fn, int reps) {
 }  // namespace
 
 void autotune(Flow& f, double* report, int n_report) {
-    double rep[6][3];
+    double rep[9][3];
     for (auto& r : rep) r[0] = r[1] = r[2] = 0.0;
     const unsigned saved_mask = f.prof_mask;
     f.prof_mask = 0;  // the candidates' launches are not clocked
@@ -87,11 +91,48 @@ void autotune(Flow& f, double* report, int n_report) {
             ++row;
         }
     }
+    // occupancy caps: the fastest of none / 16 / 8 waves per CU
+    auto caps = [&](int& cap, int r, auto&& run) {
+        double tb = 1e300, t0 = 0.0;
+        int best = 0;
+        for (int c : {0, 16, 8}) {
+            cap = c;
+            const double t = time_ms(f, run, 3);
+            if (c == 0) t0 = t;
+            if (t < tb) {
+                tb = t;
+                best = c;
+            }
+        }
+        cap = best;
+        rep[r][0] = best;
+        rep[r][1] = tb;
+        rep[r][2] = t0;
+    };
+    if (f.sst_on) caps(f.tune.cap_scalar, 6, [&] { launch_assemble_scalar(f, 0); });
+    {
+        DBuf<double> md(f.mdot.n), sd;
+        OAGPU_HIP(hipMemcpyAsync(md.p, f.mdot.p, f.mdot.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                 f.stream));
+        if (f.msd.p) {
+            sd.alloc(f.msd.n);
+            OAGPU_HIP(hipMemcpyAsync(sd.p, f.msd.p, f.msd.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                     f.stream));
+        }
+        caps(f.tune.cap_mdot, 7, [&] { launch_update_mdot(f); });
+        OAGPU_HIP(hipMemcpyAsync(f.mdot.p, md.p, f.mdot.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                 f.stream));
+        if (f.msd.p)
+            OAGPU_HIP(hipMemcpyAsync(f.msd.p, sd.p, f.msd.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                     f.stream));
+        OAGPU_HIP(hipStreamSynchronize(f.stream));
+    }
+    caps(f.tune.cap_grad, 8, [&] { launch_gradient(f, 1); });
     OAGPU_HIP(hipStreamSynchronize(f.stream));
     f.prof_mask = saved_mask;
     f.tune.tuned = true;
     if (report)
-        for (int i = 0; i < 6 && 3 * i + 2 < n_report; ++i)
+        for (int i = 0; i < 9 && 3 * i + 2 < n_report; ++i)
             for (int k = 0; k < 3; ++k) report[3 * i + k] = rep[i][k];
 }
 

@@ -78,3 +78,32 @@ def test_spmv_block_sizes_and_report(oracle, oa, meshgen):
         assert rep[k][0] in (64, 128, 256) and rep[k][1] > 0 and rep[k][2] >= rep[k][1]
     with pytest.raises(oa.OagpuError):
         f.tune_set("spmv1", 100)
+
+
+def test_occupancy_caps_bitwise_and_state_kept(oracle, oa, meshgen):
+    """Capping the resident waves of the mass-flow update and the gradients
+    changes no bit of their results, and the tuner, which times the
+    mass-flow update on the flow's state, leaves the stored rates as they
+    were."""
+    print("[case] occupancy caps of the mass-flow update and gradients", flush=True)
+    c, dm, f = pair(oracle, oa, meshgen)
+    f.assemble_momentum()
+    md0 = f.download("mdot")
+    out = []
+    for cap in (0, 8, 16):
+        f.tune_set("mdot", cap)
+        f.tune_set("grad", cap)
+        f.upload("mdot", md0)
+        f.update_mdot()
+        f.gradient("p")
+        out.append((f.download("mdot"), f.download("gradP")))
+    for o in out[1:]:
+        np.testing.assert_array_equal(o[0], out[0][0])
+        np.testing.assert_array_equal(o[1], out[0][1])
+    f.upload("mdot", md0)
+    rep = f.autotune()
+    np.testing.assert_array_equal(f.download("mdot"), md0)
+    for k in ("mdot", "grad"):
+        assert rep[k][0] in (0, 8, 16) and rep[k][1] > 0
+    with pytest.raises(oa.OagpuError):
+        f.tune_set("mdot", 12)
