@@ -141,10 +141,12 @@ def _find_csv(d, suffix):
     raise FileNotFoundError(f"no *{suffix} under {d}")
 
 
-def summarize_traffic(d, workload_desc):
+def summarize_traffic(d, workload_desc, timed=None):
     """profiles/traffic_<clock>_bfs.json from the FETCH_SIZE / WRITE_SIZE runs:
     per form, the mean counter value (KiB per dispatch) over the dispatches
-    of the fine-level grid, and the copy-kernel calibration."""
+    of the fine-level grid, and the copy-kernel calibration.  timed: per
+    form, the kernel symbols of the timed iterations (the tuner's other
+    candidates are left out)."""
     import re
     cal = {}
     for cnt in ("fetch", "write"):
@@ -176,6 +178,9 @@ def summarize_traffic(d, workload_desc):
             for cnt in ("fetch", "write"):
                 rows, t64 = data[cnt]
                 sel = [r for r in rows if re.search(pat, r["Kernel_Name"])]
+                if timed:
+                    sel = [r for r in sel
+                           if r["Kernel_Name"].split("(")[0].replace("void ", "") in timed[form]]
                 if pats[1 - k] is None or pats[k] is None or pats[0] == pats[1]:
                     sel = [r for r in sel if (int(r["Start_Timestamp"]) >= t64) == (form == "fp64")]
                 if not sel:
@@ -196,9 +201,10 @@ def summarize_traffic(d, workload_desc):
     return written
 
 
-def summarize_trace(d, steps=3):
+def summarize_trace(d, steps=3, timed=None):
     """Per-iteration kernel table of the timed outer iterations of each form
-    (windows from one momentum assembly launch to the next)."""
+    (windows from one momentum assembly launch to the next); the kernel
+    symbols met there go to timed[form] when a dict is given."""
     import re
     rows = sorted(_csv_rows(_find_csv(os.path.join(d, "trace"), "kernel_trace.csv")),
                   key=lambda r: int(r["Start_Timestamp"]))
@@ -223,6 +229,8 @@ def summarize_trace(d, steps=3):
                 e[0] += 1
                 e[1] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
                 e[5] = max(e[5], int(r["Grid_Size_X"]))
+        if timed is not None:
+            timed[form] = {"oagpu::" + k if not k.startswith(("oagpu::", "__")) else k for k in agg}
         busy = sum(w[2] for w in ws) / W
         wall = sum(w[3] for w in ws) / W
         nl = sum(v[0] for v in agg.values()) / W
@@ -535,7 +543,8 @@ def bench_cavity(args):
     ne = mesh["conn"].shape[0]
     print(f"[bench] cavity {args.n}^3 perturb={args.perturb}: {ne} elements", file=sys.stderr,
           flush=True)
-    pm = oa.OAGPU_CG_AMG if args.p_method == "cg_amg" else oa.OAGPU_BICGSTAB_AMG
+    pm = {"cg_amg": oa.OAGPU_CG_AMG, "bicgstab_amg": oa.OAGPU_BICGSTAB_AMG,
+          "bicgstab_sa": oa.OAGPU_BICGSTAB_SA}[args.p_method]
     dm = oa.DeviceMesh(mesh, caller_order=args.caller_order)
     nn, nnz = dm.nn, dm.nnz
 
@@ -586,9 +595,8 @@ def bench_cavity(args):
         "dtype": "f32 local / f64 global", "data": "synthetic",
         "config": {"workload": f"lid-driven cavity, uniform{' perturbed' if args.perturb else ''} "
                                f"{args.n}^3 hex8 ({ne} elements, {nn} nodes), Re=100, mixed form; "
-                               f"Jacobi-GMRES(20 it, rtol 0.1) momentum, aggregation-AMG-"
-                               f"{'CG' if args.p_method == 'cg_amg' else 'BiCGStab'} "
-                               f"(200 it, rtol 1e-6) p'",
+                               f"Jacobi-GMRES(20 it, rtol 0.1) momentum, "
+                               f"{P_METHOD_NAMES[args.p_method]} (200 it, rtol 1e-6) p'",
                    "n": args.n, "elements": ne, "nodes": nn, "nnz": nnz},
         "roofline": roofline(rmx, ab_mx, f"{wl}_{args.n}", 1),
         "kernels": kernel_table(rmx, ab_mx, ne),
@@ -687,7 +695,9 @@ def bench_duct(args):
     dm = oa.DeviceMesh(mesh, caller_order=args.caller_order)
     res = {}
     for prec in (oa.OAGPU_MIXED, oa.OAGPU_FP64):
-        f = oa.FlowCase(dm, oa.default_params(prec, p_method=oa.OAGPU_BICGSTAB_AMG, mu=DUCT_MU,
+        pm = {"cg_amg": oa.OAGPU_CG_AMG, "bicgstab_amg": oa.OAGPU_BICGSTAB_AMG,
+              "bicgstab_sa": oa.OAGPU_BICGSTAB_SA}[args.bfs_p_method]
+        f = oa.FlowCase(dm, oa.default_params(prec, p_method=pm, mu=DUCT_MU,
                                               high_res=1), DUCT_BC)
         duct_setup(f, mesh)
         res[prec] = run_form(oa, f, lambda: f.outer_iteration(DUCT_CTL), args.steps, args.warmup)
@@ -726,7 +736,8 @@ def bench_duct(args):
         "config": {"workload": f"mixed-topology duct 1 x 0.2 x 0.2, {ne} elements ({counts[1]} tet4, "
                                f"{counts[0]} hex8, {counts[2]} wedge6, {counts[3]} pyramid5), node "
                                f"jitter N(0, 0.1 h); plug inflow U=1, outlet p=0, rho=1, "
-                               f"mu={DUCT_MU}, high-resolution advection; mixed form",
+                               f"mu={DUCT_MU}, high-resolution advection; mixed form; "
+                               f"{P_METHOD_NAMES[args.bfs_p_method]} p'",
                    "m": m, "layers": layers, "elements": ne, "nodes": nn},
         "fp64": {"value": round(it64, 4), "ms_per_step": round(1e3 * r64["seconds"] / args.steps, 3),
                  "kernel_ms_per_step": {k: round(v["ms_per_step"], 3) for k, v in r64["warm"].items()}},
@@ -895,10 +906,11 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=64, help="cavity: oracle sample grid")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--perturb", type=float, default=0.0)
-    ap.add_argument("--p-method", choices=("cg_amg", "bicgstab_amg"), default="cg_amg")
+    ap.add_argument("--p-method", choices=("cg_amg", "bicgstab_amg", "bicgstab_sa"), default="cg_amg",
+                    help="cavity: p' solver")
     ap.add_argument("--bfs-p-method", choices=("cg_amg", "bicgstab_amg", "bicgstab_sa"),
                     default="bicgstab_sa",
-                    help="backward-facing step: p' solver")
+                    help="backward-facing step and duct: p' solver")
     ap.add_argument("--to-residual", type=float, default=0.0,
                     help="cavity: time the outer iterations until every scaled residual is below this")
     ap.add_argument("--max-iters", type=int, default=4000)
@@ -924,11 +936,13 @@ def main():
         print("\n".join(c.format(dir="gpurun_out/final") for c in PROFILE_COMMANDS))
         return
     if args.summarize_profiles:
+        timed = {}
+        table = summarize_trace(args.summarize_profiles, timed=timed)
         for p in summarize_traffic(args.summarize_profiles,
                                    "backward-facing step k-omega SST, 7996800 hex8 (8120619 nodes), "
-                                   "bench.py defaults"):
+                                   "bench.py defaults", timed):
             print(p, file=sys.stderr)
-        print(summarize_trace(args.summarize_profiles))
+        print(table)
         return
     if args.copy_probe:
         import torch
