@@ -369,7 +369,8 @@ BFS_MU, BFS_KIN = 2.0e-5, 1.5e-4
 BFS_WIN = BFS_KIN / (10.0 * BFS_MU)
 BFS_CTL, BFS_SCTL = [20, 1.0e-2, 200, 1.0e-2], [20, 1.0e-2]
 P_METHOD_NAMES = {"cg_amg": "AMG-CG (K-cycle)", "bicgstab_amg": "AMG-BiCGStab",
-                  "bicgstab_sa": "smoothed-aggregation AMG-BiCGStab"}
+                  "bicgstab_sa": "smoothed-aggregation AMG-BiCGStab",
+                  "cg_sa": "smoothed-aggregation AMG-CG"}
 BFS_BC = np.array([[1.0, 0.0, 0.0], [0.0] * 3, [0.0] * 3, [0.0] * 3])
 
 
@@ -544,7 +545,7 @@ def bench_cavity(args):
     print(f"[bench] cavity {args.n}^3 perturb={args.perturb}: {ne} elements", file=sys.stderr,
           flush=True)
     pm = {"cg_amg": oa.OAGPU_CG_AMG, "bicgstab_amg": oa.OAGPU_BICGSTAB_AMG,
-          "bicgstab_sa": oa.OAGPU_BICGSTAB_SA}[args.p_method]
+          "bicgstab_sa": oa.OAGPU_BICGSTAB_SA, "cg_sa": oa.OAGPU_CG_SA}[args.p_method]
     dm = oa.DeviceMesh(mesh, caller_order=args.caller_order)
     nn, nnz = dm.nn, dm.nnz
 
@@ -578,7 +579,7 @@ def bench_cavity(args):
         threads = args.cpu_threads or cores
         cn = min(args.cpu_n, args.n)
         print(f"[bench] cpu baseline (oracle, {threads} threads) at {cn}^3", file=sys.stderr, flush=True)
-        cb = cavity_cpu_baseline(cn, args.perturb, threads, 1 if pm == oa.OAGPU_CG_AMG else 2)
+        cb = cavity_cpu_baseline(cn, args.perturb, threads, 1 if pm in (oa.OAGPU_CG_AMG, oa.OAGPU_CG_SA) else 2)
         scale = (cn / args.n) ** 3
         cpu = {"value": round(scale / cb["outer_s"], 6), "unit": "outer iterations/s",
                "cores": threads, "kind": "port",
@@ -811,7 +812,8 @@ def bench_batch(args):
 
     def flow(c, precision):
         dm = meshes[c["n"]]
-        f = oa.FlowCase(dm, oa.default_params(precision, mu=c["mu"], p_method=oa.OAGPU_CG_AMG),
+        pm = {"cg_amg": oa.OAGPU_CG_AMG, "cg_sa": oa.OAGPU_CG_SA}[args.p_method]
+        f = oa.FlowCase(dm, oa.default_params(precision, mu=c["mu"], p_method=pm),
                         np.array([c["lid"], [0.0, 0.0, 0.0]]))
         f.upload("U", meshgen.initial_velocity(dm.nn, c["n"]))
         f.initialize()
@@ -906,8 +908,9 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=64, help="cavity: oracle sample grid")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--perturb", type=float, default=0.0)
-    ap.add_argument("--p-method", choices=("cg_amg", "bicgstab_amg", "bicgstab_sa"), default="cg_amg",
-                    help="cavity: p' solver")
+    ap.add_argument("--p-method", choices=("cg_amg", "bicgstab_amg", "bicgstab_sa", "cg_sa"),
+                    default="cg_sa",
+                    help="cavity and batched cavities: p' solver")
     ap.add_argument("--bfs-p-method", choices=("cg_amg", "bicgstab_amg", "bicgstab_sa"),
                     default="bicgstab_sa",
                     help="backward-facing step and duct: p' solver")

@@ -129,6 +129,9 @@ typedef struct oagpu_control_data {
                             systems of wall-resolved meshes, whose cell aspect
                             ratios stall the unsmoothed V-cycle; FP32 inner
                             solve under FP64 refinement in the mixed form */
+#define OAGPU_CG_SA 6        /* CG preconditioned by the same smoothed-aggregation
+                            V(1,1)-cycle (symmetric scalar systems); FP32 inner
+                            solve under FP64 refinement in the mixed form */
 
 /* ---------------------------------------------------------------- runtime */
 int oagpu_init(int device);

@@ -520,6 +520,31 @@ __global__ void __launch_bounds__(kRedBlock) k_cg_update1(size_t n, const V* p, 
 
 __global__ void k_clear_flag(int* flag) { flag[0] = 0; }
 
+// Block partials (fixed grid of kRedGrid blocks) of r . z into rz_part and,
+// with q, of q . z into qz_part; with p, p = z: what the unsmoothed cycle's
+// last kernel produces on the way, for the smoothed-aggregation cycle.
+template <typename V>
+__global__ void __launch_bounds__(kRedBlock) k_pc_partials(size_t n, const V* r, const V* z,
+                                                           const V* q, V* p, double* rz_part,
+                                                           double* qz_part, const int* flag) {
+    __shared__ double sh[kRedBlock];
+    if (flag[0]) return;
+    double rz = 0.0, qz = 0.0;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x) {
+        const V zi = z[i];
+        rz += double(r[i]) * double(zi);
+        if (q) qz += double(q[i]) * double(zi);
+        if (p) p[i] = zi;
+    }
+    rz = block_sum(rz, sh);
+    if (q) qz = block_sum(qz, sh);
+    if (threadIdx.x == 0) {
+        rz_part[blockIdx.x] = rz;
+        if (q) qz_part[blockIdx.x] = qz;
+    }
+}
+
 // FLEX: flexible CG (the preconditioner is the nonlinear K-cycle):
 // beta = -(z . q) / (p . q) keeps the new direction A-orthogonal to the last
 // one (Notay, SIAM J. Sci. Comput. 22 (2000)); else beta = r.z / r_old.z_old
@@ -565,7 +590,8 @@ __global__ void k_cg_tick(int* flag) {
 // initial / final ||r||_2 in V arithmetic.  amg: preconditioner = one AMG
 // K-cycle under flexible CG (N = 1), else Jacobi under CG.
 template <int N, typename V>
-static void cg_iteration(Flow& f, const V* A, const V* dinv, V* x, V* r, V* z, V* p, V* q, bool amg) {
+static void cg_iteration(Flow& f, const V* A, const V* dinv, V* x, V* r, V* z, V* p, V* q, bool amg,
+                         bool sa = false) {
     DevMesh& m = *f.m;
     const size_t n = size_t(m.nn) * N;
     k_cg_spmv<N, V><<<kRedGrid, kRedBlock, 0, f.stream>>>(m.nn, m.slice_ptr.p, m.slice_w.p,
@@ -576,8 +602,14 @@ static void cg_iteration(Flow& f, const V* A, const V* dinv, V* x, V* r, V* z, V
             k_cg_update1<V, true><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, p, q, dinv, x, r, z,
                                                                          f.part.p, f.scal.p,
                                                                          f.flag.p);
-            amg_vcycle<V>(f, A, dinv, r, z, f.part.p + 3 * kRedGrid, nullptr, false, true, q,
-                          f.part.p + 5 * kRedGrid);
+            if (sa) {
+                sa_vcycle<V>(f, A, dinv, r, z);
+                k_pc_partials<V><<<kRedGrid, kRedBlock, 0, f.stream>>>(
+                    n, r, z, q, nullptr, f.part.p + 3 * kRedGrid, f.part.p + 5 * kRedGrid, f.flag.p);
+            } else {
+                amg_vcycle<V>(f, A, dinv, r, z, f.part.p + 3 * kRedGrid, nullptr, false, true, q,
+                              f.part.p + 5 * kRedGrid);
+            }
             k_cg_update2<V, true><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, z, p, f.part.p, f.scal.p,
                                                                          f.flag.p);
         }
@@ -600,14 +632,23 @@ constexpr int kPoll = 8;
 
 template <int N, typename V>
 static int cg_core(Flow& f, const V* A, const V* b, const V* dinv, V* x, V* r, V* z, V* p, V* q,
-                   int max_it, double rtol, double atol, double* r0, double* r1, bool amg = false) {
+                   int max_it, double rtol, double atol, double* r0, double* r1, bool amg = false,
+                   bool sa = false) {
     DevMesh& m = *f.m;
     const size_t n = size_t(m.nn) * N;
     if constexpr (N != 1) OAGPU_REQUIRE(!amg, "AMG preconditioning is for scalar systems");
     k_clear_flag<<<1, 1, 0, f.stream>>>(f.flag.p);  // a kernel, not a memset: capturable
     if (amg) {
         k_cg_init<V, true><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, b, dinv, x, r, z, p, f.part.p);
-        if constexpr (N == 1) amg_vcycle<V>(f, A, dinv, r, z, f.part.p, p, true, true);  // z, p = z, r.z
+        if constexpr (N == 1) {
+            if (sa) {
+                sa_vcycle<V>(f, A, dinv, r, z);
+                k_pc_partials<V><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, r, z, nullptr, p, f.part.p,
+                                                                        nullptr, f.flag.p);
+            } else {
+                amg_vcycle<V>(f, A, dinv, r, z, f.part.p, p, true, true);  // z, p = z, r.z
+            }
+        }
     } else {
         k_cg_init<V, false><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, b, dinv, x, r, z, p, f.part.p);
     }
@@ -615,13 +656,14 @@ static int cg_core(Flow& f, const V* A, const V* b, const V* dinv, V* x, V* r, V
     OAGPU_HIP(hipGetLastError());
     int hflag[2] = {0, 0};
     const std::array<const void*, 10> key = {A, dinv, x, r, z, p, q, (const void*)(size_t)N,
-                                             (const void*)(size_t)sizeof(V), (const void*)(size_t)amg};
+                                             (const void*)(size_t)sizeof(V),
+                                             (const void*)(size_t)(amg + 2 * sa)};
     const bool graphs = f.prm.use_graphs != 0;
     if (graphs && (f.cg_graph.exec == nullptr || f.cg_graph.key != key)) {
         f.cg_graph.reset();
         hipGraph_t g = nullptr;
         OAGPU_HIP(hipStreamBeginCapture(f.stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < kPoll; ++k) cg_iteration<N, V>(f, A, dinv, x, r, z, p, q, amg);
+        for (int k = 0; k < kPoll; ++k) cg_iteration<N, V>(f, A, dinv, x, r, z, p, q, amg, sa);
         OAGPU_HIP(hipStreamEndCapture(f.stream, &g));
         const hipError_t e = hipGraphInstantiate(&f.cg_graph.exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -632,7 +674,7 @@ static int cg_core(Flow& f, const V* A, const V* b, const V* dinv, V* x, V* r, V
         if (graphs) {
             OAGPU_HIP(hipGraphLaunch(f.cg_graph.exec, f.stream));
         } else {
-            for (int k = 0; k < kPoll; ++k) cg_iteration<N, V>(f, A, dinv, x, r, z, p, q, amg);
+            for (int k = 0; k < kPoll; ++k) cg_iteration<N, V>(f, A, dinv, x, r, z, p, q, amg, sa);
             OAGPU_HIP(hipGetLastError());
         }
         f.flag.download(hflag, 2, f.stream);
@@ -648,17 +690,23 @@ static int cg_core(Flow& f, const V* A, const V* b, const V* dinv, V* x, V* r, V
     return hflag[1];
 }
 
+// sa: the smoothed-aggregation V-cycle (OAGPU_CG_SA), else the unsmoothed
+// hierarchy's K-cycle (OAGPU_CG_AMG)
 template <int N>
-static int cg(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd, bool amg) {
+static int cg(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd, bool amg,
+              bool sa = false) {
     const double* A = N == 3 ? f.A3.p : f.A1.p;
     const double* b = N == 3 ? f.b3.p : f.b1.p;
     double* x = N == 3 ? f.x3.p : f.x1.p;
-    if (amg) {
+    if (amg && sa) {
+        sa_setup(f, A, false);
+        amg = sa = f.sa.lv.size() > 1;  // no coarse level (tiny system): Jacobi-CG
+    } else if (amg) {
         amg_setup<double>(f, A, f.dinv.p);
         amg = f.amg.lv.size() > 1;  // no coarse level (tiny system): Jacobi-CG
     }
     return cg_core<N, double>(f, A, b, f.dinv.p, x, f.r.p, f.z.p, f.pk.p, f.q.p, max_it, rtol, atol,
-                              &cd->solver_initial_res, &cd->solver_final_res, amg);
+                              &cd->solver_initial_res, &cd->solver_final_res, amg, sa);
 }
 
 // ---------------------------------------------- mixed: FP32 CG + FP64 refinement
@@ -676,7 +724,8 @@ __global__ void k_refine(size_t n, double s, const float* d, double* x) {
 // x += ||r|| d in FP64, until ||r|| <= max(rtol ||b||, atol) or the total
 // inner iteration budget max_it is spent.  The FP32 matrix is A rounded.
 template <int N>
-static int cg_mixed(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd, bool amg) {
+static int cg_mixed(Flow& f, int max_it, double rtol, double atol, oagpu_control_data* cd, bool amg,
+                    bool sa = false) {
     DevMesh& m = *f.m;
     const size_t n = size_t(m.nn) * N, nv = m.nsell * N * N;
     const double* A = N == 3 ? f.A3.p : f.A1.p;
@@ -688,7 +737,10 @@ static int cg_mixed(Flow& f, int max_it, double rtol, double atol, oagpu_control
     const unsigned B = 256;
     k_to_float<<<unsigned((nv + B - 1) / B), B, 0, f.stream>>>(nv, 1.0, A, f.Af.p);
     k_to_float<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, 1.0, f.dinv.p, f.fdinv.p);
-    if (amg) {
+    if (amg && sa) {
+        sa_setup(f, A, true);  // Galerkin products in FP64, FP32 copies for the inner solve
+        amg = sa = f.sa.lv.size() > 1;
+    } else if (amg) {
         amg_setup<float>(f, f.Af.p, f.fdinv.p);
         amg = f.amg.lv.size() > 1;
     }
@@ -715,7 +767,7 @@ static int cg_mixed(Flow& f, int max_it, double rtol, double atol, oagpu_control
         k_to_float<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, 1.0 / rn, f.r.p, f.fb.p);
         double i0, i1;
         const int it = cg_core<N, float>(f, f.Af.p, f.fb.p, f.fdinv.p, f.fx.p, f.fr.p, f.fz.p, f.fp.p,
-                                         f.fq.p, max_it - used, inner, 0.0, &i0, &i1, amg);
+                                         f.fq.p, max_it - used, inner, 0.0, &i0, &i1, amg, sa);
         used += std::max(it, 1);
         k_refine<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, rn, f.fx.p, x);
         spmv<N>(f, A, x, nullptr, f.w.p);
@@ -1150,14 +1202,15 @@ int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double ato
             it = N == 3 ? gmres<3>(f, max_it, rtol, atol, restart, cd)
                         : gmres<1>(f, max_it, rtol, atol, restart, cd);
     }
-    else if (method == OAGPU_CG || method == OAGPU_CG_AMG) {
-        const bool amg = method == OAGPU_CG_AMG;
+    else if (method == OAGPU_CG || method == OAGPU_CG_AMG || method == OAGPU_CG_SA) {
+        const bool amg = method != OAGPU_CG, sa = method == OAGPU_CG_SA;
         OAGPU_REQUIRE(!amg || N == 1, "AMG preconditioning is for scalar systems (N = 1)");
         if (f.prm.precision == OAGPU_MIXED)
             it = N == 3 ? cg_mixed<3>(f, max_it, rtol, atol, cd, false)
-                        : cg_mixed<1>(f, max_it, rtol, atol, cd, amg);
+                        : cg_mixed<1>(f, max_it, rtol, atol, cd, amg, sa);
         else
-            it = N == 3 ? cg<3>(f, max_it, rtol, atol, cd, false) : cg<1>(f, max_it, rtol, atol, cd, amg);
+            it = N == 3 ? cg<3>(f, max_it, rtol, atol, cd, false)
+                        : cg<1>(f, max_it, rtol, atol, cd, amg, sa);
     }
     else if (method == OAGPU_BICGSTAB)
         it = N == 3 ? bicgstab<3>(f, max_it, rtol, atol, cd) : bicgstab<1>(f, max_it, rtol, atol, cd);

@@ -190,3 +190,40 @@ def test_sa_outer_iterations_match_oracle(oracle, oa, meshgen):
     for name in ("U", "p", "k", "omega"):
         assert rel_err(f.download(name), c.field(name)) < 1e-8, name
     f.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_cg_sa_cavity(oracle, oa, meshgen, precision):
+    """CG with the smoothed-aggregation V-cycle on a symmetric p' system (a
+    cavity): the direct solution, in fewer iterations than Jacobi-CG, and
+    graph replays of its iteration blocks equal to eager launches bitwise."""
+    m = meshgen.cavity_hex(16)
+    print(f"[case] cavity 16^3 CG-SA precision {precision}", flush=True)
+    BCc = np.array([[1.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    out = []
+    for graphs in (1, 0):
+        dm = oa.DeviceMesh(m)
+        f = oa.FlowCase(dm, oa.default_params(precision, use_graphs=graphs), BCc)
+        f.upload("U", meshgen.initial_velocity(dm.nn, 0))
+        f.initialize()
+        f.outer_iteration([20, 0.1, 200, 1e-6])
+        f.assemble_momentum()
+        f.solve(3, oa.OAGPU_GMRES, 20, 0.1)
+        f.correct(3, 1.0)
+        f.update_mdot()
+        f.assemble_pcorr()
+        vals, b = f.system(1)
+        rp, cols, _ = dm.graph()
+        A = csr(rp, cols, vals.ravel(), dm.nn).tocsc()
+        x_ref = spla.spsolve(A, b.ravel())
+        cd = f.solve(1, oa.OAGPU_CG_SA, 400, 1e-10)
+        x = f.solution(1).ravel()
+        err = rel_err(x, x_ref)
+        f.set_system(1, vals, b)
+        it_j = f.solve(1, oa.OAGPU_CG, 2000, 1e-10).n_iterations
+        print(f"[sa] graphs {graphs}: CG-SA {cd.n_iterations} iterations, error {err:.2e}; "
+              f"Jacobi-CG {it_j}", flush=True)
+        assert err < 1e-7 and cd.n_iterations < it_j
+        out.append(x)
+        f.close()
+    np.testing.assert_array_equal(out[0], out[1])
