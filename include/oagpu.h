@@ -204,22 +204,35 @@ int oagpu_element_geometry_noalloc(int topology, int shifted, int n_elems, const
  * treatment of p', pressureCorrectionAssemblerElemBoundaryConditions.cpp:49,
  * which adds nothing on a static mesh; no side mass flow; its nodes' rhs
  * relaxed like other boundary nodes in steady runs, navierStokesAssembler.cpp:289;
- * k and omega zero-gradient) */
+ * k and omega zero-gradient), and static-pressure opening
+ * (boundaryPhysicalType::opening with a specifiedDirection velocity and a
+ * staticPressure pressure condition: side mass flows rho U_ip . A, not
+ * clipped, updated by Rhie-Chow with the nodal pressure, flowModel.cpp:
+ * 4817-4992 / 8094-8380; no reversal flags, :15575; momentum outflow ips as
+ * an outlet, inflow ips entrained along the bc_U direction,
+ * navierStokesAssemblerElemBoundaryConditions.cpp:3234-3552; p' the
+ * outlet's terms at every ip, pressureCorrectionAssemblerElemBoundary
+ * Conditions.cpp:3026-3560; k and omega implicit upwind,
+ * phiAssemblerElemBoundaryConditions.hpp:1439-1760).  An outlet or opening
+ * replaces the pressure-reference row. */
 #define OAGPU_BC_WALL 0
 #define OAGPU_BC_INLET 1
 #define OAGPU_BC_OUTLET 2
 #define OAGPU_BC_SYMMETRY 3
 #define OAGPU_BC_SLIP 4
+#define OAGPU_BC_OPENING 5
 
 /* bc_U[n_bc][3]: velocity of each boundary id (no-slip walls: the wall
- * velocity; velocity inlets: the specified inlet velocity). */
+ * velocity; velocity inlets: the specified inlet velocity; openings: the
+ * entrainment direction, normalised, or all zero for the inward normal of
+ * each face ip, velocity.cpp:4813-4970). */
 int oagpu_flow_create(oagpu_mesh *m, const oagpu_flow_params *prm, int n_bc,
                       const double *bc_U, oagpu_flow **out);
 int oagpu_flow_destroy(oagpu_flow *f);
 /* fields: "U" [nn][3], "p" [nn], "gradU" [nn][9], "gradP" [nn][3],
  * "du" [nn][3], "mdot" [ne][12], "div" [nn], "mu" [nn] (effective
- * viscosity), the SST fields once enabled, and with inlets or outlets
- * "msd" [n_bfaces][4] (side mass flow rate per boundary face ip, outward
+ * viscosity), the SST fields once enabled, and with inlets, outlets or
+ * openings "msd" [n_bfaces][4] (side mass flow rate per boundary face ip, outward
  * positive) */
 int oagpu_field_upload(oagpu_flow *f, const char *name, const double *host, size_t count);
 int oagpu_field_download(oagpu_flow *f, const char *name, double *host, size_t count);
@@ -256,11 +269,12 @@ int oagpu_update_scales(oagpu_flow *f, double *U_scale, double *p_scale); /* nod
  * scalar flux.  Velocity inlets (the bc_U row; zero-gradient pressure) and
  * static-pressure outlets (zero-gradient velocity, k, omega) carry side mass
  * flows (flowModel.cpp:3802-4997 initial, :6470-6598 inlet and :7323-7690
- * outlet Rhie-Chow updates, :15226-15454 outlet reversal flags); with an
- * outlet the p' system has no pressure-reference row.  Other types are
- * rejected (OAGPU_EINVAL).  Resets the side mass flows and reversal flags. */
+ * outlet Rhie-Chow updates, :15226-15454 outlet reversal flags), and so do
+ * static-pressure openings; with an outlet or opening the p' system has no
+ * pressure-reference row.  Other types are rejected (OAGPU_EINVAL).  Resets
+ * the side mass flows and reversal flags. */
 int oagpu_flow_set_bc_types(oagpu_flow *f, int n_bc, const int *type);
-/* Per boundary id: p[n_bc] outlet static pressure, k[n_bc] / w[n_bc] inlet
+/* Per boundary id: p[n_bc] outlet / opening static pressure, k[n_bc] / w[n_bc] inlet
  * turbulent kinetic energy / eddy frequency (the specified values of the
  * outlet's pressure and the inlet's k and omega boundary conditions);
  * NULL leaves that table unchanged; all zero until set. */

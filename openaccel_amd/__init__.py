@@ -36,6 +36,7 @@ OAGPU_MESH_CALLER_ORDER = 0x100
 OAGPU_GMRES, OAGPU_CG, OAGPU_BICGSTAB, OAGPU_CG_AMG, OAGPU_BICGSTAB_AMG = 0, 1, 2, 3, 4
 OAGPU_BICGSTAB_SA, OAGPU_CG_SA = 5, 6
 OAGPU_BC_WALL, OAGPU_BC_INLET, OAGPU_BC_OUTLET, OAGPU_BC_SYMMETRY, OAGPU_BC_SLIP = 0, 1, 2, 3, 4
+OAGPU_BC_OPENING = 5
 OAGPU_HEX8, OAGPU_TET4, OAGPU_WED6, OAGPU_PYR5 = 0, 1, 2, 3
 OAGPU_BDF1, OAGPU_BDF2 = 1, 2
 TOPOLOGY_SIZES = {OAGPU_HEX8: (8, 12), OAGPU_TET4: (4, 6), OAGPU_WED6: (6, 9), OAGPU_PYR5: (5, 12)}
@@ -350,12 +351,14 @@ class FlowCase:
 
     def set_bc_types(self, types):
         """Physical type per boundary id: OAGPU_BC_WALL, OAGPU_BC_INLET,
-        OAGPU_BC_OUTLET, OAGPU_BC_SYMMETRY or OAGPU_BC_SLIP."""
+        OAGPU_BC_OUTLET, OAGPU_BC_SYMMETRY, OAGPU_BC_SLIP or OAGPU_BC_OPENING
+        (whose bc_U row is the entrainment direction; zero: the inward
+        normal)."""
         t = np.ascontiguousarray(types, dtype=np.int32)
         _check(load_library().oagpu_flow_set_bc_types(self.h, len(t), _i(t)))
 
     def set_bc_values(self, p=None, k=None, w=None):
-        """Per boundary id: outlet static pressure p, inlet k and omega (None:
+        """Per boundary id: outlet / opening static pressure p, inlet k and omega (None:
         unchanged)."""
         arrs = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (p, k, w)]
         n = next(a.size for a in arrs if a is not None)

@@ -1,8 +1,8 @@
-// bc.hip -- side mass flow rates and reversal flags of velocity inlets and
-// static-pressure outlets.  Face-owned: one lane per inlet / outlet face ip
-// (mass flows) or per outlet face (reversal flags), over the list of such
-// faces; each value is written by exactly one lane.  Sides of every element
-// topology: a tri side's fourth ip lane does nothing.
+// bc.hip -- side mass flow rates and reversal flags of velocity inlets,
+// static-pressure outlets and static-pressure openings.  Face-owned: one
+// lane per such face ip (mass flows) or per outlet face (reversal flags),
+// over the list of those faces; each value is written by exactly one lane.
+// Sides of every element topology: a tri side's fourth ip lane does nothing.
 #include "sides.hpp"
 
 namespace oagpu {
@@ -31,9 +31,12 @@ struct SideArgs {
 };
 
 // init: initializeMassFlowRate side part (flowModel.cpp:3802-4997): inlet
-// rho U_bc . A, outlet max(0, rho U_ip . A).  Update (flowModel.cpp:6470-6598
-// inlet, :7323-7690 outlet): the new rate relaxed with urf against the
-// previous one; a reversed outlet ip carries no flow.
+// rho U_bc . A, outlet max(0, rho U_ip . A), opening rho U_ip . A
+// (:4817-4992).  Update (flowModel.cpp:6470-6598 inlet, :7323-7690 outlet,
+// :8094-8380 opening): the new rate relaxed with urf against the previous
+// one; a reversed outlet ip carries no flow; the outlet's Rhie-Chow rate
+// takes the specified pressure on the side's nodes, the opening's the
+// nodal pressure.
 __global__ void __launch_bounds__(256) k_side_mdot(const SideArgs a) {
     const int i = int(xcd_block() * blockDim.x + threadIdx.x);
     if (i >= 4 * a.nio) return;
@@ -46,7 +49,7 @@ __global__ void __launch_bounds__(256) k_side_mdot(const SideArgs a) {
     const double xr[3] = {a.xyz[on], a.xyz[on + 1], a.xyz[on + 2]};
     int bad = 0;
     FaceIp<double> fi;
-    face_ip<true>(a.sg, a.xyz, ce, e, f, ord, fk, t == OAGPU_BC_OUTLET ? a.sp : a.us, xr, fi, bad);
+    face_ip<true>(a.sg, a.xyz, ce, e, f, ord, fk, t == OAGPU_BC_INLET ? a.us : a.sp, xr, fi, bad);
     double& md = a.msd[4 * size_t(f) + fk];
     double rip = 0;
     for (int q = 0; q < fi.npf; ++q) rip += fi.shape(a.us, fk, q) * a.rho;
@@ -63,12 +66,12 @@ __global__ void __launch_bounds__(256) k_side_mdot(const SideArgs a) {
         }
         double m = 0;
         for (int j = 0; j < 3; ++j) m += rip * u[j] * fi.A[j];
-        md = fmax(0.0, m);
-    } else if (a.rfl[4 * size_t(f) + fk] == 1) {
+        md = t == OAGPU_BC_OUTLET ? fmax(0.0, m) : m;
+    } else if (t == OAGPU_BC_OUTLET && a.rfl[4 * size_t(f) + fk] == 1) {
         md = 0.0;  // reversed: a slip wall
     } else {
         const double m = outlet_rhie_chow(fi, ce, fk, a.us, a.U, a.du, a.p, a.gradP, a.rho,
-                                          a.bc_p[bc]);
+                                          a.bc_p[bc], t == OAGPU_BC_OUTLET);
         md = a.urf * m + (1.0 - a.urf) * md;
     }
     if (bad) *a.err = 1;

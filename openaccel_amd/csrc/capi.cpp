@@ -564,8 +564,9 @@ int oagpu_flow_set_bc_types(oagpu_flow* h, int n_bc, const int* type) {
         OAGPU_REQUIRE(n_bc == f.nbc, "one type per boundary id");
         bool sym = false, slip = false;
         for (int b = 0; b < n_bc; ++b) {
-            OAGPU_REQUIRE(type[b] >= OAGPU_BC_WALL && type[b] <= OAGPU_BC_SLIP,
-                          "unsupported boundary type (wall, inlet, outlet, symmetry, slip wall)");
+            OAGPU_REQUIRE(type[b] >= OAGPU_BC_WALL && type[b] <= OAGPU_BC_OPENING,
+                          "unsupported boundary type (wall, inlet, outlet, symmetry, slip wall, "
+                          "opening)");
             sym = sym || type[b] == OAGPU_BC_SYMMETRY;
             slip = slip || type[b] == OAGPU_BC_SLIP;
         }
@@ -575,28 +576,32 @@ int oagpu_flow_set_bc_types(oagpu_flow* h, int n_bc, const int* type) {
         f.has_sym = sym;
         f.has_slip = slip;
         if (sym && !f.symA.p) f.symA.alloc(size_t(f.m->nn) * 3);
-        // inlet and outlet faces, in face order
+        // faces with side mass flows (inlets, outlets, openings), in face order
         const int nbf = f.m->nbf;
         std::vector<int> ids(size_t(nbf > 0 ? nbf : 1)), io;
         if (nbf) f.m->bf_bc.download(ids.data(), size_t(nbf));
         OAGPU_HIP(hipDeviceSynchronize());
-        f.has_outlet = false;
+        auto flow_side = [](int t) {
+            return t == OAGPU_BC_INLET || t == OAGPU_BC_OUTLET || t == OAGPU_BC_OPENING;
+        };
+        f.has_outlet = f.has_opening = false;
         for (int i = 0; i < nbf; ++i) {
             const int t = type[ids[i]];
-            if (t == OAGPU_BC_INLET || t == OAGPU_BC_OUTLET) io.push_back(i);
+            if (flow_side(t)) io.push_back(i);
             f.has_outlet = f.has_outlet || t == OAGPU_BC_OUTLET;
+            f.has_opening = f.has_opening || t == OAGPU_BC_OPENING;
         }
         f.nio = int(io.size());
         f.has_io = f.nio > 0;
         if (f.has_io) {
             f.io_face.upload(io);
-            // rows nearest to an inlet / outlet face ip, ascending
+            // rows nearest to an inlet / outlet / opening face ip, ascending
             const oagpu::DevMesh& dm = *f.m;
             std::vector<int> rows;
             for (int r = 0; r < dm.nn; ++r)
                 for (int k = dm.h_binc_ptr[r]; k < dm.h_binc_ptr[r + 1]; ++k) {
                     const int t = type[ids[dm.h_binc[k] >> 2]];
-                    if (t == OAGPU_BC_INLET || t == OAGPU_BC_OUTLET) {
+                    if (flow_side(t)) {
                         rows.push_back(r);
                         break;
                     }

@@ -67,8 +67,9 @@ void launch_mesh_geometry(DevMesh& m, hipStream_t s) {
 }
 
 // updateMassDivergenceField_ interior ips (flowModel.cpp:13733-13772): divL += mdot,
-// divR -= mdot; then the side mass flows of inlet and outlet face ips whose
-// nearest node this is (flowModel.cpp:14237-14500; msd null without them)
+// divR -= mdot; then the side mass flows of inlet, outlet and opening face
+// ips whose nearest node this is (flowModel.cpp:14237-14500, 14850-14852;
+// msd null without them)
 __global__ void __launch_bounds__(256) k_mass_divergence(int nn, const int* inc_ptr, const int* inc,
                                                          const double* mdot, const int* binc_ptr,
                                                          const int* binc, const int* bf_bc,
@@ -98,7 +99,8 @@ __global__ void __launch_bounds__(256) k_mass_divergence(int nn, const int* inc_
     if (msd)
         for (int k = binc_ptr[a]; k < binc_ptr[a + 1]; ++k) {
             const int f = binc[k] >> 2, t = bc_type[bf_bc[f]];
-            if (t == OAGPU_BC_INLET || t == OAGPU_BC_OUTLET) s += msd[4 * size_t(f) + (binc[k] & 3)];
+            if (t == OAGPU_BC_INLET || t == OAGPU_BC_OUTLET || t == OAGPU_BC_OPENING)
+                s += msd[4 * size_t(f) + (binc[k] & 3)];
         }
     div[a] = s;
 }

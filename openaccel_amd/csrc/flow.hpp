@@ -192,13 +192,14 @@ struct Flow {
     DBuf<double> symA;
     bool has_sym = false;
     bool has_slip = false;     // some boundary id is a slip wall
-    // velocity inlets and static-pressure outlets: the boundary faces of
-    // either type, the outlet static pressure and inlet k / omega per
-    // boundary id, and per face ip the side mass flow rate and (outlets) the
-    // flow-reversal flag; allocated only when such a boundary exists
-    bool has_io = false, has_outlet = false;
+    // velocity inlets, static-pressure outlets and openings: the boundary
+    // faces of these types, the outlet / opening static pressure and inlet
+    // k / omega per boundary id, and per face ip the side mass flow rate and
+    // (outlets) the flow-reversal flag; allocated only when such a boundary
+    // exists
+    bool has_io = false, has_outlet = false, has_opening = false;
     int nio = 0, nio_row = 0;
-    DBuf<int> io_face, io_row;  // inlet / outlet faces; nodes nearest to one of their ips
+    DBuf<int> io_face, io_row;  // those faces; nodes nearest to one of their ips
     std::vector<double> h_bc_p, h_bc_k, h_bc_w;
     DBuf<double> bc_p, bc_k, bc_w;
     DBuf<double> msd;  // [nbf][4]

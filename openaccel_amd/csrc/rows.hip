@@ -704,10 +704,21 @@ __device__ void mom_generic(const MomArgs& a, int la, int e, uint64_t sl, const 
 //   velocity inlet    full, fixed  (:791-1124)
 //   pressure outlet   ZNS, free    (:1468-1775)
 //   symmetry plane    full, free   (:2082-2375)
+//   opening, outflow  ZNS, free    (:3505-3510)
+//   opening, inflow   ZNS without the grad(U)^T terms
+//                     (IP_ZERO_NORMAL_STRESS_NO_GRADU_TRANSPOSE__ :471-519),
+//                     free, after the explicit entrainment flux msd Iu d_i,
+//                     Iu = U_ip . n / (d . n + SMALL) (:3511-3528)
+// entrain: the opening's inflow case (ZNS only; the transpose terms are
+// scaled by tr = 0).  The entrainment direction d is the boundary id's bc_U
+// row normalised, or for a zero row the inward unit normal of the face ip
+// (velocity.cpp:4813-4970).
 template <typename T, bool ZNS, bool MX, typename LT>
 __device__ __forceinline__ void mom_face(const MomArgs& a, int f, int fk, bool fixed, uint64_t sl,
                                          const SellRow& sr, int dof, const double xr[3],
-                                         double D[9], double R[3], int& bad, LT* lt) {
+                                         double D[9], double R[3], int& bad, LT* lt,
+                                         bool entrain = false) {
+    const T tr = entrain ? T(0) : T(1);
     const int e = a.bf_elem[f], ord = a.bf_ord[f], bc = a.bf_bc[f];
     const int* ce = a.conn + 8 * size_t(e);
     FaceIp<T> fi;
@@ -716,6 +727,32 @@ __device__ __forceinline__ void mom_face(const MomArgs& a, int f, int fk, bool f
     if constexpr (ZNS) {
         const T amag = sqrt(fi.A[0] * fi.A[0] + fi.A[1] * fi.A[1] + fi.A[2] * fi.A[2]);
         for (int d = 0; d < 3; ++d) nx[d] = fi.A[d] / amag;
+        if (entrain) {  // FP64 in both forms, like the other advective fluxes
+            double Ad[3], d[3], uip[3] = {0, 0, 0};
+            for (int j = 0; j < 3; ++j) Ad[j] = double(fi.A[j]);
+            const double am = sqrt(Ad[0] * Ad[0] + Ad[1] * Ad[1] + Ad[2] * Ad[2]);
+            const double* v = a.bc_U + 3 * bc;
+            const double n2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            if (n2 > 0.0) {
+                const double nv = sqrt(n2);
+                for (int j = 0; j < 3; ++j) d[j] = v[j] / nv;
+            } else {
+                for (int j = 0; j < 3; ++j) d[j] = -Ad[j] / am;
+            }
+            for (int q = 0; q < fi.npf; ++q) {
+                const double r = fi.shape(a.us, fk, q);
+                const size_t o = 3 * size_t(ce[fi.sn[q]]);
+                for (int j = 0; j < 3; ++j) uip[j] += r * a.U[o + j];
+            }
+            double num = 0, den = 0;
+            for (int j = 0; j < 3; ++j) {
+                const double nj = Ad[j] / am;
+                num += uip[j] * nj;
+                den += d[j] * nj;
+            }
+            const double tm = a.msd[4 * size_t(f) + fk], Iu = num / (den + kSmall);
+            for (int i = 0; i < 3; ++i) R[i] -= tm * Iu * d[i];
+        }
     }
     T mu = T(a.mu);
     if (a.mueff) {
@@ -744,7 +781,7 @@ __device__ __forceinline__ void mom_face(const MomArgs& a, int f, int fk, bool f
                 T lf = -mu * dnj * axj * om;
                 acc[3 * i + i] += lf * bcm;
                 rl[i] -= lf * uxi;
-                lf = -mu * dni * axj * om;
+                lf = -mu * dni * axj * om * tr;
                 acc[3 * i + j] += lf * bcm;
                 rl[i] -= lf * uxj;
                 if constexpr (ZNS) {
@@ -754,7 +791,7 @@ __device__ __forceinline__ void mom_face(const MomArgs& a, int f, int fk, bool f
                         lf = mu * dnj * axj * nn_;
                         acc[3 * i + l] += lf * bcm;
                         rl[i] -= lf * uc[l];
-                        lf = mu * g[l] * axj * nn_;
+                        lf = mu * g[l] * axj * nn_ * tr;
                         acc[3 * i + j] += lf * bcm;
                         rl[i] -= lf * uxj;
                     }
@@ -770,8 +807,11 @@ __device__ __forceinline__ void mom_face(const MomArgs& a, int f, int fk, bool f
 // its boundary; inlets add the explicit advective flux msd U_bc
 // (IP_EXPLICIT_ADVECTIVE_FLUX__), outlets the implicit upwind one
 // (IP_IMPLICIT_UPWIND_ADVECTIVE_FLUX__: msd on the diagonal, msd U_row on
-// the rhs) unless the ip is reversed, when it takes no terms (a slip wall)
-template <typename T, bool MX, typename LT>
+// the rhs) unless the ip is reversed, when it takes no terms (a slip wall);
+// openings the outlet's terms where msd > 0, else the entrainment terms.
+// OP: openings may occur (a kernel variant of its own: the entrainment path
+// costs registers in the others).
+template <typename T, bool MX, bool OP, typename LT>
 __device__ __forceinline__ void mom_boundary(const MomArgs& a, int pk, uint64_t sl,
                                              const SellRow& sr, int dof, int row,
                                              const double xr[3], double D[9], double R[3],
@@ -786,9 +826,15 @@ __device__ __forceinline__ void mom_boundary(const MomArgs& a, int pk, uint64_t 
         const int bc = a.bf_bc[f];
         for (int i = 0; i < 3; ++i) R[i] -= tm * a.bc_U[3 * bc + i];
         mom_face<T, false, MX>(a, f, fk, true, sl, sr, dof, xr, D, R, bad, lt);
-    } else if (t == OAGPU_BC_OUTLET) {
-        if (a.rfl[4 * size_t(f) + fk] == 1) return;
+    } else if (t == OAGPU_BC_OUTLET || (OP && t == OAGPU_BC_OPENING)) {
+        if (t == OAGPU_BC_OUTLET && a.rfl[4 * size_t(f) + fk] == 1) return;
         const double tm = a.msd[4 * size_t(f) + fk];
+        if constexpr (OP) {
+            if (t == OAGPU_BC_OPENING && !(tm > 0.0)) {
+                mom_face<T, true, MX>(a, f, fk, false, sl, sr, dof, xr, D, R, bad, lt, true);
+                return;
+            }
+        }
         for (int i = 0; i < 3; ++i) {
             R[i] -= tm * a.U[3 * size_t(row) + i];
             D[4 * i] += tm;
@@ -806,7 +852,7 @@ __device__ __forceinline__ void mom_boundary(const MomArgs& a, int pk, uint64_t 
 // LDS: 0 = off-diagonal blocks read-modify-written in HBM (256-lane blocks);
 // 1 / 2 = summed in a per-wavefront LDS tile in float / double and written
 // once (64-lane blocks, one wavefront each).
-template <typename T, int SH, int LDS, bool HR, bool MX>
+template <typename T, int SH, int LDS, bool HR, bool MX, bool OP = false>
 __global__ void __launch_bounds__(LDS ? 64 : 256) k_momentum_rows(const MomArgs a) {
     using LT = std::conditional_t<LDS == 0, void, std::conditional_t<LDS == 2, double, float>>;
     using LS = std::conditional_t<LDS == 2, double, float>;
@@ -877,7 +923,7 @@ __global__ void __launch_bounds__(LDS ? 64 : 256) k_momentum_rows(const MomArgs 
     }
 #pragma unroll 1
     for (int k = a.binc_ptr[row]; k < a.binc_ptr[row + 1]; ++k)
-        mom_boundary<T, MX>(a, a.binc[k], a.binc_slots[k], sr, dof, row, xr, D, R, bad, lt);
+        mom_boundary<T, MX, OP>(a, a.binc[k], a.binc_slots[k], sr, dof, row, xr, D, R, bad, lt);
     if constexpr (LDS != 0) {
         const int ns = sr.width < kLdsSlots ? sr.width : kLdsSlots;
 #pragma unroll 1
@@ -999,8 +1045,8 @@ void launch_assemble_momentum(Flow& f) {
     };
     const bool hr = a.beta != nullptr;
     // the off-diagonal blocks accumulated in HBM instead of the LDS tile (an
-    // autotuning candidate; hex-only meshes, upwind advection)
-    if (f.tune.mom_tile == 0 && !hr && !m.mixed) {
+    // autotuning candidate; hex-only meshes, upwind advection, no openings)
+    if (f.tune.mom_tile == 0 && !hr && !m.mixed && !f.has_opening) {
         if (a.ugrad_shifted) go(mixed ? k_momentum_rows<float, 1, 0, false, false>
                                       : k_momentum_rows<double, 1, 0, false, false>, 256);
         else go(mixed ? k_momentum_rows<float, 0, 0, false, false>
@@ -1008,18 +1054,23 @@ void launch_assemble_momentum(Flow& f) {
         OAGPU_HIP(hipGetLastError());
         return;
     }
-    auto pick = [&](auto mx) {
-        constexpr bool MX = decltype(mx)::value;
+    auto pick = [&](auto mx, auto op) {
+        constexpr bool MX = decltype(mx)::value, OP = decltype(op)::value;
         if (a.ugrad_shifted) {
-            if (mixed) go(hr ? k_momentum_rows<float, 1, 1, true, MX> : k_momentum_rows<float, 1, 1, false, MX>, 64);
-            else go(hr ? k_momentum_rows<double, 1, 2, true, MX> : k_momentum_rows<double, 1, 2, false, MX>, 64);
+            if (mixed) go(hr ? k_momentum_rows<float, 1, 1, true, MX, OP> : k_momentum_rows<float, 1, 1, false, MX, OP>, 64);
+            else go(hr ? k_momentum_rows<double, 1, 2, true, MX, OP> : k_momentum_rows<double, 1, 2, false, MX, OP>, 64);
         } else {
-            if (mixed) go(hr ? k_momentum_rows<float, 0, 1, true, MX> : k_momentum_rows<float, 0, 1, false, MX>, 64);
-            else go(hr ? k_momentum_rows<double, 0, 2, true, MX> : k_momentum_rows<double, 0, 2, false, MX>, 64);
+            if (mixed) go(hr ? k_momentum_rows<float, 0, 1, true, MX, OP> : k_momentum_rows<float, 0, 1, false, MX, OP>, 64);
+            else go(hr ? k_momentum_rows<double, 0, 2, true, MX, OP> : k_momentum_rows<double, 0, 2, false, MX, OP>, 64);
         }
     };
-    if (m.mixed) pick(std::true_type{});
-    else pick(std::false_type{});
+    if (f.has_opening) {
+        if (m.mixed) pick(std::true_type{}, std::true_type{});
+        else pick(std::false_type{}, std::true_type{});
+    } else {
+        if (m.mixed) pick(std::true_type{}, std::false_type{});
+        else pick(std::false_type{}, std::false_type{});
+    }
     OAGPU_HIP(hipGetLastError());
 }
 
@@ -1262,24 +1313,25 @@ __device__ void pc_generic(const PcArgs& a, int la, int e, uint64_t sl, const Se
     }
 }
 
-// inlet / outlet face ip (f, fk) nearest to this p' row, in FP64 in both
-// forms.  Inlet (pressureCorrectionAssemblerElemBoundaryConditions.cpp:
-// 923-1393): the specified mass flow rho U_bc . A leaves the rhs.  Outlet
-// (:1895-2530), unless reversed: the Rhie-Chow pressure term
-// -rho du_ip grad N . A for the columns off the side (the side's pressure is
-// specified) and the Rhie-Chow mass flow on the rhs.
+// inlet / outlet / opening face ip (f, fk) nearest to this p' row, in FP64
+// in both forms.  Inlet (pressureCorrectionAssemblerElemBoundaryConditions.
+// cpp:923-1393): the specified mass flow rho U_bc . A leaves the rhs.
+// Outlet (:1895-2530), unless reversed, and opening (:3026-3560), at every
+// ip: the Rhie-Chow pressure term -rho du_ip grad N . A for the columns off
+// the side (the side's pressure is specified) and the Rhie-Chow mass flow
+// on the rhs.
 template <int SP>
 __device__ __forceinline__ void pc_boundary(const PcArgs& a, int pk, uint64_t sl,
                                             const SellRow& sr, int dof, const double xr[3],
                                             double& D, double& R, int& bad) {
     const int f = pk >> 2, fk = pk & 3;
     const int bc = a.bf_bc[f], t = a.bc_type[bc];
-    if (t != OAGPU_BC_INLET && t != OAGPU_BC_OUTLET) return;
+    if (t != OAGPU_BC_INLET && t != OAGPU_BC_OUTLET && t != OAGPU_BC_OPENING) return;
     if (t == OAGPU_BC_OUTLET && a.rfl[4 * size_t(f) + fk] != 0) return;
     const int e = a.bf_elem[f], ord = a.bf_ord[f];
     const int* ce = a.conn + 8 * size_t(e);
     FaceIp<double> fi;
-    face_ip<true>(a.sg, a.xyz, ce, e, f, ord, fk, t == OAGPU_BC_OUTLET ? SP : a.u_shifted, xr, fi,
+    face_ip<true>(a.sg, a.xyz, ce, e, f, ord, fk, t == OAGPU_BC_INLET ? a.u_shifted : SP, xr, fi,
                   bad);
     const int us = a.u_shifted ? 1 : 0;
     double rip = 0;
@@ -1309,10 +1361,11 @@ __device__ __forceinline__ void pc_boundary(const PcArgs& a, int pk, uint64_t sl
     R -= outlet_rhie_chow(fi, ce, fk, us, a.U, a.du, a.p, a.gradP, a.rho, a.bc_p[bc]);
 }
 
-// The inlet / outlet face ips of the p' rows in io_row (each row one lane,
-// its stored row updated in place), after k_pcorr_rows.  A pressure-
-// reference row exists only without outlets, where the only face term is
-// the inlet rhs, which the reference row drops: it is skipped.
+// The inlet / outlet / opening face ips of the p' rows in io_row (each row
+// one lane, its stored row updated in place), after k_pcorr_rows.  A
+// pressure-reference row exists only without outlets and openings, where
+// the only face term is the inlet rhs, which the reference row drops: it is
+// skipped.
 template <int SP>
 __global__ void __launch_bounds__(256) k_pcorr_boundary(const PcArgs a, int nrow,
                                                         const int* io_row) {
@@ -1437,7 +1490,8 @@ void launch_assemble_pcorr(Flow& f) {
     a.etype = m.etype.p;
     a.rho = f.prm.rho;
     a.u_shifted = f.prm.u_shifted;
-    a.p_ref = f.has_outlet ? -1 : f.prm.p_ref_node;  // domain::pressureLevelRequired
+    // domain::pressureLevelRequired: none with an outlet or an opening
+    a.p_ref = f.has_outlet || f.has_opening ? -1 : f.prm.p_ref_node;
     a.vals = f.A1.p;
     a.b = f.b1.p;
     a.err = f.err.p;
@@ -1776,13 +1830,14 @@ __device__ void sc_generic(const ScArgs& a, int la, int e, uint64_t sl, const Se
     }
 }
 
-// inlet / outlet face ip (f, fk) nearest to this k / omega row, FP64 in
-// both forms, skipped while the ip is reversed.  Inlet
+// inlet / outlet / opening face ip (f, fk) nearest to this k / omega row,
+// FP64 in both forms, skipped while the ip is reversed.  Inlet
 // (phiAssemblerElemBoundaryConditions.hpp:894-1229, fixed value): the
 // advective flux msd phi_bc and the diffusive flux with phi_bc on the
 // side's nodes (their columns dropped), Gamma interpolated from the side's
-// nodes.  Outlet (:1231-1437, zero gradient): implicit upwind, msd on the
-// diagonal and msd phi_row on the rhs.
+// nodes.  Outlet (:1231-1437, zero gradient) and opening (:1439-1760, whose
+// ips are never reversed): implicit upwind, msd on the diagonal and
+// msd phi_row on the rhs.
 template <int SG>
 __device__ __forceinline__ void sc_boundary(const ScArgs& a, int pk, uint64_t sl,
                                             const SellRow& sr, int dof, int row,
@@ -1790,10 +1845,10 @@ __device__ __forceinline__ void sc_boundary(const ScArgs& a, int pk, uint64_t sl
                                             int& bad) {
     const int f = pk >> 2, fk = pk & 3;
     const int bc = a.bf_bc[f], t = a.bc_type[bc];
-    if (t != OAGPU_BC_INLET && t != OAGPU_BC_OUTLET) return;
+    if (t != OAGPU_BC_INLET && t != OAGPU_BC_OUTLET && t != OAGPU_BC_OPENING) return;
     if (a.rfl[4 * size_t(f) + fk] != 0) return;
     const double tm = a.msd[4 * size_t(f) + fk];
-    if (t == OAGPU_BC_OUTLET) {
+    if (t != OAGPU_BC_INLET) {
         R -= tm * a.phi[row];
         D += tm;
         return;
@@ -1827,7 +1882,7 @@ __device__ __forceinline__ void sc_boundary(const ScArgs& a, int pk, uint64_t sl
     }
 }
 
-// The inlet / outlet face ips of the k / omega rows in io_row (each row one
+// The inlet / outlet / opening face ips of the k / omega rows in io_row (each row one
 // lane, its stored row updated in place), after k_scalar_rows: the diagonal
 // part relaxed by urf, the rhs part by the boundary-row factor 0.75 (the
 // rows are boundary rows).  Omega rows fixed at no-slip wall nodes keep

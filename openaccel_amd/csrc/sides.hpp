@@ -112,16 +112,19 @@ __device__ inline void face_ip(const SideGeo& sg, const double* xyz, const int* 
     fi.g = q + 3 + (shifted ? 18 : 0);
 }
 
-// Rhie-Chow mass flow rate at an outlet face ip (flowModel.cpp:7323-7690,
-// SIMPLE: duRhs = du): rho U_ip . A - rho du_ip . (grad p - Gp) A with U
-// and du interpolated by the face shape functions (us: shifted), grad p by
-// the face ip's dN/dx with the specified pressure pbc on the side's nodes,
-// Gp the side nodes' mean nodal pressure gradient (weights 1/npf).
-// Pressures enter relative to pbc (dN/dx sums to zero over the nodes).
+// Rhie-Chow mass flow rate at an outlet / opening face ip (flowModel.cpp:
+// 7323-7690 / 8094-8380, SIMPLE: duRhs = du): rho U_ip . A - rho du_ip .
+// (grad p - Gp) A with U and du interpolated by the face shape functions
+// (us: shifted), grad p by the face ip's dN/dx, Gp the side nodes' mean
+// nodal pressure gradient (weights 1/npf).  side_p: the specified pressure
+// pbc on the side's nodes (outlets, and the p' rhs of openings), pressures
+// then entering relative to pbc (dN/dx sums to zero over the nodes); else
+// the nodal pressure on every node (the opening's rate update).
 template <typename T>
 __device__ inline double outlet_rhie_chow(const FaceIp<T>& fi, const int* ce, int fk, int us,
                                           const double* U, const double* du, const double* p,
-                                          const double* gradP, double rho, double pbc) {
+                                          const double* gradP, double rho, double pbc,
+                                          bool side_p = true) {
     const double fw = 1.0 / fi.npf;
     double rip = 0, u[3] = {0, 0, 0}, d[3] = {0, 0, 0}, Gp[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
     for (int q = 0; q < fi.npf; ++q) {
@@ -134,12 +137,13 @@ __device__ inline double outlet_rhie_chow(const FaceIp<T>& fi, const int* ce, in
             Gp[j] += fw * gradP[o + j];
         }
     }
+    const double p0 = side_p ? pbc : 0.0;
 #pragma unroll 1
     for (int ic = 0; ic < fi.npe; ++ic) {
-        if (fi.on_side(ic)) continue;
+        if (side_p && fi.on_side(ic)) continue;
         T g[3];
         fi.grad(ic, g);
-        const double pv = p[ce[ic]] - pbc;
+        const double pv = p[ce[ic]] - p0;
         for (int j = 0; j < 3; ++j) dp[j] += double(g[j]) * pv;
     }
     double m = 0;

@@ -221,14 +221,16 @@ void or_case_destroy(or_case *c)
 int or_case_set_bc_types(or_case *c, const int *type)
 {
     for (int b = 0; b < c->nbc; ++b)
-        if (type[b] < OR_BC_WALL || type[b] > OR_BC_SLIP) return 1;
+        if (type[b] < OR_BC_WALL || type[b] > OR_BC_OPENING) return 1;
     memcpy(c->bc_type, type, sizeof(int) * (size_t)c->nbc);
     memset(c->wnode, 0, sizeof(int) * (size_t)c->nn);
     memset(c->symA, 0, sizeof(double) * 3 * (size_t)c->nn);
     c->has_sym = 0;
     c->has_outlet = 0;
-    for (int f = 0; f < c->nbf; ++f)
-        if (c->bc_type[c->bf_bc[f]] == OR_BC_OUTLET) c->has_outlet = 1;
+    for (int f = 0; f < c->nbf; ++f) {  /* a pressure-specified boundary */
+        const int t = c->bc_type[c->bf_bc[f]];
+        if (t == OR_BC_OUTLET || t == OR_BC_OPENING) c->has_outlet = 1;
+    }
     for (int f = 0; f < c->nbf; ++f) {
         or_side sd;
         or_side_get(c, f, &sd);
@@ -680,7 +682,7 @@ void or_assemble_momentum(or_case *c)
         for (int a = 0; a < npe; ++a) nodes[a] = en[a];
         memset(lhs, 0, sizeof(lhs));
         memset(rhs, 0, sizeof(rhs));
-        if (btype == OR_BC_INLET || btype == OR_BC_OUTLET) {
+        if (btype == OR_BC_INLET || btype == OR_BC_OUTLET || btype == OR_BC_OPENING) {
             or_bc_momentum(c, f, lhs, rhs);
             or_apply_coeff(3, npe, nodes, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A3, c->b3);
             continue;
@@ -879,10 +881,10 @@ void or_assemble_pcorr(or_case *c)
     }
     }
     /* no-slip static walls and symmetry planes add nothing to p' (no
-     * frame/mesh motion); inlets and outlets do */
+     * frame/mesh motion); inlets, outlets and openings do */
     for (int f = 0; f < c->nbf; ++f) {
         const int btype = c->bc_type[c->bf_bc[f]];
-        if (btype != OR_BC_INLET && btype != OR_BC_OUTLET) continue;
+        if (btype != OR_BC_INLET && btype != OR_BC_OUTLET && btype != OR_BC_OPENING) continue;
         memset(lhs, 0, sizeof(lhs));
         memset(rhs, 0, sizeof(rhs));
         const int npe = or_topo_npe(or_etype(c, c->bf_elem[f]));

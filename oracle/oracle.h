@@ -178,10 +178,15 @@ void or_normal_relaxation(or_case *c);
  * (the bc_U row; zero-gradient pressure), static-pressure outlet
  * (zero-gradient velocity), symmetry plane, slip wall (no face terms in
  * the momentum equations, navierStokesAssemblerElemBoundaryConditions.cpp:
- * 2376-2657; nothing in p' or the side mass flows; k, omega zero-gradient) */
-enum { OR_BC_WALL = 0, OR_BC_INLET = 1, OR_BC_OUTLET = 2, OR_BC_SYMMETRY = 3, OR_BC_SLIP = 4 };
+ * 2376-2657; nothing in p' or the side mass flows; k, omega zero-gradient),
+ * static-pressure opening (:3234-3552: the bc_U row is the entrainment
+ * direction, normalised; a zero row takes the inward side normal,
+ * velocity.cpp:4830-4887 normal_to_boundary_condition) */
+enum { OR_BC_WALL = 0, OR_BC_INLET = 1, OR_BC_OUTLET = 2, OR_BC_SYMMETRY = 3, OR_BC_SLIP = 4,
+       OR_BC_OPENING = 5 };
 int or_case_set_bc_types(or_case *c, const int *type);
-/* per boundary id: outlet static pressure, inlet k and omega (NULL: zero) */
+/* per boundary id: outlet / opening static pressure, inlet k and omega
+ * (NULL: zero) */
 void or_case_set_bc_values(or_case *c, const double *p, const double *k, const double *w);
 /* flow-reversal flags of the boundary face ips [nbf][4] */
 const int *or_case_reversal(const or_case *c);

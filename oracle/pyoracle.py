@@ -244,7 +244,7 @@ class Case:
     def normal_relaxation(self):
         lib().or_normal_relaxation(self.h)
 
-    BC_WALL, BC_INLET, BC_OUTLET, BC_SYMMETRY = 0, 1, 2, 3
+    BC_WALL, BC_INLET, BC_OUTLET, BC_SYMMETRY, BC_SLIP, BC_OPENING = 0, 1, 2, 3, 4, 5
 
     def set_bc_values(self, p=None, k=None, w=None):
         """Per boundary id: outlet static pressure, inlet k and omega."""

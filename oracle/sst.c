@@ -254,11 +254,12 @@ void or_assemble_scalar(or_case *c, int which)
         or_apply_coeff(1, npe, en, lhs, rhs, nn, c->row_ptr, c->cols, c->diag_off, c->A1, c->b1);
     }
     }
-    /* inlets (fixed value) and outlets (zero gradient); nothing at walls (k
+    /* inlets (fixed value), outlets and openings (zero gradient while the
+     * ip is not reversed; an opening ip never is); nothing at walls (k
      * zero-gradient, omega rows fixed below) or symmetry planes */
     for (int f = 0; f < ((c->scalar_terms & 4) ? c->nbf : 0); ++f) {
         const int btype = c->bc_type[c->bf_bc[f]];
-        if (btype != OR_BC_INLET && btype != OR_BC_OUTLET) continue;
+        if (btype != OR_BC_INLET && btype != OR_BC_OUTLET && btype != OR_BC_OPENING) continue;
         const int *en = &c->conn[8 * c->bf_elem[f]];
         const int npe = or_topo_npe(or_etype(c, c->bf_elem[f]));
         double G[8];
