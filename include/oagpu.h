@@ -324,6 +324,32 @@ int oagpu_sa_level_copy(oagpu_flow *f, int level, int *agg, int *p_row_ptr, int 
  * hist[6] = U, V, W, P control residuals, momentum / p' Krylov iterations. */
 int oagpu_outer_iteration(oagpu_flow *f, const double *ctl, double *hist);
 
+/* Outer iterations of simulation::run's loop (simulation.cpp:166-204), each
+ * one HIP graph launch: segregatedFlowEquations::solve
+ * (segregatedFlowEquations.cpp:76-297) and, with sctl, the k-omega SST pass
+ * of segregatedShearStressTransportEquations::solve (:86-99) as
+ * oagpu_sst_iteration runs it, replayed from one captured graph with no host
+ * read inside the iteration.  ctl as oagpu_outer_iteration's, sctl =
+ * {k / omega GMRES max_it, rtol} or NULL (flow only).  The flow's p_method
+ * must be OAGPU_BICGSTAB_SA or (FP64 only) OAGPU_CG_SA; momentum, k and
+ * omega are solved by GMRES(30).  Runs up to n iterations, fewer when tol > 0
+ * and every reported scaled residual falls below tol; hist (NULL: not wanted)
+ * [n][10] = U, V, W, P residuals, momentum / p' iterations, k, omega
+ * residuals, k / omega iterations (0 without sctl); *done = iterations run.
+ * The flow's first such iteration runs without a graph (it builds the AMG
+ * hierarchy and sizes the graph's iteration caps); an iteration whose solve
+ * needs more than its cap is rolled back and run again uncapped, so results
+ * never depend on the caps.  eager != 0: the same device-controlled launch
+ * sequence without graphs (bitwise-equal results).  No reference
+ * counterpart beyond the loop it replaces. */
+int oagpu_graph_iterations(oagpu_flow *f, const double *ctl, const double *sctl, int n,
+                           double tol, int eager, double *hist, int *done);
+/* Counters of a flow's graph iterations: graph launches, iterations rolled
+ * back and run uncapped, graph captures; caps[4] = iteration slots of the
+ * momentum, p', k and omega solves in the current graph (NULL: not wanted). */
+int oagpu_graph_stats(oagpu_flow *f, long long *launches, long long *redone, long long *captures,
+                      int *caps);
+
 /* Batched independent cases (many small flows of any meshes, sizes and
  * parameters): each of the n flows is advanced by up to `iters` outer
  * iterations (oagpu_outer_iteration, in its own HIP stream) on a pool of

@@ -132,6 +132,9 @@ _SIGS = {
     "oagpu_sa_info": [C.c_void_p, _ip, _ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _dp],
     "oagpu_sa_level_copy": [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _ip, _ip, _dp],
     "oagpu_outer_iteration": [C.c_void_p, _dp, _dp],
+    "oagpu_graph_iterations": [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_int, _dp, _ip],
+    "oagpu_graph_stats": [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                          C.POINTER(C.c_longlong), _ip],
     "oagpu_time_step": [C.c_void_p, C.c_int, C.c_double],
     "oagpu_profile": [C.c_void_p, C.c_char_p],
     "oagpu_autotune": [C.c_void_p, _dp, C.c_int],
@@ -478,6 +481,26 @@ class FlowCase:
         h = np.zeros(6)
         _check(load_library().oagpu_outer_iteration(self.h, _d(c), _d(h)))
         return h
+
+    def graph_iterations(self, ctl, sctl=None, n=1, tol=0.0, eager=False):
+        """Up to n outer iterations (flow, then SST with sctl), each one HIP
+        graph launch (eager: the same device-controlled launches without a
+        graph).  Returns hist[iterations run][10]: U, V, W, P residuals,
+        momentum / p' iterations, k, omega residuals, k / omega iterations."""
+        c = np.ascontiguousarray(ctl, dtype=np.float64)
+        sc = None if sctl is None else np.ascontiguousarray(sctl, dtype=np.float64)
+        h = np.zeros((max(n, 1), 10))
+        done = C.c_int()
+        _check(load_library().oagpu_graph_iterations(self.h, _d(c), None if sc is None else _d(sc), n,
+                                                     tol, 1 if eager else 0, _d(h), C.byref(done)))
+        return h[:done.value]
+
+    def graph_stats(self):
+        """{launches, redone, captures, caps} of this flow's graph iterations."""
+        a, b, c = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        caps = np.zeros(4, dtype=np.int32)
+        _check(load_library().oagpu_graph_stats(self.h, C.byref(a), C.byref(b), C.byref(c), _i(caps)))
+        return {"launches": a.value, "redone": b.value, "captures": c.value, "caps": caps.tolist()}
 
     CLOCKS = ("momentum", "pcorr", "scalar", "mdot", "gradU", "gradP", "amg_pre", "amg_post",
               "amg_tail", "spmv1", "spmv3", "spmv1_f32", "spmv3_f32")

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "liboagpu.so")
 OBJDIR = os.path.join(ROOT, "_build")
 
 SOURCES = ["mesh.cpp", "fields.hip", "rows.hip", "bc.hip", "krylov.hip", "amg.hip", "sst.hip", "topo.hip",
-           "tune.hip", "capi.cpp"]
+           "tune.hip", "graph.hip", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

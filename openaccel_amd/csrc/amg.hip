@@ -1801,13 +1801,14 @@ static V* sav(SaLevel& l, int which) {  // 0 R, 1 X, 2 S, 3 T, 4 Y, 5 A, 6 dinv,
 }
 
 template <typename V>
-void sa_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, bool timing) {
+void sa_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, bool timing,
+               const int* stop) {
     Sa& sa = f.sa;
     const int nl = int(sa.lv.size());
     OAGPU_REQUIRE(nl > 1, "smoothed-aggregation hierarchy without a coarse level");
     const int last = nl - 1;
     const V om = V(kOmega);
-    const int* flag = f.flag.p;
+    const int* flag = stop ? stop : f.flag.p;
     hipStream_t st = f.stream;
     auto rhs = [&](int l) -> const V* { return l == 0 ? r : sav<V>(sa.lv[l], 0); };
     // down: pre-smoothing from zero with the residual, restriction
@@ -1870,7 +1871,9 @@ template void amg_vcycle<double>(Flow&, const double*, const double*, const doub
                                  const float*);
 template void amg_vcycle<float>(Flow&, const float*, const float*, const float*, float*, double*,
                                 float*, bool, bool, const float*, double*, const float*);
-template void sa_vcycle<double>(Flow&, const double*, const double*, const double*, double*, bool);
-template void sa_vcycle<float>(Flow&, const float*, const float*, const float*, float*, bool);
+template void sa_vcycle<double>(Flow&, const double*, const double*, const double*, double*, bool,
+                                const int*);
+template void sa_vcycle<float>(Flow&, const float*, const float*, const float*, float*, bool,
+                               const int*);
 
 }  // namespace oagpu

@@ -1052,6 +1052,27 @@ int oagpu_batch_outer_iterations(oagpu_flow* const* flows, int n, const double* 
     });
 }
 
+int oagpu_graph_iterations(oagpu_flow* h, const double* ctl, const double* sctl, int n, double tol,
+                           int eager, double* hist, int* done) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h && ctl && n >= 0, "bad argument");
+        const int it = oagpu::graph_iterations(h->f, ctl, sctl, n, tol, hist, eager != 0);
+        if (done) *done = it;
+    });
+}
+
+int oagpu_graph_stats(oagpu_flow* h, long long* launches, long long* redone, long long* captures,
+                      int* caps) {
+    return guarded([&] {
+        OAGPU_REQUIRE(h, "null flow");
+        const oagpu::IterGraph& g = h->f.ig;
+        if (launches) *launches = g.launches;
+        if (redone) *redone = g.replays_redone;
+        if (captures) *captures = g.captures;
+        if (caps) std::copy(g.cap, g.cap + 4, caps);
+    });
+}
+
 int oagpu_autotune(oagpu_flow* h, double* report, int n_report) {
     return guarded([&] {
         OAGPU_REQUIRE(h, "null flow");
@@ -1081,6 +1102,14 @@ int oagpu_tune_set(oagpu_flow* h, const char* which, int variant) {
             OAGPU_REQUIRE(variant == 0 || variant == 8 || variant == 16,
                           "occupancy cap: 0, 8 or 16 waves per CU");
             (which[0] == 's' ? t.cap_scalar : which[0] == 'm' ? t.cap_mdot : t.cap_grad) = variant;
+        } else if (!std::strcmp(which, "graph_cap")) {
+            // iteration slots of every solve in the next captured iteration graph
+            // (a solve that needs more rolls its iteration back and raises them)
+            OAGPU_REQUIRE(variant >= 1, "graph_cap: at least 1 slot");
+            OAGPU_REQUIRE(h->f.ig.primed, "graph_cap: run a graph iteration first");
+            oagpu::IterGraph& g = h->f.ig;
+            for (int& c : g.cap) c = variant;
+            g.reset();
         } else {
             throw oagpu::Error(OAGPU_EINVAL, std::string("unknown tunable ") + which);
         }

@@ -139,6 +139,33 @@ struct CgGraph {
     ~CgGraph() { reset(); }
 };
 
+// The whole outer iteration captured as one HIP graph (graph.hip): the
+// instantiated graph and what it was captured with, the iteration-slot caps
+// of its four Krylov solves (momentum, p', k, omega), and the device state
+// its one-workgroup control kernels keep.
+struct IterGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<double> key;    // controls, caps and operands of the captured sequence
+    int cap[4] = {0, 0, 0, 0};  // iteration slots per solve in the graph
+    int seen[4] = {0, 0, 0, 0}; // largest iteration count of each solve since the last re-capture
+    int since = 0;              // iterations since the caps were last set
+    bool primed = false;        // one uncaptured iteration has run (hierarchy built, caps known)
+    long long launches = 0, replays_redone = 0, captures = 0;
+    DBuf<double> sd;            // Krylov scalars of the running solve (GMRES H, g, rotations; BiCGStab rho, alpha, omega; tolerances)
+    DBuf<int> si;               // its flags and counters
+    DBuf<double> rec;           // per-iteration record: raw residuals, scales, iterations, caps exceeded
+    DBuf<double> sc;            // field scales U, p, k, omega
+    DBuf<double> snap;          // saved iteration state (restored when a solve exceeds its cap)
+    DBuf<unsigned long long> tab;  // copy table of the saved state: {src, dst, 32-bit words}
+    int ntab = 0;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        key.clear();
+    }
+    ~IterGraph() { reset(); }
+};
+
 struct Flow {
     DevMesh* m = nullptr;
     oagpu_flow_params prm{};
@@ -180,6 +207,7 @@ struct Flow {
     Amg amg;
     Sa sa;
     CgGraph cg_graph;
+    IterGraph ig;
     // effective viscosity mu + mu_t per node (mu alone while laminar)
     DBuf<double> mueff;
     // blending factors of the limited deferred correction (high resolution)
@@ -313,6 +341,21 @@ int* scratch_device_ptr();
 void autotune(Flow& f, double* report, int n_report);
 void time_spmv_variants(Flow& f, int N, bool f32, double* best_ms, double* default_ms);
 
+// krylov.hip: CG launches of the graph-captured iteration (no host reads):
+// set-up (x = 0, r = b, first preconditioner application, tolerance, flags
+// in f.flag / f.scal) and one iteration; sa: smoothed-aggregation V-cycle,
+// else Jacobi
+void cg_device_begin(Flow& f, int max_it, double rtol, double atol, bool sa);
+void cg_device_iteration(Flow& f, bool sa);
+
+// graph.hip: n outer iterations (flow, then SST when sctl is given) each
+// replayed from one HIP graph; hist[10] per iteration: scaled residuals U
+// (3) and p', momentum / p' iterations, scaled residuals k and omega, their
+// iterations.  eager: the same device-controlled launch sequence without a
+// graph (every solve polled to its end).  Returns the iterations run.
+int graph_iterations(Flow& f, const double* ctl, const double* sctl, int n, double tol,
+                     double* hist, bool eager);
+
 // amg.hip
 void amg_build(Flow& f);                       // hierarchy structure (once per mesh)
 template <typename V>
@@ -336,6 +379,7 @@ void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* 
 void sa_build(Flow& f);
 void sa_setup(Flow& f, const double* A0, bool want_float);
 template <typename V>
-void sa_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, bool timing = false);
+void sa_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, bool timing = false,
+               const int* stop = nullptr);  // stop: the kernels' stop flag (default f.flag)
 
 }  // namespace oagpu

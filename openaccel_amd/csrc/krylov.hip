@@ -690,6 +690,33 @@ static int cg_core(Flow& f, const V* A, const V* b, const V* dinv, V* x, V* r, V
     return hflag[1];
 }
 
+// The CG of the graph-captured iteration (FP64 scalar system, sa: one
+// smoothed-aggregation V-cycle as preconditioner, else Jacobi): the launches
+// of cg_core without its host polls.  f.flag[0] stops every later launch once
+// the solve has converged or used max_it iterations; f.flag[1] counts them,
+// f.scal[4] / [3] hold the initial / latest r . r.
+void cg_device_begin(Flow& f, int max_it, double rtol, double atol, bool sa) {
+    const size_t n = size_t(f.m->nn);
+    k_clear_flag<<<1, 1, 0, f.stream>>>(f.flag.p);
+    if (sa) {
+        k_cg_init<double, true><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, f.b1.p, f.dinv.p, f.x1.p, f.r.p,
+                                                                       f.z.p, f.pk.p, f.part.p);
+        sa_vcycle<double>(f, f.A1.p, f.dinv.p, f.r.p, f.z.p);
+        k_pc_partials<double><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, f.r.p, f.z.p, nullptr, f.pk.p,
+                                                                     f.part.p, nullptr, f.flag.p);
+    } else {
+        k_cg_init<double, false><<<kRedGrid, kRedBlock, 0, f.stream>>>(n, f.b1.p, f.dinv.p, f.x1.p, f.r.p,
+                                                                        f.z.p, f.pk.p, f.part.p);
+    }
+    k_cg_start<<<1, kRedBlock, 0, f.stream>>>(f.part.p, rtol, atol, max_it, f.scal.p, f.flag.p);
+    OAGPU_HIP(hipGetLastError());
+}
+
+void cg_device_iteration(Flow& f, bool sa) {
+    cg_iteration<1, double>(f, f.A1.p, f.dinv.p, f.x1.p, f.r.p, f.z.p, f.pk.p, f.q.p, sa, sa);
+    OAGPU_HIP(hipGetLastError());
+}
+
 // sa: the smoothed-aggregation V-cycle (OAGPU_CG_SA), else the unsmoothed
 // hierarchy's K-cycle (OAGPU_CG_AMG)
 template <int N>
