@@ -256,28 +256,39 @@ def fp64_counterpart(clock):
     return clock[:-4] if clock.endswith("_f32") else clock
 
 
-def run_form(oa, f, step, steps, warmup, dominant=None, tune=True):
+def run_form(oa, f, step, steps, warmup, dominant=None, tune=True, clock_step=None, clock_steps=2):
     """Autotune the flow's launches (set-up, untimed), warm up (the first
     step builds the AMG hierarchy), then `steps` timed steps with every main
     kernel clocked (events recorded on the flow's stream, read after the
-    steps).  The dominant kernel is `dominant` or the one with the largest
-    device time per timed step.  Returns wall seconds, step outputs,
-    per-kernel ms/step over the timed steps, the dominant kernel's
-    (launches, total ms) and the tuner's choices."""
+    steps).  With clock_step (graph-replayed steps, whose launches are not
+    clocked) the kernels are clocked over `clock_steps` further steps of
+    clock_step -- the same launch sequence issued without a graph -- right
+    after the timed ones.  The dominant kernel is `dominant` or the one with
+    the largest device time per clocked step.  Returns wall seconds, step
+    outputs, per-kernel ms/step over the clocked steps, the dominant
+    kernel's (launches, total ms) and the tuner's choices."""
     tuning = {k: [float(x) for x in v] for k, v in f.autotune().items()} if tune else None
     hist = [step() for _ in range(warmup)]
-    f.profile("all")
+    if clock_step is None:
+        f.profile("all")
     oa.load_library().oagpu_synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         hist.append(step())
     oa.load_library().oagpu_synchronize()
     t = time.perf_counter() - t0
+    nclk = steps
+    if clock_step is not None:
+        f.profile("all")
+        for _ in range(clock_steps):
+            clock_step()
+        oa.load_library().oagpu_synchronize()
+        nclk = clock_steps
     timed = {}
     for k in f.CLOCKS:
         ms, n = f.kernel_stats(k)
         if n:
-            timed[k] = {"ms_per_step": ms / steps, "launches_per_step": n / steps, "ms": ms, "n": n}
+            timed[k] = {"ms_per_step": ms / nclk, "launches_per_step": n / nclk, "ms": ms, "n": n}
     f.profile("none")
     if dominant is None or dominant not in timed:
         dominant = max(timed, key=lambda k: timed[k]["ms_per_step"])
@@ -448,12 +459,21 @@ def bench_bfs(args):
         f = make_flow(prec)
         bfs_setup(oa, mesh, f)
 
-        def step():
-            return np.concatenate([f.outer_iteration(BFS_CTL), f.sst_iteration(BFS_SCTL)])
+        clock_step = None
+        if args.iteration == "graph":
+            def step():  # one flow + SST outer iteration, one graph launch
+                return f.graph_iterations(BFS_CTL, BFS_SCTL, n=1)[0]
+
+            def clock_step():  # the same launches without the graph, clocked
+                return f.graph_iterations(BFS_CTL, BFS_SCTL, n=1, eager=True)[0]
+        else:
+            def step():
+                return np.concatenate([f.outer_iteration(BFS_CTL), f.sst_iteration(BFS_SCTL)])
 
         dom = None if prec == oa.OAGPU_MIXED else fp64_counterpart(res[oa.OAGPU_MIXED]["dominant"])
-        res[prec] = run_form(oa, f, step, args.steps, args.warmup, dom)
+        res[prec] = run_form(oa, f, step, args.steps, args.warmup, dom, clock_step=clock_step)
         res[prec]["sa_levels"] = f.sa_levels() if args.bfs_p_method == "bicgstab_sa" else []
+        res[prec]["graph"] = f.graph_stats() if args.iteration == "graph" else None
         f.close()
         print(f"[bench] {'mixed' if prec else 'fp64'} {args.steps / res[prec]['seconds']:.3f} it/s, "
               f"dominant {res[prec]['dominant']}", file=sys.stderr, flush=True)
@@ -499,7 +519,9 @@ def bench_bfs(args):
                                f"iteration + SST iteration; mixed form: FP32 local matrices into "
                                f"FP64 CRS, FP32 inner Krylov solves ("
                                f"{P_METHOD_NAMES[args.bfs_p_method]} p', GMRES "
-                               f"momentum / k / omega) under FP64 refinement",
+                               f"momentum / k / omega) under FP64 refinement; "
+                               + ("each outer iteration one HIP graph launch" if args.iteration == "graph"
+                                  else "launches issued by the host"),
                    "dims": dims, "elements": ne, "nodes": nn, "nnz": nnz},
         "roofline": roofline(rmx, ab_mx, "bfs", 1),
         "kernels": kernel_table(rmx, ab_mx, ne),
@@ -508,6 +530,8 @@ def bench_bfs(args):
                  "roofline": roofline(r64, ab_64, "bfs", 0), "kernels": kernel_table(r64, ab_64, ne),
                  **hist(r64)},
         "mixed_over_fp64": round(itmx / it64, 3),
+        "iteration": args.iteration,
+        "graph": {"mixed": rmx["graph"], "fp64": r64["graph"]} if args.iteration == "graph" else None,
         "coef_err_vs_fp64": cerr,
         "p_amg_levels": [{"rows": l[0], "entries": l[2], "p_entries": l[1], "omega_p": round(l[3], 4)}
                          for l in rmx["sa_levels"]],
@@ -558,12 +582,20 @@ def bench_cavity(args):
 
     if args.to_residual:
         return bench_cavity_to_residual(args, oa, dm, make_flow, setup)
+    # each outer iteration one graph launch (the smoothed-aggregation p' solvers)
+    graph_mode = args.iteration == "graph" and pm in (oa.OAGPU_CG_SA, oa.OAGPU_BICGSTAB_SA)
     res = {}
     for prec in (oa.OAGPU_MIXED, oa.OAGPU_FP64):
         f = make_flow(prec)
         setup(f)
         dom = None if prec == oa.OAGPU_MIXED else fp64_counterpart(res[oa.OAGPU_MIXED]["dominant"])
-        res[prec] = run_form(oa, f, lambda: f.outer_iteration(CTL), args.steps, args.warmup, dom)
+        if graph_mode:
+            res[prec] = run_form(oa, f, lambda: f.graph_iterations(CTL, None, n=1)[0], args.steps,
+                                 args.warmup, dom,
+                                 clock_step=lambda: f.graph_iterations(CTL, None, n=1, eager=True)[0])
+            res[prec]["graph"] = f.graph_stats()
+        else:
+            res[prec] = run_form(oa, f, lambda: f.outer_iteration(CTL), args.steps, args.warmup, dom)
         f.close()
         print(f"[bench] {'mixed' if prec else 'fp64'} {args.steps / res[prec]['seconds']:.3f} it/s, "
               f"dominant {res[prec]['dominant']}", file=sys.stderr, flush=True)
@@ -597,8 +629,11 @@ def bench_cavity(args):
         "config": {"workload": f"lid-driven cavity, uniform{' perturbed' if args.perturb else ''} "
                                f"{args.n}^3 hex8 ({ne} elements, {nn} nodes), Re=100, mixed form; "
                                f"Jacobi-GMRES(20 it, rtol 0.1) momentum, "
-                               f"{P_METHOD_NAMES[args.p_method]} (200 it, rtol 1e-6) p'",
+                               f"{P_METHOD_NAMES[args.p_method]} (200 it, rtol 1e-6) p'; "
+                               + ("each outer iteration one HIP graph launch" if graph_mode
+                                  else "launches issued by the host"),
                    "n": args.n, "elements": ne, "nodes": nn, "nnz": nnz},
+        "graph": {"mixed": rmx.get("graph"), "fp64": r64.get("graph")} if graph_mode else None,
         "roofline": roofline(rmx, ab_mx, f"{wl}_{args.n}", 1),
         "kernels": kernel_table(rmx, ab_mx, ne),
         "autotune": {"mixed": rmx["tuning"], "fp64": r64["tuning"]},
@@ -838,11 +873,14 @@ def bench_batch(args):
     t_seq = seq.sum()
     print(f"[bench] fp64 one at a time: {t_seq:.2f} s", file=sys.stderr, flush=True)
 
+    graph_mode = args.iteration == "graph" and args.p_method == "cg_sa"
+
     def batched(precision):
         flows = [flow(c, precision) for c in cases]
         oa.load_library().oagpu_synchronize()
         t0 = time.perf_counter()
-        hist, done, sec = oa.batch_outer_iterations(flows, CTL, iters, tol=tol, threads=16)
+        run = oa.batch_graph_iterations if graph_mode else oa.batch_outer_iterations
+        hist, done, sec = run(flows, CTL, iters, tol=tol, threads=16)
         t = time.perf_counter() - t0
         for f in flows:
             f.close()
@@ -868,7 +906,9 @@ def bench_batch(args):
                                f"random unit vector in the lid plane (seeded); each case runs until "
                                f"its scaled residuals are below {tol:g} or {iters} outer "
                                f"iterations; mixed form batched (one stream per case, 16 host "
-                               f"threads) vs FP64 one case at a time",
+                               f"threads"
+                               + ("; each outer iteration of a case one HIP graph launch" if graph_mode else "")
+                               + ") vs FP64 one case at a time (launches issued by the host)",
                    "cases": len(cases), "elements": elems, "iteration_cap": iters},
         "fp64_one_at_a_time": {"value": round(len(cases) / t_seq, 4), "unit": "cases/s",
                                "per_case_s_p50": round(float(np.percentile(seq, 50)), 4),
@@ -900,6 +940,8 @@ def main():
                          "cavity: the lid-driven cavity; duct: the 3.9M-cell mixed-topology "
                          "duct; batch: 96 independent cavities")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--iteration", choices=("graph", "eager"), default="graph",
+                    help="bfs: each outer iteration one HIP graph launch, or launched by the host")
     ap.add_argument("--caller-order", action="store_true",
                     help="keep the generator's node numbering on the device (no locality renumbering)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: all host cores)")

@@ -344,6 +344,12 @@ int oagpu_outer_iteration(oagpu_flow *f, const double *ctl, double *hist);
  * counterpart beyond the loop it replaces. */
 int oagpu_graph_iterations(oagpu_flow *f, const double *ctl, const double *sctl, int n,
                            double tol, int eager, double *hist, int *done);
+/* oagpu_batch_outer_iterations with every case advanced by
+ * oagpu_graph_iterations (each outer iteration one graph launch in the
+ * case's stream); hist [n][iters][10] as oagpu_graph_iterations'. */
+int oagpu_batch_graph_iterations(oagpu_flow *const *flows, int n, const double *ctl, int iters,
+                                 double tol, int threads, double *hist, int *done,
+                                 double *seconds);
 /* Counters of a flow's graph iterations: graph launches, iterations rolled
  * back and run uncapped, graph captures; caps[4] = iteration slots of the
  * momentum, p', k and omega solves in the current graph (NULL: not wanted). */

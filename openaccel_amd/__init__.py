@@ -133,6 +133,8 @@ _SIGS = {
     "oagpu_sa_level_copy": [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, _ip, _ip, _dp],
     "oagpu_outer_iteration": [C.c_void_p, _dp, _dp],
     "oagpu_graph_iterations": [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_int, _dp, _ip],
+    "oagpu_batch_graph_iterations": [C.POINTER(C.c_void_p), C.c_int, _dp, C.c_int, C.c_double,
+                                     C.c_int, _dp, _ip, _dp],
     "oagpu_graph_stats": [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                           C.POINTER(C.c_longlong), _ip],
     "oagpu_time_step": [C.c_void_p, C.c_int, C.c_double],
@@ -217,6 +219,21 @@ def batch_outer_iterations(flows, ctl, iters, tol=0.0, threads=0):
     done = np.zeros(max(n, 1), np.int32)
     sec = np.zeros(max(n, 1))
     _check(load_library().oagpu_batch_outer_iterations(
+        C.cast(hs, C.POINTER(C.c_void_p)), n, _d(c), iters, tol, threads, _d(hist), _i(done),
+        _d(sec)))
+    return hist[:n, :iters], done[:n], sec[:n]
+
+
+def batch_graph_iterations(flows, ctl, iters, tol=0.0, threads=0):
+    """batch_outer_iterations with every outer iteration of a case one HIP
+    graph launch (oagpu_batch_graph_iterations): hist [n][iters][10]."""
+    n = len(flows)
+    hs = (C.c_void_p * max(n, 1))(*[f.h.value if hasattr(f.h, "value") else f.h for f in flows])
+    c = np.ascontiguousarray(ctl, dtype=np.float64)
+    hist = np.zeros((max(n, 1), max(iters, 1), 10))
+    done = np.zeros(max(n, 1), np.int32)
+    sec = np.zeros(max(n, 1))
+    _check(load_library().oagpu_batch_graph_iterations(
         C.cast(hs, C.POINTER(C.c_void_p)), n, _d(c), iters, tol, threads, _d(hist), _i(done),
         _d(sec)))
     return hist[:n, :iters], done[:n], sec[:n]

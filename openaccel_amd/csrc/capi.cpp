@@ -1061,6 +1061,51 @@ int oagpu_graph_iterations(oagpu_flow* h, const double* ctl, const double* sctl,
     });
 }
 
+int oagpu_batch_graph_iterations(oagpu_flow* const* flows, int n, const double* ctl, int iters,
+                                 double tol, int threads, double* hist, int* done, double* seconds) {
+    return guarded([&] {
+        OAGPU_REQUIRE(flows && ctl && n >= 0 && iters >= 0, "bad batch arguments");
+        for (int c = 0; c < n; ++c) OAGPU_REQUIRE(flows[c], "null flow in batch");
+        for (int c = 0; c < n; ++c)
+            for (int d = 0; d < c; ++d) OAGPU_REQUIRE(flows[c] != flows[d], "flow listed twice");
+        const int nt = std::max(1, std::min(threads > 0 ? threads : 16, n));
+        std::atomic<int> next{0};
+        std::mutex mu;
+        std::string first_error;
+        int first_code = OAGPU_OK;
+        auto worker = [&] {
+            for (int c = next++; c < n; c = next++) {
+                const auto t0 = std::chrono::steady_clock::now();
+                int it = 0;
+                try {
+                    it = oagpu::graph_iterations(flows[c]->f, ctl, nullptr, iters, tol,
+                                                 hist ? hist + size_t(c) * iters * 10 : nullptr, false);
+                } catch (const oagpu::Error& e) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (first_code == OAGPU_OK) {
+                        first_code = e.code;
+                        first_error = "case " + std::to_string(c) + ": " + e.what();
+                    }
+                } catch (const std::exception& e) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (first_code == OAGPU_OK) {
+                        first_code = OAGPU_EINTERNAL;
+                        first_error = "case " + std::to_string(c) + ": " + e.what();
+                    }
+                }
+                const std::chrono::duration<double> dt = std::chrono::steady_clock::now() - t0;
+                if (done) done[c] = it;
+                if (seconds) seconds[c] = dt.count();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+        worker();
+        for (auto& t : pool) t.join();
+        if (first_code != OAGPU_OK) throw oagpu::Error(first_code, first_error);
+    });
+}
+
 int oagpu_graph_stats(oagpu_flow* h, long long* launches, long long* redone, long long* captures,
                       int* caps) {
     return guarded([&] {

@@ -270,7 +270,8 @@ inline unsigned occupancy_lds(unsigned block, unsigned static_lds, int cap) {
 
 // start / stop of a clocked launch on the flow's stream (no-ops unless the
 // flow's profiling covers the kernel); a pool of 4096 pairs is folded in
-// (after a stream synchronisation) when full.
+// (after a stream synchronisation) when full.  Launches captured into a
+// graph are not clocked (the flow's clocks are off while it is captured).
 inline void clock_begin(Flow& f, int id) {
     if (!(f.prof_mask & (1u << id))) return;
     KernelClock& c = f.clk[id];
@@ -347,6 +348,7 @@ void time_spmv_variants(Flow& f, int N, bool f32, double* best_ms, double* defau
 // else Jacobi
 void cg_device_begin(Flow& f, int max_it, double rtol, double atol, bool sa);
 void cg_device_iteration(Flow& f, bool sa);
+void cg_device_iteration_f32(Flow& f, bool sa);  // the mixed form's FP32 inner CG
 
 // graph.hip: n outer iterations (flow, then SST when sctl is given) each
 // replayed from one HIP graph; hist[10] per iteration: scaled residuals U

@@ -19,15 +19,14 @@
 // refinement step is due).  A solve that reaches its cap unfinished is
 // recorded: the host then restores the state saved by the graph's first
 // launch and runs the same sequence uncapped, polling the finished flag, so
-// the result never depends on the caps; the caps follow the iteration counts
-// seen (largest + 2).  The same sequence run without a graph (eager) makes
+// the result never depends on the caps; a cap is the solve's iteration limit
+// when that is small, else the largest count seen plus a quarter (at least 4).  The same sequence run without a graph (eager) makes
 // the same decisions on the same values: graph replays and eager runs are
 // bitwise equal.
 //
 // Solves covered: Jacobi-GMRES (FP64, and FP32 under FP64 refinement) for
 // the momentum, k and omega systems; for p' the smoothed-aggregation
-// AMG-BiCGStab (FP64, and FP32 under FP64 refinement) and the smoothed-
-// aggregation AMG-CG (FP64).
+// AMG-BiCGStab and AMG-CG (FP64, and FP32 under FP64 refinement).
 #include "device_util.hpp"
 #include "flow.hpp"
 
@@ -58,7 +57,8 @@ enum : int {
     I_RESTART, // BiCGStab: r0 = r and p = r at the next direction update
     I_MRES,    // GMRES restart length
     I_MIXED,   // FP32 inner solves under FP64 refinement
-    I_NI = 16
+    I_NSTART,  // stop flag of the V-cycle that starts an inner CG: no start due
+    I_NI = 20
 };
 constexpr int kMaxRes = 32;  // GMRES restart length limit of the device Arnoldi step
 // doubles (IterGraph::sd)
@@ -97,6 +97,7 @@ __device__ __forceinline__ bool gate_open(const int* si, int g) {
 
 __device__ __forceinline__ void set_vstop(int* si) {
     si[I_VSTOP] = (si[I_DONE] || si[I_SKIP] || si[I_IDONE] || si[I_ISTART]) ? 1 : 0;
+    si[I_NSTART] = (si[I_ISTART] && !si[I_DONE]) ? 0 : 1;
 }
 
 __device__ inline double g_block_sum(double v, double* sh) {
@@ -150,12 +151,31 @@ struct SellArgs {
     const int *slice_ptr, *slice_w, *cols;
 };
 
+// Gated launches run on a capped grid (grid-stride), so that a launch whose
+// gate is closed dispatches few workgroups.
+constexpr unsigned kGridCap = 2048;
+
+// Row blocks of a capped launch with XCD-contiguous ranges: the workgroups
+// on XCD x walk the x-th eighth of the nvb row blocks (the rows' gathered
+// neighbours then stay in that XCD's L2).
+__device__ __forceinline__ void xcd_walk(unsigned nvb, unsigned& v0, unsigned& vend, unsigned& step) {
+    const unsigned per_x = (nvb + kNumXcd - 1) / kNumXcd;
+    const unsigned x = blockIdx.x % kNumXcd;
+    v0 = x * per_x + blockIdx.x / kNumXcd;
+    vend = min(nvb, (x + 1) * per_x);
+    step = gridDim.x / kNumXcd;
+}
+
+#define GRID_STRIDE(i, n) \
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < (n); i += size_t(gridDim.x) * blockDim.x)
+
 // ------------------------------------------------------ generic kernels
 // state of a new solve
 __global__ void k_g_solve_init(int* si, double* sd, int max_it, int mres, int mixed, double rtol,
                                double atol, double inner_floor) {
     if (threadIdx.x != 0) return;
     for (int k = 0; k < I_NI; ++k) si[k] = 0;
+    si[I_NSTART] = 1;
     si[I_MAXIT] = max_it;
     si[I_BUDGET] = max_it;
     si[I_MRES] = mres;
@@ -229,8 +249,7 @@ __global__ void k_g_to_float(size_t n, const double* x, float* y) {
 // x64 += rn * d (FP32 correction)
 __global__ void k_mx_refine(size_t n, const double* sd, const float* d, double* x, const int* si) {
     if (!gate_open(si, G_REFINE)) return;
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) x[i] += sd[D_RN] * double(d[i]);
+    GRID_STRIDE(i, n) x[i] += sd[D_RN] * double(d[i]);
 }
 
 // outer residual norm after a refinement step: finish, or start the next
@@ -340,8 +359,7 @@ __global__ void __launch_bounds__(kRedBlock) k_gm_start(const double* part, doub
 template <typename T>
 __global__ void k_gm_v0(size_t n, const T* src, T* Vb, const double* sd, const int* si, int which) {
     if (si[I_DONE] || si[I_FRESH] != which) return;
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) Vb[i] = T(sd[D_SCALE] * double(src[i]));
+    GRID_STRIDE(i, n) Vb[i] = T(sd[D_SCALE] * double(src[i]));
 }
 
 // w = A (dinv .* V_j), row per lane
@@ -349,12 +367,16 @@ template <int N, typename T>
 __global__ void __launch_bounds__(256) k_gm_spmv(SellArgs S, const T* A, const T* Vb, const T* dinv,
                                                   T* w, const int* si) {
     if (!gate_open(si, G_STEP)) return;
-    const int i = int(xcd_block() * blockDim.x + threadIdx.x);
-    if (i >= S.nn) return;
     const T* x = Vb + size_t(si[I_J]) * (size_t(S.nn) * N);
-    T acc[N];
-    g_row<N, T>(i, S.slice_ptr, S.slice_w, S.cols, A, x, dinv, acc);
-    for (int m = 0; m < N; ++m) w[N * size_t(i) + m] = acc[m];
+    unsigned v, vend, step;
+    xcd_walk(unsigned((S.nn + 255) / 256), v, vend, step);
+    for (; v < vend; v += step) {
+        const int i = int(v * 256 + threadIdx.x);
+        if (i >= S.nn) break;
+        T acc[N];
+        g_row<N, T>(i, S.slice_ptr, S.slice_w, S.cols, A, x, dinv, acc);
+        for (int m = 0; m < N; ++m) w[N * size_t(i) + m] = acc[m];
+    }
 }
 
 // partials of V_{k0+k} . w for k < min(8, j + 1 - k0), to slot k0 + k
@@ -479,8 +501,8 @@ __global__ void __launch_bounds__(kRedBlock) k_gm_givens(const double* part, dou
 template <typename T>
 __global__ void k_gm_next(size_t n, const T* w, T* Vb, const double* sd, const int* si) {
     if (!gate_open(si, G_STEP)) return;
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) Vb[size_t(si[I_J]) * n + i] = T(sd[D_SCALE] * double(w[i]));
+    T* out = Vb + size_t(si[I_J]) * n;
+    GRID_STRIDE(i, n) out[i] = T(sd[D_SCALE] * double(w[i]));
 }
 
 // end of a cycle: y = H^-1 g by back substitution (every workgroup solves the
@@ -499,11 +521,11 @@ __global__ void k_gm_xupd(size_t n, const T* Vb, const T* dinv, T* x, const doub
         }
     }
     __syncthreads();
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double s = 0.0;
-    for (int k = 0; k <= jend; ++k) s += y[k] * double(Vb[size_t(k) * n + i]);
-    x[i] = T(double(x[i]) + double(dinv[i]) * s);
+    GRID_STRIDE(i, n) {
+        double s = 0.0;
+        for (int k = 0; k <= jend; ++k) s += y[k] * double(Vb[size_t(k) * n + i]);
+        x[i] = T(double(x[i]) + double(dinv[i]) * s);
+    }
 }
 
 // true residual norm after a cycle (slot 0): restart, or finish the (inner) solve
@@ -566,14 +588,16 @@ __global__ void k_bi_begin(double* sd, int* si) {
 template <typename V>
 __global__ void k_bi_p(size_t n, const V* r, const V* v, V* p, V* r0, const double* sd, const int* si) {
     if (!gate_open(si, G_BICG)) return;
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const V ri = r[i];
-    if (si[I_RESTART]) {
-        r0[i] = ri;
-        p[i] = V(double(ri) + sd[D_BBETA] * (0.0 - sd[D_OMEGA] * 0.0));
-    } else {
-        p[i] = V(double(ri) + sd[D_BBETA] * (double(p[i]) - sd[D_OMEGA] * double(v[i])));
+    const bool restart = si[I_RESTART] != 0;
+    const double beta = sd[D_BBETA], om = sd[D_OMEGA];
+    GRID_STRIDE(i, n) {
+        const V ri = r[i];
+        if (restart) {
+            r0[i] = ri;
+            p[i] = V(double(ri) + beta * (0.0 - om * 0.0));
+        } else {
+            p[i] = V(double(ri) + beta * (double(p[i]) - om * double(v[i])));
+        }
     }
 }
 
@@ -629,8 +653,8 @@ template <typename V>
 __global__ void k_bi_lin(size_t n, const V* x, const V* y, V* out, const double* sd, int which,
                          const int* si) {
     if (!gate_open(si, G_BICG)) return;
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = V(1.0 * double(x[i]) + (-sd[which]) * double(y[i]));
+    const double c = sd[which];
+    GRID_STRIDE(i, n) out[i] = V(1.0 * double(x[i]) + (-c) * double(y[i]));
 }
 
 // omega = (s . t) / (t . t) (0 when t . t <= 0)
@@ -679,6 +703,85 @@ __global__ void __launch_bounds__(kRedBlock) k_bi_end(const double* part, double
     si[I_IT] += 1;
     if (sd[D_OMEGA] == 0.0 || !(si[I_IT] < si[I_BUDGET] && rn > sd[D_TOLI])) finish_inner(sd, si, rn);
     set_vstop(si);
+}
+
+// ---------------------------------------------- CG, mixed (FP32 inner CG)
+// The inner CG runs on krylov.hip's device CG kernels and their flags
+// (flag[0] stop, [1] iterations, [2] converged, [3] limit; scal[0..4]).
+
+__global__ void k_cgm_stop(int* flag) {
+    if (threadIdx.x == 0) flag[0] = 1;
+}
+
+// start of an inner CG on A d = r64 / rn in FP32: r = r64 / rn, x = 0 and the
+// partials of r . r (slot 1; slot 0 gets r . z after the preconditioner)
+__global__ void __launch_bounds__(kRedBlock) k_cgm_tofloat(size_t n, const double* sd, const double* r64,
+                                                            float* r, float* x, double* part,
+                                                            const int* si) {
+    __shared__ double sh[kRedBlock];
+    if (!gate_open(si, G_ISTART)) return;
+    const double s = 1.0 / sd[D_RN];
+    double rr = 0.0;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+        const float v = float(s * r64[i]);
+        r[i] = v;
+        x[i] = 0.f;
+        rr += double(v) * double(v);
+    }
+    const double t = g_block_sum(rr, sh);
+    if (threadIdx.x == 0) part[kRedGrid + blockIdx.x] = t;
+}
+
+// p = z (the first direction) and the partials of r . z (slot 0)
+__global__ void __launch_bounds__(kRedBlock) k_cgm_pc(size_t n, const float* r, const float* z, float* p,
+                                                      double* part, const int* si) {
+    __shared__ double sh[kRedBlock];
+    if (!gate_open(si, G_ISTART)) return;
+    double rz = 0.0;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+        const float zi = z[i];
+        p[i] = zi;
+        rz += double(r[i]) * double(zi);
+    }
+    const double t = g_block_sum(rz, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// inner tolerance max(max(rtol, 1e-2), tol / (2 rn)) on its right-hand side
+// and its iteration budget; the inner CG's scalars and flags (k_cg_start)
+__global__ void __launch_bounds__(kRedBlock) k_cgm_start(const double* part, double* sd, int* si,
+                                                         double* scal, int* flag) {
+    __shared__ double sh[kRedBlock];
+    if (!gate_open(si, G_ISTART)) return;
+    const double rz = g_reduce(part, 0, sh);
+    const double rr = g_reduce(part, 1, sh);
+    if (threadIdx.x != 0) return;
+    const double inner = fmax(fmax(sd[D_RTOL], 1.0e-2), 0.5 * sd[D_TOLO] / sd[D_RN]);
+    const int budget = si[I_MAXIT] - si[I_USED];
+    const double tol2 = fmax(inner * inner * rr, 0.0);
+    scal[0] = rz;
+    scal[2] = tol2;
+    scal[3] = rr;
+    scal[4] = rr;
+    flag[0] = rr <= tol2 || budget <= 0 ? 1 : 0;
+    flag[1] = 0;
+    flag[2] = rr <= tol2 ? 1 : 0;
+    flag[3] = budget;
+    si[I_ISTART] = 0;
+    set_vstop(si);
+}
+
+// the inner CG has stopped: a refinement step with its iteration count
+__global__ void k_cgm_check(const int* flag, int* si) {
+    if (threadIdx.x != 0 || si[I_DONE] || si[I_IDONE] || si[I_ISTART] || !flag[0]) return;
+    si[I_IDONE] = 1;
+    si[I_IT] = flag[1];
+    set_vstop(si);
+}
+
+__global__ void k_bi_jacobi(size_t n, const float* dinv, const float* r, float* z, const int* si) {
+    if (!gate_open(si, G_ISTART)) return;
+    GRID_STRIDE(i, n) z[i] = dinv[i] * r[i];
 }
 
 // ------------------------------------------------------- iteration record
@@ -859,6 +962,8 @@ __global__ void k_g_state_copy(const unsigned long long* tab, int dir) {
 SellArgs sell_of(const DevMesh& m) { return {m.nn, m.slice_ptr.p, m.slice_w.p, m.sell_cols.p}; }
 
 inline unsigned nblk(size_t n) { return unsigned((n + 255) / 256); }
+// grid of a gated grid-stride launch
+inline unsigned gblk(size_t n) { return std::min(nblk(n), kGridCap); }
 
 bool is_mixed(const Flow& f) { return f.prm.precision == OAGPU_MIXED; }
 
@@ -883,10 +988,18 @@ void jacobi(Flow& f, int N) {
                                                           f.A1.p, f.dinv.p);
 }
 
-// one Arnoldi step of the device GMRES on vectors T and, in the same slot,
+// Slots that also carry the end-of-cycle and refinement launches: every
+// kBlockPeriod-th and the last.  Between them a finished cycle or inner solve
+// waits (its step launches are gated off), so where these blocks sit changes
+// no result, only how many launches a slot without them costs.
+constexpr int kBlockPeriod = 4;
+bool block_slot(int s, int nslots) { return s % kBlockPeriod == kBlockPeriod - 1 || s == nslots - 1; }
+
+// one Arnoldi step of the device GMRES on vectors T and, in block slots,
 // the end of a cycle and (mixed) a refinement step and the next inner start
 template <int N, typename T>
-void gm_slot(Flow& f, int s, int mres, const T* A, const T* b, const T* dinv, T* x, T* Vb, T* w, T* r) {
+void gm_slot(Flow& f, int s, int mres, bool blocks, const T* A, const T* b, const T* dinv, T* x, T* Vb,
+             T* w, T* r) {
     DevMesh& m = *f.m;
     IterGraph& g = f.ig;
     hipStream_t st = f.stream;
@@ -896,7 +1009,7 @@ void gm_slot(Flow& f, int s, int mres, const T* A, const T* b, const T* dinv, T*
     double* sd = g.sd.p;
     double* part = f.part.p;
     const int jmax = std::min(s, mres - 1);
-    k_gm_spmv<N, T><<<xcd_grid(size_t(m.nn), 256), 256, 0, st>>>(S, A, Vb, dinv, w, si);
+    k_gm_spmv<N, T><<<std::min(xcd_grid(size_t(m.nn), 256), kGridCap), 256, 0, st>>>(S, A, Vb, dinv, w, si);
     for (int k0 = 0; k0 <= jmax; k0 += 8)
         k_gm_dots<T><<<kRedGrid, kRedBlock, 0, st>>>(n, k0, Vb, w, part, si);
     k_gm_reduce<<<jmax + 1, kRedBlock, 0, st>>>(part, sd + D_H1, si);
@@ -904,22 +1017,23 @@ void gm_slot(Flow& f, int s, int mres, const T* A, const T* b, const T* dinv, T*
     k_gm_reduce<<<jmax + 1, kRedBlock, 0, st>>>(part, sd + D_H2, si);
     k_gm_upd_norm<T><<<kRedGrid, kRedBlock, 0, st>>>(n, Vb, sd + D_H2, w, part, si);
     k_gm_givens<<<1, kRedBlock, 0, st>>>(part, sd, si);
-    k_gm_next<T><<<nblk(n), 256, 0, st>>>(n, w, Vb, sd, si);
+    k_gm_next<T><<<gblk(n), 256, 0, st>>>(n, w, Vb, sd, si);
+    if (!blocks) return;
     // end of a cycle: least-squares update, true residual, restart or finish
-    k_gm_xupd<T><<<nblk(n), 256, 0, st>>>(n, Vb, dinv, x, sd, si);
+    k_gm_xupd<T><<<gblk(n), 256, 0, st>>>(n, Vb, dinv, x, sd, si);
     k_g_resid<N, T><<<kRedGrid, kRedBlock, 0, st>>>(S, A, x, b, r, part, si, G_END);
     k_gm_cycle<<<1, kRedBlock, 0, st>>>(part, sd, si);
-    k_gm_v0<T><<<nblk(n), 256, 0, st>>>(n, r, Vb, sd, si, 1);
+    k_gm_v0<T><<<gblk(n), 256, 0, st>>>(n, r, Vb, sd, si, 1);
     if constexpr (std::is_same_v<T, float>) {  // refinement step, next inner solve
         const double* A64 = N == 3 ? f.A3.p : f.A1.p;
         const double* b64 = N == 3 ? f.b3.p : f.b1.p;
         double* x64 = N == 3 ? f.x3.p : f.x1.p;
-        k_mx_refine<<<nblk(n), 256, 0, st>>>(n, sd, x, x64, si);
+        k_mx_refine<<<gblk(n), 256, 0, st>>>(n, sd, x, x64, si);
         k_g_resid<N, double><<<kRedGrid, kRedBlock, 0, st>>>(S, A64, x64, b64, f.r.p, part, si, G_REFINE);
         k_mx_decide<<<1, kRedBlock, 0, st>>>(part, sd, si);
         k_mx_tofloat<<<kRedGrid, kRedBlock, 0, st>>>(n, sd, f.r.p, f.fb.p, x, part, si);
         k_gm_start<<<1, kRedBlock, 0, st>>>(part, sd, si, G_ISTART);
-        k_gm_v0<T><<<nblk(n), 256, 0, st>>>(n, f.fb.p, Vb, sd, si, 2);
+        k_gm_v0<T><<<gblk(n), 256, 0, st>>>(n, f.fb.p, Vb, sd, si, 2);
     }
 }
 
@@ -947,10 +1061,10 @@ void enq_gmres(Flow& f, int max_it, double rtol, double atol, int restart, int c
     if (!mixed) {
         k_g_start_vec<double><<<kRedGrid, kRedBlock, 0, st>>>(n, b, x, nullptr, part, si);
         k_gm_start<<<1, kRedBlock, 0, st>>>(part, sd, si, G_LIVE);
-        k_gm_v0<double><<<nblk(n), 256, 0, st>>>(n, b, f.V.p, sd, si, 2);
+        k_gm_v0<double><<<gblk(n), 256, 0, st>>>(n, b, f.V.p, sd, si, 2);
         for (int s = 0; s < nslots; ++s) {
             if (poll && solve_done(f, false)) break;
-            gm_slot<N, double>(f, s, mres, A, b, f.dinv.p, x, f.V.p, f.w.p, f.r.p);
+            gm_slot<N, double>(f, s, mres, block_slot(s, nslots), A, b, f.dinv.p, x, f.V.p, f.w.p, f.r.p);
         }
     } else {
         k_g_to_float<<<nblk(nv), 256, 0, st>>>(nv, A, f.Af.p);
@@ -959,10 +1073,11 @@ void enq_gmres(Flow& f, int max_it, double rtol, double atol, int restart, int c
         k_mx_start<<<1, kRedBlock, 0, st>>>(part, sd, si);
         k_mx_tofloat<<<kRedGrid, kRedBlock, 0, st>>>(n, sd, f.r.p, f.fb.p, f.fx.p, part, si);
         k_gm_start<<<1, kRedBlock, 0, st>>>(part, sd, si, G_ISTART);
-        k_gm_v0<float><<<nblk(n), 256, 0, st>>>(n, f.fb.p, f.fV.p, sd, si, 2);
+        k_gm_v0<float><<<gblk(n), 256, 0, st>>>(n, f.fb.p, f.fV.p, sd, si, 2);
         for (int s = 0; s < nslots; ++s) {
             if (poll && solve_done(f, false)) break;
-            gm_slot<N, float>(f, s, mres, f.Af.p, f.fb.p, f.fdinv.p, f.fx.p, f.fV.p, f.fw.p, f.fr.p);
+            gm_slot<N, float>(f, s, mres, block_slot(s, nslots), f.Af.p, f.fb.p, f.fdinv.p, f.fx.p, f.fV.p,
+                              f.fw.p, f.fr.p);
         }
     }
     OAGPU_HIP(hipGetLastError());
@@ -977,7 +1092,7 @@ struct BiVecs {
 // smoothed-aggregation V-cycle) and, mixed, a refinement step and the next
 // inner start
 template <typename V>
-void bi_slot(Flow& f, const V* A, const V* dinv, V* x, const BiVecs<V>& w) {
+void bi_slot(Flow& f, bool blocks, const V* A, const V* dinv, V* x, const BiVecs<V>& w) {
     DevMesh& m = *f.m;
     IterGraph& g = f.ig;
     hipStream_t st = f.stream;
@@ -987,18 +1102,19 @@ void bi_slot(Flow& f, const V* A, const V* dinv, V* x, const BiVecs<V>& w) {
     double* sd = g.sd.p;
     double* part = f.part.p;
     k_bi_begin<<<1, 64, 0, st>>>(sd, si);
-    k_bi_p<V><<<nblk(n), 256, 0, st>>>(n, w.r, w.v, w.p, w.r0, sd, si);
+    k_bi_p<V><<<gblk(n), 256, 0, st>>>(n, w.r, w.v, w.p, w.r0, sd, si);
     sa_vcycle<V>(f, A, dinv, w.p, w.ph, false, si + I_VSTOP);
     k_bi_spmv_dots<V><<<kRedGrid, kRedBlock, 0, st>>>(S, A, w.ph, w.v, w.r0, nullptr, part, si);
     k_bi_alpha<<<1, kRedBlock, 0, st>>>(part, sd, si);
-    k_bi_lin<V><<<nblk(n), 256, 0, st>>>(n, w.r, w.v, w.s, sd, D_ALPHA, si);
+    k_bi_lin<V><<<gblk(n), 256, 0, st>>>(n, w.r, w.v, w.s, sd, D_ALPHA, si);
     sa_vcycle<V>(f, A, dinv, w.s, w.sh, false, si + I_VSTOP);
     k_bi_spmv_dots<V><<<kRedGrid, kRedBlock, 0, st>>>(S, A, w.sh, w.t, nullptr, w.s, part, si);
     k_bi_omega<<<1, kRedBlock, 0, st>>>(part, sd, si);
     k_bi_xr<V><<<kRedGrid, kRedBlock, 0, st>>>(n, w.ph, w.sh, w.s, w.t, w.r0, x, w.r, part, sd, si);
     k_bi_end<<<1, kRedBlock, 0, st>>>(part, sd, si);
+    if (!blocks) return;
     if constexpr (std::is_same_v<V, float>) {
-        k_mx_refine<<<nblk(n), 256, 0, st>>>(n, sd, x, f.x1.p, si);
+        k_mx_refine<<<gblk(n), 256, 0, st>>>(n, sd, x, f.x1.p, si);
         k_g_resid<1, double><<<kRedGrid, kRedBlock, 0, st>>>(S, f.A1.p, f.x1.p, f.b1.p, f.r.p, part, si,
                                                               G_REFINE);
         k_mx_decide<<<1, kRedBlock, 0, st>>>(part, sd, si);
@@ -1029,7 +1145,7 @@ void enq_bicgstab_sa(Flow& f, int max_it, double rtol, double atol, int cap, boo
         k_bi_start<<<1, kRedBlock, 0, st>>>(part, sd, si, G_LIVE);
         for (int s = 0; s < nslots; ++s) {
             if (poll && solve_done(f, false)) break;
-            bi_slot<double>(f, f.A1.p, f.dinv.p, f.x1.p, w);
+            bi_slot<double>(f, block_slot(s, nslots), f.A1.p, f.dinv.p, f.x1.p, w);
         }
     } else {
         k_g_to_float<<<nblk(m.nsell), 256, 0, st>>>(m.nsell, f.A1.p, f.Af.p);
@@ -1042,23 +1158,68 @@ void enq_bicgstab_sa(Flow& f, int max_it, double rtol, double atol, int cap, boo
         k_bi_start<<<1, kRedBlock, 0, st>>>(part, sd, si, G_ISTART);
         for (int s = 0; s < nslots; ++s) {
             if (poll && solve_done(f, false)) break;
-            bi_slot<float>(f, f.Af.p, f.fdinv.p, f.fx.p, w);
+            bi_slot<float>(f, block_slot(s, nslots), f.Af.p, f.fdinv.p, f.fx.p, w);
         }
     }
     OAGPU_HIP(hipGetLastError());
 }
 
+// the start of an inner FP32 CG (gated: a start is due)
+void cgm_inner_start(Flow& f, bool sa) {
+    IterGraph& g = f.ig;
+    hipStream_t st = f.stream;
+    const size_t n = size_t(f.m->nn);
+    k_cgm_tofloat<<<kRedGrid, kRedBlock, 0, st>>>(n, g.sd.p, f.r.p, f.fr.p, f.fx.p, f.part.p, g.si.p);
+    if (sa) {
+        sa_vcycle<float>(f, f.Af.p, f.fdinv.p, f.fr.p, f.fz.p, false, g.si.p + I_NSTART);
+    } else {  // no coarse level: Jacobi, z = dinv .* r
+        k_bi_jacobi<<<gblk(n), 256, 0, st>>>(n, f.fdinv.p, f.fr.p, f.fz.p, g.si.p);
+    }
+    k_cgm_pc<<<kRedGrid, kRedBlock, 0, st>>>(n, f.fr.p, f.fz.p, f.fp.p, f.part.p, g.si.p);
+    k_cgm_start<<<1, kRedBlock, 0, st>>>(f.part.p, g.sd.p, g.si.p, f.scal.p, f.flag.p);
+}
+
 // smoothed-aggregation AMG-CG (flexible CG, krylov.hip's CG kernels) on the
-// p' system, FP64
+// p' system; mixed: FP32 inner CG with the FP32 hierarchy under FP64
+// refinement, as krylov.hip's cg / cg_mixed
 void enq_cg_sa(Flow& f, int max_it, double rtol, double atol, int cap, bool poll) {
+    IterGraph& g = f.ig;
+    hipStream_t st = f.stream;
+    const size_t n = size_t(f.m->nn);
+    sa_build(f);  // once per flow, from its first p' system (never inside a capture: primed first)
     const bool sa = f.sa.lv.size() > 1;
-    jacobi(f, 1);
-    if (sa) sa_setup(f, f.A1.p, false);
-    cg_device_begin(f, max_it, rtol, atol, sa);
     const int nslots = poll ? max_it + 1 : cap;
+    jacobi(f, 1);
+    if (!is_mixed(f)) {
+        if (sa) sa_setup(f, f.A1.p, false);
+        cg_device_begin(f, max_it, rtol, atol, sa);
+        for (int s = 0; s < nslots; ++s) {
+            if (poll && solve_done(f, true)) break;
+            cg_device_iteration(f, sa);
+        }
+        return;
+    }
+    int* si = g.si.p;
+    double* sd = g.sd.p;
+    double* part = f.part.p;
+    k_g_to_float<<<nblk(f.m->nsell), 256, 0, st>>>(f.m->nsell, f.A1.p, f.Af.p);
+    k_g_to_float<<<nblk(n), 256, 0, st>>>(n, f.dinv.p, f.fdinv.p);
+    if (sa) sa_setup(f, f.A1.p, true);
+    k_g_solve_init<<<1, 1, 0, st>>>(si, sd, max_it, 0, 1, rtol, atol, 0.0);
+    k_cgm_stop<<<1, 1, 0, st>>>(f.flag.p);
+    k_g_start_vec<double><<<kRedGrid, kRedBlock, 0, st>>>(n, f.b1.p, f.x1.p, f.r.p, part, si);
+    k_mx_start<<<1, kRedBlock, 0, st>>>(part, sd, si);
+    cgm_inner_start(f, sa);
     for (int s = 0; s < nslots; ++s) {
-        if (poll && solve_done(f, true)) break;
-        cg_device_iteration(f, sa);
+        if (poll && solve_done(f, false)) break;
+        cg_device_iteration_f32(f, sa);
+        if (!block_slot(s, nslots)) continue;
+        k_cgm_check<<<1, 1, 0, st>>>(f.flag.p, si);
+        k_mx_refine<<<gblk(n), 256, 0, st>>>(n, sd, f.fx.p, f.x1.p, si);
+        k_g_resid<1, double><<<kRedGrid, kRedBlock, 0, st>>>(sell_of(*f.m), f.A1.p, f.x1.p, f.b1.p, f.r.p,
+                                                              part, si, G_REFINE);
+        k_mx_decide<<<1, kRedBlock, 0, st>>>(part, sd, si);
+        cgm_inner_start(f, sa);
     }
 }
 
@@ -1076,7 +1237,8 @@ void enq_solve(Flow& f, int which, int N, int method, int max_it, double rtol, i
         k_g_solve_record<<<1, 1, 0, f.stream>>>(g.si.p, g.sd.p, g.rec.p, which);
     } else {  // OAGPU_CG_SA
         enq_cg_sa(f, max_it, rtol, atol, cap, poll);
-        k_g_cg_record<<<1, 1, 0, f.stream>>>(f.flag.p, f.scal.p, g.rec.p, which);
+        if (is_mixed(f)) k_g_solve_record<<<1, 1, 0, f.stream>>>(g.si.p, g.sd.p, g.rec.p, which);
+        else k_g_cg_record<<<1, 1, 0, f.stream>>>(f.flag.p, f.scal.p, g.rec.p, which);
     }
     OAGPU_HIP(hipGetLastError());
 }
@@ -1171,8 +1333,6 @@ void prepare(Flow& f, const double* sctl) {
     const size_t nn = size_t(m.nn), n3 = 3 * nn;
     OAGPU_REQUIRE(f.prm.p_method == OAGPU_BICGSTAB_SA || f.prm.p_method == OAGPU_CG_SA,
                   "graph iterations need the smoothed-aggregation p' solver (OAGPU_BICGSTAB_SA or OAGPU_CG_SA)");
-    OAGPU_REQUIRE(!(f.prm.p_method == OAGPU_CG_SA && is_mixed(f)),
-                  "graph iterations: the mixed form's p' solver is OAGPU_BICGSTAB_SA");
     OAGPU_REQUIRE(!sctl || f.sst_on, "SST controls given but SST is not enabled on this flow");
     OAGPU_REQUIRE(f.tscheme == 0 || f.tdt > 0, "transient flow without a time step");
     if (g.sd.n < size_t(D_N)) g.sd.alloc(D_N);
@@ -1246,19 +1406,20 @@ std::vector<double> graph_key(const Flow& f, const double* ctl, const double* sc
     return k;
 }
 
+// capture the iteration (the flow's kernel clocks stay off inside it)
 void capture(Flow& f, const double* ctl, const double* sctl, const int* maxit) {
     IterGraph& g = f.ig;
     g.reset();
     const unsigned mask = f.prof_mask;
-    f.prof_mask = 0;  // no event records inside the graph
+    f.prof_mask = 0;
     hipGraph_t gr = nullptr;
     OAGPU_HIP(hipStreamBeginCapture(f.stream, hipStreamCaptureModeThreadLocal));
     try {
         enq_iteration(f, ctl, sctl, maxit, false);
     } catch (...) {
+        f.prof_mask = mask;
         (void)hipStreamEndCapture(f.stream, &gr);
         if (gr) (void)hipGraphDestroy(gr);
-        f.prof_mask = mask;
         throw;
     }
     f.prof_mask = mask;
@@ -1294,10 +1455,18 @@ int graph_iterations(Flow& f, const double* ctl, const double* sctl, int n, doub
             throw Error(OAGPU_EGEOM, "non-positive Jacobian determinant in a kernel");
         }
     };
+    // Krylov solves with a small iteration limit (the GMRES ones, 20) get it as
+    // their cap and never roll back (their slots are cheap when unused); the
+    // others (p', up to 200) get the largest count seen plus a quarter, at
+    // least 4 slots more: a roll-back costs about one iteration and a re-capture
+    auto cap_of = [&](int k, int count) {
+        if (maxit[k] <= 40) return std::max(1, maxit[k]);
+        return std::max(1, std::min(maxit[k], count + std::max(4, count / 4)));
+    };
     auto set_caps = [&](const int* want) {
         bool changed = false;
         for (int k = 0; k < 4; ++k) {
-            const int c = std::max(1, std::min(maxit[k], want[k]));
+            const int c = cap_of(k, want[k]);
             changed = changed || c != g.cap[k];
             g.cap[k] = c;
         }
@@ -1311,7 +1480,7 @@ int graph_iterations(Flow& f, const double* ctl, const double* sctl, int n, doub
             read_rec();
             if (!eager) {
                 int want[4];
-                for (int k = 0; k < 4; ++k) want[k] = int(rec[R_IT + k]) + 2;
+                for (int k = 0; k < 4; ++k) want[k] = int(rec[R_IT + k]);
                 set_caps(want);
                 g.primed = true;
             }
@@ -1328,7 +1497,7 @@ int graph_iterations(Flow& f, const double* ctl, const double* sctl, int n, doub
                 read_rec();
                 ++g.replays_redone;
                 int want[4];
-                for (int k = 0; k < 4; ++k) want[k] = std::max(g.cap[k], int(rec[R_IT + k]) + 2);
+                for (int k = 0; k < 4; ++k) want[k] = std::max(g.cap[k], int(rec[R_IT + k]));
                 set_caps(want);
             } else {
                 for (int k = 0; k < 4; ++k) g.seen[k] = std::max(g.seen[k], int(rec[R_IT + k]));
@@ -1336,8 +1505,8 @@ int graph_iterations(Flow& f, const double* ctl, const double* sctl, int n, doub
                     int want[4];
                     bool shrink = false;
                     for (int k = 0; k < 4; ++k) {
-                        want[k] = g.seen[k] + 2;
-                        shrink = shrink || want[k] + 2 < g.cap[k];
+                        want[k] = g.seen[k];
+                        shrink = shrink || cap_of(k, want[k]) + 4 < g.cap[k];
                     }
                     if (shrink) set_caps(want);
                     else {

@@ -717,6 +717,13 @@ void cg_device_iteration(Flow& f, bool sa) {
     OAGPU_HIP(hipGetLastError());
 }
 
+// one iteration of the FP32 inner CG of the mixed form (FP32 values, the
+// FP32 hierarchy's V-cycle or Jacobi)
+void cg_device_iteration_f32(Flow& f, bool sa) {
+    cg_iteration<1, float>(f, f.Af.p, f.fdinv.p, f.fx.p, f.fr.p, f.fz.p, f.fp.p, f.fq.p, sa, sa);
+    OAGPU_HIP(hipGetLastError());
+}
+
 // sa: the smoothed-aggregation V-cycle (OAGPU_CG_SA), else the unsmoothed
 // hierarchy's K-cycle (OAGPU_CG_AMG)
 template <int N>

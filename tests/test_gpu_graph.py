@@ -5,7 +5,7 @@
   iterations, the residual history and the Krylov iteration counts -- on a
   wall-resolved step slice with k-omega SST (smoothed-aggregation
   AMG-BiCGStab p', FP64 and mixed) and on a perturbed cavity (smoothed-
-  aggregation AMG-CG p', FP64); replays run with the program's own
+  aggregation AMG-CG p', FP64 and mixed); replays run with the program's own
   synchronisations and downloads between them.
 * An iteration whose solves need more slots than the graph has is rolled
   back and run uncapped: forced to one slot per solve, the results are still
@@ -62,10 +62,10 @@ def step_flow(oa, meshgen, m, precision):
     return f, dm
 
 
-def cavity_flow(oa, meshgen, n=14):
+def cavity_flow(oa, meshgen, precision, n=14):
     m = meshgen.cavity_hex(n, perturb=0.15, seed=3)
     dm = oa.DeviceMesh(m)
-    f = oa.FlowCase(dm, oa.default_params(oa.OAGPU_FP64, p_method=oa.OAGPU_CG_SA),
+    f = oa.FlowCase(dm, oa.default_params(precision, p_method=oa.OAGPU_CG_SA),
                     np.array([[1.0, 0, 0], [0, 0, 0]]))
     f.upload("U", meshgen.initial_velocity(dm.nn, 5))
     f.initialize()
@@ -98,10 +98,12 @@ def test_step_graph_equals_eager(oa, meshgen, step_mesh, precision):
     assert_bitwise(fields(fe, FIELDS), fields(fg, FIELDS))
 
 
-def test_cavity_cg_sa_graph_equals_eager(oa, meshgen):
-    print("[case] perturbed 14^3 cavity, CG-SA, fp64", flush=True)
-    fe, _ = cavity_flow(oa, meshgen)
-    fg, _ = cavity_flow(oa, meshgen)
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cavity_cg_sa_graph_equals_eager(oa, meshgen, precision):
+    prec = oa.OAGPU_FP64 if precision == "fp64" else oa.OAGPU_MIXED
+    print(f"[case] perturbed 14^3 cavity, CG-SA, {precision}", flush=True)
+    fe, _ = cavity_flow(oa, meshgen, prec)
+    fg, _ = cavity_flow(oa, meshgen, prec)
     he = fe.graph_iterations(CTL, None, n=6, eager=True)
     hg = np.vstack([fg.graph_iterations(CTL, None, n=3), fg.graph_iterations(CTL, None, n=3)])
     st = fg.graph_stats()
@@ -143,6 +145,23 @@ def test_device_solves_match_host_solves(oa, meshgen, step_mesh, precision):
     assert np.array_equal(hh[:, [4, 5, 8, 9]], hg[:, [4, 5, 8, 9]])
     np.testing.assert_allclose(hg[:, [0, 1, 2, 3, 6, 7]], hh[:, [0, 1, 2, 3, 6, 7]], rtol=1e-9, atol=0)
     a, b = fields(fh, FIELDS), fields(fg, FIELDS)
+    for k in a:
+        err = np.abs(a[k] - b[k]).max() / np.abs(a[k]).max()
+        assert err < 1e-9, f"{k}: {err:.3e}"
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cavity_device_cg_matches_host_cg(oa, meshgen, precision):
+    prec = oa.OAGPU_FP64 if precision == "fp64" else oa.OAGPU_MIXED
+    print(f"[case] perturbed 14^3 cavity, CG-SA, {precision}: graph vs oagpu_outer_iteration", flush=True)
+    fh, _ = cavity_flow(oa, meshgen, prec)
+    fg, _ = cavity_flow(oa, meshgen, prec)
+    hh = np.array([fh.outer_iteration(CTL) for _ in range(3)])
+    hg = fg.graph_iterations(CTL, None, n=3)
+    print(f"[iters] host {hh[:, 4:6].tolist()} graph {hg[:, 4:6].tolist()}", flush=True)
+    assert np.array_equal(hh[:, 4:6], hg[:, 4:6])
+    np.testing.assert_allclose(hg[:, :4], hh[:, :4], rtol=1e-9, atol=0)
+    a, b = fields(fh, ("U", "p", "mdot")), fields(fg, ("U", "p", "mdot"))
     for k in a:
         err = np.abs(a[k] - b[k]).max() / np.abs(a[k]).max()
         assert err < 1e-9, f"{k}: {err:.3e}"
