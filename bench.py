@@ -43,20 +43,24 @@ CTL = [20, 0.1, 200, 1.0e-6]   # momentum GMRES max_it/rtol, p' max_it/rtol
 
 
 # ------------------------------------------------------------------ bytes
-def algorithmic_bytes(nn, ne, nnz, value_bytes, vector_bytes, high_res=False, sa=None):
+def algorithmic_bytes(nn, ne, nnz, value_bytes, vector_bytes, high_res=False, sa=None, scalar_eqs=1):
     """Compulsory HBM bytes of one launch of each clocked kernel on a hex8
     mesh: each array read once, each output written once, FP64 fields, int32
     indices.  value_bytes / vector_bytes: the AMG fine level's matrix values
     and vectors (FP64 form 8 / 8; the mixed form's FP32 inner solves 4 / 4).
     sa = (prolongation entries of the fine level, rows of level 1): the
-    smoothed-aggregation cycle, whose fine-level post-smoothing reads P."""
+    smoothed-aggregation cycle, whose fine-level post-smoothing reads P.
+    scalar_eqs: equations of one k / omega launch (2: k and omega assembled
+    together, the mesh and shared fields read once, phi and the system per
+    equation)."""
     conn, mdot, slots = 4 * 8, 8 * 12, 4 * 32
     elem = ne * (conn + mdot + slots)
     hr = nn * (72 + 24) if high_res else 0
     mom = (nn * (24 + 24 + 8) + elem + nnz * 72 + nn * 24    # x, U, mu_eff; A, b
            + nn * (8 + 8 + 24) + nn * 24 + hr)                # V, div, grad p node terms; du
     pc = nn * (24 + 24 + 8 + 24 + 24 + 8) + elem + nnz * 8 + nn * 8
-    sc = nn * (24 + 8 + 8 + 8 + 8 + 8) + elem + nnz * 8 + nn * 8   # x, V, div, phi, mu_t, F1
+    sc = (nn * (24 + 8 + 8 + 8 + 8) + elem                          # x, V, div, mu_t, F1
+          + scalar_eqs * (nn * 8 + nnz * 8 + nn * 8))               # phi; A, b per equation
     md = nn * 112 + ne * (conn + 16 * 12)
     gu = nn * (24 + 24 + 8 + 72 + 72) + ne * conn           # x, U, V, grad U in and out
     gp = nn * (24 + 8 + 8 + 24 + 24) + ne * conn
@@ -480,8 +484,8 @@ def bench_bfs(args):
     rmx, r64 = res[oa.OAGPU_MIXED], res[oa.OAGPU_FP64]
     lv = rmx["sa_levels"]
     sa = (lv[0][1], lv[1][0]) if len(lv) > 1 else None
-    ab_mx = algorithmic_bytes(nn, ne, nnz, 4, 4, sa=sa)
-    ab_64 = algorithmic_bytes(nn, ne, nnz, 8, 8, sa=sa)
+    ab_mx = algorithmic_bytes(nn, ne, nnz, 4, 4, sa=sa, scalar_eqs=2)
+    ab_64 = algorithmic_bytes(nn, ne, nnz, 8, 8, sa=sa, scalar_eqs=2)
     it64, itmx = args.steps / r64["seconds"], args.steps / rmx["seconds"]
     print("[bench] coefficient error of the mixed form", file=sys.stderr, flush=True)
     cerr = coefficient_error(oa, dm, make_flow, lambda f: bfs_setup(oa, mesh, f))

@@ -785,11 +785,28 @@ int oagpu_sst_iteration(oagpu_flow* h, const double* ctl, double* hist) {
         oagpu::launch_sst_update(f);
         oagpu_control_data cdk{}, cdw{};
         double rk, rw;
-        oagpu::launch_assemble_scalar(f, 0);
+        const bool pair = oagpu::scalar_pair_ok(f);
+        if (pair) oagpu::launch_assemble_scalar_pair(f);  // omega's system to A1w / b1w
+        else oagpu::launch_assemble_scalar(f, 0);
         oagpu::scaled_residual(f, 1, &rk);
         oagpu::krylov_solve(f, 1, OAGPU_GMRES, int(ctl[0]), ctl[1], 1e-30, 30, &cdk);
         oagpu::launch_correct_scalar(f, 0);
-        oagpu::launch_assemble_scalar(f, 1);
+        if (pair) {  // the omega system in place of the k one while it is solved
+            std::swap(f.A1, f.A1w);
+            std::swap(f.b1, f.b1w);
+        } else {
+            oagpu::launch_assemble_scalar(f, 1);
+        }
+        struct SwapBack {
+            oagpu::Flow& f;
+            bool on;
+            ~SwapBack() {
+                if (on) {
+                    std::swap(f.A1, f.A1w);
+                    std::swap(f.b1, f.b1w);
+                }
+            }
+        } back{f, pair};
         oagpu::scaled_residual(f, 1, &rw);
         oagpu::krylov_solve(f, 1, OAGPU_GMRES, int(ctl[0]), ctl[1], 1e-30, 30, &cdw);
         oagpu::launch_correct_scalar(f, 1);
@@ -1147,6 +1164,9 @@ int oagpu_tune_set(oagpu_flow* h, const char* which, int variant) {
             OAGPU_REQUIRE(variant == 0 || variant == 8 || variant == 16,
                           "occupancy cap: 0, 8 or 16 waves per CU");
             (which[0] == 's' ? t.cap_scalar : which[0] == 'm' ? t.cap_mdot : t.cap_grad) = variant;
+        } else if (!std::strcmp(which, "scalar_pair")) {
+            OAGPU_REQUIRE(variant == 0 || variant == 1, "scalar_pair: 0 or 1");
+            t.scalar_pair = variant;
         } else if (!std::strcmp(which, "graph_cap")) {
             // iteration slots of every solve in the next captured iteration graph
             // (a solve that needs more rolls its iteration back and raises them)

@@ -199,6 +199,9 @@ struct Flow {
         // reserving LDS per workgroup: a smaller window of rows in flight per
         // XCD keeps more of their gathered node and element data in its L2
         int cap_scalar = 0, cap_mdot = 0, cap_grad = 0;
+        // k and omega assembled by one launch sharing the geometry (1), or
+        // by one launch each (0)
+        int scalar_pair = 1;
         bool tuned = false;
     } tune;
     // device time of the clocked kernels (profiling on only)
@@ -236,6 +239,7 @@ struct Flow {
     bool sst_on = false;
     oagpu_sst_params sst{};
     DBuf<double> k, om, gradK, gradW, ymin, mut, F1, Pk, Pk_wall, kprev;  // om: omega
+    DBuf<double> A1w, b1w;  // the omega system while k is solved (k and omega assembled together)
     double k_scale = 1.0, w_scale = 1.0;
     // transient (oagpu_time_step): scheme 0 steady, 1 BDF1, 2 BDF2; the time
     // step and the previous one (0 on the first step); U, k and omega of the
@@ -316,6 +320,10 @@ int krylov_solve(Flow& f, int N, int method, int max_it, double rtol, double ato
 // k-omega SST (sst.hip, rows.hip)
 void launch_sst_update(Flow& f);               // mu_t, mu_eff, F1, P_k
 void launch_assemble_scalar(Flow& f, int which);  // 0 k, 1 omega -> system N = 1
+// k -> system N = 1 and omega -> A1w / b1w in one launch (geometry shared);
+// possible when both use the same gradient and interpolation operators
+bool scalar_pair_ok(const Flow& f);
+void launch_assemble_scalar_pair(Flow& f);
 void launch_correct_scalar(Flow& f, int which);   // phi += x, clipped; field scale
 // topo.hip: master-element geometry of a batch of elements of one topology
 // (0 hex8, 1 tet4, 2 wedge6, 3 pyramid5), device arrays

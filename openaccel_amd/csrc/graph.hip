@@ -1312,11 +1312,28 @@ void enq_iteration(Flow& f, const double* ctl, const double* sctl, const int* ma
     if (sctl) {
         k_g_copy<<<nblk(nn), 256, 0, st>>>(nn, f.k.p, f.kprev.p);
         launch_sst_update(f);
-        launch_assemble_scalar(f, 0);
+        const bool pair = scalar_pair_ok(f);
+        if (pair) launch_assemble_scalar_pair(f);  // omega's system to A1w / b1w
+        else launch_assemble_scalar(f, 0);
         enq_scaled_res(f, 1, R_RES + 4);
         enq_solve(f, 2, 1, OAGPU_GMRES, maxit[2], sctl[1], 30, g.cap[2], poll);
         enq_correct_scalar(f, 0);
-        launch_assemble_scalar(f, 1);
+        if (pair) {  // the omega system in place of the k one while it is solved
+            std::swap(f.A1, f.A1w);
+            std::swap(f.b1, f.b1w);
+        } else {
+            launch_assemble_scalar(f, 1);
+        }
+        struct SwapBack {
+            Flow& f;
+            bool on;
+            ~SwapBack() {
+                if (on) {
+                    std::swap(f.A1, f.A1w);
+                    std::swap(f.b1, f.b1w);
+                }
+            }
+        } back{f, pair};
         enq_scaled_res(f, 1, R_RES + 5);
         enq_solve(f, 3, 1, OAGPU_GMRES, maxit[3], sctl[1], 30, g.cap[3], poll);
         enq_correct_scalar(f, 1);
@@ -1386,7 +1403,7 @@ std::vector<double> graph_key(const Flow& f, const double* ctl, const double* sc
     const Flow::Tuning& t = f.tune;
     for (double v : {double(f.prm.p_method), double(f.prm.precision), double(f.prm.high_res),
                      double(f.has_outlet), double(t.mom_tile), double(t.pc_occupancy),
-                     double(t.cap_scalar), double(t.cap_mdot), double(t.cap_grad),
+                     double(t.cap_scalar), double(t.cap_mdot), double(t.cap_grad), double(t.scalar_pair),
                      double(t.spmv_block[0][0]), double(t.spmv_block[0][1]),
                      double(t.spmv_block[1][0]), double(t.spmv_block[1][1]),
                      double(f.sa.lv.size()), double(f.tscheme), f.tdt, f.tdt_prev,
@@ -1401,7 +1418,8 @@ std::vector<double> graph_key(const Flow& f, const double* ctl, const double* sc
                           (const void*)f.fz.p, (const void*)f.fp.p, (const void*)f.fq.p, (const void*)f.fr0.p,
                           (const void*)f.fs.p, (const void*)f.ft.p, (const void*)f.fsh.p, (const void*)f.dinv.p,
                           (const void*)f.ig.snap.p, (const void*)f.ig.sd.p, (const void*)f.ig.si.p,
-                          (const void*)f.ig.rec.p, (const void*)f.ig.sc.p})
+                          (const void*)f.ig.rec.p, (const void*)f.ig.sc.p, (const void*)f.A1.p,
+                          (const void*)f.A1w.p, (const void*)f.b1.p, (const void*)f.b1w.p})
         k.push_back(double(uintptr_t(q)));
     return k;
 }

@@ -1644,76 +1644,98 @@ __device__ void sst_node_terms(const ScArgs& a, int i, double& lhs, double& rhs)
 // explicit diffusion flux uses grad phi at the ip; the lumped columns take
 // minus the kept columns' coefficients (the shape-function gradients sum to
 // zero).  Nodal values: in registers, rotated with the coordinates (REG),
-// or read per ip from L1/L2.
-template <typename T, int SG, bool HR, bool REG>
-__device__ __forceinline__ void sc_scs(const ScArgs& a, int r, int e, const int nd[8],
-                                       const T c[8][3], const T ph[8][1], const T gm[8][1],
-                                       uint64_t sl, const SellRow& sr, int dof, double& D,
-                                       double& R, int& bad, double* lt) {
+// or read per ip from L1/L2.  NE equations (k, or k and omega) share the
+// ip's geometry and mass flow; each keeps its own arithmetic, term for term
+// that of a single equation.
+template <typename T, int SG, bool HR, bool REG, int NE>
+__device__ __forceinline__ void sc_scs(const ScArgs* as, int r, int e, const int nd[8],
+                                       const T c[8][3], const T (*ph)[8][1], const T (*gm)[8][1],
+                                       uint64_t sl, const SellRow& sr, int dof, double* D,
+                                       double* R, int& bad, double* const* lt) {
     constexpr int IP = 0;
+    const ScArgs& a = as[0];
     const T m = T(c_rot.sg[r]) * T(a.mdot[12 * size_t(e) + c_rot.ip[r]]);
     T A[3], Ji[9];
     area_c(IP, c, A);
     jinv_c<SG>(IP, c, Ji, bad);
-    T Gip = T(0), P[3] = {0, 0, 0};
+    T Gip[NE], P[NE][3];
+#pragma unroll
+    for (int q = 0; q < NE; ++q) Gip[q] = P[q][0] = P[q][1] = P[q][2] = T(0);
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
-        T pv, gv;
-        if constexpr (REG) {
-            pv = ph[n][0];
-            gv = gm[n][0];
-        } else {
-            pv = T(a.phi[nd[n]]);
-            const double F1 = a.F1[nd[n]];
-            gv = T(a.mu + a.mut[nd[n]] * (F1 * a.sig1 + (1.0 - F1) * a.sig2));
+        double F1 = 0.0, mt = 0.0;
+        if constexpr (!REG) {
+            F1 = a.F1[nd[n]];
+            mt = a.mut[nd[n]];
         }
-        Gip += (a.us ? T(kHex.N[1][IP][n]) : T(kHex.N[0][IP][n])) * gv;
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-            if (kHex.dn[SG][IP][n][d] != 0.0) P[d] += T(kHex.dn[SG][IP][n][d]) * pv;
+        for (int q = 0; q < NE; ++q) {
+            T pv, gv;
+            if constexpr (REG) {
+                pv = ph[q][n][0];
+                gv = gm[q][n][0];
+            } else {
+                pv = T(as[q].phi[nd[n]]);
+                gv = T(a.mu + mt * (F1 * as[q].sig1 + (1.0 - F1) * as[q].sig2));
+            }
+            Gip[q] += (a.us ? T(kHex.N[1][IP][n]) : T(kHex.N[0][IP][n])) * gv;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                if (kHex.dn[SG][IP][n][d] != 0.0) P[q][d] += T(kHex.dn[SG][IP][n][d]) * pv;
+        }
     }
     // upwind advection plus the limited deferred correction
     // (phiAssemblerElemTerms.hpp:291-340)
     const bool fw = m > T(0);
     const size_t nu = size_t(fw ? nd[0] : nd[1]);
-    T up = T(a.phi[nu]);
-    if constexpr (HR) {
-        T dc = T(0);
+    T rl[NE];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            T xi = T(0);
+    for (int q = 0; q < NE; ++q) {
+        T up = T(as[q].phi[nu]);
+        if constexpr (HR) {
+            T dc = T(0);
 #pragma unroll
-            for (int n = 0; n < 8; ++n) xi += T(kHex.N[0][IP][n]) * c[n][d];
-            dc += T(a.beta[nu]) * (xi - (fw ? c[0][d] : c[1][d])) * T(a.gphi[3 * nu + d]);
+            for (int d = 0; d < 3; ++d) {
+                T xi = T(0);
+#pragma unroll
+                for (int n = 0; n < 8; ++n) xi += T(kHex.N[0][IP][n]) * c[n][d];
+                dc += T(as[q].beta[nu]) * (xi - (fw ? c[0][d] : c[1][d])) * T(as[q].gphi[3 * nu + d]);
+            }
+            up += dc;
         }
-        up += dc;
-    }
-    T rl = -(m * up);
-    // explicit diffusion flux: + Gamma grad phi . A
-    {
+        rl[q] = -(m * up);
+        // explicit diffusion flux: + Gamma grad phi . A
         T f = T(0);
 #pragma unroll
-        for (int j = 0; j < 3; ++j) f += (P[0] * Ji[j] + P[1] * Ji[3 + j] + P[2] * Ji[6 + j]) * A[j];
-        rl += Gip * f;
+        for (int j = 0; j < 3; ++j) f += (P[q][0] * Ji[j] + P[q][1] * Ji[3 + j] + P[q][2] * Ji[6 + j]) * A[j];
+        rl[q] += Gip[q] * f;
     }
-    T lD = T(0);
+    T lD[NE];
+#pragma unroll
+    for (int q = 0; q < NE; ++q) lD[q] = T(0);
+    T lf[NE];
     auto column = [&](int k) {
         T g[3];
         gradN_c<SG>(IP, k, Ji, g);
-        T lf = T(0);
 #pragma unroll
-        for (int j = 0; j < 3; ++j) lf += -Gip * g[j] * A[j];
-        lD -= lf;
-        return lf;
+        for (int q = 0; q < NE; ++q) {
+            lf[q] = T(0);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) lf[q] += -Gip[q] * g[j] * A[j];
+            lD[q] -= lf[q];
+        }
     };
 #pragma unroll
     for (int k = 1; k < 5; ++k) {  // the row node's SCS neighbours 1 (across ip 0), 3, 4
         if (k == 2) continue;
-        T lf = column(k);
-        if (k == 1) lf += T(0.5) * (m - fabs(m));
+        column(k);
         const int sp = slot_of(sl, c_rot.perm[r][k]);
-        if (sp == 0xFF || sp == dof) D += double(lf);
-        else scalar_scatter(a.vals, sr, lt, sp, double(lf));
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+            if (k == 1) lf[q] += T(0.5) * (m - fabs(m));
+            if (sp == 0xFF || sp == dof) D[q] += double(lf[q]);
+            else scalar_scatter(as[q].vals, sr, lt[q], sp, double(lf[q]));
+        }
     }
 #pragma unroll
     for (int p = 0; p < 4; ++p) {  // kept only by full-stencil rows
@@ -1722,43 +1744,57 @@ __device__ __forceinline__ void sc_scs(const ScArgs& a, int r, int e, const int 
             continue;  // zero coefficient at this ip
         const int sp = slot_of(sl, c_rot.perm[r][k]);
         if (sp == 0xFF) continue;
-        const T lf = column(k);
-        if (sp == dof) D += double(lf);
-        else scalar_scatter(a.vals, sr, lt, sp, double(lf));
+        column(k);
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+            if (sp == dof) D[q] += double(lf[q]);
+            else scalar_scatter(as[q].vals, sr, lt[q], sp, double(lf[q]));
+        }
     }
-    D += double(lD + T(0.5) * (m + fabs(m)));
-    R += double(rl);
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+        D[q] += double(lD[q] + T(0.5) * (m + fabs(m)));
+        R[q] += double(rl[q]);
+    }
 }
 
-// element e's contribution to the k / omega row of its local node la:
+// element e's contribution to the k / omega row(s) of its local node la:
 // gathered once, relabelled so that la is node 0, its three SCS ips touching
 // la added as ip 0 of the three relabellings
-template <typename T, int SG, bool HR>
-__device__ __forceinline__ void sc_incidence(const ScArgs& a, int la, int e, uint64_t sl,
+template <typename T, int SG, bool HR, int NE>
+__device__ __forceinline__ void sc_incidence(const ScArgs* as, int la, int e, uint64_t sl,
                                              const SellRow& sr, int dof, const double xr[3],
-                                             double& D, double& R, int& bad, double* lt) {
+                                             double* D, double* R, int& bad, double* const* lt) {
     constexpr bool REG = sizeof(T) == 4;
+    const ScArgs& a = as[0];
     const int* ce = a.conn + 8 * size_t(e);
     int nd[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) nd[k] = ce[c_rot.perm[3 * la][k]];
-    T c[8][3], ph[REG ? 8 : 1][1], gm[REG ? 8 : 1][1];
+    T c[8][3], ph[NE][REG ? 8 : 1][1], gm[NE][REG ? 8 : 1][1];
     gather_rel(a.xyz, nd, xr, c);
     if constexpr (REG) {
 #pragma unroll
         for (int n = 0; n < 8; ++n) {
-            ph[n][0] = T(a.phi[nd[n]]);
-            const double F1 = a.F1[nd[n]];
-            gm[n][0] = T(a.mu + a.mut[nd[n]] * (F1 * a.sig1 + (1.0 - F1) * a.sig2));
+            const double F1 = a.F1[nd[n]], mt = a.mut[nd[n]];
+#pragma unroll
+            for (int q = 0; q < NE; ++q) {
+                ph[q][n][0] = T(as[q].phi[nd[n]]);
+                gm[q][n][0] = T(a.mu + mt * (F1 * as[q].sig1 + (1.0 - F1) * as[q].sig2));
+            }
         }
     }
 #pragma unroll 1
-    for (int q = 0; q < 3; ++q) {
-        sc_scs<T, SG, HR, REG>(a, 3 * la + q, e, nd, c, ph, gm, sl, sr, dof, D, R, bad, lt);
+    for (int q3 = 0; q3 < 3; ++q3) {
+        sc_scs<T, SG, HR, REG, NE>(as, 3 * la + q3, e, nd, c, reinterpret_cast<const T(*)[8][1]>(ph),
+                                   reinterpret_cast<const T(*)[8][1]>(gm), sl, sr, dof, D, R, bad, lt);
         rotate_nodes<T, 3>(c);
         if constexpr (REG) {
-            rotate_nodes<T, 1>(ph);
-            rotate_nodes<T, 1>(gm);
+#pragma unroll
+            for (int q = 0; q < NE; ++q) {
+                rotate_nodes<T, 1>(ph[q]);
+                rotate_nodes<T, 1>(gm[q]);
+            }
         }
         int t[8];
 #pragma unroll
@@ -1769,23 +1805,27 @@ __device__ __forceinline__ void sc_incidence(const ScArgs& a, int la, int e, uin
 }
 
 // the SCS ips of a tet4 / wedge6 / pyramid5 element touching the k / omega
-// row of its local node la (phiAssemblerElemTerms.hpp:171-406 per ip), from
-// the row's side: upwind advection with the limited deferred correction,
-// diffusion -Gamma_ip grad N . A per column with its explicit flux on the rhs
-template <typename T, int TOPO, bool HR>
-__device__ void sc_generic(const ScArgs& a, int la, int e, uint64_t sl, const SellRow& sr, int dof,
-                           const double xr[3], int sg_shift, double& D, double& R, int& bad,
-                           double* lt) {
+// row(s) of its local node la (phiAssemblerElemTerms.hpp:171-406 per ip),
+// from the row's side: upwind advection with the limited deferred
+// correction, diffusion -Gamma_ip grad N . A per column with its explicit
+// flux on the rhs; NE equations share the ip's geometry and mass flow
+template <typename T, int TOPO, bool HR, int NE>
+__device__ void sc_generic(const ScArgs* as, int la, int e, uint64_t sl, const SellRow& sr, int dof,
+                           const double xr[3], int sg_shift, double* D, double* R, int& bad,
+                           double* const* lt) {
     constexpr int npe = topo::Traits<TOPO>::npe;
+    const ScArgs& a = as[0];
     const int* ce = a.conn + 8 * size_t(e);
     const double* geo = a.geo + a.geo_off[e];
-    T c[npe][3], ph[npe], gm[npe];
+    T c[npe][3], ph[NE][npe], gm[NE][npe];
     for (int k = 0; k < npe; ++k) {
         if constexpr (HR)
             for (int d = 0; d < 3; ++d) c[k][d] = T(a.xyz[3 * size_t(ce[k]) + d] - xr[d]);
-        ph[k] = T(a.phi[ce[k]]);
-        const double F1 = a.F1[ce[k]];
-        gm[k] = T(a.mu + a.mut[ce[k]] * (F1 * a.sig1 + (1.0 - F1) * a.sig2));
+        const double F1 = a.F1[ce[k]], mt = a.mut[ce[k]];
+        for (int q = 0; q < NE; ++q) {
+            ph[q][k] = T(as[q].phi[ce[k]]);
+            gm[q][k] = T(a.mu + mt * (F1 * as[q].sig1 + (1.0 - F1) * as[q].sig2));
+        }
     }
 #pragma unroll 1
     for (int ip = 0; ip < topo::Traits<TOPO>::nscs; ++ip) {
@@ -1798,35 +1838,37 @@ __device__ void sc_generic(const ScArgs& a, int la, int e, uint64_t sl, const Se
         topo::ip_geom<TOPO>(geo, ip, sg_shift, a.us, A, g, Nc, Nu);
         for (int d = 0; d < 3; ++d) A[d] *= sgn;
         const T m = sgn * T(a.mdot[12 * size_t(e) + ip]);
-        T Gip = T(0);
-        for (int k = 0; k < npe; ++k) Gip += Nu[k] * gm[k];
         const int ul = m > T(0) ? la : o;
-        T up = topo::pick<npe>(ph, ul);
-        if constexpr (HR) {
-            const size_t nu = size_t(ce[ul]);
-            T dc = T(0), cu[3];
-            topo::pick_row<npe>(c, ul, cu);
-            for (int d = 0; d < 3; ++d) {
-                T xi = T(0);
-                for (int k = 0; k < npe; ++k) xi += Nc[k] * c[k][d];
-                dc += T(a.beta[nu]) * (xi - cu[d]) * T(a.gphi[3 * nu + d]);
-            }
-            up += dc;
-        }
         const T aS = T(0.5) * (m + fabs(m)), aO = T(0.5) * (m - fabs(m));
-        T rl = -(m * up), lD = aS;
+        for (int q = 0; q < NE; ++q) {
+            T Gip = T(0);
+            for (int k = 0; k < npe; ++k) Gip += Nu[k] * gm[q][k];
+            T up = topo::pick<npe>(ph[q], ul);
+            if constexpr (HR) {
+                const size_t nu = size_t(ce[ul]);
+                T dc = T(0), cu[3];
+                topo::pick_row<npe>(c, ul, cu);
+                for (int d = 0; d < 3; ++d) {
+                    T xi = T(0);
+                    for (int k = 0; k < npe; ++k) xi += Nc[k] * c[k][d];
+                    dc += T(as[q].beta[nu]) * (xi - cu[d]) * T(as[q].gphi[3 * nu + d]);
+                }
+                up += dc;
+            }
+            T rl = -(m * up), lD = aS;
 #pragma unroll
-        for (int ic = 0; ic < npe; ++ic) {
-            T lf = T(0);
-            for (int j = 0; j < 3; ++j) lf += -Gip * g[ic][j] * A[j];
-            rl -= lf * ph[ic];
-            if (ic == o) lf += aO;
-            const int s = slot_of(sl, ic);
-            if (s == 0xFF || s == dof) lD += lf;
-            else scalar_scatter(a.vals, sr, lt, s, double(lf));
+            for (int ic = 0; ic < npe; ++ic) {
+                T lf = T(0);
+                for (int j = 0; j < 3; ++j) lf += -Gip * g[ic][j] * A[j];
+                rl -= lf * ph[q][ic];
+                if (ic == o) lf += aO;
+                const int s = slot_of(sl, ic);
+                if (s == 0xFF || s == dof) lD += lf;
+                else scalar_scatter(as[q].vals, sr, lt[q], s, double(lf));
+            }
+            D[q] += double(lD);
+            R[q] += double(rl);
         }
-        D += double(lD);
-        R += double(rl);
     }
 }
 
@@ -1908,14 +1950,30 @@ __global__ void __launch_bounds__(256) k_scalar_boundary(const ScArgs a, int nro
     if (bad) *a.err = 1;
 }
 
-// one k / omega row: node terms and sources, interior terms, relaxation
-// (phiAssembler.h:966-1006), boundary rhs relaxation x 0.75
-// (turbulentKineticEnergyAssembler.h:44-48, phiAssembler.h:1008-1152), and
-// for omega the no-slip wall rows reduced to their diagonal with b = 0
-template <typename T, int SG, bool HR, bool MX>
-__global__ void __launch_bounds__(64) k_scalar_rows(const ScArgs a) {
-    __shared__ double tile[kLdsSlots * 64];
-    double* lt = tile + threadIdx.x;
+// the k / omega equations assembled by one launch: each its own arguments
+// (system, nodal values, diffusivity, relaxation), the mesh and the shared
+// fields those of eq[0]
+template <int NE>
+struct ScSet {
+    ScArgs eq[NE];
+};
+
+// one k / omega row of each of the NE equations: node terms and sources,
+// interior terms, relaxation (phiAssembler.h:966-1006), boundary rhs
+// relaxation x 0.75 (turbulentKineticEnergyAssembler.h:44-48,
+// phiAssembler.h:1008-1152), and for omega the no-slip wall rows reduced to
+// their diagonal with b = 0.  With NE = 2 (k and omega) the row's elements
+// are gathered and their geometry computed once for both: the omega system
+// depends only on fields the k solve does not change (its node terms use
+// the k of the iteration's start, kprev).
+template <typename T, int SG, bool HR, bool MX, int NE>
+__global__ void __launch_bounds__(64) k_scalar_rows(const ScSet<NE> set) {
+    __shared__ double tile[NE * kLdsSlots * 64];
+    const ScArgs* as = set.eq;
+    const ScArgs& a = as[0];
+    double* lt[NE];
+#pragma unroll
+    for (int q = 0; q < NE; ++q) lt[q] = tile + q * kLdsSlots * 64 + threadIdx.x;
     const int row = ordered_item(a.slice_order, a.nslice, a.nn);
     if (row < 0) return;
     const SellRow sr = sell_row<1>(a.slice_ptr, a.slice_w, row);
@@ -1923,46 +1981,52 @@ __global__ void __launch_bounds__(64) k_scalar_rows(const ScArgs a) {
     const double xr[3] = {a.xyz[3 * size_t(row)], a.xyz[3 * size_t(row) + 1],
                           a.xyz[3 * size_t(row) + 2]};
     int bad = 0;
-    double D, R;
-    sst_node_terms(a, row, D, R);
-    scalar_row_zero(a.vals, sr, lt, dof);
+    double D[NE], R[NE];
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+        sst_node_terms(as[q], row, D[q], R[q]);
+        scalar_row_zero(as[q].vals, sr, lt[q], dof);
+    }
 #pragma unroll 1
     for (int la = 0; la < 8; ++la) {
         const size_t o = size_t(la) * a.nn + row;
         const int e = a.inc_la[o];
-        if (e >= 0) sc_incidence<T, SG, HR>(a, la, e, a.slots_la[o], sr, dof, xr, D, R, bad, lt);
+        if (e >= 0) sc_incidence<T, SG, HR, NE>(as, la, e, a.slots_la[o], sr, dof, xr, D, R, bad, lt);
     }
 #pragma unroll 1
     for (int k = a.ext_ptr[row]; k < a.ext_ptr[row + 1]; ++k)
-        sc_incidence<T, SG, HR>(a, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, D, R,
-                                bad, lt);
+        sc_incidence<T, SG, HR, NE>(as, a.ext[k] & 7, a.ext[k] >> 3, a.ext_slots[k], sr, dof, xr, D, R,
+                                    bad, lt);
     if constexpr (MX) {
 #pragma unroll 1
         for (int k = a.gen_ptr[row]; k < a.gen_ptr[row + 1]; ++k) {
             const int e = a.gen[k] >> 3, la = a.gen[k] & 7, t = a.etype[e];
             if (t == OAGPU_TET4)
-                sc_generic<T, topo::kTet, HR>(a, la, e, a.gen_slots[k], sr, dof, xr, SG, D, R, bad, lt);
+                sc_generic<T, topo::kTet, HR, NE>(as, la, e, a.gen_slots[k], sr, dof, xr, SG, D, R, bad, lt);
             else if (t == OAGPU_WED6)
-                sc_generic<T, topo::kWed, HR>(a, la, e, a.gen_slots[k], sr, dof, xr, SG, D, R, bad, lt);
+                sc_generic<T, topo::kWed, HR, NE>(as, la, e, a.gen_slots[k], sr, dof, xr, SG, D, R, bad, lt);
             else
-                sc_generic<T, topo::kPyr, HR>(a, la, e, a.gen_slots[k], sr, dof, xr, SG, D, R, bad, lt);
+                sc_generic<T, topo::kPyr, HR, NE>(as, la, e, a.gen_slots[k], sr, dof, xr, SG, D, R, bad, lt);
         }
     }
-    D *= a.urf_inv;
-    if (a.bnode[row]) R *= 0.75;
-    if (a.which == 1 && a.wnode[row]) {  // omega rows fixed at no-slip wall nodes
-        scalar_row_zero(a.vals, sr, lt, dof);
-        R = 0.0;
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+        const ScArgs& aq = as[q];
+        D[q] *= aq.urf_inv;
+        if (a.bnode[row]) R[q] *= 0.75;
+        if (aq.which == 1 && a.wnode[row]) {  // omega rows fixed at no-slip wall nodes
+            scalar_row_zero(aq.vals, sr, lt[q], dof);
+            R[q] = 0.0;
+        }
+        scalar_row_store(aq.vals, sr, lt[q], dof);
+        *sell_val<1>(aq.vals, sr, dof, 0) = D[q];
+        aq.b[row] = R[q];
     }
-    scalar_row_store(a.vals, sr, lt, dof);
-    *sell_val<1>(a.vals, sr, dof, 0) = D;
-    a.b[row] = R;
     if (bad) *a.err = 1;
 }
 
-void launch_assemble_scalar(Flow& f, int which) {
-    OAGPU_REQUIRE(f.sst_on, "SST not enabled on this flow");
-    OAGPU_REQUIRE(which == 0 || which == 1, "scalar system: 0 k, 1 omega");
+// the arguments of the k (which 0) or omega (1) system, written to vals / b
+static ScArgs scalar_args(Flow& f, int which, double* vals, double* b) {
     DevMesh& m = *f.m;
     ScArgs a;
     a.nn = m.nn;
@@ -2032,38 +2096,78 @@ void launch_assemble_scalar(Flow& f, int which) {
     a.sig2 = which == 0 ? sstc::kSigK2 : sstc::kSigW2;
     a.which = which;
     a.us = (which == 0 ? f.sst.k_shifted : f.sst.w_shifted) ? 1 : 0;
-    a.vals = f.A1.p;
-    a.b = f.b1.p;
+    a.vals = vals;
+    a.b = b;
     a.err = f.err.p;
+    return a;
+}
+
+static bool scalar_gs(const Flow& f, int which) {
+    return (which == 0 ? f.sst.kgrad_shifted : f.sst.wgrad_shifted) != 0;
+}
+
+// the boundary rows (inlets, outlets, openings) of one scalar system
+static void scalar_boundary(Flow& f, const ScArgs& a, bool gs) {
+    if (!f.has_io) return;
+    const unsigned GB = xcd_grid(size_t(f.nio_row), 256);
+    if (gs) k_scalar_boundary<1><<<GB, 256, 0, f.stream>>>(a, f.nio_row, f.io_row.p);
+    else k_scalar_boundary<0><<<GB, 256, 0, f.stream>>>(a, f.nio_row, f.io_row.p);
+    OAGPU_HIP(hipGetLastError());
+}
+
+template <int NE>
+static void scalar_rows(Flow& f, const ScSet<NE>& set, bool gs) {
+    DevMesh& m = *f.m;
     const unsigned B = 64, G = xcd_grid(m.nn, B);
     const bool mixed = f.prm.precision == OAGPU_MIXED;
-    const bool gs = (which == 0 ? f.sst.kgrad_shifted : f.sst.wgrad_shifted) != 0;
-    const bool hr = a.beta != nullptr;
-    const unsigned lds = occupancy_lds(B, kLdsSlots * 64 * sizeof(double), f.tune.cap_scalar);
+    const bool hr = set.eq[0].beta != nullptr;
+    const unsigned lds = occupancy_lds(B, NE * kLdsSlots * 64 * sizeof(double), f.tune.cap_scalar);
     auto go = [&](auto kern) {
         clock_begin(f, kClkScalar);
-        kern<<<G, B, lds, f.stream>>>(a);
+        kern<<<G, B, lds, f.stream>>>(set);
         clock_end(f, kClkScalar);
     };
     auto pick = [&](auto mx) {
         constexpr bool MX = decltype(mx)::value;
         if (gs) {
-            if (mixed) go(hr ? k_scalar_rows<float, 1, true, MX> : k_scalar_rows<float, 1, false, MX>);
-            else go(hr ? k_scalar_rows<double, 1, true, MX> : k_scalar_rows<double, 1, false, MX>);
+            if (mixed) go(hr ? k_scalar_rows<float, 1, true, MX, NE> : k_scalar_rows<float, 1, false, MX, NE>);
+            else go(hr ? k_scalar_rows<double, 1, true, MX, NE> : k_scalar_rows<double, 1, false, MX, NE>);
         } else {
-            if (mixed) go(hr ? k_scalar_rows<float, 0, true, MX> : k_scalar_rows<float, 0, false, MX>);
-            else go(hr ? k_scalar_rows<double, 0, true, MX> : k_scalar_rows<double, 0, false, MX>);
+            if (mixed) go(hr ? k_scalar_rows<float, 0, true, MX, NE> : k_scalar_rows<float, 0, false, MX, NE>);
+            else go(hr ? k_scalar_rows<double, 0, true, MX, NE> : k_scalar_rows<double, 0, false, MX, NE>);
         }
     };
     if (m.mixed) pick(std::true_type{});
     else pick(std::false_type{});
     OAGPU_HIP(hipGetLastError());
-    if (f.has_io) {
-        const unsigned GB = xcd_grid(size_t(f.nio_row), 256);
-        if (gs) k_scalar_boundary<1><<<GB, 256, 0, f.stream>>>(a, f.nio_row, f.io_row.p);
-        else k_scalar_boundary<0><<<GB, 256, 0, f.stream>>>(a, f.nio_row, f.io_row.p);
-        OAGPU_HIP(hipGetLastError());
-    }
+}
+
+void launch_assemble_scalar(Flow& f, int which) {
+    OAGPU_REQUIRE(f.sst_on, "SST not enabled on this flow");
+    OAGPU_REQUIRE(which == 0 || which == 1, "scalar system: 0 k, 1 omega");
+    ScSet<1> set;
+    set.eq[0] = scalar_args(f, which, f.A1.p, f.b1.p);
+    scalar_rows<1>(f, set, scalar_gs(f, which));
+    scalar_boundary(f, set.eq[0], scalar_gs(f, which));
+}
+
+bool scalar_pair_ok(const Flow& f) {
+    return f.sst_on && f.tune.scalar_pair != 0 && scalar_gs(f, 0) == scalar_gs(f, 1) &&
+           (f.sst.k_shifted != 0) == (f.sst.w_shifted != 0);
+}
+
+void launch_assemble_scalar_pair(Flow& f) {
+    OAGPU_REQUIRE(scalar_pair_ok(f), "k and omega cannot share one assembly pass on this flow");
+    DevMesh& m = *f.m;
+    if (f.A1w.n < m.nsell) f.A1w.alloc(m.nsell);
+    if (f.b1w.n < size_t(m.nn)) f.b1w.alloc(size_t(m.nn));
+    ScSet<2> set;
+    set.eq[0] = scalar_args(f, 0, f.A1.p, f.b1.p);
+    set.eq[1] = scalar_args(f, 1, f.A1w.p, f.b1w.p);
+    const bool gs = scalar_gs(f, 0);
+    scalar_rows<2>(f, set, gs);
+    scalar_boundary(f, set.eq[0], gs);
+    scalar_boundary(f, set.eq[1], gs);
 }
 
 // --------------------------------------------------------------- mass flow
