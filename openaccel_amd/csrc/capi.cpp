@@ -810,8 +810,7 @@ int oagpu_sst_iteration(oagpu_flow* h, const double* ctl, double* hist) {
         oagpu::scaled_residual(f, 1, &rw);
         oagpu::krylov_solve(f, 1, OAGPU_GMRES, int(ctl[0]), ctl[1], 1e-30, 30, &cdw);
         oagpu::launch_correct_scalar(f, 1);
-        oagpu::launch_gradient(f, 2);
-        oagpu::launch_gradient(f, 3);
+        oagpu::launch_gradient_kw(f);
         check_err(f);
         if (hist) {
             hist[0] = rk / (ks0 + kSmall);

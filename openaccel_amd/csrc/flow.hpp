@@ -303,6 +303,7 @@ void launch_mdot_init(Flow& f);
 void launch_update_mdot(Flow& f);
 void launch_mass_divergence(Flow& f);
 void launch_gradient(Flow& f, int which);
+void launch_gradient_kw(Flow& f);           // grad k and grad omega, one launch when they can share it
 void launch_blending(Flow& f, int which);   // 0 U, 2 k, 3 omega
 void launch_boundary_types(Flow& f);        // wall-node flags and symmetry areas
 // bc.hip: inlet / outlet side mass flow rates (init: from U; update: inlet

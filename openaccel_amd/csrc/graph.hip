@@ -1337,8 +1337,7 @@ void enq_iteration(Flow& f, const double* ctl, const double* sctl, const int* ma
         enq_scaled_res(f, 1, R_RES + 5);
         enq_solve(f, 3, 1, OAGPU_GMRES, maxit[3], sctl[1], 30, g.cap[3], poll);
         enq_correct_scalar(f, 1);
-        launch_gradient(f, 2);
-        launch_gradient(f, 3);
+        launch_gradient_kw(f);
     }
     OAGPU_HIP(hipGetLastError());
 }

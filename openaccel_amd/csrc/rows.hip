@@ -2393,9 +2393,25 @@ void launch_update_mdot(Flow& f) {
 // -------------------------------------------------------------- gradients
 // nodeField::updateGradientField (nodeField.hpp:3253-4950): incremental
 // Green-Gauss with relaxation, interior ips then wall-side ips
+
+// component i of node n: N = 2 are two scalar fields (phi, phi2; k and
+// omega, whose gradients share the element gathers and area vectors), else
+// the N components of phi interleaved
+template <int N>
+__device__ __forceinline__ double fval(const double* phi, const double* phi2, size_t n, int i) {
+    if constexpr (N == 2) return i == 0 ? phi[n] : phi2[n];
+    else return phi[N * n + i];
+}
+// gradient entry k (3 per component) of node a
+template <int N>
+__device__ __forceinline__ double* gslot(double* grad, double* grad2, size_t a, int k) {
+    if constexpr (N == 2) return k < 3 ? grad + 3 * a + k : grad2 + 3 * a + (k - 3);
+    else return grad + 3 * N * a + k;
+}
+
 template <int N, int SH>
 __device__ __forceinline__ void grad_incidence(const double* xyz, const int* conn, const double* phi,
-                                               int la, int e, const double xr[3],
+                                               const double* phi2, int la, int e, const double xr[3],
                                                const double pa[N], double iv, double gurf,
                                                double G[3 * N]) {
     // the element relabelled so that la is node 0: its ips are 0, 3, 8, each
@@ -2412,7 +2428,7 @@ __device__ __forceinline__ void grad_incidence(const double* xyz, const int* con
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
         double pn[N];
-        for (int i = 0; i < N; ++i) pn[i] = phi[N * size_t(nd[n]) + i];
+        for (int i = 0; i < N; ++i) pn[i] = fval<N>(phi, phi2, size_t(nd[n]), i);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const double r = kHex.N[SH][kCanIp[q]][n];
@@ -2432,8 +2448,8 @@ __device__ __forceinline__ void grad_incidence(const double* xyz, const int* con
 // the same for a tet4 / wedge6 / pyramid5 element: its SCS ips with the row
 // node (local la) on either side, area vectors oriented away from the row
 template <int N, int SH, int TOPO>
-__device__ void grad_generic(const double* xyz, const int* conn, const double* phi, int la, int e,
-                             const double xr[3], const double pa[N], double iv, double gurf,
+__device__ void grad_generic(const double* xyz, const int* conn, const double* phi, const double* phi2,
+                             int la, int e, const double xr[3], const double pa[N], double iv, double gurf,
                              double G[3 * N]) {
     constexpr int npe = topo::Traits<TOPO>::npe;
     int nd[npe];
@@ -2453,7 +2469,7 @@ __device__ void grad_generic(const double* xyz, const int* conn, const double* p
         topo::shape<TOPO>(SH, l[0], l[1], l[2], Nv);
         for (int i = 0; i < N; ++i) {
             double pip = 0.0;
-            for (int k = 0; k < npe; ++k) pip += Nv[k] * phi[N * size_t(nd[k]) + i];
+            for (int k = 0; k < npe; ++k) pip += Nv[k] * fval<N>(phi, phi2, size_t(nd[k]), i);
             for (int j = 0; j < 3; ++j) G[3 * i + j] += gurf * (pip - pa[i]) * (sg * A[j]) * iv;
         }
     }
@@ -2470,30 +2486,31 @@ __global__ void __launch_bounds__(256) k_gradient(int nn, int nslice, const int*
                                                   const int* bc_type, const double* symA,
                                                   const double* vol, const double* phi,
                                                   const int* gen_ptr, const int* gen,
-                                                  const int* etype, double gurf, double* grad) {
+                                                  const int* etype, double gurf, double* grad,
+                                                  const double* phi2, double* grad2) {
     const int a = ordered_item(slice_order, nslice, nn);
     if (a < 0) return;
     double G[3 * N];
-    for (int k = 0; k < 3 * N; ++k) G[k] = grad[3 * N * size_t(a) + k] * (1.0 - gurf);
+    for (int k = 0; k < 3 * N; ++k) G[k] = *gslot<N>(grad, grad2, size_t(a), k) * (1.0 - gurf);
     double pa[N];
-    for (int i = 0; i < N; ++i) pa[i] = phi[N * size_t(a) + i];
+    for (int i = 0; i < N; ++i) pa[i] = fval<N>(phi, phi2, size_t(a), i);
     const double iv = 1.0 / vol[a];
     const double xr[3] = {xyz[3 * size_t(a)], xyz[3 * size_t(a) + 1], xyz[3 * size_t(a) + 2]};
 #pragma unroll 1
     for (int la = 0; la < 8; ++la) {
         const int e = inc_la[size_t(la) * nn + a];
-        if (e >= 0) grad_incidence<N, SH>(xyz, conn, phi, la, e, xr, pa, iv, gurf, G);
+        if (e >= 0) grad_incidence<N, SH>(xyz, conn, phi, phi2, la, e, xr, pa, iv, gurf, G);
     }
 #pragma unroll 1
     for (int k = ext_ptr[a]; k < ext_ptr[a + 1]; ++k)
-        grad_incidence<N, SH>(xyz, conn, phi, ext[k] & 7, ext[k] >> 3, xr, pa, iv, gurf, G);
+        grad_incidence<N, SH>(xyz, conn, phi, phi2, ext[k] & 7, ext[k] >> 3, xr, pa, iv, gurf, G);
     if constexpr (MX)
 #pragma unroll 1
         for (int k = gen_ptr[a]; k < gen_ptr[a + 1]; ++k) {
             const int e = gen[k] >> 3, la = gen[k] & 7, t = etype[e];
-            if (t == OAGPU_TET4) grad_generic<N, SH, topo::kTet>(xyz, conn, phi, la, e, xr, pa, iv, gurf, G);
-            else if (t == OAGPU_WED6) grad_generic<N, SH, topo::kWed>(xyz, conn, phi, la, e, xr, pa, iv, gurf, G);
-            else grad_generic<N, SH, topo::kPyr>(xyz, conn, phi, la, e, xr, pa, iv, gurf, G);
+            if (t == OAGPU_TET4) grad_generic<N, SH, topo::kTet>(xyz, conn, phi, phi2, la, e, xr, pa, iv, gurf, G);
+            else if (t == OAGPU_WED6) grad_generic<N, SH, topo::kWed>(xyz, conn, phi, phi2, la, e, xr, pa, iv, gurf, G);
+            else grad_generic<N, SH, topo::kPyr>(xyz, conn, phi, phi2, la, e, xr, pa, iv, gurf, G);
         }
 #pragma unroll 1
     for (int k = binc_ptr[a]; k < binc_ptr[a + 1]; ++k) {
@@ -2506,7 +2523,7 @@ __global__ void __launch_bounds__(256) k_gradient(int nn, int nslice, const int*
         for (int q = 0; q < npf; ++q) {
             const int n = conn[8 * size_t(e) + sn[q]];
             for (int d = 0; d < 3; ++d) fp[q][d] = xyz[3 * size_t(n) + d] - xr[d];
-            for (int i = 0; i < N; ++i) pf[q][i] = phi[N * size_t(n) + i];
+            for (int i = 0; i < N; ++i) pf[q][i] = fval<N>(phi, phi2, size_t(n), i);
         }
         double A[3];
         face_areav(npf, fp, fk, A);
@@ -2516,6 +2533,7 @@ __global__ void __launch_bounds__(256) k_gradient(int nn, int nslice, const int*
             for (int j = 0; j < 3; ++j) G[3 * i + j] += gurf * (pip - pa[i]) * A[j] * iv;
         }
     }
+    if constexpr (N != 2) {  // (k and omega: no symmetry correction)
     if (symA) {  // symmetry planes: nodeVectorField.cpp:46-140, nodeScalarField.cpp:45-110
         const double* sa = symA + 3 * size_t(a);
         const double amag = sqrt(sa[0] * sa[0] + sa[1] * sa[1] + sa[2] * sa[2]);
@@ -2541,7 +2559,8 @@ __global__ void __launch_bounds__(256) k_gradient(int nn, int nslice, const int*
             }
         }
     }
-    for (int k = 0; k < 3 * N; ++k) grad[3 * N * size_t(a) + k] = G[k];
+    }
+    for (int k = 0; k < 3 * N; ++k) *gslot<N>(grad, grad2, size_t(a), k) = G[k];
 }
 
 // ---------------------------------------------------------- blending factor
@@ -2747,7 +2766,7 @@ void launch_gradient(Flow& f, int which) {
                                     m.slots_la.p, m.ext_ptr.p, m.ext.p, m.ext_slots.p, m.binc_ptr.p,
                                     m.binc.p, m.bf_elem.p, m.bf_ord.p, m.bf_bc.p, types, sym,
                                     m.vol.p, phi, m.mixed ? m.gen_ptr.p : nullptr, m.gen.p,
-                                    m.etype.p, f.prm.grad_urf, grad);
+                                    m.etype.p, f.prm.grad_urf, grad, nullptr, nullptr);
         clock_end(f, clk);
     };
     auto pick = [&](auto kern_hex, auto kern_mixed, const double* phi, double* grad) {
@@ -2770,6 +2789,34 @@ void launch_gradient(Flow& f, int which) {
         if (sh) pick(k_gradient<1, 1, false>, k_gradient<1, 1, true>, phi, grad);
         else pick(k_gradient<1, 0, false>, k_gradient<1, 0, true>, phi, grad);
     }
+    OAGPU_HIP(hipGetLastError());
+}
+
+// grad k and grad omega (the end of an SST iteration): one launch sharing the
+// element gathers and area vectors when both use the same interpolation,
+// else one launch each
+void launch_gradient_kw(Flow& f) {
+    OAGPU_REQUIRE(f.sst_on, "SST not enabled on this flow");
+    if (f.tune.scalar_pair == 0 || (f.sst.k_shifted != 0) != (f.sst.w_shifted != 0)) {
+        launch_gradient(f, 2);
+        launch_gradient(f, 3);
+        return;
+    }
+    DevMesh& m = *f.m;
+    const unsigned B = 256, G = xcd_grid(m.nn, B);
+    const int* types = f.has_sym ? f.bc_type.p : nullptr;  // symmetry faces add no term
+    const unsigned lds = occupancy_lds(B, 0, f.tune.cap_grad);
+    auto go = [&](auto kern) {
+        clock_begin(f, kClkGradP);
+        kern<<<G, B, lds, f.stream>>>(m.nn, m.nslice, m.slice_order.p, m.xyz.p, m.conn.p, m.inc_la.p,
+                                    m.slots_la.p, m.ext_ptr.p, m.ext.p, m.ext_slots.p, m.binc_ptr.p,
+                                    m.binc.p, m.bf_elem.p, m.bf_ord.p, m.bf_bc.p, types, nullptr,
+                                    m.vol.p, f.k.p, m.mixed ? m.gen_ptr.p : nullptr, m.gen.p,
+                                    m.etype.p, f.prm.grad_urf, f.gradK.p, f.om.p, f.gradW.p);
+        clock_end(f, kClkGradP);
+    };
+    if (f.sst.k_shifted) go(m.mixed ? k_gradient<2, 1, true> : k_gradient<2, 1, false>);
+    else go(m.mixed ? k_gradient<2, 0, true> : k_gradient<2, 0, false>);
     OAGPU_HIP(hipGetLastError());
 }
 

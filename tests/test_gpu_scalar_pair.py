@@ -1,10 +1,11 @@
 """k and omega assembled by one launch (the row's elements gathered and
-their geometry computed once for both equations) give the k-omega SST
-iterations of one launch per equation: the omega system depends only on
-fields the k solve leaves unchanged (its node terms read the k of the
-iteration's start).  Same scaled residuals, Krylov iteration counts and k,
-omega fields (row-relative 1e-13; each equation's arithmetic is term for
-term the single-equation one) over three flow + SST iterations, on the
+their geometry computed once for both equations), and their gradients by
+one launch, give the k-omega SST iterations of one launch per equation: the
+omega system depends only on fields the k solve leaves unchanged (its node
+terms read the k of the iteration's start).  Same scaled residuals, Krylov
+iteration counts, k and omega fields and their gradients (relative 1e-13;
+each equation's arithmetic is term for term the single-equation one; they
+come out bitwise equal) over three flow + SST iterations, on the
 wall-resolved step slice (hex8, FP64 and mixed) and on the mixed-topology
 duct with its tet4 / wedge6 / pyramid5 elements and inlet / outlet rows.
 """
@@ -73,7 +74,7 @@ def run(f):
         a = f.outer_iteration(CTL)
         b = f.sst_iteration(SCTL)
         h.append(np.concatenate([a, b]))
-    return np.array(h), {k: f.download(k).copy() for k in ("U", "p", "k", "omega")}
+    return np.array(h), {k: f.download(k).copy() for k in ("U", "p", "k", "omega", "gradK", "gradW")}
 
 
 def compare(h1, s1, h2, s2):
