@@ -64,6 +64,7 @@ def algorithmic_bytes(nn, ne, nnz, value_bytes, vector_bytes, high_res=False, sa
     md = nn * 112 + ne * (conn + 16 * 12)
     gu = nn * (24 + 24 + 8 + 72 + 72) + ne * conn           # x, U, V, grad U in and out
     gp = nn * (24 + 8 + 8 + 24 + 24) + ne * conn
+    gkw = nn * (24 + 8 + 2 * (8 + 24 + 24)) + ne * conn      # x, V; k, omega, their gradients in and out
     v, w = value_bytes, vector_bytes
     amg_pre = nnz * (v + 4) + nn * w * 4                     # A, cols; r, dinv in; x, res out
     amg_post = nnz * (v + 4) + nn * (w * 5 + 4)              # A, cols; x, r, dinv, e, z; agg
@@ -74,7 +75,7 @@ def algorithmic_bytes(nn, ne, nnz, value_bytes, vector_bytes, high_res=False, sa
     spmv3 = nnz * (72 + 4) + nn * 48
     spmv1f = nnz * (4 + 4) + nn * 8
     spmv3f = nnz * (36 + 4) + nn * 24
-    return {"momentum": mom, "pcorr": pc, "scalar": sc, "mdot": md, "gradU": gu, "gradP": gp,
+    return {"momentum": mom, "pcorr": pc, "scalar": sc, "mdot": md, "gradU": gu, "gradP": gp, "gradKW": gkw,
             "amg_pre": amg_pre, "amg_post": amg_post, "spmv1": spmv1, "spmv3": spmv3,
             "spmv1_f32": spmv1f, "spmv3_f32": spmv3f}
 
@@ -102,13 +103,13 @@ PROFILE_REGEX = ("k_scalar_rows|k_mdot_update|k_momentum_rows|k_pcorr_rows|k_gra
                  "k_amg_pre_residual|k_sa_prolong|k_amg_smooth|copyBuffer")
 PROFILE_COMMANDS = [
     "cd /tmp && export TMPDIR=/tmp && timeout -k 10 600 rocprofv3 --kernel-trace --stats --output-format csv "
-    "-d {dir}/trace -o trace -- python3 /root/repo/bench.py --no-cpu --steps 3 --warmup 2",
+    "-d {dir}/trace -o trace -- python3 /root/repo/bench.py --no-cpu --steps 3 --warmup 2 --graph-steps 0",
     "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 300 rocprofv3 --pmc FETCH_SIZE --output-format csv "
     f"--kernel-include-regex \"{PROFILE_REGEX}\" -d {{dir}}/fetch -o fetch -- python3 /root/repo/bench.py "
-    "--steps 1 --warmup 1 --no-cpu",
+    "--steps 1 --warmup 1 --no-cpu --graph-steps 0",
     "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 300 rocprofv3 --pmc WRITE_SIZE --output-format csv "
     f"--kernel-include-regex \"{PROFILE_REGEX}\" -d {{dir}}/write -o write -- python3 /root/repo/bench.py "
-    "--steps 1 --warmup 1 --no-cpu",
+    "--steps 1 --warmup 1 --no-cpu --graph-steps 0",
     "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 60 rocprofv3 --pmc FETCH_SIZE --output-format csv "
     "-d {dir}/fetchcopy -o fetchcopy -- python3 /root/repo/bench.py --copy-probe 1024",
     "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 60 rocprofv3 --pmc WRITE_SIZE --output-format csv "
@@ -123,6 +124,7 @@ PROFILE_SYMBOLS = {
     "mdot": (r"k_mdot_update<", None),
     "gradU": (r"k_gradient<3,", None),
     "gradP": (r"k_gradient<1,", None),
+    "gradKW": (r"k_gradient<2,", None),
     "spmv1_f32": (r"k_spmv<1, float>", None),
     "spmv1": (None, r"k_spmv<1, double>"),
     "spmv3_f32": (r"k_spmv<3, float>", None),
@@ -442,6 +444,22 @@ def bfs_cpu_baseline(dims, threads):
     return {"outer_s": time.perf_counter() - t0, "ne": c.ne}
 
 
+def graph_mode_steps(oa, f, steps, warmup=2):
+    """Continue the step's flow with `steps` timed outer iterations (flow +
+    SST) each replayed from one captured HIP graph, after `warmup` (the
+    first runs uncaptured and sizes the iteration caps, the next captures)."""
+    for _ in range(warmup):
+        f.graph_iterations(BFS_CTL, BFS_SCTL, n=1)
+    oa.load_library().oagpu_synchronize()
+    t0 = time.perf_counter()
+    h = [f.graph_iterations(BFS_CTL, BFS_SCTL, n=1)[0] for _ in range(steps)]
+    oa.load_library().oagpu_synchronize()
+    t = time.perf_counter() - t0
+    return {"value": round(steps / t, 4), "ms_per_step": round(1e3 * t / steps, 3), "steps": steps,
+            "krylov_iters": [[int(x[4]), int(x[5]), int(x[8]), int(x[9])] for x in h],
+            **f.graph_stats()}
+
+
 def bench_bfs(args):
     import openaccel_amd as oa
     from openaccel_amd import meshgen
@@ -477,7 +495,7 @@ def bench_bfs(args):
         dom = None if prec == oa.OAGPU_MIXED else fp64_counterpart(res[oa.OAGPU_MIXED]["dominant"])
         res[prec] = run_form(oa, f, step, args.steps, args.warmup, dom, clock_step=clock_step)
         res[prec]["sa_levels"] = f.sa_levels() if args.bfs_p_method == "bicgstab_sa" else []
-        res[prec]["graph"] = f.graph_stats() if args.iteration == "graph" else None
+        res[prec]["graph"] = graph_mode_steps(oa, f, args.graph_steps) if args.graph_steps else None
         f.close()
         print(f"[bench] {'mixed' if prec else 'fp64'} {args.steps / res[prec]['seconds']:.3f} it/s, "
               f"dominant {res[prec]['dominant']}", file=sys.stderr, flush=True)
@@ -535,7 +553,11 @@ def bench_bfs(args):
                  **hist(r64)},
         "mixed_over_fp64": round(itmx / it64, 3),
         "iteration": args.iteration,
-        "graph": {"mixed": rmx["graph"], "fp64": r64["graph"]} if args.iteration == "graph" else None,
+        "graph_mode": ({"mixed": rmx["graph"], "fp64": r64["graph"],
+                        "mixed_over_fp64": round(rmx["graph"]["value"] / r64["graph"]["value"], 3),
+                        "note": "the same flows continued for the given steps, each outer iteration "
+                                "(flow + SST) one HIP graph launch"}
+                       if args.graph_steps else None),
         "coef_err_vs_fp64": cerr,
         "p_amg_levels": [{"rows": l[0], "entries": l[2], "p_entries": l[1], "omega_p": round(l[3], 4)}
                          for l in rmx["sa_levels"]],
@@ -944,8 +966,10 @@ def main():
                          "cavity: the lid-driven cavity; duct: the 3.9M-cell mixed-topology "
                          "duct; batch: 96 independent cavities")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--iteration", choices=("graph", "eager"), default="graph",
-                    help="bfs: each outer iteration one HIP graph launch, or launched by the host")
+    ap.add_argument("--iteration", choices=("graph", "eager"), default="eager",
+                    help="timed steps: each outer iteration one HIP graph launch, or launched by the host")
+    ap.add_argument("--graph-steps", type=int, default=5,
+                    help="bfs: further timed steps of each form in graph mode (0: none)")
     ap.add_argument("--caller-order", action="store_true",
                     help="keep the generator's node numbering on the device (no locality renumbering)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: all host cores)")

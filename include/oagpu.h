@@ -425,15 +425,21 @@ int oagpu_autotune(oagpu_flow *f, double *report, int n_report);
 /* Set one tunable's variant directly (e.g. from an earlier oagpu_autotune
  * report): which = "momentum" (1 / 0), "pcorr" (0 / 1), "spmv1", "spmv3",
  * "spmv1_f32", "spmv3_f32" (64 / 128 / 256), "scalar", "mdot", "grad"
- * (resident waves per CU: 0, 8 or 16). */
+ * (resident waves per CU: 0, 4, 8 or 16), "scalar_pair" (1: k and omega,
+ * and their gradients, assembled by one launch each pair when their
+ * operators agree; 0: one launch per equation), "graph_cap" (iteration
+ * slots of every solve in the next captured graph of oagpu_graph_iterations,
+ * after its first iteration; a solve needing more rolls back and raises
+ * them). */
 int oagpu_tune_set(oagpu_flow *f, const char *which, int variant);
 
 /* Device time of the library's main kernels, measured with HIP events on the
  * flow's stream (no counterpart in the reference, whose only instruments are
  * the per-iteration CPU time and a profiler region around each solve).
  * oagpu_profile(f, which) clocks "all", "none" or one of: "momentum",
- * "pcorr", "scalar" (k / omega rows), "mdot", "gradU", "gradP" (every scalar
- * gradient), "amg_pre" / "amg_post" (the fine level's pre-smoothing +
+ * "pcorr", "scalar" (k / omega rows), "mdot", "gradU", "gradP" (the scalar
+ * gradients but grad k and grad omega computed together), "gradKW" (grad k
+ * and grad omega by one launch), "amg_pre" / "amg_post" (the fine level's pre-smoothing +
  * residual and prolongation + post-smoothing kernels), "amg_tail" (the
  * one-workgroup coarse levels), "spmv1" / "spmv3" (Krylov products in
  * FP64), "spmv1_f32" / "spmv3_f32" (FP32 inner solves of the mixed form), and

@@ -520,7 +520,7 @@ class FlowCase:
         return {"launches": a.value, "redone": b.value, "captures": c.value, "caps": caps.tolist()}
 
     CLOCKS = ("momentum", "pcorr", "scalar", "mdot", "gradU", "gradP", "amg_pre", "amg_post",
-              "amg_tail", "spmv1", "spmv3", "spmv1_f32", "spmv3_f32")
+              "amg_tail", "spmv1", "spmv3", "spmv1_f32", "spmv3_f32", "gradKW")
 
     TUNABLES = ("momentum", "pcorr", "spmv1", "spmv3", "spmv1_f32", "spmv3_f32", "scalar", "mdot", "grad")
 

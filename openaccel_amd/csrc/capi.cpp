@@ -1160,8 +1160,8 @@ int oagpu_tune_set(oagpu_flow* h, const char* which, int variant) {
             t.spmv_block[which[4] == '3'][std::strstr(which, "f32") != nullptr] = unsigned(variant);
         } else if (!std::strcmp(which, "scalar") || !std::strcmp(which, "mdot") ||
                    !std::strcmp(which, "grad")) {
-            OAGPU_REQUIRE(variant == 0 || variant == 8 || variant == 16,
-                          "occupancy cap: 0, 8 or 16 waves per CU");
+            OAGPU_REQUIRE(variant == 0 || variant == 4 || variant == 8 || variant == 16,
+                          "occupancy cap: 0, 4, 8 or 16 waves per CU");
             (which[0] == 's' ? t.cap_scalar : which[0] == 'm' ? t.cap_mdot : t.cap_grad) = variant;
         } else if (!std::strcmp(which, "scalar_pair")) {
             OAGPU_REQUIRE(variant == 0 || variant == 1, "scalar_pair: 0 or 1");

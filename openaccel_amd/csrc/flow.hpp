@@ -101,12 +101,12 @@ struct Amg {
 // clocked.
 enum ClockId {
     kClkMomentum, kClkPcorr, kClkScalar, kClkMdot, kClkGradU, kClkGradP, kClkAmgPre,
-    kClkAmgPost, kClkAmgTail, kClkSpmv1, kClkSpmv3, kClkSpmv1f, kClkSpmv3f, kClkCount
+    kClkAmgPost, kClkAmgTail, kClkSpmv1, kClkSpmv3, kClkSpmv1f, kClkSpmv3f, kClkGradKW, kClkCount
 };
 inline const char* clock_name(int id) {
     static const char* names[kClkCount] = {"momentum", "pcorr", "scalar", "mdot", "gradU",
                                            "gradP", "amg_pre", "amg_post", "amg_tail", "spmv1",
-                                           "spmv3", "spmv1_f32", "spmv3_f32"};
+                                           "spmv3", "spmv1_f32", "spmv3_f32", "gradKW"};
     return names[id];
 }
 struct KernelClock {

@@ -411,16 +411,16 @@ __global__ void __launch_bounds__(kRedBlock) k_gm_reduce(const double* part, dou
 }
 
 // w -= V h fused with the partials of V_k . w for the updated w (second
-// Gram-Schmidt pass)
-template <typename T>
+// Gram-Schmidt pass); MAXK >= j + 1 (the slot's largest possible step)
+template <typename T, int MAXK>
 __global__ void __launch_bounds__(kRedBlock) k_gm_upd_dots(size_t n, const T* Vb, const double* h, T* w,
                                                            double* part, const int* si) {
     __shared__ double sh[kRedBlock];
     if (!gate_open(si, G_STEP)) return;
     const int nk = si[I_J] + 1;
-    double acc[kMaxRes];
+    double acc[MAXK];
 #pragma unroll
-    for (int k = 0; k < kMaxRes; ++k) acc[k] = 0.0;
+    for (int k = 0; k < MAXK; ++k) acc[k] = 0.0;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
         double s = 0.0;
         for (int k = 0; k < nk; ++k) s += h[k] * double(Vb[size_t(k) * n + i]);
@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(kRedBlock) k_gm_upd_dots(size_t n, const T* Vb
         w[i] = wt;
         const double wi = double(wt);
 #pragma unroll
-        for (int k = 0; k < kMaxRes; ++k)
+        for (int k = 0; k < MAXK; ++k)
             if (k < nk) acc[k] += double(Vb[size_t(k) * n + i]) * wi;
     }
     for (int k = 0; k < nk; ++k) {
@@ -1013,7 +1013,9 @@ void gm_slot(Flow& f, int s, int mres, bool blocks, const T* A, const T* b, cons
     for (int k0 = 0; k0 <= jmax; k0 += 8)
         k_gm_dots<T><<<kRedGrid, kRedBlock, 0, st>>>(n, k0, Vb, w, part, si);
     k_gm_reduce<<<jmax + 1, kRedBlock, 0, st>>>(part, sd + D_H1, si);
-    k_gm_upd_dots<T><<<kRedGrid, kRedBlock, 0, st>>>(n, Vb, sd + D_H1, w, part, si);
+    if (jmax < 8) k_gm_upd_dots<T, 8><<<kRedGrid, kRedBlock, 0, st>>>(n, Vb, sd + D_H1, w, part, si);
+    else if (jmax < 16) k_gm_upd_dots<T, 16><<<kRedGrid, kRedBlock, 0, st>>>(n, Vb, sd + D_H1, w, part, si);
+    else k_gm_upd_dots<T, kMaxRes><<<kRedGrid, kRedBlock, 0, st>>>(n, Vb, sd + D_H1, w, part, si);
     k_gm_reduce<<<jmax + 1, kRedBlock, 0, st>>>(part, sd + D_H2, si);
     k_gm_upd_norm<T><<<kRedGrid, kRedBlock, 0, st>>>(n, Vb, sd + D_H2, w, part, si);
     k_gm_givens<<<1, kRedBlock, 0, st>>>(part, sd, si);

@@ -2807,13 +2807,13 @@ void launch_gradient_kw(Flow& f) {
     const int* types = f.has_sym ? f.bc_type.p : nullptr;  // symmetry faces add no term
     const unsigned lds = occupancy_lds(B, 0, f.tune.cap_grad);
     auto go = [&](auto kern) {
-        clock_begin(f, kClkGradP);
+        clock_begin(f, kClkGradKW);
         kern<<<G, B, lds, f.stream>>>(m.nn, m.nslice, m.slice_order.p, m.xyz.p, m.conn.p, m.inc_la.p,
                                     m.slots_la.p, m.ext_ptr.p, m.ext.p, m.ext_slots.p, m.binc_ptr.p,
                                     m.binc.p, m.bf_elem.p, m.bf_ord.p, m.bf_bc.p, types, nullptr,
                                     m.vol.p, f.k.p, m.mixed ? m.gen_ptr.p : nullptr, m.gen.p,
                                     m.etype.p, f.prm.grad_urf, f.gradK.p, f.om.p, f.gradW.p);
-        clock_end(f, kClkGradP);
+        clock_end(f, kClkGradKW);
     };
     if (f.sst.k_shifted) go(m.mixed ? k_gradient<2, 1, true> : k_gradient<2, 1, false>);
     else go(m.mixed ? k_gradient<2, 0, true> : k_gradient<2, 0, false>);

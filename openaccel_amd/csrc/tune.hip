@@ -12,7 +12,7 @@
 //   and the mixed form's FP32 inner solves): 64, 128 or 256 rows per
 //   workgroup;
 // * k / omega rows, mass-flow update and gradients: resident waves per CU
-//   capped at 8, 16 or not at all (LDS reserved per workgroup).  The
+//   capped at 4, 8, 16 or not at all (LDS reserved per workgroup).  The
 //   mass-flow update relaxes the stored rates in place, so its candidates run
 //   on a saved copy of them that is put back afterwards.
 #include "flow.hpp"
@@ -91,11 +91,11 @@ void autotune(Flow& f, double* report, int n_report) {
             ++row;
         }
     }
-    // occupancy caps: the fastest of none / 16 / 8 waves per CU
+    // occupancy caps: the fastest of none / 16 / 8 / 4 waves per CU
     auto caps = [&](int& cap, int r, auto&& run) {
         double tb = 1e300, t0 = 0.0;
         int best = 0;
-        for (int c : {0, 16, 8}) {
+        for (int c : {0, 16, 8, 4}) {
             cap = c;
             const double t = time_ms(f, run, 3);
             if (c == 0) t0 = t;
@@ -109,7 +109,11 @@ void autotune(Flow& f, double* report, int n_report) {
         rep[r][1] = tb;
         rep[r][2] = t0;
     };
-    if (f.sst_on) caps(f.tune.cap_scalar, 6, [&] { launch_assemble_scalar(f, 0); });
+    if (f.sst_on)  // the k / omega assembly as the iterations run it (one launch for both when possible)
+        caps(f.tune.cap_scalar, 6, [&] {
+            if (scalar_pair_ok(f)) launch_assemble_scalar_pair(f);
+            else launch_assemble_scalar(f, 0);
+        });
     {
         DBuf<double> md(f.mdot.n), sd;
         OAGPU_HIP(hipMemcpyAsync(md.p, f.mdot.p, f.mdot.n * sizeof(double), hipMemcpyDeviceToDevice,

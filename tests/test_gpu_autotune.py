@@ -90,7 +90,7 @@ def test_occupancy_caps_bitwise_and_state_kept(oracle, oa, meshgen):
     f.assemble_momentum()
     md0 = f.download("mdot")
     out = []
-    for cap in (0, 8, 16):
+    for cap in (0, 4, 8, 16):
         f.tune_set("mdot", cap)
         f.tune_set("grad", cap)
         f.upload("mdot", md0)
@@ -104,6 +104,6 @@ def test_occupancy_caps_bitwise_and_state_kept(oracle, oa, meshgen):
     rep = f.autotune()
     np.testing.assert_array_equal(f.download("mdot"), md0)
     for k in ("mdot", "grad"):
-        assert rep[k][0] in (0, 8, 16) and rep[k][1] > 0
+        assert rep[k][0] in (0, 4, 8, 16) and rep[k][1] > 0
     with pytest.raises(oa.OagpuError):
         f.tune_set("mdot", 12)
