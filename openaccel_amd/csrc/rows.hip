@@ -2249,19 +2249,36 @@ __global__ void __launch_bounds__(256) k_mdot_update(int ne, int negroup, const 
             area_scs(ip, c, A);
             jinv_scs(SP, ip, c, Ji, bad);
             double uip[3] = {0, 0, 0}, duip[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
+            if constexpr (US == 1) {
+                // shifted interpolation: the ip is its edge's midpoint, where the
+                // only non-zero shape functions are those of the edge's two nodes
+                // (c_lrscv, in node order: the terms of the full sum, same order)
+#pragma unroll
+                for (int s2 = 0; s2 < 2; ++s2) {
+                    const int n = hex8::c_lrscv[ip][s2];
+                    const double r = c_hex.N[US][ip][n];
+                    const size_t o = 3 * size_t(nd[n]);
+                    for (int j = 0; j < 3; ++j) {
+                        duip[j] += r * du[o + j];
+                        uip[j] += r * U[o + j];
+                    }
+                }
+            }
 #pragma unroll 2
             for (int n = 0; n < 8; ++n) {
-                const double r = c_hex.N[US][ip][n];
                 const int node = nd[n];
-                const size_t o = 3 * size_t(node);
                 double gv[3];
                 gradN_scs(SP, ip, n, Ji, gv);
                 const double pn = p[node] - pr;
-                for (int j = 0; j < 3; ++j) {
-                    duip[j] += r * du[o + j];
-                    uip[j] += r * U[o + j];
-                    dp[j] += gv[j] * pn;
+                if constexpr (US != 1) {
+                    const double r = c_hex.N[US][ip][n];
+                    const size_t o = 3 * size_t(node);
+                    for (int j = 0; j < 3; ++j) {
+                        duip[j] += r * du[o + j];
+                        uip[j] += r * U[o + j];
+                    }
                 }
+                for (int j = 0; j < 3; ++j) dp[j] += gv[j] * pn;
             }
             // constant density: the upwind density is rho whatever the flow direction
             double t = 0;
