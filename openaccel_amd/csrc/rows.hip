@@ -462,7 +462,7 @@ struct MomArgs {
 // -mu (grad U + grad U^T) . A), and the columns lumped into the diagonal need
 // no blocks of their own: the shape-function gradients sum to zero, so the
 // diagonal takes minus the gradients of the kept columns.
-template <typename T, int SH, bool HR, typename LT>
+template <typename T, int SH, bool HR, int US, typename LT>
 __device__ __forceinline__ void mom_scs(const MomArgs& a, int r, int e, const int nd[8],
                                         const T c[8][3], uint64_t sl, const SellRow& sr, int dof,
                                         double D[9], double R[3], int& bad, LT* lt) {
@@ -497,14 +497,15 @@ __device__ __forceinline__ void mom_scs(const MomArgs& a, int r, int e, const in
 #pragma unroll
         for (int i = 0; i < 3; ++i) rl[i] = -m * up[i];
     }
-    // effective viscosity at the ip (velocity shape functions); a constant
-    // viscosity needs no interpolation: the shape functions sum to one
+    // effective viscosity at the ip (velocity shape functions US; their zero
+    // weights and those nodes' loads drop out); a constant viscosity needs no
+    // interpolation: the shape functions sum to one
     T mu = T(a.mu);
     if (a.mueff) {
         mu = T(0);
 #pragma unroll
         for (int n = 0; n < 8; ++n)
-            mu += (a.us ? T(kHex.N[1][IP][n]) : T(kHex.N[0][IP][n])) * T(a.mueff[nd[n]]);
+            if (kHex.N[US][IP][n] != 0.0) mu += T(kHex.N[US][IP][n]) * T(a.mueff[nd[n]]);
     }
     // explicit viscous flux + mu (G + G^T) . A, G_ij = du_i/dx_j at the ip
     // from the parametric derivatives P_id = sum_n u_n,i dN_n/ds_d
@@ -601,7 +602,8 @@ __device__ __forceinline__ void mom_incidence(const MomArgs& a, int la, int e, u
     gather_rel(a.xyz, nd, xr, c);
 #pragma unroll 1
     for (int q = 0; q < 3; ++q) {
-        mom_scs<T, SH, HR>(a, 3 * la + q, e, nd, c, sl, sr, dof, D, R, bad, lt);
+        if (a.us) mom_scs<T, SH, HR, 1>(a, 3 * la + q, e, nd, c, sl, sr, dof, D, R, bad, lt);
+        else mom_scs<T, SH, HR, 0>(a, 3 * la + q, e, nd, c, sl, sr, dof, D, R, bad, lt);
         int t[8];
         T u[8][3];
 #pragma unroll
@@ -1138,7 +1140,7 @@ __device__ __forceinline__ void rotate_nodes(V v[8][W]) {
 // kept columns' coefficients: the shape-function gradients sum to zero.
 // Nodal values: in registers, rotated with the coordinates (REG, the FP32
 // form), or read per ip from L1/L2 (the FP64 form, whose registers are full).
-template <typename T, int SP, bool REG>
+template <typename T, int SP, bool REG, int US>
 __device__ __forceinline__ void pc_scs(const PcArgs& a, int r, const int nd[8], double pr,
                                        const T c[8][3], const T un[8][3], const T dn[8][3],
                                        const T pn[8][1], const T Gp[3], uint64_t sl,
@@ -1148,28 +1150,37 @@ __device__ __forceinline__ void pc_scs(const PcArgs& a, int r, const int nd[8], 
     T A[3], Ji[9];
     area_c(IP, c, A);
     jinv_c<SP>(IP, c, Ji, bad);
-    const bool us = a.u_shifted;
     T uip[3] = {0, 0, 0}, duip[3] = {0, 0, 0}, P[3] = {0, 0, 0};
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
-        const T w = us ? T(kHex.N[1][IP][n]) : T(kHex.N[0][IP][n]);
-        T u[3], du[3], pv;
-        if constexpr (REG) {
-            for (int j = 0; j < 3; ++j) u[j] = un[n][j], du[j] = dn[n][j];
-            pv = pn[n][0];
-        } else {
-            const size_t o = 3 * size_t(nd[n]);
-            for (int j = 0; j < 3; ++j) u[j] = T(a.U[o + j]), du[j] = T(a.du[o + j]);
-            pv = T(a.p[nd[n]] - pr);
-        }
+        // velocity interpolation US (1: edge midpoint, two non-zero weights);
+        // the zero weights drop out, and with them their nodes' loads
+        const bool wz = kHex.N[US][IP][n] == 0.0;
+        const bool gz = kHex.dn[SP][IP][n][0] == 0.0 && kHex.dn[SP][IP][n][1] == 0.0 &&
+                        kHex.dn[SP][IP][n][2] == 0.0;
+        const T w = T(kHex.N[US][IP][n]);
+        if (!wz) {
+            T u[3], du[3];
+            if constexpr (REG) {
+                for (int j = 0; j < 3; ++j) u[j] = un[n][j], du[j] = dn[n][j];
+            } else {
+                const size_t o = 3 * size_t(nd[n]);
+                for (int j = 0; j < 3; ++j) u[j] = T(a.U[o + j]), du[j] = T(a.du[o + j]);
+            }
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            uip[j] += w * u[j];
-            duip[j] += w * du[j];
+            for (int j = 0; j < 3; ++j) {
+                uip[j] += w * u[j];
+                duip[j] += w * du[j];
+            }
         }
+        if (!gz) {
+            T pv;
+            if constexpr (REG) pv = pn[n][0];
+            else pv = T(a.p[nd[n]] - pr);
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-            if (kHex.dn[SP][IP][n][d] != 0.0) P[d] += T(kHex.dn[SP][IP][n][d]) * pv;
+            for (int d = 0; d < 3; ++d)
+                if (kHex.dn[SP][IP][n][d] != 0.0) P[d] += T(kHex.dn[SP][IP][n][d]) * pv;
+        }
     }
     const T rho = T(a.rho);
     T md = T(0);
@@ -1215,16 +1226,19 @@ __device__ __forceinline__ void pc_scs(const PcArgs& a, int r, const int nd[8], 
 // element e's contribution to the p' row of its local node la: gathered
 // once, relabelled so that la is node 0, its three SCS ips touching la added
 // as ip 0 of the three relabellings
-template <typename T, int SP>
-__device__ __forceinline__ void pc_incidence(const PcArgs& a, int la, int e, uint64_t sl,
-                                             const SellRow& sr, int dof, const double xr[3],
-                                             double pr, double& D, double& R, int& bad,
-                                             double* lt) {
+// Nodal values in registers only for the FP32 form with the full
+// interpolation (all 8 nodes weighted at every ip); with the shifted one an
+// ip reads just its edge's two nodes, cheaper from L1 than 8 nodes' registers.
+template <typename T, int SP, int US>
+__device__ __forceinline__ void pc_incidence_us(const PcArgs& a, int la, int e, uint64_t sl,
+                                                const SellRow& sr, int dof, const double xr[3],
+                                                double pr, double& D, double& R, int& bad,
+                                                double* lt) {
     const int* ce = a.conn + 8 * size_t(e);
     int nd[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) nd[k] = ce[c_rot.perm[3 * la][k]];
-    constexpr bool REG = sizeof(T) == 4;
+    constexpr bool REG = sizeof(T) == 4 && US == 0;
     T c[8][3], un[REG ? 8 : 1][3], dn[REG ? 8 : 1][3], pn[REG ? 8 : 1][1], Gp[3] = {0, 0, 0};
     gather_rel(a.xyz, nd, xr, c);
 #pragma unroll
@@ -1244,7 +1258,7 @@ __device__ __forceinline__ void pc_incidence(const PcArgs& a, int la, int e, uin
     }
 #pragma unroll 1
     for (int q = 0; q < 3; ++q) {
-        pc_scs<T, SP, REG>(a, 3 * la + q, nd, pr, c, un, dn, pn, Gp, sl, sr, dof, D, R, bad, lt);
+        pc_scs<T, SP, REG, US>(a, 3 * la + q, nd, pr, c, un, dn, pn, Gp, sl, sr, dof, D, R, bad, lt);
         rotate_nodes<T, 3>(c);
         if constexpr (REG) {
             rotate_nodes<T, 3>(un);
@@ -1258,6 +1272,15 @@ __device__ __forceinline__ void pc_incidence(const PcArgs& a, int la, int e, uin
             for (int k = 0; k < 8; ++k) nd[k] = t[k];
         }
     }
+}
+
+template <typename T, int SP>
+__device__ __forceinline__ void pc_incidence(const PcArgs& a, int la, int e, uint64_t sl,
+                                             const SellRow& sr, int dof, const double xr[3],
+                                             double pr, double& D, double& R, int& bad,
+                                             double* lt) {
+    if (a.u_shifted) pc_incidence_us<T, SP, 1>(a, la, e, sl, sr, dof, xr, pr, D, R, bad, lt);
+    else pc_incidence_us<T, SP, 0>(a, la, e, sl, sr, dof, xr, pr, D, R, bad, lt);
 }
 
 // the SCS ips of a tet4 / wedge6 / pyramid5 element touching the p' row of
@@ -1647,7 +1670,7 @@ __device__ void sst_node_terms(const ScArgs& a, int i, double& lhs, double& rhs)
 // or read per ip from L1/L2.  NE equations (k, or k and omega) share the
 // ip's geometry and mass flow; each keeps its own arithmetic, term for term
 // that of a single equation.
-template <typename T, int SG, bool HR, bool REG, int NE>
+template <typename T, int SG, bool HR, bool REG, int NE, int US>
 __device__ __forceinline__ void sc_scs(const ScArgs* as, int r, int e, const int nd[8],
                                        const T c[8][3], const T (*ph)[8][1], const T (*gm)[8][1],
                                        uint64_t sl, const SellRow& sr, int dof, double* D,
@@ -1663,25 +1686,34 @@ __device__ __forceinline__ void sc_scs(const ScArgs* as, int r, int e, const int
     for (int q = 0; q < NE; ++q) Gip[q] = P[q][0] = P[q][1] = P[q][2] = T(0);
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
+        // the interpolation US's zero weights and the gradient's zero rows drop
+        // out, and with them their nodes' loads
+        const bool wz = kHex.N[US][IP][n] == 0.0;
+        const bool gz = kHex.dn[SG][IP][n][0] == 0.0 && kHex.dn[SG][IP][n][1] == 0.0 &&
+                        kHex.dn[SG][IP][n][2] == 0.0;
         double F1 = 0.0, mt = 0.0;
         if constexpr (!REG) {
-            F1 = a.F1[nd[n]];
-            mt = a.mut[nd[n]];
+            if (!wz) {
+                F1 = a.F1[nd[n]];
+                mt = a.mut[nd[n]];
+            }
         }
 #pragma unroll
         for (int q = 0; q < NE; ++q) {
-            T pv, gv;
-            if constexpr (REG) {
-                pv = ph[q][n][0];
-                gv = gm[q][n][0];
-            } else {
-                pv = T(as[q].phi[nd[n]]);
-                gv = T(a.mu + mt * (F1 * as[q].sig1 + (1.0 - F1) * as[q].sig2));
+            if (!wz) {
+                T gv;
+                if constexpr (REG) gv = gm[q][n][0];
+                else gv = T(a.mu + mt * (F1 * as[q].sig1 + (1.0 - F1) * as[q].sig2));
+                Gip[q] += T(kHex.N[US][IP][n]) * gv;
             }
-            Gip[q] += (a.us ? T(kHex.N[1][IP][n]) : T(kHex.N[0][IP][n])) * gv;
+            if (!gz) {
+                T pv;
+                if constexpr (REG) pv = ph[q][n][0];
+                else pv = T(as[q].phi[nd[n]]);
 #pragma unroll
-            for (int d = 0; d < 3; ++d)
-                if (kHex.dn[SG][IP][n][d] != 0.0) P[q][d] += T(kHex.dn[SG][IP][n][d]) * pv;
+                for (int d = 0; d < 3; ++d)
+                    if (kHex.dn[SG][IP][n][d] != 0.0) P[q][d] += T(kHex.dn[SG][IP][n][d]) * pv;
+            }
         }
     }
     // upwind advection plus the limited deferred correction
@@ -1786,8 +1818,12 @@ __device__ __forceinline__ void sc_incidence(const ScArgs* as, int la, int e, ui
     }
 #pragma unroll 1
     for (int q3 = 0; q3 < 3; ++q3) {
-        sc_scs<T, SG, HR, REG, NE>(as, 3 * la + q3, e, nd, c, reinterpret_cast<const T(*)[8][1]>(ph),
-                                   reinterpret_cast<const T(*)[8][1]>(gm), sl, sr, dof, D, R, bad, lt);
+        if (a.us)
+            sc_scs<T, SG, HR, REG, NE, 1>(as, 3 * la + q3, e, nd, c, reinterpret_cast<const T(*)[8][1]>(ph),
+                                          reinterpret_cast<const T(*)[8][1]>(gm), sl, sr, dof, D, R, bad, lt);
+        else
+            sc_scs<T, SG, HR, REG, NE, 0>(as, 3 * la + q3, e, nd, c, reinterpret_cast<const T(*)[8][1]>(ph),
+                                          reinterpret_cast<const T(*)[8][1]>(gm), sl, sr, dof, D, R, bad, lt);
         rotate_nodes<T, 3>(c);
         if constexpr (REG) {
 #pragma unroll
