@@ -460,6 +460,65 @@ def graph_mode_steps(oa, f, steps, warmup=2):
             **f.graph_stats()}
 
 
+def bench_bfs_to_residual(args, oa, mesh, make_flow):
+    """Wall time until every scaled residual of an outer iteration (U, V, W,
+    p', k, omega) falls below the target, mixed and FP64, each outer
+    iteration one graph launch (the device tests the target); a form stops
+    at the iteration cap or after --time-cap seconds."""
+    target, cap, tcap = args.to_residual, args.max_iters, args.time_cap
+    out = {}
+    forms = [{"mixed": oa.OAGPU_MIXED, "fp64": oa.OAGPU_FP64}[x] for x in args.forms.split(",")]
+    for prec in forms:
+        f = make_flow(prec)
+        bfs_setup(oa, mesh, f)
+        f.autotune()
+        oa.load_library().oagpu_synchronize()
+        t0 = time.perf_counter()
+        hist = []
+        while len(hist) < cap and time.perf_counter() - t0 < tcap:
+            nb = min(args.report_every, cap - len(hist))
+            if args.iteration == "graph":
+                h = f.graph_iterations(BFS_CTL, BFS_SCTL, n=nb, tol=target)
+            else:
+                h = []
+                for _ in range(nb):
+                    h.append(np.concatenate([f.outer_iteration(BFS_CTL), f.sst_iteration(BFS_SCTL)]))
+                    if max(h[-1][:4].max(), h[-1][6:8].max()) < target:
+                        break
+                h = np.array(h)
+            hist.extend(h)
+            if len(h) and max(h[-1][:4].max(), h[-1][6:8].max()) < target:
+                break
+            print(f"[bench] {'mixed' if prec else 'fp64'} {len(hist)} iterations, "
+                  f"{time.perf_counter() - t0:.1f} s, residuals {np.round(np.log10(h[-1][[0, 1, 2, 3, 6, 7]]), 2).tolist()}",
+                  file=sys.stderr, flush=True)
+        oa.load_library().oagpu_synchronize()
+        t = time.perf_counter() - t0
+        hist = np.array(hist)
+        res = lambda h: [float(x) for x in np.concatenate([h[:4], h[6:8]])]
+        out[prec] = {"seconds": round(t, 3), "iterations": len(hist),
+                     "iterations_per_s": round(len(hist) / t, 4),
+                     "reached": bool(max(hist[-1, :4].max(), hist[-1, 6:8].max()) < target),
+                     "final_residuals_U_V_W_p_k_omega": res(hist[-1]),
+                     "history_every_50": [res(h) for h in hist[::50]],
+                     **(f.graph_stats() if args.iteration == "graph" else {})}
+        f.close()
+    mx, f64 = out.get(oa.OAGPU_MIXED), out.get(oa.OAGPU_FP64)
+    line = {"metric": f"time to scaled residual {target:g}", "value": (mx or f64)["seconds"], "unit": "s",
+            "n_gpus": 1, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+            "dtype": "f32 local / f64 global", "data": "synthetic",
+            "config": {"workload": f"backward-facing step k-omega SST, {mesh['conn'].shape[0]} hex8, "
+                                   f"outer iterations (flow + SST"
+                                   + (", each one HIP graph launch" if args.iteration == "graph" else "")
+                                   + ") until "
+                                   f"every scaled residual < {target:g} (cap {cap} iterations or "
+                                   f"{tcap:g} s per form)"},
+            "mixed": mx, "fp64": f64,
+            "mixed_over_fp64_iterations_per_s": (round(mx["iterations_per_s"] / f64["iterations_per_s"], 3)
+                                                 if mx and f64 else None)}
+    print(json.dumps(line))
+
+
 def bench_bfs(args):
     import openaccel_amd as oa
     from openaccel_amd import meshgen
@@ -476,6 +535,8 @@ def bench_bfs(args):
     def make_flow(prec):
         return oa.FlowCase(dm, bfs_params(oa, prec, args.bfs_p_method), BFS_BC)
 
+    if args.to_residual:
+        return bench_bfs_to_residual(args, oa, mesh, make_flow)
     res = {}
     for prec in (oa.OAGPU_MIXED, oa.OAGPU_FP64):
         f = make_flow(prec)
@@ -987,6 +1048,11 @@ def main():
     ap.add_argument("--to-residual", type=float, default=0.0,
                     help="cavity: time the outer iterations until every scaled residual is below this")
     ap.add_argument("--max-iters", type=int, default=4000)
+    ap.add_argument("--time-cap", type=float, default=240.0,
+                    help="bfs --to-residual: wall seconds per form")
+    ap.add_argument("--forms", default="mixed,fp64", help="bfs --to-residual: forms to run")
+    ap.add_argument("--report-every", type=int, default=25,
+                    help="bfs --to-residual: iterations between progress lines")
     ap.add_argument("--batch-cases", type=int, default=BATCH_N)
     ap.add_argument("--batch-iters", type=int, default=BATCH_ITERS)
     ap.add_argument("--batch-tol", type=float, default=0.0)

@@ -1112,71 +1112,83 @@ static unsigned sa_grid(int n) {
     }
 
 // ---------------------------------------------------------- set-up kernels
-// s[e] = sum over the listed couplings of P entry e (values of A_l)
+// s[e] = sum over the listed couplings of P entry e (values of A_l; V: the
+// set-up's value type, FP64 or the mixed form's FP32, sums in double)
+template <typename V>
 __global__ void __launch_bounds__(256) k_sa_psum(int ne, const int* pt_ptr, const int* pt_idx,
-                                                 const double* A, double* sum) {
+                                                 const V* A, V* sum) {
     const int e = int(xcd_block() * blockDim.x + threadIdx.x);
     if (e >= ne) return;
     double s = 0.0;
     const int k1 = pt_ptr[e + 1];
-    for (int k = pt_ptr[e]; k < k1; ++k) s += A[pt_idx[k]];
-    sum[e] = s;
+    for (int k = pt_ptr[e]; k < k1; ++k) s += double(A[pt_idx[k]]);
+    sum[e] = V(s);
 }
 
 // P row i from its row sum and diagonal: d_F = rowsum - sum_e s_e (a_ii if
 // not positive), P_e = [own] (1 - omega_p) - omega_p s_e / d_F
+template <typename V>
 __device__ __forceinline__ void sa_prow(int i, double rowsum, double aii, const int* prp,
                                         const unsigned char* own, const int* ppos,
-                                        const double* sum, double omega_p, double* P) {
+                                        const V* sum, double omega_p, V* P) {
     const int e0 = prp[i], e1 = prp[i + 1];
     double st = 0.0;
-    for (int e = e0; e < e1; ++e) st += sum[e];
+    for (int e = e0; e < e1; ++e) st += double(sum[e]);
     double dF = rowsum - st;
     if (!(dF > 0.0)) dF = aii;
     const double sc = dF != 0.0 ? omega_p / dF : 0.0;
-    for (int e = e0; e < e1; ++e) P[ppos[e]] = (own[e] ? 1.0 - omega_p : 0.0) - sc * sum[e];
+    for (int e = e0; e < e1; ++e) P[ppos[e]] = V((own[e] ? 1.0 - omega_p : 0.0) - sc * double(sum[e]));
 }
 
 // fine level (sliced-ELL rows, short): one lane per row
-__global__ void __launch_bounds__(256) k_sa_prow_sell(SellView s, const double* A, const int* prp,
+template <typename V>
+__global__ void __launch_bounds__(256) k_sa_prow_sell(SellView s, const V* A, const int* prp,
                                                       const unsigned char* own, const int* ppos,
-                                                      const double* sum, double omega_p, double* P) {
+                                                      const V* sum, double omega_p, V* P) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n) return;
     int w;
     const size_t b = sv_base(s, i, w);
     double rs = 0.0;
-    for (int q = 0; q < w; ++q) rs += A[b + size_t(q) * 64];
-    sa_prow(i, rs, A[b + size_t(s.diag_off[i]) * 64], prp, own, ppos, sum, omega_p, P);
+    for (int q = 0; q < w; ++q) rs += double(A[b + size_t(q) * 64]);
+    sa_prow<V>(i, rs, double(A[b + size_t(s.diag_off[i]) * 64]), prp, own, ppos, sum, omega_p, P);
 }
 
 // coarse level (CSR rows): G lanes sum the row, its first lane writes P's row
-template <int G>
-__global__ void __launch_bounds__(256) k_sa_prow_csr(int n, const int* rp, const int* diag, const double* A,
+template <int G, typename V>
+__global__ void __launch_bounds__(256) k_sa_prow_csr(int n, const int* rp, const int* diag, const V* A,
                                                      const int* prp, const unsigned char* own,
-                                                     const int* ppos, const double* sum, double omega_p,
-                                                     double* P) {
+                                                     const int* ppos, const V* sum, double omega_p,
+                                                     V* P) {
     const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int i = int(t / G), lane = int(t % G);
     if (i >= n) return;
     double rs = 0.0;
-    for (int k = rp[i] + lane; k < rp[i + 1]; k += G) rs += A[k];
+    for (int k = rp[i] + lane; k < rp[i + 1]; k += G) rs += double(A[k]);
     rs = sub_sum<G>(rs);
-    if (lane == 0) sa_prow(i, rs, A[diag[i]], prp, own, ppos, sum, omega_p, P);
+    if (lane == 0) sa_prow<V>(i, rs, double(A[diag[i]]), prp, own, ppos, sum, omega_p, P);
 }
 
 // out[pos ? pos[e] : e] = sum over the terms t of entry e of X[t.x] * Y[t.y]
+template <typename V>
 __global__ void __launch_bounds__(256) k_sa_terms(int ne, const int* ptr, const int2* terms,
-                                                  const double* X, const double* Y, double* out) {
+                                                  const V* X, const V* Y, V* out) {
     const int e = int(xcd_block() * blockDim.x + threadIdx.x);
     if (e >= ne) return;
     double s = 0.0;
     const int k1 = ptr[e + 1];
     for (int k = ptr[e]; k < k1; ++k) {
         const int2 t = terms[k];
-        s += X[t.x] * Y[t.y];
+        s += double(X[t.x]) * double(Y[t.y]);
     }
-    out[e] = s;
+    out[e] = V(s);
+}
+
+__global__ void __launch_bounds__(256) k_sa_dinv_f(int n, const int* diag, const float* A, float* fdinv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double d = double(A[diag[i]]);
+    fdinv[i] = float(d != 0.0 ? 1.0 / d : 0.0);
 }
 
 __global__ void __launch_bounds__(256) k_sa_dinv(int n, const int* diag, const double* A, double* dinv,
@@ -1195,9 +1207,9 @@ __global__ void __launch_bounds__(256) k_sa_to_float(size_t n, const double* x, 
 
 // Gauss-Jordan inverse of the coarsest matrix (CSR, n <= kDenseMax) in LDS,
 // as k_amg_dense_inverse; stored transposed for k_amg_dense_apply
-template <typename V>
+template <typename V, typename IN = double>
 __global__ void __launch_bounds__(256) k_sa_dense_inverse(int n, const int* rp, const int* ci,
-                                                          const double* A, V* vinv) {
+                                                          const IN* A, V* vinv) {
     __shared__ double M[kDenseMax][kDenseMax + 1];
     __shared__ double colk[kDenseMax];
     __shared__ double piv, dmax;
@@ -1206,7 +1218,7 @@ __global__ void __launch_bounds__(256) k_sa_dense_inverse(int n, const int* rp, 
     __syncthreads();
     if (int(threadIdx.x) < n) {
         const int i = threadIdx.x;
-        for (int k = rp[i]; k < rp[i + 1]; ++k) M[i][ci[k]] += A[k];
+        for (int k = rp[i]; k < rp[i + 1]; ++k) M[i][ci[k]] += double(A[k]);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1336,11 +1348,11 @@ static void launch_csr_restrict(hipStream_t st, int nc, const int* rptr, const i
                                 const V* res, V* rc, const int* flag) {
     k_csr_restrict<V, G><<<sa_grid<G>(nc), 256, 0, st>>>(nc, rptr, rt, P, res, rc, flag);
 }
-template <int G>
-static void launch_prow_csr(hipStream_t st, int n, const int* rp, const int* diag, const double* A,
-                            const int* prp, const unsigned char* own, const int* ppos, const double* sum,
-                            double om, double* P) {
-    k_sa_prow_csr<G><<<sa_grid<G>(n), 256, 0, st>>>(n, rp, diag, A, prp, own, ppos, sum, om, P);
+template <int G, typename V>
+static void launch_prow_csr(hipStream_t st, int n, const int* rp, const int* diag, const V* A,
+                            const int* prp, const unsigned char* own, const int* ppos, const V* sum,
+                            double om, V* P) {
+    k_sa_prow_csr<G, V><<<sa_grid<G>(n), 256, 0, st>>>(n, rp, diag, A, prp, own, ppos, sum, om, P);
 }
 
 // ----------------------------------------------------------- host build
@@ -1760,18 +1772,18 @@ void sa_setup(Flow& f, const double* A0, bool want_float) {
         const double* Al = l == 0 ? A0 : L.A.p;
         const int np = L.h_prp.back();
         if (np)
-            k_sa_psum<<<xcd_grid(size_t(np), 256), 256, 0, st>>>(np, L.pt_ptr.p, L.pt_idx.p, Al, L.psum.p);
+            k_sa_psum<double><<<xcd_grid(size_t(np), 256), 256, 0, st>>>(np, L.pt_ptr.p, L.pt_idx.p, Al, L.psum.p);
         if (l == 0)
-            k_sa_prow_sell<<<blocks(L.n), 256, 0, st>>>(view_of(*f.m), Al, L.prp.p, L.own.p, L.ppos.p,
+            k_sa_prow_sell<double><<<blocks(L.n), 256, 0, st>>>(view_of(*f.m), Al, L.prp.p, L.own.p, L.ppos.p,
                                                         L.psum.p, L.omega_p, L.P.p);
         else
             SA_DISPATCH(L.lanes, launch_prow_csr, st, L.n, L.a_rp.p, L.a_diag.p, Al, L.prp.p, L.own.p,
                         L.ppos.p, L.psum.p, L.omega_p, L.P.p)
         if (L.nb)
-            k_sa_terms<<<xcd_grid(L.nb, 256), 256, 0, st>>>(int(L.nb), L.b_ptr.p, L.b_terms.p, Al, L.P.p,
+            k_sa_terms<double><<<xcd_grid(L.nb, 256), 256, 0, st>>>(int(L.nb), L.b_ptr.p, L.b_terms.p, Al, L.P.p,
                                                             L.B.p);
         if (L.nac)
-            k_sa_terms<<<xcd_grid(L.nac, 256), 256, 0, st>>>(int(L.nac), L.c_ptr.p, L.c_terms.p, L.P.p,
+            k_sa_terms<double><<<xcd_grid(L.nac, 256), 256, 0, st>>>(int(L.nac), L.c_ptr.p, L.c_terms.p, L.P.p,
                                                              L.B.p, C.A.p);
         k_sa_dinv<<<blocks(C.n), 256, 0, st>>>(C.n, C.a_diag.p, C.A.p, C.dinv.p,
                                                want_float ? C.fdinv.p : nullptr);
@@ -1785,6 +1797,46 @@ void sa_setup(Flow& f, const double* A0, bool want_float) {
         k_sa_dense_inverse<double><<<1, 256, 0, st>>>(C.n, C.a_rp.p, C.a_ci.p, C.A.p, sa.dense.p);
         if (want_float)
             k_sa_dense_inverse<float><<<1, 256, 0, st>>>(C.n, C.a_rp.p, C.a_ci.p, C.A.p, sa.fdense.p);
+    }
+    OAGPU_HIP(hipGetLastError());
+}
+
+// The mixed form's numeric set-up, FP32 throughout: P, the products B = A P
+// and P^T B, the inverse diagonals and the coarsest dense inverse computed
+// from the FP32 fine values A0f straight into the FP32 level arrays the
+// FP32 V-cycle reads (sums accumulated in double); the FP64 level arrays are
+// left as the last FP64 set-up wrote them.
+void sa_setup_f32(Flow& f, const float* A0f) {
+    sa_build(f);
+    Sa& sa = f.sa;
+    const int nl = int(sa.lv.size());
+    hipStream_t st = f.stream;
+    for (int l = 0; l + 1 < nl; ++l) {
+        SaLevel& L = sa.lv[l];
+        SaLevel& C = sa.lv[l + 1];
+        const float* Al = l == 0 ? A0f : L.fA.p;
+        float* psum = reinterpret_cast<float*>(L.psum.p);  // the FP64 arrays' space, half used
+        float* B = reinterpret_cast<float*>(L.B.p);
+        const int np = L.h_prp.back();
+        if (np)
+            k_sa_psum<float><<<xcd_grid(size_t(np), 256), 256, 0, st>>>(np, L.pt_ptr.p, L.pt_idx.p, Al, psum);
+        if (l == 0)
+            k_sa_prow_sell<float><<<blocks(L.n), 256, 0, st>>>(view_of(*f.m), Al, L.prp.p, L.own.p, L.ppos.p,
+                                                               psum, L.omega_p, L.fP.p);
+        else
+            SA_DISPATCH(L.lanes, launch_prow_csr, st, L.n, L.a_rp.p, L.a_diag.p, Al, L.prp.p, L.own.p,
+                        L.ppos.p, (const float*)psum, L.omega_p, L.fP.p)
+        if (L.nb)
+            k_sa_terms<float><<<xcd_grid(L.nb, 256), 256, 0, st>>>(int(L.nb), L.b_ptr.p, L.b_terms.p, Al,
+                                                                   L.fP.p, B);
+        if (L.nac)
+            k_sa_terms<float><<<xcd_grid(L.nac, 256), 256, 0, st>>>(int(L.nac), L.c_ptr.p, L.c_terms.p,
+                                                                    L.fP.p, B, C.fA.p);
+        k_sa_dinv_f<<<blocks(C.n), 256, 0, st>>>(C.n, C.a_diag.p, C.fA.p, C.fdinv.p);
+    }
+    if (sa.dense_n) {
+        SaLevel& C = sa.lv[nl - 1];
+        k_sa_dense_inverse<float, float><<<1, 256, 0, st>>>(C.n, C.a_rp.p, C.a_ci.p, C.fA.p, sa.fdense.p);
     }
     OAGPU_HIP(hipGetLastError());
 }

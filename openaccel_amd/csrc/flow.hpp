@@ -389,6 +389,7 @@ void amg_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, double* 
 // and vectors of type V (A0 / dinv0: the fine level's, of type V).
 void sa_build(Flow& f);
 void sa_setup(Flow& f, const double* A0, bool want_float);
+void sa_setup_f32(Flow& f, const float* A0f);  // the mixed form's: FP32 levels from FP32 fine values
 template <typename V>
 void sa_vcycle(Flow& f, const V* A0, const V* dinv0, const V* r, V* z, bool timing = false,
                const int* stop = nullptr);  // stop: the kernels' stop flag (default f.flag)

@@ -1152,7 +1152,7 @@ void enq_bicgstab_sa(Flow& f, int max_it, double rtol, double atol, int cap, boo
     } else {
         k_g_to_float<<<nblk(m.nsell), 256, 0, st>>>(m.nsell, f.A1.p, f.Af.p);
         k_g_to_float<<<nblk(n), 256, 0, st>>>(n, f.dinv.p, f.fdinv.p);
-        sa_setup(f, f.A1.p, true);
+        sa_setup_f32(f, f.Af.p);
         const BiVecs<float> w{f.fr.p, f.fr0.p, f.fp.p, f.fq.p, f.fs.p, f.ft.p, f.fz.p, f.fsh.p};
         k_g_start_vec<double><<<kRedGrid, kRedBlock, 0, st>>>(n, f.b1.p, f.x1.p, f.r.p, part, si);
         k_mx_start<<<1, kRedBlock, 0, st>>>(part, sd, si);
@@ -1206,7 +1206,7 @@ void enq_cg_sa(Flow& f, int max_it, double rtol, double atol, int cap, bool poll
     double* part = f.part.p;
     k_g_to_float<<<nblk(f.m->nsell), 256, 0, st>>>(f.m->nsell, f.A1.p, f.Af.p);
     k_g_to_float<<<nblk(n), 256, 0, st>>>(n, f.dinv.p, f.fdinv.p);
-    if (sa) sa_setup(f, f.A1.p, true);
+    if (sa) sa_setup_f32(f, f.Af.p);
     k_g_solve_init<<<1, 1, 0, st>>>(si, sd, max_it, 0, 1, rtol, atol, 0.0);
     k_cgm_stop<<<1, 1, 0, st>>>(f.flag.p);
     k_g_start_vec<double><<<kRedGrid, kRedBlock, 0, st>>>(n, f.b1.p, f.x1.p, f.r.p, part, si);

@@ -772,7 +772,7 @@ static int cg_mixed(Flow& f, int max_it, double rtol, double atol, oagpu_control
     k_to_float<<<unsigned((nv + B - 1) / B), B, 0, f.stream>>>(nv, 1.0, A, f.Af.p);
     k_to_float<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, 1.0, f.dinv.p, f.fdinv.p);
     if (amg && sa) {
-        sa_setup(f, A, true);  // Galerkin products in FP64, FP32 copies for the inner solve
+        sa_setup_f32(f, f.Af.p);  // FP32 levels for the FP32 inner solve
         amg = sa = f.sa.lv.size() > 1;
     } else if (amg) {
         amg_setup<float>(f, f.Af.p, f.fdinv.p);
@@ -1114,7 +1114,7 @@ static int bicgstab_amg_mixed(Flow& f, int max_it, double rtol, double atol, oag
         if (v->n < n) v->alloc(n);
     k_to_float<<<unsigned((n + B - 1) / B), B, 0, f.stream>>>(n, 1.0, f.dinv.p, f.fdinv.p);
     if (sa)
-        sa_setup(f, A, true);  // Galerkin products in FP64, FP32 copies for the inner solve
+        sa_setup_f32(f, f.Af.p);  // FP32 levels for the FP32 inner solve
     else
         amg_setup<float>(f, f.Af.p, f.fdinv.p);
     OAGPU_HIP(hipMemsetAsync(x, 0, n * sizeof(double), f.stream));
