@@ -957,21 +957,48 @@ def bench_batch(args):
         seq[k] = time.perf_counter() - t1
         seq_it[k] = it + 1
         f.close()
+        print(f"[bench] case {k}: {c['n']}^3, Re {1.0 / c['mu']:.0f}, {it + 1} iterations, "
+              f"{seq[k]:.2f} s", file=sys.stderr, flush=True)
     t_seq = seq.sum()
     print(f"[bench] fp64 one at a time: {t_seq:.2f} s", file=sys.stderr, flush=True)
 
     graph_mode = args.iteration == "graph" and args.p_method == "cg_sa"
 
-    def batched(precision):
+    def batched(precision, chunk=200):
+        """All cases on 16 host threads, one stream each, in rounds of up to
+        `chunk` outer iterations (a progress line per round); a case leaves
+        once its residuals are below tol; per-case seconds sum its rounds."""
         flows = [flow(c, precision) for c in cases]
+        n = len(cases)
+        hist = [[] for _ in range(n)]
+        done, sec = np.zeros(n, dtype=int), np.zeros(n)
+        live = list(range(n))
         oa.load_library().oagpu_synchronize()
         t0 = time.perf_counter()
         run = oa.batch_graph_iterations if graph_mode else oa.batch_outer_iterations
-        hist, done, sec = run(flows, CTL, iters, tol=tol, threads=16)
+        while live:
+            step = min(chunk, iters - int(done[live[0]]))
+            if step <= 0:
+                break
+            h, d, sc = run([flows[k] for k in live], CTL, step, tol=tol, threads=16)
+            nxt = []
+            for j, k in enumerate(live):
+                hist[k].extend(h[j][:d[j]])
+                done[k] += d[j]
+                sec[k] += sc[j]
+                if d[j] == step and not (tol > 0 and h[j][d[j] - 1][:4].max() < tol):
+                    nxt.append(k)
+            live = nxt
+            print(f"[bench] {'mixed' if precision else 'fp64'} batched: {n - len(live)} of {n} cases done, "
+                  f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
         t = time.perf_counter() - t0
         for f in flows:
             f.close()
-        return t, hist, done, sec
+        width = max(len(x) for x in hist)
+        H = np.zeros((n, width, len(hist[0][0])))
+        for k in range(n):
+            H[k, :len(hist[k])] = hist[k]
+        return t, H, done, sec
 
     # the concurrency alone (FP64 batched), then concurrency + the mixed form
     t_b64, hist64, done64, sec64 = batched(oa.OAGPU_FP64)
