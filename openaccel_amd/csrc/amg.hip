@@ -1184,6 +1184,46 @@ __global__ void __launch_bounds__(256) k_sa_terms(int ne, const int* ptr, const 
     out[e] = V(s);
 }
 
+// as k_sa_terms with G lanes per entry: the group's lanes read consecutive
+// terms (coalesced), each sums every G-th term in double, the group adds the
+// partial sums in a fixed order (bitwise reproducible)
+template <typename V, int G>
+__global__ void __launch_bounds__(256) k_sa_terms_g(int ne, const int* ptr, const int2* terms,
+                                                    const V* X, const V* Y, V* out) {
+    const size_t t = size_t(xcd_block()) * blockDim.x + threadIdx.x;
+    const int e = int(t / G), lane = int(t % G);
+    if (e >= ne) return;  // a group of G lanes (G divides 64) leaves together
+    double s = 0.0;
+    const int k1 = ptr[e + 1];
+    for (int k = ptr[e] + lane; k < k1; k += G) {
+        const int2 q = terms[k];
+        s += double(X[q.x]) * double(Y[q.y]);
+    }
+    s = sub_sum<G>(s);
+    if (lane == 0) out[e] = V(s);
+}
+
+// lanes per entry of a term list from its mean term count
+static int sa_term_lanes(size_t nterms, size_t ne) {
+    const double len = ne ? double(nterms) / double(ne) : 0.0;
+    return len < 6.0 ? 1 : len < 24.0 ? 4 : 16;
+}
+
+template <typename V>
+static void launch_sa_terms(hipStream_t st, int lanes, size_t ne, const int* ptr, const int2* terms,
+                            const V* X, const V* Y, V* out) {
+    switch (lanes) {
+    case 4:
+        k_sa_terms_g<V, 4><<<xcd_grid(ne * 4, 256), 256, 0, st>>>(int(ne), ptr, terms, X, Y, out);
+        break;
+    case 16:
+        k_sa_terms_g<V, 16><<<xcd_grid(ne * 16, 256), 256, 0, st>>>(int(ne), ptr, terms, X, Y, out);
+        break;
+    default:
+        k_sa_terms<V><<<xcd_grid(ne, 256), 256, 0, st>>>(int(ne), ptr, terms, X, Y, out);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_sa_dinv_f(int n, const int* diag, const float* A, float* fdinv) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1708,10 +1748,12 @@ void sa_build(Flow& f) {
         L.nb = size_t(nb);
         L.b_ptr.upload(bptr);
         L.b_terms.upload(bterms);
+        L.b_lanes = sa_term_lanes(bterms.size(), L.nb);
         L.B.alloc(std::max<size_t>(L.nb, 1));
         L.nac = size_t(ncn);
         L.c_ptr.upload(cptr);
         L.c_terms.upload(cterms);
+        L.c_lanes = sa_term_lanes(cterms.size(), size_t(ncn));
         L.r_ptr.upload(rptr);
         L.r_terms.upload(rterms);
         L.r_lanes = sa_lanes(np, nc);
@@ -1780,11 +1822,9 @@ void sa_setup(Flow& f, const double* A0, bool want_float) {
             SA_DISPATCH(L.lanes, launch_prow_csr, st, L.n, L.a_rp.p, L.a_diag.p, Al, L.prp.p, L.own.p,
                         L.ppos.p, L.psum.p, L.omega_p, L.P.p)
         if (L.nb)
-            k_sa_terms<double><<<xcd_grid(L.nb, 256), 256, 0, st>>>(int(L.nb), L.b_ptr.p, L.b_terms.p, Al, L.P.p,
-                                                            L.B.p);
+            launch_sa_terms<double>(st, L.b_lanes, L.nb, L.b_ptr.p, L.b_terms.p, Al, L.P.p, L.B.p);
         if (L.nac)
-            k_sa_terms<double><<<xcd_grid(L.nac, 256), 256, 0, st>>>(int(L.nac), L.c_ptr.p, L.c_terms.p, L.P.p,
-                                                             L.B.p, C.A.p);
+            launch_sa_terms<double>(st, L.c_lanes, L.nac, L.c_ptr.p, L.c_terms.p, L.P.p, L.B.p, C.A.p);
         k_sa_dinv<<<blocks(C.n), 256, 0, st>>>(C.n, C.a_diag.p, C.A.p, C.dinv.p,
                                                want_float ? C.fdinv.p : nullptr);
         if (want_float) {
@@ -1827,11 +1867,9 @@ void sa_setup_f32(Flow& f, const float* A0f) {
             SA_DISPATCH(L.lanes, launch_prow_csr, st, L.n, L.a_rp.p, L.a_diag.p, Al, L.prp.p, L.own.p,
                         L.ppos.p, (const float*)psum, L.omega_p, L.fP.p)
         if (L.nb)
-            k_sa_terms<float><<<xcd_grid(L.nb, 256), 256, 0, st>>>(int(L.nb), L.b_ptr.p, L.b_terms.p, Al,
-                                                                   L.fP.p, B);
+            launch_sa_terms<float>(st, L.b_lanes, L.nb, L.b_ptr.p, L.b_terms.p, Al, L.fP.p, B);
         if (L.nac)
-            k_sa_terms<float><<<xcd_grid(L.nac, 256), 256, 0, st>>>(int(L.nac), L.c_ptr.p, L.c_terms.p,
-                                                                    L.fP.p, B, C.fA.p);
+            launch_sa_terms<float>(st, L.c_lanes, L.nac, L.c_ptr.p, L.c_terms.p, L.fP.p, B, C.fA.p);
         k_sa_dinv_f<<<blocks(C.n), 256, 0, st>>>(C.n, C.a_diag.p, C.fA.p, C.fdinv.p);
     }
     if (sa.dense_n) {

@@ -57,6 +57,8 @@ struct SaLevel {
     // B_l = A_l P_l, CSR over this level's rows: entry e = sum over its terms
     // (value index of A_l, slot of P_l)
     size_t nb = 0;
+    int b_lanes = 1;  // lanes per entry of B's and A_{l+1}'s term sums (1, 4 or 16)
+    int c_lanes = 1;
     DBuf<int> b_ptr;
     DBuf<int2> b_terms;
     DBuf<double> B;
