@@ -175,7 +175,8 @@ def summarize_traffic(d, workload_desc, timed=None):
                "fetch_scale": round(fscale, 5), "write_scale": round(wscale, 5),
                "calibration": f"{cal['fetch'][1].split('(')[0]} of 1 GiB (bench.py --copy-probe 1024): "
                               f"FETCH_SIZE {cal['fetch'][0]:.1f} KiB, WRITE_SIZE {cal['write'][0]:.1f} KiB",
-               "commands": [c.format(dir="gpurun_out/final") for c in PROFILE_COMMANDS[1:]]}
+               "commands": [c.format(dir=os.path.abspath(d))
+                            for c in PROFILE_COMMANDS[1:]]}
         for form, k in (("mixed", 0), ("fp64", 1)):
             pat = pats[k] if pats[k] is not None else pats[1 - k]
             if pat is None:
@@ -1089,8 +1090,10 @@ def main():
     ap.add_argument("--summarize-profiles", metavar="DIR", default="",
                     help="write profiles/traffic_*_bfs.json and print the per-iteration kernel table "
                          "from the rocprofv3 runs under DIR (--profile-commands lists them)")
-    ap.add_argument("--profile-commands", action="store_true",
-                    help="print the rocprofv3 commands whose output --summarize-profiles reads")
+    ap.add_argument("--profile-commands", nargs="?", const=os.path.join(ROOT, "prof_out"), default=None,
+                    metavar="DIR",
+                    help="print the rocprofv3 commands, writing under DIR (default prof_out), whose "
+                         "output --summarize-profiles DIR reads")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -1099,7 +1102,7 @@ def main():
               "halo exchange) is designed but not built; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if args.profile_commands:
-        print("\n".join(c.format(dir="gpurun_out/final") for c in PROFILE_COMMANDS))
+        print("\n".join(c.format(dir=args.profile_commands) for c in PROFILE_COMMANDS))
         return
     if args.summarize_profiles:
         timed = {}
